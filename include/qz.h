@@ -1,0 +1,127 @@
+/*
+ * qz.h -- native C ABI of the MI355X particle-mesh gravity library (libqzhip.so).
+ *
+ * One context holds the device-resident state of one simulation: particles
+ * (structure of arrays, kept sorted by head-cell key), the head grid with its
+ * multigrid hierarchy, and the refinement nodes flattened into one pool per
+ * level.  Every entry point returns 0 on success and 1 on failure (the
+ * reference convention, common.h:81-82) and returns only after the device
+ * work it queued has finished; qz_last_error() describes the last failure.
+ *
+ * Entry points mirror the reference's per-step functions (main.c:369-535);
+ * the reference-named drop-in wrappers are declared in quantizer_abi.h.
+ */
+#ifndef QZ_H
+#define QZ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qz_ctx qz_ctx;
+
+enum
+{
+  QZ_PREC_FP64 = 0,  /* particles, grids and V-cycles in FP64 (the "plain" path) */
+  QZ_PREC_MIXED = 1  /* FP32 particles/grids/V-cycles, FP64 outer residual + correction */
+};
+
+enum
+{
+  QZ_BC_MONOPOLE = 0, /* head faces at -GM/r about the centre (initial_potential_and_acceleration_head.c:297-335) */
+  QZ_BC_ZERO = 1      /* zero Dirichlet faces */
+};
+
+typedef struct
+{
+  int lmin;               /* head level: head grid has 2^lmin cells per side (global_variables.c:252) */
+  int lmax;               /* finest level allowed (global_variables.c:253) */
+  int n_particles;
+  int precision;          /* QZ_PREC_* */
+  int head_bc;            /* QZ_BC_* */
+  int ref_criterion_ptcl; /* refine a cell holding >= this many particles (global_variables.c:287) */
+  int n_exp;              /* refinement dilation (global_variables.c:289); only 1 is supported */
+  double G;               /* gravitational constant in code units */
+  double tol;             /* RMS residual / (4 pi G rho_mean) threshold (global_variables.c:318) */
+  double tol_change;      /* pointwise |phi/phi_old - 1| threshold (global_variables.c:319) */
+  int max_iterations;     /* V-cycle / CG iteration cap (global_variables.c:313) */
+  int inner_cycles;       /* V-cycles per outer FP64 refinement iteration (0 = default) */
+  int pre_sweeps;         /* red-black GS sweeps before restriction */
+  int post_sweeps;        /* red-black GS sweeps after prolongation */
+} qz_config;
+
+typedef struct
+{
+  int head_outer;      /* outer FP64 residual/correction iterations of the last head solve */
+  int head_vcycles;    /* inner V-cycles of the last head solve */
+  double head_residual; /* final RMS residual / (4 pi G rho_mean) of the head */
+  double head_change;   /* final max |phi/phi_old - 1| of the head */
+  int n_nodes;         /* refinement nodes in the tree */
+  int n_levels;        /* levels in use including the head */
+  long long branch_points;    /* grid points over all refinement nodes */
+  long long branch_intr;      /* interior (unknown) points over all refinement nodes */
+  int branch_iterations;      /* max CG iterations over nodes in the last branch solve */
+  long long branch_iter_points; /* sum over nodes of iterations x interior points */
+} qz_stats;
+
+void qz_default_config(qz_config *cfg);
+qz_ctx *qz_create(const qz_config *cfg);
+void qz_destroy(qz_ctx *ctx);
+const char *qz_last_error(void);
+
+/* host <-> device particle transfer (original particle order, FP64 host arrays) */
+int qz_set_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                     const double *vy, const double *vz, const double *m);
+int qz_get_particles(qz_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                     double *ax, double *ay, double *az);
+
+/* per-step stages (main.c:391-478) */
+int qz_particle_updating_A(qz_ctx *ctx, double dt); /* particle_updating.c:2584 */
+int qz_reset(qz_ctx *ctx);                          /* reset.c:29 */
+int qz_tree_adaptation(qz_ctx *ctx);                /* tree_adaptation.c:8250 (rebuild) */
+int qz_grid_density(qz_ctx *ctx);                   /* grid_density.c:269 */
+int qz_potential(qz_ctx *ctx);                      /* potential.c:29 */
+int qz_potential_head_node(qz_ctx *ctx);            /* potential_head_node.c:99 */
+int qz_potential_branches(qz_ctx *ctx);             /* potential_branches.c:401 */
+int qz_grid_acceleration(qz_ctx *ctx);              /* grid_acceleration.c:694 */
+int qz_particle_acceleration(qz_ctx *ctx);          /* particle_acceleration.c:522 */
+int qz_particle_updating_B(qz_ctx *ctx, double dt); /* particle_updating.c:2787 */
+
+/* composites: initial force pass (main.c:280-338) and one fixed-dt step (main.c:391-478) */
+int qz_force(qz_ctx *ctx);
+int qz_step(qz_ctx *ctx, double dt);
+
+/* field access: head rho (= 4 pi G rho), pot, ax, ay, az on the (2^lmin+1)^3
+ * solve domain, x fastest; out5 holds 5 such arrays */
+int qz_get_head_fields(qz_ctx *ctx, double *out5);
+int qz_set_head_potential(qz_ctx *ctx, const double *pot);
+
+/* refinement nodes: count, 24-int headers (same layout as the oracle dumps),
+ * and per-node fields on the node's (rd+1)^3 box grid */
+int qz_num_nodes(qz_ctx *ctx);
+int qz_node_header(qz_ctx *ctx, int i, int *hdr24);
+int qz_node_grid_lists(qz_ctx *ctx, int i, int *intr_xyz, int *bdry_xyz, int *sim_xyz);
+int qz_node_fields(qz_ctx *ctx, int i, double *out5);
+
+int qz_get_stats(qz_ctx *ctx, qz_stats *st);
+
+/* FP64 diagnostics (observables.c:661-696): E_kin, E_pot, E_tot, px, py, pz.
+ * exact_pe = 1 uses the pairwise sum (observables.c:77-112), 0 the grid
+ * interpolation (observables.c:594-619). */
+int qz_observables(qz_ctx *ctx, int exact_pe, double *out6);
+
+/* A1 (ptcl_idx_to_box_idx.c:30-139) on the device for n host positions:
+ * key = (f(x 2^lv) - ts) linearised with the real box dims rd, f(p) = p<0 ? -1 : (int)p */
+int qz_keys(const double *x, const double *y, const double *z, int n, int lv, const int *ts, const int *rd,
+            int *out);
+
+/* device timing of the hot kernels over the calls since the last reset:
+ * names[i] / total ms / launches, up to max entries; returns count */
+int qz_timing_enable(qz_ctx *ctx, int on);
+int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, int max);
+int qz_sync(qz_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,319 @@
+"""Python host mirror of the QUANTIZER per-step PM interface over libqzhip.so.
+
+The method names and their order follow the reference step loop
+(source/main.c:369-535): ``particle_updating_A(dt)``, ``reset()``,
+``tree_adaptation()``, ``grid_density()``, ``potential()`` (=
+``potential_head_node()`` + ``potential_branches()``), ``grid_acceleration()``,
+``particle_acceleration()``, ``particle_updating_B(dt)``.  The reference
+returns ``_SUCCESS_``/``_FAILURE_`` (0/1, common.h:81-82) and prints a reason;
+here a failure raises :class:`QuantizerError` carrying ``qz_last_error()``,
+and success returns ``SUCCESS`` (0).
+
+All compute runs in the HIP library; this module only moves arguments.  There
+is no CPU fallback: if the library is missing or no GPU is present, creating a
+:class:`Simulation` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqzhip.so")
+
+SUCCESS = 0
+FAILURE = 1
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+BC_MONOPOLE = 0
+BC_ZERO = 1
+
+
+class QuantizerError(RuntimeError):
+    pass
+
+
+class QzConfig(ctypes.Structure):
+    _fields_ = [
+        ("lmin", ctypes.c_int),
+        ("lmax", ctypes.c_int),
+        ("n_particles", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("head_bc", ctypes.c_int),
+        ("ref_criterion_ptcl", ctypes.c_int),
+        ("n_exp", ctypes.c_int),
+        ("G", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("tol_change", ctypes.c_double),
+        ("max_iterations", ctypes.c_int),
+        ("inner_cycles", ctypes.c_int),
+        ("pre_sweeps", ctypes.c_int),
+        ("post_sweeps", ctypes.c_int),
+    ]
+
+
+class QzStats(ctypes.Structure):
+    _fields_ = [
+        ("head_outer", ctypes.c_int),
+        ("head_vcycles", ctypes.c_int),
+        ("head_residual", ctypes.c_double),
+        ("head_change", ctypes.c_double),
+        ("n_nodes", ctypes.c_int),
+        ("n_levels", ctypes.c_int),
+        ("branch_points", ctypes.c_longlong),
+        ("branch_intr", ctypes.c_longlong),
+        ("branch_iterations", ctypes.c_int),
+        ("branch_iter_points", ctypes.c_longlong),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_LIB = None
+
+EXPORTED = [
+    "qz_default_config", "qz_create", "qz_destroy", "qz_last_error", "qz_set_particles", "qz_get_particles",
+    "qz_particle_updating_A", "qz_reset", "qz_tree_adaptation", "qz_grid_density", "qz_potential",
+    "qz_potential_head_node", "qz_potential_branches", "qz_grid_acceleration", "qz_particle_acceleration",
+    "qz_particle_updating_B", "qz_force", "qz_step", "qz_get_head_fields", "qz_set_head_potential",
+    "qz_num_nodes", "qz_node_header", "qz_node_grid_lists", "qz_node_fields", "qz_get_stats", "qz_observables",
+    "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
+]
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libqzhip.so and declare the C ABI of include/qz.h."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise QuantizerError(f"HIP library not built: {path} (run quantizer_amd/build.py)")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.qz_default_config.argtypes = [ctypes.POINTER(QzConfig)]
+    lib.qz_default_config.restype = None
+    lib.qz_create.argtypes = [ctypes.POINTER(QzConfig)]
+    lib.qz_create.restype = vp
+    lib.qz_destroy.argtypes = [vp]
+    lib.qz_destroy.restype = None
+    lib.qz_last_error.restype = ctypes.c_char_p
+    lib.qz_set_particles.argtypes = [vp] + [_dp] * 7
+    lib.qz_get_particles.argtypes = [vp] + [_dp] * 9
+    for nm in ("qz_reset", "qz_tree_adaptation", "qz_grid_density", "qz_potential", "qz_potential_head_node",
+               "qz_potential_branches", "qz_grid_acceleration", "qz_particle_acceleration", "qz_force", "qz_sync",
+               "qz_num_nodes"):
+        getattr(lib, nm).argtypes = [vp]
+        getattr(lib, nm).restype = ctypes.c_int
+    for nm in ("qz_particle_updating_A", "qz_particle_updating_B", "qz_step"):
+        getattr(lib, nm).argtypes = [vp, ctypes.c_double]
+        getattr(lib, nm).restype = ctypes.c_int
+    lib.qz_get_head_fields.argtypes = [vp, _dp]
+    lib.qz_set_head_potential.argtypes = [vp, _dp]
+    lib.qz_node_header.argtypes = [vp, ctypes.c_int, _ip]
+    lib.qz_node_grid_lists.argtypes = [vp, ctypes.c_int, _ip, _ip, _ip]
+    lib.qz_node_fields.argtypes = [vp, ctypes.c_int, _dp]
+    lib.qz_get_stats.argtypes = [vp, ctypes.POINTER(QzStats)]
+    lib.qz_observables.argtypes = [vp, ctypes.c_int, _dp]
+    lib.qz_keys.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]
+    lib.qz_timing_enable.argtypes = [vp, ctypes.c_int]
+    lib.qz_timing_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _dp, _ip, ctypes.c_int]
+    lib.qz_timing_report.restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def last_error() -> str:
+    return load_library().qz_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise QuantizerError(f"{what}: {last_error()}")
+    return SUCCESS
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+HDR_FIELDS = ["lv", "lvrel", "idx", "parent", "ts_x", "ts_y", "ts_z", "rd_x", "rd_y", "rd_z",
+              "min_x", "min_y", "min_z", "max_x", "max_y", "max_z", "ncell", "nintr", "nbdry", "nsim",
+              "nptcl_full", "nptcl_outs", "nchild", "gcap"]
+
+
+class Simulation:
+    """One device-resident PM simulation (head grid + refinement nodes + particles)."""
+
+    def __init__(self, particles, lmin: int, lmax: int | None = None, precision: int = PREC_FP64,
+                 head_bc: int = BC_MONOPOLE, ref_criterion_ptcl: int = 8, tol: float = 1e-10,
+                 tol_change: float | None = None, inner_cycles: int = 0, max_iterations: int = 5000,
+                 pre_sweeps: int = 2, post_sweeps: int = 2, G: float = 1.0):
+        lib = load_library()
+        cfg = QzConfig()
+        lib.qz_default_config(ctypes.byref(cfg))
+        cfg.lmin = lmin
+        cfg.lmax = lmin if lmax is None else lmax
+        cfg.n_particles = particles.n
+        cfg.precision = precision
+        cfg.head_bc = head_bc
+        cfg.ref_criterion_ptcl = ref_criterion_ptcl
+        cfg.tol = tol
+        cfg.tol_change = tol if tol_change is None else tol_change
+        cfg.inner_cycles = inner_cycles
+        cfg.max_iterations = max_iterations
+        cfg.pre_sweeps = pre_sweeps
+        cfg.post_sweeps = post_sweeps
+        cfg.G = G
+        self.cfg = cfg
+        self.lmin, self.lmax, self.n = cfg.lmin, cfg.lmax, particles.n
+        self._h = lib.qz_create(ctypes.byref(cfg))
+        if not self._h:
+            raise QuantizerError(f"qz_create: {last_error()}")
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in particles.arrays()]
+        _check(lib.qz_set_particles(self._h, *[_d(a) for a in arrs]), "qz_set_particles")
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().qz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------ step stages (main.c:391-478)
+    def particle_updating_A(self, dt: float):
+        return _check(load_library().qz_particle_updating_A(self._h, dt), "particle_updating_A")
+
+    def reset(self):
+        return _check(load_library().qz_reset(self._h), "reset")
+
+    def tree_adaptation(self):
+        return _check(load_library().qz_tree_adaptation(self._h), "tree_adaptation")
+
+    def grid_density(self):
+        return _check(load_library().qz_grid_density(self._h), "grid_density")
+
+    def potential(self):
+        return _check(load_library().qz_potential(self._h), "potential")
+
+    def potential_head_node(self):
+        return _check(load_library().qz_potential_head_node(self._h), "potential_head_node")
+
+    def potential_branches(self):
+        return _check(load_library().qz_potential_branches(self._h), "potential_branches")
+
+    def grid_acceleration(self):
+        return _check(load_library().qz_grid_acceleration(self._h), "grid_acceleration")
+
+    def particle_acceleration(self):
+        return _check(load_library().qz_particle_acceleration(self._h), "particle_acceleration")
+
+    def particle_updating_B(self, dt: float):
+        return _check(load_library().qz_particle_updating_B(self._h, dt), "particle_updating_B")
+
+    def force(self):
+        """Initial force pass (main.c:280-338): reset, tree, density, potential, accelerations."""
+        return _check(load_library().qz_force(self._h), "force")
+
+    def step(self, dt: float):
+        """One fixed-dt KDK step (main.c:391-478)."""
+        return _check(load_library().qz_step(self._h, dt), "step")
+
+    def sync(self):
+        return _check(load_library().qz_sync(self._h), "sync")
+
+    # ------------------------------------------------------------ data access
+    def particles(self) -> np.ndarray:
+        """(9, N) array x,y,z,vx,vy,vz,ax,ay,az in the original particle order."""
+        out = np.empty((9, self.n))
+        _check(load_library().qz_get_particles(self._h, *[_d(out[q]) for q in range(9)]), "qz_get_particles")
+        return out
+
+    def head_fields(self) -> np.ndarray:
+        """(5, n1, n1, n1) rho, pot, ax, ay, az of the head solve domain, indexed [field, z, y, x]."""
+        n1 = (1 << self.lmin) + 1
+        out = np.empty(5 * n1 ** 3)
+        _check(load_library().qz_get_head_fields(self._h, _d(out)), "qz_get_head_fields")
+        return out.reshape(5, n1, n1, n1)
+
+    def set_head_potential(self, pot: np.ndarray):
+        a = np.ascontiguousarray(pot, dtype=np.float64).reshape(-1)
+        return _check(load_library().qz_set_head_potential(self._h, _d(a)), "qz_set_head_potential")
+
+    def stats(self) -> dict:
+        st = QzStats()
+        _check(load_library().qz_get_stats(self._h, ctypes.byref(st)), "qz_get_stats")
+        return {f: getattr(st, f) for f, _ in QzStats._fields_}
+
+    def observables(self, exact_pe: bool = True) -> np.ndarray:
+        out = np.empty(6)
+        _check(load_library().qz_observables(self._h, 1 if exact_pe else 0, _d(out)), "qz_observables")
+        return out
+
+    def nodes(self) -> list:
+        lib = load_library()
+        res = []
+        for i in range(lib.qz_num_nodes(self._h)):
+            hdr = np.zeros(24, dtype=np.int32)
+            _check(lib.qz_node_header(self._h, i, _i(hdr)), "qz_node_header")
+            d = dict(zip(HDR_FIELDS, [int(v) for v in hdr]))
+            ix = np.zeros((max(d["nintr"], 1), 3), dtype=np.int32)
+            bx = np.zeros((max(d["nbdry"], 1), 3), dtype=np.int32)
+            sx = np.zeros((max(d["nsim"], 1), 3), dtype=np.int32)
+            _check(lib.qz_node_grid_lists(self._h, i, _i(ix), _i(bx), _i(sx)), "qz_node_grid_lists")
+            d["intr_xyz"], d["bdry_xyz"], d["sim_xyz"] = ix[: d["nintr"]], bx[: d["nbdry"]], sx[: d["nsim"]]
+            f = np.empty(5 * d["gcap"])
+            _check(lib.qz_node_fields(self._h, i, _d(f)), "qz_node_fields")
+            f = f.reshape(5, d["gcap"])
+            for k, nm in enumerate(("rho", "pot", "ax", "ay", "az")):
+                d[nm] = f[k]
+            res.append(d)
+        return res
+
+    # ------------------------------------------------------------ timing
+    def timing(self, on: bool = True):
+        return _check(load_library().qz_timing_enable(self._h, 1 if on else 0), "qz_timing_enable")
+
+    def timing_report(self) -> dict:
+        lib = load_library()
+        mx, nl = 64, 48
+        names = ctypes.create_string_buffer(mx * nl)
+        ms = np.zeros(mx)
+        cnt = np.zeros(mx, dtype=np.int32)
+        k = lib.qz_timing_report(self._h, names, nl, _d(ms), _i(cnt), mx)
+        raw = names.raw
+        res = {}
+        for q in range(k):
+            nm = raw[q * nl:(q + 1) * nl].split(b"\0", 1)[0].decode()
+            res[nm] = (float(ms[q]), int(cnt[q]))
+        return res
+
+
+def box_keys(x, y, z, lv: int, ts, rd) -> np.ndarray:
+    """ptcl_idx_to_box_idx (ptcl_idx_to_box_idx.c:30-139) for many points, on the device."""
+    lib = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    out = np.empty(x.shape[0], dtype=np.int32)
+    ts = np.ascontiguousarray(ts, dtype=np.int32)
+    rd = np.ascontiguousarray(rd, dtype=np.int32)
+    _check(lib.qz_keys(_d(x), _d(y), _d(z), x.shape[0], lv, _i(ts), _i(rd), _i(out)), "qz_keys")
+    return out

@@ -1,0 +1,547 @@
+// Refinement-node (branch) stages of Engine<P,G>.  Included by qz_core.hip.
+//
+// Tree build per step (the reference adapts its tree in tree_adaptation.c; this
+// rebuilds it the way tree_construction.c:2521-2577 builds it):
+//   device: refinement counts per cell from the sorted octree keys
+//   host:   dilated marks, 6-connected zones, child boxes (qz_tree.cpp)
+//   device: child cells, box status, particle ranges, masses, grid-point lists
+// Only the compact list of flagged cells crosses to the host, and only node
+// descriptors and zone cells cross back.
+#include "qz_branch_kernels.h"
+#include "qz_tree.h"
+
+namespace qz
+{
+
+struct TreeDev
+{
+  DNode *nodes = nullptr;
+  int node_cap = 0;
+  int *zone_node = nullptr, *zone_xyz = nullptr;
+  long long zone_cap = 0;
+  int *cell_xyz = nullptr, *cell_box = nullptr, *cstart = nullptr, *cend = nullptr;
+  long long cell_cap = 0;
+  int *box = nullptr, *bstart = nullptr, *bend = nullptr;
+  long long box_cap = 0;
+  int *intr_g = nullptr, *intr_xyz = nullptr, *bdry_g = nullptr, *bdry_xyz = nullptr, *sim_g = nullptr,
+      *sim_xyz = nullptr, *pn_i = nullptr, *pn_b = nullptr, *pn_s = nullptr;
+  long long pt_cap = 0;
+  double *phi = nullptr, *phi_old = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
+  void *rho = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr;
+  long long grid_cap = 0;
+  size_t gsize = 0; // sizeof(G)
+  int *head_child = nullptr;
+  int *flags = nullptr, *flag_count = nullptr;
+  int flag_cap = 0;
+  double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
+  long long part_cap = 0;
+  int *done_count = nullptr;
+  int *hflag_count = nullptr; // pinned
+  std::vector<DNode> hnodes;
+  // per relative level (index 1..): node range and pool ranges
+  std::vector<int> lfirst, lcount;
+  std::vector<long long> lc0, lncell, lb0, lnb, lz0, lnz, lg0, lng, lpmax;
+  long long used_zone = 0, used_cell = 0, used_box = 0, used_pt = 0, used_grid = 0;
+};
+
+template <typename T> static bool regrow(T *&p, long long newcap, long long used, const char *what)
+{
+  T *np = nullptr;
+  if (!hip_ok(hipMalloc((void **)&np, sizeof(T) * (size_t)std::max<long long>(newcap, 1)), what))
+    return false;
+  if (p && used > 0)
+    (void)hipMemcpy(np, p, sizeof(T) * (size_t)used, hipMemcpyDeviceToDevice);
+  (void)hipFree(p);
+  p = np;
+  return true;
+}
+
+static long long next_cap(long long cap, long long need) { return need <= cap ? cap : std::max(need, cap + cap / 2 + 1024); }
+
+template <typename P, typename G> void Engine<P, G>::release_tree()
+{
+  if (!tree)
+    return;
+  TreeDev *d = tree->dev;
+  if (d)
+  {
+    void *ptrs[] = {d->nodes, d->zone_node, d->zone_xyz, d->cell_xyz, d->cell_box, d->cstart, d->cend, d->box,
+                    d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
+                    d->pn_i, d->pn_b, d->pn_s, d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
+                    d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
+                    d->done_count};
+    for (void *q : ptrs)
+      (void)hipFree(q);
+    if (d->hflag_count)
+      (void)hipHostFree(d->hflag_count);
+    delete d;
+  }
+  delete tree->host;
+  delete tree;
+  tree = nullptr;
+}
+
+// Reserve pools for the level about to be uploaded (grow-only, contents kept).
+template <typename G> static bool reserve_level(TreeDev *d, long long nz, long long ncell, long long nb, long long npt,
+                                                long long ng)
+{
+  long long c = next_cap(d->zone_cap, d->used_zone + nz);
+  if (c != d->zone_cap)
+  {
+    if (!regrow(d->zone_node, c, d->used_zone, "zone pool") || !regrow(d->zone_xyz, 3 * c, 3 * d->used_zone, "zone pool"))
+      return false;
+    d->zone_cap = c;
+  }
+  c = next_cap(d->cell_cap, d->used_cell + ncell);
+  if (c != d->cell_cap)
+  {
+    long long u = d->used_cell;
+    if (!regrow(d->cell_box, c, u, "cell pool") || !regrow(d->cstart, c, u, "cell pool") ||
+        !regrow(d->cend, c, u, "cell pool") || !regrow(d->cell_xyz, 3 * c, 3 * u, "cell pool"))
+      return false;
+    d->cell_cap = c;
+  }
+  c = next_cap(d->box_cap, d->used_box + nb);
+  if (c != d->box_cap)
+  {
+    long long u = d->used_box;
+    if (!regrow(d->box, c, u, "box pool") || !regrow(d->bstart, c, u, "box pool") || !regrow(d->bend, c, u, "box pool"))
+      return false;
+    d->box_cap = c;
+  }
+  c = next_cap(d->pt_cap, d->used_pt + npt);
+  if (c != d->pt_cap)
+  {
+    long long u = d->used_pt;
+    if (!regrow(d->intr_g, c, u, "point pool") || !regrow(d->bdry_g, c, u, "point pool") ||
+        !regrow(d->sim_g, c, u, "point pool") || !regrow(d->pn_i, c, u, "point pool") ||
+        !regrow(d->pn_b, c, u, "point pool") || !regrow(d->pn_s, c, u, "point pool") ||
+        !regrow(d->intr_xyz, 3 * c, 3 * u, "point pool") || !regrow(d->bdry_xyz, 3 * c, 3 * u, "point pool") ||
+        !regrow(d->sim_xyz, 3 * c, 3 * u, "point pool"))
+      return false;
+    d->pt_cap = c;
+  }
+  c = next_cap(d->grid_cap, d->used_grid + ng);
+  if (c != d->grid_cap)
+  {
+    long long u = d->used_grid;
+    G *rho = (G *)d->rho, *ax = (G *)d->ax, *ay = (G *)d->ay, *az = (G *)d->az;
+    if (!regrow(d->phi, c, u, "grid pool") || !regrow(d->phi_old, c, u, "grid pool") ||
+        !regrow(d->cg_r, c, u, "grid pool") || !regrow(d->cg_p, c, u, "grid pool") ||
+        !regrow(d->cg_Ap, c, u, "grid pool") || !regrow(rho, c, u, "grid pool") || !regrow(ax, c, u, "grid pool") ||
+        !regrow(ay, c, u, "grid pool") || !regrow(az, c, u, "grid pool"))
+      return false;
+    d->rho = rho;
+    d->ax = ax;
+    d->ay = ay;
+    d->az = az;
+    d->grid_cap = c;
+  }
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
+{
+  if (cfg.lmin == cfg.lmax)
+    return true;
+  if (!tree)
+  {
+    tree = new TreeState();
+    tree->host = new TreeHost();
+    tree->host->init(cfg.lmin, cfg.lmax, cfg.ref_criterion_ptcl);
+    tree->dev = new TreeDev();
+    TreeDev *d = tree->dev;
+    d->gsize = sizeof(G);
+    size_t ncell = (size_t)n * n * n;
+    d->flag_cap = std::min<long long>((long long)ncell, (long long)N / std::max(1, cfg.ref_criterion_ptcl) + 16);
+    if (!hip_ok(hipMalloc((void **)&d->head_child, sizeof(int) * ncell), "head_child") ||
+        !hip_ok(hipMalloc((void **)&d->flags, sizeof(int) * 2 * (size_t)d->flag_cap), "flags") ||
+        !hip_ok(hipMalloc((void **)&d->flag_count, sizeof(int) * 4), "flag count") ||
+        !hip_ok(hipMalloc((void **)&d->done_count, sizeof(int) * 4), "done count") ||
+        !hip_ok(hipHostMalloc((void **)&d->hflag_count, sizeof(int) * 4), "pinned"))
+      return false;
+  }
+  if (!sort_particles())
+    return false;
+  TreeHost &th = *tree->host;
+  TreeDev &d = *tree->dev;
+  const size_t ncell_head = (size_t)n * n * n;
+  // ---- head refinement counts (tree_construction.c:527-575)
+  QZ_LAUNCH("tree_fill", k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
+  QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.flag_count, 0, (size_t)1);
+  QZ_LAUNCH("tree_flags", k_flag_head, nblk(ncell_head), cell_start, (int)ncell_head, cfg.ref_criterion_ptcl,
+            d.flag_count, d.flags, d.flag_cap);
+  if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "flags") ||
+      !hip_ok(hipStreamSynchronize(stream), "flags"))
+    return false;
+  int nflag = std::min(*d.hflag_count, d.flag_cap);
+  std::vector<int> flagged(nflag);
+  if (nflag > 0 &&
+      !hip_ok(hipMemcpy(flagged.data(), d.flags, sizeof(int) * nflag, hipMemcpyDeviceToHost), "flag list"))
+    return false;
+  std::sort(flagged.begin(), flagged.end());
+  th.refine_head(flagged);
+  d.hnodes.clear();
+  d.lfirst.assign(1, 0);
+  d.lcount.assign(1, 1);
+  d.lc0.assign(1, 0);
+  d.lncell.assign(1, 0);
+  d.lb0.assign(1, 0);
+  d.lnb.assign(1, 0);
+  d.lz0.assign(1, 0);
+  d.lnz.assign(1, 0);
+  d.lg0.assign(1, 0);
+  d.lng.assign(1, 0);
+  d.lpmax.assign(1, 0);
+  d.used_zone = d.used_cell = d.used_box = d.used_pt = d.used_grid = 0;
+  int lr = 1;
+  while (lr < (int)th.lev_start.size())
+  {
+    int first = th.lev_start[lr], count = th.lev_count[lr];
+    // ---- descriptors and zone cells of this level
+    long long nz = 0, ncell = 0, nb = 0, npt = 0, ng = 0, pmax = 0;
+    for (int q = first; q < first + count; q++)
+    {
+      const HNode &h = th.nodes[q];
+      nz += h.ncell / 8;
+      ncell += h.ncell;
+      nb += h.bcap;
+      npt += h.pcap;
+      ng += h.gcap;
+      pmax = std::max<long long>(pmax, h.pcap);
+    }
+    if (!reserve_level<G>(&d, nz, ncell, nb, npt, ng))
+      return false;
+    std::vector<int> zn(nz), zx(3 * nz);
+    long long zo = 0;
+    for (int q = first; q < first + count; q++)
+    {
+      HNode &h = th.nodes[q];
+      DNode dn{};
+      dn.lv = h.lv;
+      dn.parent = h.parent;
+      for (int a = 0; a < 3; a++)
+      {
+        dn.ts[a] = h.ts[a];
+        dn.rd[a] = h.rd[a];
+        dn.bmin[a] = h.bmin[a];
+        dn.bmax[a] = h.bmax[a];
+        if (h.sc[a])
+          dn.simc |= 1 << a;
+      }
+      dn.ncell = h.ncell;
+      dn.coff = (int)d.used_cell;
+      dn.zoff = (int)d.used_zone;
+      dn.pcap = h.pcap;
+      dn.poff = (int)d.used_pt;
+      dn.goff = d.used_grid;
+      dn.boff = d.used_box;
+      h.goff = dn.goff;
+      h.boff = dn.boff;
+      h.coff = dn.coff;
+      h.poff = dn.poff;
+      for (int j = 0; j < h.ncell / 8; j++)
+      {
+        // the zone cell is the parent of octant 0 of child cell 8j
+        zn[zo] = q;
+        for (int a = 0; a < 3; a++)
+          zx[3 * zo + a] = h.cxyz[3 * (8 * j) + a] >> 1;
+        zo++;
+      }
+      d.used_zone += h.ncell / 8;
+      d.used_cell += h.ncell;
+      d.used_box += h.bcap;
+      d.used_pt += h.pcap;
+      d.used_grid += h.gcap;
+      d.hnodes.push_back(dn);
+    }
+    long long z0 = d.used_zone - nz, c0 = d.used_cell - ncell, b0 = d.used_box - nb, g0 = d.used_grid - ng;
+    d.lfirst.push_back(first);
+    d.lcount.push_back(count);
+    d.lc0.push_back(c0);
+    d.lncell.push_back(ncell);
+    d.lb0.push_back(b0);
+    d.lnb.push_back(nb);
+    d.lz0.push_back(z0);
+    d.lnz.push_back(nz);
+    d.lg0.push_back(g0);
+    d.lng.push_back(ng);
+    d.lpmax.push_back(pmax);
+    if ((long long)d.hnodes.size() > d.node_cap)
+    {
+      long long cap = next_cap(d.node_cap, (long long)d.hnodes.size() + 64);
+      if (!regrow(d.nodes, cap, (long long)first, "node table"))
+        return false;
+      d.node_cap = (int)cap;
+    }
+    if (!hip_ok(hipMemcpyAsync(d.nodes + first, d.hnodes.data() + first, sizeof(DNode) * count,
+                               hipMemcpyHostToDevice, stream),
+                "node table") ||
+        !hip_ok(hipMemcpyAsync(d.zone_node + z0, zn.data(), sizeof(int) * nz, hipMemcpyHostToDevice, stream),
+                "zones") ||
+        !hip_ok(hipMemcpyAsync(d.zone_xyz + 3 * z0, zx.data(), sizeof(int) * 3 * nz, hipMemcpyHostToDevice, stream),
+                "zones"))
+      return false;
+    // ---- device: boxes, cells, ranges, masses, grid-point lists
+    QZ_LAUNCH("tree_boxes", k_init_boxes, nblk(nb), d.nodes, first, count, b0, nb, d.box, d.bstart, d.bend);
+    QZ_LAUNCH("tree_cells", k_expand_cells, nblk(ncell), d.nodes, d.zone_node, d.zone_xyz, (int)c0, (int)ncell,
+              cfg.lmin, D, key_out, N, d.cell_xyz, d.cell_box, d.cstart, d.cend, d.box, d.bstart, d.bend);
+    timer.begin(stream);
+    hipLaunchKernelGGL(k_node_mass<P>, dim3(count), dim3(kBlock), 0, stream, d.nodes, first, d.cstart, d.cend, pa[9]);
+    hipLaunchKernelGGL(k_classify, dim3(count), dim3(kClassBlock), 0, stream, d.nodes, first, d.box, d.intr_g,
+                       d.intr_xyz, d.bdry_g, d.bdry_xyz, d.sim_g, d.sim_xyz, d.pn_i, d.pn_b, d.pn_s);
+    timer.end(stream, "tree_classify");
+    QZ_LAUNCH("tree_children", k_mark_children, nblk(nz), d.nodes, d.zone_node, d.zone_xyz, (int)z0, (int)nz, n,
+              d.head_child, d.box);
+    // zero the level's grid fields (points that are never listed stay 0, as calloc leaves them)
+    QZ_LAUNCH("tree_fill", k_fill<double>, nblk(ng), d.phi + g0, 0.0, (size_t)ng);
+    QZ_LAUNCH("tree_fill", k_fill<G>, nblk(ng), (G *)d.ax + g0, (G)0, (size_t)ng);
+    QZ_LAUNCH("tree_fill", k_fill<G>, nblk(ng), (G *)d.ay + g0, (G)0, (size_t)ng);
+    QZ_LAUNCH("tree_fill", k_fill<G>, nblk(ng), (G *)d.az + g0, (G)0, (size_t)ng);
+    // ---- next level's refinement counts
+    int lv = cfg.lmin + lr;
+    if (lv < cfg.lmax)
+    {
+      QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.flag_count, 0, (size_t)1);
+      QZ_LAUNCH("tree_flags", k_flag_cells, nblk(ncell), d.nodes, d.zone_node, first, (int)c0, (int)ncell, d.cstart,
+                d.cend, cfg.ref_criterion_ptcl, d.flag_count, d.flags, d.flag_cap);
+      if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "flags") ||
+          !hip_ok(hipStreamSynchronize(stream), "flags"))
+        return false;
+      int nf = std::min(*d.hflag_count, d.flag_cap);
+      std::vector<int> fl(2 * (size_t)nf);
+      if (nf > 0 && !hip_ok(hipMemcpy(fl.data(), d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost), "flags"))
+        return false;
+      std::vector<std::pair<int, int>> pr(nf);
+      for (int i = 0; i < nf; i++)
+        pr[i] = {fl[2 * i], fl[2 * i + 1]};
+      std::sort(pr.begin(), pr.end());
+      std::vector<int> fn(nf), fc(nf);
+      for (int i = 0; i < nf; i++)
+      {
+        fn[i] = pr[i].first;
+        fc[i] = pr[i].second;
+      }
+      th.refine_level(lr, fn, fc);
+    }
+    lr++;
+  }
+  tree->n_nodes = (int)th.nodes.size();
+  // fetch the classified counts and masses
+  if (tree->n_nodes > 0 &&
+      !hip_ok(hipMemcpyAsync(d.hnodes.data(), d.nodes, sizeof(DNode) * tree->n_nodes, hipMemcpyDeviceToHost, stream),
+              "node table"))
+    return false;
+  if (!hip_ok(hipStreamSynchronize(stream), "tree_adaptation"))
+    return false;
+  stats.n_nodes = tree->n_nodes;
+  stats.n_levels = (int)d.lfirst.size();
+  stats.branch_points = 0;
+  stats.branch_intr = 0;
+  for (auto &dn : d.hnodes)
+  {
+    stats.branch_points += dn.nintr + dn.nbdry + dn.nsim;
+    stats.branch_intr += dn.nintr;
+  }
+  // the partial-sum buffers: nodes x blocks per node
+  long long need = 0;
+  for (size_t l = 1; l < d.lfirst.size(); l++)
+    need = std::max<long long>(need, (long long)d.lcount[l] * blocks_per_node(d.lpmax[l]));
+  if (need > d.part_cap)
+  {
+    (void)hipFree(d.part1);
+    (void)hipFree(d.part2);
+    (void)hipFree(d.part3);
+    d.part_cap = need + 1024;
+    if (!hip_ok(hipMalloc((void **)&d.part1, sizeof(double) * d.part_cap), "partials") ||
+        !hip_ok(hipMalloc((void **)&d.part2, sizeof(double) * d.part_cap), "partials") ||
+        !hip_ok(hipMalloc((void **)&d.part3, sizeof(double) * d.part_cap), "partials"))
+      return false;
+  }
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::branch_density()
+{
+  TreeDev &d = *tree->dev;
+  for (size_t l = 1; l < d.lfirst.size(); l++)
+  {
+    QZ_LAUNCH("branch_fill", k_fill<G>, nblk(d.lng[l]), (G *)d.rho + d.lg0[l], (G)0, (size_t)d.lng[l]);
+    dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
+    timer.begin(stream);
+    hipLaunchKernelGGL((k_branch_deposit<P, G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
+                       d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz, d.bstart, d.bend, pa[0], pa[1], pa[2],
+                       pa[9], (G *)d.rho, 4.0 * cfg.G * kPi);
+    timer.end(stream, "deposit_branch");
+  }
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::potential_branches()
+{
+  if (!has_tree())
+    return true;
+  TreeDev &d = *tree->dev;
+  stats.branch_iterations = 0;
+  stats.branch_iter_points = 0;
+  const int zero_bc = cfg.head_bc == QZ_BC_ZERO ? 1 : 0;
+  for (size_t l = 1; l < d.lfirst.size(); l++)
+  {
+    const int first = d.lfirst[l], count = d.lcount[l];
+    const int nbx = blocks_per_node(d.lpmax[l]);
+    dim3 grid(nbx, count);
+    timer.begin(stream);
+    hipLaunchKernelGGL(k_branch_init_pot, grid, dim3(kBlock), 0, stream, d.nodes, first, d.sim_g, d.sim_xyz, d.bdry_g,
+                       d.bdry_xyz, d.intr_g, d.intr_xyz, d.phi, d.phi_old, phi, n, cfg.G * total_mass, zero_bc);
+    timer.end(stream, "branch_init_pot");
+    QZ_LAUNCH("branch_fill", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
+    timer.begin(stream);
+    hipLaunchKernelGGL((k_cg_init<G>), grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.phi, (G *)d.rho,
+                       d.cg_r, d.cg_p, d.part2);
+    hipLaunchKernelGGL(k_cg_init_fold, dim3(count), dim3(kBlock), 0, stream, d.nodes, first, d.part2, nbx);
+    timer.end(stream, "cg_init");
+    QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.done_count, 0, (size_t)1);
+    int max_it = cfg.max_iterations; // cycle_size = min(size, 5000) per node; see conjugate_gradient.c:377
+    int it = 0;
+    bool all_done = false;
+    for (; it < max_it; it++)
+    {
+      int check = ((it + 1) % 5 == 0) ? 1 : 0;
+      timer.begin(stream);
+      hipLaunchKernelGGL(k_cg_ap, grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.cg_p, d.cg_Ap, d.part1);
+      hipLaunchKernelGGL(k_cg_update, grid, dim3(kBlock), 0, stream, d.nodes, first, it, check, d.intr_g, d.phi,
+                         d.phi_old, d.cg_r, d.cg_p, d.cg_Ap, d.part1, d.part2, d.part3, cfg.tol_change);
+      hipLaunchKernelGGL(k_cg_direction, grid, dim3(kBlock), 0, stream, d.nodes, first, it, check, d.intr_g, d.phi,
+                         d.phi_old, d.cg_r, d.cg_p, d.part2, d.part3, cfg.tol, cfg.G, d.done_count);
+      timer.end(stream, "branch_cg");
+      if (check)
+      {
+        if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "cg") ||
+            !hip_ok(hipStreamSynchronize(stream), "cg"))
+          return false;
+        if (*d.hflag_count >= count)
+        {
+          all_done = true;
+          break;
+        }
+      }
+    }
+    if (!all_done)
+    {
+      set_error("conjugate_gradient did not converge on level %d after %d iterations", cfg.lmin + (int)l, it);
+      return false;
+    }
+    if (!hip_ok(hipMemcpyAsync(d.hnodes.data() + first, d.nodes + first, sizeof(DNode) * count,
+                               hipMemcpyDeviceToHost, stream),
+                "node table") ||
+        !hip_ok(hipStreamSynchronize(stream), "cg"))
+      return false;
+    for (int q = first; q < first + count; q++)
+    {
+      stats.branch_iterations = std::max(stats.branch_iterations, d.hnodes[q].done);
+      stats.branch_iter_points += (long long)d.hnodes[q].done * d.hnodes[q].nintr;
+    }
+  }
+  return hip_ok(hipStreamSynchronize(stream), "potential_branches");
+}
+
+template <typename P, typename G> bool Engine<P, G>::branch_grid_acceleration()
+{
+  TreeDev &d = *tree->dev;
+  const int zero_bc = cfg.head_bc == QZ_BC_ZERO ? 1 : 0;
+  for (size_t l = 1; l < d.lfirst.size(); l++)
+  {
+    dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
+    timer.begin(stream);
+    hipLaunchKernelGGL((k_branch_grid_acc<G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
+                       d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.phi, (G *)d.ax, (G *)d.ay, (G *)d.az, gax, gay, gaz,
+                       n, cfg.G * total_mass, zero_bc);
+    timer.end(stream, "grid_acc_branch");
+  }
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::branch_particle_acceleration()
+{
+  TreeDev &d = *tree->dev;
+  QZ_LAUNCH("gather_amr", (k_gather_amr<P, G>), nblk(N), pa[0], pa[1], pa[2], gax, gay, gaz, d.head_child, d.nodes,
+            d.box, (G *)d.ax, (G *)d.ay, (G *)d.az, pa[6], pa[7], pa[8], N, cfg.lmin);
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::node_header(int i, int *h)
+{
+  if (!has_tree() || i < 0 || i >= tree->n_nodes)
+  {
+    set_error("node index out of range");
+    return false;
+  }
+  const HNode &hn = tree->host->nodes[i];
+  const DNode &dn = tree->dev->hnodes[i];
+  int lvrel = hn.lv - cfg.lmin;
+  const auto &ls = tree->host->lev_start;
+  int parent_pos = hn.parent < 0 ? 0 : hn.parent - ls[lvrel - 1];
+  int nch = 0;
+  for (const auto &o : tree->host->nodes)
+    if (o.parent == i)
+      nch++;
+  int hdr[24] = {hn.lv, lvrel, i - ls[lvrel], parent_pos, hn.ts[0], hn.ts[1], hn.ts[2], hn.rd[0], hn.rd[1], hn.rd[2],
+                 hn.bmin[0], hn.bmin[1], hn.bmin[2], hn.bmax[0], hn.bmax[1], hn.bmax[2], hn.ncell, dn.nintr, dn.nbdry,
+                 dn.nsim, dn.nptcl, 0, nch, hn.gcap};
+  // particles outside the refinement zones (tree_construction.c:2286-2287)
+  int inside = 0;
+  for (const auto &o : tree->host->nodes)
+    if (o.parent == i)
+      inside += tree->dev->hnodes[&o - &tree->host->nodes[0]].nptcl;
+  hdr[21] = dn.nptcl - inside;
+  memcpy(h, hdr, sizeof(hdr));
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::node_lists(int i, int *ixyz, int *bxyz, int *sxyz)
+{
+  if (!has_tree() || i < 0 || i >= tree->n_nodes)
+  {
+    set_error("node index out of range");
+    return false;
+  }
+  const DNode &dn = tree->dev->hnodes[i];
+  TreeDev &d = *tree->dev;
+  return hip_ok(hipMemcpy(ixyz, d.intr_xyz + 3 * (size_t)dn.poff, sizeof(int) * 3 * dn.nintr, hipMemcpyDeviceToHost),
+                "lists") &&
+         hip_ok(hipMemcpy(bxyz, d.bdry_xyz + 3 * (size_t)dn.poff, sizeof(int) * 3 * dn.nbdry, hipMemcpyDeviceToHost),
+                "lists") &&
+         hip_ok(hipMemcpy(sxyz, d.sim_xyz + 3 * (size_t)dn.poff, sizeof(int) * 3 * dn.nsim, hipMemcpyDeviceToHost),
+                "lists");
+}
+
+template <typename P, typename G> bool Engine<P, G>::node_fields(int i, double *out5)
+{
+  if (!has_tree() || i < 0 || i >= tree->n_nodes)
+  {
+    set_error("node index out of range");
+    return false;
+  }
+  const DNode &dn = tree->dev->hnodes[i];
+  const HNode &hn = tree->host->nodes[i];
+  TreeDev &d = *tree->dev;
+  size_t g = hn.gcap;
+  std::vector<G> tmp(g);
+  G *src[5] = {(G *)d.rho, nullptr, (G *)d.ax, (G *)d.ay, (G *)d.az};
+  for (int a = 0; a < 5; a++)
+  {
+    if (a == 1)
+    {
+      if (!hip_ok(hipMemcpy(out5 + g, d.phi + dn.goff, sizeof(double) * g, hipMemcpyDeviceToHost), "fields"))
+        return false;
+      continue;
+    }
+    if (!hip_ok(hipMemcpy(tmp.data(), src[a] + dn.goff, sizeof(G) * g, hipMemcpyDeviceToHost), "fields"))
+      return false;
+    for (size_t t = 0; t < g; t++)
+      out5[a * g + t] = (double)tmp[t];
+  }
+  return true;
+}
+
+} // namespace qz

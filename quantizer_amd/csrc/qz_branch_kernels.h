@@ -1,0 +1,835 @@
+// Device kernels for refinement nodes ("branches").  All nodes of one level are
+// processed by one launch: a 2-D grid whose y index is the node and whose x
+// blocks stride over that node's points (or one block per node where an
+// ordered compaction is needed).  Deterministic: no float atomics; per-node
+// reductions fold fixed per-block partials in a fixed order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "qz_kernels.h"
+
+namespace qz
+{
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// x-blocks per node for the 2-D node launches: enough to cover the largest
+// node's point list with ~8 points per thread, at most 64.
+inline int blocks_per_node(long long pmax)
+{
+  long long b = (pmax + 8 * kBlock - 1) / (8 * kBlock);
+  return (int)std::min<long long>(64, std::max<long long>(1, b));
+}
+
+struct DNode
+{
+  int lv;     // absolute level
+  int parent; // global node index, -1 = head
+  int ts[3], rd[3], bmin[3], bmax[3];
+  int simc;   // bit a set: node touches the simulation boundary on axis a
+  int ncell, coff, zoff;
+  int pcap, poff; // point-list slot range (upper bound of points)
+  int nintr, nbdry, nsim;
+  long long goff, boff; // grid pool and box pool offsets
+  double mass;
+  int nptcl;
+  int done; // CG converged
+  double rs[2];
+};
+
+struct ParentView
+{
+  const double *phi; // parent potential (head: head array)
+  const void *ax, *ay, *az;
+};
+
+__device__ inline int dnode_gX(const DNode &d) { return d.rd[0] + 1; }
+__device__ inline int dnode_gXY(const DNode &d) { return (d.rd[0] + 1) * (d.rd[1] + 1); }
+
+// Key range of a cell at relative level lr (1..D) with global coordinates c.
+__device__ inline void cell_key_range(int cx, int cy, int cz, int lr, int lmin, int D, unsigned long long &lo,
+                                      unsigned long long &hi)
+{
+  const int n = 1 << lmin;
+  unsigned long long k = (unsigned long long)(cx >> lr) +
+                         (unsigned long long)n * ((unsigned long long)(cy >> lr) + (unsigned long long)n * (cz >> lr));
+  for (int t = lr - 1; t >= 0; t--)
+    k = (k << 3) | (unsigned long long)(((cx >> t) & 1) | (((cy >> t) & 1) << 1) | (((cz >> t) & 1) << 2));
+  lo = k << (3 * (D - lr));
+  hi = (k + 1) << (3 * (D - lr));
+}
+
+__device__ inline int lower_bound_u64(const unsigned long long *a, int n, unsigned long long v)
+{
+  int lo = 0, hi = n;
+  while (lo < hi)
+  {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Which node of the level owns a slot of a concatenated pool (offsets ascending).
+__device__ inline int find_node_ll(const DNode *nodes, int first, int count, long long slot, int which)
+{
+  int lo = 0, hi = count - 1;
+  while (lo < hi)
+  {
+    int mid = (lo + hi + 1) >> 1;
+    long long off = which == 0 ? nodes[first + mid].boff : nodes[first + mid].goff;
+    if (off <= slot)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return first + lo;
+}
+
+// Box status of the level's nodes: -4 everywhere, -5 on simulation-boundary
+// slabs (tree_construction.c:1718-1935, non-pbc).
+__global__ void k_init_boxes(const DNode *__restrict__ nodes, int first, int count, long long b0, long long nb,
+                             int *__restrict__ box, int *__restrict__ bstart, int *__restrict__ bend)
+{
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nb)
+    return;
+  long long slot = b0 + s;
+  int q = find_node_ll(nodes, first, count, slot, 0);
+  const DNode &d = nodes[q];
+  long long b = slot - d.boff;
+  int bcap = d.rd[0] * d.rd[1] * d.rd[2];
+  if (b >= bcap)
+    return;
+  int i = (int)(b % d.rd[0]), j = (int)((b / d.rd[0]) % d.rd[1]), k = (int)(b / ((long long)d.rd[0] * d.rd[1]));
+  int v = -4;
+  const int top = (1 << d.lv) - 1;
+  const int c[3] = {i, j, k};
+  for (int a = 0; a < 3; a++)
+  {
+    if (!((d.simc >> a) & 1))
+      continue;
+    int lo_end = d.bmin[a] == 0 ? d.bmin[a] - d.ts[a] : 0;
+    int hi_beg = d.bmax[a] == top ? d.bmax[a] + 1 - d.ts[a] : d.rd[a];
+    if (c[a] < lo_end || c[a] >= hi_beg)
+      v = -5;
+  }
+  box[slot] = v;
+  bstart[slot] = 0;
+  bend[slot] = 0;
+}
+
+// Child cells (tree_construction.c:1663-1716): cell s of the level is octant
+// (s & 7) of zone cell s >> 3; mark it existing (-3) and find its particle
+// slot range in the sorted octree keys.
+__global__ void k_expand_cells(const DNode *__restrict__ nodes, const int *__restrict__ zone_node,
+                               const int *__restrict__ zone_xyz, int c0, int ncell, int lmin, int D,
+                               const unsigned long long *__restrict__ keys, int N, int *__restrict__ cell_xyz,
+                               int *__restrict__ cell_box, int *__restrict__ cstart, int *__restrict__ cend,
+                               int *__restrict__ box, int *__restrict__ bstart, int *__restrict__ bend)
+{
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ncell)
+    return;
+  int cs = c0 + s;
+  int z = cs >> 3, o = cs & 7;
+  int q = zone_node[z];
+  const DNode &d = nodes[q];
+  int gx = 2 * zone_xyz[3 * z] + (o & 1), gy = 2 * zone_xyz[3 * z + 1] + ((o >> 1) & 1),
+      gz = 2 * zone_xyz[3 * z + 2] + ((o >> 2) & 1);
+  int b = (gx - d.ts[0]) + (gy - d.ts[1]) * d.rd[0] + (gz - d.ts[2]) * d.rd[0] * d.rd[1];
+  cell_xyz[3 * cs] = gx;
+  cell_xyz[3 * cs + 1] = gy;
+  cell_xyz[3 * cs + 2] = gz;
+  cell_box[cs] = b;
+  unsigned long long lo, hi;
+  cell_key_range(gx, gy, gz, d.lv - lmin, lmin, D, lo, hi);
+  int a = lower_bound_u64(keys, N, lo), e = lower_bound_u64(keys, N, hi);
+  cstart[cs] = a;
+  cend[cs] = e;
+  box[d.boff + b] = -3;
+  bstart[d.boff + b] = a;
+  bend[d.boff + b] = e;
+}
+
+// Per-node particle count and mass (tree_construction.c:2020-2022), one block per node.
+template <typename P>
+__global__ void k_node_mass(DNode *__restrict__ nodes, int first, const int *__restrict__ cstart,
+                            const int *__restrict__ cend, const P *__restrict__ m)
+{
+  DNode &d = nodes[first + blockIdx.x];
+  double mass = 0.0, cnt = 0.0;
+  for (int c = threadIdx.x; c < d.ncell; c += blockDim.x)
+  {
+    int a = cstart[d.coff + c], e = cend[d.coff + c];
+    cnt += (double)(e - a);
+    for (int p = a; p < e; p++)
+      mass += (double)m[p];
+  }
+  mass = block_reduce<0>(mass);
+  cnt = block_reduce<0>(cnt);
+  if (threadIdx.x == 0)
+  {
+    d.mass = mass;
+    d.nptcl = (int)cnt;
+  }
+}
+
+// Grid-point classification of create_child_nodes (tree_construction.c:2038-2268)
+// from the existence pattern of the 8 cells around a point: bit q of the index
+// is cell q in the order i0j0k0, im1j0k0, i0jm1k0, im1jm1k0, i0j0km1, im1j0km1,
+// i0jm1km1, im1jm1km1.  Table value: bit0 exists, bit1 boundary.
+__host__ __device__ inline int classify_pattern(int e)
+{
+  auto X = [&](int q) { return (e >> q) & 1; };
+  auto NX = [&](int q) { return !((e >> q) & 1); };
+  int exist = 0, bdry = 0;
+  if (X(0))
+  {
+    exist = 1;
+    bdry = (NX(1) && NX(3) && NX(5) && NX(7)) || (NX(2) && NX(3) && NX(6) && NX(7)) ||
+           (NX(4) && NX(5) && NX(6) && NX(7));
+  }
+  else if (X(1))
+  {
+    exist = 1;
+    bdry = (NX(2) && NX(4) && NX(6)) || (NX(2) && NX(3) && NX(6) && NX(7)) || (NX(4) && NX(5) && NX(6) && NX(7));
+  }
+  else if (X(2))
+  {
+    exist = 1;
+    bdry = (NX(3) && NX(5) && NX(7)) || (NX(4) && NX(5)) || (NX(4) && NX(5) && NX(6) && NX(7));
+  }
+  else if (X(3))
+  {
+    exist = 1;
+    bdry = (NX(4) && NX(6)) || (NX(4) && NX(5)) || (NX(4) && NX(5) && NX(6) && NX(7));
+  }
+  else if (X(4) || X(5) || X(6) || X(7))
+  {
+    exist = 1;
+    bdry = 1;
+  }
+  return exist | (bdry << 1);
+}
+
+// One block per node: classify the points of the smallest box grid in the
+// reference's loop order (kk, jj, ii) and compact them, in order, into the
+// node's interior / boundary / simulation-boundary slots.
+constexpr int kClassBlock = 1024;
+__global__ void __launch_bounds__(kClassBlock)
+    k_classify(DNode *__restrict__ nodes, int first, const int *__restrict__ box, int *__restrict__ intr_g,
+               int *__restrict__ intr_xyz, int *__restrict__ bdry_g, int *__restrict__ bdry_xyz,
+               int *__restrict__ sim_g, int *__restrict__ sim_xyz, int *__restrict__ pt_node_i,
+               int *__restrict__ pt_node_b, int *__restrict__ pt_node_s)
+{
+  __shared__ int wsum[3][kClassBlock / 64];
+  __shared__ int base[3];
+  const int q = first + blockIdx.x;
+  DNode &d = nodes[q];
+  const int lx = d.bmax[0] - d.bmin[0] + 2, ly = d.bmax[1] - d.bmin[1] + 2, lz = d.bmax[2] - d.bmin[2] + 2;
+  const int total = lx * ly * lz;
+  const int rX = d.rd[0], rXY = d.rd[0] * d.rd[1];
+  const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
+  const int *bx = box + d.boff;
+  const int top = 1 << d.lv;
+  if (threadIdx.x < 3)
+    base[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int chunk = 0; chunk < total; chunk += kClassBlock)
+  {
+    int t = chunk + threadIdx.x;
+    int kind = -1; // 0 intr, 1 bdry, 2 sim
+    int gidx = 0, gx = 0, gy = 0, gz = 0;
+    if (t < total)
+    {
+      int ii = d.bmin[0] - d.ts[0] + t % lx;
+      int jj = d.bmin[1] - d.ts[1] + (t / lx) % ly;
+      int kk = d.bmin[2] - d.ts[2] + t / (lx * ly);
+      int b = ii + jj * rX + kk * rXY;
+      const int nb[8] = {b, b - 1, b - rX, b - 1 - rX, b - rXY, b - 1 - rXY, b - rX - rXY, b - 1 - rX - rXY};
+      int e = 0;
+      for (int qq = 0; qq < 8; qq++)
+        e |= (bx[nb[qq]] > -4 ? 1 : 0) << qq;
+      int cls = classify_pattern(e);
+      if (cls & 1)
+      {
+        int bd = (cls >> 1) & 1;
+        gx = ii + d.ts[0];
+        gy = jj + d.ts[1];
+        gz = kk + d.ts[2];
+        bool simb = d.simc != 0 && bd && (gx == 0 || gx == top || gy == 0 || gy == top || gz == 0 || gz == top);
+        kind = simb ? 2 : (bd ? 1 : 0);
+        gidx = ii + jj * gX + kk * gXY;
+      }
+    }
+    // ordered compaction: wave ballots, then a block prefix over waves
+    int my_off[3];
+    for (int c = 0; c < 3; c++)
+    {
+      unsigned long long mask = __ballot(kind == c);
+      int below = __popcll(mask & ((1ull << lane) - 1ull));
+      my_off[c] = below;
+      if (lane == 0)
+        wsum[c][wid] = __popcll(mask);
+    }
+    __syncthreads();
+    int nw = kClassBlock / 64;
+    int wpre[3];
+    for (int c = 0; c < 3; c++)
+    {
+      int acc = 0;
+      for (int w = 0; w < wid; w++)
+        acc += wsum[c][w];
+      wpre[c] = acc;
+    }
+    if (kind >= 0)
+    {
+      int slot = d.poff + base[kind] + wpre[kind] + my_off[kind];
+      int *g = kind == 0 ? intr_g : (kind == 1 ? bdry_g : sim_g);
+      int *xyz = kind == 0 ? intr_xyz : (kind == 1 ? bdry_xyz : sim_xyz);
+      int *pn = kind == 0 ? pt_node_i : (kind == 1 ? pt_node_b : pt_node_s);
+      g[slot] = gidx;
+      xyz[3 * slot] = gx;
+      xyz[3 * slot + 1] = gy;
+      xyz[3 * slot + 2] = gz;
+      pn[slot] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+    {
+      int acc = 0;
+      for (int w = 0; w < nw; w++)
+        acc += wsum[threadIdx.x][w];
+      base[threadIdx.x] += acc;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+  {
+    d.nintr = base[0];
+    d.nbdry = base[1];
+    d.nsim = base[2];
+  }
+}
+
+// Refinement criterion counts (tree_construction.c:529): cells of the level
+// holding >= crit particles, appended as (node position in level, cell position).
+__global__ void k_flag_cells(const DNode *__restrict__ nodes, const int *__restrict__ zone_node, int first, int c0,
+                             int ncell, const int *__restrict__ cstart, const int *__restrict__ cend, int crit,
+                             int *__restrict__ count, int *__restrict__ out, int cap)
+{
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ncell)
+    return;
+  int cs = c0 + s;
+  if (cend[cs] - cstart[cs] >= crit)
+  {
+    int q = zone_node[cs >> 3];
+    int at = atomicAdd(count, 1);
+    if (at < cap)
+    {
+      out[2 * at] = q - first;
+      out[2 * at + 1] = cs - nodes[q].coff;
+    }
+  }
+}
+
+__global__ void k_flag_head(const int *__restrict__ start, int ncell, int crit, int *__restrict__ count,
+                            int *__restrict__ out, int cap)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell)
+    return;
+  if (start[c + 1] - start[c] >= crit)
+  {
+    int at = atomicAdd(count, 1);
+    if (at < cap)
+      out[at] = c;
+  }
+}
+
+// Refined cells point at their child node: head_child[c] / box[parent] = child index.
+__global__ void k_mark_children(const DNode *__restrict__ nodes, const int *__restrict__ zone_node,
+                                const int *__restrict__ zone_xyz, int z0, int nz, int n, int *__restrict__ head_child,
+                                int *__restrict__ box)
+{
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nz)
+    return;
+  int z = z0 + s;
+  int q = zone_node[z];
+  int pq = nodes[q].parent;
+  int cx = zone_xyz[3 * z], cy = zone_xyz[3 * z + 1], cz = zone_xyz[3 * z + 2];
+  if (pq < 0)
+    head_child[cx + n * (cy + n * cz)] = q;
+  else
+  {
+    const DNode &p = nodes[pq];
+    int b = (cx - p.ts[0]) + (cy - p.ts[1]) * p.rd[0] + (cz - p.ts[2]) * p.rd[0] * p.rd[1];
+    box[p.boff + b] = q;
+  }
+}
+
+// 8-point parent average at ((c >> 1), (c + 1) >> 1) per axis
+// (potential_branches.c:204-331, grid_acceleration.c:316-383).
+template <typename T>
+__device__ inline double parent_avg(const DNode *nodes, int pq, const T *pool, const T *head, int n, int cx, int cy,
+                                    int cz)
+{
+  int a1x = cx >> 1, a1y = cy >> 1, a1z = cz >> 1;
+  int a2x = (cx + 1) >> 1, a2y = (cy + 1) >> 1, a2z = (cz + 1) >> 1;
+  const T *f;
+  long long X, XY;
+  int tx, ty, tz;
+  if (pq < 0)
+  {
+    f = head;
+    X = n + 1;
+    XY = (long long)(n + 1) * (n + 1);
+    tx = ty = tz = 0;
+  }
+  else
+  {
+    const DNode &p = nodes[pq];
+    f = pool + p.goff;
+    X = p.rd[0] + 1;
+    XY = (long long)(p.rd[0] + 1) * (p.rd[1] + 1);
+    tx = p.ts[0];
+    ty = p.ts[1];
+    tz = p.ts[2];
+  }
+  long long b1x = a1x - tx, b1y = a1y - ty, b1z = a1z - tz;
+  long long b2x = a2x - tx, b2y = a2y - ty, b2z = a2z - tz;
+  double s = (double)f[b1x + b1y * X + b1z * XY] + (double)f[b2x + b1y * X + b1z * XY] +
+             (double)f[b1x + b2y * X + b1z * XY] + (double)f[b2x + b2y * X + b1z * XY] +
+             (double)f[b1x + b1y * X + b2z * XY] + (double)f[b2x + b1y * X + b2z * XY] +
+             (double)f[b1x + b2y * X + b2z * XY] + (double)f[b2x + b2y * X + b2z * XY];
+  return 0.125 * s;
+}
+
+// Simulation-boundary coordinates as the reference computes them: an index of
+// 0 or 2^lv maps to 0.0 (potential_branches.c:114-145); for accelerations the
+// y coordinate lacks the factor H (grid_acceleration.c:240).
+__device__ inline void simb_coords(int lv, int x, int y, int z, bool accel_quirk, double &ax, double &ay, double &az)
+{
+  double H = 1.0 / (1 << lv);
+  int top = 1 << lv;
+  ax = (x == 0 || x == top) ? 0.0 : x * H;
+  ay = (y == 0 || y == top) ? 0.0 : (accel_quirk ? (double)y : y * H);
+  az = (z == 0 || z == top) ? 0.0 : z * H;
+}
+
+// A8 initial potential of the level's nodes (potential_branches.c:153-332) and pot_old := pot.
+__global__ void k_branch_init_pot(const DNode *__restrict__ nodes, int first, const int *__restrict__ sim_g,
+                                  const int *__restrict__ sim_xyz, const int *__restrict__ bdry_g,
+                                  const int *__restrict__ bdry_xyz, const int *__restrict__ intr_g,
+                                  const int *__restrict__ intr_xyz, double *__restrict__ phi,
+                                  double *__restrict__ phi_old, const double *__restrict__ head_phi, int n, double GM,
+                                  int zero_bc)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  double *ph = phi + d.goff;
+  double *po = phi_old + d.goff;
+  const int tot = d.nsim + d.nbdry + d.nintr;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x)
+  {
+    if (t < d.nsim)
+    {
+      int s = d.poff + t;
+      double ax, ay, az;
+      simb_coords(d.lv, sim_xyz[3 * s], sim_xyz[3 * s + 1], sim_xyz[3 * s + 2], false, ax, ay, az);
+      double dist = sqrt((ax - 0.5) * (ax - 0.5) + (ay - 0.5) * (ay - 0.5) + (az - 0.5) * (az - 0.5));
+      double v = zero_bc ? 0.0 : -GM / dist;
+      ph[sim_g[s]] = v;
+      po[sim_g[s]] = v;
+    }
+    else if (t < d.nsim + d.nbdry)
+    {
+      int s = d.poff + t - d.nsim;
+      double v = parent_avg<double>(nodes, d.parent, phi, head_phi, n, bdry_xyz[3 * s], bdry_xyz[3 * s + 1],
+                                    bdry_xyz[3 * s + 2]);
+      ph[bdry_g[s]] = v;
+      po[bdry_g[s]] = v;
+    }
+    else
+    {
+      int s = d.poff + t - d.nsim - d.nbdry;
+      double v = parent_avg<double>(nodes, d.parent, phi, head_phi, n, intr_xyz[3 * s], intr_xyz[3 * s + 1],
+                                    intr_xyz[3 * s + 2]);
+      ph[intr_g[s]] = v;
+      po[intr_g[s]] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- batched CG (conjugate_gradient.c:322-463)
+// Partial-sum layout: part[(node - first) * gridDim.x + blockIdx.x].
+__device__ inline double fold_partials(const double *part, int row, int nb)
+{
+  __shared__ double s_tot;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+    v += part[row * nb + i];
+  v = block_reduce<0>(v);
+  if (threadIdx.x == 0)
+    s_tot = v;
+  __syncthreads();
+  double r = s_tot;
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double fold_max(const double *part, int row, int nb)
+{
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+    v = fmax(v, part[row * nb + i]);
+  return block_reduce<1>(v);
+}
+
+// r0 = rho - Lap(phi), p = r, partial r.r (conjugate_gradient.c:359-375)
+template <typename G>
+__global__ void k_cg_init(DNode *__restrict__ nodes, int first, const int *__restrict__ intr_g,
+                          const double *__restrict__ phi, const G *__restrict__ rho, double *__restrict__ r,
+                          double *__restrict__ p, double *__restrict__ part)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
+  const int X = dnode_gX(d), XY = dnode_gXY(d);
+  const double *ph = phi + d.goff;
+  double acc = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  {
+    int g = intr_g[d.poff + t];
+    double v = (double)rho[d.goff + g] +
+               inv_h2 * (6.0 * ph[g] - ph[g + 1] - ph[g - 1] - ph[g + X] - ph[g - X] - ph[g + XY] - ph[g - XY]);
+    r[d.goff + g] = v;
+    p[d.goff + g] = v;
+    acc += v * v;
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+__global__ void k_cg_init_fold(DNode *__restrict__ nodes, int first, const double *__restrict__ part, int nb)
+{
+  DNode &d = nodes[first + blockIdx.x];
+  double v = fold_partials(part, blockIdx.x, nb);
+  if (threadIdx.x == 0)
+  {
+    d.rs[0] = v;
+    d.done = 0;
+  }
+}
+
+// Ap = Lap(p) on interior points (p = 0 elsewhere), partial p.Ap (:393-400)
+__global__ void k_cg_ap(const DNode *__restrict__ nodes, int first, const int *__restrict__ intr_g,
+                        const double *__restrict__ p, double *__restrict__ Ap, double *__restrict__ part)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  if (d.done)
+    return;
+  const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
+  const int X = dnode_gX(d), XY = dnode_gXY(d);
+  const double *pp = p + d.goff;
+  double acc = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  {
+    int g = intr_g[d.poff + t];
+    double v = -inv_h2 * (6.0 * pp[g] - pp[g + 1] - pp[g - 1] - pp[g + X] - pp[g - X] - pp[g + XY] - pp[g - XY]);
+    Ap[d.goff + g] = v;
+    acc += pp[g] * v;
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// phi += alpha p, r -= alpha Ap, partial r.r; on check iterations also the
+// pointwise change flag against phi_old (poisson_error.c:36-81) (:401-417)
+__global__ void k_cg_update(const DNode *__restrict__ nodes, int first, int it, int check,
+                            const int *__restrict__ intr_g, double *__restrict__ phi,
+                            const double *__restrict__ phi_old, double *__restrict__ r, const double *__restrict__ p,
+                            const double *__restrict__ Ap, const double *__restrict__ part_pap,
+                            double *__restrict__ part_rs, double *__restrict__ part_viol, double tol2)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  if (d.done)
+    return;
+  const int nb = gridDim.x;
+  double pap = fold_partials(part_pap, blockIdx.y, nb);
+  double alpha = d.rs[it & 1] / pap;
+  double acc = 0.0, viol = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  {
+    long long g = d.goff + intr_g[d.poff + t];
+    double ph = phi[g] + alpha * p[g];
+    phi[g] = ph;
+    double rr = r[g] - alpha * Ap[g];
+    r[g] = rr;
+    acc += rr * rr;
+    if (check)
+    {
+      double a = ph / phi_old[g];
+      if ((1.0 - tol2 > a) || (1.0 + tol2 < a) || a != a)
+        viol = 1.0;
+    }
+  }
+  acc = block_reduce<0>(acc);
+  if (check)
+    viol = block_reduce<1>(viol);
+  if (threadIdx.x == 0)
+  {
+    part_rs[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    if (check)
+      part_viol[blockIdx.y * gridDim.x + blockIdx.x] = viol;
+  }
+}
+
+// Convergence (conjugate_gradient.c:426-442 with conj_grad_error method 0,
+// poisson_error.c:236-288), pot_old refresh on failure, p = r + beta p (:447-455).
+__global__ void k_cg_direction(DNode *__restrict__ nodes, int first, int it, int check,
+                               const int *__restrict__ intr_g, const double *__restrict__ phi,
+                               double *__restrict__ phi_old, const double *__restrict__ r, double *__restrict__ p,
+                               const double *__restrict__ part_rs, const double *__restrict__ part_viol,
+                               double tol, double G, int *__restrict__ done_count)
+{
+  DNode &d = nodes[first + blockIdx.y];
+  if (d.done)
+    return;
+  const int nb = gridDim.x;
+  double rsnew = fold_partials(part_rs, blockIdx.y, nb);
+  double rsold = d.rs[it & 1];
+  if (check)
+  {
+    double viol = fold_max(part_viol, blockIdx.y, nb);
+    double H = 1.0 / (1 << d.lv);
+    double rhomean = 4.0 * G * kPiD * d.mass / ((double)d.ncell * H * H * H);
+    double aux_tol = (double)d.nintr * (tol * rhomean) * (tol * rhomean);
+    bool ok = viol == 0.0 && rsnew < aux_tol;
+    __syncthreads();
+    if (ok)
+    {
+      // every block of the node reaches the same decision; block 0 records it
+      if (blockIdx.x == 0 && threadIdx.x == 0)
+      {
+        d.done = it + 1;
+        atomicAdd(done_count, 1);
+      }
+      return;
+    }
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    {
+      long long g = d.goff + intr_g[d.poff + t];
+      phi_old[g] = phi[g];
+    }
+  }
+  double beta = rsnew / rsold;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  {
+    long long g = d.goff + intr_g[d.poff + t];
+    p[g] = r[g] + beta * p[g];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    d.rs[(it + 1) & 1] = rsnew;
+}
+
+// A11 branch accelerations (grid_acceleration.c:199-394).
+template <typename G>
+__global__ void k_branch_grid_acc(const DNode *__restrict__ nodes, int first, const int *__restrict__ sim_g,
+                                  const int *__restrict__ sim_xyz, const int *__restrict__ bdry_g,
+                                  const int *__restrict__ bdry_xyz, const int *__restrict__ intr_g,
+                                  const double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay,
+                                  G *__restrict__ az, const G *__restrict__ hax, const G *__restrict__ hay,
+                                  const G *__restrict__ haz, int n, double GM, int zero_bc)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const int tot = d.nsim + d.nbdry + d.nintr;
+  const double H = 1.0 / (1 << d.lv);
+  const double o2H = 1.0 / (2.0 * H);
+  const int X = dnode_gX(d), XY = dnode_gXY(d);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x)
+  {
+    if (t < d.nsim)
+    {
+      int s = d.poff + t;
+      double qx, qy, qz;
+      simb_coords(d.lv, sim_xyz[3 * s], sim_xyz[3 * s + 1], sim_xyz[3 * s + 2], true, qx, qy, qz);
+      double dist = sqrt((qx - 0.5) * (qx - 0.5) + (qy - 0.5) * (qy - 0.5) + (qz - 0.5) * (qz - 0.5));
+      double c2 = zero_bc ? 0.0 : -GM / (dist * dist * dist);
+      long long g = d.goff + sim_g[s];
+      ax[g] = (G)(c2 * (qx - 0.5));
+      ay[g] = (G)(c2 * (qy - 0.5));
+      az[g] = (G)(c2 * (qz - 0.5));
+    }
+    else if (t < d.nsim + d.nbdry)
+    {
+      int s = d.poff + t - d.nsim;
+      int cx = bdry_xyz[3 * s], cy = bdry_xyz[3 * s + 1], cz = bdry_xyz[3 * s + 2];
+      long long g = d.goff + bdry_g[s];
+      ax[g] = (G)parent_avg<G>(nodes, d.parent, ax, hax, n, cx, cy, cz);
+      ay[g] = (G)parent_avg<G>(nodes, d.parent, ay, hay, n, cx, cy, cz);
+      az[g] = (G)parent_avg<G>(nodes, d.parent, az, haz, n, cx, cy, cz);
+    }
+    else
+    {
+      int s = d.poff + t - d.nsim - d.nbdry;
+      int gl = intr_g[s];
+      const double *ph = phi + d.goff;
+      long long g = d.goff + gl;
+      ax[g] = (G)(o2H * (ph[gl - 1] - ph[gl + 1]));
+      ay[g] = (G)(o2H * (ph[gl - X] - ph[gl + X]));
+      az[g] = (G)(o2H * (ph[gl - XY] - ph[gl + XY]));
+    }
+  }
+}
+
+// A2 branch deposit (grid_density.c:113-267) as a gather over the 8 box cells
+// around each listed grid point, in slot order.  coeff = 4 pi G / H^3.
+template <typename P, typename G>
+__global__ void k_branch_deposit(const DNode *__restrict__ nodes, int first, const int *__restrict__ sim_g,
+                                 const int *__restrict__ sim_xyz, const int *__restrict__ bdry_g,
+                                 const int *__restrict__ bdry_xyz, const int *__restrict__ intr_g,
+                                 const int *__restrict__ intr_xyz, const int *__restrict__ bstart,
+                                 const int *__restrict__ bend, const P *__restrict__ x, const P *__restrict__ y,
+                                 const P *__restrict__ z, const P *__restrict__ m, G *__restrict__ rho, double G4pi)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const int tot = d.nsim + d.nbdry + d.nintr;
+  const double s = (double)(1 << d.lv);
+  const double H = 1.0 / s;
+  const double coeff = G4pi / (H * H * H);
+  const int rX = d.rd[0], rXY = d.rd[0] * d.rd[1];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x)
+  {
+    int sl, gl;
+    const int *xyz;
+    if (t < d.nsim)
+    {
+      sl = d.poff + t;
+      gl = sim_g[sl];
+      xyz = sim_xyz;
+    }
+    else if (t < d.nsim + d.nbdry)
+    {
+      sl = d.poff + t - d.nsim;
+      gl = bdry_g[sl];
+      xyz = bdry_xyz;
+    }
+    else
+    {
+      sl = d.poff + t - d.nsim - d.nbdry;
+      gl = intr_g[sl];
+      xyz = intr_xyz;
+    }
+    int gx = xyz[3 * sl], gy = xyz[3 * sl + 1], gz = xyz[3 * sl + 2];
+    double acc = 0.0;
+    for (int ck = gz - 1; ck <= gz; ck++)
+      for (int cj = gy - 1; cj <= gy; cj++)
+        for (int ci = gx - 1; ci <= gx; ci++)
+        {
+          long long b = d.boff + (ci - d.ts[0]) + (long long)(cj - d.ts[1]) * rX + (long long)(ck - d.ts[2]) * rXY;
+          int a = bstart[b], e = bend[b];
+          for (int p = a; p < e; p++)
+          {
+            double px = (double)x[p] * s, py = (double)y[p] * s, pz = (double)z[p] * s;
+            double wx1 = px - ci, wy1 = py - cj, wz1 = pz - ck;
+            double wx = ci == gx ? 1.0 - wx1 : wx1;
+            double wy = cj == gy ? 1.0 - wy1 : wy1;
+            double wz = ck == gz ? 1.0 - wz1 : wz1;
+            acc += (double)m[p] * (wx * wy * wz);
+          }
+        }
+    rho[d.goff + gl] = (G)(coeff * acc);
+  }
+}
+
+// A12 with refinement (particle_acceleration.c:134-355): each particle takes
+// the trilinear acceleration of the finest node holding it.
+template <typename P, typename G>
+__global__ void k_gather_amr(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                             const G *__restrict__ hax, const G *__restrict__ hay, const G *__restrict__ haz,
+                             const int *__restrict__ head_child, const DNode *__restrict__ nodes,
+                             const int *__restrict__ box, const G *__restrict__ bax, const G *__restrict__ bay,
+                             const G *__restrict__ baz, P *__restrict__ ax, P *__restrict__ ay, P *__restrict__ az,
+                             int N, int lmin)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N)
+    return;
+  const int n = 1 << lmin;
+  const double px0 = (double)x[p], py0 = (double)y[p], pz0 = (double)z[p];
+  int hx = min((int)(px0 * n), n - 1), hy = min((int)(py0 * n), n - 1), hz = min((int)(pz0 * n), n - 1);
+  int q = head_child[hx + n * (hy + n * hz)];
+  int lv = lmin;
+  while (q >= 0)
+  {
+    const DNode &d = nodes[q];
+    double s = (double)(1 << d.lv);
+    int cx = (int)(px0 * s), cy = (int)(py0 * s), cz = (int)(pz0 * s);
+    int b = (cx - d.ts[0]) + (cy - d.ts[1]) * d.rd[0] + (cz - d.ts[2]) * d.rd[0] * d.rd[1];
+    int st = box[d.boff + b];
+    lv = d.lv;
+    if (st < 0)
+      break;
+    q = st;
+  }
+  double s = (double)(1 << lv);
+  double pxs = px0 * s, pys = py0 * s, pzs = pz0 * s;
+  int fx = (int)pxs, fy = (int)pys, fz = (int)pzs;
+  const G *gx, *gy, *gz;
+  long long X, XY, g;
+  if (q < 0)
+  {
+    fx = min(fx, n - 1);
+    fy = min(fy, n - 1);
+    fz = min(fz, n - 1);
+    gx = hax;
+    gy = hay;
+    gz = haz;
+    X = n + 1;
+    XY = (long long)(n + 1) * (n + 1);
+    g = fx + fy * X + fz * XY;
+  }
+  else
+  {
+    const DNode &d = nodes[q];
+    gx = bax + d.goff;
+    gy = bay + d.goff;
+    gz = baz + d.goff;
+    X = d.rd[0] + 1;
+    XY = (long long)(d.rd[0] + 1) * (d.rd[1] + 1);
+    g = (fx - d.ts[0]) + (fy - d.ts[1]) * X + (fz - d.ts[2]) * XY;
+  }
+  double wx1 = pxs - fx, wy1 = pys - fy, wz1 = pzs - fz;
+  double wx2 = 1.0 - wx1, wy2 = 1.0 - wy1, wz2 = 1.0 - wz1;
+  double w[8] = {wx2 * wy2 * wz2, wx1 * wy2 * wz2, wx2 * wy1 * wz2, wx1 * wy1 * wz2,
+                 wx2 * wy2 * wz1, wx1 * wy2 * wz1, wx2 * wy1 * wz1, wx1 * wy1 * wz1};
+  double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++)
+      {
+        long long gn = g + ii + jj * X + kk * XY;
+        double wq = w[ii + 2 * jj + 4 * kk];
+        sx += (double)gx[gn] * wq;
+        sy += (double)gy[gn] * wq;
+        sz += (double)gz[gn] * wq;
+      }
+  ax[p] = (P)sx;
+  ay[p] = (P)sy;
+  az[p] = (P)sz;
+}
+
+} // namespace qz

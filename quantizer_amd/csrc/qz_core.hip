@@ -1,0 +1,847 @@
+// libqzhip.so: MI355X-native per-step particle-mesh gravity cycle.
+//
+// Host-side driver of the device kernels in qz_kernels.h / qz_branch.hip and
+// the native C ABI declared in include/qz.h.  One HIP stream per context; every
+// entry point returns after its device work has finished (the host tree code
+// of the reference reads results between stages, SURVEY-free restatement of
+// main.c:369-535).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qz.h"
+#include "qz_engine.h"
+#include "qz_internal.h"
+#include "qz_kernels.h"
+
+namespace qz
+{
+
+thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...)
+{
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+bool hip_ok(hipError_t e, const char *what)
+{
+  if (e != hipSuccess)
+  {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return false;
+  }
+  return true;
+}
+
+static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ timing
+void Timer::begin(hipStream_t s)
+{
+  if (!on)
+    return;
+  hipEvent_t a;
+  (void)hipEventCreate(&a);
+  (void)hipEventRecord(a, s);
+  pending_start = a;
+}
+
+void Timer::end(hipStream_t s, const char *name)
+{
+  if (!on)
+    return;
+  hipEvent_t b;
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(b, s);
+  recs.push_back({name, pending_start, b});
+}
+
+void Timer::collect()
+{
+  for (auto &r : recs)
+  {
+    (void)hipEventSynchronize(r.b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    auto it = std::find_if(acc.begin(), acc.end(), [&](const Acc &x) { return x.name == r.name; });
+    if (it == acc.end())
+      acc.push_back({r.name, (double)ms, 1});
+    else
+    {
+      it->ms += ms;
+      it->count++;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  recs.clear();
+}
+
+#define QZ_LAUNCH(name, kern, grid, ...)                                                                              \
+  do                                                                                                                  \
+  {                                                                                                                   \
+    timer.begin(stream);                                                                                              \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__);                                       \
+    timer.end(stream, name);                                                                                          \
+  } while (0)
+
+// ------------------------------------------------------------------ engine
+template <typename T> static T *dmalloc(size_t n, size_t &total)
+{
+  T *p = nullptr;
+  if (n == 0)
+    n = 1;
+  if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess)
+    return nullptr;
+  total += n * sizeof(T);
+  return p;
+}
+
+template <typename P, typename G> struct Engine final : EngineBase
+{
+  // particles
+  P *pa[10] = {};
+  P *pb[10] = {};
+  int *id_a = nullptr, *id_b = nullptr;
+  unsigned long long *key_in = nullptr, *key_out = nullptr;
+  int *slot_in = nullptr, *slot_out = nullptr;
+  int *cell_start = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  // head grid
+  G *rho = nullptr;
+  double *phi = nullptr;
+  G *gax = nullptr, *gay = nullptr, *gaz = nullptr;
+  std::vector<G *> lev_e, lev_r;
+  std::vector<int> lev_n;
+  double *part = nullptr; // reduction partials
+  double *red = nullptr;  // device scalars
+  double *hred = nullptr; // pinned host mirror
+  size_t part_cap = 0;
+  bool keys_valid = false;
+
+  ~Engine() override { release(); }
+
+  void release()
+  {
+    for (int a = 0; a < 10; a++)
+    {
+      (void)hipFree(pa[a]);
+      (void)hipFree(pb[a]);
+    }
+    (void)hipFree(id_a);
+    (void)hipFree(id_b);
+    (void)hipFree(key_in);
+    (void)hipFree(key_out);
+    (void)hipFree(slot_in);
+    (void)hipFree(slot_out);
+    (void)hipFree(cell_start);
+    (void)hipFree(sort_tmp);
+    (void)hipFree(rho);
+    (void)hipFree(phi);
+    (void)hipFree(gax);
+    (void)hipFree(gay);
+    (void)hipFree(gaz);
+    for (size_t l = 1; l < lev_e.size(); l++)
+    {
+      (void)hipFree(lev_e[l]);
+      (void)hipFree(lev_r[l]);
+    }
+    if (!lev_e.empty())
+    {
+      (void)hipFree(lev_e[0]);
+      (void)hipFree(lev_r[0]);
+    }
+    (void)hipFree(part);
+    (void)hipFree(red);
+    if (hred)
+      (void)hipHostFree(hred);
+    release_tree();
+  }
+
+  bool init() override
+  {
+    N = cfg.n_particles;
+    n = 1 << cfg.lmin;
+    D = std::min(cfg.lmax - cfg.lmin, 7);
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    for (int a = 0; a < 10; a++)
+    {
+      pa[a] = dmalloc<P>(N, bytes);
+      pb[a] = dmalloc<P>(N, bytes);
+      if (!pa[a] || !pb[a])
+        return hip_ok(hipErrorOutOfMemory, "particle arrays");
+    }
+    id_a = dmalloc<int>(N, bytes);
+    id_b = dmalloc<int>(N, bytes);
+    key_in = dmalloc<unsigned long long>(N, bytes);
+    key_out = dmalloc<unsigned long long>(N, bytes);
+    slot_in = dmalloc<int>(N, bytes);
+    slot_out = dmalloc<int>(N, bytes);
+    cell_start = dmalloc<int>((size_t)n * n * n + 1, bytes);
+    end_bit = 3 * (cfg.lmin + D);
+    sort_tmp_bytes = qz_sort64_temp_bytes(N, end_bit);
+    sort_tmp = dmalloc<char>(sort_tmp_bytes, bytes);
+    rho = dmalloc<G>(npts, bytes);
+    phi = dmalloc<double>(npts, bytes);
+    gax = dmalloc<G>(npts, bytes);
+    gay = dmalloc<G>(npts, bytes);
+    gaz = dmalloc<G>(npts, bytes);
+    if (!id_a || !id_b || !key_in || !key_out || !slot_in || !slot_out || !cell_start || !sort_tmp || !rho || !phi ||
+        !gax || !gay || !gaz)
+      return hip_ok(hipErrorOutOfMemory, "head arrays");
+    // multigrid hierarchy: level 0 = head (n cells), halve down to 4 cells
+    for (int m = n; m >= 4; m /= 2)
+    {
+      size_t p1 = (size_t)m + 1;
+      lev_n.push_back(m);
+      lev_e.push_back(dmalloc<G>(p1 * p1 * p1, bytes));
+      lev_r.push_back(dmalloc<G>(p1 * p1 * p1, bytes));
+      if (!lev_e.back() || !lev_r.back())
+        return hip_ok(hipErrorOutOfMemory, "multigrid levels");
+    }
+    if (lev_n.back() > 8)
+    {
+      set_error("head grid too small for the multigrid hierarchy");
+      return false;
+    }
+    part_cap = std::max<size_t>(nblk(npts), 4 * (size_t)nblk(N)) + 16;
+    part = dmalloc<double>(part_cap, bytes);
+    red = dmalloc<double>(16, bytes);
+    if (!hip_ok(hipHostMalloc((void **)&hred, 16 * sizeof(double)), "pinned scalars"))
+      return false;
+    QZ_LAUNCH("fill", k_fill<G>, nblk(npts), rho, (G)0, npts);
+    return hip_ok(hipStreamSynchronize(stream), "init");
+  }
+
+  bool set_particles(const double *const *src) override
+  {
+    std::vector<P> tmp(N);
+    for (int a = 0; a < 7; a++)
+    {
+      int dst = a < 6 ? a : 9;
+      for (int i = 0; i < N; i++)
+        tmp[i] = (P)src[a][i];
+      if (!hip_ok(hipMemcpy(pa[dst], tmp.data(), sizeof(P) * N, hipMemcpyHostToDevice), "upload particles"))
+        return false;
+    }
+    std::vector<int> ids(N);
+    double mt = 0.0;
+    for (int i = 0; i < N; i++)
+    {
+      ids[i] = i;
+      mt += src[6][i];
+    }
+    total_mass = mt;
+    if (!hip_ok(hipMemcpy(id_a, ids.data(), sizeof(int) * N, hipMemcpyHostToDevice), "upload ids"))
+      return false;
+    for (int a = 6; a < 9; a++)
+      QZ_LAUNCH("fill", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, n, cfg.G * total_mass,
+              cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
+    keys_valid = false;
+    return hip_ok(hipStreamSynchronize(stream), "set_particles");
+  }
+
+  bool get_particles(double *const *dst) override
+  {
+    std::vector<P> tmp(N);
+    std::vector<int> ids(N);
+    if (!hip_ok(hipMemcpy(ids.data(), id_a, sizeof(int) * N, hipMemcpyDeviceToHost), "download ids"))
+      return false;
+    const int order[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+    for (int q = 0; q < 9; q++)
+    {
+      if (!dst[q])
+        continue;
+      if (!hip_ok(hipMemcpy(tmp.data(), pa[order[q]], sizeof(P) * N, hipMemcpyDeviceToHost), "download particles"))
+        return false;
+      for (int i = 0; i < N; i++)
+        dst[q][ids[i]] = (double)tmp[i];
+    }
+    return true;
+  }
+
+  // ---------------------------------------------------------------- stages
+  bool update_A(double dt) override
+  {
+    QZ_LAUNCH("update_A", k_update_A<P>, nblk(N), pa[0], pa[1], pa[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
+              (P)dt);
+    keys_valid = false;
+    return hip_ok(hipStreamSynchronize(stream), "particle_updating_A");
+  }
+
+  bool update_B(double dt) override
+  {
+    QZ_LAUNCH("update_B", k_update_B<P>, nblk(N), pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, (P)dt);
+    return hip_ok(hipStreamSynchronize(stream), "particle_updating_B");
+  }
+
+  bool reset() override
+  {
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    for (int a = 6; a < 9; a++)
+      QZ_LAUNCH("reset", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
+    QZ_LAUNCH("reset", k_fill<G>, nblk(npts), rho, (G)0, npts);
+    return hip_ok(hipStreamSynchronize(stream), "reset");
+  }
+
+  // Sort particles by octree key and rebuild the head cell ranges.
+  bool sort_particles()
+  {
+    if (keys_valid)
+      return true;
+    QZ_LAUNCH("keys", k_keys<P>, nblk(N), pa[0], pa[1], pa[2], key_in, slot_in, N, cfg.lmin, D);
+    timer.begin(stream);
+    if (!hip_ok(qz_sort64_pairs(sort_tmp, sort_tmp_bytes, key_in, key_out, slot_in, slot_out, N, end_bit, stream),
+                "radix sort"))
+      return false;
+    timer.end(stream, "sort");
+    PtclArrays<P> src, dst;
+    for (int a = 0; a < 10; a++)
+    {
+      src.a[a] = pa[a];
+      dst.a[a] = pb[a];
+    }
+    src.id = id_a;
+    dst.id = id_b;
+    QZ_LAUNCH("permute", k_permute<P>, nblk(N), src, dst, slot_out, N);
+    for (int a = 0; a < 10; a++)
+      std::swap(pa[a], pb[a]);
+    std::swap(id_a, id_b);
+    QZ_LAUNCH("cell_start", k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n, cell_start);
+    keys_valid = true;
+    return true;
+  }
+
+  bool grid_density() override
+  {
+    if (!sort_particles())
+      return false;
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    double H = 1.0 / n;
+    double coeff = 4.0 * cfg.G * kPi / (H * H * H);
+    QZ_LAUNCH("deposit_head", (k_deposit_head<P, G>), nblk(npts), pa[0], pa[1], pa[2], pa[9], cell_start, rho, n,
+              coeff);
+    if (has_tree() && !branch_density())
+      return false;
+    return hip_ok(hipStreamSynchronize(stream), "grid_density");
+  }
+
+  void vcycle(size_t l)
+  {
+    int m = lev_n[l];
+    size_t p1 = (size_t)m + 1, np = p1 * p1 * p1;
+    G h2 = (G)(1.0 / ((double)m * m));
+    if (l + 1 == lev_n.size())
+    {
+      timer.begin(stream);
+      hipLaunchKernelGGL(k_coarse_cg<G>, dim3(1), dim3(kBlock), 0, stream, lev_e[l], lev_r[l], m, 64);
+      timer.end(stream, "coarse_cg");
+      return;
+    }
+    size_t nred = (size_t)(m / 2 + 1) * (m - 1) * (m - 1);
+    for (int s = 0; s < cfg.pre_sweeps; s++)
+    {
+      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
+      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
+    }
+    int mc = lev_n[l + 1];
+    size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
+    QZ_LAUNCH("restrict", k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l], lev_r[l + 1], m,
+              (G)((double)m * m));
+    QZ_LAUNCH("fill", k_fill<G>, nblk(nc), lev_e[l + 1], (G)0, nc);
+    vcycle(l + 1);
+    QZ_LAUNCH("prolong", k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1], m);
+    for (int s = 0; s < cfg.post_sweeps; s++)
+    {
+      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
+      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
+    }
+  }
+
+  // Head solve: FP64 outer residual/correction around inner V-cycles in G.
+  // Convergence follows poisson_error.c:290-347 (method 0): the RMS residual
+  // over interior points divided by 4 pi G rho_mean, and the pointwise change
+  // of phi over the last outer iteration (= the last `inner` V-cycles, the
+  // reference compares with phi from iter_between_check = 5 cycles earlier).
+  bool potential_head() override
+  {
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    double H = 1.0 / n;
+    double inv_h2 = 1.0 / (H * H);
+    double rhomean = 4.0 * cfg.G * kPi * total_mass / ((double)n * n * n * H * H * H);
+    double ninterior = (double)(n - 1) * (n - 1) * (n - 1);
+    int inner = cfg.inner_cycles;
+    int max_outer = std::max(1, cfg.max_iterations / std::max(1, inner));
+    unsigned nb = nblk(npts);
+    stats.head_outer = 0;
+    stats.head_vcycles = 0;
+    double change = 1e300;
+    for (int outer = 0; outer <= max_outer; outer++)
+    {
+      QZ_LAUNCH("outer_residual", k_outer_residual<G>, nb, phi, rho, lev_r[0], n, inv_h2, part);
+      hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+      if (!hip_ok(hipMemcpyAsync(hred, red, 2 * sizeof(double), hipMemcpyDeviceToHost, stream), "residual readback") ||
+          !hip_ok(hipStreamSynchronize(stream), "residual"))
+        return false;
+      double res = std::sqrt(hred[0] / ninterior) / rhomean;
+      if (outer > 0)
+        change = hred[1];
+      stats.head_residual = res;
+      stats.head_change = change;
+      bool change_ok = outer > 0 && change <= 1.0; // flag values > 1 mark points outside [1-tol, 1+tol]
+      if (change_ok && res < cfg.tol)
+        return true;
+      if (outer == max_outer)
+        break;
+      QZ_LAUNCH("fill", k_fill<G>, nb, lev_e[0], (G)0, npts);
+      for (int c = 0; c < inner; c++)
+        vcycle(0);
+      stats.head_vcycles += inner;
+      QZ_LAUNCH("outer_correct", k_outer_correct<G>, nb, phi, lev_e[0], n, cfg.tol_change, part);
+      hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
+      stats.head_outer++;
+    }
+    set_error("The precision was not reached in the parent node after %d V-cycles (residual %.3e, change %.3e)",
+              stats.head_vcycles, stats.head_residual, stats.head_change);
+    return false;
+  }
+
+  bool grid_acceleration() override
+  {
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    double H = 1.0 / n;
+    QZ_LAUNCH("grid_acc_head", k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz, n, 1.0 / (2.0 * H));
+    if (has_tree() && !branch_grid_acceleration())
+      return false;
+    return hip_ok(hipStreamSynchronize(stream), "grid_acceleration");
+  }
+
+  bool particle_acceleration() override
+  {
+    if (!sort_particles())
+      return false;
+    if (has_tree())
+    {
+      if (!branch_particle_acceleration())
+        return false;
+    }
+    else
+    {
+      QZ_LAUNCH("gather_head", (k_gather_head<P, G>), nblk(N), pa[0], pa[1], pa[2], gax, gay, gaz, pa[6], pa[7],
+                pa[8], N, n);
+    }
+    return hip_ok(hipStreamSynchronize(stream), "particle_acceleration");
+  }
+
+  bool head_fields(double *out5) override
+  {
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    std::vector<G> tmp(npts);
+    G *src[5] = {rho, nullptr, gax, gay, gaz};
+    for (int a = 0; a < 5; a++)
+    {
+      if (a == 1)
+      {
+        if (!hip_ok(hipMemcpy(out5 + npts, phi, sizeof(double) * npts, hipMemcpyDeviceToHost), "download phi"))
+          return false;
+        continue;
+      }
+      if (!hip_ok(hipMemcpy(tmp.data(), src[a], sizeof(G) * npts, hipMemcpyDeviceToHost), "download field"))
+        return false;
+      for (size_t i = 0; i < npts; i++)
+        out5[a * npts + i] = (double)tmp[i];
+    }
+    return true;
+  }
+
+  bool set_head_potential(const double *pot) override
+  {
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    return hip_ok(hipMemcpy(phi, pot, sizeof(double) * npts, hipMemcpyHostToDevice), "upload phi");
+  }
+
+  bool observables(int exact_pe, double *out6) override
+  {
+    unsigned nb = nblk(N);
+    QZ_LAUNCH("kin_mom", k_kin_mom<P>, nb, pa[3], pa[4], pa[5], pa[9], N, part);
+    for (int q = 0; q < 4; q++)
+      hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part + q * nb, (int)nb, red + q);
+    if (!hip_ok(hipMemcpyAsync(hred, red, 4 * sizeof(double), hipMemcpyDeviceToHost, stream), "readback") ||
+        !hip_ok(hipStreamSynchronize(stream), "observables"))
+      return false;
+    double ek = 0.5 * hred[0], px = hred[1], py = hred[2], pz = hred[3];
+    if (exact_pe)
+      QZ_LAUNCH("pairwise_pe", k_pairwise_pe<P>, nb, pa[0], pa[1], pa[2], pa[9], N, part);
+    else
+      QZ_LAUNCH("grid_pe", k_grid_pe_head<P>, nb, pa[0], pa[1], pa[2], pa[9], phi, N, n, part);
+    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+    if (!hip_ok(hipMemcpyAsync(hred, red, sizeof(double), hipMemcpyDeviceToHost, stream), "readback") ||
+        !hip_ok(hipStreamSynchronize(stream), "observables"))
+      return false;
+    double ep = exact_pe ? -cfg.G * hred[0] : 0.5 * cfg.G * hred[0];
+    out6[0] = ek;
+    out6[1] = ep;
+    out6[2] = ek + ep;
+    out6[3] = px;
+    out6[4] = py;
+    out6[5] = pz;
+    return true;
+  }
+
+  // ---------------------------------------------------------------- branches (qz_branch.hip)
+  bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
+  bool tree_adaptation() override;
+  bool potential_branches() override;
+  bool branch_density();
+  bool branch_grid_acceleration();
+  bool branch_particle_acceleration();
+  void release_tree();
+  int num_nodes() override { return tree ? tree->n_nodes : 0; }
+  bool node_header(int i, int *hdr) override;
+  bool node_lists(int i, int *ixyz, int *bxyz, int *sxyz) override;
+  bool node_fields(int i, double *out5) override;
+};
+
+} // namespace qz
+
+#include "qz_branch.inl"
+
+namespace qz
+{
+
+EngineBase *make_engine(const qz_config &cfg)
+{
+  EngineBase *e = nullptr;
+  if (cfg.precision == QZ_PREC_MIXED)
+    e = new Engine<float, float>();
+  else
+    e = new Engine<double, double>();
+  e->cfg = cfg;
+  return e;
+}
+
+} // namespace qz
+
+// ======================================================================== C ABI
+using qz::EngineBase;
+
+struct qz_ctx
+{
+  EngineBase *eng;
+};
+
+extern "C" {
+
+void qz_default_config(qz_config *c)
+{
+  c->lmin = 5;
+  c->lmax = 5;
+  c->n_particles = 0;
+  c->precision = QZ_PREC_FP64;
+  c->head_bc = QZ_BC_MONOPOLE;
+  c->ref_criterion_ptcl = 4;
+  c->n_exp = 1;
+  c->G = 1.0;
+  c->tol = 1e-10;
+  c->tol_change = 1e-10;
+  c->max_iterations = 5000;
+  c->inner_cycles = 0;
+  c->pre_sweeps = 2;
+  c->post_sweeps = 2;
+}
+
+const char *qz_last_error(void) { return qz::g_last_error.c_str(); }
+
+qz_ctx *qz_create(const qz_config *cfg)
+{
+  if (!cfg || cfg->lmin < 2 || cfg->lmin > 10 || cfg->lmax < cfg->lmin || cfg->n_particles <= 0)
+  {
+    qz::set_error("qz_create: invalid configuration");
+    return nullptr;
+  }
+  if (cfg->n_exp != 1)
+  {
+    qz::set_error("qz_create: only n_exp = 1 is supported");
+    return nullptr;
+  }
+  if (cfg->lmax - cfg->lmin > 7)
+  {
+    qz::set_error("qz_create: at most 7 refinement levels are supported");
+    return nullptr;
+  }
+  int dev = 0;
+  if (hipGetDeviceCount(&dev) != hipSuccess || dev == 0)
+  {
+    qz::set_error("qz_create: no HIP device available");
+    return nullptr;
+  }
+  qz_config c = *cfg;
+  if (c.inner_cycles <= 0)
+    c.inner_cycles = 5;
+  if (c.pre_sweeps <= 0)
+    c.pre_sweeps = 2;
+  if (c.post_sweeps <= 0)
+    c.post_sweeps = 2;
+  EngineBase *e = qz::make_engine(c);
+  if (!qz::hip_ok(hipStreamCreate(&e->stream), "stream") || !e->init())
+  {
+    delete e;
+    return nullptr;
+  }
+  qz_ctx *ctx = new qz_ctx{e};
+  return ctx;
+}
+
+void qz_destroy(qz_ctx *ctx)
+{
+  if (!ctx)
+    return;
+  if (ctx->eng)
+  {
+    (void)hipStreamSynchronize(ctx->eng->stream);
+    hipStream_t s = ctx->eng->stream;
+    ctx->eng->timer.collect();
+    delete ctx->eng;
+    (void)hipStreamDestroy(s);
+  }
+  delete ctx;
+}
+
+#define QZ_CHECK_CTX(ctx)                                                                                             \
+  if (!(ctx) || !(ctx)->eng)                                                                                          \
+  {                                                                                                                   \
+    qz::set_error("null context");                                                                                    \
+    return 1;                                                                                                         \
+  }
+
+int qz_set_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                     const double *vy, const double *vz, const double *m)
+{
+  QZ_CHECK_CTX(ctx);
+  const double *src[7] = {x, y, z, vx, vy, vz, m};
+  for (int a = 0; a < 7; a++)
+    if (!src[a])
+    {
+      qz::set_error("qz_set_particles: null array");
+      return 1;
+    }
+  for (int i = 0; i < ctx->eng->N; i++)
+  {
+    if (!(x[i] >= 0.0 && x[i] < 1.0 && y[i] >= 0.0 && y[i] < 1.0 && z[i] >= 0.0 && z[i] < 1.0))
+    {
+      qz::set_error("The input particle ID = %d is outside of the simulation box", i);
+      return 1;
+    }
+  }
+  return ctx->eng->set_particles(src) ? 0 : 1;
+}
+
+int qz_get_particles(qz_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *ax,
+                     double *ay, double *az)
+{
+  QZ_CHECK_CTX(ctx);
+  double *dst[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+  return ctx->eng->get_particles(dst) ? 0 : 1;
+}
+
+int qz_particle_updating_A(qz_ctx *ctx, double dt)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->update_A(dt) ? 0 : 1;
+}
+int qz_particle_updating_B(qz_ctx *ctx, double dt)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->update_B(dt) ? 0 : 1;
+}
+int qz_reset(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->reset() ? 0 : 1;
+}
+int qz_tree_adaptation(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  if (ctx->eng->cfg.lmin == ctx->eng->cfg.lmax)
+    return 0;
+  return ctx->eng->tree_adaptation() ? 0 : 1;
+}
+int qz_grid_density(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->grid_density() ? 0 : 1;
+}
+int qz_potential_head_node(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->potential_head() ? 0 : 1;
+}
+int qz_potential_branches(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->potential_branches() ? 0 : 1;
+}
+int qz_potential(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  if (!ctx->eng->potential_head())
+    return 1;
+  if (ctx->eng->cfg.lmin < ctx->eng->cfg.lmax && !ctx->eng->potential_branches())
+    return 1;
+  return 0;
+}
+int qz_grid_acceleration(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->grid_acceleration() ? 0 : 1;
+}
+int qz_particle_acceleration(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->particle_acceleration() ? 0 : 1;
+}
+
+int qz_force(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  if (qz_reset(ctx) || qz_tree_adaptation(ctx) || qz_grid_density(ctx) || qz_potential(ctx) ||
+      qz_grid_acceleration(ctx) || qz_particle_acceleration(ctx))
+    return 1;
+  return 0;
+}
+
+int qz_step(qz_ctx *ctx, double dt)
+{
+  QZ_CHECK_CTX(ctx);
+  if (qz_particle_updating_A(ctx, dt) || qz_force(ctx) || qz_particle_updating_B(ctx, dt))
+    return 1;
+  return 0;
+}
+
+int qz_get_head_fields(qz_ctx *ctx, double *out5)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->head_fields(out5) ? 0 : 1;
+}
+
+int qz_set_head_potential(qz_ctx *ctx, const double *pot)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->set_head_potential(pot) ? 0 : 1;
+}
+
+int qz_num_nodes(qz_ctx *ctx)
+{
+  if (!ctx || !ctx->eng)
+    return 0;
+  return ctx->eng->num_nodes();
+}
+
+int qz_node_header(qz_ctx *ctx, int i, int *hdr24)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->node_header(i, hdr24) ? 0 : 1;
+}
+
+int qz_node_grid_lists(qz_ctx *ctx, int i, int *intr_xyz, int *bdry_xyz, int *sim_xyz)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->node_lists(i, intr_xyz, bdry_xyz, sim_xyz) ? 0 : 1;
+}
+
+int qz_node_fields(qz_ctx *ctx, int i, double *out5)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->node_fields(i, out5) ? 0 : 1;
+}
+
+int qz_get_stats(qz_ctx *ctx, qz_stats *st)
+{
+  QZ_CHECK_CTX(ctx);
+  *st = ctx->eng->stats;
+  return 0;
+}
+
+int qz_observables(qz_ctx *ctx, int exact_pe, double *out6)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->observables(exact_pe, out6) ? 0 : 1;
+}
+
+int qz_keys(const double *x, const double *y, const double *z, int n, int lv, const int *ts, const int *rd, int *out)
+{
+  if (n <= 0)
+    return 0;
+  double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+  int *dout = nullptr;
+  size_t b = sizeof(double) * (size_t)n;
+  bool ok = qz::hip_ok(hipMalloc((void **)&dx, b), "alloc") && qz::hip_ok(hipMalloc((void **)&dy, b), "alloc") &&
+            qz::hip_ok(hipMalloc((void **)&dz, b), "alloc") &&
+            qz::hip_ok(hipMalloc((void **)&dout, sizeof(int) * (size_t)n), "alloc") &&
+            qz::hip_ok(hipMemcpy(dx, x, b, hipMemcpyHostToDevice), "copy") &&
+            qz::hip_ok(hipMemcpy(dy, y, b, hipMemcpyHostToDevice), "copy") &&
+            qz::hip_ok(hipMemcpy(dz, z, b, hipMemcpyHostToDevice), "copy");
+  if (ok)
+  {
+    hipLaunchKernelGGL(qz::k_box_keys, dim3((n + qz::kBlock - 1) / qz::kBlock), dim3(qz::kBlock), 0, 0, dx, dy, dz, n,
+                       lv, ts[0], ts[1], ts[2], rd[0], rd[1], dout);
+    ok = qz::hip_ok(hipGetLastError(), "launch") &&
+         qz::hip_ok(hipMemcpy(out, dout, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost), "copy back");
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  (void)hipFree(dz);
+  (void)hipFree(dout);
+  return ok ? 0 : 1;
+}
+
+int qz_timing_enable(qz_ctx *ctx, int on)
+{
+  QZ_CHECK_CTX(ctx);
+  ctx->eng->timer.collect();
+  ctx->eng->timer.acc.clear();
+  ctx->eng->timer.on = on != 0;
+  return 0;
+}
+
+int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, int max)
+{
+  if (!ctx || !ctx->eng)
+    return 0;
+  (void)hipStreamSynchronize(ctx->eng->stream);
+  ctx->eng->timer.collect();
+  int k = 0;
+  for (auto &a : ctx->eng->timer.acc)
+  {
+    if (k >= max)
+      break;
+    snprintf(names + (size_t)k * name_len, name_len, "%s", a.name.c_str());
+    ms[k] = a.ms;
+    launches[k] = a.count;
+    k++;
+  }
+  return k;
+}
+
+int qz_sync(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return qz::hip_ok(hipStreamSynchronize(ctx->eng->stream), "sync") ? 0 : 1;
+}
+
+} // extern "C"

@@ -1,0 +1,89 @@
+// Engine interface shared by the core driver and the refinement-node code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qz.h"
+
+namespace qz
+{
+
+constexpr double kPi = 3.14159265358979323846; // global_variables.c:221
+
+void set_error(const char *fmt, ...);
+bool hip_ok(hipError_t e, const char *what);
+extern thread_local std::string g_last_error;
+
+struct Timer
+{
+  struct Rec
+  {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  struct Acc
+  {
+    std::string name;
+    double ms;
+    int count;
+  };
+  bool on = false;
+  hipEvent_t pending_start{};
+  std::vector<Rec> recs;
+  std::vector<Acc> acc;
+  void begin(hipStream_t s);
+  void end(hipStream_t s, const char *name);
+  void collect();
+};
+
+struct TreeHost; // host tree (qz_tree.h)
+struct TreeDev;  // device mirror of the tree
+
+struct TreeState
+{
+  int n_nodes = 0;
+  TreeHost *host = nullptr;
+  TreeDev *dev = nullptr;
+};
+
+struct EngineBase
+{
+  qz_config cfg{};
+  hipStream_t stream{};
+  Timer timer;
+  qz_stats stats{};
+  int N = 0;       // particles
+  int n = 0;       // head cells per side
+  int D = 0;       // octree key depth below the head
+  int end_bit = 0; // sorted key bits
+  double total_mass = 0.0;
+  size_t bytes = 0;
+  TreeState *tree = nullptr;
+
+  virtual ~EngineBase() {}
+  virtual bool init() = 0;
+  virtual bool set_particles(const double *const *src) = 0;
+  virtual bool get_particles(double *const *dst) = 0;
+  virtual bool update_A(double dt) = 0;
+  virtual bool update_B(double dt) = 0;
+  virtual bool reset() = 0;
+  virtual bool tree_adaptation() = 0;
+  virtual bool grid_density() = 0;
+  virtual bool potential_head() = 0;
+  virtual bool potential_branches() = 0;
+  virtual bool grid_acceleration() = 0;
+  virtual bool particle_acceleration() = 0;
+  virtual bool head_fields(double *out5) = 0;
+  virtual bool set_head_potential(const double *pot) = 0;
+  virtual bool observables(int exact_pe, double *out6) = 0;
+  virtual int num_nodes() = 0;
+  virtual bool node_header(int i, int *hdr) = 0;
+  virtual bool node_lists(int i, int *ixyz, int *bxyz, int *sxyz) = 0;
+  virtual bool node_fields(int i, double *out5) = 0;
+};
+
+EngineBase *make_engine(const qz_config &cfg);
+
+} // namespace qz

@@ -1,0 +1,617 @@
+// Device kernels of the PM gravity step for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// Every kernel here is memory-bound integer/stencil/scatter-gather work; none
+// is a dense contraction, so nothing is mapped to MFMA.  Layout conventions:
+//   * particles: structure of arrays, permanently kept sorted by the octree
+//     key (head cell, then child-octant path), so every cell at every level is
+//     a contiguous slot range;
+//   * grids: (n+1)^3 points of a cube with n cells per side, x fastest.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qz
+{
+
+constexpr int kBlock = 256;
+
+__host__ __device__ inline size_t pidx(int i, int j, int k, int n1) { return (size_t)i + (size_t)n1 * ((size_t)j + (size_t)n1 * k); }
+
+// ---------------------------------------------------------------- generic
+template <typename T> __global__ void k_fill(T *__restrict__ a, T v, size_t n)
+{
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n)
+    a[t] = v;
+}
+
+// Deterministic block reduction of doubles: sum (op=0) or max (op=1).
+template <int OP> __device__ inline double block_reduce(double v)
+{
+  __shared__ double sh[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1)
+  {
+    double o = __shfl_down(v, off, 64);
+    v = OP == 0 ? v + o : fmax(v, o);
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    double r = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+      r = OP == 0 ? r + sh[w] : fmax(r, sh[w]);
+    sh[0] = r;
+  }
+  __syncthreads();
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Second stage: one block folds the per-block partials in a fixed order.
+template <int OP> __global__ void k_reduce_final(const double *__restrict__ part, int np, double *__restrict__ out)
+{
+  double v = OP == 0 ? 0.0 : 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x)
+    v = OP == 0 ? v + part[i] : fmax(v, part[i]);
+  v = block_reduce<OP>(v);
+  if (threadIdx.x == 0)
+    *out = v;
+}
+
+// ---------------------------------------------------------------- particles
+// Octree key: finest coordinate F = floor(x 2^(lmin+D)); head cell from F >> D,
+// then D octant digits (x + 2y + 4z) from the most to the least significant
+// level.  floor(x 2^l) = floor(x 2^(lmin+D)) >> (lmin+D-l) exactly for x in [0,1),
+// so one key reproduces ptcl_idx_to_box_idx (ptcl_idx_to_box_idx.c:62-69) at
+// every level.
+template <typename P>
+__device__ inline unsigned long long octree_key(P x, P y, P z, int lmin, int D)
+{
+  const P s = (P)(1 << (lmin + D));
+  const int top = (1 << (lmin + D)) - 1;
+  // a reflected particle can land on exactly 1.0; it stays in the last cell
+  int fx = min(max((int)(x * s), 0), top), fy = min(max((int)(y * s), 0), top), fz = min(max((int)(z * s), 0), top);
+  int n = 1 << lmin;
+  unsigned long long hc = (unsigned long long)(fx >> D) +
+                          (unsigned long long)n * ((unsigned long long)(fy >> D) + (unsigned long long)n * (fz >> D));
+  unsigned long long k = hc;
+  for (int l = D - 1; l >= 0; l--)
+    k = (k << 3) | (unsigned long long)(((fx >> l) & 1) | (((fy >> l) & 1) << 1) | (((fz >> l) & 1) << 2));
+  return k;
+}
+
+template <typename P>
+__global__ void k_keys(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                       unsigned long long *__restrict__ key, int *__restrict__ slot, int N, int lmin, int D)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N)
+    return;
+  key[i] = octree_key<P>(x[i], y[i], z[i], lmin, D);
+  slot[i] = i;
+}
+
+// Gather every per-particle array into the sorted order (out[i] = in[perm[i]]).
+template <typename P> struct PtclArrays
+{
+  P *a[10]; // x y z vx vy vz ax ay az m
+  int *id;
+};
+
+template <typename P>
+__global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__restrict__ perm, int N)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N)
+    return;
+  int s = perm[i];
+#pragma unroll
+  for (int a = 0; a < 10; a++)
+    dst.a[a][i] = src.a[a][s];
+  dst.id[i] = src.id[s];
+}
+
+// start[c] = first sorted slot whose head cell is >= c, for c in [0, ncell].
+__global__ void k_cell_start(const unsigned long long *__restrict__ key, int N, int D, int ncell,
+                             int *__restrict__ start)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > N)
+    return;
+  long long prev = i == 0 ? -1 : (long long)(key[i - 1] >> (3 * D));
+  long long cur = i == N ? (long long)ncell : (long long)(key[i] >> (3 * D));
+  for (long long c = prev + 1; c <= cur; c++)
+    start[c] = i;
+}
+
+// A2 head deposit (grid_density.c:35-111) as a deterministic gather: each grid
+// point sums m * w over the particles of its 8 adjacent cells in slot order.
+// coeff = 4 pi G / H^3 (grid_density.c:62).  No atomics.
+template <typename P, typename G>
+__global__ void k_deposit_head(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                               const P *__restrict__ m, const int *__restrict__ start, G *__restrict__ rho, int n,
+                               double coeff)
+{
+  const int n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n1 * n1 * n1)
+    return;
+  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  const P s = (P)n;
+  double acc = 0.0;
+  for (int ck = k - 1; ck <= k; ck++)
+  {
+    if (ck < 0 || ck >= n)
+      continue;
+    for (int cj = j - 1; cj <= j; cj++)
+    {
+      if (cj < 0 || cj >= n)
+        continue;
+      for (int ci = i - 1; ci <= i; ci++)
+      {
+        if (ci < 0 || ci >= n)
+          continue;
+        int c = ci + n * (cj + n * ck);
+        int e = start[c + 1];
+        for (int p = start[c]; p < e; p++)
+        {
+          double px = (double)(x[p] * s), py = (double)(y[p] * s), pz = (double)(z[p] * s);
+          double wx1 = px - ci, wy1 = py - cj, wz1 = pz - ck;
+          double wx = ci == i ? 1.0 - wx1 : wx1;
+          double wy = cj == j ? 1.0 - wy1 : wy1;
+          double wz = ck == k ? 1.0 - wz1 : wz1;
+          acc += (double)m[p] * (wx * wy * wz);
+        }
+      }
+    }
+  }
+  rho[t] = (G)(coeff * acc);
+}
+
+// A12 head gather (particle_acceleration.c:33-132).
+template <typename P, typename G>
+__global__ void k_gather_head(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                              const G *__restrict__ gax, const G *__restrict__ gay, const G *__restrict__ gaz,
+                              P *__restrict__ ax, P *__restrict__ ay, P *__restrict__ az, int N, int n)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N)
+    return;
+  const int n1 = n + 1;
+  const P s = (P)n;
+  P pxs = x[p] * s, pys = y[p] * s, pzs = z[p] * s;
+  int fx = min((int)pxs, n - 1), fy = min((int)pys, n - 1), fz = min((int)pzs, n - 1);
+  double wx1 = (double)pxs - fx, wy1 = (double)pys - fy, wz1 = (double)pzs - fz;
+  double wx2 = 1.0 - wx1, wy2 = 1.0 - wy1, wz2 = 1.0 - wz1;
+  double w[8] = {wx2 * wy2 * wz2, wx1 * wy2 * wz2, wx2 * wy1 * wz2, wx1 * wy1 * wz2,
+                 wx2 * wy2 * wz1, wx1 * wy2 * wz1, wx2 * wy1 * wz1, wx1 * wy1 * wz1};
+  size_t g = pidx(fx, fy, fz, n1);
+  double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++)
+      {
+        size_t gn = g + ii + (size_t)jj * n1 + (size_t)kk * n1 * n1;
+        double wq = w[ii + 2 * jj + 4 * kk];
+        sx += (double)gax[gn] * wq;
+        sy += (double)gay[gn] * wq;
+        sz += (double)gaz[gn] * wq;
+      }
+  ax[p] = (P)sx;
+  ay[p] = (P)sy;
+  az[p] = (P)sz;
+}
+
+// A14 kick + drift + reflexive boundary (particle_updating.c:92-147).
+template <typename P>
+__global__ void k_update_A(P *__restrict__ x, P *__restrict__ y, P *__restrict__ z, P *__restrict__ vx,
+                           P *__restrict__ vy, P *__restrict__ vz, P *__restrict__ ax, P *__restrict__ ay,
+                           P *__restrict__ az, int N, P dt)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N)
+    return;
+  const P h = (P)0.5;
+  P pos[3] = {x[i], y[i], z[i]};
+  P vel[3] = {vx[i], vy[i], vz[i]};
+  P acc[3] = {ax[i], ay[i], az[i]};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+  {
+    vel[a] += acc[a] * dt * h;
+    pos[a] += vel[a] * dt;
+    if (pos[a] < (P)0 || pos[a] >= (P)1)
+    {
+      pos[a] = pos[a] >= (P)1 ? (P)2 - pos[a] : -pos[a];
+      vel[a] = -vel[a];
+      acc[a] = -acc[a];
+    }
+  }
+  x[i] = pos[0];
+  y[i] = pos[1];
+  z[i] = pos[2];
+  vx[i] = vel[0];
+  vy[i] = vel[1];
+  vz[i] = vel[2];
+  ax[i] = acc[0];
+  ay[i] = acc[1];
+  az[i] = acc[2];
+}
+
+// A15 kick (particle_updating.c:2677-2691).
+template <typename P>
+__global__ void k_update_B(P *__restrict__ vx, P *__restrict__ vy, P *__restrict__ vz, const P *__restrict__ ax,
+                           const P *__restrict__ ay, const P *__restrict__ az, int N, P dt)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N)
+    return;
+  const P h = (P)0.5;
+  vx[i] += ax[i] * dt * h;
+  vy[i] += ay[i] * dt * h;
+  vz[i] += az[i] * dt * h;
+}
+
+// A16 sums: m v^2, m v (observables.c:38-75, 622-659); 4 partials per block.
+template <typename P>
+__global__ void k_kin_mom(const P *__restrict__ vx, const P *__restrict__ vy, const P *__restrict__ vz,
+                          const P *__restrict__ m, int N, double *__restrict__ part)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ek = 0, px = 0, py = 0, pz = 0;
+  if (i < N)
+  {
+    double a = vx[i], b = vy[i], c = vz[i], mm = m[i];
+    ek = mm * (a * a + b * b + c * c);
+    px = mm * a;
+    py = mm * b;
+    pz = mm * c;
+  }
+  ek = block_reduce<0>(ek);
+  px = block_reduce<0>(px);
+  py = block_reduce<0>(py);
+  pz = block_reduce<0>(pz);
+  if (threadIdx.x == 0)
+  {
+    part[blockIdx.x] = ek;
+    part[gridDim.x + blockIdx.x] = px;
+    part[2 * gridDim.x + blockIdx.x] = py;
+    part[3 * gridDim.x + blockIdx.x] = pz;
+  }
+}
+
+// Exact pairwise potential energy (observables.c:77-112), FP64, tiled through LDS.
+template <typename P>
+__global__ void k_pairwise_pe(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                              const P *__restrict__ m, int N, double *__restrict__ part)
+{
+  __shared__ double tx[kBlock], ty[kBlock], tz[kBlock], tm[kBlock];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xi = 0, yi = 0, zi = 0, mi = 0;
+  if (i < N)
+  {
+    xi = x[i];
+    yi = y[i];
+    zi = z[i];
+    mi = m[i];
+  }
+  double acc = 0.0;
+  for (int base = 0; base < N; base += kBlock)
+  {
+    int j = base + threadIdx.x;
+    tx[threadIdx.x] = j < N ? (double)x[j] : 0.0;
+    ty[threadIdx.x] = j < N ? (double)y[j] : 0.0;
+    tz[threadIdx.x] = j < N ? (double)z[j] : 0.0;
+    tm[threadIdx.x] = j < N ? (double)m[j] : 0.0;
+    __syncthreads();
+    if (i < N)
+    {
+      int lim = min(kBlock, N - base);
+      for (int q = 0; q < lim; q++)
+      {
+        int jj = base + q;
+        if (jj < i)
+        {
+          double dx = xi - tx[q], dy = yi - ty[q], dz = zi - tz[q];
+          acc += mi * tm[q] / sqrt(dx * dx + dy * dy + dz * dz);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = acc;
+}
+
+// Grid-interpolated potential energy 1/2 sum m phi(x) (observables.c:594-619), head only.
+template <typename P>
+__global__ void k_grid_pe_head(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                               const P *__restrict__ m, const double *__restrict__ phi, int N, int n,
+                               double *__restrict__ part)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (p < N)
+  {
+    const int n1 = n + 1;
+    const P s = (P)n;
+    P pxs = x[p] * s, pys = y[p] * s, pzs = z[p] * s;
+    int fx = min((int)pxs, n - 1), fy = min((int)pys, n - 1), fz = min((int)pzs, n - 1);
+    double wx1 = (double)pxs - fx, wy1 = (double)pys - fy, wz1 = (double)pzs - fz;
+    double wx[2] = {1.0 - wx1, wx1}, wy[2] = {1.0 - wy1, wy1}, wz[2] = {1.0 - wz1, wz1};
+    for (int kk = 0; kk < 2; kk++)
+      for (int jj = 0; jj < 2; jj++)
+        for (int ii = 0; ii < 2; ii++)
+          acc += phi[pidx(fx + ii, fy + jj, fz + kk, n1)] * wx[ii] * wy[jj] * wz[kk];
+    acc *= (double)m[p];
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = acc;
+}
+
+// ---------------------------------------------------------------- head grid
+// A10 (initial_potential_and_acceleration_head.c:297-335): monopole -GM/r and
+// its gradient about the box centre on every point of the solve domain.
+template <typename G>
+__global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay, G *__restrict__ az,
+                            int n, double GM, int zero_bc)
+{
+  const int n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n1 * n1 * n1)
+    return;
+  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  double H = 1.0 / n;
+  double ai = i * H - 0.5, aj = j * H - 0.5, ak = k * H - 0.5;
+  double dist = sqrt(ai * ai + aj * aj + ak * ak);
+  if (dist == 0.0)
+    dist = H * 1.0e-3;
+  if (zero_bc)
+  {
+    phi[t] = 0.0;
+    ax[t] = ay[t] = az[t] = (G)0;
+    return;
+  }
+  double c = -GM / (dist * dist * dist);
+  phi[t] = -GM / dist;
+  ax[t] = (G)(c * ai);
+  ay[t] = (G)(c * aj);
+  az[t] = (G)(c * ak);
+}
+
+// Outer FP64 residual r = rho - Lap(phi) on interior points (multigrid.c:140-150,
+// poisson_error.c:116-117), stored in the V-cycle type, with per-block sum of r^2.
+template <typename G>
+__global__ void k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho, G *__restrict__ r, int n,
+                                 double inv_h2, double *__restrict__ part)
+{
+  const int n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (t < (size_t)n1 * n1 * n1)
+  {
+    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+    double res = 0.0;
+    if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+    {
+      size_t s1 = n1, s2 = (size_t)n1 * n1;
+      double c = phi[t];
+      res = (double)rho[t] + inv_h2 * (6.0 * c - phi[t + 1] - phi[t - 1] - phi[t + s1] - phi[t - s1] -
+                                       phi[t + s2] - phi[t - s2]);
+      sq = res * res;
+    }
+    r[t] = (G)res;
+  }
+  sq = block_reduce<0>(sq);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = sq;
+}
+
+// Outer correction phi += e (multigrid.c:199-209) with the pointwise change
+// test of poisson_error.c:36-81 folded in: per-block max of a flag
+// (1 where phi/phi_old leaves [1-tol, 1+tol]) and of |phi/phi_old - 1|.
+template <typename G>
+__global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ e, int n, double tol,
+                                double *__restrict__ part)
+{
+  const int n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double dev = 0.0;
+  if (t < (size_t)n1 * n1 * n1)
+  {
+    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+    if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+    {
+      double old = phi[t];
+      double nw = old + (double)e[t];
+      phi[t] = nw;
+      double ratio = nw / old;
+      dev = (1.0 - tol > ratio || 1.0 + tol < ratio) ? 2.0 + fabs(ratio - 1.0) : fabs(ratio - 1.0);
+      if (dev != dev)
+        dev = 1e300; // 0/0: treat as not converged
+    }
+  }
+  dev = block_reduce<1>(dev);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = dev;
+}
+
+// Red-black Gauss-Seidel half sweep on the error equation Lap e = r
+// (gauss_saidel.c:40-53 restated with red-black ordering, omega = 1).
+// Thread layout: one thread per point of the given colour in a row pair.
+template <typename G>
+__global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, int n, G h2, int color)
+{
+  const int n1 = n + 1;
+  const int half = n / 2 + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t tot = (size_t)half * (n - 1) * (n - 1);
+  if (t >= tot)
+    return;
+  int ih = (int)(t % half);
+  int j = 1 + (int)((t / half) % (n - 1));
+  int k = 1 + (int)(t / ((size_t)half * (n - 1)));
+  int i = 2 * ih + ((j + k + color) & 1);
+  if (i < 1 || i > n - 1)
+    return;
+  size_t g = pidx(i, j, k, n1);
+  size_t s1 = n1, s2 = (size_t)n1 * n1;
+  G sum = e[g + 1] + e[g - 1] + e[g + s1] + e[g - s1] + e[g + s2] + e[g - s2];
+  e[g] = (sum - h2 * r[g]) * (G)(1.0 / 6.0);
+}
+
+// Residual of the fine error equation, full-weighting restricted to the coarse
+// grid (27-point stencil, weights 1/64 * (1,2,1)^3).  rc at coarse interior.
+template <typename G>
+__global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict__ r, G *__restrict__ rc, int n,
+                                    G inv_h2)
+{
+  const int nc = n / 2, nc1 = nc + 1, n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nc1 * nc1 * nc1)
+    return;
+  int I = (int)(t % nc1), J = (int)((t / nc1) % nc1), K = (int)(t / ((size_t)nc1 * nc1));
+  if (I == 0 || I == nc || J == 0 || J == nc || K == 0 || K == nc)
+  {
+    rc[t] = (G)0;
+    return;
+  }
+  const size_t s1 = n1, s2 = (size_t)n1 * n1;
+  G acc = (G)0;
+  for (int dk = -1; dk <= 1; dk++)
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++)
+      {
+        int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+        if (i <= 0 || i >= n || j <= 0 || j >= n || k <= 0 || k >= n)
+          continue;
+        size_t g = pidx(i, j, k, n1);
+        G lap = (e[g + 1] + e[g - 1] + e[g + s1] + e[g - s1] + e[g + s2] + e[g - s2] - (G)6 * e[g]) * inv_h2;
+        G res = r[g] - lap;
+        G w = (G)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1)) * (G)(1.0 / 64.0);
+        acc += w * res;
+      }
+  rc[t] = acc;
+}
+
+// Trilinear prolongation of the coarse correction into the fine interior
+// (multigrid.c:51-104) fused with the correction e += P ec (:199-209).
+template <typename G>
+__global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, int n)
+{
+  const int n1 = n + 1, nc1 = n / 2 + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n1 * n1 * n1)
+    return;
+  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
+    return;
+  int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
+  G v = (ec[pidx(i0, j0, k0, nc1)] + ec[pidx(i1, j0, k0, nc1)] + ec[pidx(i0, j1, k0, nc1)] +
+         ec[pidx(i1, j1, k0, nc1)] + ec[pidx(i0, j0, k1, nc1)] + ec[pidx(i1, j0, k1, nc1)] +
+         ec[pidx(i0, j1, k1, nc1)] + ec[pidx(i1, j1, k1, nc1)]) *
+        (G)0.125;
+  e[t] += v;
+}
+
+// Coarsest level: conjugate gradients in one workgroup (FP64 internally) on a
+// grid of at most 9^3 points, zero Dirichlet faces, solving Lap e = r.
+template <typename G> __global__ void k_coarse_cg(G *__restrict__ e, const G *__restrict__ r, int n, int iters)
+{
+  constexpr int MAXP = 729;
+  __shared__ double xs[MAXP], rs[MAXP], ps[MAXP], aps[MAXP];
+  const int n1 = n + 1, np = n1 * n1 * n1;
+  const double inv_h2 = (double)n * n;
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+  {
+    int i = t % n1, j = (t / n1) % n1, k = t / (n1 * n1);
+    bool in = i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
+    xs[t] = 0.0;
+    // A = -Lap (SPD); solve A x = -r
+    rs[t] = in ? -(double)r[t] : 0.0;
+    ps[t] = rs[t];
+  }
+  __syncthreads();
+  double rr = 0.0;
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+    rr += rs[t] * rs[t];
+  rr = block_reduce<0>(rr);
+  const double rr0 = rr;
+  for (int it = 0; it < iters && rr > 1e-28 * rr0 && rr > 0.0; it++)
+  {
+    double pap = 0.0;
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+    {
+      int i = t % n1, j = (t / n1) % n1, k = t / (n1 * n1);
+      bool in = i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
+      double v = 0.0;
+      if (in)
+        v = inv_h2 * (6.0 * ps[t] - ps[t + 1] - ps[t - 1] - ps[t + n1] - ps[t - n1] - ps[t + n1 * n1] -
+                      ps[t - n1 * n1]);
+      aps[t] = v;
+      pap += ps[t] * v;
+    }
+    pap = block_reduce<0>(pap);
+    double alpha = rr / pap;
+    double rn = 0.0;
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+    {
+      xs[t] += alpha * ps[t];
+      rs[t] -= alpha * aps[t];
+      rn += rs[t] * rs[t];
+    }
+    rn = block_reduce<0>(rn);
+    double beta = rn / rr;
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+      ps[t] = rs[t] + beta * ps[t];
+    rr = rn;
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+    e[t] = (G)xs[t];
+}
+
+// A11 head gradient on interior points (grid_acceleration.c:36-76); the faces
+// keep their A10 values.
+template <typename G>
+__global__ void k_grid_acc_head(const double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay,
+                                G *__restrict__ az, int n, double inv_2h)
+{
+  const int n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n1 * n1 * n1)
+    return;
+  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
+    return;
+  size_t s1 = n1, s2 = (size_t)n1 * n1;
+  ax[t] = (G)(inv_2h * (phi[t - 1] - phi[t + 1]));
+  ay[t] = (G)(inv_2h * (phi[t - s1] - phi[t + s1]));
+  az[t] = (G)(inv_2h * (phi[t - s2] - phi[t + s2]));
+}
+
+// A1 for arbitrary node geometry (host-facing utility).
+__global__ void k_box_keys(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                           int n, int lv, int tsx, int tsy, int tsz, int rdx, int rdy, int *__restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double s = (double)(1 << lv);
+  double px = x[i] * s, py = y[i] * s, pz = z[i] * s;
+  int fx = px < 0 ? -1 : (int)px, fy = py < 0 ? -1 : (int)py, fz = pz < 0 ? -1 : (int)pz;
+  out[i] = (fx - tsx) + (fy - tsy) * rdx + (fz - tsz) * rdx * rdy;
+}
+
+} // namespace qz

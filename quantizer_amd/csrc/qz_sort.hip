@@ -1,0 +1,48 @@
+// Stable radix sort of (cell key, particle slot) pairs on the device.
+// Kept in its own translation unit: rocPRIM's radix sort is slow to compile
+// and never changes with the rest of the library.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "qz_internal.h"
+
+size_t qz_sort_temp_bytes(int n, int end_bit)
+{
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                     (const int *)nullptr, (int *)nullptr, n, 0, end_bit);
+  return bytes;
+}
+
+hipError_t qz_sort_pairs(void *tmp, size_t bytes, const unsigned *kin, unsigned *kout, const int *vin, int *vout,
+                         int n, int end_bit, hipStream_t s)
+{
+  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout, n, 0, end_bit, s);
+}
+
+size_t qz_sort64_temp_bytes(int n, int end_bit)
+{
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned long long *)nullptr,
+                                     (unsigned long long *)nullptr, (const int *)nullptr, (int *)nullptr, n, 0,
+                                     end_bit);
+  return bytes;
+}
+
+hipError_t qz_sort64_pairs(void *tmp, size_t bytes, const unsigned long long *kin, unsigned long long *kout,
+                           const int *vin, int *vout, int n, int end_bit, hipStream_t s)
+{
+  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout, n, 0, end_bit, s);
+}
+
+size_t qz_scan_temp_bytes(int n)
+{
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int *)nullptr, (int *)nullptr, n);
+  return bytes;
+}
+
+hipError_t qz_exclusive_scan(void *tmp, size_t bytes, const int *in, int *out, int n, hipStream_t s)
+{
+  return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, s);
+}

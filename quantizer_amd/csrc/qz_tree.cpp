@@ -1,0 +1,318 @@
+// Host side of the refinement tree (see qz_tree.h).  Restates the marking,
+// zoning and child-box steps of tree_construction.c (non-periodic boundaries):
+//   fill_cell_ref       tree_construction.c:508-598
+//   fill_zones_ref      tree_construction.c:685-926
+//   create_child_nodes  tree_construction.c:1032-1661 (box geometry part)
+// The node is rebuilt from scratch every step instead of adapted
+// (tree_adaptation.c); fields are compared at global grid coordinates.
+#include "qz_tree.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace qz
+{
+
+void TreeHost::init(int lmin_, int lmax_, int crit)
+{
+  lmin = lmin_;
+  lmax = lmax_;
+  n = 1 << lmin;
+  ref_crit = crit;
+  extra = 10; // min_box_extra_size_per_side_real for n_exp = 1 (global_variables.c:294-295)
+  int side = n + 2;
+  head_box.assign((size_t)side * side * side, -3);
+  for (int k = 0; k < side; k++)
+    for (int j = 0; j < side; j++)
+      for (int i = 0; i < side; i++)
+      {
+        bool border = i == 0 || i == side - 1 || j == 0 || j == side - 1 || k == 0 || k == side - 1;
+        if (border)
+          head_box[(size_t)i + side * ((size_t)j + side * k)] = -5; // initialization.c:96-113
+      }
+  head_marked.clear();
+  nodes.clear();
+  lev_start.assign(1, 0);
+  lev_count.assign(1, 1);
+}
+
+void TreeHost::clear()
+{
+  for (int b : head_marked)
+    head_box[b] = -3;
+  head_marked.clear();
+  head_child.clear();
+  nodes.clear();
+  lev_start.assign(1, 0);
+  lev_count.assign(1, 1);
+}
+
+// 6-connected flood fill of the -1 cells (fill_zones_ref, tree_construction.c:748-891).
+// box: statuses, rX/rXY strides, cell_ref: box indices of marked cells in cell order.
+static void zone_fill(std::vector<int> &box, int rX, int rXY, const std::vector<int> &ref_box,
+                      std::vector<int> &zone_size, std::vector<std::vector<int>> &zones_by_ref)
+{
+  int nref = (int)ref_box.size();
+  std::vector<int> aux(nref > 0 ? nref : 1);
+  int added_all = 0, ref_i = 0, zone = 0;
+  zone_size.clear();
+  while (nref > added_all)
+  {
+    int b = ref_box[ref_i];
+    if (box[b] == -1)
+    {
+      int zs = 0;
+      aux[0] = b;
+      box[b] = zone;
+      zs++;
+      added_all++;
+      int insp = 0;
+      while (zs > insp && nref > added_all)
+      {
+        int add = 0;
+        int c = aux[insp];
+        const int nbs[6] = {c + 1, c - 1, c + rX, c - rX, c + rXY, c - rXY};
+        for (int q = 0; q < 6; q++)
+        {
+          if (box[nbs[q]] == -1)
+          {
+            aux[zs + add] = nbs[q];
+            box[nbs[q]] = zone;
+            add++;
+          }
+        }
+        zs += add;
+        added_all += add;
+        insp++;
+      }
+      zone_size.push_back(zs);
+      zone++;
+    }
+    ref_i++;
+  }
+  // zone member lists in cell_ref order (tree_construction.c:906-914)
+  zones_by_ref.assign(zone, {});
+  for (int z = 0; z < zone; z++)
+    zones_by_ref[z].reserve(zone_size[z]);
+  for (int i = 0; i < nref; i++)
+    zones_by_ref[box[ref_box[i]]].push_back(i);
+}
+
+void TreeHost::build_node_box(HNode &nd)
+{
+  int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
+  size_t cap = (size_t)nd.rd[0] * nd.rd[1] * nd.rd[2];
+  nd.box.assign(cap, -4);
+  nd.pos_of_box.assign(cap, -1);
+  for (int c = 0; c < nd.ncell; c++)
+  {
+    nd.box[nd.cbox[c]] = -3;
+    nd.pos_of_box[nd.cbox[c]] = c;
+  }
+  if (nd.sim_contact)
+  {
+    int top = (1 << nd.lv) - 1;
+    for (int a = 0; a < 3; a++)
+    {
+      if (!nd.sc[a])
+        continue;
+      int lo_end = nd.bmin[a] == 0 ? nd.bmin[a] - nd.ts[a] : 0;
+      int hi_beg = nd.bmax[a] == top ? nd.bmax[a] + 1 - nd.ts[a] : nd.rd[a];
+      for (int k = 0; k < nd.rd[2]; k++)
+        for (int j = 0; j < nd.rd[1]; j++)
+          for (int i = 0; i < nd.rd[0]; i++)
+          {
+            int q = a == 0 ? i : (a == 1 ? j : k);
+            if (q < lo_end || q >= hi_beg)
+              nd.box[i + j * rX + k * rXY] = -5;
+          }
+    }
+  }
+}
+
+// create_child_nodes box geometry (tree_construction.c:1092-1661, non-pbc paths)
+int TreeHost::make_children(int parent_global, int parent_lv, const int *pcxyz, const std::vector<int> &zone_sizes,
+                            const std::vector<std::vector<int>> &zone_cells)
+{
+  int nz = (int)zone_sizes.size();
+  int first = (int)nodes.size();
+  for (int z = 0; z < nz; z++)
+  {
+    HNode ch;
+    ch.lv = parent_lv + 1;
+    ch.parent = parent_global;
+    ch.zone = z;
+    for (int a = 0; a < 3; a++)
+    {
+      ch.bmin[a] = 1 << 30;
+      ch.bmax[a] = -(1 << 30);
+    }
+    for (int pos : zone_cells[z])
+      for (int a = 0; a < 3; a++)
+      {
+        int v = pcxyz[3 * pos + a];
+        ch.bmin[a] = std::min(ch.bmin[a], v);
+        ch.bmax[a] = std::max(ch.bmax[a], v);
+      }
+    for (int a = 0; a < 3; a++)
+      if (ch.bmin[a] == 0 || ch.bmax[a] == (1 << parent_lv) - 1)
+      {
+        ch.sim_contact = true;
+        ch.sc[a] = true;
+      }
+    for (int a = 0; a < 3; a++)
+    {
+      ch.bmin[a] *= 2;
+      ch.bmax[a] = 2 * ch.bmax[a] + 1;
+      int dim = ch.bmax[a] - ch.bmin[a] + 1;
+      ch.rd[a] = dim + extra;
+      ch.ts[a] = ch.bmin[a] - (ch.rd[a] - dim) / 2;
+    }
+    ch.ncell = 8 * (int)zone_cells[z].size();
+    ch.cxyz.resize(3 * (size_t)ch.ncell);
+    ch.cbox.resize(ch.ncell);
+    int rX = ch.rd[0], rXY = ch.rd[0] * ch.rd[1];
+    int j = 0;
+    for (int pos : zone_cells[z])
+    {
+      int ax = 2 * pcxyz[3 * pos], ay = 2 * pcxyz[3 * pos + 1], az = 2 * pcxyz[3 * pos + 2];
+      for (int kk = 0; kk < 2; kk++)
+        for (int jj = 0; jj < 2; jj++)
+          for (int ii = 0; ii < 2; ii++)
+          {
+            int c = ii + 2 * jj + 4 * kk + 8 * j;
+            int gx = ax + ii, gy = ay + jj, gz = az + kk;
+            ch.cxyz[3 * c] = gx;
+            ch.cxyz[3 * c + 1] = gy;
+            ch.cxyz[3 * c + 2] = gz;
+            ch.cbox[c] = (gx - ch.ts[0]) + (gy - ch.ts[1]) * rX + (gz - ch.ts[2]) * rXY;
+          }
+      j++;
+    }
+    ch.bcap = ch.rd[0] * ch.rd[1] * ch.rd[2];
+    ch.gcap = (ch.rd[0] + 1) * (ch.rd[1] + 1) * (ch.rd[2] + 1);
+    ch.pcap = (ch.bmax[0] - ch.bmin[0] + 2) * (ch.bmax[1] - ch.bmin[1] + 2) * (ch.bmax[2] - ch.bmin[2] + 2);
+    if (ch.lv < lmax)
+      build_node_box(ch);
+    nodes.push_back(std::move(ch));
+  }
+  return first;
+}
+
+int TreeHost::refine_head(const std::vector<int> &flagged)
+{
+  clear();
+  const int side = n + 2, s2 = side * side;
+  auto box_of = [&](int c) {
+    int i = c % n, j = (c / n) % n, k = c / (n * n);
+    return (i + 1) + side * ((j + 1) + side * (k + 1));
+  };
+  // fill_cell_ref (tree_construction.c:527-575), n_exp = 1
+  for (int c : flagged)
+  {
+    int b = box_of(c);
+    if (head_box[b] == -3)
+    {
+      head_box[b] = -1;
+      head_marked.push_back(b);
+    }
+    for (int kk = -1; kk <= 1; kk++)
+      for (int jj = -1; jj <= 1; jj++)
+        for (int ii = -1; ii <= 1; ii++)
+        {
+          int nb = b + ii + jj * side + kk * s2;
+          if (head_box[nb] == -3)
+          {
+            head_box[nb] = -1;
+            head_marked.push_back(nb);
+          }
+        }
+  }
+  // cell_ref in cell order: head box order equals cell order
+  std::vector<int> ref_box = head_marked;
+  std::sort(ref_box.begin(), ref_box.end());
+  std::vector<int> zone_sizes;
+  std::vector<std::vector<int>> zones;
+  zone_fill(head_box, side, s2, ref_box, zone_sizes, zones);
+  // parent cell coordinates of the cell_ref entries
+  std::vector<int> pc(3 * ref_box.size());
+  for (size_t q = 0; q < ref_box.size(); q++)
+  {
+    int b = ref_box[q];
+    pc[3 * q] = b % side - 1;
+    pc[3 * q + 1] = (b / side) % side - 1;
+    pc[3 * q + 2] = b / s2 - 1;
+  }
+  int first = make_children(-1, lmin, pc.data(), zone_sizes, zones);
+  int cnt = (int)nodes.size() - first;
+  head_child.assign(cnt, 0);
+  for (int z = 0; z < cnt; z++)
+    head_child[z] = first + z;
+  lev_start.push_back(first);
+  lev_count.push_back(cnt);
+  return cnt;
+}
+
+int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell)
+{
+  int start = lev_start[lr], cnt = lev_count[lr];
+  int next_first = (int)nodes.size();
+  size_t f = 0;
+  for (int q = 0; q < cnt; q++)
+  {
+    int g = start + q;
+    HNode &nd = nodes[g];
+    int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
+    std::vector<int> marked_pos;
+    while (f < flag_node.size() && flag_node[f] == q)
+    {
+      int c = flag_cell[f++];
+      int b = nd.cbox[c];
+      if (nd.box[b] == -3)
+      {
+        nd.box[b] = -1;
+        marked_pos.push_back(c);
+      }
+      for (int kk = -1; kk <= 1; kk++)
+        for (int jj = -1; jj <= 1; jj++)
+          for (int ii = -1; ii <= 1; ii++)
+          {
+            int nb = b + ii + jj * rX + kk * rXY;
+            if (nd.box[nb] == -3)
+            {
+              nd.box[nb] = -1;
+              marked_pos.push_back(nd.pos_of_box[nb]);
+            }
+          }
+    }
+    if (marked_pos.empty())
+      continue;
+    std::sort(marked_pos.begin(), marked_pos.end());
+    std::vector<int> ref_box(marked_pos.size());
+    std::vector<int> pc(3 * marked_pos.size());
+    for (size_t t = 0; t < marked_pos.size(); t++)
+    {
+      ref_box[t] = nd.cbox[marked_pos[t]];
+      for (int a = 0; a < 3; a++)
+        pc[3 * t + a] = nd.cxyz[3 * marked_pos[t] + a];
+    }
+    std::vector<int> zone_sizes;
+    std::vector<std::vector<int>> zones;
+    zone_fill(nd.box, rX, rXY, ref_box, zone_sizes, zones);
+    int lv = nd.lv;
+    int first = make_children(g, lv, pc.data(), zone_sizes, zones);
+    HNode &nd2 = nodes[g]; // nodes may have been reallocated
+    nd2.child.clear();
+    for (int z = 0; z < (int)zone_sizes.size(); z++)
+      nd2.child.push_back(first + z);
+  }
+  int made = (int)nodes.size() - next_first;
+  if (made > 0)
+  {
+    lev_start.push_back(next_first);
+    lev_count.push_back(made);
+  }
+  return made;
+}
+
+} // namespace qz

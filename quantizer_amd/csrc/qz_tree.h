@@ -1,0 +1,61 @@
+// Host side of the refinement tree: the irregular, graph-shaped part of
+// tree_construction.c (dilated refinement marks, 6-connected zones, child box
+// geometry).  Bulk per-cell and per-grid-point work (child cell expansion, box
+// status, grid-point classification, particle ranges, refinement counts) runs
+// on the device from what this produces.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace qz
+{
+
+// One refinement node as the host knows it.  Geometry fields follow struct node
+// (initialize_node.h:63-216): box_ts_*, box_real_dim_*, box_min_*, box_max_*.
+struct HNode
+{
+  int lv = 0;       // absolute level
+  int parent = -1;  // global node index, -1 = head
+  int zone = 0;     // zone id in the parent (= ID, tree_construction.c:1090)
+  int ts[3] = {0, 0, 0};
+  int rd[3] = {0, 0, 0};
+  int bmin[3] = {0, 0, 0};
+  int bmax[3] = {0, 0, 0};
+  bool sc[3] = {false, false, false};
+  bool sim_contact = false;
+  // cells: 8 per zone cell of the parent, in zone order then octant order
+  std::vector<int> cxyz;      // 3 per cell, global level-lv coordinates
+  std::vector<int> cbox;      // box index of each cell
+  std::vector<int> pos_of_box; // box index -> cell position, -1 if not a cell (built for refinable nodes)
+  std::vector<int> box;       // box status (built for refinable nodes)
+  std::vector<int> child;     // global node indices by zone id
+  std::vector<int> marked;    // box indices whose status this build changed
+  // pools on the device (filled by the device mirror)
+  long long goff = 0, boff = 0, coff = 0, poff = 0;
+  int gcap = 0, bcap = 0, ncell = 0, pcap = 0;
+};
+
+struct TreeHost
+{
+  int lmin = 0, lmax = 0, n = 0, ref_crit = 8, extra = 10;
+  std::vector<int> head_box;    // (n+2)^3 status, ts = -1 (initialization.c:96-113)
+  std::vector<int> head_marked; // box indices changed by the last build
+  std::vector<int> head_child;  // global node index per refined head box cell (by zone)
+  std::vector<HNode> nodes;     // refinement nodes, level order (tentacles order, tree_construction.c:2465-2495)
+  std::vector<int> lev_start, lev_count; // relative level 1.. -> ranges in nodes
+
+  void init(int lmin_, int lmax_, int crit);
+  // Head level: flagged head cells (linear index i + n (j + n k), ascending).
+  // Creates the level-1 nodes; returns their count.
+  int refine_head(const std::vector<int> &flagged);
+  // Level lr >= 1: flagged cells as (node position in level, cell position), sorted.
+  int refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell);
+  void clear();
+
+private:
+  int make_children(int parent_global, int parent_lv, const int *cell_xyz_parent, const std::vector<int> &zone_sizes,
+                    const std::vector<std::vector<int>> &zones);
+  void build_node_box(HNode &nd);
+};
+
+} // namespace qz

@@ -1,0 +1,57 @@
+"""Head-only PM step on the GPU vs the oracle (configs shaped like the 32^3 / 64^3 head-only runs)."""
+import numpy as np
+import pytest
+
+from quantizer_amd import core
+from quantizer_amd.workloads import plummer, plummer_dt
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("prec,tol_rho,tol_pot,tol_acc", [
+    (core.PREC_FP64, 1e-12, 1e-8, 1e-6),
+    (core.PREC_MIXED, 2e-6, 2e-6, 2e-5),
+])
+def test_head_force_pass_matches_oracle(oracle, prec, tol_rho, tol_pot, tol_acc):
+    print(f"case: head force pass prec={prec} N=4000 lmin=5", flush=True)
+    p = plummer(4000, seed=11)
+    o = oracle.OracleSim(p, 5, 5)
+    o.force()
+    ho = o.head()
+    with core.Simulation(p, 5, 5, precision=prec) as s:
+        s.force()
+        hg = s.head_fields()
+        pg = s.particles()
+        st = s.stats()
+    po = o.particles()
+    assert rel(hg[0], ho[0]) < tol_rho
+    assert rel(hg[1], ho[1]) < tol_pot
+    for k in (2, 3, 4):
+        assert rel(hg[k], ho[k]) < tol_acc
+    for k in (6, 7, 8):
+        assert rel(pg[k], po[k]) < tol_acc
+    assert st["head_residual"] < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol_x,tol_v", [(core.PREC_FP64, 1e-10, 1e-7), (core.PREC_MIXED, 1e-6, 1e-4)])
+def test_head_steps_match_oracle(oracle, prec, tol_x, tol_v):
+    print(f"case: head 5 steps prec={prec} N=3000 lmin=5", flush=True)
+    p = plummer(3000, seed=5)
+    dt = plummer_dt(0.01)
+    o = oracle.OracleSim(p, 5, 5)
+    o.force()
+    with core.Simulation(p, 5, 5, precision=prec) as s:
+        s.force()
+        for _ in range(5):
+            o.step(dt)
+            s.step(dt)
+        pg = s.particles()
+    po = o.particles()
+    for k in range(3):
+        assert np.max(np.abs(pg[k] - po[k])) < tol_x
+    for k in range(3, 6):
+        assert rel(pg[k], po[k]) < tol_v
