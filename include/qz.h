@@ -110,15 +110,22 @@ int qz_get_stats(qz_ctx *ctx, qz_stats *st);
  * interpolation (observables.c:594-619). */
 int qz_observables(qz_ctx *ctx, int exact_pe, double *out6);
 
+/* CFL time step with velocity and acceleration (timestep.c:589-699 with
+ * time_step_method = 1): min over particles and axes, each particle using the
+ * cell size of the finest node holding it, capped at dt_max */
+int qz_timestep(qz_ctx *ctx, double cfl, double dt_max, double *dt);
+
 /* A1 (ptcl_idx_to_box_idx.c:30-139) on the device for n host positions:
  * key = (f(x 2^lv) - ts) linearised with the real box dims rd, f(p) = p<0 ? -1 : (int)p */
 int qz_keys(const double *x, const double *y, const double *z, int n, int lv, const int *ts, const int *rd,
             int *out);
 
-/* device timing of the hot kernels over the calls since the last reset:
- * names[i] / total ms / launches, up to max entries; returns count */
+/* device timing (HIP events on the context's stream) of every kernel launched
+ * since qz_timing_enable(ctx, 1): per kernel name the total ms, the launch
+ * count and the algorithmic bytes (each input read once, each output written
+ * once), up to max entries; returns the number of entries */
 int qz_timing_enable(qz_ctx *ctx, int on);
-int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, int max);
+int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, double *bytes, int max);
 int qz_sync(qz_ctx *ctx);
 
 #ifdef __cplusplus

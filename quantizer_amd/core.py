@@ -80,7 +80,7 @@ EXPORTED = [
     "qz_potential_head_node", "qz_potential_branches", "qz_grid_acceleration", "qz_particle_acceleration",
     "qz_particle_updating_B", "qz_force", "qz_step", "qz_get_head_fields", "qz_set_head_potential",
     "qz_num_nodes", "qz_node_header", "qz_node_grid_lists", "qz_node_fields", "qz_get_stats", "qz_observables",
-    "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
+    "qz_timestep", "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
 ]
 
 
@@ -117,9 +117,10 @@ def load_library(path: str = LIB_PATH):
     lib.qz_node_fields.argtypes = [vp, ctypes.c_int, _dp]
     lib.qz_get_stats.argtypes = [vp, ctypes.POINTER(QzStats)]
     lib.qz_observables.argtypes = [vp, ctypes.c_int, _dp]
+    lib.qz_timestep.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp]
     lib.qz_keys.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]
     lib.qz_timing_enable.argtypes = [vp, ctypes.c_int]
-    lib.qz_timing_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _dp, _ip, ctypes.c_int]
+    lib.qz_timing_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int]
     lib.qz_timing_report.restype = ctypes.c_int
     _LIB = lib
     return lib
@@ -287,6 +288,12 @@ class Simulation:
             res.append(d)
         return res
 
+    def timestep(self, cfl: float = 0.5, dt_max: float = 1e300) -> float:
+        """CFL time step from the current velocities and accelerations (timestep.c:589-699)."""
+        out = np.zeros(1)
+        _check(load_library().qz_timestep(self._h, cfl, dt_max, _d(out)), "qz_timestep")
+        return float(out[0])
+
     # ------------------------------------------------------------ timing
     def timing(self, on: bool = True):
         return _check(load_library().qz_timing_enable(self._h, 1 if on else 0), "qz_timing_enable")
@@ -297,12 +304,13 @@ class Simulation:
         names = ctypes.create_string_buffer(mx * nl)
         ms = np.zeros(mx)
         cnt = np.zeros(mx, dtype=np.int32)
-        k = lib.qz_timing_report(self._h, names, nl, _d(ms), _i(cnt), mx)
+        byt = np.zeros(mx)
+        k = lib.qz_timing_report(self._h, names, nl, _d(ms), _i(cnt), _d(byt), mx)
         raw = names.raw
         res = {}
         for q in range(k):
             nm = raw[q * nl:(q + 1) * nl].split(b"\0", 1)[0].decode()
-            res[nm] = (float(ms[q]), int(cnt[q]))
+            res[nm] = {"ms": float(ms[q]), "launches": int(cnt[q]), "bytes": float(byt[q])}
         return res
 
 

@@ -370,7 +370,7 @@ template <typename P, typename G> bool Engine<P, G>::branch_density()
     dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
     timer.begin(stream);
     hipLaunchKernelGGL((k_branch_deposit<P, G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
-                       d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz, d.bstart, d.bend, pa[0], pa[1], pa[2],
+                       d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz, d.bstart, d.bend, pos[0], pos[1], pos[2],
                        pa[9], (G *)d.rho, 4.0 * cfg.G * kPi);
     timer.end(stream, "deposit_branch");
   }
@@ -464,7 +464,7 @@ template <typename P, typename G> bool Engine<P, G>::branch_grid_acceleration()
 template <typename P, typename G> bool Engine<P, G>::branch_particle_acceleration()
 {
   TreeDev &d = *tree->dev;
-  QZ_LAUNCH("gather_amr", (k_gather_amr<P, G>), nblk(N), pa[0], pa[1], pa[2], gax, gay, gaz, d.head_child, d.nodes,
+  QZ_LAUNCH("gather_amr", (k_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, d.head_child, d.nodes,
             d.box, (G *)d.ax, (G *)d.ay, (G *)d.az, pa[6], pa[7], pa[8], N, cfg.lmin);
   return true;
 }
@@ -541,6 +541,24 @@ template <typename P, typename G> bool Engine<P, G>::node_fields(int i, double *
     for (size_t t = 0; t < g; t++)
       out5[a * g + t] = (double)tmp[t];
   }
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::timestep(double cfl, double dt_max, double *dt)
+{
+  unsigned nb = nblk(N);
+  const int *hc = has_tree() ? tree->dev->head_child : nullptr;
+  const DNode *nd = has_tree() ? tree->dev->nodes : nullptr;
+  const int *bx = has_tree() ? tree->dev->box : nullptr;
+  QZ_LAUNCHB("cfl_dt", 9.0 * sizeof(P) * N, k_cfl_dt<P>, nb, pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7],
+             pa[8], N, hc, nd, bx, cfg.lmin, cfl, dt_max, part);
+  // fold the per-block minima through max(-dt)
+  hipLaunchKernelGGL(k_negate, dim3(nb / kBlock + 1), dim3(kBlock), 0, stream, part, (int)nb);
+  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+  if (!hip_ok(hipMemcpyAsync(hred, red, sizeof(double), hipMemcpyDeviceToHost, stream), "readback") ||
+      !hip_ok(hipStreamSynchronize(stream), "timestep"))
+    return false;
+  *dt = -hred[0];
   return true;
 }
 

@@ -699,8 +699,9 @@ __global__ void k_branch_deposit(const DNode *__restrict__ nodes, int first, con
                                  const int *__restrict__ sim_xyz, const int *__restrict__ bdry_g,
                                  const int *__restrict__ bdry_xyz, const int *__restrict__ intr_g,
                                  const int *__restrict__ intr_xyz, const int *__restrict__ bstart,
-                                 const int *__restrict__ bend, const P *__restrict__ x, const P *__restrict__ y,
-                                 const P *__restrict__ z, const P *__restrict__ m, G *__restrict__ rho, double G4pi)
+                                 const int *__restrict__ bend, const double *__restrict__ x,
+                                 const double *__restrict__ y, const double *__restrict__ z, const P *__restrict__ m,
+                                 G *__restrict__ rho, double G4pi)
 {
   const DNode &d = nodes[first + blockIdx.y];
   const int tot = d.nsim + d.nbdry + d.nintr;
@@ -740,7 +741,7 @@ __global__ void k_branch_deposit(const DNode *__restrict__ nodes, int first, con
           int a = bstart[b], e = bend[b];
           for (int p = a; p < e; p++)
           {
-            double px = (double)x[p] * s, py = (double)y[p] * s, pz = (double)z[p] * s;
+            double px = x[p] * s, py = y[p] * s, pz = z[p] * s;
             double wx1 = px - ci, wy1 = py - cj, wz1 = pz - ck;
             double wx = ci == gx ? 1.0 - wx1 : wx1;
             double wy = cj == gy ? 1.0 - wy1 : wy1;
@@ -755,7 +756,7 @@ __global__ void k_branch_deposit(const DNode *__restrict__ nodes, int first, con
 // A12 with refinement (particle_acceleration.c:134-355): each particle takes
 // the trilinear acceleration of the finest node holding it.
 template <typename P, typename G>
-__global__ void k_gather_amr(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+__global__ void k_gather_amr(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                              const G *__restrict__ hax, const G *__restrict__ hay, const G *__restrict__ haz,
                              const int *__restrict__ head_child, const DNode *__restrict__ nodes,
                              const int *__restrict__ box, const G *__restrict__ bax, const G *__restrict__ bay,
@@ -830,6 +831,80 @@ __global__ void k_gather_amr(const P *__restrict__ x, const P *__restrict__ y, c
   ax[p] = (P)sx;
   ay[p] = (P)sy;
   az[p] = (P)sz;
+}
+
+// Finest node holding a position (the walk of k_gather_amr): returns the node
+// index (-1 = head) and sets its level.
+__device__ inline int finest_node(double px0, double py0, double pz0, const int *head_child, const DNode *nodes,
+                                  const int *box, int lmin, int &lv)
+{
+  const int n = 1 << lmin;
+  int hx = min((int)(px0 * n), n - 1), hy = min((int)(py0 * n), n - 1), hz = min((int)(pz0 * n), n - 1);
+  int q = head_child ? head_child[hx + n * (hy + n * hz)] : -1;
+  lv = lmin;
+  int found = -1;
+  while (q >= 0)
+  {
+    const DNode &d = nodes[q];
+    double s = (double)(1 << d.lv);
+    int cx = (int)(px0 * s), cy = (int)(py0 * s), cz = (int)(pz0 * s);
+    int b = (cx - d.ts[0]) + (cy - d.ts[1]) * d.rd[0] + (cz - d.ts[2]) * d.rd[0] * d.rd[1];
+    int st = box[d.boff + b];
+    lv = d.lv;
+    found = q;
+    if (st < 0)
+      break;
+    q = st;
+  }
+  return found;
+}
+
+// CFL time step with velocity and acceleration (timestep.c:73-195, 370-587):
+// per axis the dt that moves a particle CFL * H of its finest node, min over
+// particles and axes; per-block minima for a deterministic second stage.
+__device__ inline double cfl_axis(double v, double a, double H, double cfl, double dt)
+{
+  double av = fabs(v), aa = fabs(a);
+  if (aa < 1.0e-16)
+  {
+    if (av > 1.0e-16)
+      dt = fmin(dt, cfl * H / av);
+    return dt;
+  }
+  double c = v * v + 2.0 * aa * H * cfl;
+  double t;
+  if (1.0 - v * v / c > 1.0e-8)
+    t = (-av + sqrt(c)) / aa;
+  else
+  {
+    double eps = fabs(a * H * cfl / (v * v));
+    t = fabs(v / a) * (eps - eps * eps * 0.5);
+  }
+  return fmin(dt, t);
+}
+
+template <typename P>
+__global__ void k_cfl_dt(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                         const P *__restrict__ vx, const P *__restrict__ vy, const P *__restrict__ vz,
+                         const P *__restrict__ ax, const P *__restrict__ ay, const P *__restrict__ az, int N,
+                         const int *__restrict__ head_child, const DNode *__restrict__ nodes,
+                         const int *__restrict__ box, int lmin, double cfl, double dt_max, double *__restrict__ part)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = dt_max;
+  if (p < N)
+  {
+    int lv;
+    finest_node((double)x[p], (double)y[p], (double)z[p], head_child, nodes, box, lmin, lv);
+    double H = 1.0 / (1 << lv);
+    dt = cfl_axis((double)vx[p], (double)ax[p], H, cfl, dt);
+    dt = cfl_axis((double)vy[p], (double)ay[p], H, cfl, dt);
+    dt = cfl_axis((double)vz[p], (double)az[p], H, cfl, dt);
+  }
+  // min through the max reduction of -dt
+  double r = -block_reduce<1>(-dt);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = r;
 }
 
 } // namespace qz

@@ -1,10 +1,10 @@
 // libqzhip.so: MI355X-native per-step particle-mesh gravity cycle.
 //
-// Host-side driver of the device kernels in qz_kernels.h / qz_branch.hip and
-// the native C ABI declared in include/qz.h.  One HIP stream per context; every
-// entry point returns after its device work has finished (the host tree code
-// of the reference reads results between stages, SURVEY-free restatement of
-// main.c:369-535).
+// Host-side driver of the device kernels in qz_kernels.h / qz_branch_kernels.h
+// and the native C ABI declared in include/qz.h.  One HIP stream per context;
+// every entry point returns after its device work has finished, because the
+// reference's host tree code reads results between the stages of its step
+// loop (main.c:369-535).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,14 +58,14 @@ void Timer::begin(hipStream_t s)
   pending_start = a;
 }
 
-void Timer::end(hipStream_t s, const char *name)
+void Timer::end(hipStream_t s, const char *name, double bytes)
 {
   if (!on)
     return;
   hipEvent_t b;
   (void)hipEventCreate(&b);
   (void)hipEventRecord(b, s);
-  recs.push_back({name, pending_start, b});
+  recs.push_back({name, pending_start, b, bytes});
 }
 
 void Timer::collect()
@@ -77,11 +77,12 @@ void Timer::collect()
     (void)hipEventElapsedTime(&ms, r.a, r.b);
     auto it = std::find_if(acc.begin(), acc.end(), [&](const Acc &x) { return x.name == r.name; });
     if (it == acc.end())
-      acc.push_back({r.name, (double)ms, 1});
+      acc.push_back({r.name, (double)ms, 1, r.bytes});
     else
     {
       it->ms += ms;
       it->count++;
+      it->bytes += r.bytes;
     }
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -89,12 +90,13 @@ void Timer::collect()
   recs.clear();
 }
 
-#define QZ_LAUNCH(name, kern, grid, ...)                                                                              \
+#define QZ_LAUNCH(name, kern, grid, ...) QZ_LAUNCHB(name, 0.0, kern, grid, __VA_ARGS__)
+#define QZ_LAUNCHB(name, bytes, kern, grid, ...)                                                                      \
   do                                                                                                                  \
   {                                                                                                                   \
     timer.begin(stream);                                                                                              \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__);                                       \
-    timer.end(stream, name);                                                                                          \
+    timer.end(stream, name, (double)(bytes));                                                                         \
   } while (0)
 
 // ------------------------------------------------------------------ engine
@@ -111,7 +113,8 @@ template <typename T> static T *dmalloc(size_t n, size_t &total)
 
 template <typename P, typename G> struct Engine final : EngineBase
 {
-  // particles
+  // particles: pos[0..2] FP64 positions; pa[3..9] = vx vy vz ax ay az m in P
+  double *pos[3] = {}, *posb[3] = {};
   P *pa[10] = {};
   P *pb[10] = {};
   int *id_a = nullptr, *id_b = nullptr;
@@ -136,7 +139,12 @@ template <typename P, typename G> struct Engine final : EngineBase
 
   void release()
   {
-    for (int a = 0; a < 10; a++)
+    for (int a = 0; a < 3; a++)
+    {
+      (void)hipFree(pos[a]);
+      (void)hipFree(posb[a]);
+    }
+    for (int a = 3; a < 10; a++)
     {
       (void)hipFree(pa[a]);
       (void)hipFree(pb[a]);
@@ -177,7 +185,14 @@ template <typename P, typename G> struct Engine final : EngineBase
     n = 1 << cfg.lmin;
     D = std::min(cfg.lmax - cfg.lmin, 7);
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
-    for (int a = 0; a < 10; a++)
+    for (int a = 0; a < 3; a++)
+    {
+      pos[a] = dmalloc<double>(N, bytes);
+      posb[a] = dmalloc<double>(N, bytes);
+      if (!pos[a] || !posb[a])
+        return hip_ok(hipErrorOutOfMemory, "particle arrays");
+    }
+    for (int a = 3; a < 10; a++)
     {
       pa[a] = dmalloc<P>(N, bytes);
       pb[a] = dmalloc<P>(N, bytes);
@@ -229,7 +244,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool set_particles(const double *const *src) override
   {
     std::vector<P> tmp(N);
-    for (int a = 0; a < 7; a++)
+    for (int a = 0; a < 3; a++)
+      if (!hip_ok(hipMemcpy(pos[a], src[a], sizeof(double) * N, hipMemcpyHostToDevice), "upload positions"))
+        return false;
+    for (int a = 3; a < 7; a++)
     {
       int dst = a < 6 ? a : 9;
       for (int i = 0; i < N; i++)
@@ -262,12 +280,21 @@ template <typename P, typename G> struct Engine final : EngineBase
     std::vector<int> ids(N);
     if (!hip_ok(hipMemcpy(ids.data(), id_a, sizeof(int) * N, hipMemcpyDeviceToHost), "download ids"))
       return false;
-    const int order[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
-    for (int q = 0; q < 9; q++)
+    std::vector<double> tx(N);
+    for (int q = 0; q < 3; q++)
     {
       if (!dst[q])
         continue;
-      if (!hip_ok(hipMemcpy(tmp.data(), pa[order[q]], sizeof(P) * N, hipMemcpyDeviceToHost), "download particles"))
+      if (!hip_ok(hipMemcpy(tx.data(), pos[q], sizeof(double) * N, hipMemcpyDeviceToHost), "download positions"))
+        return false;
+      for (int i = 0; i < N; i++)
+        dst[q][ids[i]] = tx[i];
+    }
+    for (int q = 3; q < 9; q++)
+    {
+      if (!dst[q])
+        continue;
+      if (!hip_ok(hipMemcpy(tmp.data(), pa[q], sizeof(P) * N, hipMemcpyDeviceToHost), "download particles"))
         return false;
       for (int i = 0; i < N; i++)
         dst[q][ids[i]] = (double)tmp[i];
@@ -278,15 +305,15 @@ template <typename P, typename G> struct Engine final : EngineBase
   // ---------------------------------------------------------------- stages
   bool update_A(double dt) override
   {
-    QZ_LAUNCH("update_A", k_update_A<P>, nblk(N), pa[0], pa[1], pa[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
-              (P)dt);
+    QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P)) * N, k_update_A<P>, nblk(N), pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
+              dt);
     keys_valid = false;
     return hip_ok(hipStreamSynchronize(stream), "particle_updating_A");
   }
 
   bool update_B(double dt) override
   {
-    QZ_LAUNCH("update_B", k_update_B<P>, nblk(N), pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, (P)dt);
+    QZ_LAUNCHB("update_B", 9.0 * sizeof(P) * N, k_update_B<P>, nblk(N), pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, (P)dt);
     return hip_ok(hipStreamSynchronize(stream), "particle_updating_B");
   }
 
@@ -304,25 +331,32 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     if (keys_valid)
       return true;
-    QZ_LAUNCH("keys", k_keys<P>, nblk(N), pa[0], pa[1], pa[2], key_in, slot_in, N, cfg.lmin, D);
+    QZ_LAUNCHB("keys", (3.0 * 8 + 12.0) * N, k_keys<double>, nblk(N), pos[0], pos[1], pos[2], key_in, slot_in, N, cfg.lmin, D);
     timer.begin(stream);
     if (!hip_ok(qz_sort64_pairs(sort_tmp, sort_tmp_bytes, key_in, key_out, slot_in, slot_out, N, end_bit, stream),
                 "radix sort"))
       return false;
-    timer.end(stream, "sort");
+    timer.end(stream, "sort", 24.0 * N);
     PtclArrays<P> src, dst;
-    for (int a = 0; a < 10; a++)
+    for (int a = 0; a < 3; a++)
     {
-      src.a[a] = pa[a];
-      dst.a[a] = pb[a];
+      src.x[a] = pos[a];
+      dst.x[a] = posb[a];
+    }
+    for (int a = 0; a < 7; a++)
+    {
+      src.a[a] = pa[3 + a];
+      dst.a[a] = pb[3 + a];
     }
     src.id = id_a;
     dst.id = id_b;
-    QZ_LAUNCH("permute", k_permute<P>, nblk(N), src, dst, slot_out, N);
-    for (int a = 0; a < 10; a++)
+    QZ_LAUNCHB("permute", (6.0 * 8 + 14.0 * sizeof(P) + 12.0) * N, k_permute<P>, nblk(N), src, dst, slot_out, N);
+    for (int a = 0; a < 3; a++)
+      std::swap(pos[a], posb[a]);
+    for (int a = 3; a < 10; a++)
       std::swap(pa[a], pb[a]);
     std::swap(id_a, id_b);
-    QZ_LAUNCH("cell_start", k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n, cell_start);
+    QZ_LAUNCHB("cell_start", 8.0 * N + 4.0 * n * n * n, k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n, cell_start);
     keys_valid = true;
     return true;
   }
@@ -334,7 +368,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
     double coeff = 4.0 * cfg.G * kPi / (H * H * H);
-    QZ_LAUNCH("deposit_head", (k_deposit_head<P, G>), nblk(npts), pa[0], pa[1], pa[2], pa[9], cell_start, rho, n,
+    QZ_LAUNCHB("deposit_head", (24.0 + sizeof(P)) * N + 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head<P, G>), nblk(npts), pos[0], pos[1], pos[2], pa[9], cell_start, rho, n,
               coeff);
     if (has_tree() && !branch_density())
       return false;
@@ -356,20 +390,20 @@ template <typename P, typename G> struct Engine final : EngineBase
     size_t nred = (size_t)(m / 2 + 1) * (m - 1) * (m - 1);
     for (int s = 0; s < cfg.pre_sweeps; s++)
     {
-      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
-      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
+      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
+      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
     }
     int mc = lev_n[l + 1];
     size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
-    QZ_LAUNCH("restrict", k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l], lev_r[l + 1], m,
+    QZ_LAUNCHB("restrict", (double)sizeof(G) * (2.0 * np + nc), k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l], lev_r[l + 1], m,
               (G)((double)m * m));
-    QZ_LAUNCH("fill", k_fill<G>, nblk(nc), lev_e[l + 1], (G)0, nc);
+    QZ_LAUNCHB("fill", (double)sizeof(G) * nc, k_fill<G>, nblk(nc), lev_e[l + 1], (G)0, nc);
     vcycle(l + 1);
-    QZ_LAUNCH("prolong", k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1], m);
+    QZ_LAUNCHB("prolong", (double)sizeof(G) * (2.0 * np + nc), k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1], m);
     for (int s = 0; s < cfg.post_sweeps; s++)
     {
-      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
-      QZ_LAUNCH("rbgs", k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
+      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
+      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
     }
   }
 
@@ -393,7 +427,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     double change = 1e300;
     for (int outer = 0; outer <= max_outer; outer++)
     {
-      QZ_LAUNCH("outer_residual", k_outer_residual<G>, nb, phi, rho, lev_r[0], n, inv_h2, part);
+      QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, phi, rho, lev_r[0], n, inv_h2, part);
       hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
       if (!hip_ok(hipMemcpyAsync(hred, red, 2 * sizeof(double), hipMemcpyDeviceToHost, stream), "residual readback") ||
           !hip_ok(hipStreamSynchronize(stream), "residual"))
@@ -408,11 +442,11 @@ template <typename P, typename G> struct Engine final : EngineBase
         return true;
       if (outer == max_outer)
         break;
-      QZ_LAUNCH("fill", k_fill<G>, nb, lev_e[0], (G)0, npts);
+      QZ_LAUNCHB("fill", (double)sizeof(G) * npts, k_fill<G>, nb, lev_e[0], (G)0, npts);
       for (int c = 0; c < inner; c++)
         vcycle(0);
       stats.head_vcycles += inner;
-      QZ_LAUNCH("outer_correct", k_outer_correct<G>, nb, phi, lev_e[0], n, cfg.tol_change, part);
+      QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, phi, lev_e[0], n, cfg.tol_change, part);
       hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
       stats.head_outer++;
     }
@@ -425,7 +459,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
-    QZ_LAUNCH("grid_acc_head", k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz, n, 1.0 / (2.0 * H));
+    QZ_LAUNCHB("grid_acc_head", (8.0 + 3.0 * sizeof(G)) * npts, k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz, n, 1.0 / (2.0 * H));
     if (has_tree() && !branch_grid_acceleration())
       return false;
     return hip_ok(hipStreamSynchronize(stream), "grid_acceleration");
@@ -435,6 +469,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     if (!sort_particles())
       return false;
+    const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     if (has_tree())
     {
       if (!branch_particle_acceleration())
@@ -442,7 +477,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
     else
     {
-      QZ_LAUNCH("gather_head", (k_gather_head<P, G>), nblk(N), pa[0], pa[1], pa[2], gax, gay, gaz, pa[6], pa[7],
+      QZ_LAUNCHB("gather_head", (24.0 + 3.0 * sizeof(P)) * N + 3.0 * sizeof(G) * npts, (k_gather_head<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, pa[6], pa[7],
                 pa[8], N, n);
     }
     return hip_ok(hipStreamSynchronize(stream), "particle_acceleration");
@@ -486,9 +521,9 @@ template <typename P, typename G> struct Engine final : EngineBase
       return false;
     double ek = 0.5 * hred[0], px = hred[1], py = hred[2], pz = hred[3];
     if (exact_pe)
-      QZ_LAUNCH("pairwise_pe", k_pairwise_pe<P>, nb, pa[0], pa[1], pa[2], pa[9], N, part);
+      QZ_LAUNCH("pairwise_pe", k_pairwise_pe<P>, nb, pos[0], pos[1], pos[2], pa[9], N, part);
     else
-      QZ_LAUNCH("grid_pe", k_grid_pe_head<P>, nb, pa[0], pa[1], pa[2], pa[9], phi, N, n, part);
+      QZ_LAUNCH("grid_pe", k_grid_pe_head<P>, nb, pos[0], pos[1], pos[2], pa[9], phi, N, n, part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
     if (!hip_ok(hipMemcpyAsync(hred, red, sizeof(double), hipMemcpyDeviceToHost, stream), "readback") ||
         !hip_ok(hipStreamSynchronize(stream), "observables"))
@@ -503,7 +538,9 @@ template <typename P, typename G> struct Engine final : EngineBase
     return true;
   }
 
-  // ---------------------------------------------------------------- branches (qz_branch.hip)
+  bool timestep(double cfl, double dt_max, double *dt) override;
+
+  // ---------------------------------------------------------------- branches (qz_branch.inl)
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
   bool potential_branches() override;
@@ -783,6 +820,12 @@ int qz_observables(qz_ctx *ctx, int exact_pe, double *out6)
   return ctx->eng->observables(exact_pe, out6) ? 0 : 1;
 }
 
+int qz_timestep(qz_ctx *ctx, double cfl, double dt_max, double *dt)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->timestep(cfl, dt_max, dt) ? 0 : 1;
+}
+
 int qz_keys(const double *x, const double *y, const double *z, int n, int lv, const int *ts, const int *rd, int *out)
 {
   if (n <= 0)
@@ -819,7 +862,7 @@ int qz_timing_enable(qz_ctx *ctx, int on)
   return 0;
 }
 
-int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, int max)
+int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, double *bytes, int max)
 {
   if (!ctx || !ctx->eng)
     return 0;
@@ -833,6 +876,8 @@ int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *la
     snprintf(names + (size_t)k * name_len, name_len, "%s", a.name.c_str());
     ms[k] = a.ms;
     launches[k] = a.count;
+    if (bytes)
+      bytes[k] = a.bytes;
     k++;
   }
   return k;

@@ -22,19 +22,21 @@ struct Timer
   {
     std::string name;
     hipEvent_t a, b;
+    double bytes;
   };
   struct Acc
   {
     std::string name;
     double ms;
     int count;
+    double bytes; // algorithmic bytes (each input read once, each output written once)
   };
   bool on = false;
   hipEvent_t pending_start{};
   std::vector<Rec> recs;
   std::vector<Acc> acc;
   void begin(hipStream_t s);
-  void end(hipStream_t s, const char *name);
+  void end(hipStream_t s, const char *name, double bytes = 0.0);
   void collect();
 };
 
@@ -78,6 +80,7 @@ struct EngineBase
   virtual bool head_fields(double *out5) = 0;
   virtual bool set_head_potential(const double *pot) = 0;
   virtual bool observables(int exact_pe, double *out6) = 0;
+  virtual bool timestep(double cfl, double dt_max, double *dt) = 0;
   virtual int num_nodes() = 0;
   virtual bool node_header(int i, int *hdr) = 0;
   virtual bool node_lists(int i, int *ixyz, int *bxyz, int *sxyz) = 0;

@@ -55,12 +55,19 @@ template <int OP> __device__ inline double block_reduce(double v)
 // Second stage: one block folds the per-block partials in a fixed order.
 template <int OP> __global__ void k_reduce_final(const double *__restrict__ part, int np, double *__restrict__ out)
 {
-  double v = OP == 0 ? 0.0 : 0.0;
+  double v = OP == 0 ? 0.0 : -1.0e308;
   for (int i = threadIdx.x; i < np; i += blockDim.x)
     v = OP == 0 ? v + part[i] : fmax(v, part[i]);
   v = block_reduce<OP>(v);
   if (threadIdx.x == 0)
     *out = v;
+}
+
+__global__ void k_negate(double *__restrict__ a, int n)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    a[i] = -a[i];
 }
 
 // ---------------------------------------------------------------- particles
@@ -97,9 +104,13 @@ __global__ void k_keys(const P *__restrict__ x, const P *__restrict__ y, const P
 }
 
 // Gather every per-particle array into the sorted order (out[i] = in[perm[i]]).
+// Positions are FP64 in every precision mode: the CIC weights of a level-lv
+// cell carry the position error times 2^lv, so FP32 positions would cost
+// 2^lv * 6e-8 relative density error at the refined levels.
 template <typename P> struct PtclArrays
 {
-  P *a[10]; // x y z vx vy vz ax ay az m
+  double *x[3];
+  P *a[7]; // vx vy vz ax ay az m
   int *id;
 };
 
@@ -111,7 +122,10 @@ __global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__res
     return;
   int s = perm[i];
 #pragma unroll
-  for (int a = 0; a < 10; a++)
+  for (int a = 0; a < 3; a++)
+    dst.x[a][i] = src.x[a][s];
+#pragma unroll
+  for (int a = 0; a < 7; a++)
     dst.a[a][i] = src.a[a][s];
   dst.id[i] = src.id[s];
 }
@@ -133,7 +147,7 @@ __global__ void k_cell_start(const unsigned long long *__restrict__ key, int N, 
 // point sums m * w over the particles of its 8 adjacent cells in slot order.
 // coeff = 4 pi G / H^3 (grid_density.c:62).  No atomics.
 template <typename P, typename G>
-__global__ void k_deposit_head(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+__global__ void k_deposit_head(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                                const P *__restrict__ m, const int *__restrict__ start, G *__restrict__ rho, int n,
                                double coeff)
 {
@@ -142,7 +156,7 @@ __global__ void k_deposit_head(const P *__restrict__ x, const P *__restrict__ y,
   if (t >= (size_t)n1 * n1 * n1)
     return;
   int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
-  const P s = (P)n;
+  const double s = (double)n;
   double acc = 0.0;
   for (int ck = k - 1; ck <= k; ck++)
   {
@@ -160,7 +174,7 @@ __global__ void k_deposit_head(const P *__restrict__ x, const P *__restrict__ y,
         int e = start[c + 1];
         for (int p = start[c]; p < e; p++)
         {
-          double px = (double)(x[p] * s), py = (double)(y[p] * s), pz = (double)(z[p] * s);
+          double px = x[p] * s, py = y[p] * s, pz = z[p] * s;
           double wx1 = px - ci, wy1 = py - cj, wz1 = pz - ck;
           double wx = ci == i ? 1.0 - wx1 : wx1;
           double wy = cj == j ? 1.0 - wy1 : wy1;
@@ -175,7 +189,7 @@ __global__ void k_deposit_head(const P *__restrict__ x, const P *__restrict__ y,
 
 // A12 head gather (particle_acceleration.c:33-132).
 template <typename P, typename G>
-__global__ void k_gather_head(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+__global__ void k_gather_head(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                               const G *__restrict__ gax, const G *__restrict__ gay, const G *__restrict__ gaz,
                               P *__restrict__ ax, P *__restrict__ ay, P *__restrict__ az, int N, int n)
 {
@@ -183,10 +197,10 @@ __global__ void k_gather_head(const P *__restrict__ x, const P *__restrict__ y, 
   if (p >= N)
     return;
   const int n1 = n + 1;
-  const P s = (P)n;
-  P pxs = x[p] * s, pys = y[p] * s, pzs = z[p] * s;
+  const double s = (double)n;
+  double pxs = x[p] * s, pys = y[p] * s, pzs = z[p] * s;
   int fx = min((int)pxs, n - 1), fy = min((int)pys, n - 1), fz = min((int)pzs, n - 1);
-  double wx1 = (double)pxs - fx, wy1 = (double)pys - fy, wz1 = (double)pzs - fz;
+  double wx1 = pxs - fx, wy1 = pys - fy, wz1 = pzs - fz;
   double wx2 = 1.0 - wx1, wy2 = 1.0 - wy1, wz2 = 1.0 - wz1;
   double w[8] = {wx2 * wy2 * wz2, wx1 * wy2 * wz2, wx2 * wy1 * wz2, wx1 * wy1 * wz2,
                  wx2 * wy2 * wz1, wx1 * wy2 * wz1, wx2 * wy1 * wz1, wx1 * wy1 * wz1};
@@ -212,25 +226,26 @@ __global__ void k_gather_head(const P *__restrict__ x, const P *__restrict__ y, 
 
 // A14 kick + drift + reflexive boundary (particle_updating.c:92-147).
 template <typename P>
-__global__ void k_update_A(P *__restrict__ x, P *__restrict__ y, P *__restrict__ z, P *__restrict__ vx,
+__global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, P *__restrict__ vx,
                            P *__restrict__ vy, P *__restrict__ vz, P *__restrict__ ax, P *__restrict__ ay,
-                           P *__restrict__ az, int N, P dt)
+                           P *__restrict__ az, int N, double dt)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N)
     return;
   const P h = (P)0.5;
-  P pos[3] = {x[i], y[i], z[i]};
+  const P dtp = (P)dt;
+  double pos[3] = {x[i], y[i], z[i]};
   P vel[3] = {vx[i], vy[i], vz[i]};
   P acc[3] = {ax[i], ay[i], az[i]};
 #pragma unroll
   for (int a = 0; a < 3; a++)
   {
-    vel[a] += acc[a] * dt * h;
-    pos[a] += vel[a] * dt;
-    if (pos[a] < (P)0 || pos[a] >= (P)1)
+    vel[a] += acc[a] * dtp * h;
+    pos[a] += (double)vel[a] * dt;
+    if (pos[a] < 0.0 || pos[a] >= 1.0)
     {
-      pos[a] = pos[a] >= (P)1 ? (P)2 - pos[a] : -pos[a];
+      pos[a] = pos[a] >= 1.0 ? 2.0 - pos[a] : -pos[a];
       vel[a] = -vel[a];
       acc[a] = -acc[a];
     }
@@ -290,7 +305,7 @@ __global__ void k_kin_mom(const P *__restrict__ vx, const P *__restrict__ vy, co
 
 // Exact pairwise potential energy (observables.c:77-112), FP64, tiled through LDS.
 template <typename P>
-__global__ void k_pairwise_pe(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+__global__ void k_pairwise_pe(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                               const P *__restrict__ m, int N, double *__restrict__ part)
 {
   __shared__ double tx[kBlock], ty[kBlock], tz[kBlock], tm[kBlock];
@@ -334,7 +349,7 @@ __global__ void k_pairwise_pe(const P *__restrict__ x, const P *__restrict__ y, 
 
 // Grid-interpolated potential energy 1/2 sum m phi(x) (observables.c:594-619), head only.
 template <typename P>
-__global__ void k_grid_pe_head(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+__global__ void k_grid_pe_head(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                                const P *__restrict__ m, const double *__restrict__ phi, int N, int n,
                                double *__restrict__ part)
 {
@@ -343,10 +358,10 @@ __global__ void k_grid_pe_head(const P *__restrict__ x, const P *__restrict__ y,
   if (p < N)
   {
     const int n1 = n + 1;
-    const P s = (P)n;
-    P pxs = x[p] * s, pys = y[p] * s, pzs = z[p] * s;
+    const double s = (double)n;
+    double pxs = x[p] * s, pys = y[p] * s, pzs = z[p] * s;
     int fx = min((int)pxs, n - 1), fy = min((int)pys, n - 1), fz = min((int)pzs, n - 1);
-    double wx1 = (double)pxs - fx, wy1 = (double)pys - fy, wz1 = (double)pzs - fz;
+    double wx1 = pxs - fx, wy1 = pys - fy, wz1 = pzs - fz;
     double wx[2] = {1.0 - wx1, wx1}, wy[2] = {1.0 - wy1, wy1}, wz[2] = {1.0 - wz1, wz1};
     for (int kk = 0; kk < 2; kk++)
       for (int jj = 0; jj < 2; jj++)

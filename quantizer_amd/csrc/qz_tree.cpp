@@ -248,8 +248,11 @@ int TreeHost::refine_head(const std::vector<int> &flagged)
   head_child.assign(cnt, 0);
   for (int z = 0; z < cnt; z++)
     head_child[z] = first + z;
-  lev_start.push_back(first);
-  lev_count.push_back(cnt);
+  if (cnt > 0)
+  {
+    lev_start.push_back(first);
+    lev_count.push_back(cnt);
+  }
   return cnt;
 }
 
