@@ -35,12 +35,15 @@ struct TreeDev
   int flag_cap = 0;
   double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
   long long part_cap = 0;
+  double *mpart = nullptr;
+  long long mpart_cap = 0;
   int *done_count = nullptr;
   int *hflag_count = nullptr; // pinned
   std::vector<DNode> hnodes;
   // per relative level (index 1..): node range and pool ranges
   std::vector<int> lfirst, lcount;
   std::vector<long long> lc0, lncell, lb0, lnb, lz0, lnz, lg0, lng, lpmax;
+  std::vector<int> lnbm; // x blocks of the node-mass reduction per level
   long long used_zone = 0, used_cell = 0, used_box = 0, used_pt = 0, used_grid = 0;
 };
 
@@ -69,7 +72,7 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
                     d->pn_i, d->pn_b, d->pn_s, d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
-                    d->done_count};
+                    d->done_count, d->mpart};
     for (void *q : ptrs)
       (void)hipFree(q);
     if (d->hflag_count)
@@ -193,6 +196,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
   d.lg0.assign(1, 0);
   d.lng.assign(1, 0);
   d.lpmax.assign(1, 0);
+  d.lnbm.assign(1, 0);
   d.used_zone = d.used_cell = d.used_box = d.used_pt = d.used_grid = 0;
   int lr = 1;
   while (lr < (int)th.lev_start.size())
@@ -286,8 +290,22 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
     QZ_LAUNCH("tree_boxes", k_init_boxes, nblk(nb), d.nodes, first, count, b0, nb, d.box, d.bstart, d.bend);
     QZ_LAUNCH("tree_cells", k_expand_cells, nblk(ncell), d.nodes, d.zone_node, d.zone_xyz, (int)c0, (int)ncell,
               cfg.lmin, D, key_out, N, d.cell_xyz, d.cell_box, d.cstart, d.cend, d.box, d.bstart, d.bend);
+    {
+      long long maxcell = 0;
+      for (int q = first; q < first + count; q++)
+        maxcell = std::max<long long>(maxcell, th.nodes[q].ncell);
+      int nbm = blocks_per_node(maxcell);
+      d.lnbm.push_back(nbm);
+      long long need = 2LL * count * nbm;
+      if (need > d.mpart_cap)
+      {
+        long long cap = next_cap(d.mpart_cap, need);
+        if (!regrow(d.mpart, cap, 0, "mass partials"))
+          return false;
+        d.mpart_cap = cap;
+      }
+    }
     timer.begin(stream);
-    hipLaunchKernelGGL(k_node_mass<P>, dim3(count), dim3(kBlock), 0, stream, d.nodes, first, d.cstart, d.cend, pa[9]);
     hipLaunchKernelGGL(k_classify, dim3(count), dim3(kClassBlock), 0, stream, d.nodes, first, d.box, d.intr_g,
                        d.intr_xyz, d.bdry_g, d.bdry_xyz, d.sim_g, d.sim_xyz, d.pn_i, d.pn_b, d.pn_s);
     timer.end(stream, "tree_classify");
@@ -367,12 +385,22 @@ template <typename P, typename G> bool Engine<P, G>::branch_density()
   for (size_t l = 1; l < d.lfirst.size(); l++)
   {
     QZ_LAUNCH("branch_fill", k_fill<G>, nblk(d.lng[l]), (G *)d.rho + d.lg0[l], (G)0, (size_t)d.lng[l]);
+    const int lv = cfg.lmin + (int)l;
+    QZ_LAUNCHB("deposit_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0], pos[1],
+               pos[2], pa[9], N, lv, 3 * (D - (int)l), (1 << lv) - 1, tails);
     dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
     timer.begin(stream);
-    hipLaunchKernelGGL((k_branch_deposit<P, G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
-                       d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz, d.bstart, d.bend, pos[0], pos[1], pos[2],
-                       pa[9], (G *)d.rho, 4.0 * cfg.G * kPi);
-    timer.end(stream, "deposit_branch");
+    hipLaunchKernelGGL((k_branch_deposit_fold<G, G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
+                       d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz, d.bstart, d.bend, tails, (G *)d.rho,
+                       4.0 * cfg.G * kPi);
+    timer.end(stream, "deposit_branch_fold");
+    // node masses for the CG threshold (poisson_error.c:240) from this level's runs
+    timer.begin(stream);
+    hipLaunchKernelGGL(k_node_mass<G>, dim3(d.lnbm[l], d.lcount[l]), dim3(kBlock), 0, stream, d.nodes, d.lfirst[l],
+                       d.cstart, d.cend, tails, d.mpart);
+    hipLaunchKernelGGL(k_node_mass_fold, dim3(d.lcount[l]), dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.mpart,
+                       d.lnbm[l]);
+    timer.end(stream, "node_mass");
   }
   return true;
 }
@@ -400,32 +428,59 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
                        d.cg_r, d.cg_p, d.part2);
     hipLaunchKernelGGL(k_cg_init_fold, dim3(count), dim3(kBlock), 0, stream, d.nodes, first, d.part2, nbx);
     timer.end(stream, "cg_init");
-    QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.done_count, 0, (size_t)1);
-    int max_it = cfg.max_iterations; // cycle_size = min(size, 5000) per node; see conjugate_gradient.c:377
+    QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.done_count, 0, (size_t)2);
+    int *it_base = d.done_count + 1;
+    // one batch = 5 CG iterations (conjugate_gradient.c:379-456), the last one
+    // with the convergence check (iter_between_check_potential_solution = 5)
+    auto batch = [&]() {
+      for (int il = 0; il < 5; il++)
+      {
+        timer.begin(stream);
+        hipLaunchKernelGGL(k_cg_ap, grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.cg_p, d.cg_Ap,
+                           d.part1);
+        hipLaunchKernelGGL(k_cg_update, grid, dim3(kBlock), 0, stream, d.nodes, first, it_base, il, d.intr_g, d.phi,
+                           d.phi_old, d.cg_r, d.cg_p, d.cg_Ap, d.part1, d.part2, d.part3, cfg.tol_change);
+        hipLaunchKernelGGL(k_cg_direction, grid, dim3(kBlock), 0, stream, d.nodes, first, it_base, il, d.intr_g,
+                           d.phi, d.phi_old, d.cg_r, d.cg_p, d.part2, d.part3, cfg.tol, cfg.G, d.done_count);
+        timer.end(stream, "branch_cg", 0.0);
+      }
+      hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 5);
+    };
+    hipGraphExec_t gexec = nullptr;
+    if (!timer.on)
+    {
+      hipGraph_t g = nullptr;
+      if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
+        return false;
+      batch();
+      if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
+          !hip_ok(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0), "graph instantiate"))
+        return false;
+      (void)hipGraphDestroy(g);
+    }
+    int max_it = cfg.max_iterations; // cycle_size = min(size, 5000) per node (conjugate_gradient.c:377)
     int it = 0;
     bool all_done = false;
-    for (; it < max_it; it++)
+    for (; it < max_it; it += 5)
     {
-      int check = ((it + 1) % 5 == 0) ? 1 : 0;
-      timer.begin(stream);
-      hipLaunchKernelGGL(k_cg_ap, grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.cg_p, d.cg_Ap, d.part1);
-      hipLaunchKernelGGL(k_cg_update, grid, dim3(kBlock), 0, stream, d.nodes, first, it, check, d.intr_g, d.phi,
-                         d.phi_old, d.cg_r, d.cg_p, d.cg_Ap, d.part1, d.part2, d.part3, cfg.tol_change);
-      hipLaunchKernelGGL(k_cg_direction, grid, dim3(kBlock), 0, stream, d.nodes, first, it, check, d.intr_g, d.phi,
-                         d.phi_old, d.cg_r, d.cg_p, d.part2, d.part3, cfg.tol, cfg.G, d.done_count);
-      timer.end(stream, "branch_cg");
-      if (check)
+      if (gexec)
       {
-        if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "cg") ||
-            !hip_ok(hipStreamSynchronize(stream), "cg"))
+        if (!hip_ok(hipGraphLaunch(gexec, stream), "graph launch"))
           return false;
-        if (*d.hflag_count >= count)
-        {
-          all_done = true;
-          break;
-        }
+      }
+      else
+        batch();
+      if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "cg") ||
+          !hip_ok(hipStreamSynchronize(stream), "cg"))
+        return false;
+      if (*d.hflag_count >= count)
+      {
+        all_done = true;
+        break;
       }
     }
+    if (gexec)
+      (void)hipGraphExecDestroy(gexec);
     if (!all_done)
     {
       set_error("conjugate_gradient did not converge on level %d after %d iterations", cfg.lmin + (int)l, it);
@@ -476,6 +531,11 @@ template <typename P, typename G> bool Engine<P, G>::node_header(int i, int *h)
     set_error("node index out of range");
     return false;
   }
+  // refresh the host mirror: counts and masses are filled on the device after the build
+  if (i == 0 && !hip_ok(hipMemcpy(tree->dev->hnodes.data(), tree->dev->nodes, sizeof(DNode) * tree->n_nodes,
+                                  hipMemcpyDeviceToHost),
+                        "node table"))
+    return false;
   const HNode &hn = tree->host->nodes[i];
   const DNode &dn = tree->dev->hnodes[i];
   int lvrel = hn.lv - cfg.lmin;

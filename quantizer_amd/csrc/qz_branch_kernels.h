@@ -157,19 +157,43 @@ __global__ void k_expand_cells(const DNode *__restrict__ nodes, const int *__res
   bend[d.boff + b] = e;
 }
 
-// Per-node particle count and mass (tree_construction.c:2020-2022), one block per node.
-template <typename P>
-__global__ void k_node_mass(DNode *__restrict__ nodes, int first, const int *__restrict__ cstart,
-                            const int *__restrict__ cend, const P *__restrict__ m)
+// Per-node particle count and mass (tree_construction.c:2020-2022): x blocks
+// of each node stride over its cells; a cell's mass is the sum of its 8 CIC
+// corner sums (the weights of a particle add up to 1), read from the deposit
+// runs of this level.  Per-block partials folded in order.
+template <typename S>
+__global__ void k_node_mass(const DNode *__restrict__ nodes, int first, const int *__restrict__ cstart,
+                            const int *__restrict__ cend, const S *__restrict__ tails, double *__restrict__ part)
 {
-  DNode &d = nodes[first + blockIdx.x];
+  const DNode &d = nodes[first + blockIdx.y];
   double mass = 0.0, cnt = 0.0;
-  for (int c = threadIdx.x; c < d.ncell; c += blockDim.x)
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < d.ncell; c += gridDim.x * blockDim.x)
   {
     int a = cstart[d.coff + c], e = cend[d.coff + c];
     cnt += (double)(e - a);
-    for (int p = a; p < e; p++)
-      mass += (double)m[p];
+    if (a < e)
+      for (int q = 0; q < 8; q++)
+        mass += cell_corner_sum(tails, a, e, q);
+  }
+  mass = block_reduce<0>(mass);
+  cnt = block_reduce<0>(cnt);
+  if (threadIdx.x == 0)
+  {
+    size_t row = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+    part[2 * row] = mass;
+    part[2 * row + 1] = cnt;
+  }
+}
+
+__global__ void k_node_mass_fold(DNode *__restrict__ nodes, int first, const double *__restrict__ part, int nb)
+{
+  DNode &d = nodes[first + blockIdx.x];
+  double mass = 0.0, cnt = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+  {
+    size_t row = (size_t)blockIdx.x * nb + b;
+    mass += part[2 * row];
+    cnt += part[2 * row + 1];
   }
   mass = block_reduce<0>(mass);
   cnt = block_reduce<0>(cnt);
@@ -555,7 +579,7 @@ __global__ void k_cg_ap(const DNode *__restrict__ nodes, int first, const int *_
 
 // phi += alpha p, r -= alpha Ap, partial r.r; on check iterations also the
 // pointwise change flag against phi_old (poisson_error.c:36-81) (:401-417)
-__global__ void k_cg_update(const DNode *__restrict__ nodes, int first, int it, int check,
+__global__ void k_cg_update(const DNode *__restrict__ nodes, int first, const int *__restrict__ it_base, int it_local,
                             const int *__restrict__ intr_g, double *__restrict__ phi,
                             const double *__restrict__ phi_old, double *__restrict__ r, const double *__restrict__ p,
                             const double *__restrict__ Ap, const double *__restrict__ part_pap,
@@ -564,6 +588,8 @@ __global__ void k_cg_update(const DNode *__restrict__ nodes, int first, int it, 
   const DNode &d = nodes[first + blockIdx.y];
   if (d.done)
     return;
+  const int it = *it_base + it_local;
+  const int check = (it + 1) % 5 == 0;
   const int nb = gridDim.x;
   double pap = fold_partials(part_pap, blockIdx.y, nb);
   double alpha = d.rs[it & 1] / pap;
@@ -596,7 +622,7 @@ __global__ void k_cg_update(const DNode *__restrict__ nodes, int first, int it, 
 
 // Convergence (conjugate_gradient.c:426-442 with conj_grad_error method 0,
 // poisson_error.c:236-288), pot_old refresh on failure, p = r + beta p (:447-455).
-__global__ void k_cg_direction(DNode *__restrict__ nodes, int first, int it, int check,
+__global__ void k_cg_direction(DNode *__restrict__ nodes, int first, const int *__restrict__ it_base, int it_local,
                                const int *__restrict__ intr_g, const double *__restrict__ phi,
                                double *__restrict__ phi_old, const double *__restrict__ r, double *__restrict__ p,
                                const double *__restrict__ part_rs, const double *__restrict__ part_viol,
@@ -605,6 +631,8 @@ __global__ void k_cg_direction(DNode *__restrict__ nodes, int first, int it, int
   DNode &d = nodes[first + blockIdx.y];
   if (d.done)
     return;
+  const int it = *it_base + it_local;
+  const int check = (it + 1) % 5 == 0;
   const int nb = gridDim.x;
   double rsnew = fold_partials(part_rs, blockIdx.y, nb);
   double rsold = d.rs[it & 1];
@@ -640,6 +668,12 @@ __global__ void k_cg_direction(DNode *__restrict__ nodes, int first, int it, int
   }
   if (blockIdx.x == 0 && threadIdx.x == 0)
     d.rs[(it + 1) & 1] = rsnew;
+}
+
+__global__ void k_add_int(int *__restrict__ a, int v)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *a += v;
 }
 
 // A11 branch accelerations (grid_acceleration.c:199-394).
@@ -748,6 +782,58 @@ __global__ void k_branch_deposit(const DNode *__restrict__ nodes, int first, con
             double wz = ck == gz ? 1.0 - wz1 : wz1;
             acc += (double)m[p] * (wx * wy * wz);
           }
+        }
+    rho[d.goff + gl] = (G)(coeff * acc);
+  }
+}
+
+// A2 branch deposit, stage 2 (grid_density.c:113-267): each listed grid point
+// of the level's nodes adds the corner sums of its 8 box cells.
+template <typename S, typename G>
+__global__ void k_branch_deposit_fold(const DNode *__restrict__ nodes, int first, const int *__restrict__ sim_g,
+                                      const int *__restrict__ sim_xyz, const int *__restrict__ bdry_g,
+                                      const int *__restrict__ bdry_xyz, const int *__restrict__ intr_g,
+                                      const int *__restrict__ intr_xyz, const int *__restrict__ bstart,
+                                      const int *__restrict__ bend, const S *__restrict__ tails, G *__restrict__ rho,
+                                      double G4pi)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const int tot = d.nsim + d.nbdry + d.nintr;
+  const double H = 1.0 / (double)(1 << d.lv);
+  const double coeff = G4pi / (H * H * H);
+  const int rX = d.rd[0], rXY = d.rd[0] * d.rd[1];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x)
+  {
+    int sl, gl;
+    const int *xyz;
+    if (t < d.nsim)
+    {
+      sl = d.poff + t;
+      gl = sim_g[sl];
+      xyz = sim_xyz;
+    }
+    else if (t < d.nsim + d.nbdry)
+    {
+      sl = d.poff + t - d.nsim;
+      gl = bdry_g[sl];
+      xyz = bdry_xyz;
+    }
+    else
+    {
+      sl = d.poff + t - d.nsim - d.nbdry;
+      gl = intr_g[sl];
+      xyz = intr_xyz;
+    }
+    int gx = xyz[3 * sl], gy = xyz[3 * sl + 1], gz = xyz[3 * sl + 2];
+    double acc = 0.0;
+    for (int ck = gz - 1; ck <= gz; ck++)
+      for (int cj = gy - 1; cj <= gy; cj++)
+        for (int ci = gx - 1; ci <= gx; ci++)
+        {
+          long long b = d.boff + (ci - d.ts[0]) + (long long)(cj - d.ts[1]) * rX + (long long)(ck - d.ts[2]) * rXY;
+          int a = bstart[b], e = bend[b];
+          if (a < e)
+            acc += cell_corner_sum(tails, a, e, (gx - ci) + 2 * (gy - cj) + 4 * (gz - ck));
         }
     rho[d.goff + gl] = (G)(coeff * acc);
   }

@@ -121,6 +121,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   unsigned long long *key_in = nullptr, *key_out = nullptr;
   int *slot_in = nullptr, *slot_out = nullptr;
   int *cell_start = nullptr;
+  G *tails = nullptr; // 8 corner sums per run tail (deposit stage 1)
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   // head grid
@@ -134,6 +135,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
   bool keys_valid = false;
+  hipGraphExec_t head_graph = nullptr; // one outer iteration of the head solve
+  int head_graph_inner = -1;
 
   ~Engine() override { release(); }
 
@@ -156,6 +159,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     (void)hipFree(slot_in);
     (void)hipFree(slot_out);
     (void)hipFree(cell_start);
+    (void)hipFree(tails);
     (void)hipFree(sort_tmp);
     (void)hipFree(rho);
     (void)hipFree(phi);
@@ -176,6 +180,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     (void)hipFree(red);
     if (hred)
       (void)hipHostFree(hred);
+    if (head_graph)
+      (void)hipGraphExecDestroy(head_graph);
     release_tree();
   }
 
@@ -206,15 +212,16 @@ template <typename P, typename G> struct Engine final : EngineBase
     slot_in = dmalloc<int>(N, bytes);
     slot_out = dmalloc<int>(N, bytes);
     cell_start = dmalloc<int>((size_t)n * n * n + 1, bytes);
+    tails = dmalloc<G>(8 * (size_t)N, bytes);
     end_bit = 3 * (cfg.lmin + D);
-    sort_tmp_bytes = qz_sort64_temp_bytes(N, end_bit);
+    sort_tmp_bytes = end_bit <= 32 ? qz_sort_temp_bytes(N, end_bit) : qz_sort64_temp_bytes(N, end_bit);
     sort_tmp = dmalloc<char>(sort_tmp_bytes, bytes);
     rho = dmalloc<G>(npts, bytes);
     phi = dmalloc<double>(npts, bytes);
     gax = dmalloc<G>(npts, bytes);
     gay = dmalloc<G>(npts, bytes);
     gaz = dmalloc<G>(npts, bytes);
-    if (!id_a || !id_b || !key_in || !key_out || !slot_in || !slot_out || !cell_start || !sort_tmp || !rho || !phi ||
+    if (!id_a || !id_b || !key_in || !key_out || !slot_in || !slot_out || !cell_start || !tails || !sort_tmp || !rho || !phi ||
         !gax || !gay || !gaz)
       return hip_ok(hipErrorOutOfMemory, "head arrays");
     // multigrid hierarchy: level 0 = head (n cells), halve down to 4 cells
@@ -231,6 +238,12 @@ template <typename P, typename G> struct Engine final : EngineBase
     {
       set_error("head grid too small for the multigrid hierarchy");
       return false;
+    }
+    for (size_t l = 0; l < lev_n.size(); l++)
+    {
+      size_t p1 = (size_t)lev_n[l] + 1, np = p1 * p1 * p1;
+      QZ_LAUNCH("fill", k_fill<G>, nblk(np), lev_e[l], (G)0, np);
+      QZ_LAUNCH("fill", k_fill<G>, nblk(np), lev_r[l], (G)0, np);
     }
     part_cap = std::max<size_t>(nblk(npts), 4 * (size_t)nblk(N)) + 16;
     part = dmalloc<double>(part_cap, bytes);
@@ -333,10 +346,20 @@ template <typename P, typename G> struct Engine final : EngineBase
       return true;
     QZ_LAUNCHB("keys", (3.0 * 8 + 12.0) * N, k_keys<double>, nblk(N), pos[0], pos[1], pos[2], key_in, slot_in, N, cfg.lmin, D);
     timer.begin(stream);
-    if (!hip_ok(qz_sort64_pairs(sort_tmp, sort_tmp_bytes, key_in, key_out, slot_in, slot_out, N, end_bit, stream),
-                "radix sort"))
+    if (end_bit <= 32)
+    {
+      // keys fit 32 bits: sort them as such (fewer bytes per pass), then widen
+      QZ_LAUNCH("keys", k_narrow_keys, nblk(N), key_in, (unsigned *)key_out, N);
+      if (!hip_ok(qz_sort_pairs(sort_tmp, sort_tmp_bytes, (const unsigned *)key_out, (unsigned *)key_in, slot_in,
+                                slot_out, N, end_bit, stream),
+                  "radix sort"))
+        return false;
+      QZ_LAUNCH("keys", k_widen_keys, nblk(N), (const unsigned *)key_in, key_out, N);
+    }
+    else if (!hip_ok(qz_sort64_pairs(sort_tmp, sort_tmp_bytes, key_in, key_out, slot_in, slot_out, N, end_bit, stream),
+                     "radix sort"))
       return false;
-    timer.end(stream, "sort", 24.0 * N);
+    timer.end(stream, "sort", (end_bit <= 32 ? 16.0 : 24.0) * N);
     PtclArrays<P> src, dst;
     for (int a = 0; a < 3; a++)
     {
@@ -368,14 +391,18 @@ template <typename P, typename G> struct Engine final : EngineBase
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
     double coeff = 4.0 * cfg.G * kPi / (H * H * H);
-    QZ_LAUNCHB("deposit_head", (24.0 + sizeof(P)) * N + 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head<P, G>), nblk(npts), pos[0], pos[1], pos[2], pa[9], cell_start, rho, n,
-              coeff);
+    QZ_LAUNCHB("deposit_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0], pos[1],
+               pos[2], pa[9], N, cfg.lmin, 3 * D, n - 1, tails);
+    QZ_LAUNCHB("deposit_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>), nblk(npts),
+               cell_start, tails, rho, n, coeff);
     if (has_tree() && !branch_density())
       return false;
     return hip_ok(hipStreamSynchronize(stream), "grid_density");
   }
 
-  void vcycle(size_t l)
+  // V-cycle on the error equation of level l; the correction starts from zero
+  // (the first red half sweep ignores the array), so no fill is needed.
+  void vcycle(size_t l, bool zero_start)
   {
     int m = lev_n[l];
     size_t p1 = (size_t)m + 1, np = p1 * p1 * p1;
@@ -387,31 +414,77 @@ template <typename P, typename G> struct Engine final : EngineBase
       timer.end(stream, "coarse_cg");
       return;
     }
-    size_t nred = (size_t)(m / 2 + 1) * (m - 1) * (m - 1);
-    for (int s = 0; s < cfg.pre_sweeps; s++)
+    const size_t nhalf = rbhalf(np);
+    for (int sw = 0; sw < cfg.pre_sweeps; sw++)
     {
-      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
-      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
+      int z = (zero_start && sw == 0) ? 1 : 0;
+      QZ_LAUNCHB("rbgs", (z ? 1.0 : 1.5) * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], m, h2, 0, z);
+      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], m, h2, 1, 0);
     }
     int mc = lev_n[l + 1];
     size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
-    QZ_LAUNCHB("restrict", (double)sizeof(G) * (2.0 * np + nc), k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l], lev_r[l + 1], m,
-              (G)((double)m * m));
-    QZ_LAUNCHB("fill", (double)sizeof(G) * nc, k_fill<G>, nblk(nc), lev_e[l + 1], (G)0, nc);
-    vcycle(l + 1);
-    QZ_LAUNCHB("prolong", (double)sizeof(G) * (2.0 * np + nc), k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1], m);
-    for (int s = 0; s < cfg.post_sweeps; s++)
+    QZ_LAUNCHB("restrict", (double)sizeof(G) * (2.0 * np + nc), k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l],
+               lev_r[l + 1], m, (G)((double)m * m));
+    vcycle(l + 1, true);
+    QZ_LAUNCHB("prolong", (double)sizeof(G) * (2.0 * np + nc), k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1],
+               m);
+    for (int sw = 0; sw < cfg.post_sweeps; sw++)
     {
-      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 0);
-      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nred), lev_e[l], lev_r[l], m, h2, 1);
+      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], m, h2, 0, 0);
+      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], m, h2, 1, 0);
     }
+  }
+
+  // One outer iteration of the head solve after the first residual: zero the
+  // correction, `inner` V-cycles in G, FP64 correction with the pointwise change
+  // test, then the new FP64 residual and its sum of squares.
+  void head_outer_body(int inner)
+  {
+    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    double H = 1.0 / n;
+    unsigned nb = nblk(npts);
+    for (int c = 0; c < inner; c++)
+      vcycle(0, c == 0);
+    QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, phi, lev_e[0], n, cfg.tol_change,
+               part);
+    hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
+    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, phi, rho, lev_r[0], n,
+               1.0 / (H * H), part);
+    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+  }
+
+  bool launch_head_body(int inner)
+  {
+    if (timer.on)
+    {
+      head_outer_body(inner); // eager launches so each kernel is timed
+      return true;
+    }
+    if (!head_graph || head_graph_inner != inner)
+    {
+      if (head_graph)
+      {
+        (void)hipGraphExecDestroy(head_graph);
+        head_graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
+        return false;
+      head_outer_body(inner);
+      if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
+          !hip_ok(hipGraphInstantiate(&head_graph, g, nullptr, nullptr, 0), "graph instantiate"))
+        return false;
+      (void)hipGraphDestroy(g);
+      head_graph_inner = inner;
+    }
+    return hip_ok(hipGraphLaunch(head_graph, stream), "graph launch");
   }
 
   // Head solve: FP64 outer residual/correction around inner V-cycles in G.
   // Convergence follows poisson_error.c:290-347 (method 0): the RMS residual
   // over interior points divided by 4 pi G rho_mean, and the pointwise change
-  // of phi over the last outer iteration (= the last `inner` V-cycles, the
-  // reference compares with phi from iter_between_check = 5 cycles earlier).
+  // of phi over the last outer iteration (the reference compares with phi from
+  // iter_between_check_potential_solution = 5 V-cycles earlier).
   bool potential_head() override
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
@@ -425,10 +498,11 @@ template <typename P, typename G> struct Engine final : EngineBase
     stats.head_outer = 0;
     stats.head_vcycles = 0;
     double change = 1e300;
-    for (int outer = 0; outer <= max_outer; outer++)
+    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, phi, rho, lev_r[0], n,
+               inv_h2, part);
+    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+    for (int outer = 0;; outer++)
     {
-      QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, phi, rho, lev_r[0], n, inv_h2, part);
-      hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
       if (!hip_ok(hipMemcpyAsync(hred, red, 2 * sizeof(double), hipMemcpyDeviceToHost, stream), "residual readback") ||
           !hip_ok(hipStreamSynchronize(stream), "residual"))
         return false;
@@ -442,12 +516,9 @@ template <typename P, typename G> struct Engine final : EngineBase
         return true;
       if (outer == max_outer)
         break;
-      QZ_LAUNCHB("fill", (double)sizeof(G) * npts, k_fill<G>, nb, lev_e[0], (G)0, npts);
-      for (int c = 0; c < inner; c++)
-        vcycle(0);
+      if (!launch_head_body(inner))
+        return false;
       stats.head_vcycles += inner;
-      QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, phi, lev_e[0], n, cfg.tol_change, part);
-      hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
       stats.head_outer++;
     }
     set_error("The precision was not reached in the parent node after %d V-cycles (residual %.3e, change %.3e)",
@@ -629,7 +700,7 @@ qz_ctx *qz_create(const qz_config *cfg)
   }
   qz_config c = *cfg;
   if (c.inner_cycles <= 0)
-    c.inner_cycles = 5;
+    c.inner_cycles = 1;
   if (c.pre_sweeps <= 0)
     c.pre_sweeps = 2;
   if (c.post_sweeps <= 0)

@@ -18,6 +18,14 @@ constexpr int kBlock = 256;
 
 __host__ __device__ inline size_t pidx(int i, int j, int k, int n1) { return (size_t)i + (size_t)n1 * ((size_t)j + (size_t)n1 * k); }
 
+// Colour-split layout of the multigrid level arrays.  Every level has an odd
+// number of points per side (2^k + 1), so the parity of the linear index
+// i + n1 (j + n1 k) equals the red/black colour (i + j + k) mod 2: red points
+// (even index) are stored first, black points after them.  All six stencil
+// neighbours of a point have the other colour.
+__host__ __device__ inline size_t rbpos(size_t idx, size_t half) { return (idx & 1) * half + (idx >> 1); }
+__host__ __device__ inline size_t rbhalf(size_t npts) { return (npts + 1) >> 1; }
+
 // ---------------------------------------------------------------- generic
 template <typename T> __global__ void k_fill(T *__restrict__ a, T v, size_t n)
 {
@@ -103,6 +111,20 @@ __global__ void k_keys(const P *__restrict__ x, const P *__restrict__ y, const P
   slot[i] = i;
 }
 
+__global__ void k_narrow_keys(const unsigned long long *__restrict__ in, unsigned *__restrict__ out, int N)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N)
+    out[i] = (unsigned)in[i];
+}
+
+__global__ void k_widen_keys(const unsigned *__restrict__ in, unsigned long long *__restrict__ out, int N)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N)
+    out[i] = in[i];
+}
+
 // Gather every per-particle array into the sorted order (out[i] = in[perm[i]]).
 // Positions are FP64 in every precision mode: the CIC weights of a level-lv
 // cell carry the position error times 2^lv, so FP32 positions would cost
@@ -181,6 +203,133 @@ __global__ void k_deposit_head(const double *__restrict__ x, const double *__res
           double wz = ck == k ? 1.0 - wz1 : wz1;
           acc += (double)m[p] * (wx * wy * wz);
         }
+      }
+    }
+  }
+  rho[t] = (G)(coeff * acc);
+}
+
+// A2 deposit, stage 1 (grid_density.c:35-267): every particle computes its 8
+// CIC corner weights times its mass at level lv; one wavefront (64 consecutive
+// sorted slots) then runs a segmented inclusive scan over runs of equal cell
+// (key >> seg_shift).  The last lane of each run stores the run's 8 corner sums
+// at its slot.  A cell's total is the ordered sum of its runs' tails, one per
+// 64-slot window it overlaps.  Deterministic: fixed shuffle order, no atomics.
+__device__ inline unsigned long long shfl_up_u64(unsigned long long v, int d)
+{
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo = __shfl_up(lo, d, 64);
+  hi = __shfl_up(hi, d, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ inline unsigned long long shfl_down_u64(unsigned long long v, int d)
+{
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo = __shfl_down(lo, d, 64);
+  hi = __shfl_down(hi, d, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+template <typename P, typename S>
+__global__ void k_deposit_runs(const unsigned long long *__restrict__ key, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z, const P *__restrict__ m,
+                               int N, int lv, int seg_shift, int clamp_top, S *__restrict__ tails)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = p < N;
+  unsigned long long c = live ? (key[p] >> seg_shift) : ~0ull;
+  double v[8];
+  if (live)
+  {
+    const double s = (double)(1 << lv);
+    double px = x[p] * s, py = y[p] * s, pz = z[p] * s;
+    int fx = min((int)px, clamp_top), fy = min((int)py, clamp_top), fz = min((int)pz, clamp_top);
+    double wx1 = px - fx, wy1 = py - fy, wz1 = pz - fz;
+    double wx2 = 1.0 - wx1, wy2 = 1.0 - wy1, wz2 = 1.0 - wz1;
+    double mm = (double)m[p];
+    v[0] = mm * (wx2 * wy2 * wz2);
+    v[1] = mm * (wx1 * wy2 * wz2);
+    v[2] = mm * (wx2 * wy1 * wz2);
+    v[3] = mm * (wx1 * wy1 * wz2);
+    v[4] = mm * (wx2 * wy2 * wz1);
+    v[5] = mm * (wx1 * wy2 * wz1);
+    v[6] = mm * (wx2 * wy1 * wz1);
+    v[7] = mm * (wx1 * wy1 * wz1);
+  }
+  else
+  {
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      v[q] = 0.0;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1)
+  {
+    unsigned long long co = shfl_up_u64(c, d);
+    bool same = lane >= d && co == c;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+    {
+      double o = __shfl_up(v[q], d, 64);
+      if (same)
+        v[q] += o;
+    }
+  }
+  unsigned long long cn = shfl_down_u64(c, 1);
+  bool tail = live && (lane == 63 || p == N - 1 || cn != c);
+  if (tail)
+  {
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      tails[8 * (size_t)p + q] = (S)v[q];
+  }
+}
+
+// Sum of corner q of the cell whose sorted slot range is [a, e): its runs end at
+// min(e, next 64-slot boundary) - 1.
+template <typename S> __device__ inline double cell_corner_sum(const S *__restrict__ tails, int a, int e, int q)
+{
+  double acc = 0.0;
+  int t = a;
+  while (t < e)
+  {
+    int tl = min(e, ((t >> 6) + 1) << 6) - 1;
+    acc += (double)tails[8 * (size_t)tl + q];
+    t = tl + 1;
+  }
+  return acc;
+}
+
+// A2 head deposit, stage 2: each grid point adds corner (i-ci, j-cj, k-ck) of
+// its 8 adjacent cells; coeff = 4 pi G / H^3 (grid_density.c:62).
+template <typename S, typename G>
+__global__ void k_deposit_head_fold(const int *__restrict__ start, const S *__restrict__ tails, G *__restrict__ rho,
+                                    int n, double coeff)
+{
+  const int n1 = n + 1;
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n1 * n1 * n1)
+    return;
+  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  double acc = 0.0;
+  for (int ck = k - 1; ck <= k; ck++)
+  {
+    if (ck < 0 || ck >= n)
+      continue;
+    for (int cj = j - 1; cj <= j; cj++)
+    {
+      if (cj < 0 || cj >= n)
+        continue;
+      for (int ci = i - 1; ci <= i; ci++)
+      {
+        if (ci < 0 || ci >= n)
+          continue;
+        int c = ci + n * (cj + n * ck);
+        int a = start[c], e = start[c + 1];
+        if (a < e)
+          acc += cell_corner_sum(tails, a, e, (i - ci) + 2 * (j - cj) + 4 * (k - ck));
       }
     }
   }
@@ -425,7 +574,7 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
                                        phi[t + s2] - phi[t - s2]);
       sq = res * res;
     }
-    r[t] = (G)res;
+    r[rbpos(t, rbhalf((size_t)n1 * n1 * n1))] = (G)res;
   }
   sq = block_reduce<0>(sq);
   if (threadIdx.x == 0)
@@ -448,7 +597,7 @@ __global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ 
     if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
     {
       double old = phi[t];
-      double nw = old + (double)e[t];
+      double nw = old + (double)e[rbpos(t, rbhalf((size_t)n1 * n1 * n1))];
       phi[t] = nw;
       double ratio = nw / old;
       dev = (1.0 - tol > ratio || 1.0 + tol < ratio) ? 2.0 + fabs(ratio - 1.0) : fabs(ratio - 1.0);
@@ -462,27 +611,34 @@ __global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ 
 }
 
 // Red-black Gauss-Seidel half sweep on the error equation Lap e = r
-// (gauss_saidel.c:40-53 restated with red-black ordering, omega = 1).
-// Thread layout: one thread per point of the given colour in a row pair.
+// (gauss_saidel.c:40-53 restated with red-black ordering, omega = 1), on the
+// colour-split layout: one thread per point of the colour, neighbours read
+// from the other colour's half.
 template <typename G>
-__global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, int n, G h2, int color)
+__global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, int n, G h2, int color, int zero_start)
 {
   const int n1 = n + 1;
-  const int half = n / 2 + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t tot = (size_t)half * (n - 1) * (n - 1);
-  if (t >= tot)
+  const size_t npts = (size_t)n1 * n1 * n1;
+  const size_t half = rbhalf(npts);
+  const size_t count = color == 0 ? half : npts - half;
+  size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= count)
     return;
-  int ih = (int)(t % half);
-  int j = 1 + (int)((t / half) % (n - 1));
-  int k = 1 + (int)(t / ((size_t)half * (n - 1)));
-  int i = 2 * ih + ((j + k + color) & 1);
-  if (i < 1 || i > n - 1)
+  size_t idx = 2 * h + color;
+  int i = (int)(idx % n1);
+  size_t jk = idx / n1;
+  int j = (int)(jk % n1), k = (int)(jk / n1);
+  if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
     return;
-  size_t g = pidx(i, j, k, n1);
-  size_t s1 = n1, s2 = (size_t)n1 * n1;
-  G sum = e[g + 1] + e[g - 1] + e[g + s1] + e[g - s1] + e[g + s2] + e[g - s2];
-  e[g] = (sum - h2 * r[g]) * (G)(1.0 / 6.0);
+  const size_t s1 = n1, s2 = (size_t)n1 * n1;
+  const G *o = e + (color == 0 ? half : 0); // the other colour
+  // zero_start: first half sweep from a zero initial guess (the other colour
+  // counts as 0 whatever the array holds), which replaces a separate fill
+  G sum = zero_start ? (G)0
+                     : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + s1) >> 1] + o[(idx - s1) >> 1] +
+                           o[(idx + s2) >> 1] + o[(idx - s2) >> 1];
+  size_t me = (size_t)color * half + h;
+  e[me] = (sum - h2 * r[me]) * (G)(1.0 / 6.0);
 }
 
 // Residual of the fine error equation, full-weighting restricted to the coarse
@@ -496,12 +652,14 @@ __global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict
   if (t >= (size_t)nc1 * nc1 * nc1)
     return;
   int I = (int)(t % nc1), J = (int)((t / nc1) % nc1), K = (int)(t / ((size_t)nc1 * nc1));
+  const size_t hc = rbhalf((size_t)nc1 * nc1 * nc1);
   if (I == 0 || I == nc || J == 0 || J == nc || K == 0 || K == nc)
   {
-    rc[t] = (G)0;
+    rc[rbpos(t, hc)] = (G)0;
     return;
   }
   const size_t s1 = n1, s2 = (size_t)n1 * n1;
+  const size_t hf = rbhalf((size_t)n1 * n1 * n1);
   G acc = (G)0;
   for (int dk = -1; dk <= 1; dk++)
     for (int dj = -1; dj <= 1; dj++)
@@ -511,12 +669,14 @@ __global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict
         if (i <= 0 || i >= n || j <= 0 || j >= n || k <= 0 || k >= n)
           continue;
         size_t g = pidx(i, j, k, n1);
-        G lap = (e[g + 1] + e[g - 1] + e[g + s1] + e[g - s1] + e[g + s2] + e[g - s2] - (G)6 * e[g]) * inv_h2;
-        G res = r[g] - lap;
+        G lap = (e[rbpos(g + 1, hf)] + e[rbpos(g - 1, hf)] + e[rbpos(g + s1, hf)] + e[rbpos(g - s1, hf)] +
+                 e[rbpos(g + s2, hf)] + e[rbpos(g - s2, hf)] - (G)6 * e[rbpos(g, hf)]) *
+                inv_h2;
+        G res = r[rbpos(g, hf)] - lap;
         G w = (G)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1)) * (G)(1.0 / 64.0);
         acc += w * res;
       }
-  rc[t] = acc;
+  rc[rbpos(t, hc)] = acc;
 }
 
 // Trilinear prolongation of the coarse correction into the fine interior
@@ -533,11 +693,13 @@ __global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, i
     return;
   int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
   int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
-  G v = (ec[pidx(i0, j0, k0, nc1)] + ec[pidx(i1, j0, k0, nc1)] + ec[pidx(i0, j1, k0, nc1)] +
-         ec[pidx(i1, j1, k0, nc1)] + ec[pidx(i0, j0, k1, nc1)] + ec[pidx(i1, j0, k1, nc1)] +
-         ec[pidx(i0, j1, k1, nc1)] + ec[pidx(i1, j1, k1, nc1)]) *
+  const size_t hc = rbhalf((size_t)nc1 * nc1 * nc1), hf = rbhalf((size_t)n1 * n1 * n1);
+  G v = (ec[rbpos(pidx(i0, j0, k0, nc1), hc)] + ec[rbpos(pidx(i1, j0, k0, nc1), hc)] +
+         ec[rbpos(pidx(i0, j1, k0, nc1), hc)] + ec[rbpos(pidx(i1, j1, k0, nc1), hc)] +
+         ec[rbpos(pidx(i0, j0, k1, nc1), hc)] + ec[rbpos(pidx(i1, j0, k1, nc1), hc)] +
+         ec[rbpos(pidx(i0, j1, k1, nc1), hc)] + ec[rbpos(pidx(i1, j1, k1, nc1), hc)]) *
         (G)0.125;
-  e[t] += v;
+  e[rbpos(t, hf)] += v;
 }
 
 // Coarsest level: conjugate gradients in one workgroup (FP64 internally) on a
@@ -554,7 +716,7 @@ template <typename G> __global__ void k_coarse_cg(G *__restrict__ e, const G *__
     bool in = i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
     xs[t] = 0.0;
     // A = -Lap (SPD); solve A x = -r
-    rs[t] = in ? -(double)r[t] : 0.0;
+    rs[t] = in ? -(double)r[rbpos(t, rbhalf(np))] : 0.0;
     ps[t] = rs[t];
   }
   __syncthreads();
@@ -594,7 +756,7 @@ template <typename G> __global__ void k_coarse_cg(G *__restrict__ e, const G *__
     __syncthreads();
   }
   for (int t = threadIdx.x; t < np; t += blockDim.x)
-    e[t] = (G)xs[t];
+    e[rbpos(t, rbhalf(np))] = (G)xs[t];
 }
 
 // A11 head gradient on interior points (grid_acceleration.c:36-76); the faces

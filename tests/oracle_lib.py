@@ -127,6 +127,10 @@ class OracleSim:
         a = np.ascontiguousarray(arr9, dtype=np.float64).reshape(-1)
         lib().oq_set_particles(self.h, _d(a))
 
+    def set_head_pot(self, pot):
+        a = np.ascontiguousarray(pot, dtype=np.float64).reshape(-1)
+        lib().oq_set_head_pot(self.h, _d(a))
+
     def head(self):
         n1 = (1 << self.lmin) + 1
         out = np.empty(5 * n1 ** 3)

@@ -111,3 +111,24 @@ def test_amr_steps_fp64(oracle):
     assert np.max(np.abs(pg[:3] - po[:3])) < 1e-10
     for k in range(3, 9):
         assert rel(pg[k], po[k]) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [core.PREC_FP64, core.PREC_MIXED])
+def test_graph_replays_match_eager_launches(prec):
+    """The head solve and branch CG replay captured HIP graphs; with timing on the
+    same kernels are launched eagerly.  Several steps (many replays, with host
+    work and syncs between them) must agree bit for bit."""
+    print(f"case: graph vs eager prec={prec} clumps N=20000 lmin=5 lmax=7", flush=True)
+    p = clumps(20000, 5, n_clumps=4, seed=3)
+    out = []
+    for eager in (False, True):
+        with core.Simulation(p, 5, 7, precision=prec, ref_criterion_ptcl=8) as s:
+            s.timing(eager)
+            s.force()
+            dt = s.timestep()
+            for _ in range(4):
+                s.step(dt)
+            out.append((s.particles(), s.head_fields()))
+            s.timing(False)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])

@@ -1,0 +1,193 @@
+"""Pin the oracle (CPU restatement) to the compiled reference's golden vectors and
+to closed forms.  CPU only."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from quantizer_amd.workloads import Particles
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+HDR = ["lv", "lvrel", "idx", "parent", "ts_x", "ts_y", "ts_z", "rd_x", "rd_y", "rd_z", "min_x", "min_y", "min_z",
+       "max_x", "max_y", "max_z", "ncell", "nintr", "nbdry", "nsim", "nptcl_full", "nptcl_outs", "nchild", "gcap"]
+
+
+def parts_from(arr):
+    return Particles(*[arr[i].copy() for i in range(7)])
+
+
+def listed(node, kind):
+    xyz = node[kind + "_xyz"]
+    X = node["rd_x"] + 1
+    XY = X * (node["rd_y"] + 1)
+    return (xyz[:, 0] - node["ts_x"]) + (xyz[:, 1] - node["ts_y"]) * X + (xyz[:, 2] - node["ts_z"]) * XY
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_matches_reference_force_pass(oracle, path):
+    g = np.load(path)
+    n, lmin, lmax, crit, nsteps = [int(v) for v in g["meta"]]
+    o = oracle.OracleSim(parts_from(g["input"]), lmin, lmax, crit)
+    o.force()
+    # the restatement reproduces the reference bit for bit on the first force pass
+    np.testing.assert_array_equal(o.head(), g["head0"])
+    np.testing.assert_array_equal(o.particles()[6:9], g["acc0"])
+    nodes = o.nodes()
+    assert len(nodes) == int(g["n_nodes"][0])
+    for i, nd in enumerate(nodes):
+        assert [nd[k] for k in HDR] == [int(v) for v in g[f"node{i}_hdr"]]
+        np.testing.assert_array_equal(nd["cell_xyz"], g[f"node{i}_cell_xyz"])
+        for k in ("intr", "bdry", "sim"):
+            np.testing.assert_array_equal(nd[k + "_xyz"], g[f"node{i}_{k}_xyz"])
+            idx = listed(nd, k)
+            got = np.stack([nd[f][idx] for f in ("rho", "pot", "ax", "ay", "az")])
+            np.testing.assert_array_equal(got, g[f"node{i}_{k}_fields"])
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_matches_reference_steps(oracle, path):
+    g = np.load(path)
+    n, lmin, lmax, crit, nsteps = [int(v) for v in g["meta"]]
+    dt = float(g["dt"][0])
+    o = oracle.OracleSim(parts_from(g["input"]), lmin, lmax, crit)
+    o.force()
+    e0 = o.observables()
+    np.testing.assert_allclose(e0, g["energies"][0], rtol=1e-12, atol=1e-15)
+    for _ in range(nsteps):
+        o.step(dt)
+    fin = o.particles()
+    # the reference adapts its tree incrementally, the oracle rebuilds it; the
+    # particle-list order (hence FP summation order) differs after step 0
+    np.testing.assert_allclose(fin[:3], g["final"][:3], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(fin[3:], g["final"][3:], rtol=1e-9, atol=1e-9)
+
+
+def test_key_closed_form(oracle):
+    """A1: floor rule x<0 -> -1 else trunc(x 2^lv), minus box_ts, linearised."""
+    rng = np.random.default_rng(0)
+    lv = 6
+    ts = np.array([-1, 3, -5], dtype=np.int32)
+    rd = np.array([40, 37, 71], dtype=np.int32)
+    s = 2.0 ** lv
+    xs = list(rng.uniform(-0.05, 1.0, 200))
+    xs += [k / s for k in range(0, 64)]                       # exact dyadic points
+    xs += [np.nextafter(k / s, -1.0) for k in range(1, 64)]   # 1 ulp below a face
+    xs += [-1e-300, -0.0, 0.0, np.nextafter(1.0, 0.0)]
+    for x in xs:
+        for (y, z) in ((0.3, 0.7), (x, 0.5), (0.9, x)):
+            f = [(-1 if v * s < 0 else int(v * s)) for v in (x, y, z)]
+            want = (f[0] - ts[0]) + (f[1] - ts[1]) * rd[0] + (f[2] - ts[2]) * rd[0] * rd[1]
+            assert oracle.oracle_key(lv, ts, rd, x, y, z) == want
+
+
+def test_deposit_conserves_mass(oracle):
+    from quantizer_amd.workloads import plummer
+    p = plummer(3000, seed=4)
+    o = oracle.OracleSim(p, 5, 5)
+    o.force()
+    rho = o.head()[0]
+    H = 1.0 / 32
+    assert abs(rho.sum() * H ** 3 / (4 * np.pi) - p.m.sum()) < 1e-12
+
+
+def test_head_solver_discrete_eigenmode(oracle):
+    """phi = sin(pi x) sin(pi y) sin(pi z) on the grid is an exact eigenvector of
+    the 7-point operator with zero faces: Lap_h phi = -lambda phi."""
+    lmin = 5
+    n = 1 << lmin
+    h = 1.0 / n
+    x = np.arange(n + 1) * h
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    ex = np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    lam = (2.0 / h ** 2) * 3 * (1 - np.cos(np.pi * h))
+    rho = -lam * ex
+    phi0 = np.zeros_like(ex)
+    # packed cube is indexed [k][j][i] with i fastest: transpose to (z, y, x)
+    phi, it = oracle.oracle_head_solve(lmin, phi0.transpose(2, 1, 0), rho.transpose(2, 1, 0),
+                                       node_mass=lam / (4 * np.pi), tol=1e-10)
+    assert it > 0
+    assert np.max(np.abs(phi - ex.transpose(2, 1, 0))) < 1e-9
+
+
+def test_grid_gradient_exact_on_quadratic(oracle):
+    from quantizer_amd.workloads import plummer
+    p = plummer(500, seed=1)
+    o = oracle.OracleSim(p, 4, 4)
+    n1 = 17
+    h = 1.0 / 16
+    c = np.arange(n1) * h
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    o.set_head_pot((X ** 2 + 2 * Y ** 2 - Z ** 2).ravel())
+    oracle.lib().oq_grid_acceleration(o.h)
+    hd = o.head()
+    sl = (slice(1, -1),) * 3
+    np.testing.assert_allclose(hd[2][sl], -2 * X[sl], atol=1e-12)
+    np.testing.assert_allclose(hd[3][sl], -4 * Y[sl], atol=1e-12)
+    np.testing.assert_allclose(hd[4][sl], 2 * Z[sl], atol=1e-12)
+
+
+def test_gather_exact_on_linear_field(oracle):
+    """CIC interpolation reproduces a linear acceleration field exactly."""
+    rng = np.random.default_rng(2)
+    n = 400
+    pos = rng.uniform(0.2, 0.8, size=(3, n))
+    z0 = np.zeros(n)
+    p = Particles(pos[0], pos[1], pos[2], z0, z0, z0, np.full(n, 1.0 / n))
+    o = oracle.OracleSim(p, 4, 4)
+    h = 1.0 / 16
+    c = np.arange(17) * h
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    o.set_head_pot((X ** 2 + 2 * Y ** 2 - Z ** 2).ravel())
+    lib = oracle.lib()
+    lib.oq_reset(o.h)
+    lib.oq_grid_acceleration(o.h)
+    lib.oq_particle_acceleration(o.h)
+    acc = o.particles()[6:9]
+    np.testing.assert_allclose(acc[0], -2 * pos[0], atol=1e-12)
+    np.testing.assert_allclose(acc[1], -4 * pos[1], atol=1e-12)
+    np.testing.assert_allclose(acc[2], 2 * pos[2], atol=1e-12)
+
+
+def test_kick_drift_closed_form_and_reflection(oracle):
+    x = np.array([0.5, 0.999, 0.001])
+    v = np.array([0.1, 0.5, -0.5])
+    a = np.array([2.0, 0.0, 0.0])
+    n = 3
+    p = Particles(x.copy(), x.copy(), x.copy(), v.copy(), v.copy(), v.copy(), np.full(n, 1.0))
+    o = oracle.OracleSim(p, 3, 3)
+    st = o.particles()
+    st[6:9] = a
+    o.set_particles(st)
+    dt = 0.01
+    oracle.lib().oq_update_A(o.h, dt)
+    out = o.particles()
+    v1 = v + a * dt * 0.5
+    x1 = x + v1 * dt
+    # reflexive faces: x -> -x or 2 - x, v and a flip (particle_updating.c:92-147)
+    ref = x1 >= 1.0
+    x1 = np.where(ref, 2.0 - x1, np.where(x1 < 0, -x1, x1))
+    flip = (x + v1 * dt >= 1.0) | (x + v1 * dt < 0)
+    np.testing.assert_allclose(out[0], x1, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(out[3], np.where(flip, -v1, v1), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(out[6], np.where(flip, -a, a), rtol=0, atol=1e-15)
+    oracle.lib().oq_update_B(o.h, dt)
+    out2 = o.particles()
+    np.testing.assert_allclose(out2[3], out[3] + out[6] * dt * 0.5, rtol=0, atol=1e-15)
+
+
+def test_observables_exact_sums(oracle):
+    x = np.array([0.2, 0.5, 0.7])
+    y = np.array([0.5, 0.5, 0.4])
+    z = np.array([0.5, 0.6, 0.5])
+    v = np.array([0.1, -0.2, 0.3])
+    m = np.array([1.0, 2.0, 3.0])
+    p = Particles(x, y, z, v, 2 * v, -v, m)
+    o = oracle.OracleSim(p, 3, 3)
+    e = o.observables()
+    ek = 0.5 * np.sum(m * (v ** 2 + 4 * v ** 2 + v ** 2))
+    ep = 0.0
+    for i in range(3):
+        for j in range(i):
+            ep -= m[i] * m[j] / np.sqrt((x[i] - x[j]) ** 2 + (y[i] - y[j]) ** 2 + (z[i] - z[j]) ** 2)
+    np.testing.assert_allclose(e, [ek, ep, ek + ep, np.sum(m * v), np.sum(2 * m * v), -np.sum(m * v)], rtol=1e-14)
