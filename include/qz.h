@@ -115,6 +115,20 @@ int qz_observables(qz_ctx *ctx, int exact_pe, double *out6);
  * cell size of the finest node holding it, capped at dt_max */
 int qz_timestep(qz_ctx *ctx, double cfl, double dt_max, double *dt);
 
+/* Adjoint of the head-grid step (lmin == lmax).  Cotangent and gradient
+ * arrays hold 3N doubles, x then y then z components, in the original
+ * particle order.
+ *  qz_force_vjp: xbar = J_A(x)^T abar for the force pass a = A(x) at the
+ *    current positions (CIC deposit, multigrid solve, gradient, CIC gather).
+ *  qz_adjoint_forward: from the current state (force pass done) run nsteps
+ *    fixed-dt steps keeping checkpoints every checkpoint_every steps.
+ *  qz_adjoint_backward: given the cotangents of the final positions and
+ *    velocities, return those of the initial ones; forward states are
+ *    recomputed from the checkpoints.  The context is left at the final state. */
+int qz_force_vjp(qz_ctx *ctx, const double *abar, double *xbar);
+int qz_adjoint_forward(qz_ctx *ctx, int nsteps, double dt, int checkpoint_every);
+int qz_adjoint_backward(qz_ctx *ctx, const double *xbar_n, const double *vbar_n, double *xbar_0, double *vbar_0);
+
 /* A1 (ptcl_idx_to_box_idx.c:30-139) on the device for n host positions:
  * key = (f(x 2^lv) - ts) linearised with the real box dims rd, f(p) = p<0 ? -1 : (int)p */
 int qz_keys(const double *x, const double *y, const double *z, int n, int lv, const int *ts, const int *rd,

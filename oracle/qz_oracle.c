@@ -1588,3 +1588,199 @@ void oq_stats(const oq_sim *s, int *out4)
   out4[2] = s->stat_cg_solves;
   out4[3] = s->nnodes - 1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FP64 reverse mode of the head-grid step (checker for the GPU adjoint).      */
+/* Forward force pass a = A(x): CIC deposit (grid_density.c:35-111), Poisson   */
+/* solve (potential_head_node.c), central-difference gradient                 */
+/* (grid_acceleration.c:36-76), CIC gather (particle_acceleration.c:33-132).   */
+/* ------------------------------------------------------------------------ */
+
+/* solve Lap psi = rhs on the packed cube with zero faces until the RMS residual
+ * falls below tol times the RMS right-hand side (V-cycles of multigrid.c) */
+static int adj_solve(int lmin, double *psi, const double *rhs, double tol)
+{
+  int n = 1 << lmin, gs = n + 1;
+  size_t g3 = (size_t)gs * gs * gs;
+  mgpar mp = {2, 2, 2, 1.0};
+  double oneH2 = (double)n * n;
+  double r0 = 0.0;
+  size_t cnt = 0;
+  for (int k = 1; k < n; k++)
+    for (int j = 1; j < n; j++)
+      for (int i = 1; i < n; i++)
+      {
+        size_t g = i + (size_t)gs * (j + (size_t)gs * k);
+        r0 += rhs[g] * rhs[g];
+        cnt++;
+      }
+  r0 = sqrt(r0 / cnt);
+  memset(psi, 0, sizeof(double) * g3);
+  if (r0 == 0.0)
+    return 0;
+  for (int it = 1; it <= 500; it++)
+  {
+    vcycle(psi, rhs, gs, &mp);
+    double err = 0.0;
+    for (int k = 1; k < n; k++)
+      for (int j = 1; j < n; j++)
+        for (int i = 1; i < n; i++)
+        {
+          size_t g = i + (size_t)gs * (j + (size_t)gs * k);
+          double d = rhs[g] - oneH2 * (psi[g + 1] + psi[g - 1] + psi[g + gs] + psi[g - gs] + psi[g + (size_t)gs * gs] +
+                                       psi[g - (size_t)gs * gs] - 6.0 * psi[g]);
+          err += d * d;
+        }
+    if (sqrt(err / cnt) < tol * r0)
+      return it;
+  }
+  return -1;
+}
+
+/* xbar += J_A(x)^T abar at the current positions (head only).  abar, xbar: 3N. */
+int oq_force_vjp(oq_sim *s, const double *abar, double *xbar)
+{
+  if (s->lmin != s->lmax)
+    return 1;
+  oq_reset(s);
+  oq_grid_density(s);
+  if (oq_potential(s))
+    return 1;
+  oq_grid_acceleration(s);
+  onode *h = &s->nodes[0];
+  int n = 1 << s->lmin, n1 = n + 1;
+  int gX = gdx(h), gXY = gdxy(h);
+  size_t p3 = (size_t)n1 * n1 * n1;
+  double *gb = (double *)calloc(3 * p3, sizeof(double));
+  double *rhs = (double *)calloc(p3, sizeof(double));
+  double *psi = (double *)calloc(p3, sizeof(double));
+  const double H = 1.0 / n;
+  /* Gbar = sum_i W(x_i, g) abar_i on the packed solve domain */
+  for (int i = 0; i < s->n; i++)
+  {
+    double px = s->x[i] * n, py = s->y[i] * n, pz = s->z[i] * n;
+    int fx = (int)px, fy = (int)py, fz = (int)pz;
+    double tx = px - fx, ty = py - fy, tz = pz - fz;
+    double wx[2] = {1.0 - tx, tx}, wy[2] = {1.0 - ty, ty}, wz[2] = {1.0 - tz, tz};
+    for (int kk = 0; kk < 2; kk++)
+      for (int jj = 0; jj < 2; jj++)
+        for (int ii = 0; ii < 2; ii++)
+        {
+          size_t g = (fx + ii) + (size_t)n1 * ((fy + jj) + (size_t)n1 * (fz + kk));
+          double w = wx[ii] * wy[jj] * wz[kk];
+          for (int a = 0; a < 3; a++)
+            gb[a * p3 + g] += w * abar[(size_t)a * s->n + i];
+        }
+  }
+  /* transpose of the interior central difference */
+  for (int k = 1; k < n; k++)
+    for (int j = 1; j < n; j++)
+      for (int i = 1; i < n; i++)
+      {
+        size_t g = i + (size_t)n1 * (j + (size_t)n1 * k);
+        double v = 0.0;
+        if (i + 1 < n)
+          v += gb[g + 1];
+        if (i - 1 > 0)
+          v -= gb[g - 1];
+        if (j + 1 < n)
+          v += gb[p3 + g + n1];
+        if (j - 1 > 0)
+          v -= gb[p3 + g - n1];
+        if (k + 1 < n)
+          v += gb[2 * p3 + g + (size_t)n1 * n1];
+        if (k - 1 > 0)
+          v -= gb[2 * p3 + g - (size_t)n1 * n1];
+        rhs[g] = v / (2.0 * H);
+      }
+  int rc = adj_solve(s->lmin, psi, rhs, 1e-12) < 0;
+  /* gather of weight gradients: grad W (c m psi + abar . G) */
+  const double coeff = 4.0 * s->G * OQ_PI / (H * H * H);
+  for (int i = 0; i < s->n; i++)
+  {
+    double px = s->x[i] * n, py = s->y[i] * n, pz = s->z[i] * n;
+    int fx = (int)px, fy = (int)py, fz = (int)pz;
+    double tx = px - fx, ty = py - fy, tz = pz - fz;
+    double wx[2] = {1.0 - tx, tx}, wy[2] = {1.0 - ty, ty}, wz[2] = {1.0 - tz, tz};
+    double dw[2] = {-(double)n, (double)n};
+    double ab[3] = {abar[i], abar[(size_t)s->n + i], abar[2 * (size_t)s->n + i]};
+    for (int kk = 0; kk < 2; kk++)
+      for (int jj = 0; jj < 2; jj++)
+        for (int ii = 0; ii < 2; ii++)
+        {
+          size_t g = (fx + ii) + (size_t)n1 * ((fy + jj) + (size_t)n1 * (fz + kk));
+          int gb_ = (fx + ii + 1) + gX * (fy + jj + 1) + gXY * (fz + kk + 1); /* reference box grid index */
+          double val = coeff * s->m[i] * psi[g] + ab[0] * h->ax[gb_] + ab[1] * h->ay[gb_] + ab[2] * h->az[gb_];
+          xbar[i] += dw[ii] * wy[jj] * wz[kk] * val;
+          xbar[(size_t)s->n + i] += wx[ii] * dw[jj] * wz[kk] * val;
+          xbar[2 * (size_t)s->n + i] += wx[ii] * wy[jj] * dw[kk] * val;
+        }
+  }
+  free(gb);
+  free(rhs);
+  free(psi);
+  return rc;
+}
+
+/* Reverse mode of nsteps fixed-dt KDK steps from the current state (a force
+ * pass done).  Stores every state; on return the sim holds the final state. */
+int oq_adjoint(oq_sim *s, int nsteps, double dt, const double *xbN, const double *vbN, double *xb0, double *vb0)
+{
+  size_t N = (size_t)s->n, n9 = 9 * N, n3 = 3 * N;
+  double *st = (double *)malloc(sizeof(double) * n9 * (size_t)(nsteps + 1));
+  unsigned char *refl = (unsigned char *)calloc(N * (size_t)(nsteps + 1), 1);
+  oq_get_particles(s, st);
+  for (int j = 1; j <= nsteps; j++)
+  {
+    /* reflection flags of step j from the drift of step j */
+    double *prev = st + n9 * (size_t)(j - 1);
+    for (size_t i = 0; i < N; i++)
+      for (int a = 0; a < 3; a++)
+      {
+        double u = prev[(3 + a) * N + i] + prev[(6 + a) * N + i] * dt * 0.5;
+        double y = prev[a * N + i] + u * dt;
+        if (y < 0.0 || y >= 1.0)
+          refl[N * j + i] |= (unsigned char)(1 << a);
+      }
+    if (oq_step(s, dt))
+      return 1;
+    oq_get_particles(s, st + n9 * (size_t)j);
+  }
+  double *xb = (double *)malloc(sizeof(double) * n3), *vb = (double *)malloc(sizeof(double) * n3);
+  double *abA = (double *)calloc(n3, sizeof(double)), *ab = (double *)malloc(sizeof(double) * n3);
+  memcpy(xb, xbN, sizeof(double) * n3);
+  memcpy(vb, vbN, sizeof(double) * n3);
+  const double h = 0.5 * dt;
+  for (int j = nsteps; j >= 1; j--)
+  {
+    oq_set_particles(s, st + n9 * (size_t)j);
+    for (size_t k = 0; k < n3; k++)
+      ab[k] = h * vb[k] + abA[k];
+    if (oq_force_vjp(s, ab, xb))
+      return 1;
+    for (size_t i = 0; i < N; i++)
+      for (int a = 0; a < 3; a++)
+      {
+        size_t k = a * N + i;
+        double sg = ((refl[N * j + i] >> a) & 1) ? -1.0 : 1.0;
+        double yb = sg * xb[k];
+        double ub = sg * vb[k] + dt * yb;
+        xb[k] = yb;
+        vb[k] = ub;
+        abA[k] = h * ub;
+      }
+  }
+  oq_set_particles(s, st);
+  if (oq_force_vjp(s, abA, xb))
+    return 1;
+  memcpy(xb0, xb, sizeof(double) * n3);
+  memcpy(vb0, vb, sizeof(double) * n3);
+  oq_set_particles(s, st + n9 * (size_t)nsteps);
+  free(st);
+  free(refl);
+  free(xb);
+  free(vb);
+  free(abA);
+  free(ab);
+  return 0;
+}

@@ -69,6 +69,14 @@ int oq_key(int lv, const int *ts, const int *rd, double x, double y, double z);
  * poisson_error.c).  Returns V-cycles used, or -1 on failure. */
 int oq_head_solve(int lmin, double *phi, const double *rho, double node_mass, double tol, int max_iter);
 
+/* FP64 reverse mode of the head-grid step (lmin == lmax).  abar/xbar and the
+ * cotangent arrays are 3N doubles (x, then y, then z components).
+ * oq_force_vjp: xbar += J_A(x)^T abar for the force pass at the current positions.
+ * oq_adjoint: from the current state (force pass done) run nsteps fixed-dt
+ * steps and return the cotangents of the initial x, v from those of the final. */
+int oq_force_vjp(oq_sim *s, const double *abar, double *xbar);
+int oq_adjoint(oq_sim *s, int nsteps, double dt, const double *xbN, const double *vbN, double *xb0, double *vb0);
+
 /* Configuration knobs (defaults = reference defaults, global_variables.c:313-334). */
 void oq_set_tolerance(oq_sim *s, double tol);
 

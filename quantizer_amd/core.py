@@ -80,7 +80,7 @@ EXPORTED = [
     "qz_potential_head_node", "qz_potential_branches", "qz_grid_acceleration", "qz_particle_acceleration",
     "qz_particle_updating_B", "qz_force", "qz_step", "qz_get_head_fields", "qz_set_head_potential",
     "qz_num_nodes", "qz_node_header", "qz_node_grid_lists", "qz_node_fields", "qz_get_stats", "qz_observables",
-    "qz_timestep", "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
+    "qz_timestep", "qz_force_vjp", "qz_adjoint_forward", "qz_adjoint_backward", "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
 ]
 
 
@@ -118,6 +118,9 @@ def load_library(path: str = LIB_PATH):
     lib.qz_get_stats.argtypes = [vp, ctypes.POINTER(QzStats)]
     lib.qz_observables.argtypes = [vp, ctypes.c_int, _dp]
     lib.qz_timestep.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp]
+    lib.qz_force_vjp.argtypes = [vp, _dp, _dp]
+    lib.qz_adjoint_forward.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.qz_adjoint_backward.argtypes = [vp, _dp, _dp, _dp, _dp]
     lib.qz_keys.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip]
     lib.qz_timing_enable.argtypes = [vp, ctypes.c_int]
     lib.qz_timing_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int]
@@ -293,6 +296,26 @@ class Simulation:
         out = np.zeros(1)
         _check(load_library().qz_timestep(self._h, cfl, dt_max, _d(out)), "qz_timestep")
         return float(out[0])
+
+    # ------------------------------------------------------------ adjoint (head grid)
+    def force_vjp(self, abar: np.ndarray) -> np.ndarray:
+        """J_A(x)^T abar for the force pass at the current positions; abar is (3, N)."""
+        a = np.ascontiguousarray(abar, dtype=np.float64).reshape(-1)
+        out = np.empty(3 * self.n)
+        _check(load_library().qz_force_vjp(self._h, _d(a), _d(out)), "qz_force_vjp")
+        return out.reshape(3, self.n)
+
+    def adjoint_forward(self, nsteps: int, dt: float, checkpoint_every: int):
+        return _check(load_library().qz_adjoint_forward(self._h, nsteps, dt, checkpoint_every), "qz_adjoint_forward")
+
+    def adjoint_backward(self, xbar_n: np.ndarray, vbar_n: np.ndarray):
+        """Cotangents of the initial (x, v) from those of the final state, each (3, N)."""
+        xb = np.ascontiguousarray(xbar_n, dtype=np.float64).reshape(-1)
+        vb = np.ascontiguousarray(vbar_n, dtype=np.float64).reshape(-1)
+        ox = np.empty(3 * self.n)
+        ov = np.empty(3 * self.n)
+        _check(load_library().qz_adjoint_backward(self._h, _d(xb), _d(vb), _d(ox), _d(ov)), "qz_adjoint_backward")
+        return ox.reshape(3, self.n), ov.reshape(3, self.n)
 
     # ------------------------------------------------------------ timing
     def timing(self, on: bool = True):

@@ -55,17 +55,18 @@ void Timer::begin(hipStream_t s)
   hipEvent_t a;
   (void)hipEventCreate(&a);
   (void)hipEventRecord(a, s);
-  pending_start = a;
+  pending.push_back(a);
 }
 
 void Timer::end(hipStream_t s, const char *name, double bytes)
 {
-  if (!on)
+  if (!on || pending.empty())
     return;
   hipEvent_t b;
   (void)hipEventCreate(&b);
   (void)hipEventRecord(b, s);
-  recs.push_back({name, pending_start, b, bytes});
+  recs.push_back({name, pending.back(), b, bytes});
+  pending.pop_back();
 }
 
 void Timer::collect()
@@ -88,6 +89,9 @@ void Timer::collect()
     (void)hipEventDestroy(r.b);
   }
   recs.clear();
+  for (hipEvent_t e : pending)
+    (void)hipEventDestroy(e);
+  pending.clear();
 }
 
 #define QZ_LAUNCH(name, kern, grid, ...) QZ_LAUNCHB(name, 0.0, kern, grid, __VA_ARGS__)
@@ -135,8 +139,34 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
   bool keys_valid = false;
-  hipGraphExec_t head_graph = nullptr; // one outer iteration of the head solve
-  int head_graph_inner = -1;
+  // captured outer iteration of the head solve, per (phi, rhs) pair: [0] the
+  // potential, [1] the adjoint solve
+  hipGraphExec_t head_graph[2] = {nullptr, nullptr};
+  int head_graph_inner[2] = {-1, -1};
+  unsigned char *refl_rec = nullptr; // when set, update_A records reflected axes (ID order)
+  // adjoint state (qz_adjoint.inl)
+  double *adj_xb = nullptr, *adj_vb = nullptr, *adj_ab = nullptr, *adj_abA = nullptr, *adj_psi = nullptr;
+  P *adj_slot = nullptr;
+  G *adj_gbar[3] = {nullptr, nullptr, nullptr}, *adj_rhs = nullptr;
+  struct Ckpt
+  {
+    double *x[3];
+    P *a[7];
+    int *id;
+  };
+  struct StepRec
+  {
+    double *x[3];
+    int *id;
+    unsigned long long *key;
+    int *cstart;
+    G *g[3];
+    unsigned char *refl;
+  };
+  std::vector<Ckpt> ckpts;
+  std::vector<StepRec> recs;
+  int adj_nsteps = 0, adj_every = 0;
+  double adj_dt = 0.0;
 
   ~Engine() override { release(); }
 
@@ -180,8 +210,10 @@ template <typename P, typename G> struct Engine final : EngineBase
     (void)hipFree(red);
     if (hred)
       (void)hipHostFree(hred);
-    if (head_graph)
-      (void)hipGraphExecDestroy(head_graph);
+    for (int q = 0; q < 2; q++)
+      if (head_graph[q])
+        (void)hipGraphExecDestroy(head_graph[q]);
+    release_adjoint();
     release_tree();
   }
 
@@ -319,7 +351,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool update_A(double dt) override
   {
     QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P)) * N, k_update_A<P>, nblk(N), pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
-              dt);
+              dt, refl_rec, refl_rec ? id_a : nullptr);
     keys_valid = false;
     return hip_ok(hipStreamSynchronize(stream), "particle_updating_A");
   }
@@ -435,57 +467,59 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
   }
 
-  // One outer iteration of the head solve after the first residual: zero the
-  // correction, `inner` V-cycles in G, FP64 correction with the pointwise change
-  // test, then the new FP64 residual and its sum of squares.
-  void head_outer_body(int inner)
+  // One outer iteration of a head solve Lap_h ph = rh after the first residual:
+  // `inner` V-cycles in G on the correction, FP64 correction with the pointwise
+  // change test, then the new FP64 residual and its sum of squares.
+  void head_outer_body(double *ph, const G *rh, int inner)
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
     unsigned nb = nblk(npts);
     for (int c = 0; c < inner; c++)
       vcycle(0, c == 0);
-    QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, phi, lev_e[0], n, cfg.tol_change,
+    QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, ph, lev_e[0], n, cfg.tol_change,
                part);
     hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
-    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, phi, rho, lev_r[0], n,
+    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0], n,
                1.0 / (H * H), part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
   }
 
-  bool launch_head_body(int inner)
+  bool launch_head_body(double *ph, const G *rh, int inner, int slot)
   {
     if (timer.on)
     {
-      head_outer_body(inner); // eager launches so each kernel is timed
+      head_outer_body(ph, rh, inner); // eager launches so each kernel is timed
       return true;
     }
-    if (!head_graph || head_graph_inner != inner)
+    if (!head_graph[slot] || head_graph_inner[slot] != inner)
     {
-      if (head_graph)
+      if (head_graph[slot])
       {
-        (void)hipGraphExecDestroy(head_graph);
-        head_graph = nullptr;
+        (void)hipGraphExecDestroy(head_graph[slot]);
+        head_graph[slot] = nullptr;
       }
       hipGraph_t g = nullptr;
       if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
         return false;
-      head_outer_body(inner);
+      head_outer_body(ph, rh, inner);
       if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
-          !hip_ok(hipGraphInstantiate(&head_graph, g, nullptr, nullptr, 0), "graph instantiate"))
+          !hip_ok(hipGraphInstantiate(&head_graph[slot], g, nullptr, nullptr, 0), "graph instantiate"))
         return false;
       (void)hipGraphDestroy(g);
-      head_graph_inner = inner;
+      head_graph_inner[slot] = inner;
     }
-    return hip_ok(hipGraphLaunch(head_graph, stream), "graph launch");
+    return hip_ok(hipGraphLaunch(head_graph[slot], stream), "graph launch");
   }
 
   // Head solve: FP64 outer residual/correction around inner V-cycles in G.
-  // Convergence follows poisson_error.c:290-347 (method 0): the RMS residual
-  // over interior points divided by 4 pi G rho_mean, and the pointwise change
-  // of phi over the last outer iteration (the reference compares with phi from
-  // iter_between_check_potential_solution = 5 V-cycles earlier).
-  bool potential_head() override
+  // Potential (adjoint = false): convergence follows poisson_error.c:290-347
+  // (method 0): the RMS residual over interior points divided by 4 pi G
+  // rho_mean, and the pointwise change of phi over the last outer iteration (the
+  // reference compares with phi from iter_between_check_potential_solution = 5
+  // V-cycles earlier).  Adjoint solve (zero faces, ph starts at 0): the RMS
+  // residual relative to the RMS right-hand side.
+  bool solve_head(double *ph, const G *rh, bool adjoint)
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
@@ -495,36 +529,49 @@ template <typename P, typename G> struct Engine final : EngineBase
     int inner = cfg.inner_cycles;
     int max_outer = std::max(1, cfg.max_iterations / std::max(1, inner));
     unsigned nb = nblk(npts);
-    stats.head_outer = 0;
-    stats.head_vcycles = 0;
-    double change = 1e300;
-    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, phi, rho, lev_r[0], n,
+    int outers = 0, vcycles = 0;
+    double change = 1e300, res = 0.0, scale = rhomean;
+    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0], n,
                inv_h2, part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+    bool ok = false;
     for (int outer = 0;; outer++)
     {
       if (!hip_ok(hipMemcpyAsync(hred, red, 2 * sizeof(double), hipMemcpyDeviceToHost, stream), "residual readback") ||
           !hip_ok(hipStreamSynchronize(stream), "residual"))
         return false;
-      double res = std::sqrt(hred[0] / ninterior) / rhomean;
+      double rms = std::sqrt(hred[0] / ninterior);
+      if (adjoint && outer == 0)
+        scale = rms > 0.0 ? rms : 1.0;
+      res = rms / scale;
       if (outer > 0)
         change = hred[1];
-      stats.head_residual = res;
-      stats.head_change = change;
-      bool change_ok = outer > 0 && change <= 1.0; // flag values > 1 mark points outside [1-tol, 1+tol]
-      if (change_ok && res < cfg.tol)
-        return true;
+      if (adjoint ? (res < cfg.tol || rms == 0.0) : (outer > 0 && change <= 1.0 && res < cfg.tol))
+      {
+        ok = true; // change flag values > 1 mark points outside [1-tol, 1+tol]
+        break;
+      }
       if (outer == max_outer)
         break;
-      if (!launch_head_body(inner))
+      if (!launch_head_body(ph, rh, inner, adjoint ? 1 : 0))
         return false;
-      stats.head_vcycles += inner;
-      stats.head_outer++;
+      vcycles += inner;
+      outers++;
     }
-    set_error("The precision was not reached in the parent node after %d V-cycles (residual %.3e, change %.3e)",
-              stats.head_vcycles, stats.head_residual, stats.head_change);
-    return false;
+    if (!adjoint)
+    {
+      stats.head_outer = outers;
+      stats.head_vcycles = vcycles;
+      stats.head_residual = res;
+      stats.head_change = change;
+    }
+    if (!ok)
+      set_error("The precision was not reached in the %s solve after %d V-cycles (residual %.3e, change %.3e)",
+                adjoint ? "adjoint head" : "parent node", vcycles, res, change);
+    return ok;
   }
+
+  bool potential_head() override { return solve_head(phi, rho, false); }
 
   bool grid_acceleration() override
   {
@@ -611,6 +658,14 @@ template <typename P, typename G> struct Engine final : EngineBase
 
   bool timestep(double cfl, double dt_max, double *dt) override;
 
+  // ---------------------------------------------------------------- adjoint (qz_adjoint.inl)
+  void release_adjoint();
+  bool ensure_adjoint();
+  bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
+  bool force_vjp(const double *abar, double *xbar) override;
+  bool adjoint_forward(int nsteps, double dt, int every) override;
+  bool adjoint_backward(const double *xbN, const double *vbN, double *xb0, double *vb0) override;
+
   // ---------------------------------------------------------------- branches (qz_branch.inl)
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
@@ -628,6 +683,7 @@ template <typename P, typename G> struct Engine final : EngineBase
 } // namespace qz
 
 #include "qz_branch.inl"
+#include "qz_adjoint.inl"
 
 namespace qz
 {
@@ -895,6 +951,24 @@ int qz_timestep(qz_ctx *ctx, double cfl, double dt_max, double *dt)
 {
   QZ_CHECK_CTX(ctx);
   return ctx->eng->timestep(cfl, dt_max, dt) ? 0 : 1;
+}
+
+int qz_force_vjp(qz_ctx *ctx, const double *abar, double *xbar)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->force_vjp(abar, xbar) ? 0 : 1;
+}
+
+int qz_adjoint_forward(qz_ctx *ctx, int nsteps, double dt, int checkpoint_every)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->adjoint_forward(nsteps, dt, checkpoint_every) ? 0 : 1;
+}
+
+int qz_adjoint_backward(qz_ctx *ctx, const double *xbar_n, const double *vbar_n, double *xbar_0, double *vbar_0)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->adjoint_backward(xbar_n, vbar_n, xbar_0, vbar_0) ? 0 : 1;
 }
 
 int qz_keys(const double *x, const double *y, const double *z, int n, int lv, const int *ts, const int *rd, int *out)

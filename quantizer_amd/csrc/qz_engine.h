@@ -32,7 +32,7 @@ struct Timer
     double bytes; // algorithmic bytes (each input read once, each output written once)
   };
   bool on = false;
-  hipEvent_t pending_start{};
+  std::vector<hipEvent_t> pending; // open begin() events; timed regions may nest
   std::vector<Rec> recs;
   std::vector<Acc> acc;
   void begin(hipStream_t s);
@@ -81,6 +81,9 @@ struct EngineBase
   virtual bool set_head_potential(const double *pot) = 0;
   virtual bool observables(int exact_pe, double *out6) = 0;
   virtual bool timestep(double cfl, double dt_max, double *dt) = 0;
+  virtual bool force_vjp(const double *abar, double *xbar) = 0;
+  virtual bool adjoint_forward(int nsteps, double dt, int every) = 0;
+  virtual bool adjoint_backward(const double *xbN, const double *vbN, double *xb0, double *vb0) = 0;
   virtual int num_nodes() = 0;
   virtual bool node_header(int i, int *hdr) = 0;
   virtual bool node_lists(int i, int *ixyz, int *bxyz, int *sxyz) = 0;

@@ -377,13 +377,15 @@ __global__ void k_gather_head(const double *__restrict__ x, const double *__rest
 template <typename P>
 __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, P *__restrict__ vx,
                            P *__restrict__ vy, P *__restrict__ vz, P *__restrict__ ax, P *__restrict__ ay,
-                           P *__restrict__ az, int N, double dt)
+                           P *__restrict__ az, int N, double dt, unsigned char *__restrict__ refl,
+                           const int *__restrict__ id)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N)
     return;
   const P h = (P)0.5;
   const P dtp = (P)dt;
+  unsigned char flags = 0;
   double pos[3] = {x[i], y[i], z[i]};
   P vel[3] = {vx[i], vy[i], vz[i]};
   P acc[3] = {ax[i], ay[i], az[i]};
@@ -397,8 +399,11 @@ __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, doubl
       pos[a] = pos[a] >= 1.0 ? 2.0 - pos[a] : -pos[a];
       vel[a] = -vel[a];
       acc[a] = -acc[a];
+      flags |= (unsigned char)(1 << a);
     }
   }
+  if (refl)
+    refl[id[i]] = flags; // reflected axes, for the adjoint of this step
   x[i] = pos[0];
   y[i] = pos[1];
   z[i] = pos[2];

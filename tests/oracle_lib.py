@@ -58,6 +58,10 @@ def _lib():
     lib.oq_head_solve.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     lib.oq_head_solve.restype = ctypes.c_int
     lib.oq_set_tolerance.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.oq_force_vjp.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.oq_force_vjp.restype = ctypes.c_int
+    lib.oq_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+    lib.oq_adjoint.restype = ctypes.c_int
     return lib
 
 
@@ -146,6 +150,22 @@ class OracleSim:
         out = np.empty(6)
         lib().oq_observables(self.h, _d(out))
         return out
+
+    def force_vjp(self, abar):
+        a = np.ascontiguousarray(abar, dtype=np.float64).reshape(-1)
+        out = np.zeros(3 * self.n)
+        if lib().oq_force_vjp(self.h, _d(a), _d(out)):
+            raise RuntimeError("oracle force_vjp failed")
+        return out.reshape(3, self.n)
+
+    def adjoint(self, nsteps, dt, xbar_n, vbar_n):
+        xb = np.ascontiguousarray(xbar_n, dtype=np.float64).reshape(-1)
+        vb = np.ascontiguousarray(vbar_n, dtype=np.float64).reshape(-1)
+        ox = np.zeros(3 * self.n)
+        ov = np.zeros(3 * self.n)
+        if lib().oq_adjoint(self.h, nsteps, dt, _d(xb), _d(vb), _d(ox), _d(ov)):
+            raise RuntimeError("oracle adjoint failed")
+        return ox.reshape(3, self.n), ov.reshape(3, self.n)
 
     def nodes(self):
         res = []

@@ -1,0 +1,97 @@
+"""Adjoint of the head-grid step on the GPU: force VJP and checkpointed trajectory
+gradients against the FP64 CPU reverse mode (oracle) and central differences."""
+import numpy as np
+import pytest
+
+from quantizer_amd import core
+from quantizer_amd.workloads import plummer, plummer_dt
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("prec,tol", [(core.PREC_FP64, 1e-8), (core.PREC_MIXED, 2e-4)])
+def test_force_vjp_matches_oracle(oracle, prec, tol):
+    print(f"case: force vjp prec={prec} N=4000 lmin=5", flush=True)
+    p = plummer(4000, seed=12)
+    rng = np.random.default_rng(4)
+    ab = rng.standard_normal((3, p.n))
+    o = oracle.OracleSim(p, 5, 5)
+    want = o.force_vjp(ab)
+    with core.Simulation(p, 5, 5, precision=prec) as s:
+        got = s.force_vjp(ab)
+    assert rel(got, want) < tol
+
+
+def _target(parts, lmin, nsteps, dt, seed, prec):
+    rng = np.random.default_rng(seed)
+    t = parts.copy()
+    for a in (t.x, t.y, t.z):
+        a += rng.normal(0.0, 0.01 / 40, parts.n)  # N(0, 0.01 a) in a 40 a box
+    with core.Simulation(t, lmin, lmin, precision=prec) as s:
+        s.force()
+        for _ in range(nsteps):
+            s.step(dt)
+        return s.particles()[:3]
+
+
+@pytest.mark.parametrize("prec,tol", [(core.PREC_FP64, 1e-7), (core.PREC_MIXED, 5e-3)])
+def test_trajectory_adjoint_matches_oracle_reverse_mode(oracle, prec, tol):
+    print(f"case: trajectory adjoint prec={prec} N=3000 lmin=5 10 steps ckpt 4", flush=True)
+    p = plummer(3000, seed=13)
+    lmin, nsteps, dt = 5, 10, plummer_dt(0.01)
+    target = _target(p, lmin, nsteps, dt, 5, core.PREC_FP64)
+    with core.Simulation(p, lmin, lmin, precision=prec) as s:
+        s.force()
+        s.adjoint_forward(nsteps, dt, 4)
+        xN = s.particles()[:3]
+        xbN = 2.0 * (xN - target) / p.n
+        gx, gv = s.adjoint_backward(xbN, np.zeros_like(xbN))
+        x_after = s.particles()[:3]
+    np.testing.assert_array_equal(x_after, xN)  # context left at the final state
+    o = oracle.OracleSim(p, lmin, lmin)
+    o.force()
+    wx, wv = o.adjoint(nsteps, dt, xbN, np.zeros_like(xbN))
+    assert rel(gx, wx) < tol
+    assert rel(gv, wv) < tol
+
+
+def test_trajectory_gradient_central_differences():
+    """Central differences of the loss on 64 seeded coordinates (FP64 path)."""
+    print("case: FD check N=2048 lmin=5 8 steps", flush=True)
+    p = plummer(2048, seed=14)
+    lmin, nsteps, dt = 5, 8, plummer_dt(0.01)
+    target = _target(p, lmin, nsteps, dt, 6, core.PREC_FP64)
+
+    def loss(parts):
+        with core.Simulation(parts, lmin, lmin, precision=core.PREC_FP64) as s:
+            s.force()
+            for _ in range(nsteps):
+                s.step(dt)
+            x = s.particles()[:3]
+        return float(np.mean(np.sum((x - target) ** 2, axis=0)))
+
+    with core.Simulation(p, lmin, lmin, precision=core.PREC_FP64) as s:
+        s.force()
+        s.adjoint_forward(nsteps, dt, 3)
+        xN = s.particles()[:3]
+        gx, gv = s.adjoint_backward(2.0 * (xN - target) / p.n, np.zeros((3, p.n)))
+    rng = np.random.default_rng(7)
+    eps = 1e-3 / 40 * 1e-3  # 1e-3 a scaled down so no particle changes cell during the pair
+    bad = 0
+    names = ["x", "y", "z", "vx", "vy", "vz"]
+    for _ in range(64):
+        k = int(rng.integers(0, 6))
+        i = int(rng.integers(0, p.n))
+        plus, minus = p.copy(), p.copy()
+        getattr(plus, names[k])[i] += eps
+        getattr(minus, names[k])[i] -= eps
+        fd = (loss(plus) - loss(minus)) / (2 * eps)
+        g = (gx if k < 3 else gv)[k % 3, i]
+        if abs(fd - g) > 1e-4 * abs(g) + 1e-10:
+            bad += 1
+    # a perturbed particle can cross a cell face, where CIC weights have a kink
+    assert bad <= 2

@@ -191,3 +191,48 @@ def test_observables_exact_sums(oracle):
         for j in range(i):
             ep -= m[i] * m[j] / np.sqrt((x[i] - x[j]) ** 2 + (y[i] - y[j]) ** 2 + (z[i] - z[j]) ** 2)
     np.testing.assert_allclose(e, [ek, ep, ek + ep, np.sum(m * v), np.sum(2 * m * v), -np.sum(m * v)], rtol=1e-14)
+
+
+def _traj_loss(oracle, parts, lmin, nsteps, dt, target):
+    o = oracle.OracleSim(parts, lmin, lmin)
+    o.force()
+    for _ in range(nsteps):
+        o.step(dt)
+    x = o.particles()[:3]
+    return float(np.mean(np.sum((x - target) ** 2, axis=0)))
+
+
+def test_oracle_reverse_mode_matches_central_differences(oracle):
+    """The FP64 reverse mode of the step (checker of the GPU adjoint) against
+    central differences of the loss mean |x_N - x_T|^2."""
+    from quantizer_amd.workloads import plummer, plummer_dt
+    p = plummer(1500, seed=8)
+    lmin, nsteps, dt = 4, 6, plummer_dt(0.01)
+    rng = np.random.default_rng(3)
+    tgt_p = p.copy()
+    tgt_p.x += rng.normal(0, 0.01 / 40, p.n)
+    tgt_p.y += rng.normal(0, 0.01 / 40, p.n)
+    tgt_p.z += rng.normal(0, 0.01 / 40, p.n)
+    ot = oracle.OracleSim(tgt_p, lmin, lmin)
+    ot.force()
+    for _ in range(nsteps):
+        ot.step(dt)
+    target = ot.particles()[:3]
+    o = oracle.OracleSim(p, lmin, lmin)
+    o.force()
+    for _ in range(nsteps):
+        o.step(dt)
+    xN = o.particles()[:3]
+    xbN = 2.0 * (xN - target) / p.n
+    o0 = oracle.OracleSim(p, lmin, lmin)
+    o0.force()
+    gx, gv = o0.adjoint(nsteps, dt, xbN, np.zeros_like(xbN))
+    eps = 1e-7
+    for (arr, comp, i) in [("x", 0, 3), ("y", 1, 100), ("z", 2, 777), ("vx", 0, 5), ("vz", 2, 1400)]:
+        plus, minus = p.copy(), p.copy()
+        getattr(plus, arr)[i] += eps
+        getattr(minus, arr)[i] -= eps
+        fd = (_traj_loss(oracle, plus, lmin, nsteps, dt, target) -
+              _traj_loss(oracle, minus, lmin, nsteps, dt, target)) / (2 * eps)
+        g = (gx if arr in "xyz" else gv)[comp, i]
+        assert abs(fd - g) <= 1e-5 * max(abs(fd), 1e-12) + 1e-12, (arr, i, fd, g)
