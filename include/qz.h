@@ -32,6 +32,12 @@ enum
   QZ_BC_ZERO = 1      /* zero Dirichlet faces */
 };
 
+enum
+{
+  QZ_BRANCH_CG = 0, /* unpreconditioned CG per node (conjugate_gradient.c:327-460) */
+  QZ_BRANCH_MG = 1  /* batched masked multigrid per tree level, FP64 outer refinement */
+};
+
 typedef struct
 {
   int lmin;               /* head level: head grid has 2^lmin cells per side (global_variables.c:252) */
@@ -48,6 +54,7 @@ typedef struct
   int inner_cycles;       /* V-cycles per outer FP64 refinement iteration (0 = default) */
   int pre_sweeps;         /* red-black GS sweeps before restriction */
   int post_sweeps;        /* red-black GS sweeps after prolongation */
+  int branch_solver;      /* QZ_BRANCH_* solver of the refinement-node boxes */
 } qz_config;
 
 typedef struct

@@ -30,6 +30,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 BC_MONOPOLE = 0
 BC_ZERO = 1
+BRANCH_CG = 0
+BRANCH_MG = 1
 
 
 class QuantizerError(RuntimeError):
@@ -52,6 +54,7 @@ class QzConfig(ctypes.Structure):
         ("inner_cycles", ctypes.c_int),
         ("pre_sweeps", ctypes.c_int),
         ("post_sweeps", ctypes.c_int),
+        ("branch_solver", ctypes.c_int),
     ]
 
 
@@ -158,7 +161,8 @@ class Simulation:
     def __init__(self, particles, lmin: int, lmax: int | None = None, precision: int = PREC_FP64,
                  head_bc: int = BC_MONOPOLE, ref_criterion_ptcl: int = 8, tol: float = 1e-10,
                  tol_change: float | None = None, inner_cycles: int = 0, max_iterations: int = 5000,
-                 pre_sweeps: int = 2, post_sweeps: int = 2, G: float = 1.0):
+                 pre_sweeps: int = 2, post_sweeps: int = 2, G: float = 1.0,
+                 branch_solver: int | None = None):
         lib = load_library()
         cfg = QzConfig()
         lib.qz_default_config(ctypes.byref(cfg))
@@ -174,6 +178,8 @@ class Simulation:
         cfg.max_iterations = max_iterations
         cfg.pre_sweeps = pre_sweeps
         cfg.post_sweeps = post_sweeps
+        if branch_solver is not None:
+            cfg.branch_solver = branch_solver
         cfg.G = G
         self.cfg = cfg
         self.lmin, self.lmax, self.n = cfg.lmin, cfg.lmax, particles.n

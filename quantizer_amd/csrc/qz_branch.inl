@@ -8,6 +8,7 @@
 // Only the compact list of flagged cells crosses to the host, and only node
 // descriptors and zone cells cross back.
 #include "qz_branch_kernels.h"
+#include "qz_branch_mg_kernels.h"
 #include "qz_tree.h"
 
 namespace qz
@@ -44,6 +45,14 @@ struct TreeDev
   std::vector<int> lfirst, lcount;
   std::vector<long long> lc0, lncell, lb0, lnb, lz0, lnz, lg0, lng, lpmax;
   std::vector<int> lnbm; // x blocks of the node-mass reduction per level
+  std::vector<long long> lnintr; // interior points per level
+  // batched multigrid: per-node hierarchy table and level pools (shared by all tree levels)
+  std::vector<BMg> hbmg;
+  BMg *bmg = nullptr;
+  int bmg_cap = 0;
+  void *mg_e = nullptr, *mg_r = nullptr;
+  unsigned char *mg_mask = nullptr;
+  long long mg_cap = 0;
   long long used_zone = 0, used_cell = 0, used_box = 0, used_pt = 0, used_grid = 0;
 };
 
@@ -72,7 +81,7 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
                     d->pn_i, d->pn_b, d->pn_s, d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
-                    d->done_count, d->mpart};
+                    d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask};
     for (void *q : ptrs)
       (void)hipFree(q);
     if (d->hflag_count)
@@ -356,6 +365,10 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
   stats.n_levels = (int)d.lfirst.size();
   stats.branch_points = 0;
   stats.branch_intr = 0;
+  d.lnintr.assign(d.lfirst.size(), 0);
+  for (size_t l = 1; l < d.lfirst.size(); l++)
+    for (int q = d.lfirst[l]; q < d.lfirst[l] + d.lcount[l]; q++)
+      d.lnintr[l] += d.hnodes[q].nintr;
   for (auto &dn : d.hnodes)
   {
     stats.branch_points += dn.nintr + dn.nbdry + dn.nsim;
@@ -405,6 +418,9 @@ template <typename P, typename G> bool Engine<P, G>::branch_density()
   return true;
 }
 
+// Potential of the refinement nodes, level by level (potential_branches.c:401-470):
+// boundary and starting values from the parent, then either the reference's
+// per-node CG or the batched masked multigrid over all nodes of the level.
 template <typename P, typename G> bool Engine<P, G>::potential_branches()
 {
   if (!has_tree())
@@ -416,80 +432,20 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
   for (size_t l = 1; l < d.lfirst.size(); l++)
   {
     const int first = d.lfirst[l], count = d.lcount[l];
-    const int nbx = blocks_per_node(d.lpmax[l]);
-    dim3 grid(nbx, count);
+    dim3 grid(blocks_per_node(d.lpmax[l]), count);
     timer.begin(stream);
     hipLaunchKernelGGL(k_branch_init_pot, grid, dim3(kBlock), 0, stream, d.nodes, first, d.sim_g, d.sim_xyz, d.bdry_g,
                        d.bdry_xyz, d.intr_g, d.intr_xyz, d.phi, d.phi_old, phi, n, cfg.G * total_mass, zero_bc);
     timer.end(stream, "branch_init_pot");
-    QZ_LAUNCH("branch_fill", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
-    timer.begin(stream);
-    hipLaunchKernelGGL((k_cg_init<G>), grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.phi, (G *)d.rho,
-                       d.cg_r, d.cg_p, d.part2);
-    hipLaunchKernelGGL(k_cg_init_fold, dim3(count), dim3(kBlock), 0, stream, d.nodes, first, d.part2, nbx);
-    timer.end(stream, "cg_init");
-    QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.done_count, 0, (size_t)2);
-    int *it_base = d.done_count + 1;
-    // one batch = 5 CG iterations (conjugate_gradient.c:379-456), the last one
-    // with the convergence check (iter_between_check_potential_solution = 5)
-    auto batch = [&]() {
-      for (int il = 0; il < 5; il++)
-      {
-        timer.begin(stream);
-        hipLaunchKernelGGL(k_cg_ap, grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.cg_p, d.cg_Ap,
-                           d.part1);
-        hipLaunchKernelGGL(k_cg_update, grid, dim3(kBlock), 0, stream, d.nodes, first, it_base, il, d.intr_g, d.phi,
-                           d.phi_old, d.cg_r, d.cg_p, d.cg_Ap, d.part1, d.part2, d.part3, cfg.tol_change);
-        hipLaunchKernelGGL(k_cg_direction, grid, dim3(kBlock), 0, stream, d.nodes, first, it_base, il, d.intr_g,
-                           d.phi, d.phi_old, d.cg_r, d.cg_p, d.part2, d.part3, cfg.tol, cfg.G, d.done_count);
-        timer.end(stream, "branch_cg", 0.0);
-      }
-      hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 5);
-    };
-    hipGraphExec_t gexec = nullptr;
-    if (!timer.on)
-    {
-      hipGraph_t g = nullptr;
-      if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
-        return false;
-      batch();
-      if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
-          !hip_ok(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0), "graph instantiate"))
-        return false;
-      (void)hipGraphDestroy(g);
-    }
-    int max_it = cfg.max_iterations; // cycle_size = min(size, 5000) per node (conjugate_gradient.c:377)
-    int it = 0;
-    bool all_done = false;
-    for (; it < max_it; it += 5)
-    {
-      if (gexec)
-      {
-        if (!hip_ok(hipGraphLaunch(gexec, stream), "graph launch"))
-          return false;
-      }
-      else
-        batch();
-      if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "cg") ||
-          !hip_ok(hipStreamSynchronize(stream), "cg"))
-        return false;
-      if (*d.hflag_count >= count)
-      {
-        all_done = true;
-        break;
-      }
-    }
-    if (gexec)
-      (void)hipGraphExecDestroy(gexec);
-    if (!all_done)
-    {
-      set_error("conjugate_gradient did not converge on level %d after %d iterations", cfg.lmin + (int)l, it);
+    bool solved = false;
+    if (cfg.branch_solver == QZ_BRANCH_MG && !branch_mg_level(l, solved))
       return false;
-    }
+    if (!solved && !branch_cg_level(l))
+      return false;
     if (!hip_ok(hipMemcpyAsync(d.hnodes.data() + first, d.nodes + first, sizeof(DNode) * count,
                                hipMemcpyDeviceToHost, stream),
                 "node table") ||
-        !hip_ok(hipStreamSynchronize(stream), "cg"))
+        !hip_ok(hipStreamSynchronize(stream), "potential_branches"))
       return false;
     for (int q = first; q < first + count; q++)
     {
@@ -498,6 +454,245 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
     }
   }
   return hip_ok(hipStreamSynchronize(stream), "potential_branches");
+}
+
+// Replay a captured launch sequence (or run it eagerly while kernels are being
+// timed) until done_count reaches `count` or `max_it` iterations have run.
+template <typename P, typename G>
+template <typename F>
+bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done)
+{
+  TreeDev &d = *tree->dev;
+  hipGraphExec_t gexec = nullptr;
+  if (!timer.on)
+  {
+    hipGraph_t g = nullptr;
+    if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
+      return false;
+    batch();
+    if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
+        !hip_ok(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0), "graph instantiate"))
+      return false;
+    (void)hipGraphDestroy(g);
+  }
+  all_done = false;
+  bool ok = true;
+  for (it = 0; it < max_it; it += per_batch)
+  {
+    if (gexec)
+      ok = hip_ok(hipGraphLaunch(gexec, stream), "graph launch");
+    else
+      batch();
+    ok = ok && hip_ok(hipMemcpyAsync(d.hflag_count, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "done") &&
+         hip_ok(hipStreamSynchronize(stream), "done");
+    if (!ok)
+      break;
+    if (*d.hflag_count >= count)
+    {
+      all_done = true;
+      it += per_batch;
+      break;
+    }
+  }
+  if (gexec)
+    (void)hipGraphExecDestroy(gexec);
+  return ok;
+}
+
+// Unpreconditioned CG per node, all nodes of the level batched
+// (conjugate_gradient.c:327-460, interior points only).
+template <typename P, typename G> bool Engine<P, G>::branch_cg_level(size_t l)
+{
+  TreeDev &d = *tree->dev;
+  const int first = d.lfirst[l], count = d.lcount[l];
+  const int nbx = blocks_per_node(d.lpmax[l]);
+  dim3 grid(nbx, count);
+  const double ni = (double)d.lnintr[l];
+  QZ_LAUNCH("branch_fill", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
+  timer.begin(stream);
+  hipLaunchKernelGGL((k_cg_init<G>), grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.phi, (G *)d.rho,
+                     d.cg_r, d.cg_p, d.part2);
+  hipLaunchKernelGGL(k_cg_init_fold, dim3(count), dim3(kBlock), 0, stream, d.nodes, first, d.part2, nbx);
+  timer.end(stream, "cg_init", ni * (4.0 + 8.0 + sizeof(G) + 16.0));
+  QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.done_count, 0, (size_t)2);
+  int *it_base = d.done_count + 1;
+  // one batch = 5 CG iterations (conjugate_gradient.c:379-456), the last one
+  // with the convergence check (iter_between_check_potential_solution = 5).
+  // Algorithmic bytes per interior point: Ap reads idx + p, writes Ap; update
+  // reads idx, phi, r, p, Ap, writes phi, r (+ phi_old on checks); direction
+  // reads idx, r, p, writes p.
+  auto batch = [&]() {
+    for (int il = 0; il < 5; il++)
+    {
+      QZ_LAUNCHB("branch_cg_ap", ni * 20.0, k_cg_ap, grid, d.nodes, first, d.intr_g, d.cg_p, d.cg_Ap, d.part1);
+      QZ_LAUNCHB("branch_cg_update", ni * 52.0, k_cg_update, grid, d.nodes, first, it_base, il, d.intr_g, d.phi,
+                 d.phi_old, d.cg_r, d.cg_p, d.cg_Ap, d.part1, d.part2, d.part3, cfg.tol_change);
+      QZ_LAUNCHB("branch_cg_direction", ni * 28.0, k_cg_direction, grid, d.nodes, first, it_base, il, d.intr_g, d.phi,
+                 d.phi_old, d.cg_r, d.cg_p, d.part2, d.part3, cfg.tol, cfg.G, d.done_count);
+    }
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 5);
+  };
+  int it = 0;
+  bool all_done = false;
+  // cycle_size = min(size, 5000) per node (conjugate_gradient.c:377)
+  if (!iterate_batches(batch, 5, cfg.max_iterations, count, it, all_done))
+    return false;
+  if (!all_done)
+  {
+    set_error("conjugate_gradient did not converge on level %d after %d iterations", cfg.lmin + (int)l, it);
+    return false;
+  }
+  return true;
+}
+
+// Batched masked multigrid for all nodes of a tree level.  Per node the box
+// grid is the finest level, padded on the high side to 2^(M-1) c + 1 points per
+// axis with c <= kBmgCoarseCells; unknowns are the interior points, everything
+// else is Dirichlet.  Outer FP64 iterations: residual of the current phi,
+// convergence test, V-cycle(s) in G precision on the correction, FP64 update
+// with the pointwise change test.  `solved` stays false when a box is too large
+// for the hierarchy or the iteration cap is reached; the caller then runs CG
+// from the current phi.
+template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, bool &solved)
+{
+  solved = false;
+  TreeDev &d = *tree->dev;
+  const int first = d.lfirst[l], count = d.lcount[l];
+  std::vector<BMg> &hb = d.hbmg;
+  hb.assign(count, BMg{});
+  int mmax = 0;
+  long long pool = 0;
+  long long npm[kMaxMgLevels] = {0};   // largest node grid at MG level m
+  double ptsm[kMaxMgLevels] = {0.0};   // points over nodes at MG level m
+  for (int q = 0; q < count; q++)
+  {
+    const HNode &hn = tree->host->nodes[first + q];
+    int M = 0, c[3] = {0, 0, 0};
+    for (int t = 1; t <= kMaxMgLevels && !M; t++)
+    {
+      const int s = 1 << (t - 1);
+      bool fit = true;
+      for (int a = 0; a < 3; a++)
+      {
+        c[a] = (hn.rd[a] + s - 1) / s;
+        fit = fit && c[a] <= kBmgCoarseCells;
+      }
+      if (fit)
+        M = t;
+    }
+    if (!M)
+      return true; // a box too large for the coarsest workgroup: CG solves this level
+    BMg &b = hb[q];
+    b.nlev = M;
+    b.done = 0;
+    for (int m = 0; m < M; m++)
+    {
+      const int f = 1 << (M - 1 - m);
+      b.X[m] = f * c[0] + 1;
+      b.Y[m] = f * c[1] + 1;
+      b.Z[m] = f * c[2] + 1;
+      b.off[m] = pool;
+      const long long np = (long long)b.X[m] * b.Y[m] * b.Z[m];
+      pool += np;
+      npm[m] = std::max(npm[m], np);
+      ptsm[m] += (double)np;
+    }
+    mmax = std::max(mmax, M);
+  }
+  if (pool > d.mg_cap)
+  {
+    (void)hipFree(d.mg_e);
+    (void)hipFree(d.mg_r);
+    (void)hipFree(d.mg_mask);
+    d.mg_e = d.mg_r = nullptr;
+    d.mg_mask = nullptr;
+    d.mg_cap = next_cap(d.mg_cap, pool);
+    if (!hip_ok(hipMalloc(&d.mg_e, sizeof(G) * d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc(&d.mg_r, sizeof(G) * d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc((void **)&d.mg_mask, d.mg_cap), "multigrid pool"))
+      return false;
+  }
+  if (count > d.bmg_cap)
+  {
+    (void)hipFree(d.bmg);
+    d.bmg_cap = std::max(count, 2 * d.bmg_cap);
+    if (!hip_ok(hipMalloc((void **)&d.bmg, sizeof(BMg) * d.bmg_cap), "multigrid table"))
+      return false;
+  }
+  G *e = (G *)d.mg_e, *r = (G *)d.mg_r;
+  unsigned char *mk = d.mg_mask;
+  BMg *bm = d.bmg;
+  const int nbx = blocks_per_node(d.lpmax[l]);
+  const dim3 gpt(nbx, count);
+  // blocks per node for a sweep over n points (about two points per thread)
+  const int cap = std::max(64, 4096 / count);
+  auto gsw = [&](long long n) {
+    return dim3((unsigned)std::min<long long>(cap, std::max<long long>(1, (n + 2 * kBlock - 1) / (2 * kBlock))), count);
+  };
+  if (!hip_ok(hipMemcpyAsync(bm, hb.data(), sizeof(BMg) * count, hipMemcpyHostToDevice, stream), "multigrid table"))
+    return false;
+  QZ_LAUNCH("bmg_setup", k_fill<unsigned char>, nblk(pool), mk, (unsigned char)0, (size_t)pool);
+  QZ_LAUNCH("bmg_setup", k_fill<G>, nblk(pool), e, (G)0, (size_t)pool);
+  QZ_LAUNCH("bmg_setup", k_fill<int>, 1, d.done_count, 0, (size_t)2);
+  QZ_LAUNCH("bmg_setup", k_bmg_mask0, gpt, d.nodes, first, bm, d.intr_g, mk);
+  for (int m = 0; m + 1 < mmax; m++)
+    QZ_LAUNCH("bmg_setup", k_bmg_mask_coarsen, gsw(2 * npm[m + 1]), bm, m, mk, mk);
+
+  const double ni = (double)d.lnintr[l], gs = (double)sizeof(G);
+  const int inner = std::max(1, cfg.inner_cycles);
+  const int pre = std::max(1, cfg.pre_sweeps), post = std::max(0, cfg.post_sweeps);
+  int *it_base = d.done_count + 1;
+  auto sweep = [&](int m, int color, int zs) {
+    // colour half: mask + r + e of that colour, neighbours of the other colour
+    QZ_LAUNCHB("bmg_rbgs", ptsm[m] * (0.5 * (1.0 + 2.0 * gs) + 0.5 * gs), (k_bmg_rbgs<G>), gsw(npm[m]), d.nodes,
+               first, bm, m, e, r, mk, color, zs);
+  };
+  auto vcycle = [&]() {
+    for (int c = 0; c < inner; c++)
+    {
+      for (int m = 0; m + 1 < mmax; m++)
+      {
+        for (int s = 0; s < pre; s++)
+        {
+          sweep(m, 0, s == 0 && (m > 0 || c == 0));
+          sweep(m, 1, 0);
+        }
+        QZ_LAUNCHB("bmg_restrict", ptsm[m] * (2.0 * gs + 1.0) + ptsm[m + 1] * (gs + 1.0), (k_bmg_restrict<G>),
+                   gsw(2 * npm[m + 1]), d.nodes, first, bm, m, e, r, mk, r, mk);
+      }
+      QZ_LAUNCH("bmg_coarse", (k_bmg_coarse<G>), count, d.nodes, first, bm, e, r, mk, 200);
+      for (int m = mmax - 2; m >= 0; m--)
+      {
+        QZ_LAUNCHB("bmg_prolong", ptsm[m] * (1.0 + 2.0 * gs) + ptsm[m + 1] * gs, (k_bmg_prolong<G>), gsw(2 * npm[m]),
+                   bm, m, e, mk, e);
+        for (int s = 0; s < post; s++)
+        {
+          sweep(m, 0, 0);
+          sweep(m, 1, 0);
+        }
+      }
+    }
+  };
+  // residual: idx, phi, rho read, r written; correction: idx, e read, phi read + written
+  auto batch = [&]() {
+    for (int il = 0; il < 2; il++)
+    {
+      QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + 2.0 * gs), (k_bmg_residual<G>), gpt, d.nodes, first, bm, d.intr_g,
+                 d.phi, (const G *)d.rho, r, d.part1);
+      QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part1, d.part2, nbx, cfg.tol,
+                cfg.G, d.done_count);
+      vcycle();
+      QZ_LAUNCHB("bmg_correct", ni * (4.0 + gs + 16.0), (k_bmg_correct<G>), gpt, d.nodes, first, bm, d.intr_g, d.phi,
+                 (const G *)e, cfg.tol_change, d.part2);
+    }
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 2);
+  };
+  int it = 0;
+  bool all_done = false;
+  if (!iterate_batches(batch, 2, kBmgMaxOuter, count, it, all_done))
+    return false;
+  solved = all_done;
+  return true;
 }
 
 template <typename P, typename G> bool Engine<P, G>::branch_grid_acceleration()

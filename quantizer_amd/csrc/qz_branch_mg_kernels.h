@@ -1,0 +1,347 @@
+// Batched masked multigrid for the refinement nodes of one tree level.
+//
+// Each node's box grid ((rd+1)^3 points, rd even) is the finest multigrid
+// level, padded on the high side so every axis has 2^(M-1) * c + 1 points.
+// Unknowns ("active" points) are the node's interior grid points; boundary,
+// simulation-boundary and absent points are Dirichlet (zero correction).  A
+// coarse point is active when the fine point it injects from is active.  All
+// level arrays use the colour-split layout per node (every axis length is odd,
+// so the linear-index parity is the red/black colour).  One launch covers all
+// nodes of the tree level: blockIdx.y is the node.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qz_branch_kernels.h"
+
+namespace qz
+{
+
+constexpr int kMaxMgLevels = 8;
+constexpr int kBmgCoarseCells = 8; // cells per axis of a node's coarsest grid, at most
+constexpr int kBmgMaxOuter = 60;   // outer iterations before the level is handed to CG
+
+struct BMg
+{
+  int X[kMaxMgLevels], Y[kMaxMgLevels], Z[kMaxMgLevels];
+  long long off[kMaxMgLevels]; // offset of the node's level-m block in that level's pool
+  int nlev;
+  int done; // converged: outer iterations used
+};
+
+__device__ inline long long bmg_np(const BMg &b, int m) { return (long long)b.X[m] * b.Y[m] * b.Z[m]; }
+
+// level-0 mask: 1 at the node's interior points (box grid index -> padded index)
+__global__ void k_bmg_mask0(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                            const int *__restrict__ intr_g, unsigned char *__restrict__ mask)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const BMg &b = bm[blockIdx.y];
+  const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
+  const long long np = bmg_np(b, 0), hf = (np + 1) >> 1;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  {
+    int g = intr_g[d.poff + t];
+    int i = g % gX, j = (g / gX) % (d.rd[1] + 1), k = g / gXY;
+    long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
+    mask[b.off[0] + (idx & 1) * hf + (idx >> 1)] = 1;
+  }
+}
+
+// coarse mask by injection: active(I) = active_fine(2I)
+__global__ void k_bmg_mask_coarsen(const BMg *__restrict__ bm, int m, const unsigned char *__restrict__ mf,
+                                   unsigned char *__restrict__ mc)
+{
+  const BMg &b = bm[blockIdx.y];
+  if (m + 1 >= b.nlev)
+    return;
+  const long long npc = bmg_np(b, m + 1), hc = (npc + 1) >> 1;
+  const long long npf = bmg_np(b, m), hf = (npf + 1) >> 1;
+  for (long long t = blockIdx.x * blockDim.x + threadIdx.x; t < npc; t += (long long)gridDim.x * blockDim.x)
+  {
+    int I = (int)(t % b.X[m + 1]);
+    long long JK = t / b.X[m + 1];
+    int J = (int)(JK % b.Y[m + 1]), K = (int)(JK / b.Y[m + 1]);
+    long long f = 2 * I + (long long)b.X[m] * (2 * J + (long long)b.Y[m] * 2 * K);
+    mc[b.off[m + 1] + (t & 1) * hc + (t >> 1)] = mf[b.off[m] + (f & 1) * hf + (f >> 1)];
+  }
+}
+
+// FP64 residual r = rho - Lap(phi) on interior points (conjugate_gradient.c:359-368),
+// written into the level-0 array; per-block partial sums of r^2.
+template <typename G>
+__global__ void k_bmg_residual(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                               const int *__restrict__ intr_g, const double *__restrict__ phi,
+                               const G *__restrict__ rho, G *__restrict__ r0, double *__restrict__ part)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const BMg &b = bm[blockIdx.y];
+  double acc = 0.0;
+  if (!b.done)
+  {
+    const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
+    const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
+    const double *ph = phi + d.goff;
+    const long long np = bmg_np(b, 0), hf = (np + 1) >> 1;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    {
+      int g = intr_g[d.poff + t];
+      double v = (double)rho[d.goff + g] +
+                 inv_h2 * (6.0 * ph[g] - ph[g + 1] - ph[g - 1] - ph[g + gX] - ph[g - gX] - ph[g + gXY] - ph[g - gXY]);
+      acc += v * v;
+      int i = g % gX, j = (g / gX) % (d.rd[1] + 1), k = g / gXY;
+      long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
+      r0[b.off[0] + (idx & 1) * hf + (idx >> 1)] = (G)v;
+    }
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// phi += e on interior points with the pointwise change test (poisson_error.c:36-81).
+template <typename G>
+__global__ void k_bmg_correct(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                              const int *__restrict__ intr_g, double *__restrict__ phi, const G *__restrict__ e0,
+                              double tol_change, double *__restrict__ part)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const BMg &b = bm[blockIdx.y];
+  double viol = 0.0;
+  if (!b.done)
+  {
+    const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
+    const long long np = bmg_np(b, 0), hf = (np + 1) >> 1;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    {
+      int g = intr_g[d.poff + t];
+      int i = g % gX, j = (g / gX) % (d.rd[1] + 1), k = g / gXY;
+      long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
+      double old = phi[d.goff + g];
+      double nw = old + (double)e0[b.off[0] + (idx & 1) * hf + (idx >> 1)];
+      phi[d.goff + g] = nw;
+      double a = nw / old;
+      if ((1.0 - tol_change > a) || (1.0 + tol_change < a) || a != a)
+        viol = 1.0;
+    }
+  }
+  viol = block_reduce<1>(viol);
+  if (threadIdx.x == 0)
+    part[blockIdx.y * gridDim.x + blockIdx.x] = viol;
+}
+
+// Per node: converged when the last correction kept every point within
+// [1-tol_change, 1+tol_change] of its previous value (poisson_error.c:36-81)
+// and the residual meets the CG criterion of conj_grad_error
+// (poisson_error.c:236-288): sum r^2 < nintr (tol 4 pi G rho_mean)^2.
+__global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int first, const int *__restrict__ it_base,
+                            int it_local, const double *__restrict__ part_rs, const double *__restrict__ part_viol,
+                            int nb, double tol, double Gc, int *__restrict__ done_count)
+{
+  BMg &b = bm[blockIdx.x];
+  if (b.done)
+    return;
+  const int outer = *it_base + it_local;
+  const int have_change = outer > 0;
+  DNode &d = nodes[first + blockIdx.x];
+  double rs = 0.0, viol = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+  {
+    rs += part_rs[blockIdx.x * nb + i];
+    if (have_change)
+      viol = fmax(viol, part_viol[blockIdx.x * nb + i]);
+  }
+  rs = block_reduce<0>(rs);
+  viol = block_reduce<1>(viol);
+  if (threadIdx.x == 0)
+  {
+    const double H = 1.0 / (1 << d.lv);
+    const double rhomean = 4.0 * Gc * kPiD * d.mass / ((double)d.ncell * H * H * H);
+    const double aux_tol = (double)d.nintr * (tol * rhomean) * (tol * rhomean);
+    if ((have_change && viol == 0.0 && rs < aux_tol) || d.nintr == 0)
+    {
+      b.done = outer + 1;
+      d.done = outer + 1;
+      atomicAdd(done_count, 1);
+    }
+  }
+}
+
+// masked red-black GS half sweep on Lap e = r at MG level m
+template <typename G>
+__global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, int m,
+                           G *__restrict__ e, const G *__restrict__ r, const unsigned char *__restrict__ mask,
+                           int color, int zero_start)
+{
+  const BMg &b = bm[blockIdx.y];
+  if (b.done || m >= b.nlev - 1)
+    return;
+  const double H = (double)(1 << m) / (double)(1 << nodes[first + blockIdx.y].lv);
+  const G h2 = (G)(H * H);
+  const int X = b.X[m], Y = b.Y[m];
+  const long long XY = (long long)X * Y, np = XY * b.Z[m], hf = (np + 1) >> 1;
+  const long long cnt = color == 0 ? hf : np - hf;
+  G *eb = e + b.off[m];
+  const G *rb_ = r + b.off[m];
+  const unsigned char *mb = mask + b.off[m];
+  const G *o = eb + (color == 0 ? hf : 0);
+  for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < cnt; h += (long long)gridDim.x * blockDim.x)
+  {
+    long long me = color * hf + h;
+    if (!mb[me])
+      continue;
+    long long idx = 2 * h + color;
+    G sum = zero_start ? (G)0
+                       : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + X) >> 1] + o[(idx - X) >> 1] +
+                             o[(idx + XY) >> 1] + o[(idx - XY) >> 1];
+    eb[me] = (sum - h2 * rb_[me]) * (G)(1.0 / 6.0);
+  }
+}
+
+// full-weighting restriction of the masked fine residual to active coarse points
+template <typename G>
+__global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, int m,
+                               const G *__restrict__ ef, const G *__restrict__ rf, const unsigned char *__restrict__ mf,
+                               G *__restrict__ rc, const unsigned char *__restrict__ mc)
+{
+  const BMg &b = bm[blockIdx.y];
+  if (b.done || m + 1 >= b.nlev)
+    return;
+  const double Hf = (double)(1 << m) / (double)(1 << nodes[first + blockIdx.y].lv);
+  const G inv_h2 = (G)(1.0 / (Hf * Hf));
+  const int Xf = b.X[m], Yf = b.Y[m];
+  const long long XYf = (long long)Xf * Yf, npf = XYf * b.Z[m], hf = (npf + 1) >> 1;
+  const int Xc = b.X[m + 1], Yc = b.Y[m + 1];
+  const long long npc = (long long)Xc * Yc * b.Z[m + 1], hc = (npc + 1) >> 1;
+  const G *e = ef + b.off[m];
+  const G *r = rf + b.off[m];
+  const unsigned char *mk = mf + b.off[m];
+  auto P = [&](long long idx) { return (idx & 1) * hf + (idx >> 1); };
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npc; t += (long long)gridDim.x * blockDim.x)
+  {
+    long long cp = (t & 1) * hc + (t >> 1);
+    if (!mc[b.off[m + 1] + cp])
+      continue;
+    int I = (int)(t % Xc);
+    long long JK = t / Xc;
+    int J = (int)(JK % Yc), K = (int)(JK / Yc);
+    G acc = (G)0;
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++)
+        {
+          long long f = (2 * I + di) + (long long)Xf * ((2 * J + dj) + (long long)Yf * (2 * K + dk));
+          long long pf = P(f);
+          if (!mk[pf])
+            continue;
+          G lap = (e[P(f + 1)] + e[P(f - 1)] + e[P(f + Xf)] + e[P(f - Xf)] + e[P(f + XYf)] + e[P(f - XYf)] -
+                   (G)6 * e[pf]) *
+                  inv_h2;
+          G w = (G)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1)) * (G)(1.0 / 64.0);
+          acc += w * (r[pf] - lap);
+        }
+    rc[b.off[m + 1] + cp] = acc;
+  }
+}
+
+// trilinear prolongation of the coarse correction into active fine points
+template <typename G>
+__global__ void k_bmg_prolong(const BMg *__restrict__ bm, int m, G *__restrict__ ef,
+                              const unsigned char *__restrict__ mf, const G *__restrict__ ec)
+{
+  const BMg &b = bm[blockIdx.y];
+  if (b.done || m + 1 >= b.nlev)
+    return;
+  const int Xf = b.X[m], Yf = b.Y[m];
+  const long long npf = (long long)Xf * Yf * b.Z[m], hf = (npf + 1) >> 1;
+  const int Xc = b.X[m + 1], Yc = b.Y[m + 1];
+  const long long npc = (long long)Xc * Yc * b.Z[m + 1], hc = (npc + 1) >> 1;
+  const G *c = ec + b.off[m + 1];
+  auto PC = [&](int i, int j, int k) {
+    long long idx = i + (long long)Xc * (j + (long long)Yc * k);
+    return (idx & 1) * hc + (idx >> 1);
+  };
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npf; t += (long long)gridDim.x * blockDim.x)
+  {
+    long long fp = (t & 1) * hf + (t >> 1);
+    if (!mf[b.off[m] + fp])
+      continue;
+    int i = (int)(t % Xf);
+    long long JK = t / Xf;
+    int j = (int)(JK % Yf), k = (int)(JK / Yf);
+    int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
+    G v = (c[PC(i0, j0, k0)] + c[PC(i1, j0, k0)] + c[PC(i0, j1, k0)] + c[PC(i1, j1, k0)] + c[PC(i0, j0, k1)] +
+           c[PC(i1, j0, k1)] + c[PC(i0, j1, k1)] + c[PC(i1, j1, k1)]) *
+          (G)0.125;
+    ef[b.off[m] + fp] += v;
+  }
+}
+
+// coarsest level: masked CG per node in one workgroup (FP64 in LDS)
+constexpr int kBmgCoarseMax = (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1);
+template <typename G>
+__global__ void __launch_bounds__(256)
+    k_bmg_coarse(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, G *__restrict__ e,
+                 const G *__restrict__ r, const unsigned char *__restrict__ mask, int iters)
+{
+  __shared__ double xs[kBmgCoarseMax], rs[kBmgCoarseMax], ps[kBmgCoarseMax];
+  const BMg &b = bm[blockIdx.x];
+  if (b.done)
+    return;
+  const int m = b.nlev - 1;
+  const int X = b.X[m], Y = b.Y[m];
+  const int XY = X * Y, np = XY * b.Z[m], hf = (np + 1) >> 1;
+  const double H = (double)(1 << m) / (double)(1 << nodes[first + blockIdx.x].lv);
+  const double inv_h2 = 1.0 / (H * H);
+  G *eb = e + b.off[m];
+  const G *rb_ = r + b.off[m];
+  const unsigned char *mb = mask + b.off[m];
+  auto P = [&](int idx) { return (idx & 1) * hf + (idx >> 1); };
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+  {
+    bool in = mb[P(t)] != 0;
+    xs[t] = 0.0;
+    rs[t] = in ? -(double)rb_[P(t)] : 0.0; // A = -Lap (SPD): solve A x = -r
+    ps[t] = rs[t];
+  }
+  __syncthreads();
+  double rr = 0.0;
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+    rr += rs[t] * rs[t];
+  rr = block_reduce<0>(rr);
+  const double rr0 = rr;
+  for (int it = 0; it < iters && rr > 1e-24 * rr0 && rr > 0.0; it++)
+  {
+    double pap = 0.0;
+    double ap_local[(kBmgCoarseMax + 255) / 256];
+    int q = 0;
+    for (int t = threadIdx.x; t < np; t += blockDim.x, q++)
+    {
+      double v = 0.0;
+      if (mb[P(t)])
+        v = inv_h2 * (6.0 * ps[t] - ps[t + 1] - ps[t - 1] - ps[t + X] - ps[t - X] - ps[t + XY] - ps[t - XY]);
+      ap_local[q] = v;
+      pap += ps[t] * v;
+    }
+    pap = block_reduce<0>(pap);
+    double alpha = rr / pap;
+    double rn = 0.0;
+    q = 0;
+    for (int t = threadIdx.x; t < np; t += blockDim.x, q++)
+    {
+      xs[t] += alpha * ps[t];
+      rs[t] -= alpha * ap_local[q];
+      rn += rs[t] * rs[t];
+    }
+    rn = block_reduce<0>(rn);
+    double beta = rn / rr;
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+      ps[t] = rs[t] + beta * ps[t];
+    rr = rn;
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < np; t += blockDim.x)
+    eb[P(t)] = (G)xs[t];
+}
+
+} // namespace qz

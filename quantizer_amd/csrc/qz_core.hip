@@ -46,6 +46,8 @@ bool hip_ok(hipError_t e, const char *what)
 }
 
 static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// blocks of the grid-stride reduction kernels (one partial per block)
+static inline unsigned nred(size_t n) { return std::min<unsigned>(nblk(n), 2048u); }
 
 // ------------------------------------------------------------------ timing
 void Timer::begin(hipStream_t s)
@@ -474,7 +476,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
-    unsigned nb = nblk(npts);
+    unsigned nb = nred(npts);
     for (int c = 0; c < inner; c++)
       vcycle(0, c == 0);
     QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, ph, lev_e[0], n, cfg.tol_change,
@@ -528,7 +530,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     double ninterior = (double)(n - 1) * (n - 1) * (n - 1);
     int inner = cfg.inner_cycles;
     int max_outer = std::max(1, cfg.max_iterations / std::max(1, inner));
-    unsigned nb = nblk(npts);
+    unsigned nb = nred(npts);
     int outers = 0, vcycles = 0;
     double change = 1e300, res = 0.0, scale = rhomean;
     QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0], n,
@@ -670,6 +672,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
   bool potential_branches() override;
+  bool branch_cg_level(size_t l);
+  bool branch_mg_level(size_t l, bool &solved);
+  template <typename F> bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done);
   bool branch_density();
   bool branch_grid_acceleration();
   bool branch_particle_acceleration();
@@ -727,6 +732,7 @@ void qz_default_config(qz_config *c)
   c->inner_cycles = 0;
   c->pre_sweeps = 2;
   c->post_sweeps = 2;
+  c->branch_solver = QZ_BRANCH_CG;
 }
 
 const char *qz_last_error(void) { return qz::g_last_error.c_str(); }

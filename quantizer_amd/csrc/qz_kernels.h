@@ -565,21 +565,22 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
                                  double inv_h2, double *__restrict__ part)
 {
   const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t npts = (size_t)n1 * n1 * n1, hf = rbhalf(npts);
+  const size_t s1 = n1, s2 = (size_t)n1 * n1;
   double sq = 0.0;
-  if (t < (size_t)n1 * n1 * n1)
+  // grid-stride: a bounded number of blocks keeps the second reduction stage short
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (size_t)gridDim.x * blockDim.x)
   {
-    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / s2);
     double res = 0.0;
     if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
     {
-      size_t s1 = n1, s2 = (size_t)n1 * n1;
       double c = phi[t];
       res = (double)rho[t] + inv_h2 * (6.0 * c - phi[t + 1] - phi[t - 1] - phi[t + s1] - phi[t - s1] -
                                        phi[t + s2] - phi[t - s2]);
-      sq = res * res;
+      sq += res * res;
     }
-    r[rbpos(t, rbhalf((size_t)n1 * n1 * n1))] = (G)res;
+    r[rbpos(t, hf)] = (G)res;
   }
   sq = block_reduce<0>(sq);
   if (threadIdx.x == 0)
@@ -588,26 +589,27 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
 
 // Outer correction phi += e (multigrid.c:199-209) with the pointwise change
 // test of poisson_error.c:36-81 folded in: per-block max of a flag
-// (1 where phi/phi_old leaves [1-tol, 1+tol]) and of |phi/phi_old - 1|.
+// (>= 2 where phi/phi_old leaves [1-tol, 1+tol]) and of |phi/phi_old - 1|.
 template <typename G>
 __global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ e, int n, double tol,
                                 double *__restrict__ part)
 {
   const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t npts = (size_t)n1 * n1 * n1, hf = rbhalf(npts), s2 = (size_t)n1 * n1;
   double dev = 0.0;
-  if (t < (size_t)n1 * n1 * n1)
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (size_t)gridDim.x * blockDim.x)
   {
-    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / s2);
     if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
     {
       double old = phi[t];
-      double nw = old + (double)e[rbpos(t, rbhalf((size_t)n1 * n1 * n1))];
+      double nw = old + (double)e[rbpos(t, hf)];
       phi[t] = nw;
       double ratio = nw / old;
-      dev = (1.0 - tol > ratio || 1.0 + tol < ratio) ? 2.0 + fabs(ratio - 1.0) : fabs(ratio - 1.0);
-      if (dev != dev)
-        dev = 1e300; // 0/0: treat as not converged
+      double d = (1.0 - tol > ratio || 1.0 + tol < ratio) ? 2.0 + fabs(ratio - 1.0) : fabs(ratio - 1.0);
+      if (d != d)
+        d = 1e300; // 0/0: treat as not converged
+      dev = fmax(dev, d);
     }
   }
   dev = block_reduce<1>(dev);
