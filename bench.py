@@ -54,10 +54,11 @@ def make_particles(kind, n, lmin, seed=20261203):
     return plummer(n, seed=seed)
 
 
-def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inner):
+def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inner, solver=None):
     from quantizer_amd import core
     t0 = time.time()
-    sim = core.Simulation(parts, lmin, lmax, precision=precision, ref_criterion_ptcl=crit, inner_cycles=inner)
+    sim = core.Simulation(parts, lmin, lmax, precision=precision, ref_criterion_ptcl=crit, inner_cycles=inner,
+                          branch_solver=solver)
     sim.force()
     dt = sim.timestep(cfl=0.5)
     log(f"  prec={precision} setup+force {time.time() - t0:.2f}s dt={dt:.3e} stats={sim.stats()}")
@@ -148,26 +149,29 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=150.0)
     ap.add_argument("--cpu-n", type=int, default=1 << 24, help="particles in the CPU sample")
     ap.add_argument("--inner", type=int, default=0)
+    ap.add_argument("--branch-solver", choices=["cg", "mg"], default=None,
+                    help="refinement-node solver (default: the library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     from quantizer_amd import core
     N, lmin, lmax, crit, kind = WORKLOADS[args.workload]
+    solver = {None: None, "cg": core.BRANCH_CG, "mg": core.BRANCH_MG}[args.branch_solver]
     log(f"workload {args.workload}: N={N} lmin={lmin} lmax={lmax} crit={crit} {kind}")
     t = time.time()
     parts = make_particles(kind, N, lmin)
     log(f"  generated in {time.time() - t:.1f}s")
 
     el_mx, dt, st_mx, rep_mx = run_path(parts, lmin, lmax, crit, core.PREC_MIXED, args.steps, args.warmup,
-                                        args.prof_steps, args.inner)
+                                        args.prof_steps, args.inner, solver)
     value = N * args.steps / el_mx
     ms_step = 1e3 * el_mx / args.steps
     log(f"  mixed: {ms_step:.3f} ms/step, {value:.4e} particle-steps/s")
     fp64 = None
     if not args.skip_fp64:
         el64, _, st64, rep64 = run_path(parts, lmin, lmax, crit, core.PREC_FP64, args.steps, args.warmup, 0,
-                                        args.inner)
+                                        args.inner, solver)
         fp64 = {"value": N * args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
                 "head_vcycles": st64["head_vcycles"], "branch_iterations": st64["branch_iterations"]}
         log(f"  fp64: {fp64['ms_per_step']:.3f} ms/step")
@@ -201,7 +205,8 @@ def main():
         "adjoint_fwd_ratio": None,
         "solver": {"head_outer": st_mx["head_outer"], "head_vcycles": st_mx["head_vcycles"],
                    "head_residual": st_mx["head_residual"], "n_nodes": st_mx["n_nodes"],
-                   "branch_points": st_mx["branch_points"], "branch_iterations": st_mx["branch_iterations"]},
+                   "branch_points": st_mx["branch_points"], "branch_iterations": st_mx["branch_iterations"],
+                   "branch_cg_levels": st_mx["branch_cg_levels"]},
         "roofline": roof,
         "kernels": table,
         "cpu_baseline": cpu,

@@ -69,6 +69,7 @@ typedef struct
   long long branch_intr;      /* interior (unknown) points over all refinement nodes */
   int branch_iterations;      /* max CG iterations over nodes in the last branch solve */
   long long branch_iter_points; /* sum over nodes of iterations x interior points */
+  int branch_cg_levels;       /* tree levels of the last branch solve finished by CG */
 } qz_stats;
 
 void qz_default_config(qz_config *cfg);

@@ -70,6 +70,7 @@ class QzStats(ctypes.Structure):
         ("branch_intr", ctypes.c_longlong),
         ("branch_iterations", ctypes.c_int),
         ("branch_iter_points", ctypes.c_longlong),
+        ("branch_cg_levels", ctypes.c_int),
     ]
 
 

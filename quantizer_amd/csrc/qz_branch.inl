@@ -428,6 +428,7 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
   TreeDev &d = *tree->dev;
   stats.branch_iterations = 0;
   stats.branch_iter_points = 0;
+  stats.branch_cg_levels = 0;
   const int zero_bc = cfg.head_bc == QZ_BC_ZERO ? 1 : 0;
   for (size_t l = 1; l < d.lfirst.size(); l++)
   {
@@ -440,8 +441,12 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
     bool solved = false;
     if (cfg.branch_solver == QZ_BRANCH_MG && !branch_mg_level(l, solved))
       return false;
-    if (!solved && !branch_cg_level(l))
-      return false;
+    if (!solved)
+    {
+      stats.branch_cg_levels++;
+      if (!branch_cg_level(l))
+        return false;
+    }
     if (!hip_ok(hipMemcpyAsync(d.hnodes.data() + first, d.nodes + first, sizeof(DNode) * count,
                                hipMemcpyDeviceToHost, stream),
                 "node table") ||
@@ -548,9 +553,9 @@ template <typename P, typename G> bool Engine<P, G>::branch_cg_level(size_t l)
 // Batched masked multigrid for all nodes of a tree level.  Per node the box
 // grid is the finest level, padded on the high side to 2^(M-1) c + 1 points per
 // axis with c <= kBmgCoarseCells; unknowns are the interior points, everything
-// else is Dirichlet.  Outer FP64 iterations: residual of the current phi,
-// convergence test, V-cycle(s) in G precision on the correction, FP64 update
-// with the pointwise change test.  `solved` stays false when a box is too large
+// else is Dirichlet.  The outer solver is a flexible preconditioned CG in FP64
+// (convergence test of the reference's CG on every iteration), preconditioned
+// by V-cycle(s) in G precision.  `solved` stays false when a box is too large
 // for the hierarchy or the iteration cap is reached; the caller then runs CG
 // from the current phi.
 template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, bool &solved)
@@ -657,10 +662,11 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
           sweep(m, 0, s == 0 && (m > 0 || c == 0));
           sweep(m, 1, 0);
         }
-        QZ_LAUNCHB("bmg_restrict", ptsm[m] * (2.0 * gs + 1.0) + ptsm[m + 1] * (gs + 1.0), (k_bmg_restrict<G>),
+        QZ_LAUNCHB("bmg_restrict", ptsm[m] * (0.5 * (2.0 * gs + 1.0) + 0.5 * gs) + ptsm[m + 1] * (gs + 1.0),
+                   (k_bmg_restrict<G>),
                    gsw(2 * npm[m + 1]), d.nodes, first, bm, m, e, r, mk, r, mk);
       }
-      QZ_LAUNCH("bmg_coarse", (k_bmg_coarse<G>), count, d.nodes, first, bm, e, r, mk, 200);
+      QZ_LAUNCH("bmg_coarse", (k_bmg_coarse<G>), count, d.nodes, first, bm, e, r, mk, 100);
       for (int m = mmax - 2; m >= 0; m--)
       {
         QZ_LAUNCHB("bmg_prolong", ptsm[m] * (1.0 + 2.0 * gs) + ptsm[m + 1] * gs, (k_bmg_prolong<G>), gsw(2 * npm[m]),
@@ -673,17 +679,26 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
       }
     }
   };
-  // residual: idx, phi, rho read, r written; correction: idx, e read, phi read + written
+  // Flexible PCG in FP64 with the V-cycle as preconditioner.  Algorithmic bytes
+  // per interior point: residual reads idx, phi, rho and writes r (FP64 + G);
+  // zdots reads idx, z, r, q; direction reads idx, z, p, writes p; apply reads
+  // idx, p, writes q; update reads idx, phi, p, q, r, writes phi, r (FP64 + G).
+  QZ_LAUNCH("bmg_setup", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
+  QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + gs + 8.0 + gs), (k_bmg_residual<G>), gpt, d.nodes, first, bm, d.intr_g,
+             d.phi, (const G *)d.rho, d.cg_r, r, d.part2);
   auto batch = [&]() {
     for (int il = 0; il < 2; il++)
     {
-      QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + 2.0 * gs), (k_bmg_residual<G>), gpt, d.nodes, first, bm, d.intr_g,
-                 d.phi, (const G *)d.rho, r, d.part1);
-      QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part1, d.part2, nbx, cfg.tol,
+      QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, nbx, cfg.tol,
                 cfg.G, d.done_count);
       vcycle();
-      QZ_LAUNCHB("bmg_correct", ni * (4.0 + gs + 16.0), (k_bmg_correct<G>), gpt, d.nodes, first, bm, d.intr_g, d.phi,
-                 (const G *)e, cfg.tol_change, d.part2);
+      QZ_LAUNCHB("bmg_zdots", ni * (4.0 + gs + 16.0), (k_bmg_zdots<G>), gpt, d.nodes, first, bm, d.intr_g,
+                 (const G *)e, d.cg_r, d.cg_Ap, d.part1, d.part3);
+      QZ_LAUNCHB("bmg_direction", ni * (4.0 + gs + 16.0), (k_bmg_direction<G>), gpt, d.nodes, first, bm, it_base, il,
+                 d.intr_g, (const G *)e, d.cg_p, d.part1, d.part3);
+      QZ_LAUNCHB("bmg_apply", ni * 20.0, k_bmg_apply, gpt, d.nodes, first, bm, d.intr_g, d.cg_p, d.cg_Ap, d.part1);
+      QZ_LAUNCHB("bmg_update", ni * (4.0 + 48.0 + gs), (k_bmg_update<G>), gpt, d.nodes, first, bm, it_base, il,
+                 d.intr_g, d.phi, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
     }
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 2);
   };

@@ -25,7 +25,9 @@ struct BMg
   int X[kMaxMgLevels], Y[kMaxMgLevels], Z[kMaxMgLevels];
   long long off[kMaxMgLevels]; // offset of the node's level-m block in that level's pool
   int nlev;
-  int done; // converged: outer iterations used
+  int done;        // converged: outer iterations used
+  double rz[2];    // (-r).z of the last two iterations
+  double alpha[2]; // step lengths of the last two iterations
 };
 
 __device__ inline long long bmg_np(const BMg &b, int m) { return (long long)b.X[m] * b.Y[m] * b.Z[m]; }
@@ -66,12 +68,28 @@ __global__ void k_bmg_mask_coarsen(const BMg *__restrict__ bm, int m, const unsi
   }
 }
 
-// FP64 residual r = rho - Lap(phi) on interior points (conjugate_gradient.c:359-368),
-// written into the level-0 array; per-block partial sums of r^2.
+// padded colour-split position at MG level 0 of box-grid point g
+__device__ inline long long bmg_pos0(const DNode &d, const BMg &b, int g)
+{
+  const int gX = d.rd[0] + 1, gY = d.rd[1] + 1;
+  const int i = g % gX, j = (g / gX) % gY, k = g / (gX * gY);
+  const long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
+  const long long hf = (bmg_np(b, 0) + 1) >> 1;
+  return b.off[0] + (idx & 1) * hf + (idx >> 1);
+}
+
+// The outer solver is a flexible preconditioned CG in FP64 on A = -Lap with the
+// V-cycle (in G precision) as preconditioner.  Signs: r below is rho - Lap(phi)
+// (the reference's residual, conjugate_gradient.c:359-368); the CG residual of
+// A phi = -rho is -r, and the V-cycle applied to r returns z ~ A^-1 (-r).
+
+// r = rho - Lap(phi) on interior points: FP64 copy, G copy for the V-cycle,
+// per-block partial sums of r^2.
 template <typename G>
 __global__ void k_bmg_residual(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
                                const int *__restrict__ intr_g, const double *__restrict__ phi,
-                               const G *__restrict__ rho, G *__restrict__ r0, double *__restrict__ part)
+                               const G *__restrict__ rho, double *__restrict__ r, G *__restrict__ r0,
+                               double *__restrict__ part)
 {
   const DNode &d = nodes[first + blockIdx.y];
   const BMg &b = bm[blockIdx.y];
@@ -81,16 +99,14 @@ __global__ void k_bmg_residual(const DNode *__restrict__ nodes, int first, const
     const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
     const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
     const double *ph = phi + d.goff;
-    const long long np = bmg_np(b, 0), hf = (np + 1) >> 1;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
     {
       int g = intr_g[d.poff + t];
       double v = (double)rho[d.goff + g] +
                  inv_h2 * (6.0 * ph[g] - ph[g + 1] - ph[g - 1] - ph[g + gX] - ph[g - gX] - ph[g + gXY] - ph[g - gXY]);
       acc += v * v;
-      int i = g % gX, j = (g / gX) % (d.rd[1] + 1), k = g / gXY;
-      long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
-      r0[b.off[0] + (idx & 1) * hf + (idx >> 1)] = (G)v;
+      r[d.goff + g] = v;
+      r0[bmg_pos0(d, b, g)] = (G)v;
     }
   }
   acc = block_reduce<0>(acc);
@@ -98,35 +114,136 @@ __global__ void k_bmg_residual(const DNode *__restrict__ nodes, int first, const
     part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
-// phi += e on interior points with the pointwise change test (poisson_error.c:36-81).
+// partial dot products of the new preconditioned residual z (V-cycle output):
+// a = sum r z, c = sum z q (q = A p of the previous direction)
 template <typename G>
-__global__ void k_bmg_correct(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                              const int *__restrict__ intr_g, double *__restrict__ phi, const G *__restrict__ e0,
-                              double tol_change, double *__restrict__ part)
+__global__ void k_bmg_zdots(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                            const int *__restrict__ intr_g, const G *__restrict__ z0, const double *__restrict__ r,
+                            const double *__restrict__ q, double *__restrict__ part_a, double *__restrict__ part_c)
 {
   const DNode &d = nodes[first + blockIdx.y];
   const BMg &b = bm[blockIdx.y];
-  double viol = 0.0;
+  double a = 0.0, c = 0.0;
   if (!b.done)
   {
-    const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
-    const long long np = bmg_np(b, 0), hf = (np + 1) >> 1;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
     {
       int g = intr_g[d.poff + t];
-      int i = g % gX, j = (g / gX) % (d.rd[1] + 1), k = g / gXY;
-      long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
-      double old = phi[d.goff + g];
-      double nw = old + (double)e0[b.off[0] + (idx & 1) * hf + (idx >> 1)];
-      phi[d.goff + g] = nw;
+      double z = (double)z0[bmg_pos0(d, b, g)];
+      a += r[d.goff + g] * z;
+      c += z * q[d.goff + g];
+    }
+  }
+  a = block_reduce<0>(a);
+  c = block_reduce<0>(c);
+  if (threadIdx.x == 0)
+  {
+    part_a[blockIdx.y * gridDim.x + blockIdx.x] = a;
+    part_c[blockIdx.y * gridDim.x + blockIdx.x] = c;
+  }
+}
+
+// p = z + beta p with the flexible (Polak-Ribiere) beta = -alpha (z.q) / (z_old.(-r_old));
+// beta = 0 on the first iteration.  rz of this iteration goes to slot it & 1.
+template <typename G>
+__global__ void k_bmg_direction(const DNode *__restrict__ nodes, int first, BMg *__restrict__ bm,
+                                const int *__restrict__ it_base, int it_local, const int *__restrict__ intr_g,
+                                const G *__restrict__ z0, double *__restrict__ p, const double *__restrict__ part_a,
+                                const double *__restrict__ part_c)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  BMg &b = bm[blockIdx.y];
+  if (b.done)
+    return;
+  const int it = *it_base + it_local;
+  const int nb = gridDim.x;
+  const double rz = -fold_partials(part_a, blockIdx.y, nb); // (-r).z
+  double beta = 0.0;
+  if (it > 0)
+  {
+    const double zq = fold_partials(part_c, blockIdx.y, nb);
+    const double rz_old = b.rz[(it - 1) & 1];
+    beta = rz_old != 0.0 ? -b.alpha[(it - 1) & 1] * zq / rz_old : 0.0;
+  }
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  {
+    int g = intr_g[d.poff + t];
+    long long o = d.goff + g;
+    p[o] = (double)z0[bmg_pos0(d, b, g)] + beta * p[o];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    b.rz[it & 1] = rz;
+}
+
+// q = A p = -Lap(p) on interior points (p = 0 elsewhere), partial p.q
+__global__ void k_bmg_apply(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                            const int *__restrict__ intr_g, const double *__restrict__ p, double *__restrict__ q,
+                            double *__restrict__ part)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const BMg &b = bm[blockIdx.y];
+  double acc = 0.0;
+  if (!b.done)
+  {
+    const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
+    const int X = dnode_gX(d), XY = dnode_gXY(d);
+    const double *pp = p + d.goff;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    {
+      int g = intr_g[d.poff + t];
+      double v = inv_h2 * (6.0 * pp[g] - pp[g + 1] - pp[g - 1] - pp[g + X] - pp[g - X] - pp[g + XY] - pp[g - XY]);
+      q[d.goff + g] = v;
+      acc += pp[g] * v;
+    }
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// alpha = rz / p.q; phi += alpha p; r += alpha q (r = rho - Lap phi); G copy of r
+// for the next V-cycle; partial r^2 and the pointwise change flag of the update
+// (poisson_error.c:36-81).
+template <typename G>
+__global__ void k_bmg_update(const DNode *__restrict__ nodes, int first, BMg *__restrict__ bm,
+                             const int *__restrict__ it_base, int it_local, const int *__restrict__ intr_g,
+                             double *__restrict__ phi, const double *__restrict__ p, const double *__restrict__ q,
+                             double *__restrict__ r, G *__restrict__ r0, const double *__restrict__ part_pq,
+                             double *__restrict__ part_rs, double *__restrict__ part_viol, double tol_change)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  BMg &b = bm[blockIdx.y];
+  double acc = 0.0, viol = 0.0;
+  if (!b.done)
+  {
+    const int it = *it_base + it_local;
+    const double pq = fold_partials(part_pq, blockIdx.y, gridDim.x);
+    const double alpha = pq != 0.0 ? b.rz[it & 1] / pq : 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    {
+      int g = intr_g[d.poff + t];
+      long long o = d.goff + g;
+      double old = phi[o];
+      double nw = old + alpha * p[o];
+      phi[o] = nw;
       double a = nw / old;
       if ((1.0 - tol_change > a) || (1.0 + tol_change < a) || a != a)
         viol = 1.0;
+      double rr = r[o] + alpha * q[o];
+      r[o] = rr;
+      acc += rr * rr;
+      r0[bmg_pos0(d, b, g)] = (G)rr;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      b.alpha[it & 1] = alpha;
   }
+  acc = block_reduce<0>(acc);
   viol = block_reduce<1>(viol);
   if (threadIdx.x == 0)
-    part[blockIdx.y * gridDim.x + blockIdx.x] = viol;
+  {
+    part_rs[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    part_viol[blockIdx.y * gridDim.x + blockIdx.x] = viol;
+  }
 }
 
 // Per node: converged when the last correction kept every point within
@@ -197,7 +314,11 @@ __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg
   }
 }
 
-// full-weighting restriction of the masked fine residual to active coarse points
+// Full-weighting restriction of the masked fine residual to active coarse
+// points.  It follows a red-black sweep pair that ended on black, so the
+// residual at black fine points is zero and only the 13 red points of the
+// 27-point stencil contribute: the centre (weight 8/64) and the 12 points two
+// offsets away (weight 2/64).
 template <typename G>
 __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, int m,
                                const G *__restrict__ ef, const G *__restrict__ rf, const unsigned char *__restrict__ mf,
@@ -212,10 +333,20 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
   const long long XYf = (long long)Xf * Yf, npf = XYf * b.Z[m], hf = (npf + 1) >> 1;
   const int Xc = b.X[m + 1], Yc = b.Y[m + 1];
   const long long npc = (long long)Xc * Yc * b.Z[m + 1], hc = (npc + 1) >> 1;
-  const G *e = ef + b.off[m];
-  const G *r = rf + b.off[m];
-  const unsigned char *mk = mf + b.off[m];
-  auto P = [&](long long idx) { return (idx & 1) * hf + (idx >> 1); };
+  // red fine points live in the first half, black ones in the second
+  const G *eR = ef + b.off[m], *eB = ef + b.off[m] + hf;
+  const G *rR = rf + b.off[m];
+  const unsigned char *mR = mf + b.off[m];
+  // residual r - Lap e at the red fine point with linear index f (f even)
+  auto red_res = [&](long long f) -> G {
+    const long long h = f >> 1;
+    if (!mR[h])
+      return (G)0;
+    G lap = (eB[(f + 1) >> 1] + eB[(f - 1) >> 1] + eB[(f + Xf) >> 1] + eB[(f - Xf) >> 1] + eB[(f + XYf) >> 1] +
+             eB[(f - XYf) >> 1] - (G)6 * eR[h]) *
+            inv_h2;
+    return rR[h] - lap;
+  };
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npc; t += (long long)gridDim.x * blockDim.x)
   {
     long long cp = (t & 1) * hc + (t >> 1);
@@ -224,22 +355,11 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
     int I = (int)(t % Xc);
     long long JK = t / Xc;
     int J = (int)(JK % Yc), K = (int)(JK / Yc);
-    G acc = (G)0;
-    for (int dk = -1; dk <= 1; dk++)
-      for (int dj = -1; dj <= 1; dj++)
-        for (int di = -1; di <= 1; di++)
-        {
-          long long f = (2 * I + di) + (long long)Xf * ((2 * J + dj) + (long long)Yf * (2 * K + dk));
-          long long pf = P(f);
-          if (!mk[pf])
-            continue;
-          G lap = (e[P(f + 1)] + e[P(f - 1)] + e[P(f + Xf)] + e[P(f - Xf)] + e[P(f + XYf)] + e[P(f - XYf)] -
-                   (G)6 * e[pf]) *
-                  inv_h2;
-          G w = (G)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1)) * (G)(1.0 / 64.0);
-          acc += w * (r[pf] - lap);
-        }
-    rc[b.off[m + 1] + cp] = acc;
+    const long long f = 2 * I + (long long)Xf * (2 * J + (long long)Yf * 2 * K);
+    G edges = red_res(f + 1 + Xf) + red_res(f + 1 - Xf) + red_res(f - 1 + Xf) + red_res(f - 1 - Xf) +
+              red_res(f + 1 + XYf) + red_res(f + 1 - XYf) + red_res(f - 1 + XYf) + red_res(f - 1 - XYf) +
+              red_res(f + Xf + XYf) + red_res(f + Xf - XYf) + red_res(f - Xf + XYf) + red_res(f - Xf - XYf);
+    rc[b.off[m + 1] + cp] = (G)0.125 * red_res(f) + (G)(2.0 / 64.0) * edges;
   }
 }
 
@@ -310,7 +430,7 @@ __global__ void __launch_bounds__(256)
     rr += rs[t] * rs[t];
   rr = block_reduce<0>(rr);
   const double rr0 = rr;
-  for (int it = 0; it < iters && rr > 1e-24 * rr0 && rr > 0.0; it++)
+  for (int it = 0; it < iters && rr > 1e-12 * rr0 && rr > 0.0; it++)
   {
     double pap = 0.0;
     double ap_local[(kBmgCoarseMax + 255) / 256];

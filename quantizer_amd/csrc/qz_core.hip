@@ -732,7 +732,7 @@ void qz_default_config(qz_config *c)
   c->inner_cycles = 0;
   c->pre_sweeps = 2;
   c->post_sweeps = 2;
-  c->branch_solver = QZ_BRANCH_CG;
+  c->branch_solver = QZ_BRANCH_MG;
 }
 
 const char *qz_last_error(void) { return qz::g_last_error.c_str(); }

@@ -39,14 +39,25 @@ def make(kind, n, lmin):
     return clumps(n, lmin, n_clumps=6, seed=21)
 
 
+SOLVERS = [core.BRANCH_CG, core.BRANCH_MG]
+
+
+def check_solver(st, solver):
+    """The multigrid path must finish every tree level without the CG hand-off."""
+    if solver == core.BRANCH_MG:
+        assert st["branch_cg_levels"] == 0, st
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("kind,n,lmin,lmax,crit", CASES)
-def test_amr_geometry_and_fields_fp64(oracle, kind, n, lmin, lmax, crit):
-    print(f"case: amr fp64 {kind} N={n} lmin={lmin} lmax={lmax} crit={crit}", flush=True)
+def test_amr_geometry_and_fields_fp64(oracle, kind, n, lmin, lmax, crit, solver):
+    print(f"case: amr fp64 {kind} N={n} lmin={lmin} lmax={lmax} crit={crit} solver={solver}", flush=True)
     p = make(kind, n, lmin)
     o = oracle.OracleSim(p, lmin, lmax, crit)
     o.force()
     on = o.nodes()
-    with core.Simulation(p, lmin, lmax, precision=core.PREC_FP64, ref_criterion_ptcl=crit) as s:
+    with core.Simulation(p, lmin, lmax, precision=core.PREC_FP64, ref_criterion_ptcl=crit,
+                         branch_solver=solver) as s:
         s.force()
         gn = s.nodes()
         hg = s.head_fields()
@@ -70,19 +81,23 @@ def test_amr_geometry_and_fields_fp64(oracle, kind, n, lmin, lmax, crit):
     for k in (6, 7, 8):
         assert rel(pg[k], po[k]) < 1e-6
     assert st["n_nodes"] == len(on)
+    check_solver(st, solver)
 
 
-@pytest.mark.parametrize("kind,n,lmin,lmax,crit", CASES[:2])
-def test_amr_mixed_precision_fields(oracle, kind, n, lmin, lmax, crit):
-    print(f"case: amr mixed {kind} N={n} lmin={lmin} lmax={lmax}", flush=True)
+@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("kind,n,lmin,lmax,crit", CASES)
+def test_amr_mixed_precision_fields(oracle, kind, n, lmin, lmax, crit, solver):
+    print(f"case: amr mixed {kind} N={n} lmin={lmin} lmax={lmax} solver={solver}", flush=True)
     p = make(kind, n, lmin)
     o = oracle.OracleSim(p, lmin, lmax, crit)
     o.force()
     on = o.nodes()
-    with core.Simulation(p, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+    with core.Simulation(p, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit,
+                         branch_solver=solver) as s:
         s.force()
         gn = s.nodes()
         pg = s.particles()
+        check_solver(s.stats(), solver)
     assert len(gn) == len(on)
     for a, b in zip(gn, on):
         idx = np.concatenate([listed_index(b, k) for k in ("intr", "bdry", "sim")])
@@ -95,13 +110,14 @@ def test_amr_mixed_precision_fields(oracle, kind, n, lmin, lmax, crit):
         assert rel(pg[k], po[k]) < 1e-4
 
 
-def test_amr_steps_fp64(oracle):
-    print("case: amr fp64 3 steps plummer N=8000 lmin=4 lmax=6", flush=True)
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_amr_steps_fp64(oracle, solver):
+    print(f"case: amr fp64 3 steps plummer N=8000 lmin=4 lmax=6 solver={solver}", flush=True)
     p = plummer(8000, seed=9)
     dt = plummer_dt(0.005)
     o = oracle.OracleSim(p, 4, 6, 8)
     o.force()
-    with core.Simulation(p, 4, 6, precision=core.PREC_FP64, ref_criterion_ptcl=8) as s:
+    with core.Simulation(p, 4, 6, precision=core.PREC_FP64, ref_criterion_ptcl=8, branch_solver=solver) as s:
         s.force()
         for _ in range(3):
             o.step(dt)
@@ -113,16 +129,17 @@ def test_amr_steps_fp64(oracle):
         assert rel(pg[k], po[k]) < 1e-6
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("prec", [core.PREC_FP64, core.PREC_MIXED])
-def test_graph_replays_match_eager_launches(prec):
-    """The head solve and branch CG replay captured HIP graphs; with timing on the
-    same kernels are launched eagerly.  Several steps (many replays, with host
-    work and syncs between them) must agree bit for bit."""
-    print(f"case: graph vs eager prec={prec} clumps N=20000 lmin=5 lmax=7", flush=True)
+def test_graph_replays_match_eager_launches(prec, solver):
+    """The head solve and the branch solvers replay captured HIP graphs; with
+    timing on the same kernels are launched eagerly.  Several steps (many
+    replays, with host work and syncs between them) must agree bit for bit."""
+    print(f"case: graph vs eager prec={prec} solver={solver} clumps N=20000 lmin=5 lmax=7", flush=True)
     p = clumps(20000, 5, n_clumps=4, seed=3)
     out = []
     for eager in (False, True):
-        with core.Simulation(p, 5, 7, precision=prec, ref_criterion_ptcl=8) as s:
+        with core.Simulation(p, 5, 7, precision=prec, ref_criterion_ptcl=8, branch_solver=solver) as s:
             s.timing(eager)
             s.force()
             dt = s.timestep()
