@@ -87,7 +87,7 @@ def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inne
 
 
 def roofline_from(report, prof_steps):
-    kernels = {k: v for k, v in report.items() if not k.startswith("_")}
+    kernels = {k: v for k, v in report.items() if not k.startswith("_") and not k.startswith("host:")}
     if not kernels:
         return None, {}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])

@@ -34,6 +34,18 @@ struct TreeDev
   int *head_child = nullptr;
   int *flags = nullptr, *flag_count = nullptr;
   int flag_cap = 0;
+  // head level marks and zones (device): mark per head cell, ordered list of
+  // marked cells, union-find and zone arrays sized by the marked count
+  unsigned char *hmark = nullptr;
+  int *hlist = nullptr;
+  void *hsel_tmp = nullptr;
+  size_t hsel_bytes = 0;
+  int *hpar = nullptr, *hflag = nullptr, *hrank = nullptr, *hidx = nullptr, *hidx_s = nullptr, *hz_node = nullptr,
+      *hz_xyz = nullptr, *hzstart = nullptr, *hbbox = nullptr;
+  unsigned *hzone = nullptr, *hzone_s = nullptr;
+  long long hcap = 0;
+  void *htmp = nullptr;
+  size_t htmp_bytes = 0;
   double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
   long long part_cap = 0;
   double *mpart = nullptr;
@@ -81,7 +93,9 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
                     d->pn_i, d->pn_b, d->pn_s, d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
-                    d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask};
+                    d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask, d->hmark, d->hlist,
+                    d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
+                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp};
     for (void *q : ptrs)
       (void)hipFree(q);
     if (d->hflag_count)
@@ -152,6 +166,109 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
   return true;
 }
 
+// Head level marks and zones on the device (fill_cell_ref + fill_zones_ref for
+// the head, tree_construction.c:527-926): marks from the particle counts of the
+// sorted keys, ordered compaction, union-find zones, a stable sort by zone.  The
+// host receives zone bounding boxes and sizes, plus the zone cells only when the
+// children can refine further.  d.head_child serves as the cell -> list map
+// until the node upload overwrites the zone cells with node indices.
+template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zone_cells)
+{
+  TreeDev &d = *tree->dev;
+  TreeHost &th = *tree->host;
+  const long long ncell = (long long)n * n * n;
+  if (!d.hmark)
+  {
+    d.hsel_bytes = qz_select_temp_bytes((int)ncell);
+    if (!hip_ok(hipMalloc((void **)&d.hmark, (size_t)ncell), "head marks") ||
+        !hip_ok(hipMalloc((void **)&d.hlist, sizeof(int) * (size_t)ncell), "head marks") ||
+        !hip_ok(hipMalloc(&d.hsel_tmp, std::max<size_t>(d.hsel_bytes, 1)), "head marks"))
+      return false;
+  }
+  QZ_LAUNCHB("tree_head_mark", 5.0 * ncell, k_head_mark, nblk(ncell), cell_start, n, cfg.ref_criterion_ptcl, d.hmark);
+  timer.begin(stream);
+  if (!hip_ok(qz_select_flagged_index(d.hsel_tmp, d.hsel_bytes, d.hmark, d.hlist, d.flag_count, (int)ncell, stream),
+              "head marks"))
+    return false;
+  timer.end(stream, "tree_head_select", 5.0 * ncell);
+  if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "marks") ||
+      !hip_ok(hipStreamSynchronize(stream), "marks"))
+    return false;
+  const int M = *d.hflag_count;
+  zone_cells = M;
+  if (M == 0)
+  {
+    th.add_head_zones(0, nullptr, nullptr, nullptr);
+    return true;
+  }
+  if (M + 1 > d.hcap)
+  {
+    long long cap = next_cap(d.hcap, M + 1);
+    int **ints[] = {&d.hpar, &d.hflag, &d.hrank, &d.hidx, &d.hidx_s, &d.hz_node, &d.hzstart};
+    for (int **q : ints)
+    {
+      (void)hipFree(*q);
+      *q = nullptr;
+      if (!hip_ok(hipMalloc((void **)q, sizeof(int) * (size_t)cap), "head zones"))
+        return false;
+    }
+    (void)hipFree(d.hz_xyz);
+    (void)hipFree(d.hbbox);
+    (void)hipFree(d.hzone);
+    (void)hipFree(d.hzone_s);
+    (void)hipFree(d.htmp);
+    d.hz_xyz = d.hbbox = nullptr;
+    d.hzone = d.hzone_s = nullptr;
+    d.htmp = nullptr;
+    d.htmp_bytes = std::max(qz_sort_temp_bytes((int)cap, 32), qz_scan_temp_bytes((int)cap));
+    if (!hip_ok(hipMalloc((void **)&d.hz_xyz, sizeof(int) * 3 * (size_t)cap), "head zones") ||
+        !hip_ok(hipMalloc((void **)&d.hbbox, sizeof(int) * 6 * (size_t)cap), "head zones") ||
+        !hip_ok(hipMalloc((void **)&d.hzone, sizeof(unsigned) * (size_t)cap), "head zones") ||
+        !hip_ok(hipMalloc((void **)&d.hzone_s, sizeof(unsigned) * (size_t)cap), "head zones") ||
+        !hip_ok(hipMalloc(&d.htmp, d.htmp_bytes), "head zones"))
+      return false;
+    d.hcap = cap;
+  }
+  QZ_LAUNCH("tree_head_zones", k_head_index, nblk(M), d.hlist, M, d.head_child, d.hpar);
+  QZ_LAUNCH("tree_head_zones", k_head_hook, nblk(M), d.hlist, M, n, d.head_child, d.hpar);
+  QZ_LAUNCH("tree_head_zones", k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
+  timer.begin(stream);
+  if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "head zones"))
+    return false;
+  timer.end(stream, "tree_head_zones");
+  if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.hrank + M, sizeof(int), hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipStreamSynchronize(stream), "zones"))
+    return false;
+  const int nz = *d.hflag_count;
+  int end_bit = 1;
+  while ((1LL << end_bit) < nz)
+    end_bit++;
+  QZ_LAUNCH("tree_head_zones", k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
+  timer.begin(stream);
+  if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream),
+              "head zones"))
+    return false;
+  timer.end(stream, "tree_head_zones");
+  QZ_LAUNCH("tree_head_zones", k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
+  QZ_LAUNCH("tree_head_zones", k_head_zone_cells, nblk(M), M, d.hzone_s, d.hidx_s, d.hlist, n, 0, d.hzstart, d.hbbox,
+            d.hz_node, d.hz_xyz);
+  std::vector<int> zstart(nz + 1), bbox(6 * (size_t)nz), zsize(nz);
+  const bool need_cells = cfg.lmin + 1 < cfg.lmax;
+  std::vector<int> cells(need_cells ? 3 * (size_t)M : 0);
+  if (!hip_ok(hipMemcpyAsync(zstart.data(), d.hzstart, sizeof(int) * nz, hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipMemcpyAsync(bbox.data(), d.hbbox, sizeof(int) * 6 * nz, hipMemcpyDeviceToHost, stream), "zones") ||
+      (need_cells &&
+       !hip_ok(hipMemcpyAsync(cells.data(), d.hz_xyz, sizeof(int) * 3 * (size_t)M, hipMemcpyDeviceToHost, stream),
+               "zones")) ||
+      !hip_ok(hipStreamSynchronize(stream), "zones"))
+    return false;
+  zstart[nz] = M;
+  for (int z = 0; z < nz; z++)
+    zsize[z] = zstart[z + 1] - zstart[z];
+  th.add_head_zones(nz, bbox.data(), zsize.data(), need_cells ? cells.data() : nullptr);
+  return true;
+}
+
 template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
 {
   if (cfg.lmin == cfg.lmax)
@@ -173,26 +290,19 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         !hip_ok(hipHostMalloc((void **)&d->hflag_count, sizeof(int) * 4), "pinned"))
       return false;
   }
+  HostSpan span_all(timer, "tree_adaptation");
   if (!sort_particles())
     return false;
   TreeHost &th = *tree->host;
   TreeDev &d = *tree->dev;
   const size_t ncell_head = (size_t)n * n * n;
-  // ---- head refinement counts (tree_construction.c:527-575)
   QZ_LAUNCH("tree_fill", k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
-  QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.flag_count, 0, (size_t)1);
-  QZ_LAUNCH("tree_flags", k_flag_head, nblk(ncell_head), cell_start, (int)ncell_head, cfg.ref_criterion_ptcl,
-            d.flag_count, d.flags, d.flag_cap);
-  if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "flags") ||
-      !hip_ok(hipStreamSynchronize(stream), "flags"))
-    return false;
-  int nflag = std::min(*d.hflag_count, d.flag_cap);
-  std::vector<int> flagged(nflag);
-  if (nflag > 0 &&
-      !hip_ok(hipMemcpy(flagged.data(), d.flags, sizeof(int) * nflag, hipMemcpyDeviceToHost), "flag list"))
-    return false;
-  std::sort(flagged.begin(), flagged.end());
-  th.refine_head(flagged);
+  int head_zone_cells = 0;
+  {
+    HostSpan span(timer, "tree_head_zones");
+    if (!refine_head_device(head_zone_cells))
+      return false;
+  }
   d.hnodes.clear();
   d.lfirst.assign(1, 0);
   d.lcount.assign(1, 1);
@@ -211,6 +321,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
   while (lr < (int)th.lev_start.size())
   {
     int first = th.lev_start[lr], count = th.lev_count[lr];
+    HostSpan span_lv(timer, "tree_level_upload");
     // ---- descriptors and zone cells of this level
     long long nz = 0, ncell = 0, nb = 0, npt = 0, ng = 0, pmax = 0;
     for (int q = first; q < first + count; q++)
@@ -225,7 +336,8 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
     }
     if (!reserve_level<G>(&d, nz, ncell, nb, npt, ng))
       return false;
-    std::vector<int> zn(nz), zx(3 * nz);
+    const bool dev_zones = lr == 1; // head zones were built on the device
+    std::vector<int> zn(dev_zones ? 0 : nz), zx(dev_zones ? 0 : 3 * nz);
     long long zo = 0;
     for (int q = first; q < first + count; q++)
     {
@@ -253,7 +365,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       h.boff = dn.boff;
       h.coff = dn.coff;
       h.poff = dn.poff;
-      for (int j = 0; j < h.ncell / 8; j++)
+      for (int j = 0; !dev_zones && j < h.ncell / 8; j++)
       {
         // the zone cell is the parent of octant 0 of child cell 8j
         zn[zo] = q;
@@ -287,12 +399,19 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         return false;
       d.node_cap = (int)cap;
     }
+    if (dev_zones && nz != head_zone_cells)
+    {
+      set_error("tree_adaptation: head zone cells %lld != %d", nz, head_zone_cells);
+      return false;
+    }
     if (!hip_ok(hipMemcpyAsync(d.nodes + first, d.hnodes.data() + first, sizeof(DNode) * count,
                                hipMemcpyHostToDevice, stream),
                 "node table") ||
-        !hip_ok(hipMemcpyAsync(d.zone_node + z0, zn.data(), sizeof(int) * nz, hipMemcpyHostToDevice, stream),
+        !hip_ok(hipMemcpyAsync(d.zone_node + z0, dev_zones ? d.hz_node : zn.data(), sizeof(int) * nz,
+                               dev_zones ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
                 "zones") ||
-        !hip_ok(hipMemcpyAsync(d.zone_xyz + 3 * z0, zx.data(), sizeof(int) * 3 * nz, hipMemcpyHostToDevice, stream),
+        !hip_ok(hipMemcpyAsync(d.zone_xyz + 3 * z0, dev_zones ? d.hz_xyz : zx.data(), sizeof(int) * 3 * nz,
+                               dev_zones ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
                 "zones"))
       return false;
     // ---- device: boxes, cells, ranges, masses, grid-point lists
@@ -349,6 +468,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         fn[i] = pr[i].first;
         fc[i] = pr[i].second;
       }
+      HostSpan span(timer, "tree_refine_level");
       th.refine_level(lr, fn, fc);
     }
     lr++;
@@ -425,6 +545,7 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
 {
   if (!has_tree())
     return true;
+  HostSpan span(timer, "potential_branches");
   TreeDev &d = *tree->dev;
   stats.branch_iterations = 0;
   stats.branch_iter_points = 0;

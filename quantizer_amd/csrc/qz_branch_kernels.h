@@ -365,17 +365,167 @@ __global__ void k_flag_cells(const DNode *__restrict__ nodes, const int *__restr
   }
 }
 
-__global__ void k_flag_head(const int *__restrict__ start, int ncell, int crit, int *__restrict__ count,
-                            int *__restrict__ out, int cap)
+// ------------------------------------------------------------------ head level marks and zones
+// fill_cell_ref for the head (tree_construction.c:527-575, n_exp = 1): a head
+// cell is marked when some cell of its 3x3x3 block inside the box holds at
+// least crit particles.
+__global__ void k_head_mark(const int *__restrict__ start, int n, int crit, unsigned char *__restrict__ mark)
 {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ncell = (long long)n * n * n;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell)
     return;
-  if (start[c + 1] - start[c] >= crit)
+  const int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((long long)n * n));
+  unsigned char m = 0;
+  for (int kk = max(k - 1, 0); kk <= min(k + 1, n - 1) && !m; kk++)
+    for (int jj = max(j - 1, 0); jj <= min(j + 1, n - 1) && !m; jj++)
+    {
+      const long long row = (long long)n * (jj + (long long)n * kk);
+      for (int ii = max(i - 1, 0); ii <= min(i + 1, n - 1); ii++)
+        if (start[row + ii + 1] - start[row + ii] >= crit)
+        {
+          m = 1;
+          break;
+        }
+    }
+  mark[c] = m;
+}
+
+// map[cell] = position of the marked cell in the ordered list; every list entry
+// starts as its own union-find root
+__global__ void k_head_index(const int *__restrict__ list, int M, int *__restrict__ map, int *__restrict__ par)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M)
+    return;
+  map[list[i]] = i;
+  par[i] = i;
+}
+
+// Union-find with parent[x] <= x at all times: roots link only under smaller
+// roots (atomic compare-and-swap on the root), path halving only replaces a
+// parent by one of its ancestors, so racing finds and links are harmless and
+// the final root of every set is its smallest member.
+__device__ inline int uf_find(int *par, int x)
+{
+  while (true)
   {
-    int at = atomicAdd(count, 1);
-    if (at < cap)
-      out[at] = c;
+    const int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+    if (p == x)
+      return x;
+    const int gp = __atomic_load_n(&par[p], __ATOMIC_RELAXED);
+    if (gp != p)
+      __atomic_store_n(&par[x], gp, __ATOMIC_RELAXED);
+    x = gp;
+  }
+}
+
+__device__ inline void uf_union(int *par, int a, int b)
+{
+  while (true)
+  {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b)
+      return;
+    if (a > b)
+    {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicCAS(&par[b], b, a);
+    if (old == b)
+      return;
+    b = old;
+  }
+}
+
+// 6-connected zones of the marked cells (fill_zones_ref, tree_construction.c:748-891):
+// each marked cell joins its marked +x, +y and +z neighbours.  The zone's root is
+// its first cell in cell order, which is the cell the reference's flood fill
+// starts the zone from; zones are numbered by their roots in cell order.
+__global__ void k_head_hook(const int *__restrict__ list, int M, int n, const int *__restrict__ map, int *par)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M)
+    return;
+  const int c = list[i];
+  const int x = c % n, y = (c / n) % n, z = c / (n * n);
+  if (x + 1 < n)
+  {
+    const int j = map[c + 1];
+    if (j >= 0)
+      uf_union(par, i, j);
+  }
+  if (y + 1 < n)
+  {
+    const int j = map[c + n];
+    if (j >= 0)
+      uf_union(par, i, j);
+  }
+  if (z + 1 < n)
+  {
+    const int j = map[c + n * n];
+    if (j >= 0)
+      uf_union(par, i, j);
+  }
+}
+
+// par[i] = root; rootflag[i] = 1 for roots, rootflag[M] = 0 (for the count)
+__global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > M)
+    return;
+  if (i == M)
+  {
+    rootflag[M] = 0;
+    return;
+  }
+  const int r = uf_find(par, i);
+  par[i] = r;
+  rootflag[i] = r == i;
+}
+
+__global__ void k_head_zone_of(int M, const int *__restrict__ par, const int *__restrict__ rank,
+                               unsigned *__restrict__ zone, int *__restrict__ idx)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M)
+    return;
+  zone[i] = (unsigned)rank[par[i]];
+  idx[i] = i;
+}
+
+__global__ void k_bbox_init(int nz, int *__restrict__ bbox)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 6 * nz)
+    bbox[t] = (t % 6) < 3 ? 0x7fffffff : -1;
+}
+
+// After the stable sort by zone: zone start offsets, bounding boxes (head cell
+// coordinates) and the zone-cell arrays the node upload uses (zone order, cell
+// order within a zone, as zones_by_ref lists them: tree_construction.c:906-914).
+__global__ void k_head_zone_cells(int M, const unsigned *__restrict__ zk, const int *__restrict__ ival,
+                                  const int *__restrict__ list, int n, int first, int *__restrict__ zstart,
+                                  int *__restrict__ bbox, int *__restrict__ zone_node, int *__restrict__ zone_xyz)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M)
+    return;
+  const unsigned z = zk[t];
+  if (t == 0 || zk[t - 1] != z)
+    zstart[z] = t;
+  const int c = list[ival[t]];
+  const int xyz[3] = {c % n, (c / n) % n, c / (n * n)};
+  zone_node[t] = first + (int)z;
+  for (int a = 0; a < 3; a++)
+  {
+    zone_xyz[3 * t + a] = xyz[a];
+    atomicMin(&bbox[6 * z + a], xyz[a]);
+    atomicMax(&bbox[6 * z + 3 + a], xyz[a]);
   }
 }
 

@@ -71,6 +71,19 @@ void Timer::end(hipStream_t s, const char *name, double bytes)
   pending.pop_back();
 }
 
+void Timer::host(const char *name, double ms)
+{
+  std::string nm = std::string("host:") + name;
+  auto it = std::find_if(acc.begin(), acc.end(), [&](const Acc &x) { return x.name == nm; });
+  if (it == acc.end())
+    acc.push_back({nm, ms, 1, 0.0});
+  else
+  {
+    it->ms += ms;
+    it->count++;
+  }
+}
+
 void Timer::collect()
 {
   for (auto &r : recs)
@@ -573,7 +586,11 @@ template <typename P, typename G> struct Engine final : EngineBase
     return ok;
   }
 
-  bool potential_head() override { return solve_head(phi, rho, false); }
+  bool potential_head() override
+  {
+    HostSpan span(timer, "potential_head");
+    return solve_head(phi, rho, false);
+  }
 
   bool grid_acceleration() override
   {
@@ -671,6 +688,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   // ---------------------------------------------------------------- branches (qz_branch.inl)
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
+  bool refine_head_device(int &zone_cells);
   bool potential_branches() override;
   bool branch_cg_level(size_t l);
   bool branch_mg_level(size_t l, bool &solved);

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,21 @@ struct Timer
   void begin(hipStream_t s);
   void end(hipStream_t s, const char *name, double bytes = 0.0);
   void collect();
+  void host(const char *name, double ms); // host wall time of a stage part
+};
+
+// Host wall time of a scope, recorded as "host:<name>" while timing is on.
+struct HostSpan
+{
+  Timer &t;
+  const char *name;
+  std::chrono::steady_clock::time_point t0;
+  HostSpan(Timer &t_, const char *n) : t(t_), name(n), t0(std::chrono::steady_clock::now()) {}
+  ~HostSpan()
+  {
+    if (t.on)
+      t.host(name, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
 };
 
 struct TreeHost; // host tree (qz_tree.h)

@@ -1,5 +1,7 @@
 // Host side of the refinement tree (see qz_tree.h).  Restates the marking,
-// zoning and child-box steps of tree_construction.c (non-periodic boundaries):
+// zoning and child-box steps of tree_construction.c (non-periodic boundaries)
+// for nodes below the head; the head level's marks and zones are found on the
+// device and arrive here as zone bounding boxes:
 //   fill_cell_ref       tree_construction.c:508-598
 //   fill_zones_ref      tree_construction.c:685-926
 //   create_child_nodes  tree_construction.c:1032-1661 (box geometry part)
@@ -20,17 +22,6 @@ void TreeHost::init(int lmin_, int lmax_, int crit)
   n = 1 << lmin;
   ref_crit = crit;
   extra = 10; // min_box_extra_size_per_side_real for n_exp = 1 (global_variables.c:294-295)
-  int side = n + 2;
-  head_box.assign((size_t)side * side * side, -3);
-  for (int k = 0; k < side; k++)
-    for (int j = 0; j < side; j++)
-      for (int i = 0; i < side; i++)
-      {
-        bool border = i == 0 || i == side - 1 || j == 0 || j == side - 1 || k == 0 || k == side - 1;
-        if (border)
-          head_box[(size_t)i + side * ((size_t)j + side * k)] = -5; // initialization.c:96-113
-      }
-  head_marked.clear();
   nodes.clear();
   lev_start.assign(1, 0);
   lev_count.assign(1, 1);
@@ -38,10 +29,6 @@ void TreeHost::init(int lmin_, int lmax_, int crit)
 
 void TreeHost::clear()
 {
-  for (int b : head_marked)
-    head_box[b] = -3;
-  head_marked.clear();
-  head_child.clear();
   nodes.clear();
   lev_start.assign(1, 0);
   lev_count.assign(1, 1);
@@ -130,130 +117,109 @@ void TreeHost::build_node_box(HNode &nd)
   }
 }
 
-// create_child_nodes box geometry (tree_construction.c:1092-1661, non-pbc paths)
+// create_child_nodes box geometry (tree_construction.c:1092-1661, non-pbc paths):
+// the child box spans the zone's cells refined once, plus `extra` cells split
+// over both sides.
+HNode TreeHost::child_geometry(int parent_global, int parent_lv, int zone, const int *bmin, const int *bmax,
+                               int nzcell) const
+{
+  HNode ch;
+  ch.lv = parent_lv + 1;
+  ch.parent = parent_global;
+  ch.zone = zone;
+  for (int a = 0; a < 3; a++)
+    if (bmin[a] == 0 || bmax[a] == (1 << parent_lv) - 1)
+    {
+      ch.sim_contact = true;
+      ch.sc[a] = true;
+    }
+  for (int a = 0; a < 3; a++)
+  {
+    ch.bmin[a] = 2 * bmin[a];
+    ch.bmax[a] = 2 * bmax[a] + 1;
+    int dim = ch.bmax[a] - ch.bmin[a] + 1;
+    ch.rd[a] = dim + extra;
+    ch.ts[a] = ch.bmin[a] - (ch.rd[a] - dim) / 2;
+  }
+  ch.ncell = 8 * nzcell;
+  ch.bcap = ch.rd[0] * ch.rd[1] * ch.rd[2];
+  ch.gcap = (ch.rd[0] + 1) * (ch.rd[1] + 1) * (ch.rd[2] + 1);
+  ch.pcap = (ch.bmax[0] - ch.bmin[0] + 2) * (ch.bmax[1] - ch.bmin[1] + 2) * (ch.bmax[2] - ch.bmin[2] + 2);
+  return ch;
+}
+
+// Child cells: 8 per zone cell in zone order, octant order x fastest
+// (tree_construction.c:1200-1260), with their box indices; the box status of
+// nodes that can refine further.
+void TreeHost::child_cells(HNode &ch, const int *zxyz, int nzcell)
+{
+  ch.cxyz.resize(3 * (size_t)ch.ncell);
+  ch.cbox.resize(ch.ncell);
+  int rX = ch.rd[0], rXY = ch.rd[0] * ch.rd[1];
+  for (int j = 0; j < nzcell; j++)
+  {
+    int ax = 2 * zxyz[3 * j], ay = 2 * zxyz[3 * j + 1], az = 2 * zxyz[3 * j + 2];
+    for (int kk = 0; kk < 2; kk++)
+      for (int jj = 0; jj < 2; jj++)
+        for (int ii = 0; ii < 2; ii++)
+        {
+          int c = ii + 2 * jj + 4 * kk + 8 * j;
+          int gx = ax + ii, gy = ay + jj, gz = az + kk;
+          ch.cxyz[3 * c] = gx;
+          ch.cxyz[3 * c + 1] = gy;
+          ch.cxyz[3 * c + 2] = gz;
+          ch.cbox[c] = (gx - ch.ts[0]) + (gy - ch.ts[1]) * rX + (gz - ch.ts[2]) * rXY;
+        }
+  }
+  if (ch.lv < lmax)
+    build_node_box(ch);
+}
+
 int TreeHost::make_children(int parent_global, int parent_lv, const int *pcxyz, const std::vector<int> &zone_sizes,
                             const std::vector<std::vector<int>> &zone_cells)
 {
   int nz = (int)zone_sizes.size();
   int first = (int)nodes.size();
+  std::vector<int> zx;
   for (int z = 0; z < nz; z++)
   {
-    HNode ch;
-    ch.lv = parent_lv + 1;
-    ch.parent = parent_global;
-    ch.zone = z;
-    for (int a = 0; a < 3; a++)
-    {
-      ch.bmin[a] = 1 << 30;
-      ch.bmax[a] = -(1 << 30);
-    }
-    for (int pos : zone_cells[z])
+    int bmin[3] = {1 << 30, 1 << 30, 1 << 30}, bmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    const int m = (int)zone_cells[z].size();
+    zx.resize(3 * (size_t)m);
+    for (int t = 0; t < m; t++)
       for (int a = 0; a < 3; a++)
       {
-        int v = pcxyz[3 * pos + a];
-        ch.bmin[a] = std::min(ch.bmin[a], v);
-        ch.bmax[a] = std::max(ch.bmax[a], v);
+        int v = pcxyz[3 * zone_cells[z][t] + a];
+        zx[3 * t + a] = v;
+        bmin[a] = std::min(bmin[a], v);
+        bmax[a] = std::max(bmax[a], v);
       }
-    for (int a = 0; a < 3; a++)
-      if (ch.bmin[a] == 0 || ch.bmax[a] == (1 << parent_lv) - 1)
-      {
-        ch.sim_contact = true;
-        ch.sc[a] = true;
-      }
-    for (int a = 0; a < 3; a++)
-    {
-      ch.bmin[a] *= 2;
-      ch.bmax[a] = 2 * ch.bmax[a] + 1;
-      int dim = ch.bmax[a] - ch.bmin[a] + 1;
-      ch.rd[a] = dim + extra;
-      ch.ts[a] = ch.bmin[a] - (ch.rd[a] - dim) / 2;
-    }
-    ch.ncell = 8 * (int)zone_cells[z].size();
-    ch.cxyz.resize(3 * (size_t)ch.ncell);
-    ch.cbox.resize(ch.ncell);
-    int rX = ch.rd[0], rXY = ch.rd[0] * ch.rd[1];
-    int j = 0;
-    for (int pos : zone_cells[z])
-    {
-      int ax = 2 * pcxyz[3 * pos], ay = 2 * pcxyz[3 * pos + 1], az = 2 * pcxyz[3 * pos + 2];
-      for (int kk = 0; kk < 2; kk++)
-        for (int jj = 0; jj < 2; jj++)
-          for (int ii = 0; ii < 2; ii++)
-          {
-            int c = ii + 2 * jj + 4 * kk + 8 * j;
-            int gx = ax + ii, gy = ay + jj, gz = az + kk;
-            ch.cxyz[3 * c] = gx;
-            ch.cxyz[3 * c + 1] = gy;
-            ch.cxyz[3 * c + 2] = gz;
-            ch.cbox[c] = (gx - ch.ts[0]) + (gy - ch.ts[1]) * rX + (gz - ch.ts[2]) * rXY;
-          }
-      j++;
-    }
-    ch.bcap = ch.rd[0] * ch.rd[1] * ch.rd[2];
-    ch.gcap = (ch.rd[0] + 1) * (ch.rd[1] + 1) * (ch.rd[2] + 1);
-    ch.pcap = (ch.bmax[0] - ch.bmin[0] + 2) * (ch.bmax[1] - ch.bmin[1] + 2) * (ch.bmax[2] - ch.bmin[2] + 2);
-    if (ch.lv < lmax)
-      build_node_box(ch);
+    HNode ch = child_geometry(parent_global, parent_lv, z, bmin, bmax, m);
+    child_cells(ch, zx.data(), m);
     nodes.push_back(std::move(ch));
   }
   return first;
 }
 
-int TreeHost::refine_head(const std::vector<int> &flagged)
+int TreeHost::add_head_zones(int nz, const int *bbox6, const int *zsize, const int *cells)
 {
   clear();
-  const int side = n + 2, s2 = side * side;
-  auto box_of = [&](int c) {
-    int i = c % n, j = (c / n) % n, k = c / (n * n);
-    return (i + 1) + side * ((j + 1) + side * (k + 1));
-  };
-  // fill_cell_ref (tree_construction.c:527-575), n_exp = 1
-  for (int c : flagged)
+  const int first = (int)nodes.size();
+  size_t at = 0;
+  for (int z = 0; z < nz; z++)
   {
-    int b = box_of(c);
-    if (head_box[b] == -3)
-    {
-      head_box[b] = -1;
-      head_marked.push_back(b);
-    }
-    for (int kk = -1; kk <= 1; kk++)
-      for (int jj = -1; jj <= 1; jj++)
-        for (int ii = -1; ii <= 1; ii++)
-        {
-          int nb = b + ii + jj * side + kk * s2;
-          if (head_box[nb] == -3)
-          {
-            head_box[nb] = -1;
-            head_marked.push_back(nb);
-          }
-        }
+    HNode ch = child_geometry(-1, lmin, z, bbox6 + 6 * z, bbox6 + 6 * z + 3, zsize[z]);
+    if (cells)
+      child_cells(ch, cells + 3 * at, zsize[z]);
+    at += zsize[z];
+    nodes.push_back(std::move(ch));
   }
-  // cell_ref in cell order: head box order equals cell order
-  std::vector<int> ref_box = head_marked;
-  std::sort(ref_box.begin(), ref_box.end());
-  std::vector<int> zone_sizes;
-  std::vector<std::vector<int>> zones;
-  zone_fill(head_box, side, s2, ref_box, zone_sizes, zones);
-  // parent cell coordinates of the cell_ref entries
-  std::vector<int> pc(3 * ref_box.size());
-  for (size_t q = 0; q < ref_box.size(); q++)
-  {
-    int b = ref_box[q];
-    pc[3 * q] = b % side - 1;
-    pc[3 * q + 1] = (b / side) % side - 1;
-    pc[3 * q + 2] = b / s2 - 1;
-  }
-  int first = make_children(-1, lmin, pc.data(), zone_sizes, zones);
-  int cnt = (int)nodes.size() - first;
-  head_child.assign(cnt, 0);
-  for (int z = 0; z < cnt; z++)
-    head_child[z] = first + z;
-  if (cnt > 0)
+  if (nz > 0)
   {
     lev_start.push_back(first);
-    lev_count.push_back(cnt);
+    lev_count.push_back(nz);
   }
-  return cnt;
+  return nz;
 }
 
 int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell)

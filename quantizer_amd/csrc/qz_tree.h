@@ -38,21 +38,22 @@ struct HNode
 struct TreeHost
 {
   int lmin = 0, lmax = 0, n = 0, ref_crit = 8, extra = 10;
-  std::vector<int> head_box;    // (n+2)^3 status, ts = -1 (initialization.c:96-113)
-  std::vector<int> head_marked; // box indices changed by the last build
-  std::vector<int> head_child;  // global node index per refined head box cell (by zone)
   std::vector<HNode> nodes;     // refinement nodes, level order (tentacles order, tree_construction.c:2465-2495)
   std::vector<int> lev_start, lev_count; // relative level 1.. -> ranges in nodes
 
   void init(int lmin_, int lmax_, int crit);
-  // Head level: flagged head cells (linear index i + n (j + n k), ascending).
-  // Creates the level-1 nodes; returns their count.
-  int refine_head(const std::vector<int> &flagged);
+  // Head level, zones found on the device: per zone the bounding box of its
+  // head cells (min xyz, max xyz) and its cell count; `cells` holds the head
+  // cell coordinates of every zone in zone order (needed only when the
+  // children can refine further, else null).  Returns the level-1 node count.
+  int add_head_zones(int nz, const int *bbox6, const int *zsize, const int *cells);
   // Level lr >= 1: flagged cells as (node position in level, cell position), sorted.
   int refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell);
   void clear();
 
 private:
+  HNode child_geometry(int parent_global, int parent_lv, int zone, const int *bmin, const int *bmax, int nzcell) const;
+  void child_cells(HNode &ch, const int *zone_cell_xyz, int nzcell);
   int make_children(int parent_global, int parent_lv, const int *cell_xyz_parent, const std::vector<int> &zone_sizes,
                     const std::vector<std::vector<int>> &zones);
   void build_node_box(HNode &nd);
