@@ -145,7 +145,9 @@ int qz_keys(const double *x, const double *y, const double *z, int n, int lv, co
 /* device timing (HIP events on the context's stream) of every kernel launched
  * since qz_timing_enable(ctx, 1): per kernel name the total ms, the launch
  * count and the algorithmic bytes (each input read once, each output written
- * once), up to max entries; returns the number of entries */
+ * once), up to max entries; returns the number of entries.  Timing launches
+ * kernels one by one; qz_timing_enable(ctx, 2) keeps the captured graphs and
+ * records only host wall-time spans ("host:" names).  0 turns timing off. */
 int qz_timing_enable(qz_ctx *ctx, int on);
 int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, double *bytes, int max);
 int qz_sync(qz_ctx *ctx);

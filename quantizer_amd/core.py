@@ -325,8 +325,9 @@ class Simulation:
         return ox.reshape(3, self.n), ov.reshape(3, self.n)
 
     # ------------------------------------------------------------ timing
-    def timing(self, on: bool = True):
-        return _check(load_library().qz_timing_enable(self._h, 1 if on else 0), "qz_timing_enable")
+    def timing(self, on: bool = True, host_only: bool = False):
+        mode = (2 if host_only else 1) if on else 0
+        return _check(load_library().qz_timing_enable(self._h, mode), "qz_timing_enable")
 
     def timing_report(self) -> dict:
         lib = load_library()

@@ -87,10 +87,10 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
     QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
                pos[1], pos[2], adj_slot, N, cfg.lmin, 3 * D, n - 1, tails);
     QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>),
-               nblk(npts), cell_start, tails, adj_gbar[a], n, 1.0);
+               nblk(npts), cell_start, tails, adj_gbar[a], make_cube(n + 1), 1.0);
   }
   QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
-             adj_gbar[2], adj_rhs, n, 1.0 / (2.0 * H));
+             adj_gbar[2], adj_rhs, make_cube(n + 1), 1.0 / (2.0 * H));
   // Lap psi = phibar with zero faces: the same multigrid, adjoint convergence test
   QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
   if (!solve_head(adj_psi, adj_rhs, true))

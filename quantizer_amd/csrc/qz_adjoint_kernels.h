@@ -32,13 +32,14 @@ __global__ void k_by_id_to_slot(const double *__restrict__ src, const int *__res
 // each term present when that neighbour is an interior point; zero on the faces.
 template <typename G>
 __global__ void k_grad_transpose(const G *__restrict__ gx, const G *__restrict__ gy, const G *__restrict__ gz,
-                                 G *__restrict__ out, int n, double inv_2h)
+                                 G *__restrict__ out, Cube cb, double inv_2h)
 {
-  const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n1 * n1 * n1)
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)n1 * n1 * n1)
     return;
-  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  int i, j, k;
+  cube_ijk(cb, t, i, j, k);
   if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
   {
     out[t] = (G)0;

@@ -592,6 +592,7 @@ bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int cou
   hipGraphExec_t gexec = nullptr;
   if (!timer.on)
   {
+    HostSpan span(timer, "graph_instantiate");
     hipGraph_t g = nullptr;
     if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
       return false;
@@ -711,6 +712,8 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
     BMg &b = hb[q];
     b.nlev = M;
     b.done = 0;
+    b.gx = make_fdiv((unsigned)(hn.rd[0] + 1));
+    b.gxy = make_fdiv((unsigned)((hn.rd[0] + 1) * (hn.rd[1] + 1)));
     for (int m = 0; m < M; m++)
     {
       const int f = 1 << (M - 1 - m);
@@ -718,6 +721,8 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
       b.Y[m] = f * c[1] + 1;
       b.Z[m] = f * c[2] + 1;
       b.off[m] = pool;
+      b.dx[m] = make_fdiv((unsigned)b.X[m]);
+      b.dxy[m] = make_fdiv((unsigned)b.X[m] * (unsigned)b.Y[m]);
       const long long np = (long long)b.X[m] * b.Y[m] * b.Z[m];
       pool += np;
       npm[m] = std::max(npm[m], np);

@@ -513,20 +513,48 @@ __global__ void k_head_zone_cells(int M, const unsigned *__restrict__ zk, const 
                                   int *__restrict__ bbox, int *__restrict__ zone_node, int *__restrict__ zone_xyz)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M)
-    return;
-  const unsigned z = zk[t];
-  if (t == 0 || zk[t - 1] != z)
-    zstart[z] = t;
-  const int c = list[ival[t]];
-  const int xyz[3] = {c % n, (c / n) % n, c / (n * n)};
-  zone_node[t] = first + (int)z;
-  for (int a = 0; a < 3; a++)
+  const bool live = t < M;
+  const unsigned z = live ? zk[t] : 0xffffffffu;
+  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+  if (live)
   {
-    zone_xyz[3 * t + a] = xyz[a];
-    atomicMin(&bbox[6 * z + a], xyz[a]);
-    atomicMax(&bbox[6 * z + 3 + a], xyz[a]);
+    if (t == 0 || zk[t - 1] != z)
+      zstart[z] = t;
+    const int c = list[ival[t]];
+    const int xyz[3] = {c % n, (c / n) % n, c / (n * n)};
+    zone_node[t] = first + (int)z;
+    for (int a = 0; a < 3; a++)
+    {
+      zone_xyz[3 * t + a] = xyz[a];
+      lo[a] = hi[a] = xyz[a];
+    }
   }
+  // segmented min/max over the wavefront's runs of equal zone (the list is
+  // sorted by zone), then one atomic per run and axis instead of one per cell
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1)
+  {
+    const unsigned oz = __shfl_down(z, off, 64);
+    int olo[3], ohi[3];
+    for (int a = 0; a < 3; a++)
+    {
+      olo[a] = __shfl_down(lo[a], off, 64);
+      ohi[a] = __shfl_down(hi[a], off, 64);
+    }
+    if (lane + off < 64 && oz == z)
+      for (int a = 0; a < 3; a++)
+      {
+        lo[a] = min(lo[a], olo[a]);
+        hi[a] = max(hi[a], ohi[a]);
+      }
+  }
+  const unsigned pz = __shfl_up(z, 1, 64);
+  if (live && (lane == 0 || pz != z))
+    for (int a = 0; a < 3; a++)
+    {
+      atomicMin(&bbox[6 * z + a], lo[a]);
+      atomicMax(&bbox[6 * z + 3 + a], hi[a]);
+    }
 }
 
 // Refined cells point at their child node: head_child[c] / box[parent] = child index.

@@ -24,6 +24,8 @@ struct BMg
 {
   int X[kMaxMgLevels], Y[kMaxMgLevels], Z[kMaxMgLevels];
   long long off[kMaxMgLevels]; // offset of the node's level-m block in that level's pool
+  FDiv dx[kMaxMgLevels], dxy[kMaxMgLevels]; // divide by X[m], by X[m] Y[m]
+  FDiv gx, gxy;                              // divide by the box grid's rd_x + 1, (rd_x + 1)(rd_y + 1)
   int nlev;
   int done;        // converged: outer iterations used
   double rz[2];    // (-r).z of the last two iterations
@@ -43,7 +45,8 @@ __global__ void k_bmg_mask0(const DNode *__restrict__ nodes, int first, const BM
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
   {
     int g = intr_g[d.poff + t];
-    int i = g % gX, j = (g / gX) % (d.rd[1] + 1), k = g / gXY;
+    const int q = (int)fdiv((unsigned)g, b.gx), k = (int)fdiv((unsigned)g, b.gxy);
+    const int i = g - q * gX, j = q - k * (d.rd[1] + 1);
     long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
     mask[b.off[0] + (idx & 1) * hf + (idx >> 1)] = 1;
   }
@@ -60,9 +63,8 @@ __global__ void k_bmg_mask_coarsen(const BMg *__restrict__ bm, int m, const unsi
   const long long npf = bmg_np(b, m), hf = (npf + 1) >> 1;
   for (long long t = blockIdx.x * blockDim.x + threadIdx.x; t < npc; t += (long long)gridDim.x * blockDim.x)
   {
-    int I = (int)(t % b.X[m + 1]);
-    long long JK = t / b.X[m + 1];
-    int J = (int)(JK % b.Y[m + 1]), K = (int)(JK / b.Y[m + 1]);
+    const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
+    const int I = (int)t - JK * b.X[m + 1], J = JK - K * b.Y[m + 1];
     long long f = 2 * I + (long long)b.X[m] * (2 * J + (long long)b.Y[m] * 2 * K);
     mc[b.off[m + 1] + (t & 1) * hc + (t >> 1)] = mf[b.off[m] + (f & 1) * hf + (f >> 1)];
   }
@@ -72,7 +74,8 @@ __global__ void k_bmg_mask_coarsen(const BMg *__restrict__ bm, int m, const unsi
 __device__ inline long long bmg_pos0(const DNode &d, const BMg &b, int g)
 {
   const int gX = d.rd[0] + 1, gY = d.rd[1] + 1;
-  const int i = g % gX, j = (g / gX) % gY, k = g / (gX * gY);
+  const int q = (int)fdiv((unsigned)g, b.gx), k = (int)fdiv((unsigned)g, b.gxy);
+  const int i = g - q * gX, j = q - k * gY;
   const long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
   const long long hf = (bmg_np(b, 0) + 1) >> 1;
   return b.off[0] + (idx & 1) * hf + (idx >> 1);
@@ -352,9 +355,8 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
     long long cp = (t & 1) * hc + (t >> 1);
     if (!mc[b.off[m + 1] + cp])
       continue;
-    int I = (int)(t % Xc);
-    long long JK = t / Xc;
-    int J = (int)(JK % Yc), K = (int)(JK / Yc);
+    const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
+    const int I = (int)t - JK * Xc, J = JK - K * Yc;
     const long long f = 2 * I + (long long)Xf * (2 * J + (long long)Yf * 2 * K);
     G edges = red_res(f + 1 + Xf) + red_res(f + 1 - Xf) + red_res(f - 1 + Xf) + red_res(f - 1 - Xf) +
               red_res(f + 1 + XYf) + red_res(f + 1 - XYf) + red_res(f - 1 + XYf) + red_res(f - 1 - XYf) +
@@ -385,9 +387,8 @@ __global__ void k_bmg_prolong(const BMg *__restrict__ bm, int m, G *__restrict__
     long long fp = (t & 1) * hf + (t >> 1);
     if (!mf[b.off[m] + fp])
       continue;
-    int i = (int)(t % Xf);
-    long long JK = t / Xf;
-    int j = (int)(JK % Yf), k = (int)(JK / Yf);
+    const int JK = (int)fdiv((unsigned)t, b.dx[m]), k = (int)fdiv((unsigned)t, b.dxy[m]);
+    const int i = (int)t - JK * Xf, j = JK - k * Yf;
     int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
     int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
     G v = (c[PC(i0, j0, k0)] + c[PC(i1, j0, k0)] + c[PC(i0, j1, k0)] + c[PC(i1, j1, k0)] + c[PC(i0, j0, k1)] +

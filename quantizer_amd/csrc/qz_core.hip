@@ -328,7 +328,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     for (int a = 6; a < 9; a++)
       QZ_LAUNCH("fill", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
-    QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, n, cfg.G * total_mass,
+    QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1), cfg.G * total_mass,
               cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
     keys_valid = false;
     return hip_ok(hipStreamSynchronize(stream), "set_particles");
@@ -368,13 +368,13 @@ template <typename P, typename G> struct Engine final : EngineBase
     QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P)) * N, k_update_A<P>, nblk(N), pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
               dt, refl_rec, refl_rec ? id_a : nullptr);
     keys_valid = false;
-    return hip_ok(hipStreamSynchronize(stream), "particle_updating_A");
+    return hip_ok(hipGetLastError(), "particle_updating_A"); // stream-ordered; readers synchronise
   }
 
   bool update_B(double dt) override
   {
     QZ_LAUNCHB("update_B", 9.0 * sizeof(P) * N, k_update_B<P>, nblk(N), pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, (P)dt);
-    return hip_ok(hipStreamSynchronize(stream), "particle_updating_B");
+    return hip_ok(hipGetLastError(), "particle_updating_B"); // stream-ordered; readers synchronise
   }
 
   bool reset() override
@@ -383,7 +383,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     for (int a = 6; a < 9; a++)
       QZ_LAUNCH("reset", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
     QZ_LAUNCH("reset", k_fill<G>, nblk(npts), rho, (G)0, npts);
-    return hip_ok(hipStreamSynchronize(stream), "reset");
+    return hip_ok(hipGetLastError(), "reset"); // stream-ordered; readers synchronise
   }
 
   // Sort particles by octree key and rebuild the head cell ranges.
@@ -441,10 +441,10 @@ template <typename P, typename G> struct Engine final : EngineBase
     QZ_LAUNCHB("deposit_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0], pos[1],
                pos[2], pa[9], N, cfg.lmin, 3 * D, n - 1, tails);
     QZ_LAUNCHB("deposit_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>), nblk(npts),
-               cell_start, tails, rho, n, coeff);
+               cell_start, tails, rho, make_cube(n + 1), coeff);
     if (has_tree() && !branch_density())
       return false;
-    return hip_ok(hipStreamSynchronize(stream), "grid_density");
+    return hip_ok(hipGetLastError(), "grid_density"); // stream-ordered; readers synchronise
   }
 
   // V-cycle on the error equation of level l; the correction starts from zero
@@ -462,23 +462,24 @@ template <typename P, typename G> struct Engine final : EngineBase
       return;
     }
     const size_t nhalf = rbhalf(np);
+    const Cube cf = make_cube(m + 1);
     for (int sw = 0; sw < cfg.pre_sweeps; sw++)
     {
       int z = (zero_start && sw == 0) ? 1 : 0;
-      QZ_LAUNCHB("rbgs", (z ? 1.0 : 1.5) * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], m, h2, 0, z);
-      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], m, h2, 1, 0);
+      QZ_LAUNCHB("rbgs", (z ? 1.0 : 1.5) * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, z);
+      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], cf, h2, 1, 0);
     }
     int mc = lev_n[l + 1];
     size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
-    QZ_LAUNCHB("restrict", (double)sizeof(G) * (2.0 * np + nc), k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l],
-               lev_r[l + 1], m, (G)((double)m * m));
+    QZ_LAUNCHB("restrict", (double)sizeof(G) * (1.5 * np + nc), k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l],
+               lev_r[l + 1], cf, make_cube(mc + 1), (G)((double)m * m));
     vcycle(l + 1, true);
     QZ_LAUNCHB("prolong", (double)sizeof(G) * (2.0 * np + nc), k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1],
-               m);
+               cf);
     for (int sw = 0; sw < cfg.post_sweeps; sw++)
     {
-      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], m, h2, 0, 0);
-      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], m, h2, 1, 0);
+      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, 0);
+      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], cf, h2, 1, 0);
     }
   }
 
@@ -492,11 +493,11 @@ template <typename P, typename G> struct Engine final : EngineBase
     unsigned nb = nred(npts);
     for (int c = 0; c < inner; c++)
       vcycle(0, c == 0);
-    QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, ph, lev_e[0], n, cfg.tol_change,
+    QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, ph, lev_e[0], make_cube(n + 1), cfg.tol_change,
                part);
     hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
-    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0], n,
-               1.0 / (H * H), part);
+    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0],
+               make_cube(n + 1), 1.0 / (H * H), part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
   }
 
@@ -546,8 +547,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     unsigned nb = nred(npts);
     int outers = 0, vcycles = 0;
     double change = 1e300, res = 0.0, scale = rhomean;
-    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0], n,
-               inv_h2, part);
+    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0],
+               make_cube(n + 1), inv_h2, part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
     bool ok = false;
     for (int outer = 0;; outer++)
@@ -596,10 +597,11 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
-    QZ_LAUNCHB("grid_acc_head", (8.0 + 3.0 * sizeof(G)) * npts, k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz, n, 1.0 / (2.0 * H));
+    QZ_LAUNCHB("grid_acc_head", (8.0 + 3.0 * sizeof(G)) * npts, k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1),
+               1.0 / (2.0 * H));
     if (has_tree() && !branch_grid_acceleration())
       return false;
-    return hip_ok(hipStreamSynchronize(stream), "grid_acceleration");
+    return hip_ok(hipGetLastError(), "grid_acceleration"); // stream-ordered; readers synchronise
   }
 
   bool particle_acceleration() override
@@ -617,7 +619,7 @@ template <typename P, typename G> struct Engine final : EngineBase
       QZ_LAUNCHB("gather_head", (24.0 + 3.0 * sizeof(P)) * N + 3.0 * sizeof(G) * npts, (k_gather_head<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, pa[6], pa[7],
                 pa[8], N, n);
     }
-    return hip_ok(hipStreamSynchronize(stream), "particle_acceleration");
+    return hip_ok(hipGetLastError(), "particle_acceleration"); // stream-ordered; readers synchronise
   }
 
   bool head_fields(double *out5) override
@@ -1027,7 +1029,8 @@ int qz_timing_enable(qz_ctx *ctx, int on)
   QZ_CHECK_CTX(ctx);
   ctx->eng->timer.collect();
   ctx->eng->timer.acc.clear();
-  ctx->eng->timer.on = on != 0;
+  ctx->eng->timer.on = on == 1;
+  ctx->eng->timer.host_on = on == 2;
   return 0;
 }
 

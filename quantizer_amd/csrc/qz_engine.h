@@ -32,7 +32,8 @@ struct Timer
     int count;
     double bytes; // algorithmic bytes (each input read once, each output written once)
   };
-  bool on = false;
+  bool on = false;      // per-kernel events (kernels launched eagerly)
+  bool host_on = false; // host wall-time spans only (graph replays kept)
   std::vector<hipEvent_t> pending; // open begin() events; timed regions may nest
   std::vector<Rec> recs;
   std::vector<Acc> acc;
@@ -51,7 +52,7 @@ struct HostSpan
   HostSpan(Timer &t_, const char *n) : t(t_), name(n), t0(std::chrono::steady_clock::now()) {}
   ~HostSpan()
   {
-    if (t.on)
+    if (t.on || t.host_on)
       t.host(name, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
 };

@@ -24,6 +24,52 @@ __host__ __device__ inline size_t pidx(int i, int j, int k, int n1) { return (si
 // (even index) are stored first, black points after them.  All six stencil
 // neighbours of a point have the other colour.
 __host__ __device__ inline size_t rbpos(size_t idx, size_t half) { return (idx & 1) * half + (idx >> 1); }
+// Division by a fixed divisor d >= 1 for dividends x < 2^31 without an integer
+// divide: q = (x m) >> (31 + s) with s = ceil(log2 d), m = ceil(2^(31+s) / d).
+// Exact: m d - 2^(31+s) = e < d <= 2^s, so x m / 2^(31+s) exceeds x / d by
+// less than x e / (d 2^(31+s)) < 1/d, which never reaches the next integer.
+struct FDiv
+{
+  unsigned m;
+  int s;
+};
+inline FDiv make_fdiv(unsigned d)
+{
+  int s = 0;
+  while ((1ull << s) < d)
+    s++;
+  const unsigned long long p = 1ull << (31 + s);
+  return FDiv{(unsigned)((p + d - 1) / d), s};
+}
+__host__ __device__ inline unsigned fdiv(unsigned x, FDiv f)
+{
+  return (unsigned)(((unsigned long long)x * f.m) >> (31 + f.s));
+}
+
+// Workgroups of a 1-D grid go to the 8 XCDs round-robin by id.  Renumbered
+// this way each XCD takes one contiguous eighth of the range, so the +-x, y, z
+// neighbours a stencil reads were mostly fetched by the same XCD's L2.
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb)
+{
+  const unsigned per = nb >> 3;
+  return b < (per << 3) ? (b & 7u) * per + (b >> 3) : b;
+}
+
+// A cube of n1^3 grid points, x fastest, with the divisors its linear index needs.
+struct Cube
+{
+  int n1;
+  FDiv d1, d2; // by n1 and by n1^2
+};
+inline Cube make_cube(int n1) { return Cube{n1, make_fdiv((unsigned)n1), make_fdiv((unsigned)n1 * (unsigned)n1)}; }
+__host__ __device__ inline void cube_ijk(const Cube &c, unsigned t, int &i, int &j, int &k)
+{
+  const unsigned q = fdiv(t, c.d1);
+  i = (int)(t - q * (unsigned)c.n1);
+  k = (int)fdiv(t, c.d2);
+  j = (int)(q - (unsigned)k * (unsigned)c.n1);
+}
+
 __host__ __device__ inline size_t rbhalf(size_t npts) { return (npts + 1) >> 1; }
 
 // ---------------------------------------------------------------- generic
@@ -165,50 +211,6 @@ __global__ void k_cell_start(const unsigned long long *__restrict__ key, int N, 
     start[c] = i;
 }
 
-// A2 head deposit (grid_density.c:35-111) as a deterministic gather: each grid
-// point sums m * w over the particles of its 8 adjacent cells in slot order.
-// coeff = 4 pi G / H^3 (grid_density.c:62).  No atomics.
-template <typename P, typename G>
-__global__ void k_deposit_head(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-                               const P *__restrict__ m, const int *__restrict__ start, G *__restrict__ rho, int n,
-                               double coeff)
-{
-  const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n1 * n1 * n1)
-    return;
-  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
-  const double s = (double)n;
-  double acc = 0.0;
-  for (int ck = k - 1; ck <= k; ck++)
-  {
-    if (ck < 0 || ck >= n)
-      continue;
-    for (int cj = j - 1; cj <= j; cj++)
-    {
-      if (cj < 0 || cj >= n)
-        continue;
-      for (int ci = i - 1; ci <= i; ci++)
-      {
-        if (ci < 0 || ci >= n)
-          continue;
-        int c = ci + n * (cj + n * ck);
-        int e = start[c + 1];
-        for (int p = start[c]; p < e; p++)
-        {
-          double px = x[p] * s, py = y[p] * s, pz = z[p] * s;
-          double wx1 = px - ci, wy1 = py - cj, wz1 = pz - ck;
-          double wx = ci == i ? 1.0 - wx1 : wx1;
-          double wy = cj == j ? 1.0 - wy1 : wy1;
-          double wz = ck == k ? 1.0 - wz1 : wz1;
-          acc += (double)m[p] * (wx * wy * wz);
-        }
-      }
-    }
-  }
-  rho[t] = (G)(coeff * acc);
-}
-
 // A2 deposit, stage 1 (grid_density.c:35-267): every particle computes its 8
 // CIC corner weights times its mass at level lv; one wavefront (64 consecutive
 // sorted slots) then runs a segmented inclusive scan over runs of equal cell
@@ -306,13 +308,14 @@ template <typename S> __device__ inline double cell_corner_sum(const S *__restri
 // its 8 adjacent cells; coeff = 4 pi G / H^3 (grid_density.c:62).
 template <typename S, typename G>
 __global__ void k_deposit_head_fold(const int *__restrict__ start, const S *__restrict__ tails, G *__restrict__ rho,
-                                    int n, double coeff)
+                                    Cube cb, double coeff)
 {
-  const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n1 * n1 * n1)
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)n1 * n1 * n1)
     return;
-  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  int i, j, k;
+  cube_ijk(cb, t, i, j, k);
   double acc = 0.0;
   for (int ck = k - 1; ck <= k; ck++)
   {
@@ -533,13 +536,14 @@ __global__ void k_grid_pe_head(const double *__restrict__ x, const double *__res
 // its gradient about the box centre on every point of the solve domain.
 template <typename G>
 __global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay, G *__restrict__ az,
-                            int n, double GM, int zero_bc)
+                            Cube cb, double GM, int zero_bc)
 {
-  const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n1 * n1 * n1)
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)n1 * n1 * n1)
     return;
-  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  int i, j, k;
+  cube_ijk(cb, t, i, j, k);
   double H = 1.0 / n;
   double ai = i * H - 0.5, aj = j * H - 0.5, ak = k * H - 0.5;
   double dist = sqrt(ai * ai + aj * aj + ak * ak);
@@ -561,17 +565,21 @@ __global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__r
 // Outer FP64 residual r = rho - Lap(phi) on interior points (multigrid.c:140-150,
 // poisson_error.c:116-117), stored in the V-cycle type, with per-block sum of r^2.
 template <typename G>
-__global__ void k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho, G *__restrict__ r, int n,
+__global__ void k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho, G *__restrict__ r, Cube cb,
                                  double inv_h2, double *__restrict__ part)
 {
-  const int n1 = n + 1;
-  const size_t npts = (size_t)n1 * n1 * n1, hf = rbhalf(npts);
-  const size_t s1 = n1, s2 = (size_t)n1 * n1;
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned npts = (unsigned)n1 * n1 * n1, hf = (npts + 1) >> 1;
+  const unsigned s1 = n1, s2 = (unsigned)n1 * n1;
   double sq = 0.0;
-  // grid-stride: a bounded number of blocks keeps the second reduction stage short
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (size_t)gridDim.x * blockDim.x)
+  // a bounded number of blocks keeps the second reduction stage short; each
+  // block walks one contiguous chunk (XCD-contiguous, see xcd_block)
+  const unsigned chunk = ((npts + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const unsigned t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(npts, t0 + chunk);
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += blockDim.x)
   {
-    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / s2);
+    int i, j, k;
+    cube_ijk(cb, t, i, j, k);
     double res = 0.0;
     if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
     {
@@ -580,7 +588,7 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
                                        phi[t + s2] - phi[t - s2]);
       sq += res * res;
     }
-    r[rbpos(t, hf)] = (G)res;
+    r[(t & 1) * hf + (t >> 1)] = (G)res;
   }
   sq = block_reduce<0>(sq);
   if (threadIdx.x == 0)
@@ -591,19 +599,22 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
 // test of poisson_error.c:36-81 folded in: per-block max of a flag
 // (>= 2 where phi/phi_old leaves [1-tol, 1+tol]) and of |phi/phi_old - 1|.
 template <typename G>
-__global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ e, int n, double tol,
+__global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ e, Cube cb, double tol,
                                 double *__restrict__ part)
 {
-  const int n1 = n + 1;
-  const size_t npts = (size_t)n1 * n1 * n1, hf = rbhalf(npts), s2 = (size_t)n1 * n1;
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned npts = (unsigned)n1 * n1 * n1, hf = (npts + 1) >> 1;
   double dev = 0.0;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (size_t)gridDim.x * blockDim.x)
+  const unsigned chunk = ((npts + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const unsigned t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(npts, t0 + chunk);
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += blockDim.x)
   {
-    int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / s2);
+    int i, j, k;
+    cube_ijk(cb, t, i, j, k);
     if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
     {
       double old = phi[t];
-      double nw = old + (double)e[rbpos(t, hf)];
+      double nw = old + (double)e[(t & 1) * hf + (t >> 1)];
       phi[t] = nw;
       double ratio = nw / old;
       double d = (1.0 - tol > ratio || 1.0 + tol < ratio) ? 2.0 + fabs(ratio - 1.0) : fabs(ratio - 1.0);
@@ -622,91 +633,99 @@ __global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ 
 // colour-split layout: one thread per point of the colour, neighbours read
 // from the other colour's half.
 template <typename G>
-__global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, int n, G h2, int color, int zero_start)
+__global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, Cube cb, G h2, int color, int zero_start)
 {
-  const int n1 = n + 1;
-  const size_t npts = (size_t)n1 * n1 * n1;
-  const size_t half = rbhalf(npts);
-  const size_t count = color == 0 ? half : npts - half;
-  size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned npts = (unsigned)n1 * n1 * n1;
+  const unsigned half = (npts + 1) >> 1;
+  const unsigned count = color == 0 ? half : npts - half;
+  const unsigned h = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (h >= count)
     return;
-  size_t idx = 2 * h + color;
-  int i = (int)(idx % n1);
-  size_t jk = idx / n1;
-  int j = (int)(jk % n1), k = (int)(jk / n1);
+  const unsigned idx = 2 * h + color;
+  int i, j, k;
+  cube_ijk(cb, idx, i, j, k);
   if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
     return;
-  const size_t s1 = n1, s2 = (size_t)n1 * n1;
+  const unsigned s1 = n1, s2 = (unsigned)n1 * n1;
   const G *o = e + (color == 0 ? half : 0); // the other colour
   // zero_start: first half sweep from a zero initial guess (the other colour
   // counts as 0 whatever the array holds), which replaces a separate fill
   G sum = zero_start ? (G)0
                      : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + s1) >> 1] + o[(idx - s1) >> 1] +
                            o[(idx + s2) >> 1] + o[(idx - s2) >> 1];
-  size_t me = (size_t)color * half + h;
+  const unsigned me = color * half + h;
   e[me] = (sum - h2 * r[me]) * (G)(1.0 / 6.0);
 }
 
 // Residual of the fine error equation, full-weighting restricted to the coarse
-// grid (27-point stencil, weights 1/64 * (1,2,1)^3).  rc at coarse interior.
+// grid (27-point stencil, weights 1/64 * (1,2,1)^3); rc at coarse interior.
+// It follows a red-black sweep pair that ended on black, which leaves a zero
+// residual at black fine points: only the 13 red points of the stencil count,
+// the centre (8/64) and the 12 points two offsets away (2/64).
 template <typename G>
-__global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict__ r, G *__restrict__ rc, int n,
-                                    G inv_h2)
+__global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict__ r, G *__restrict__ rc, Cube cf,
+                                    Cube cc, G inv_h2)
 {
-  const int nc = n / 2, nc1 = nc + 1, n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nc1 * nc1 * nc1)
+  const int n1 = cf.n1, n = n1 - 1, nc1 = cc.n1, nc = nc1 - 1;
+  const unsigned npc = (unsigned)nc1 * nc1 * nc1, hc = (npc + 1) >> 1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= npc)
     return;
-  int I = (int)(t % nc1), J = (int)((t / nc1) % nc1), K = (int)(t / ((size_t)nc1 * nc1));
-  const size_t hc = rbhalf((size_t)nc1 * nc1 * nc1);
+  int I, J, K;
+  cube_ijk(cc, t, I, J, K);
   if (I == 0 || I == nc || J == 0 || J == nc || K == 0 || K == nc)
   {
-    rc[rbpos(t, hc)] = (G)0;
+    rc[(t & 1) * hc + (t >> 1)] = (G)0;
     return;
   }
-  const size_t s1 = n1, s2 = (size_t)n1 * n1;
-  const size_t hf = rbhalf((size_t)n1 * n1 * n1);
-  G acc = (G)0;
-  for (int dk = -1; dk <= 1; dk++)
-    for (int dj = -1; dj <= 1; dj++)
-      for (int di = -1; di <= 1; di++)
-      {
-        int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
-        if (i <= 0 || i >= n || j <= 0 || j >= n || k <= 0 || k >= n)
-          continue;
-        size_t g = pidx(i, j, k, n1);
-        G lap = (e[rbpos(g + 1, hf)] + e[rbpos(g - 1, hf)] + e[rbpos(g + s1, hf)] + e[rbpos(g - s1, hf)] +
-                 e[rbpos(g + s2, hf)] + e[rbpos(g - s2, hf)] - (G)6 * e[rbpos(g, hf)]) *
-                inv_h2;
-        G res = r[rbpos(g, hf)] - lap;
-        G w = (G)((di == 0 ? 2 : 1) * (dj == 0 ? 2 : 1) * (dk == 0 ? 2 : 1)) * (G)(1.0 / 64.0);
-        acc += w * res;
-      }
-  rc[rbpos(t, hc)] = acc;
+  const unsigned s1 = n1, s2 = (unsigned)n1 * n1;
+  const unsigned hf = ((unsigned)n1 * n1 * n1 + 1) >> 1;
+  const G *eB = e + hf; // black half
+  // red fine point f (even linear index): residual, zero on the faces
+  auto red_res = [&](unsigned f, int i, int j, int k) -> G {
+    if (i <= 0 || i >= n || j <= 0 || j >= n || k <= 0 || k >= n)
+      return (G)0;
+    const unsigned hpos = f >> 1;
+    G lap = (eB[(f + 1) >> 1] + eB[(f - 1) >> 1] + eB[(f + s1) >> 1] + eB[(f - s1) >> 1] + eB[(f + s2) >> 1] +
+             eB[(f - s2) >> 1] - (G)6 * e[hpos]) *
+            inv_h2;
+    return r[hpos] - lap;
+  };
+  const int i = 2 * I, j = 2 * J, k = 2 * K;
+  const unsigned f = (unsigned)i + s1 * (unsigned)j + s2 * (unsigned)k;
+  G edges = red_res(f + 1 + s1, i + 1, j + 1, k) + red_res(f + 1 - s1, i + 1, j - 1, k) +
+            red_res(f - 1 + s1, i - 1, j + 1, k) + red_res(f - 1 - s1, i - 1, j - 1, k) +
+            red_res(f + 1 + s2, i + 1, j, k + 1) + red_res(f + 1 - s2, i + 1, j, k - 1) +
+            red_res(f - 1 + s2, i - 1, j, k + 1) + red_res(f - 1 - s2, i - 1, j, k - 1) +
+            red_res(f + s1 + s2, i, j + 1, k + 1) + red_res(f + s1 - s2, i, j + 1, k - 1) +
+            red_res(f - s1 + s2, i, j - 1, k + 1) + red_res(f - s1 - s2, i, j - 1, k - 1);
+  rc[(t & 1) * hc + (t >> 1)] = (G)0.125 * red_res(f, i, j, k) + (G)(2.0 / 64.0) * edges;
 }
 
 // Trilinear prolongation of the coarse correction into the fine interior
 // (multigrid.c:51-104) fused with the correction e += P ec (:199-209).
 template <typename G>
-__global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, int n)
+__global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, Cube cb)
 {
-  const int n1 = n + 1, nc1 = n / 2 + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n1 * n1 * n1)
+  const int n1 = cb.n1, n = n1 - 1, nc1 = n / 2 + 1;
+  const unsigned npf = (unsigned)n1 * n1 * n1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= npf)
     return;
-  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  int i, j, k;
+  cube_ijk(cb, t, i, j, k);
   if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
     return;
-  int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-  int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
-  const size_t hc = rbhalf((size_t)nc1 * nc1 * nc1), hf = rbhalf((size_t)n1 * n1 * n1);
-  G v = (ec[rbpos(pidx(i0, j0, k0, nc1), hc)] + ec[rbpos(pidx(i1, j0, k0, nc1), hc)] +
-         ec[rbpos(pidx(i0, j1, k0, nc1), hc)] + ec[rbpos(pidx(i1, j1, k0, nc1), hc)] +
-         ec[rbpos(pidx(i0, j0, k1, nc1), hc)] + ec[rbpos(pidx(i1, j0, k1, nc1), hc)] +
-         ec[rbpos(pidx(i0, j1, k1, nc1), hc)] + ec[rbpos(pidx(i1, j1, k1, nc1), hc)]) *
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const unsigned di = i & 1, dj = (j & 1) * (unsigned)nc1, dk = (k & 1) * (unsigned)nc1 * nc1;
+  const unsigned hc = ((unsigned)nc1 * nc1 * nc1 + 1) >> 1, hf = (npf + 1) >> 1;
+  const unsigned c0 = (unsigned)i0 + (unsigned)nc1 * ((unsigned)j0 + (unsigned)nc1 * k0);
+  auto C = [&](unsigned q) { return ec[(q & 1) * hc + (q >> 1)]; };
+  G v = (C(c0) + C(c0 + di) + C(c0 + dj) + C(c0 + di + dj) + C(c0 + dk) + C(c0 + di + dk) + C(c0 + dj + dk) +
+         C(c0 + di + dj + dk)) *
         (G)0.125;
-  e[rbpos(t, hf)] += v;
+  e[(t & 1) * hf + (t >> 1)] += v;
 }
 
 // Coarsest level: conjugate gradients in one workgroup (FP64 internally) on a
@@ -770,13 +789,14 @@ template <typename G> __global__ void k_coarse_cg(G *__restrict__ e, const G *__
 // keep their A10 values.
 template <typename G>
 __global__ void k_grid_acc_head(const double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay,
-                                G *__restrict__ az, int n, double inv_2h)
+                                G *__restrict__ az, Cube cb, double inv_2h)
 {
-  const int n1 = n + 1;
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n1 * n1 * n1)
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)n1 * n1 * n1)
     return;
-  int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((size_t)n1 * n1));
+  int i, j, k;
+  cube_ijk(cb, t, i, j, k);
   if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
     return;
   size_t s1 = n1, s2 = (size_t)n1 * n1;
