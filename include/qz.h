@@ -82,6 +82,11 @@ int qz_set_particles(qz_ctx *ctx, const double *x, const double *y, const double
                      const double *vy, const double *vz, const double *m);
 int qz_get_particles(qz_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                      double *ax, double *ay, double *az);
+/* the same upload as qz_set_particles, keeping the head potential as the next
+ * solve's starting guess (the reference never resets it, reset.c:54-57); for
+ * hosts that move the particles themselves between force passes */
+int qz_load_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                      const double *vy, const double *vz, const double *m);
 
 /* per-step stages (main.c:391-478) */
 int qz_particle_updating_A(qz_ctx *ctx, double dt); /* particle_updating.c:2584 */

@@ -1,4 +1,10 @@
-"""Build libqzhip.so in-tree with hipcc for gfx950 (no torch extension machinery)."""
+"""Build libqzhip.so in-tree with hipcc for gfx950 (no torch extension machinery).
+
+Inside a QUANTIZER tree (its headers directory given by QZ_REFERENCE_HEADERS,
+or found at the default checkout path) also build the drop-in libraries that
+export the reference's own entry points: libqzdropin.so (force cycle) and
+libqzdropin_resident.so (force cycle + particle updates, particles resident).
+"""
 from __future__ import annotations
 
 import os
@@ -15,7 +21,30 @@ ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-result"]
 
-SOURCES = ["qz_sort.hip", "qz_core.hip", "qz_tree.cpp", "qz_dropin.cpp"]
+SOURCES = ["qz_sort.hip", "qz_core.hip", "qz_tree.cpp"]
+DROPIN_SRC = os.path.join(HERE, "dropin", "qz_dropin.c")
+DROPIN_OUT = {"libqzdropin.so": [], "libqzdropin_resident.so": ["-DQZ_DROPIN_RESIDENT"]}
+
+
+def reference_headers():
+    """The QUANTIZER headers the drop-in compiles against, or None."""
+    for d in (os.environ.get("QZ_REFERENCE_HEADERS"), "/root/reference/headers"):
+        if d and os.path.exists(os.path.join(d, "common.h")):
+            return d
+    return None
+
+
+def build_dropin(headers: str, verbose: bool = False) -> list:
+    outs = []
+    for name, defs in DROPIN_OUT.items():
+        out = os.path.join(HERE, name)
+        cmd = [os.environ.get("CC", "gcc"), "-O2", "-fPIC", "-shared", "-fopenmp", "-w", *defs, f"-I{headers}",
+               "-o", out, DROPIN_SRC, f"-L{HERE}", "-lqzhip", "-Wl,-rpath,$ORIGIN", "-lm"]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        outs.append(out)
+    return outs
 
 
 def _obj(src: str) -> str:
@@ -57,6 +86,9 @@ def build(force: bool = False, verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, OUT)
+    hdrs = reference_headers()
+    if hdrs:
+        build_dropin(hdrs, verbose)
     return OUT
 
 

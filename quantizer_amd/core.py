@@ -79,7 +79,8 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _LIB = None
 
 EXPORTED = [
-    "qz_default_config", "qz_create", "qz_destroy", "qz_last_error", "qz_set_particles", "qz_get_particles",
+    "qz_default_config", "qz_create", "qz_destroy", "qz_last_error", "qz_set_particles", "qz_load_particles",
+    "qz_get_particles",
     "qz_particle_updating_A", "qz_reset", "qz_tree_adaptation", "qz_grid_density", "qz_potential",
     "qz_potential_head_node", "qz_potential_branches", "qz_grid_acceleration", "qz_particle_acceleration",
     "qz_particle_updating_B", "qz_force", "qz_step", "qz_get_head_fields", "qz_set_head_potential",
@@ -105,6 +106,7 @@ def load_library(path: str = LIB_PATH):
     lib.qz_destroy.restype = None
     lib.qz_last_error.restype = ctypes.c_char_p
     lib.qz_set_particles.argtypes = [vp] + [_dp] * 7
+    lib.qz_load_particles.argtypes = [vp] + [_dp] * 7
     lib.qz_get_particles.argtypes = [vp] + [_dp] * 9
     for nm in ("qz_reset", "qz_tree_adaptation", "qz_grid_density", "qz_potential", "qz_potential_head_node",
                "qz_potential_branches", "qz_grid_acceleration", "qz_particle_acceleration", "qz_force", "qz_sync",

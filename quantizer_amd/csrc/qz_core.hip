@@ -301,7 +301,10 @@ template <typename P, typename G> struct Engine final : EngineBase
     return hip_ok(hipStreamSynchronize(stream), "init");
   }
 
-  bool set_particles(const double *const *src) override
+  // init_head: also set the head potential and acceleration to their initial
+  // monopole values (initialization.c:258); otherwise the head potential is kept
+  // as the next solve's starting guess, as the reference never resets it.
+  bool set_particles(const double *const *src, bool init_head) override
   {
     std::vector<P> tmp(N);
     for (int a = 0; a < 3; a++)
@@ -327,9 +330,12 @@ template <typename P, typename G> struct Engine final : EngineBase
       return false;
     for (int a = 6; a < 9; a++)
       QZ_LAUNCH("fill", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
-    size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
-    QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1), cfg.G * total_mass,
-              cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
+    if (init_head)
+    {
+      size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+      QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1), cfg.G * total_mass,
+                cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
+    }
     keys_valid = false;
     return hip_ok(hipStreamSynchronize(stream), "set_particles");
   }
@@ -819,15 +825,15 @@ void qz_destroy(qz_ctx *ctx)
     return 1;                                                                                                         \
   }
 
-int qz_set_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
-                     const double *vy, const double *vz, const double *m)
+static int load_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                          const double *vy, const double *vz, const double *m, bool init_head, const char *who)
 {
   QZ_CHECK_CTX(ctx);
   const double *src[7] = {x, y, z, vx, vy, vz, m};
   for (int a = 0; a < 7; a++)
     if (!src[a])
     {
-      qz::set_error("qz_set_particles: null array");
+      qz::set_error("%s: null array", who);
       return 1;
     }
   for (int i = 0; i < ctx->eng->N; i++)
@@ -838,7 +844,19 @@ int qz_set_particles(qz_ctx *ctx, const double *x, const double *y, const double
       return 1;
     }
   }
-  return ctx->eng->set_particles(src) ? 0 : 1;
+  return ctx->eng->set_particles(src, init_head) ? 0 : 1;
+}
+
+int qz_set_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                     const double *vy, const double *vz, const double *m)
+{
+  return load_particles(ctx, x, y, z, vx, vy, vz, m, true, "qz_set_particles");
+}
+
+int qz_load_particles(qz_ctx *ctx, const double *x, const double *y, const double *z, const double *vx,
+                      const double *vy, const double *vz, const double *m)
+{
+  return load_particles(ctx, x, y, z, vx, vy, vz, m, false, "qz_load_particles");
 }
 
 int qz_get_particles(qz_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *ax,

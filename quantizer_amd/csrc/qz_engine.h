@@ -83,7 +83,7 @@ struct EngineBase
 
   virtual ~EngineBase() {}
   virtual bool init() = 0;
-  virtual bool set_particles(const double *const *src) = 0;
+  virtual bool set_particles(const double *const *src, bool init_head) = 0;
   virtual bool get_particles(double *const *dst) = 0;
   virtual bool update_A(double dt) = 0;
   virtual bool update_B(double dt) = 0;

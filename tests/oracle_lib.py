@@ -254,12 +254,13 @@ def read_nodes(path):
     return res
 
 
-def run_reference(parts, outdir, lmin, lmax, ref_crit, nsteps, dt, extra=()):
-    """Run oracle/_ref/ref_harness (the compiled reference) on ``parts``."""
+def run_reference(parts, outdir, lmin, lmax, ref_crit, nsteps, dt, extra=(), binary=REF_BIN):
+    """Run oracle/_ref/ref_harness (the compiled reference) on ``parts``; ``binary``
+    selects one of the harness builds linked against the drop-in libraries."""
     os.makedirs(outdir, exist_ok=True)
     inp = os.path.join(outdir, "particles.bin")
     parts.write_bin(inp)
-    cmd = [REF_BIN, inp, outdir, str(lmin), str(lmax), str(ref_crit), str(nsteps), repr(float(dt)), *extra]
+    cmd = [binary, inp, outdir, str(lmin), str(lmax), str(ref_crit), str(nsteps), repr(float(dt)), *extra]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"reference harness failed: {p.stderr}")

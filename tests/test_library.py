@@ -11,9 +11,16 @@ from quantizer_amd import core
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+# header -> the library that must export its functions
+LIBS = {"qz.h": [core.LIB_PATH],
+        "qz_dropin.h": [os.path.join(ROOT, "quantizer_amd", "libqzdropin_resident.so")]}
+# functions of qz_dropin.h that only the resident build exports
+RESIDENT_ONLY = {"particle_updating_A", "particle_updating_B", "tree_adaptation"}
+
+
+def declared_functions(header=None):
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in glob.glob(os.path.join(ROOT, "include", header or "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
@@ -28,10 +35,30 @@ def test_headers_declare_entry_points():
         assert must in names
 
 
+def test_every_header_has_a_library():
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        assert os.path.basename(h) in LIBS, h
+
+
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(core.LIB_PATH)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    missing = [n for n in declared_functions("qz.h") if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_dropin_libraries_export_the_reference_entry_points():
+    """Both drop-in builds load on their own (the reference globals are weak
+    references) and export the reference's function names."""
+    force = os.path.join(ROOT, "quantizer_amd", "libqzdropin.so")
+    resident = LIBS["qz_dropin.h"][0]
+    if not (os.path.exists(force) and os.path.exists(resident)):
+        pytest.skip("drop-in libraries are built only against the reference headers")
+    names = declared_functions("qz_dropin.h")
+    assert {"grid_density", "potential", "ptcl_idx_to_box_idx", "particle_updating_A"} <= set(names)
+    lf, lr = ctypes.CDLL(force), ctypes.CDLL(resident)
+    assert not [n for n in names if not hasattr(lr, n)]
+    assert not [n for n in names if n not in RESIDENT_ONLY and not hasattr(lf, n)]
+    assert not [n for n in RESIDENT_ONLY if hasattr(lf, n)]
 
 
 def test_product_fails_loudly_without_gpu():
