@@ -1,21 +1,33 @@
 #!/usr/bin/env python3
 """Throughput of the per-step PM gravity cycle on one MI355X.
 
-Workload (default): N = 2^24 particles on a 256^3 head grid (lmin = 8) with one
-refinement level (lmax = 9, a cell refines at >= 8 particles), half the
+Workload (default, "c3"): N = 2^24 particles on a 256^3 head grid (lmin = 8)
+with one refinement level (lmax = 9, a cell refines at >= 8 particles), half the
 particles uniform and half in 64 Gaussian clumps (sigma uniform in [1, 4] head
 cells), equal masses, zero velocities.  A step is one fixed-dt KDK step of the
 reference loop (particle_updating_A, reset, tree rebuild, grid_density,
 potential to 1e-10 relative residual, grid_acceleration, particle_acceleration,
 particle_updating_B).  dt is the reference CFL rule (CFL 0.5) evaluated after
-the initial force pass and then frozen.
+the initial force pass and then frozen.  Inputs are resident in HBM before the
+timed region starts.
 
-Reported value: particle-steps/s of the mixed path (FP32 particles, grids and
-V-cycles; FP64 outer residual/correction and FP64 branch CG).  The plain FP64
-path is timed on the same workload for the speed-up.  The roofline object
-covers the dominant kernel (algorithmic bytes from the library's per-launch
-accounting over HIP-event time on the context stream); cpu_baseline times the
-compiled reference (oracle/_ref) on a bounded sample of the same workload.
+value: particle-steps/s of the mixed path (FP32 particle velocities and
+accelerations, FP32 grids and V-cycles; FP64 positions, potentials, outer
+residual/correction and refinement-node PCG).  The plain FP64 path is timed on
+the same workload for the speed-up.
+
+adjoint_fwd_ratio: Plummer N = 2^18, 64^3 head grid, 50 steps, checkpoints
+every 10: (checkpointed forward + reverse sweep) time over plain forward time.
+
+roofline: the dominant kernel (the head's red-black Gauss-Seidel half sweep;
+its finest-level launches carry most of its time) replayed back to back on the
+headline grid between two HIP events on the library's stream; algorithmic
+bytes per launch over that time.  traffic: HBM bytes per launch of the same
+kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/, scaled
+by the counters of a plain copy of known size (null when the summary is absent).
+
+cpu_baseline: the compiled reference (oracle/_ref) on a bounded sample of the
+same workload, 1 thread.
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
@@ -33,6 +45,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rbgs_fine_sweep_headline.json")
 
 
 def log(*a):
@@ -54,7 +67,7 @@ def make_particles(kind, n, lmin, seed=20261203):
     return plummer(n, seed=seed)
 
 
-def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inner, solver=None):
+def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inner, solver=None, sweep_reps=0):
     from quantizer_amd import core
     t0 = time.time()
     sim = core.Simulation(parts, lmin, lmax, precision=precision, ref_criterion_ptcl=crit, inner_cycles=inner,
@@ -74,48 +87,88 @@ def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inne
     report = {}
     if prof_steps > 0:
         sim.timing(True)
-        tp = time.perf_counter()
         for _ in range(prof_steps):
             sim.step(dt)
         sim.sync()
-        tprof = time.perf_counter() - tp
         report = sim.timing_report()
         sim.timing(False)
-        report["_wall_ms_per_step"] = {"ms": 1e3 * tprof / prof_steps}
+    sweep = sim.bench_sweep(sweep_reps) if sweep_reps > 0 else None
     sim.close()
-    return elapsed, dt, st, report
+    return elapsed, dt, st, report, sweep
 
 
-def roofline_from(report, prof_steps):
+def kernel_table(report, prof_steps):
+    """Per-stage table of the eager profiling steps (HIP events around each
+    launch, so small launches include their dispatch gap)."""
     kernels = {k: v for k, v in report.items() if not k.startswith("_") and not k.startswith("host:")}
-    if not kernels:
-        return None, {}
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
-    d = kernels[dom]
-    per_launch_ms = d["ms"] / max(1, d["launches"])
-    per_launch_bytes = d["bytes"] / max(1, d["launches"])
-    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
-    total_ms = sum(v["ms"] for v in kernels.values())
-    total_bytes = sum(v["bytes"] for v in kernels.values())
-    table = {k: {"ms_per_step": round(v["ms"] / prof_steps, 4), "launches_per_step": v["launches"] / prof_steps,
-                 "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 and v["bytes"] > 0 else None}
-             for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms"])}
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": per_launch_bytes, "ms_per_launch": per_launch_ms,
-            "step_kernel_ms": round(total_ms / prof_steps, 3),
-            "step_algorithmic_GBps": round(total_bytes / (total_ms * 1e-3) / 1e9, 1) if total_ms > 0 else None}
-    return roof, table
+    return {k: {"ms_per_step": round(v["ms"] / prof_steps, 4), "launches_per_step": v["launches"] / prof_steps,
+                "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 and v["bytes"] > 0 else None}
+            for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms"])}
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the fine sweep from the committed counter summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    t = json.load(open(TRAFFIC_FILE))
+    fs = t["copy_bytes_read"] / (1024.0 * t["copy_fetch_kib"])
+    ws = t["copy_bytes_written"] / (1024.0 * t["copy_write_kib"])
+    return 1024.0 * (t["sweep_fetch_kib"] * fs + t["sweep_write_kib"] * ws), {"FETCH_SIZE": fs, "WRITE_SIZE": ws}
+
+
+def roofline(sweep):
+    ms, by = sweep
+    achieved = by / (ms * 1e-3) / 1e9
+    traffic, scale = traffic_from_profile()
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_rbgs (head red-black GS half sweep, finest level: 257^3 grid, one colour)",
+            "bytes_per_launch": by, "ms_per_launch": ms, "counter_scale": scale}
+
+
+def adjoint_ratio(steps=50, every=10):
+    """Head-only Plummer N = 2^18 on 64^3, FP32 mixed path."""
+    import numpy as np
+    from quantizer_amd import core
+    from quantizer_amd.workloads import plummer, plummer_dt
+    p = plummer(1 << 18, seed=20261203)
+    dt = plummer_dt(0.01)
+    with core.Simulation(p, 6, 6, precision=core.PREC_MIXED) as s:
+        s.force()
+        s.step(dt)
+        s.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            s.step(dt)
+        s.sync()
+        t_fwd = time.perf_counter() - t
+    with core.Simulation(p, 6, 6, precision=core.PREC_MIXED) as s:
+        s.force()
+        s.adjoint_forward(2, dt, 1)  # warm-up of the adjoint buffers
+        s.sync()
+    with core.Simulation(p, 6, 6, precision=core.PREC_MIXED) as s:
+        s.force()
+        s.sync()
+        t = time.perf_counter()
+        s.adjoint_forward(steps, dt, every)
+        s.sync()
+        t_af = time.perf_counter() - t
+        xb = np.ones((3, p.n)) / p.n
+        t = time.perf_counter()
+        s.adjoint_backward(xb, np.zeros_like(xb))
+        s.sync()
+        t_ab = time.perf_counter() - t
+    return {"value": (t_af + t_ab) / t_fwd, "forward_ms_per_step": 1e3 * t_fwd / steps,
+            "checkpointed_forward_ms_per_step": 1e3 * t_af / steps, "reverse_ms_per_step": 1e3 * t_ab / steps,
+            "config": f"Plummer N=2^18, head 64^3 (lmin=6), {steps} steps, checkpoint every {every}, mixed"}
 
 
 def cpu_baseline(parts, lmin, lmax, crit, dt, budget_s):
-    """Time the compiled reference (oracle/_ref/ref_harness) on this workload: one
-    force pass plus one step when it fits the budget, else on a particle
-    subsample of the same grid."""
+    """Time the compiled reference (oracle/_ref/ref_harness) on this workload: the
+    initial force pass, then one timed step, 1 thread."""
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if not os.path.exists(ref):
         return None
-    from tests.oracle_lib import run_reference  # noqa: F401  (checker module; only the binary is timed)
     with tempfile.TemporaryDirectory() as td:
         inp = os.path.join(td, "p.bin")
         parts.write_bin(inp)
@@ -137,6 +190,20 @@ def cpu_baseline(parts, lmin, lmax, crit, dt, budget_s):
                           f"({sps:.2f} s/step)"}
 
 
+def pmc_probe(args):
+    """Short run for hardware counters: one force pass and step of the headline
+    workload, a few fine sweeps, and one plain copy of 512 MiB."""
+    from quantizer_amd import core
+    N, lmin, lmax, crit, kind = WORKLOADS[args.workload]
+    parts = make_particles(kind, N, lmin)
+    with core.Simulation(parts, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        s.force()
+        s.step(s.timestep(cfl=0.5))
+        ms, by = s.bench_sweep(3)
+        cms = s.copy_probe(512 << 20)
+    print(json.dumps({"sweep_ms": ms, "sweep_bytes": by, "copy_bytes": 2 * (512 << 20), "copy_ms": cms}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,17 +211,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--prof-steps", type=int, default=2)
+    ap.add_argument("--sweep-reps", type=int, default=50)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=150.0)
     ap.add_argument("--cpu-n", type=int, default=1 << 24, help="particles in the CPU sample")
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--branch-solver", choices=["cg", "mg"], default=None,
                     help="refinement-node solver (default: the library default)")
+    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
     from quantizer_amd import core
     N, lmin, lmax, crit, kind = WORKLOADS[args.workload]
     solver = {None: None, "cg": core.BRANCH_CG, "mg": core.BRANCH_MG}[args.branch_solver]
@@ -163,22 +236,27 @@ def main():
     parts = make_particles(kind, N, lmin)
     log(f"  generated in {time.time() - t:.1f}s")
 
-    el_mx, dt, st_mx, rep_mx = run_path(parts, lmin, lmax, crit, core.PREC_MIXED, args.steps, args.warmup,
-                                        args.prof_steps, args.inner, solver)
+    el_mx, dt, st_mx, rep_mx, sweep = run_path(parts, lmin, lmax, crit, core.PREC_MIXED, args.steps, args.warmup,
+                                               args.prof_steps, args.inner, solver, args.sweep_reps)
     value = N * args.steps / el_mx
     ms_step = 1e3 * el_mx / args.steps
     log(f"  mixed: {ms_step:.3f} ms/step, {value:.4e} particle-steps/s")
     fp64 = None
     if not args.skip_fp64:
-        el64, _, st64, rep64 = run_path(parts, lmin, lmax, crit, core.PREC_FP64, args.steps, args.warmup, 0,
-                                        args.inner, solver)
+        el64, _, st64, _, _ = run_path(parts, lmin, lmax, crit, core.PREC_FP64, args.steps, args.warmup, 0,
+                                       args.inner, solver)
         fp64 = {"value": N * args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
                 "head_vcycles": st64["head_vcycles"], "branch_iterations": st64["branch_iterations"]}
         log(f"  fp64: {fp64['ms_per_step']:.3f} ms/step")
-    roof, table = roofline_from(rep_mx, max(1, args.prof_steps))
+    roof = roofline(sweep) if sweep else None
+    if roof:
+        log(f"  roofline: {roof['achieved']} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
+    adj = None
+    if not args.skip_adjoint:
+        adj = adjoint_ratio()
+        log(f"  adjoint/forward: {adj['value']:.3f} ({adj})")
     cpu = None
     if not args.skip_cpu:
-        # bounded sample: one step of the same workload (about 20 s on one core)
         ns = min(N, args.cpu_n)
         sample = make_particles(kind, ns, lmin, seed=7) if ns < N else parts
         cpu = cpu_baseline(sample, lmin, lmax, crit, dt, args.cpu_budget)
@@ -202,13 +280,12 @@ def main():
                    "n_particles": N, "lmin": lmin, "lmax": lmax, "parallelism": "single GPU"},
         "fp64_plain": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-        "adjoint_fwd_ratio": None,
-        "solver": {"head_outer": st_mx["head_outer"], "head_vcycles": st_mx["head_vcycles"],
-                   "head_residual": st_mx["head_residual"], "n_nodes": st_mx["n_nodes"],
-                   "branch_points": st_mx["branch_points"], "branch_iterations": st_mx["branch_iterations"],
-                   "branch_cg_levels": st_mx["branch_cg_levels"]},
+        "adjoint_fwd_ratio": adj["value"] if adj else None,
+        "adjoint": adj,
+        "solver": {k: st_mx[k] for k in ("head_outer", "head_vcycles", "head_residual", "n_nodes", "branch_points",
+                                         "branch_intr", "branch_iterations", "branch_cg_levels")},
         "roofline": roof,
-        "kernels": table,
+        "kernels": kernel_table(rep_mx, max(1, args.prof_steps)) if rep_mx else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

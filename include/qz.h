@@ -14,6 +14,8 @@
 #ifndef QZ_H
 #define QZ_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -156,6 +158,13 @@ int qz_keys(const double *x, const double *y, const double *z, int n, int lv, co
 int qz_timing_enable(qz_ctx *ctx, int on);
 int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, double *bytes, int max);
 int qz_sync(qz_ctx *ctx);
+
+/* measurement helpers (no counterpart in the reference): the head's finest red
+ * half sweep of the multigrid smoother replayed `reps` times back to back,
+ * returning ms and algorithmic bytes per launch; and one plain device copy of
+ * `bytes` bytes (kernel k_copy_probe) whose hardware byte counts calibrate them */
+int qz_bench_sweep(qz_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch);
+int qz_copy_probe(qz_ctx *ctx, size_t bytes, double *ms);
 
 #ifdef __cplusplus
 }

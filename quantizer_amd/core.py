@@ -86,6 +86,7 @@ EXPORTED = [
     "qz_particle_updating_B", "qz_force", "qz_step", "qz_get_head_fields", "qz_set_head_potential",
     "qz_num_nodes", "qz_node_header", "qz_node_grid_lists", "qz_node_fields", "qz_get_stats", "qz_observables",
     "qz_timestep", "qz_force_vjp", "qz_adjoint_forward", "qz_adjoint_backward", "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
+    "qz_bench_sweep", "qz_copy_probe",
 ]
 
 
@@ -251,6 +252,23 @@ class Simulation:
 
     def sync(self):
         return _check(load_library().qz_sync(self._h), "sync")
+
+    # ------------------------------------------------------------ measurement
+    def bench_sweep(self, reps: int = 50):
+        """(ms, algorithmic bytes) per launch of the head's finest red half sweep."""
+        lib = load_library()
+        ms, by = ctypes.c_double(0), ctypes.c_double(0)
+        lib.qz_bench_sweep.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        _check(lib.qz_bench_sweep(self._h, reps, ctypes.byref(ms), ctypes.byref(by)), "qz_bench_sweep")
+        return ms.value, by.value
+
+    def copy_probe(self, nbytes: int) -> float:
+        """ms of one plain device copy of nbytes (kernel k_copy_probe)."""
+        lib = load_library()
+        ms = ctypes.c_double(0)
+        lib.qz_copy_probe.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _dp]
+        _check(lib.qz_copy_probe(self._h, nbytes, ctypes.byref(ms)), "qz_copy_probe")
+        return ms.value
 
     # ------------------------------------------------------------ data access
     def particles(self) -> np.ndarray:

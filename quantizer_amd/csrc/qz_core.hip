@@ -185,6 +185,62 @@ template <typename P, typename G> struct Engine final : EngineBase
 
   ~Engine() override { release(); }
 
+  // ---------------------------------------------------------------- measurement
+  // The head's finest red half sweep replayed `reps` times back to back between
+  // two events on the engine stream (the level-0 correction array is scratch
+  // between solves).  Average ms per launch and algorithmic bytes per launch.
+  bool bench_sweep(int reps, double *ms, double *bytes) override
+  {
+    const size_t p1 = (size_t)n + 1, np = p1 * p1 * p1, nhalf = rbhalf(np);
+    const Cube cf = make_cube(n + 1);
+    const G h2 = (G)(1.0 / ((double)n * n));
+    hipEvent_t a, b;
+    if (!hip_ok(hipEventCreate(&a), "events") || !hip_ok(hipEventCreate(&b), "events"))
+      return false;
+    hipLaunchKernelGGL(k_rbgs<G>, dim3(nblk(nhalf)), dim3(kBlock), 0, stream, lev_e[0], lev_r[0], cf, h2, 0, 0);
+    (void)hipEventRecord(a, stream);
+    for (int i = 0; i < reps; i++)
+      hipLaunchKernelGGL(k_rbgs<G>, dim3(nblk(nhalf)), dim3(kBlock), 0, stream, lev_e[0], lev_r[0], cf, h2, 0, 0);
+    (void)hipEventRecord(b, stream);
+    bool ok = hip_ok(hipEventSynchronize(b), "bench sweep");
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = (double)t / std::max(1, reps);
+    *bytes = 1.5 * sizeof(G) * (double)np; // own colour r + e, the other colour's e once
+    return ok;
+  }
+
+  // One copy of `bytes` bytes (rounded to doubles) through k_copy_probe.
+  bool copy_probe(size_t bytes, double *ms) override
+  {
+    const size_t nd = bytes / sizeof(double);
+    double *src = nullptr, *dst = nullptr;
+    if (!hip_ok(hipMalloc((void **)&src, nd * sizeof(double)), "copy probe") ||
+        !hip_ok(hipMalloc((void **)&dst, nd * sizeof(double)), "copy probe"))
+    {
+      (void)hipFree(src);
+      return false;
+    }
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nd)), dim3(kBlock), 0, stream, src, 1.0, nd);
+    (void)hipEventRecord(a, stream);
+    hipLaunchKernelGGL(k_copy_probe, dim3(nblk(nd)), dim3(kBlock), 0, stream, src, dst, nd);
+    (void)hipEventRecord(b, stream);
+    bool ok = hip_ok(hipEventSynchronize(b), "copy probe");
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, a, b);
+    *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return ok;
+  }
+
   void release()
   {
     for (int a = 0; a < 3; a++)
@@ -1071,6 +1127,28 @@ int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *la
     k++;
   }
   return k;
+}
+
+int qz_bench_sweep(qz_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch)
+{
+  QZ_CHECK_CTX(ctx);
+  if (reps < 1 || !ms_per_launch || !bytes_per_launch)
+  {
+    qz::set_error("qz_bench_sweep: bad arguments");
+    return 1;
+  }
+  return ctx->eng->bench_sweep(reps, ms_per_launch, bytes_per_launch) ? 0 : 1;
+}
+
+int qz_copy_probe(qz_ctx *ctx, size_t bytes, double *ms)
+{
+  QZ_CHECK_CTX(ctx);
+  if (!ms || bytes < sizeof(double))
+  {
+    qz::set_error("qz_copy_probe: bad arguments");
+    return 1;
+  }
+  return ctx->eng->copy_probe(bytes, ms) ? 0 : 1;
 }
 
 int qz_sync(qz_ctx *ctx)

@@ -105,6 +105,8 @@ struct EngineBase
   virtual bool node_header(int i, int *hdr) = 0;
   virtual bool node_lists(int i, int *ixyz, int *bxyz, int *sxyz) = 0;
   virtual bool node_fields(int i, double *out5) = 0;
+  virtual bool bench_sweep(int reps, double *ms, double *bytes) = 0;
+  virtual bool copy_probe(size_t bytes, double *ms) = 0;
 };
 
 EngineBase *make_engine(const qz_config &cfg);

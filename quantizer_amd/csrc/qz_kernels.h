@@ -805,6 +805,14 @@ __global__ void k_grid_acc_head(const double *__restrict__ phi, G *__restrict__ 
   az[t] = (G)(inv_2h * (phi[t - s2] - phi[t + s2]));
 }
 
+// Plain copy of n doubles (the reference traffic for hardware byte counters).
+__global__ void k_copy_probe(const double *__restrict__ src, double *__restrict__ dst, size_t n)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n)
+    dst[t] = src[t];
+}
+
 // A1 for arbitrary node geometry (host-facing utility).
 __global__ void k_box_keys(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                            int n, int lv, int tsx, int tsy, int tsz, int rdx, int rdy, int *__restrict__ out)
