@@ -65,6 +65,9 @@ struct TreeDev
   void *mg_e = nullptr, *mg_r = nullptr;
   unsigned char *mg_mask = nullptr;
   long long mg_cap = 0;
+  std::vector<Chunk> hchunks; // work tables of the level being solved
+  Chunk *chunks = nullptr;
+  long long chunk_cap = 0;
   long long used_zone = 0, used_cell = 0, used_box = 0, used_pt = 0, used_grid = 0;
 };
 
@@ -95,7 +98,7 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
                     d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask, d->hmark, d->hlist,
                     d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
-                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp};
+                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->chunks};
     for (void *q : ptrs)
       (void)hipFree(q);
     if (d->hflag_count)
@@ -750,24 +753,79 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
     if (!hip_ok(hipMalloc((void **)&d.bmg, sizeof(BMg) * d.bmg_cap), "multigrid table"))
       return false;
   }
+  // ---- chunk tables: interior points per node; per MG level the linear points
+  // of the nodes that smooth there (S, nlev > m + 1) and of all nodes that have
+  // the level (C, nlev > m)
+  std::vector<Chunk> &tab = d.hchunks;
+  tab.clear();
+  auto add_chunks = [&](int q, long long n, int step) {
+    for (long long b0 = 0; b0 < n; b0 += step)
+      tab.push_back(Chunk{q, (int)b0, (int)std::min<long long>(n, b0 + step), 0});
+  };
+  for (int q = 0; q < count; q++)
+  {
+    hb[q].ic0 = (int)tab.size();
+    add_chunks(q, d.hnodes[first + q].nintr, kBmgChunkIntr);
+    hb[q].ic1 = (int)tab.size();
+  }
+  const int n_itab = (int)tab.size();
+  int s0[kMaxMgLevels], sn[kMaxMgLevels], c0[kMaxMgLevels], cn[kMaxMgLevels];
+  double sptsm[kMaxMgLevels] = {0.0}; // points over the smoothing nodes at level m
+  for (int m = 0; m < mmax; m++)
+  {
+    s0[m] = (int)tab.size();
+    for (int q = 0; q < count; q++)
+      if (hb[q].nlev > m + 1)
+      {
+        const long long np = (long long)hb[q].X[m] * hb[q].Y[m] * hb[q].Z[m];
+        add_chunks(q, np, kBmgChunkGrid);
+        sptsm[m] += (double)np;
+      }
+    sn[m] = (int)tab.size() - s0[m];
+    c0[m] = (int)tab.size();
+    for (int q = 0; q < count; q++)
+      if (hb[q].nlev > m)
+        add_chunks(q, (long long)hb[q].X[m] * hb[q].Y[m] * hb[q].Z[m], kBmgChunkGrid);
+    cn[m] = (int)tab.size() - c0[m];
+  }
+  if ((long long)tab.size() > d.chunk_cap)
+  {
+    (void)hipFree(d.chunks);
+    d.chunks = nullptr;
+    d.chunk_cap = next_cap(d.chunk_cap, (long long)tab.size());
+    if (!hip_ok(hipMalloc((void **)&d.chunks, sizeof(Chunk) * d.chunk_cap), "chunk table"))
+      return false;
+  }
+  if (n_itab + 1024 > d.part_cap)
+  {
+    (void)hipFree(d.part1);
+    (void)hipFree(d.part2);
+    (void)hipFree(d.part3);
+    d.part1 = d.part2 = d.part3 = nullptr;
+    d.part_cap = n_itab + 1024;
+    if (!hip_ok(hipMalloc((void **)&d.part1, sizeof(double) * d.part_cap), "partials") ||
+        !hip_ok(hipMalloc((void **)&d.part2, sizeof(double) * d.part_cap), "partials") ||
+        !hip_ok(hipMalloc((void **)&d.part3, sizeof(double) * d.part_cap), "partials"))
+      return false;
+  }
   G *e = (G *)d.mg_e, *r = (G *)d.mg_r;
   unsigned char *mk = d.mg_mask;
   BMg *bm = d.bmg;
-  const int nbx = blocks_per_node(d.lpmax[l]);
-  const dim3 gpt(nbx, count);
-  // blocks per node for a sweep over n points (about two points per thread)
-  const int cap = std::max(64, 4096 / count);
-  auto gsw = [&](long long n) {
-    return dim3((unsigned)std::min<long long>(cap, std::max<long long>(1, (n + 2 * kBlock - 1) / (2 * kBlock))), count);
-  };
-  if (!hip_ok(hipMemcpyAsync(bm, hb.data(), sizeof(BMg) * count, hipMemcpyHostToDevice, stream), "multigrid table"))
+  const Chunk *itab = d.chunks;
+  auto S = [&](int m) { return (const Chunk *)d.chunks + s0[m]; };
+  auto C = [&](int m) { return (const Chunk *)d.chunks + c0[m]; };
+  if (!hip_ok(hipMemcpyAsync(bm, hb.data(), sizeof(BMg) * count, hipMemcpyHostToDevice, stream), "multigrid table") ||
+      !hip_ok(hipMemcpyAsync(d.chunks, tab.data(), sizeof(Chunk) * tab.size(), hipMemcpyHostToDevice, stream),
+              "chunk table"))
     return false;
   QZ_LAUNCH("bmg_setup", k_fill<unsigned char>, nblk(pool), mk, (unsigned char)0, (size_t)pool);
   QZ_LAUNCH("bmg_setup", k_fill<G>, nblk(pool), e, (G)0, (size_t)pool);
   QZ_LAUNCH("bmg_setup", k_fill<int>, 1, d.done_count, 0, (size_t)2);
-  QZ_LAUNCH("bmg_setup", k_bmg_mask0, gpt, d.nodes, first, bm, d.intr_g, mk);
+  if (n_itab > 0)
+    QZ_LAUNCH("bmg_setup", k_bmg_mask0, n_itab, d.nodes, first, bm, itab, d.intr_g, mk);
   for (int m = 0; m + 1 < mmax; m++)
-    QZ_LAUNCH("bmg_setup", k_bmg_mask_coarsen, gsw(2 * npm[m + 1]), bm, m, mk, mk);
+    if (cn[m + 1] > 0)
+      QZ_LAUNCH("bmg_setup", k_bmg_mask_coarsen, cn[m + 1], bm, C(m + 1), m, mk, mk);
 
   const double ni = (double)d.lnintr[l], gs = (double)sizeof(G);
   const int inner = std::max(1, cfg.inner_cycles);
@@ -775,29 +833,31 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
   int *it_base = d.done_count + 1;
   auto sweep = [&](int m, int color, int zs) {
     // colour half: mask + r + e of that colour, neighbours of the other colour
-    QZ_LAUNCHB("bmg_rbgs", ptsm[m] * (0.5 * (1.0 + 2.0 * gs) + 0.5 * gs), (k_bmg_rbgs<G>), gsw(npm[m]), d.nodes,
-               first, bm, m, e, r, mk, color, zs);
+    if (sn[m] > 0)
+      QZ_LAUNCHB("bmg_rbgs", sptsm[m] * (0.5 * (1.0 + 2.0 * gs) + 0.5 * gs), (k_bmg_rbgs<G>), sn[m], d.nodes, first,
+                 bm, S(m), m, e, r, mk, color, zs);
   };
   auto vcycle = [&]() {
     for (int c = 0; c < inner; c++)
     {
       for (int m = 0; m + 1 < mmax; m++)
       {
-        for (int s = 0; s < pre; s++)
+        for (int sw = 0; sw < pre; sw++)
         {
-          sweep(m, 0, s == 0 && (m > 0 || c == 0));
+          sweep(m, 0, sw == 0 && (m > 0 || c == 0));
           sweep(m, 1, 0);
         }
-        QZ_LAUNCHB("bmg_restrict", ptsm[m] * (0.5 * (2.0 * gs + 1.0) + 0.5 * gs) + ptsm[m + 1] * (gs + 1.0),
-                   (k_bmg_restrict<G>),
-                   gsw(2 * npm[m + 1]), d.nodes, first, bm, m, e, r, mk, r, mk);
+        if (cn[m + 1] > 0)
+          QZ_LAUNCHB("bmg_restrict", sptsm[m] * (0.5 * (2.0 * gs + 1.0) + 0.5 * gs) + ptsm[m + 1] * (gs + 1.0),
+                     (k_bmg_restrict<G>), cn[m + 1], d.nodes, first, bm, C(m + 1), m, e, r, mk, r, mk);
       }
       QZ_LAUNCH("bmg_coarse", (k_bmg_coarse<G>), count, d.nodes, first, bm, e, r, mk, 100);
       for (int m = mmax - 2; m >= 0; m--)
       {
-        QZ_LAUNCHB("bmg_prolong", ptsm[m] * (1.0 + 2.0 * gs) + ptsm[m + 1] * gs, (k_bmg_prolong<G>), gsw(2 * npm[m]),
-                   bm, m, e, mk, e);
-        for (int s = 0; s < post; s++)
+        if (sn[m] > 0)
+          QZ_LAUNCHB("bmg_prolong", sptsm[m] * (1.0 + 2.0 * gs) + ptsm[m + 1] * gs, (k_bmg_prolong<G>), sn[m], bm,
+                     S(m), m, e, mk, e);
+        for (int sw = 0; sw < post; sw++)
         {
           sweep(m, 0, 0);
           sweep(m, 1, 0);
@@ -810,21 +870,25 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
   // zdots reads idx, z, r, q; direction reads idx, z, p, writes p; apply reads
   // idx, p, writes q; update reads idx, phi, p, q, r, writes phi, r (FP64 + G).
   QZ_LAUNCH("bmg_setup", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
-  QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + gs + 8.0 + gs), (k_bmg_residual<G>), gpt, d.nodes, first, bm, d.intr_g,
-             d.phi, (const G *)d.rho, d.cg_r, r, d.part2);
+  if (n_itab > 0)
+    QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + gs + 8.0 + gs), (k_bmg_residual<G>), n_itab, d.nodes, first, bm,
+               itab, d.intr_g, d.phi, (const G *)d.rho, d.cg_r, r, d.part2);
   auto batch = [&]() {
     for (int il = 0; il < 2; il++)
     {
-      QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, nbx, cfg.tol,
-                cfg.G, d.done_count);
+      QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, cfg.tol, cfg.G,
+                d.done_count);
+      if (n_itab == 0)
+        continue;
       vcycle();
-      QZ_LAUNCHB("bmg_zdots", ni * (4.0 + gs + 16.0), (k_bmg_zdots<G>), gpt, d.nodes, first, bm, d.intr_g,
+      QZ_LAUNCHB("bmg_zdots", ni * (4.0 + gs + 16.0), (k_bmg_zdots<G>), n_itab, d.nodes, first, bm, itab, d.intr_g,
                  (const G *)e, d.cg_r, d.cg_Ap, d.part1, d.part3);
-      QZ_LAUNCHB("bmg_direction", ni * (4.0 + gs + 16.0), (k_bmg_direction<G>), gpt, d.nodes, first, bm, it_base, il,
-                 d.intr_g, (const G *)e, d.cg_p, d.part1, d.part3);
-      QZ_LAUNCHB("bmg_apply", ni * 20.0, k_bmg_apply, gpt, d.nodes, first, bm, d.intr_g, d.cg_p, d.cg_Ap, d.part1);
-      QZ_LAUNCHB("bmg_update", ni * (4.0 + 48.0 + gs), (k_bmg_update<G>), gpt, d.nodes, first, bm, it_base, il,
-                 d.intr_g, d.phi, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
+      QZ_LAUNCHB("bmg_direction", ni * (4.0 + gs + 16.0), (k_bmg_direction<G>), n_itab, d.nodes, first, bm, itab,
+                 it_base, il, d.intr_g, (const G *)e, d.cg_p, d.part1, d.part3);
+      QZ_LAUNCHB("bmg_apply", ni * 20.0, k_bmg_apply, n_itab, d.nodes, first, bm, itab, d.intr_g, d.cg_p, d.cg_Ap,
+                 d.part1);
+      QZ_LAUNCHB("bmg_update", ni * (4.0 + 48.0 + gs), (k_bmg_update<G>), n_itab, d.nodes, first, bm, itab, it_base,
+                 il, d.intr_g, d.phi, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
     }
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 2);
   };

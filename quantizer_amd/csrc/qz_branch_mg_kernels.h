@@ -6,8 +6,12 @@
 // simulation-boundary and absent points are Dirichlet (zero correction).  A
 // coarse point is active when the fine point it injects from is active.  All
 // level arrays use the colour-split layout per node (every axis length is odd,
-// so the linear-index parity is the red/black colour).  One launch covers all
-// nodes of the tree level: blockIdx.y is the node.
+// so the linear-index parity is the red/black colour).
+//
+// Work is handed out through chunk tables: one entry {node, begin, end} per
+// workgroup, over a node's interior points or over the linear points of one of
+// its multigrid levels.  A launch covers every node of the tree level without
+// idle workgroups for small nodes, and converged nodes drop out per chunk.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,58 +21,35 @@ namespace qz
 {
 
 constexpr int kMaxMgLevels = 8;
-constexpr int kBmgCoarseCells = 8; // cells per axis of a node's coarsest grid, at most
-constexpr int kBmgMaxOuter = 60;   // outer iterations before the level is handed to CG
+constexpr int kBmgCoarseCells = 8;  // cells per axis of a node's coarsest grid, at most
+constexpr int kBmgMaxOuter = 60;    // outer iterations before the level is handed to CG
+constexpr int kBmgChunkIntr = 1024; // interior points per chunk (4 per thread)
+constexpr int kBmgChunkGrid = 512;  // linear grid points per chunk (one colour point per thread)
 
 struct BMg
 {
   int X[kMaxMgLevels], Y[kMaxMgLevels], Z[kMaxMgLevels];
-  long long off[kMaxMgLevels]; // offset of the node's level-m block in that level's pool
+  long long off[kMaxMgLevels]; // offset of the node's level-m block in the pools
   FDiv dx[kMaxMgLevels], dxy[kMaxMgLevels]; // divide by X[m], by X[m] Y[m]
   FDiv gx, gxy;                              // divide by the box grid's rd_x + 1, (rd_x + 1)(rd_y + 1)
   int nlev;
   int done;        // converged: outer iterations used
+  int ic0, ic1;    // the node's interior chunks (entries of the interior table)
   double rz[2];    // (-r).z of the last two iterations
   double alpha[2]; // step lengths of the last two iterations
 };
 
+struct Chunk
+{
+  int node; // position in the tree level
+  int begin, end;
+  int pad;
+};
+
 __device__ inline long long bmg_np(const BMg &b, int m) { return (long long)b.X[m] * b.Y[m] * b.Z[m]; }
 
-// level-0 mask: 1 at the node's interior points (box grid index -> padded index)
-__global__ void k_bmg_mask0(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                            const int *__restrict__ intr_g, unsigned char *__restrict__ mask)
-{
-  const DNode &d = nodes[first + blockIdx.y];
-  const BMg &b = bm[blockIdx.y];
-  const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
-  const long long np = bmg_np(b, 0), hf = (np + 1) >> 1;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
-  {
-    int g = intr_g[d.poff + t];
-    const int q = (int)fdiv((unsigned)g, b.gx), k = (int)fdiv((unsigned)g, b.gxy);
-    const int i = g - q * gX, j = q - k * (d.rd[1] + 1);
-    long long idx = i + (long long)b.X[0] * (j + (long long)b.Y[0] * k);
-    mask[b.off[0] + (idx & 1) * hf + (idx >> 1)] = 1;
-  }
-}
-
-// coarse mask by injection: active(I) = active_fine(2I)
-__global__ void k_bmg_mask_coarsen(const BMg *__restrict__ bm, int m, const unsigned char *__restrict__ mf,
-                                   unsigned char *__restrict__ mc)
-{
-  const BMg &b = bm[blockIdx.y];
-  if (m + 1 >= b.nlev)
-    return;
-  const long long npc = bmg_np(b, m + 1), hc = (npc + 1) >> 1;
-  const long long npf = bmg_np(b, m), hf = (npf + 1) >> 1;
-  for (long long t = blockIdx.x * blockDim.x + threadIdx.x; t < npc; t += (long long)gridDim.x * blockDim.x)
-  {
-    const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
-    const int I = (int)t - JK * b.X[m + 1], J = JK - K * b.Y[m + 1];
-    long long f = 2 * I + (long long)b.X[m] * (2 * J + (long long)b.Y[m] * 2 * K);
-    mc[b.off[m + 1] + (t & 1) * hc + (t >> 1)] = mf[b.off[m] + (f & 1) * hf + (f >> 1)];
-  }
-}
+// this workgroup's chunk (XCD-contiguous numbering: a node's chunks share an L2)
+__device__ inline const Chunk &bmg_chunk(const Chunk *tab) { return tab[xcd_block(blockIdx.x, gridDim.x)]; }
 
 // padded colour-split position at MG level 0 of box-grid point g
 __device__ inline long long bmg_pos0(const DNode &d, const BMg &b, int g)
@@ -81,28 +62,76 @@ __device__ inline long long bmg_pos0(const DNode &d, const BMg &b, int g)
   return b.off[0] + (idx & 1) * hf + (idx >> 1);
 }
 
+// fold per-chunk partials over the node's chunk range [c0, c1)
+__device__ inline double fold_range(const double *part, int c0, int c1, bool max_op)
+{
+  __shared__ double s_tot;
+  double v = max_op ? -1.0e308 : 0.0;
+  for (int i = c0 + threadIdx.x; i < c1; i += blockDim.x)
+    v = max_op ? fmax(v, part[i]) : v + part[i];
+  v = max_op ? block_reduce<1>(v) : block_reduce<0>(v);
+  if (threadIdx.x == 0)
+    s_tot = v;
+  __syncthreads();
+  const double r = s_tot;
+  __syncthreads();
+  return r;
+}
+
+// level-0 mask: 1 at the node's interior points (box grid index -> padded index)
+__global__ void k_bmg_mask0(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                            const Chunk *__restrict__ itab, const int *__restrict__ intr_g,
+                            unsigned char *__restrict__ mask)
+{
+  const Chunk &c = bmg_chunk(itab);
+  const DNode &d = nodes[first + c.node];
+  const BMg &b = bm[c.node];
+  for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
+    mask[bmg_pos0(d, b, intr_g[d.poff + t])] = 1;
+}
+
+// coarse mask by injection: active(I) = active_fine(2I); table over level m+1
+__global__ void k_bmg_mask_coarsen(const BMg *__restrict__ bm, const Chunk *__restrict__ tab, int m,
+                                   const unsigned char *__restrict__ mf, unsigned char *__restrict__ mc)
+{
+  const Chunk &c = bmg_chunk(tab);
+  const BMg &b = bm[c.node];
+  const long long npc = bmg_np(b, m + 1), hc = (npc + 1) >> 1;
+  const long long npf = bmg_np(b, m), hf = (npf + 1) >> 1;
+  for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
+  {
+    const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
+    const int I = t - JK * b.X[m + 1], J = JK - K * b.Y[m + 1];
+    const long long f = 2 * I + (long long)b.X[m] * (2 * J + (long long)b.Y[m] * 2 * K);
+    mc[b.off[m + 1] + (t & 1) * hc + (t >> 1)] = mf[b.off[m] + (f & 1) * hf + (f >> 1)];
+  }
+}
+
 // The outer solver is a flexible preconditioned CG in FP64 on A = -Lap with the
 // V-cycle (in G precision) as preconditioner.  Signs: r below is rho - Lap(phi)
 // (the reference's residual, conjugate_gradient.c:359-368); the CG residual of
 // A phi = -rho is -r, and the V-cycle applied to r returns z ~ A^-1 (-r).
+// Interior kernels write one partial per chunk (table entry).
 
 // r = rho - Lap(phi) on interior points: FP64 copy, G copy for the V-cycle,
-// per-block partial sums of r^2.
+// partial sums of r^2.
 template <typename G>
 __global__ void k_bmg_residual(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                               const int *__restrict__ intr_g, const double *__restrict__ phi,
-                               const G *__restrict__ rho, double *__restrict__ r, G *__restrict__ r0,
-                               double *__restrict__ part)
+                               const Chunk *__restrict__ itab, const int *__restrict__ intr_g,
+                               const double *__restrict__ phi, const G *__restrict__ rho, double *__restrict__ r,
+                               G *__restrict__ r0, double *__restrict__ part)
 {
-  const DNode &d = nodes[first + blockIdx.y];
-  const BMg &b = bm[blockIdx.y];
+  const unsigned e = xcd_block(blockIdx.x, gridDim.x);
+  const Chunk &c = itab[e];
+  const DNode &d = nodes[first + c.node];
+  const BMg &b = bm[c.node];
   double acc = 0.0;
   if (!b.done)
   {
     const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
     const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
     const double *ph = phi + d.goff;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
     {
       int g = intr_g[d.poff + t];
       double v = (double)rho[d.goff + g] +
@@ -114,84 +143,91 @@ __global__ void k_bmg_residual(const DNode *__restrict__ nodes, int first, const
   }
   acc = block_reduce<0>(acc);
   if (threadIdx.x == 0)
-    part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    part[e] = acc;
 }
 
 // partial dot products of the new preconditioned residual z (V-cycle output):
 // a = sum r z, c = sum z q (q = A p of the previous direction)
 template <typename G>
 __global__ void k_bmg_zdots(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                            const int *__restrict__ intr_g, const G *__restrict__ z0, const double *__restrict__ r,
-                            const double *__restrict__ q, double *__restrict__ part_a, double *__restrict__ part_c)
+                            const Chunk *__restrict__ itab, const int *__restrict__ intr_g, const G *__restrict__ z0,
+                            const double *__restrict__ r, const double *__restrict__ q, double *__restrict__ part_a,
+                            double *__restrict__ part_c)
 {
-  const DNode &d = nodes[first + blockIdx.y];
-  const BMg &b = bm[blockIdx.y];
-  double a = 0.0, c = 0.0;
+  const unsigned e = xcd_block(blockIdx.x, gridDim.x);
+  const Chunk &c = itab[e];
+  const DNode &d = nodes[first + c.node];
+  const BMg &b = bm[c.node];
+  double a = 0.0, cc = 0.0;
   if (!b.done)
   {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
     {
       int g = intr_g[d.poff + t];
       double z = (double)z0[bmg_pos0(d, b, g)];
       a += r[d.goff + g] * z;
-      c += z * q[d.goff + g];
+      cc += z * q[d.goff + g];
     }
   }
   a = block_reduce<0>(a);
-  c = block_reduce<0>(c);
+  cc = block_reduce<0>(cc);
   if (threadIdx.x == 0)
   {
-    part_a[blockIdx.y * gridDim.x + blockIdx.x] = a;
-    part_c[blockIdx.y * gridDim.x + blockIdx.x] = c;
+    part_a[e] = a;
+    part_c[e] = cc;
   }
 }
 
 // p = z + beta p with the flexible (Polak-Ribiere) beta = -alpha (z.q) / (z_old.(-r_old));
-// beta = 0 on the first iteration.  rz of this iteration goes to slot it & 1.
+// beta = 0 on the first iteration.  rz of this iteration goes to slot it & 1
+// (written by the node's first chunk).
 template <typename G>
 __global__ void k_bmg_direction(const DNode *__restrict__ nodes, int first, BMg *__restrict__ bm,
-                                const int *__restrict__ it_base, int it_local, const int *__restrict__ intr_g,
-                                const G *__restrict__ z0, double *__restrict__ p, const double *__restrict__ part_a,
-                                const double *__restrict__ part_c)
+                                const Chunk *__restrict__ itab, const int *__restrict__ it_base, int it_local,
+                                const int *__restrict__ intr_g, const G *__restrict__ z0, double *__restrict__ p,
+                                const double *__restrict__ part_a, const double *__restrict__ part_c)
 {
-  const DNode &d = nodes[first + blockIdx.y];
-  BMg &b = bm[blockIdx.y];
+  const unsigned e = xcd_block(blockIdx.x, gridDim.x);
+  const Chunk &c = itab[e];
+  const DNode &d = nodes[first + c.node];
+  BMg &b = bm[c.node];
   if (b.done)
     return;
   const int it = *it_base + it_local;
-  const int nb = gridDim.x;
-  const double rz = -fold_partials(part_a, blockIdx.y, nb); // (-r).z
+  const double rz = -fold_range(part_a, b.ic0, b.ic1, false); // (-r).z
   double beta = 0.0;
   if (it > 0)
   {
-    const double zq = fold_partials(part_c, blockIdx.y, nb);
+    const double zq = fold_range(part_c, b.ic0, b.ic1, false);
     const double rz_old = b.rz[(it - 1) & 1];
     beta = rz_old != 0.0 ? -b.alpha[(it - 1) & 1] * zq / rz_old : 0.0;
   }
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+  for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
   {
     int g = intr_g[d.poff + t];
     long long o = d.goff + g;
     p[o] = (double)z0[bmg_pos0(d, b, g)] + beta * p[o];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0)
+  if ((int)e == b.ic0 && threadIdx.x == 0)
     b.rz[it & 1] = rz;
 }
 
 // q = A p = -Lap(p) on interior points (p = 0 elsewhere), partial p.q
 __global__ void k_bmg_apply(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                            const int *__restrict__ intr_g, const double *__restrict__ p, double *__restrict__ q,
-                            double *__restrict__ part)
+                            const Chunk *__restrict__ itab, const int *__restrict__ intr_g,
+                            const double *__restrict__ p, double *__restrict__ q, double *__restrict__ part)
 {
-  const DNode &d = nodes[first + blockIdx.y];
-  const BMg &b = bm[blockIdx.y];
+  const unsigned e = xcd_block(blockIdx.x, gridDim.x);
+  const Chunk &c = itab[e];
+  const DNode &d = nodes[first + c.node];
+  const BMg &b = bm[c.node];
   double acc = 0.0;
   if (!b.done)
   {
     const double inv_h2 = (double)(1 << d.lv) * (double)(1 << d.lv);
     const int X = dnode_gX(d), XY = dnode_gXY(d);
     const double *pp = p + d.goff;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
     {
       int g = intr_g[d.poff + t];
       double v = inv_h2 * (6.0 * pp[g] - pp[g + 1] - pp[g - 1] - pp[g + X] - pp[g - X] - pp[g + XY] - pp[g - XY]);
@@ -201,7 +237,7 @@ __global__ void k_bmg_apply(const DNode *__restrict__ nodes, int first, const BM
   }
   acc = block_reduce<0>(acc);
   if (threadIdx.x == 0)
-    part[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+    part[e] = acc;
 }
 
 // alpha = rz / p.q; phi += alpha p; r += alpha q (r = rho - Lap phi); G copy of r
@@ -209,20 +245,23 @@ __global__ void k_bmg_apply(const DNode *__restrict__ nodes, int first, const BM
 // (poisson_error.c:36-81).
 template <typename G>
 __global__ void k_bmg_update(const DNode *__restrict__ nodes, int first, BMg *__restrict__ bm,
-                             const int *__restrict__ it_base, int it_local, const int *__restrict__ intr_g,
-                             double *__restrict__ phi, const double *__restrict__ p, const double *__restrict__ q,
-                             double *__restrict__ r, G *__restrict__ r0, const double *__restrict__ part_pq,
-                             double *__restrict__ part_rs, double *__restrict__ part_viol, double tol_change)
+                             const Chunk *__restrict__ itab, const int *__restrict__ it_base, int it_local,
+                             const int *__restrict__ intr_g, double *__restrict__ phi, const double *__restrict__ p,
+                             const double *__restrict__ q, double *__restrict__ r, G *__restrict__ r0,
+                             const double *__restrict__ part_pq, double *__restrict__ part_rs,
+                             double *__restrict__ part_viol, double tol_change)
 {
-  const DNode &d = nodes[first + blockIdx.y];
-  BMg &b = bm[blockIdx.y];
+  const unsigned e = xcd_block(blockIdx.x, gridDim.x);
+  const Chunk &c = itab[e];
+  const DNode &d = nodes[first + c.node];
+  BMg &b = bm[c.node];
   double acc = 0.0, viol = 0.0;
   if (!b.done)
   {
     const int it = *it_base + it_local;
-    const double pq = fold_partials(part_pq, blockIdx.y, gridDim.x);
+    const double pq = fold_range(part_pq, b.ic0, b.ic1, false);
     const double alpha = pq != 0.0 ? b.rz[it & 1] / pq : 0.0;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.nintr; t += gridDim.x * blockDim.x)
+    for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
     {
       int g = intr_g[d.poff + t];
       long long o = d.goff + g;
@@ -237,15 +276,15 @@ __global__ void k_bmg_update(const DNode *__restrict__ nodes, int first, BMg *__
       acc += rr * rr;
       r0[bmg_pos0(d, b, g)] = (G)rr;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if ((int)e == b.ic0 && threadIdx.x == 0)
       b.alpha[it & 1] = alpha;
   }
   acc = block_reduce<0>(acc);
   viol = block_reduce<1>(viol);
   if (threadIdx.x == 0)
   {
-    part_rs[blockIdx.y * gridDim.x + blockIdx.x] = acc;
-    part_viol[blockIdx.y * gridDim.x + blockIdx.x] = viol;
+    part_rs[e] = acc;
+    part_viol[e] = viol;
   }
 }
 
@@ -253,9 +292,10 @@ __global__ void k_bmg_update(const DNode *__restrict__ nodes, int first, BMg *__
 // [1-tol_change, 1+tol_change] of its previous value (poisson_error.c:36-81)
 // and the residual meets the CG criterion of conj_grad_error
 // (poisson_error.c:236-288): sum r^2 < nintr (tol 4 pi G rho_mean)^2.
+// One workgroup per node.
 __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int first, const int *__restrict__ it_base,
                             int it_local, const double *__restrict__ part_rs, const double *__restrict__ part_viol,
-                            int nb, double tol, double Gc, int *__restrict__ done_count)
+                            double tol, double Gc, int *__restrict__ done_count)
 {
   BMg &b = bm[blockIdx.x];
   if (b.done)
@@ -263,15 +303,8 @@ __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int
   const int outer = *it_base + it_local;
   const int have_change = outer > 0;
   DNode &d = nodes[first + blockIdx.x];
-  double rs = 0.0, viol = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x)
-  {
-    rs += part_rs[blockIdx.x * nb + i];
-    if (have_change)
-      viol = fmax(viol, part_viol[blockIdx.x * nb + i]);
-  }
-  rs = block_reduce<0>(rs);
-  viol = block_reduce<1>(viol);
+  const double rs = fold_range(part_rs, b.ic0, b.ic1, false);
+  const double viol = have_change ? fold_range(part_viol, b.ic0, b.ic1, true) : 0.0;
   if (threadIdx.x == 0)
   {
     const double H = 1.0 / (1 << d.lv);
@@ -286,30 +319,32 @@ __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int
   }
 }
 
-// masked red-black GS half sweep on Lap e = r at MG level m
+// masked red-black GS half sweep on Lap e = r at MG level m; the chunk covers
+// linear points [begin, end) of the level, this colour's points among them
 template <typename G>
-__global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, int m,
-                           G *__restrict__ e, const G *__restrict__ r, const unsigned char *__restrict__ mask,
-                           int color, int zero_start)
+__global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                           const Chunk *__restrict__ tab, int m, G *__restrict__ e, const G *__restrict__ r,
+                           const unsigned char *__restrict__ mask, int color, int zero_start)
 {
-  const BMg &b = bm[blockIdx.y];
-  if (b.done || m >= b.nlev - 1)
+  const Chunk &c = bmg_chunk(tab);
+  const BMg &b = bm[c.node];
+  if (b.done)
     return;
-  const double H = (double)(1 << m) / (double)(1 << nodes[first + blockIdx.y].lv);
+  const double H = (double)(1 << m) / (double)(1 << nodes[first + c.node].lv);
   const G h2 = (G)(H * H);
   const int X = b.X[m], Y = b.Y[m];
   const long long XY = (long long)X * Y, np = XY * b.Z[m], hf = (np + 1) >> 1;
-  const long long cnt = color == 0 ? hf : np - hf;
   G *eb = e + b.off[m];
   const G *rb_ = r + b.off[m];
   const unsigned char *mb = mask + b.off[m];
   const G *o = eb + (color == 0 ? hf : 0);
-  for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < cnt; h += (long long)gridDim.x * blockDim.x)
+  const int h0 = (c.begin + 1 - color) >> 1, h1 = (c.end + 1 - color) >> 1;
+  for (int h = h0 + threadIdx.x; h < h1; h += blockDim.x)
   {
-    long long me = color * hf + h;
+    const long long me = color * hf + h;
     if (!mb[me])
       continue;
-    long long idx = 2 * h + color;
+    const long long idx = 2LL * h + color;
     G sum = zero_start ? (G)0
                        : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + X) >> 1] + o[(idx - X) >> 1] +
                              o[(idx + XY) >> 1] + o[(idx - XY) >> 1];
@@ -318,19 +353,21 @@ __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg
 }
 
 // Full-weighting restriction of the masked fine residual to active coarse
-// points.  It follows a red-black sweep pair that ended on black, so the
-// residual at black fine points is zero and only the 13 red points of the
-// 27-point stencil contribute: the centre (weight 8/64) and the 12 points two
-// offsets away (weight 2/64).
+// points (table over level m+1).  It follows a red-black sweep pair that
+// ended on black, so the residual at black fine points is zero and only the 13
+// red points of the 27-point stencil contribute: the centre (weight 8/64) and
+// the 12 points two offsets away (weight 2/64).
 template <typename G>
-__global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, int m,
-                               const G *__restrict__ ef, const G *__restrict__ rf, const unsigned char *__restrict__ mf,
-                               G *__restrict__ rc, const unsigned char *__restrict__ mc)
+__global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
+                               const Chunk *__restrict__ tab, int m, const G *__restrict__ ef,
+                               const G *__restrict__ rf, const unsigned char *__restrict__ mf, G *__restrict__ rc,
+                               const unsigned char *__restrict__ mc)
 {
-  const BMg &b = bm[blockIdx.y];
-  if (b.done || m + 1 >= b.nlev)
+  const Chunk &c = bmg_chunk(tab);
+  const BMg &b = bm[c.node];
+  if (b.done)
     return;
-  const double Hf = (double)(1 << m) / (double)(1 << nodes[first + blockIdx.y].lv);
+  const double Hf = (double)(1 << m) / (double)(1 << nodes[first + c.node].lv);
   const G inv_h2 = (G)(1.0 / (Hf * Hf));
   const int Xf = b.X[m], Yf = b.Y[m];
   const long long XYf = (long long)Xf * Yf, npf = XYf * b.Z[m], hf = (npf + 1) >> 1;
@@ -350,13 +387,13 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
             inv_h2;
     return rR[h] - lap;
   };
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npc; t += (long long)gridDim.x * blockDim.x)
+  for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
   {
-    long long cp = (t & 1) * hc + (t >> 1);
+    const long long cp = (t & 1) * hc + (t >> 1);
     if (!mc[b.off[m + 1] + cp])
       continue;
     const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
-    const int I = (int)t - JK * Xc, J = JK - K * Yc;
+    const int I = t - JK * Xc, J = JK - K * Yc;
     const long long f = 2 * I + (long long)Xf * (2 * J + (long long)Yf * 2 * K);
     G edges = red_res(f + 1 + Xf) + red_res(f + 1 - Xf) + red_res(f - 1 + Xf) + red_res(f - 1 - Xf) +
               red_res(f + 1 + XYf) + red_res(f + 1 - XYf) + red_res(f - 1 + XYf) + red_res(f - 1 - XYf) +
@@ -366,12 +403,14 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
 }
 
 // trilinear prolongation of the coarse correction into active fine points
+// (table over level m)
 template <typename G>
-__global__ void k_bmg_prolong(const BMg *__restrict__ bm, int m, G *__restrict__ ef,
+__global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restrict__ tab, int m, G *__restrict__ ef,
                               const unsigned char *__restrict__ mf, const G *__restrict__ ec)
 {
-  const BMg &b = bm[blockIdx.y];
-  if (b.done || m + 1 >= b.nlev)
+  const Chunk &ch = bmg_chunk(tab);
+  const BMg &b = bm[ch.node];
+  if (b.done)
     return;
   const int Xf = b.X[m], Yf = b.Y[m];
   const long long npf = (long long)Xf * Yf * b.Z[m], hf = (npf + 1) >> 1;
@@ -382,13 +421,13 @@ __global__ void k_bmg_prolong(const BMg *__restrict__ bm, int m, G *__restrict__
     long long idx = i + (long long)Xc * (j + (long long)Yc * k);
     return (idx & 1) * hc + (idx >> 1);
   };
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npf; t += (long long)gridDim.x * blockDim.x)
+  for (int t = ch.begin + threadIdx.x; t < ch.end; t += blockDim.x)
   {
-    long long fp = (t & 1) * hf + (t >> 1);
+    const long long fp = (t & 1) * hf + (t >> 1);
     if (!mf[b.off[m] + fp])
       continue;
     const int JK = (int)fdiv((unsigned)t, b.dx[m]), k = (int)fdiv((unsigned)t, b.dxy[m]);
-    const int i = (int)t - JK * Xf, j = JK - k * Yf;
+    const int i = t - JK * Xf, j = JK - k * Yf;
     int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
     int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
     G v = (c[PC(i0, j0, k0)] + c[PC(i1, j0, k0)] + c[PC(i0, j1, k0)] + c[PC(i1, j1, k0)] + c[PC(i0, j0, k1)] +
