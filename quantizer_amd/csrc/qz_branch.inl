@@ -68,6 +68,13 @@ struct TreeDev
   std::vector<Chunk> hchunks; // work tables of the level being solved
   Chunk *chunks = nullptr;
   long long chunk_cap = 0;
+  int *alist = nullptr, *acount = nullptr; // active points (pool indices) and their count
+  long long alist_cap = 0;
+  void *sel_tmp = nullptr;
+  size_t sel_bytes = 0;
+  std::vector<int> hrbeg; // region begins in the pools
+  int *rbeg = nullptr, *rstart = nullptr;
+  long long reg_cap = 0;
   long long used_zone = 0, used_cell = 0, used_box = 0, used_pt = 0, used_grid = 0;
 };
 
@@ -98,7 +105,8 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
                     d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask, d->hmark, d->hlist,
                     d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
-                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->chunks};
+                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->chunks, d->alist, d->acount,
+                    d->sel_tmp, d->rbeg, d->rstart};
     for (void *q : ptrs)
       (void)hipFree(q);
     if (d->hflag_count)
@@ -675,6 +683,83 @@ template <typename P, typename G> bool Engine<P, G>::branch_cg_level(size_t l)
   return true;
 }
 
+// Grow-only buffers of the batched node multigrid: level pools (pool points),
+// node table (count), chunk table, per-chunk partials, active list, regions.
+template <typename P, typename G>
+bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool)
+{
+  TreeDev &d = *tree->dev;
+  const int count = (int)d.hbmg.size();
+  if (pool > d.mg_cap)
+  {
+    (void)hipFree(d.mg_e);
+    (void)hipFree(d.mg_r);
+    (void)hipFree(d.mg_mask);
+    (void)hipFree(d.alist);
+    (void)hipFree(d.sel_tmp);
+    d.mg_e = d.mg_r = d.sel_tmp = nullptr;
+    d.mg_mask = nullptr;
+    d.alist = nullptr;
+    d.mg_cap = next_cap(d.mg_cap, pool);
+    d.sel_bytes = qz_select_temp_bytes((int)d.mg_cap);
+    if (!hip_ok(hipMalloc(&d.mg_e, sizeof(G) * d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc(&d.mg_r, sizeof(G) * d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc((void **)&d.mg_mask, d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc((void **)&d.alist, sizeof(int) * d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc(&d.sel_tmp, std::max<size_t>(d.sel_bytes, 1)), "multigrid pool"))
+      return false;
+  }
+  if (!d.acount && !hip_ok(hipMalloc((void **)&d.acount, sizeof(int) * 4), "multigrid pool"))
+    return false;
+  if (count > d.bmg_cap)
+  {
+    (void)hipFree(d.bmg);
+    d.bmg_cap = std::max(count, 2 * d.bmg_cap);
+    if (!hip_ok(hipMalloc((void **)&d.bmg, sizeof(BMg) * d.bmg_cap), "multigrid table"))
+      return false;
+  }
+  const long long nreg = 2LL * count * kMaxMgLevels + 1;
+  if (nreg > d.reg_cap)
+  {
+    (void)hipFree(d.rbeg);
+    (void)hipFree(d.rstart);
+    d.rbeg = d.rstart = nullptr;
+    d.reg_cap = next_cap(d.reg_cap, nreg);
+    if (!hip_ok(hipMalloc((void **)&d.rbeg, sizeof(int) * d.reg_cap), "regions") ||
+        !hip_ok(hipMalloc((void **)&d.rstart, sizeof(int) * d.reg_cap), "regions"))
+      return false;
+  }
+  if (n_chunks > d.chunk_cap)
+  {
+    // keep what is already uploaded (the dense tables precede the list tables)
+    Chunk *nc = nullptr;
+    long long cap = next_cap(d.chunk_cap, n_chunks);
+    if (!hip_ok(hipMalloc((void **)&nc, sizeof(Chunk) * cap), "chunk table"))
+      return false;
+    if (d.chunks)
+    {
+      (void)hipMemcpyAsync(nc, d.chunks, sizeof(Chunk) * d.chunk_cap, hipMemcpyDeviceToDevice, stream);
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(d.chunks);
+    }
+    d.chunks = nc;
+    d.chunk_cap = cap;
+  }
+  if (n_itab + 1024 > d.part_cap)
+  {
+    (void)hipFree(d.part1);
+    (void)hipFree(d.part2);
+    (void)hipFree(d.part3);
+    d.part1 = d.part2 = d.part3 = nullptr;
+    d.part_cap = n_itab + 1024;
+    if (!hip_ok(hipMalloc((void **)&d.part1, sizeof(double) * d.part_cap), "partials") ||
+        !hip_ok(hipMalloc((void **)&d.part2, sizeof(double) * d.part_cap), "partials") ||
+        !hip_ok(hipMalloc((void **)&d.part3, sizeof(double) * d.part_cap), "partials"))
+      return false;
+  }
+  return true;
+}
+
 // Batched masked multigrid for all nodes of a tree level.  Per node the box
 // grid is the finest level, padded on the high side to 2^(M-1) c + 1 points per
 // axis with c <= kBmgCoarseCells; unknowns are the interior points, everything
@@ -733,109 +818,133 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
     }
     mmax = std::max(mmax, M);
   }
-  if (pool > d.mg_cap)
-  {
-    (void)hipFree(d.mg_e);
-    (void)hipFree(d.mg_r);
-    (void)hipFree(d.mg_mask);
-    d.mg_e = d.mg_r = nullptr;
-    d.mg_mask = nullptr;
-    d.mg_cap = next_cap(d.mg_cap, pool);
-    if (!hip_ok(hipMalloc(&d.mg_e, sizeof(G) * d.mg_cap), "multigrid pool") ||
-        !hip_ok(hipMalloc(&d.mg_r, sizeof(G) * d.mg_cap), "multigrid pool") ||
-        !hip_ok(hipMalloc((void **)&d.mg_mask, d.mg_cap), "multigrid pool"))
-      return false;
-  }
-  if (count > d.bmg_cap)
-  {
-    (void)hipFree(d.bmg);
-    d.bmg_cap = std::max(count, 2 * d.bmg_cap);
-    if (!hip_ok(hipMalloc((void **)&d.bmg, sizeof(BMg) * d.bmg_cap), "multigrid table"))
-      return false;
-  }
   // ---- chunk tables: interior points per node; per MG level the linear points
-  // of the nodes that smooth there (S, nlev > m + 1) and of all nodes that have
-  // the level (C, nlev > m)
+  // of every node that has the level (mask set-up)
   std::vector<Chunk> &tab = d.hchunks;
   tab.clear();
-  auto add_chunks = [&](int q, long long n, int step) {
-    for (long long b0 = 0; b0 < n; b0 += step)
-      tab.push_back(Chunk{q, (int)b0, (int)std::min<long long>(n, b0 + step), 0});
+  auto add_chunks = [&](int q, long long b0, long long n, int step, int color) {
+    for (long long x = b0; x < b0 + n; x += step)
+      tab.push_back(Chunk{q, (int)x, (int)std::min<long long>(b0 + n, x + step), color});
   };
   for (int q = 0; q < count; q++)
   {
     hb[q].ic0 = (int)tab.size();
-    add_chunks(q, d.hnodes[first + q].nintr, kBmgChunkIntr);
+    add_chunks(q, 0, d.hnodes[first + q].nintr, kBmgChunkIntr, 0);
     hb[q].ic1 = (int)tab.size();
   }
   const int n_itab = (int)tab.size();
-  int s0[kMaxMgLevels], sn[kMaxMgLevels], c0[kMaxMgLevels], cn[kMaxMgLevels];
-  double sptsm[kMaxMgLevels] = {0.0}; // points over the smoothing nodes at level m
+  int c0[kMaxMgLevels], cn[kMaxMgLevels];
   for (int m = 0; m < mmax; m++)
   {
-    s0[m] = (int)tab.size();
-    for (int q = 0; q < count; q++)
-      if (hb[q].nlev > m + 1)
-      {
-        const long long np = (long long)hb[q].X[m] * hb[q].Y[m] * hb[q].Z[m];
-        add_chunks(q, np, kBmgChunkGrid);
-        sptsm[m] += (double)np;
-      }
-    sn[m] = (int)tab.size() - s0[m];
     c0[m] = (int)tab.size();
     for (int q = 0; q < count; q++)
       if (hb[q].nlev > m)
-        add_chunks(q, (long long)hb[q].X[m] * hb[q].Y[m] * hb[q].Z[m], kBmgChunkGrid);
+        add_chunks(q, 0, (long long)hb[q].X[m] * hb[q].Y[m] * hb[q].Z[m], kBmgChunkGrid, 0);
     cn[m] = (int)tab.size() - c0[m];
   }
-  if ((long long)tab.size() > d.chunk_cap)
-  {
-    (void)hipFree(d.chunks);
-    d.chunks = nullptr;
-    d.chunk_cap = next_cap(d.chunk_cap, (long long)tab.size());
-    if (!hip_ok(hipMalloc((void **)&d.chunks, sizeof(Chunk) * d.chunk_cap), "chunk table"))
-      return false;
-  }
-  if (n_itab + 1024 > d.part_cap)
-  {
-    (void)hipFree(d.part1);
-    (void)hipFree(d.part2);
-    (void)hipFree(d.part3);
-    d.part1 = d.part2 = d.part3 = nullptr;
-    d.part_cap = n_itab + 1024;
-    if (!hip_ok(hipMalloc((void **)&d.part1, sizeof(double) * d.part_cap), "partials") ||
-        !hip_ok(hipMalloc((void **)&d.part2, sizeof(double) * d.part_cap), "partials") ||
-        !hip_ok(hipMalloc((void **)&d.part3, sizeof(double) * d.part_cap), "partials"))
-      return false;
-  }
+  const size_t dense_end = tab.size();
+  if (!ensure_bmg_buffers(n_itab, (long long)dense_end + 16LL * count * mmax + 64, pool))
+    return false;
   G *e = (G *)d.mg_e, *r = (G *)d.mg_r;
   unsigned char *mk = d.mg_mask;
   BMg *bm = d.bmg;
   const Chunk *itab = d.chunks;
-  auto S = [&](int m) { return (const Chunk *)d.chunks + s0[m]; };
   auto C = [&](int m) { return (const Chunk *)d.chunks + c0[m]; };
-  if (!hip_ok(hipMemcpyAsync(bm, hb.data(), sizeof(BMg) * count, hipMemcpyHostToDevice, stream), "multigrid table") ||
-      !hip_ok(hipMemcpyAsync(d.chunks, tab.data(), sizeof(Chunk) * tab.size(), hipMemcpyHostToDevice, stream),
+  if (!hip_ok(hipMemcpyAsync(d.chunks, tab.data(), sizeof(Chunk) * dense_end, hipMemcpyHostToDevice, stream),
               "chunk table"))
     return false;
   QZ_LAUNCH("bmg_setup", k_fill<unsigned char>, nblk(pool), mk, (unsigned char)0, (size_t)pool);
   QZ_LAUNCH("bmg_setup", k_fill<G>, nblk(pool), e, (G)0, (size_t)pool);
   QZ_LAUNCH("bmg_setup", k_fill<int>, 1, d.done_count, 0, (size_t)2);
+  if (!hip_ok(hipMemcpyAsync(bm, hb.data(), sizeof(BMg) * count, hipMemcpyHostToDevice, stream), "multigrid table"))
+    return false;
   if (n_itab > 0)
     QZ_LAUNCH("bmg_setup", k_bmg_mask0, n_itab, d.nodes, first, bm, itab, d.intr_g, mk);
   for (int m = 0; m + 1 < mmax; m++)
     if (cn[m + 1] > 0)
       QZ_LAUNCH("bmg_setup", k_bmg_mask_coarsen, cn[m + 1], bm, C(m + 1), m, mk, mk);
 
+  // ---- active points: the pool indices of all mask bytes, in pool order, and
+  // where each (node, level, colour) region starts in that list
+  std::vector<int> &rbeg = d.hrbeg;
+  rbeg.clear();
+  for (int q = 0; q < count; q++)
+    for (int m = 0; m < hb[q].nlev; m++)
+    {
+      const long long np = (long long)hb[q].X[m] * hb[q].Y[m] * hb[q].Z[m];
+      rbeg.push_back((int)hb[q].off[m]);
+      rbeg.push_back((int)(hb[q].off[m] + ((np + 1) >> 1)));
+    }
+  const int nreg = (int)rbeg.size();
+  rbeg.push_back((int)pool);
+  timer.begin(stream);
+  if (!hip_ok(qz_select_flagged_index(d.sel_tmp, d.sel_bytes, mk, d.alist, d.acount, (int)pool, stream),
+              "active points") ||
+      !hip_ok(hipMemcpyAsync(d.rbeg, rbeg.data(), sizeof(int) * (nreg + 1), hipMemcpyHostToDevice, stream),
+              "regions"))
+    return false;
+  hipLaunchKernelGGL(k_bmg_region_starts, dim3(nblk(nreg + 1)), dim3(kBlock), 0, stream, d.alist, d.acount, d.rbeg,
+                     nreg + 1, d.rstart);
+  timer.end(stream, "bmg_setup");
+  std::vector<int> rstart(nreg + 1);
+  if (!hip_ok(hipMemcpyAsync(rstart.data(), d.rstart, sizeof(int) * (nreg + 1), hipMemcpyDeviceToHost, stream),
+              "regions") ||
+      !hip_ok(hipStreamSynchronize(stream), "regions"))
+    return false;
+  // active-list tables: sweeps per (level, colour) and prolongation per level over
+  // the nodes that smooth there (nlev > m + 1); restriction into level m over all
+  // nodes that have it
+  int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
+  double act_s[kMaxMgLevels] = {0.0}, act_r[kMaxMgLevels] = {0.0}; // active points per table
+  std::vector<int> reg0(count); // first region of each node
+  for (int q = 0, rr = 0; q < count; q++)
+  {
+    reg0[q] = rr;
+    rr += 2 * hb[q].nlev;
+  }
+  for (int m = 0; m < mmax; m++)
+  {
+    for (int c = 0; c < 2; c++)
+    {
+      sw0[m][c] = (int)tab.size();
+      for (int q = 0; q < count; q++)
+        if (hb[q].nlev > m + 1)
+        {
+          const int rg = reg0[q] + 2 * m + c;
+          add_chunks(q, rstart[rg], rstart[rg + 1] - rstart[rg], kBmgChunkAct, c);
+          act_s[m] += rstart[rg + 1] - rstart[rg];
+        }
+      swn[m][c] = (int)tab.size() - sw0[m][c];
+    }
+    rs0[m] = (int)tab.size();
+    for (int q = 0; q < count; q++)
+      if (hb[q].nlev > m)
+      {
+        const int rg = reg0[q] + 2 * m;
+        add_chunks(q, rstart[rg], rstart[rg + 2] - rstart[rg], kBmgChunkAct, 0);
+        act_r[m] += rstart[rg + 2] - rstart[rg];
+      }
+    rsn[m] = (int)tab.size() - rs0[m];
+  }
+  if (!ensure_bmg_buffers(n_itab, (long long)tab.size(), pool) ||
+      !hip_ok(hipMemcpyAsync(d.chunks + dense_end, tab.data() + dense_end, sizeof(Chunk) * (tab.size() - dense_end),
+                             hipMemcpyHostToDevice, stream),
+              "chunk table"))
+    return false;
+  itab = d.chunks;
+  auto SW = [&](int m, int c) { return (const Chunk *)d.chunks + sw0[m][c]; };
+  auto RS = [&](int m) { return (const Chunk *)d.chunks + rs0[m]; };
+
   const double ni = (double)d.lnintr[l], gs = (double)sizeof(G);
   const int inner = std::max(1, cfg.inner_cycles);
   const int pre = std::max(1, cfg.pre_sweeps), post = std::max(0, cfg.post_sweeps);
   int *it_base = d.done_count + 1;
   auto sweep = [&](int m, int color, int zs) {
-    // colour half: mask + r + e of that colour, neighbours of the other colour
-    if (sn[m] > 0)
-      QZ_LAUNCHB("bmg_rbgs", sptsm[m] * (0.5 * (1.0 + 2.0 * gs) + 0.5 * gs), (k_bmg_rbgs<G>), sn[m], d.nodes, first,
-                 bm, S(m), m, e, r, mk, color, zs);
+    // per active point of the colour: list entry, r, e written, 6 neighbours of
+    // the other colour (read once over the sweep)
+    if (swn[m][color] > 0)
+      QZ_LAUNCHB("bmg_rbgs", 0.5 * act_s[m] * (4.0 + 3.0 * gs), (k_bmg_rbgs<G>), swn[m][color], d.nodes, first, bm,
+                 SW(m, color), d.alist, m, e, r, color, zs);
   };
   auto vcycle = [&]() {
     for (int c = 0; c < inner; c++)
@@ -847,16 +956,17 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
           sweep(m, 0, sw == 0 && (m > 0 || c == 0));
           sweep(m, 1, 0);
         }
-        if (cn[m + 1] > 0)
-          QZ_LAUNCHB("bmg_restrict", sptsm[m] * (0.5 * (2.0 * gs + 1.0) + 0.5 * gs) + ptsm[m + 1] * (gs + 1.0),
-                     (k_bmg_restrict<G>), cn[m + 1], d.nodes, first, bm, C(m + 1), m, e, r, mk, r, mk);
+        if (rsn[m + 1] > 0)
+          QZ_LAUNCHB("bmg_restrict", act_s[m] * (1.0 + 2.0 * gs) + act_r[m + 1] * (4.0 + gs), (k_bmg_restrict<G>),
+                     rsn[m + 1], d.nodes, first, bm, RS(m + 1), d.alist, m, e, r, mk, r);
       }
       QZ_LAUNCH("bmg_coarse", (k_bmg_coarse<G>), count, d.nodes, first, bm, e, r, mk, 100);
       for (int m = mmax - 2; m >= 0; m--)
       {
-        if (sn[m] > 0)
-          QZ_LAUNCHB("bmg_prolong", sptsm[m] * (1.0 + 2.0 * gs) + ptsm[m + 1] * gs, (k_bmg_prolong<G>), sn[m], bm,
-                     S(m), m, e, mk, e);
+        const int np_ = swn[m][0] + swn[m][1];
+        if (np_ > 0)
+          QZ_LAUNCHB("bmg_prolong", act_s[m] * (4.0 + 2.0 * gs) + act_r[m + 1] * gs, (k_bmg_prolong<G>), np_, bm,
+                     SW(m, 0), d.alist, m, e, e);
         for (int sw = 0; sw < post; sw++)
         {
           sweep(m, 0, 0);

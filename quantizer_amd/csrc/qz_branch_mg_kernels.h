@@ -9,9 +9,11 @@
 // so the linear-index parity is the red/black colour).
 //
 // Work is handed out through chunk tables: one entry {node, begin, end} per
-// workgroup, over a node's interior points or over the linear points of one of
-// its multigrid levels.  A launch covers every node of the tree level without
-// idle workgroups for small nodes, and converged nodes drop out per chunk.
+// workgroup, over a node's interior points, over the linear points of one of
+// its multigrid levels (mask set-up), or over the list of active points of a
+// level and colour (sweeps, restriction, prolongation: the inactive points of
+// the padded boxes are never visited).  A launch covers every node of the tree
+// level without idle workgroups, and converged nodes drop out per chunk.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,7 +26,8 @@ constexpr int kMaxMgLevels = 8;
 constexpr int kBmgCoarseCells = 8;  // cells per axis of a node's coarsest grid, at most
 constexpr int kBmgMaxOuter = 60;    // outer iterations before the level is handed to CG
 constexpr int kBmgChunkIntr = 1024; // interior points per chunk (4 per thread)
-constexpr int kBmgChunkGrid = 512;  // linear grid points per chunk (one colour point per thread)
+constexpr int kBmgChunkGrid = 512;  // linear grid points per chunk (mask set-up)
+constexpr int kBmgChunkAct = 512;   // active points per chunk (sweeps, restriction, prolongation)
 
 struct BMg
 {
@@ -43,8 +46,29 @@ struct Chunk
 {
   int node; // position in the tree level
   int begin, end;
-  int pad;
+  int color; // colour of the active-list chunks (unused otherwise)
 };
+
+// Start of every region of the active list: rstart[r] = first list entry with
+// pool index >= rbeg[r] (the list is sorted by pool index).
+__global__ void k_bmg_region_starts(const int *__restrict__ alist, const int *__restrict__ acount,
+                                    const int *__restrict__ rbeg, int nreg, int *__restrict__ rstart)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nreg)
+    return;
+  int lo = 0, hi = *acount;
+  const int v = rbeg[r];
+  while (lo < hi)
+  {
+    const int mid = (lo + hi) >> 1;
+    if (alist[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  rstart[r] = lo;
+}
 
 __device__ inline long long bmg_np(const BMg &b, int m) { return (long long)b.X[m] * b.Y[m] * b.Z[m]; }
 
@@ -319,12 +343,12 @@ __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int
   }
 }
 
-// masked red-black GS half sweep on Lap e = r at MG level m; the chunk covers
-// linear points [begin, end) of the level, this colour's points among them
+// red-black GS half sweep on Lap e = r at MG level m over the active points of
+// one colour (chunks of the active list: pool indices of the level's points)
 template <typename G>
 __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                           const Chunk *__restrict__ tab, int m, G *__restrict__ e, const G *__restrict__ r,
-                           const unsigned char *__restrict__ mask, int color, int zero_start)
+                           const Chunk *__restrict__ tab, const int *__restrict__ alist, int m, G *__restrict__ e,
+                           const G *__restrict__ r, int color, int zero_start)
 {
   const Chunk &c = bmg_chunk(tab);
   const BMg &b = bm[c.node];
@@ -334,34 +358,29 @@ __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg
   const G h2 = (G)(H * H);
   const int X = b.X[m], Y = b.Y[m];
   const long long XY = (long long)X * Y, np = XY * b.Z[m], hf = (np + 1) >> 1;
-  G *eb = e + b.off[m];
-  const G *rb_ = r + b.off[m];
-  const unsigned char *mb = mask + b.off[m];
-  const G *o = eb + (color == 0 ? hf : 0);
-  const int h0 = (c.begin + 1 - color) >> 1, h1 = (c.end + 1 - color) >> 1;
-  for (int h = h0 + threadIdx.x; h < h1; h += blockDim.x)
+  const long long base = b.off[m] + color * hf; // this colour's half
+  const G *o = e + b.off[m] + (color == 0 ? hf : 0);
+  for (int i = c.begin + threadIdx.x; i < c.end; i += blockDim.x)
   {
-    const long long me = color * hf + h;
-    if (!mb[me])
-      continue;
-    const long long idx = 2LL * h + color;
+    const long long p = alist[i];
+    const long long idx = 2 * (p - base) + color;
     G sum = zero_start ? (G)0
                        : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + X) >> 1] + o[(idx - X) >> 1] +
                              o[(idx + XY) >> 1] + o[(idx - XY) >> 1];
-    eb[me] = (sum - h2 * rb_[me]) * (G)(1.0 / 6.0);
+    e[p] = (sum - h2 * r[p]) * (G)(1.0 / 6.0);
   }
 }
 
-// Full-weighting restriction of the masked fine residual to active coarse
-// points (table over level m+1).  It follows a red-black sweep pair that
+// Full-weighting restriction of the masked fine residual to the active coarse
+// points (active-list chunks of level m+1).  It follows a red-black sweep pair that
 // ended on black, so the residual at black fine points is zero and only the 13
 // red points of the 27-point stencil contribute: the centre (weight 8/64) and
 // the 12 points two offsets away (weight 2/64).
 template <typename G>
 __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
-                               const Chunk *__restrict__ tab, int m, const G *__restrict__ ef,
-                               const G *__restrict__ rf, const unsigned char *__restrict__ mf, G *__restrict__ rc,
-                               const unsigned char *__restrict__ mc)
+                               const Chunk *__restrict__ tab, const int *__restrict__ alist, int m,
+                               const G *__restrict__ ef, const G *__restrict__ rf, const unsigned char *__restrict__ mf,
+                               G *__restrict__ rc)
 {
   const Chunk &c = bmg_chunk(tab);
   const BMg &b = bm[c.node];
@@ -387,26 +406,25 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
             inv_h2;
     return rR[h] - lap;
   };
-  for (int t = c.begin + threadIdx.x; t < c.end; t += blockDim.x)
+  for (int i = c.begin + threadIdx.x; i < c.end; i += blockDim.x)
   {
-    const long long cp = (t & 1) * hc + (t >> 1);
-    if (!mc[b.off[m + 1] + cp])
-      continue;
+    const long long p = alist[i], cp = p - b.off[m + 1];
+    const int t = (int)(cp < hc ? 2 * cp : 2 * (cp - hc) + 1); // linear index on the coarse level
     const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
     const int I = t - JK * Xc, J = JK - K * Yc;
     const long long f = 2 * I + (long long)Xf * (2 * J + (long long)Yf * 2 * K);
     G edges = red_res(f + 1 + Xf) + red_res(f + 1 - Xf) + red_res(f - 1 + Xf) + red_res(f - 1 - Xf) +
               red_res(f + 1 + XYf) + red_res(f + 1 - XYf) + red_res(f - 1 + XYf) + red_res(f - 1 - XYf) +
               red_res(f + Xf + XYf) + red_res(f + Xf - XYf) + red_res(f - Xf + XYf) + red_res(f - Xf - XYf);
-    rc[b.off[m + 1] + cp] = (G)0.125 * red_res(f) + (G)(2.0 / 64.0) * edges;
+    rc[p] = (G)0.125 * red_res(f) + (G)(2.0 / 64.0) * edges;
   }
 }
 
-// trilinear prolongation of the coarse correction into active fine points
-// (table over level m)
+// trilinear prolongation of the coarse correction into the active fine points
+// (active-list chunks of level m)
 template <typename G>
-__global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restrict__ tab, int m, G *__restrict__ ef,
-                              const unsigned char *__restrict__ mf, const G *__restrict__ ec)
+__global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restrict__ tab, const int *__restrict__ alist,
+                              int m, G *__restrict__ ef, const G *__restrict__ ec)
 {
   const Chunk &ch = bmg_chunk(tab);
   const BMg &b = bm[ch.node];
@@ -421,11 +439,10 @@ __global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restric
     long long idx = i + (long long)Xc * (j + (long long)Yc * k);
     return (idx & 1) * hc + (idx >> 1);
   };
-  for (int t = ch.begin + threadIdx.x; t < ch.end; t += blockDim.x)
+  for (int q = ch.begin + threadIdx.x; q < ch.end; q += blockDim.x)
   {
-    const long long fp = (t & 1) * hf + (t >> 1);
-    if (!mf[b.off[m] + fp])
-      continue;
+    const long long p = alist[q], fp = p - b.off[m];
+    const int t = (int)(fp < hf ? 2 * fp : 2 * (fp - hf) + 1); // linear index on the fine level
     const int JK = (int)fdiv((unsigned)t, b.dx[m]), k = (int)fdiv((unsigned)t, b.dxy[m]);
     const int i = t - JK * Xf, j = JK - k * Yf;
     int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
@@ -433,7 +450,7 @@ __global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restric
     G v = (c[PC(i0, j0, k0)] + c[PC(i1, j0, k0)] + c[PC(i0, j1, k0)] + c[PC(i1, j1, k0)] + c[PC(i0, j0, k1)] +
            c[PC(i1, j0, k1)] + c[PC(i0, j1, k1)] + c[PC(i1, j1, k1)]) *
           (G)0.125;
-    ef[b.off[m] + fp] += v;
+    ef[p] += v;
   }
 }
 

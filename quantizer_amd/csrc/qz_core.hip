@@ -756,6 +756,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool potential_branches() override;
   bool branch_cg_level(size_t l);
   bool branch_mg_level(size_t l, bool &solved);
+  bool ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool);
   template <typename F> bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done);
   bool branch_density();
   bool branch_grid_acceleration();
