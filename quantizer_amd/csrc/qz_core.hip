@@ -523,6 +523,16 @@ template <typename P, typename G> struct Engine final : EngineBase
       timer.end(stream, "coarse_cg");
       return;
     }
+    // the small levels run in one workgroup (the top level keeps the per-level
+    // kernels: it may start from a non-zero correction)
+    if (l > 0 && zero_start && m <= Tail<G>::cells)
+    {
+      timer.begin(stream);
+      hipLaunchKernelGGL(k_vcycle_tail<G>, dim3(1), dim3(kTailThreads), 0, stream, lev_e[l], lev_r[l], m,
+                         cfg.pre_sweeps, cfg.post_sweeps, 64);
+      timer.end(stream, "vcycle_tail");
+      return;
+    }
     const size_t nhalf = rbhalf(np);
     const Cube cf = make_cube(m + 1);
     for (int sw = 0; sw < cfg.pre_sweeps; sw++)

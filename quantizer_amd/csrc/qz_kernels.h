@@ -106,6 +106,32 @@ template <int OP> __device__ inline double block_reduce(double v)
   return r;
 }
 
+// The same reduction for workgroups of up to 1024 threads.
+template <int OP> __device__ inline double block_reduce_n(double v)
+{
+  __shared__ double sh[16];
+  for (int off = 32; off > 0; off >>= 1)
+  {
+    double o = __shfl_down(v, off, 64);
+    v = OP == 0 ? v + o : fmax(v, o);
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    double r = sh[0];
+    for (int w = 1; w < (int)((blockDim.x + 63) >> 6); w++)
+      r = OP == 0 ? r + sh[w] : fmax(r, sh[w]);
+    sh[0] = r;
+  }
+  __syncthreads();
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
 // Second stage: one block folds the per-block partials in a fixed order.
 template <int OP> __global__ void k_reduce_final(const double *__restrict__ part, int np, double *__restrict__ out)
 {
@@ -726,6 +752,194 @@ __global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, C
          C(c0 + di + dj + dk)) *
         (G)0.125;
   e[(t & 1) * hf + (t >> 1)] += v;
+}
+
+// ------------------------------------------------------------------ V-cycle tail in one workgroup
+// The coarse end of a head V-cycle (levels of at most kTailCells<G> cells per
+// side down to the coarsest) in a single launch: e and r of every tail level
+// staged in LDS (colour-split, like the global levels), the same sweeps,
+// 13-point restriction, trilinear prolongation and coarsest-level CG as the
+// per-level kernels, separated by workgroup barriers instead of launches.
+template <typename G> struct Tail
+{
+  static constexpr int cells = sizeof(G) == 4 ? 16 : 8;                 // finest tail level
+  static constexpr int points = sizeof(G) == 4 ? 4913 + 729 + 125 : 729 + 125; // all tail levels
+};
+constexpr int kTailThreads = 1024;
+
+template <typename G>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail(G *__restrict__ e_top, const G *__restrict__ r_top, int n_top, int pre, int post, int cg_iters)
+{
+  __shared__ G se[Tail<G>::points], sr[Tail<G>::points];
+  __shared__ double cx[125], crr[125], cp[125], cap[125];
+  int nl = 0, off[4], nn[4];
+  for (int m = n_top, o = 0; m >= 4; m /= 2, nl++)
+  {
+    nn[nl] = m;
+    off[nl] = o;
+    o += (m + 1) * (m + 1) * (m + 1);
+  }
+  const int tid = threadIdx.x, NT = blockDim.x;
+  {
+    const int np = (n_top + 1) * (n_top + 1) * (n_top + 1);
+    for (int t = tid; t < np; t += NT)
+      sr[t] = r_top[t];
+  }
+  __syncthreads();
+  auto P = [](int idx, int hf) { return (idx & 1) * hf + (idx >> 1); };
+  auto sweep = [&](int l, int color, int zs) {
+    const int m = nn[l], p = m + 1, pp = p * p, np = pp * p, hf = (np + 1) >> 1;
+    G *E = se + off[l];
+    const G *R = sr + off[l];
+    const G h2 = (G)(1.0 / ((double)m * m));
+    const G *o = E + (color == 0 ? hf : 0);
+    const int cnt = color == 0 ? hf : np - hf;
+    for (int h = tid; h < cnt; h += NT)
+    {
+      const int idx = 2 * h + color;
+      const int i = idx % p, j = (idx / p) % p, k = idx / pp;
+      if (i == 0 || i == m || j == 0 || j == m || k == 0 || k == m)
+      {
+        E[color * hf + h] = (G)0;
+        continue;
+      }
+      G sum = zs ? (G)0
+                 : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + p) >> 1] + o[(idx - p) >> 1] +
+                       o[(idx + pp) >> 1] + o[(idx - pp) >> 1];
+      E[color * hf + h] = (sum - h2 * R[color * hf + h]) * (G)(1.0 / 6.0);
+    }
+    __syncthreads();
+  };
+  auto restrict_ = [&](int l) { // level l -> l+1, after a sweep pair ending on black
+    const int m = nn[l], p = m + 1, pp = p * p, hf = (pp * p + 1) >> 1;
+    const int mc = nn[l + 1], pc = mc + 1, ppc = pc * pc, npc = ppc * pc, hc = (npc + 1) >> 1;
+    const G *E = se + off[l], *R = sr + off[l], *EB = E + hf;
+    G *RC = sr + off[l + 1];
+    const G inv_h2 = (G)((double)m * m);
+    auto red_res = [&](int f, int i, int j, int k) -> G {
+      if (i <= 0 || i >= m || j <= 0 || j >= m || k <= 0 || k >= m)
+        return (G)0;
+      G lap = (EB[(f + 1) >> 1] + EB[(f - 1) >> 1] + EB[(f + p) >> 1] + EB[(f - p) >> 1] + EB[(f + pp) >> 1] +
+               EB[(f - pp) >> 1] - (G)6 * E[f >> 1]) *
+              inv_h2;
+      return R[f >> 1] - lap;
+    };
+    for (int t = tid; t < npc; t += NT)
+    {
+      const int I = t % pc, J = (t / pc) % pc, K = t / ppc;
+      G v = (G)0;
+      if (!(I == 0 || I == mc || J == 0 || J == mc || K == 0 || K == mc))
+      {
+        const int i = 2 * I, j = 2 * J, k = 2 * K, f = i + p * j + pp * k;
+        G edges = red_res(f + 1 + p, i + 1, j + 1, k) + red_res(f + 1 - p, i + 1, j - 1, k) +
+                  red_res(f - 1 + p, i - 1, j + 1, k) + red_res(f - 1 - p, i - 1, j - 1, k) +
+                  red_res(f + 1 + pp, i + 1, j, k + 1) + red_res(f + 1 - pp, i + 1, j, k - 1) +
+                  red_res(f - 1 + pp, i - 1, j, k + 1) + red_res(f - 1 - pp, i - 1, j, k - 1) +
+                  red_res(f + p + pp, i, j + 1, k + 1) + red_res(f + p - pp, i, j + 1, k - 1) +
+                  red_res(f - p + pp, i, j - 1, k + 1) + red_res(f - p - pp, i, j - 1, k - 1);
+        v = (G)0.125 * red_res(f, i, j, k) + (G)(2.0 / 64.0) * edges;
+      }
+      RC[P(t, hc)] = v;
+    }
+    __syncthreads();
+  };
+  auto prolong = [&](int l) { // level l+1 -> l, e_l += P e_{l+1}
+    const int m = nn[l], p = m + 1, pp = p * p, np = pp * p, hf = (np + 1) >> 1;
+    const int pc = nn[l + 1] + 1, npc = pc * pc * pc, hc = (npc + 1) >> 1;
+    G *E = se + off[l];
+    const G *EC = se + off[l + 1];
+    for (int t = tid; t < np; t += NT)
+    {
+      const int i = t % p, j = (t / p) % p, k = t / pp;
+      if (i == 0 || i == m || j == 0 || j == m || k == 0 || k == m)
+        continue;
+      const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+      const int di = i & 1, dj = (j & 1) * pc, dk = (k & 1) * pc * pc;
+      const int c0 = i0 + pc * (j0 + pc * k0);
+      G v = (EC[P(c0, hc)] + EC[P(c0 + di, hc)] + EC[P(c0 + dj, hc)] + EC[P(c0 + di + dj, hc)] + EC[P(c0 + dk, hc)] +
+             EC[P(c0 + di + dk, hc)] + EC[P(c0 + dj + dk, hc)] + EC[P(c0 + di + dj + dk, hc)]) *
+            (G)0.125;
+      E[P(t, hf)] += v;
+    }
+    __syncthreads();
+  };
+  auto coarse = [&](int l) { // CG on the 5^3 level (A = -Lap, zero faces)
+    const int m = nn[l], p = m + 1, pp = p * p, np = pp * p, hf = (np + 1) >> 1;
+    const double inv_h2 = (double)m * m;
+    G *E = se + off[l];
+    const G *R = sr + off[l];
+    auto inside = [&](int t) {
+      const int i = t % p, j = (t / p) % p, k = t / pp;
+      return i > 0 && i < m && j > 0 && j < m && k > 0 && k < m;
+    };
+    for (int t = tid; t < np; t += NT)
+    {
+      cx[t] = 0.0;
+      crr[t] = inside(t) ? -(double)R[P(t, hf)] : 0.0;
+      cp[t] = crr[t];
+    }
+    __syncthreads();
+    double rr = 0.0;
+    for (int t = tid; t < np; t += NT)
+      rr += crr[t] * crr[t];
+    rr = block_reduce_n<0>(rr);
+    const double rr0 = rr;
+    for (int it = 0; it < cg_iters && rr > 1e-28 * rr0 && rr > 0.0; it++)
+    {
+      double pap = 0.0;
+      for (int t = tid; t < np; t += NT)
+      {
+        double v = 0.0;
+        if (inside(t))
+          v = inv_h2 * (6.0 * cp[t] - cp[t + 1] - cp[t - 1] - cp[t + p] - cp[t - p] - cp[t + pp] - cp[t - pp]);
+        cap[t] = v;
+        pap += cp[t] * v;
+      }
+      pap = block_reduce_n<0>(pap);
+      const double alpha = rr / pap;
+      double rn = 0.0;
+      for (int t = tid; t < np; t += NT)
+      {
+        cx[t] += alpha * cp[t];
+        crr[t] -= alpha * cap[t];
+        rn += crr[t] * crr[t];
+      }
+      rn = block_reduce_n<0>(rn);
+      const double beta = rn / rr;
+      for (int t = tid; t < np; t += NT)
+        cp[t] = crr[t] + beta * cp[t];
+      rr = rn;
+      __syncthreads();
+    }
+    for (int t = tid; t < np; t += NT)
+      E[P(t, hf)] = (G)cx[t];
+    __syncthreads();
+  };
+  for (int l = 0; l + 1 < nl; l++)
+  {
+    for (int s = 0; s < pre; s++)
+    {
+      sweep(l, 0, s == 0);
+      sweep(l, 1, 0);
+    }
+    restrict_(l);
+  }
+  coarse(nl - 1);
+  for (int l = nl - 2; l >= 0; l--)
+  {
+    prolong(l);
+    for (int s = 0; s < post; s++)
+    {
+      sweep(l, 0, 0);
+      sweep(l, 1, 0);
+    }
+  }
+  {
+    const int np = (n_top + 1) * (n_top + 1) * (n_top + 1);
+    for (int t = tid; t < np; t += NT)
+      e_top[t] = se[t];
+  }
 }
 
 // Coarsest level: conjugate gradients in one workgroup (FP64 internally) on a
