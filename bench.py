@@ -26,6 +26,10 @@ bytes per launch over that time.  traffic: HBM bytes per launch of the same
 kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/, scaled
 by the counters of a plain copy of known size (null when the summary is absent).
 
+precision_vs_fp64: mixed against FP64 on the same workload: head potential,
+grid and particle accelerations after the first force pass, and positions and
+velocities after 20 steps.
+
 cpu_baseline: the compiled reference (oracle/_ref) on a bounded sample of the
 same workload, 1 thread.
 
@@ -163,6 +167,43 @@ def adjoint_ratio(steps=50, every=10):
             "config": f"Plummer N=2^18, head 64^3 (lmin=6), {steps} steps, checkpoint every {every}, mixed"}
 
 
+def precision_study(parts, lmin, lmax, crit, dt, steps=20):
+    """Mixed against the all-FP64 GPU path on the same workload: head potential
+    and grid acceleration, particle accelerations after the first force pass,
+    then the divergence of the trajectories over `steps` fixed-dt steps."""
+    import numpy as np
+    from quantizer_amd import core
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+    out = {}
+    sims = [core.Simulation(parts, lmin, lmax, precision=p, ref_criterion_ptcl=crit)
+            for p in (core.PREC_MIXED, core.PREC_FP64)]
+    try:
+        for s in sims:
+            s.force()
+        hm, hd = sims[0].head_fields(), sims[1].head_fields()
+        pm, pd = sims[0].particles(), sims[1].particles()
+        out["head_potential_rel_err"] = rel(hm[1], hd[1])
+        out["head_acceleration_rel_err"] = max(rel(hm[k], hd[k]) for k in (2, 3, 4))
+        out["particle_acceleration_rel_err"] = max(rel(pm[k], pd[k]) for k in (6, 7, 8))
+        del hm, hd
+        for _ in range(steps):
+            for s in sims:
+                s.step(dt)
+        pm, pd = sims[0].particles(), sims[1].particles()
+        dx = np.sqrt(np.sum((pm[:3] - pd[:3]) ** 2, axis=0))
+        cell = 1.0 / (1 << lmin)
+        out[f"positions_after_{steps}_steps_max_head_cells"] = float(dx.max() / cell)
+        out[f"positions_after_{steps}_steps_rms_head_cells"] = float(np.sqrt(np.mean(dx ** 2)) / cell)
+        out[f"velocities_after_{steps}_steps_rel_err"] = max(rel(pm[k], pd[k]) for k in (3, 4, 5))
+    finally:
+        for s in sims:
+            s.close()
+    return out
+
+
 def cpu_baseline(parts, lmin, lmax, crit, dt, budget_s):
     """Time the compiled reference (oracle/_ref/ref_harness) on this workload: the
     initial force pass, then one timed step, 1 thread."""
@@ -215,6 +256,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-precision", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=150.0)
     ap.add_argument("--cpu-n", type=int, default=1 << 24, help="particles in the CPU sample")
     ap.add_argument("--inner", type=int, default=0)
@@ -251,6 +293,10 @@ def main():
     roof = roofline(sweep) if sweep else None
     if roof:
         log(f"  roofline: {roof['achieved']} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
+    prec = None
+    if not args.skip_precision and not args.skip_fp64:
+        prec = precision_study(parts, lmin, lmax, crit, dt)
+        log(f"  precision study (mixed vs FP64): {prec}")
     adj = None
     if not args.skip_adjoint:
         adj = adjoint_ratio()
@@ -280,6 +326,7 @@ def main():
                    "n_particles": N, "lmin": lmin, "lmax": lmax, "parallelism": "single GPU"},
         "fp64_plain": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "precision_vs_fp64": prec,
         "adjoint_fwd_ratio": adj["value"] if adj else None,
         "adjoint": adj,
         "solver": {k: st_mx[k] for k in ("head_outer", "head_vcycles", "head_residual", "n_nodes", "branch_points",

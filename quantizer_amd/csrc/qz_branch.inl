@@ -25,7 +25,7 @@ struct TreeDev
   int *box = nullptr, *bstart = nullptr, *bend = nullptr;
   long long box_cap = 0;
   int *intr_g = nullptr, *intr_xyz = nullptr, *bdry_g = nullptr, *bdry_xyz = nullptr, *sim_g = nullptr,
-      *sim_xyz = nullptr, *pn_i = nullptr, *pn_b = nullptr, *pn_s = nullptr;
+      *sim_xyz = nullptr;
   long long pt_cap = 0;
   double *phi = nullptr, *phi_old = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
   void *rho = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr;
@@ -65,6 +65,14 @@ struct TreeDev
   void *mg_e = nullptr, *mg_r = nullptr;
   unsigned char *mg_mask = nullptr;
   long long mg_cap = 0;
+  // grid-point classification: chunk table, per-chunk counts and offsets
+  std::vector<int2> hctab;
+  std::vector<int> hchunk0;
+  int2 *ctab = nullptr;
+  int *chunk0 = nullptr, *ccnt = nullptr, *coff = nullptr;
+  long long ctab_cap = 0, chunk0_cap = 0;
+  void *cscan_tmp = nullptr;
+  size_t cscan_bytes = 0;
   std::vector<Chunk> hchunks; // work tables of the level being solved
   Chunk *chunks = nullptr;
   long long chunk_cap = 0;
@@ -92,6 +100,37 @@ template <typename T> static bool regrow(T *&p, long long newcap, long long used
 
 static long long next_cap(long long cap, long long need) { return need <= cap ? cap : std::max(need, cap + cap / 2 + 1024); }
 
+// Grow-only buffers of the grid-point classification.
+static bool reserve_classify(TreeDev &d, int nch, int count)
+{
+  if (nch > d.ctab_cap)
+  {
+    (void)hipFree(d.ctab);
+    (void)hipFree(d.ccnt);
+    (void)hipFree(d.coff);
+    (void)hipFree(d.cscan_tmp);
+    d.ctab = nullptr;
+    d.ccnt = d.coff = nullptr;
+    d.cscan_tmp = nullptr;
+    d.ctab_cap = next_cap(d.ctab_cap, nch);
+    d.cscan_bytes = qz_scan_temp_bytes((int)d.ctab_cap);
+    if (!hip_ok(hipMalloc((void **)&d.ctab, sizeof(int2) * d.ctab_cap), "classify") ||
+        !hip_ok(hipMalloc((void **)&d.ccnt, 3 * sizeof(int) * d.ctab_cap), "classify") ||
+        !hip_ok(hipMalloc((void **)&d.coff, 3 * sizeof(int) * d.ctab_cap), "classify") ||
+        !hip_ok(hipMalloc(&d.cscan_tmp, std::max<size_t>(d.cscan_bytes, 1)), "classify"))
+      return false;
+  }
+  if (count + 1 > d.chunk0_cap)
+  {
+    (void)hipFree(d.chunk0);
+    d.chunk0 = nullptr;
+    d.chunk0_cap = next_cap(d.chunk0_cap, count + 1);
+    if (!hip_ok(hipMalloc((void **)&d.chunk0, sizeof(int) * d.chunk0_cap), "classify"))
+      return false;
+  }
+  return true;
+}
+
 template <typename P, typename G> void Engine<P, G>::release_tree()
 {
   if (!tree)
@@ -101,12 +140,13 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
   {
     void *ptrs[] = {d->nodes, d->zone_node, d->zone_xyz, d->cell_xyz, d->cell_box, d->cstart, d->cend, d->box,
                     d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
-                    d->pn_i, d->pn_b, d->pn_s, d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
+                    d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
                     d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask, d->hmark, d->hlist,
                     d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
                     d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->chunks, d->alist, d->acount,
-                    d->sel_tmp, d->rbeg, d->rstart};
+                    d->sel_tmp, d->rbeg, d->rstart, d->ctab, d->chunk0, d->ccnt,
+                    d->coff, d->cscan_tmp};
     for (void *q : ptrs)
       (void)hipFree(q);
     if (d->hflag_count)
@@ -151,8 +191,7 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
   {
     long long u = d->used_pt;
     if (!regrow(d->intr_g, c, u, "point pool") || !regrow(d->bdry_g, c, u, "point pool") ||
-        !regrow(d->sim_g, c, u, "point pool") || !regrow(d->pn_i, c, u, "point pool") ||
-        !regrow(d->pn_b, c, u, "point pool") || !regrow(d->pn_s, c, u, "point pool") ||
+        !regrow(d->sim_g, c, u, "point pool") ||
         !regrow(d->intr_xyz, 3 * c, 3 * u, "point pool") || !regrow(d->bdry_xyz, 3 * c, 3 * u, "point pool") ||
         !regrow(d->sim_xyz, 3 * c, 3 * u, "point pool"))
       return false;
@@ -444,10 +483,44 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         d.mpart_cap = cap;
       }
     }
-    timer.begin(stream);
-    hipLaunchKernelGGL(k_classify, dim3(count), dim3(kClassBlock), 0, stream, d.nodes, first, d.box, d.intr_g,
-                       d.intr_xyz, d.bdry_g, d.bdry_xyz, d.sim_g, d.sim_xyz, d.pn_i, d.pn_b, d.pn_s);
-    timer.end(stream, "tree_classify");
+    // grid-point classification: chunks of kClassBlock points per node
+    {
+      std::vector<int2> &ct = d.hctab;
+      std::vector<int> &c0v = d.hchunk0;
+      ct.clear();
+      c0v.assign(count + 1, 0);
+      for (int q = 0; q < count; q++)
+      {
+        c0v[q] = (int)ct.size();
+        const long long pc = th.nodes[first + q].pcap;
+        for (long long t0 = 0; t0 < pc; t0 += kClassBlock)
+          ct.push_back(make_int2(q, (int)t0));
+      }
+      c0v[count] = (int)ct.size();
+      const int nch = (int)ct.size();
+      if (nch > 0)
+      {
+        if (!reserve_classify(d, nch, count))
+          return false;
+        if (!hip_ok(hipMemcpyAsync(d.ctab, ct.data(), sizeof(int2) * nch, hipMemcpyHostToDevice, stream), "classify") ||
+            !hip_ok(hipMemcpyAsync(d.chunk0, c0v.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, stream),
+                    "classify"))
+          return false;
+        timer.begin(stream);
+        hipLaunchKernelGGL(k_classify_count, dim3(nch), dim3(kClassBlock), 0, stream, d.nodes, first, d.box, d.ctab,
+                           nch, d.ccnt);
+        for (int k = 0; k < 3; k++)
+          if (!hip_ok(qz_exclusive_scan(d.cscan_tmp, d.cscan_bytes, d.ccnt + (size_t)k * nch, d.coff + (size_t)k * nch,
+                                        nch, stream),
+                      "classify"))
+            return false;
+        hipLaunchKernelGGL(k_classify_write, dim3(nch), dim3(kClassBlock), 0, stream, d.nodes, first, d.box, d.ctab,
+                           nch, d.coff, d.chunk0, d.intr_g, d.intr_xyz, d.bdry_g, d.bdry_xyz, d.sim_g, d.sim_xyz);
+        hipLaunchKernelGGL(k_classify_totals, dim3(nblk(count)), dim3(kBlock), 0, stream, d.nodes, first, count, nch,
+                           d.ccnt, d.coff, d.chunk0);
+        timer.end(stream, "tree_classify");
+      }
+    }
     QZ_LAUNCH("tree_children", k_mark_children, nblk(nz), d.nodes, d.zone_node, d.zone_xyz, (int)z0, (int)nz, n,
               d.head_child, d.box);
     // zero the level's grid fields (points that are never listed stay 0, as calloc leaves them)

@@ -242,105 +242,126 @@ __host__ __device__ inline int classify_pattern(int e)
   return exist | (bdry << 1);
 }
 
-// One block per node: classify the points of the smallest box grid in the
-// reference's loop order (kk, jj, ii) and compact them, in order, into the
+// Classification of point t (loop order kk, jj, ii over the smallest box grid)
+// of a node: -1 no point, 0 interior, 1 boundary, 2 simulation boundary.
+__device__ inline int classify_point(const DNode &d, const int *__restrict__ bx, int t, int &gidx, int &gx, int &gy,
+                                     int &gz)
+{
+  const int lx = d.bmax[0] - d.bmin[0] + 2, ly = d.bmax[1] - d.bmin[1] + 2;
+  const int rX = d.rd[0], rXY = d.rd[0] * d.rd[1];
+  const int ii = d.bmin[0] - d.ts[0] + t % lx;
+  const int jj = d.bmin[1] - d.ts[1] + (t / lx) % ly;
+  const int kk = d.bmin[2] - d.ts[2] + t / (lx * ly);
+  const int b = ii + jj * rX + kk * rXY;
+  const int nb[8] = {b, b - 1, b - rX, b - 1 - rX, b - rXY, b - 1 - rXY, b - rX - rXY, b - 1 - rX - rXY};
+  int e = 0;
+  for (int qq = 0; qq < 8; qq++)
+    e |= (bx[nb[qq]] > -4 ? 1 : 0) << qq;
+  const int cls = classify_pattern(e);
+  if (!(cls & 1))
+    return -1;
+  const int bd = (cls >> 1) & 1, top = 1 << d.lv;
+  gx = ii + d.ts[0];
+  gy = jj + d.ts[1];
+  gz = kk + d.ts[2];
+  gidx = ii + jj * (d.rd[0] + 1) + kk * (d.rd[0] + 1) * (d.rd[1] + 1);
+  const bool simb = d.simc != 0 && bd && (gx == 0 || gx == top || gy == 0 || gy == top || gz == 0 || gz == top);
+  return simb ? 2 : (bd ? 1 : 0);
+}
+
+// The node's points are split into chunks of kClassBlock consecutive points, one
+// workgroup each (ctab: node position in the level, first point).  Pass 1
+// counts each kind per chunk; an exclusive scan over the chunks of the level
+// gives every chunk its offset; pass 2 writes the points, in order, into the
 // node's interior / boundary / simulation-boundary slots.
 constexpr int kClassBlock = 1024;
 __global__ void __launch_bounds__(kClassBlock)
-    k_classify(DNode *__restrict__ nodes, int first, const int *__restrict__ box, int *__restrict__ intr_g,
-               int *__restrict__ intr_xyz, int *__restrict__ bdry_g, int *__restrict__ bdry_xyz,
-               int *__restrict__ sim_g, int *__restrict__ sim_xyz, int *__restrict__ pt_node_i,
-               int *__restrict__ pt_node_b, int *__restrict__ pt_node_s)
+    k_classify_count(const DNode *__restrict__ nodes, int first, const int *__restrict__ box,
+                     const int2 *__restrict__ ctab, int nch, int *__restrict__ cnt)
 {
   __shared__ int wsum[3][kClassBlock / 64];
-  __shared__ int base[3];
-  const int q = first + blockIdx.x;
-  DNode &d = nodes[q];
-  const int lx = d.bmax[0] - d.bmin[0] + 2, ly = d.bmax[1] - d.bmin[1] + 2, lz = d.bmax[2] - d.bmin[2] + 2;
-  const int total = lx * ly * lz;
-  const int rX = d.rd[0], rXY = d.rd[0] * d.rd[1];
-  const int gX = d.rd[0] + 1, gXY = (d.rd[0] + 1) * (d.rd[1] + 1);
-  const int *bx = box + d.boff;
-  const int top = 1 << d.lv;
-  if (threadIdx.x < 3)
-    base[threadIdx.x] = 0;
-  __syncthreads();
+  const int2 c = ctab[blockIdx.x];
+  const DNode &d = nodes[first + c.x];
+  const int total = (d.bmax[0] - d.bmin[0] + 2) * (d.bmax[1] - d.bmin[1] + 2) * (d.bmax[2] - d.bmin[2] + 2);
+  const int t = c.y + threadIdx.x;
+  int kind = -1, gidx, gx, gy, gz;
+  if (t < total)
+    kind = classify_point(d, box + d.boff, t, gidx, gx, gy, gz);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int chunk = 0; chunk < total; chunk += kClassBlock)
+  for (int k = 0; k < 3; k++)
   {
-    int t = chunk + threadIdx.x;
-    int kind = -1; // 0 intr, 1 bdry, 2 sim
-    int gidx = 0, gx = 0, gy = 0, gz = 0;
-    if (t < total)
-    {
-      int ii = d.bmin[0] - d.ts[0] + t % lx;
-      int jj = d.bmin[1] - d.ts[1] + (t / lx) % ly;
-      int kk = d.bmin[2] - d.ts[2] + t / (lx * ly);
-      int b = ii + jj * rX + kk * rXY;
-      const int nb[8] = {b, b - 1, b - rX, b - 1 - rX, b - rXY, b - 1 - rXY, b - rX - rXY, b - 1 - rX - rXY};
-      int e = 0;
-      for (int qq = 0; qq < 8; qq++)
-        e |= (bx[nb[qq]] > -4 ? 1 : 0) << qq;
-      int cls = classify_pattern(e);
-      if (cls & 1)
-      {
-        int bd = (cls >> 1) & 1;
-        gx = ii + d.ts[0];
-        gy = jj + d.ts[1];
-        gz = kk + d.ts[2];
-        bool simb = d.simc != 0 && bd && (gx == 0 || gx == top || gy == 0 || gy == top || gz == 0 || gz == top);
-        kind = simb ? 2 : (bd ? 1 : 0);
-        gidx = ii + jj * gX + kk * gXY;
-      }
-    }
-    // ordered compaction: wave ballots, then a block prefix over waves
-    int my_off[3];
-    for (int c = 0; c < 3; c++)
-    {
-      unsigned long long mask = __ballot(kind == c);
-      int below = __popcll(mask & ((1ull << lane) - 1ull));
-      my_off[c] = below;
-      if (lane == 0)
-        wsum[c][wid] = __popcll(mask);
-    }
-    __syncthreads();
-    int nw = kClassBlock / 64;
-    int wpre[3];
-    for (int c = 0; c < 3; c++)
-    {
-      int acc = 0;
-      for (int w = 0; w < wid; w++)
-        acc += wsum[c][w];
-      wpre[c] = acc;
-    }
-    if (kind >= 0)
-    {
-      int slot = d.poff + base[kind] + wpre[kind] + my_off[kind];
-      int *g = kind == 0 ? intr_g : (kind == 1 ? bdry_g : sim_g);
-      int *xyz = kind == 0 ? intr_xyz : (kind == 1 ? bdry_xyz : sim_xyz);
-      int *pn = kind == 0 ? pt_node_i : (kind == 1 ? pt_node_b : pt_node_s);
-      g[slot] = gidx;
-      xyz[3 * slot] = gx;
-      xyz[3 * slot + 1] = gy;
-      xyz[3 * slot + 2] = gz;
-      pn[slot] = q;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3)
-    {
-      int acc = 0;
-      for (int w = 0; w < nw; w++)
-        acc += wsum[threadIdx.x][w];
-      base[threadIdx.x] += acc;
-    }
-    __syncthreads();
+    const unsigned long long m = __ballot(kind == k);
+    if (lane == 0)
+      wsum[k][wid] = __popcll(m);
   }
-  if (threadIdx.x == 0)
+  __syncthreads();
+  if (threadIdx.x < 3)
   {
-    d.nintr = base[0];
-    d.nbdry = base[1];
-    d.nsim = base[2];
+    int acc = 0;
+    for (int w = 0; w < kClassBlock / 64; w++)
+      acc += wsum[threadIdx.x][w];
+    cnt[threadIdx.x * nch + blockIdx.x] = acc;
   }
+}
+
+__global__ void __launch_bounds__(kClassBlock)
+    k_classify_write(const DNode *__restrict__ nodes, int first, const int *__restrict__ box,
+                     const int2 *__restrict__ ctab, int nch, const int *__restrict__ off,
+                     const int *__restrict__ chunk0, int *__restrict__ intr_g, int *__restrict__ intr_xyz,
+                     int *__restrict__ bdry_g, int *__restrict__ bdry_xyz, int *__restrict__ sim_g,
+                     int *__restrict__ sim_xyz)
+{
+  __shared__ int wsum[3][kClassBlock / 64];
+  const int2 c = ctab[blockIdx.x];
+  const DNode &d = nodes[first + c.x];
+  const int total = (d.bmax[0] - d.bmin[0] + 2) * (d.bmax[1] - d.bmin[1] + 2) * (d.bmax[2] - d.bmin[2] + 2);
+  const int t = c.y + threadIdx.x;
+  int kind = -1, gidx = 0, gx = 0, gy = 0, gz = 0;
+  if (t < total)
+    kind = classify_point(d, box + d.boff, t, gidx, gx, gy, gz);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int my_off[3];
+  for (int k = 0; k < 3; k++)
+  {
+    const unsigned long long m = __ballot(kind == k);
+    my_off[k] = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0)
+      wsum[k][wid] = __popcll(m);
+  }
+  __syncthreads();
+  if (kind >= 0)
+  {
+    int wpre = 0;
+    for (int w = 0; w < wid; w++)
+      wpre += wsum[kind][w];
+    const int f = chunk0[c.x];
+    const int slot = d.poff + off[kind * nch + blockIdx.x] - off[kind * nch + f] + wpre + my_off[kind];
+    int *g = kind == 0 ? intr_g : (kind == 1 ? bdry_g : sim_g);
+    int *xyz = kind == 0 ? intr_xyz : (kind == 1 ? bdry_xyz : sim_xyz);
+    g[slot] = gidx;
+    xyz[3 * slot] = gx;
+    xyz[3 * slot + 1] = gy;
+    xyz[3 * slot + 2] = gz;
+  }
+}
+
+// per node: totals of each kind from the chunk counts and offsets
+__global__ void k_classify_totals(DNode *__restrict__ nodes, int first, int count, int nch,
+                                  const int *__restrict__ cnt, const int *__restrict__ off,
+                                  const int *__restrict__ chunk0)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count)
+    return;
+  const int f = chunk0[q], l = chunk0[q + 1];
+  int tot[3] = {0, 0, 0};
+  if (l > f)
+    for (int k = 0; k < 3; k++)
+      tot[k] = off[k * nch + l - 1] + cnt[k * nch + l - 1] - off[k * nch + f];
+  DNode &d = nodes[first + q];
+  d.nintr = tot[0];
+  d.nbdry = tot[1];
+  d.nsim = tot[2];
 }
 
 // Refinement criterion counts (tree_construction.c:529): cells of the level
