@@ -154,6 +154,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
   bool keys_valid = false;
+  bool acc_live = false; // particle accelerations hold values (false: all zero, as reset leaves them)
+  double *phi_prev = nullptr; // the previous converged head potential
+  int phi_hist = 0;           // converged head potentials since the last reset of the history
   // captured outer iteration of the head solve, per (phi, rhs) pair: [0] the
   // potential, [1] the adjoint solve
   hipGraphExec_t head_graph[2] = {nullptr, nullptr};
@@ -279,6 +282,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
     (void)hipFree(part);
     (void)hipFree(red);
+    (void)hipFree(phi_prev);
     if (hred)
       (void)hipHostFree(hred);
     for (int q = 0; q < 2; q++)
@@ -386,8 +390,10 @@ template <typename P, typename G> struct Engine final : EngineBase
       return false;
     for (int a = 6; a < 9; a++)
       QZ_LAUNCH("fill", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
+    acc_live = false;
     if (init_head)
     {
+      phi_hist = 0;
       size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
       QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1), cfg.G * total_mass,
                 cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
@@ -444,6 +450,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 6; a < 9; a++)
       QZ_LAUNCH("reset", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
+    acc_live = false;
     QZ_LAUNCH("reset", k_fill<G>, nblk(npts), rho, (G)0, npts);
     return hip_ok(hipGetLastError(), "reset"); // stream-ordered; readers synchronise
   }
@@ -482,11 +489,14 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
     src.id = id_a;
     dst.id = id_b;
-    QZ_LAUNCHB("permute", (6.0 * 8 + 14.0 * sizeof(P) + 12.0) * N, k_permute<P>, nblk(N), src, dst, slot_out, N);
+    const int move_acc = acc_live ? 1 : 0;
+    QZ_LAUNCHB("permute", (6.0 * 8 + (move_acc ? 14.0 : 8.0) * sizeof(P) + 12.0) * N, k_permute<P>, nblk(N), src, dst,
+               slot_out, N, move_acc);
     for (int a = 0; a < 3; a++)
       std::swap(pos[a], posb[a]);
     for (int a = 3; a < 10; a++)
-      std::swap(pa[a], pb[a]);
+      if (move_acc || a < 6 || a == 9)
+        std::swap(pa[a], pb[a]);
     std::swap(id_a, id_b);
     QZ_LAUNCHB("cell_start", 8.0 * N + 4.0 * n * n * n, k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n, cell_start);
     keys_valid = true;
@@ -659,10 +669,27 @@ template <typename P, typename G> struct Engine final : EngineBase
     return ok;
   }
 
+  // Head potential.  The solve starts from the linear extrapolation of the last
+  // two converged potentials once two exist (the reference starts from the last
+  // one, reset.c:54-57); the stopping criteria are the same either way.
   bool potential_head() override
   {
     HostSpan span(timer, "potential_head");
-    return solve_head(phi, rho, false);
+    const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    if (phi_hist == 2)
+      QZ_LAUNCHB("extrapolate", 32.0 * npts, k_extrapolate, nblk(npts), phi, phi_prev, npts);
+    const bool ok = solve_head(phi, rho, false);
+    if (ok && phi_hist == 0)
+    {
+      if (!phi_prev)
+        phi_prev = dmalloc<double>(npts, bytes);
+      if (!phi_prev || !hip_ok(hipMemcpyAsync(phi_prev, phi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream),
+                               "potential history"))
+        return false;
+    }
+    if (ok && phi_hist < 2)
+      phi_hist++;
+    return ok;
   }
 
   bool grid_acceleration() override
@@ -691,6 +718,7 @@ template <typename P, typename G> struct Engine final : EngineBase
       QZ_LAUNCHB("gather_head", (24.0 + 3.0 * sizeof(P)) * N + 3.0 * sizeof(G) * npts, (k_gather_head<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, pa[6], pa[7],
                 pa[8], N, n);
     }
+    acc_live = true;
     return hip_ok(hipGetLastError(), "particle_acceleration"); // stream-ordered; readers synchronise
   }
 
@@ -718,6 +746,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool set_head_potential(const double *pot) override
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    phi_hist = 0;
     return hip_ok(hipMemcpy(phi, pot, sizeof(double) * npts, hipMemcpyHostToDevice), "upload phi");
   }
 

@@ -209,7 +209,7 @@ template <typename P> struct PtclArrays
 };
 
 template <typename P>
-__global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__restrict__ perm, int N)
+__global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__restrict__ perm, int N, int move_acc)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N)
@@ -220,7 +220,8 @@ __global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__res
     dst.x[a][i] = src.x[a][s];
 #pragma unroll
   for (int a = 0; a < 7; a++)
-    dst.a[a][i] = src.a[a][s];
+    if (move_acc || a < 3 || a == 6) // accelerations that are all zero stay where they are
+      dst.a[a][i] = src.a[a][s];
   dst.id[i] = src.id[s];
 }
 
@@ -652,6 +653,18 @@ __global__ void k_outer_correct(double *__restrict__ phi, const G *__restrict__ 
   dev = block_reduce<1>(dev);
   if (threadIdx.x == 0)
     part[blockIdx.x] = dev;
+}
+
+// Starting guess of a head solve from the last two converged potentials:
+// phi <- 2 phi - phi_prev, phi_prev <- phi (faces are fixed and stay equal).
+__global__ void k_extrapolate(double *__restrict__ phi, double *__restrict__ phi_prev, size_t n)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n)
+    return;
+  const double c = phi[t];
+  phi[t] = 2.0 * c - phi_prev[t];
+  phi_prev[t] = c;
 }
 
 // Red-black Gauss-Seidel half sweep on the error equation Lap e = r
