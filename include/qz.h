@@ -159,6 +159,20 @@ int qz_timing_enable(qz_ctx *ctx, int on);
 int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *launches, double *bytes, int max);
 int qz_sync(qz_ctx *ctx);
 
+/* Snapshot in the reference's binary layout (output_snapshots.c:141-226): mass,
+ * x, y, z, vx, vy, vz (N values each, original particle order, user units:
+ * x_user = (x - 1/2) / dist + cm + cm_v t, v_user = v / velocity + cm_v), then,
+ * with observables, E_kin, E_pot (exact pairwise), E_tot and the momentum
+ * (observables.c:38-112, 622-696), then t = actualtime / time; all doubles.
+ * The unit factors are the reference's _conversion_dist_, _conversion_velocity_,
+ * _conversion_time_ and GL_cm_* (input.c:252-295). */
+typedef struct
+{
+  double dist, velocity, time;
+  double cm[3], cm_v[3];
+} qz_units;
+int qz_write_snapshot(qz_ctx *ctx, const char *path, const qz_units *units, double actualtime, int with_observables);
+
 /* measurement helpers (no counterpart in the reference): the head's finest red
  * half sweep of the multigrid smoother replayed `reps` times back to back,
  * returning ms and algorithmic bytes per launch; and one plain device copy of

@@ -19,6 +19,9 @@
  *   s0_acc.bin    particle ax, ay, az after the first force computation
  *   s0_nodes.bin  branch-node geometry and fields after the first force pass
  *   sK_ptcl.bin   x,y,z,vx,vy,vz,ax,ay,az after nsteps fixed-dt steps
+ * and with --snapshot dist velocity time cmx cmy cmz cmvx cmvy cmvz the
+ * reference's own snapshot file of the final state (output_snapshots.c) under
+ * ../output/ relative to the working directory
  *   sK_head.bin   head fields after the last step
  *   energies.txt  E_kin E_pot E_tot px py pz at step 0 and step K
  */
@@ -287,7 +290,8 @@ int main(int argc, char **argv)
   int ref_crit = atoi(argv[5]);
   int nsteps = atoi(argv[6]);
   double dt = atof(argv[7]);
-  int do_time = 0, threads = 1, energy = 1;
+  int do_time = 0, threads = 1, energy = 1, snap = 0;
+  double snap_u[9] = {1, 1, 1, 0, 0, 0, 0, 0, 0}; /* dist, velocity, time, cm x y z, cm v x y z */
   for (int a = 8; a < argc; a++)
   {
     if (!strcmp(argv[a], "--time"))
@@ -296,6 +300,12 @@ int main(int argc, char **argv)
       threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--noenergy"))
       energy = 0;
+    else if (!strcmp(argv[a], "--snapshot") && a + 9 < argc)
+    {
+      snap = 1;
+      for (int q = 0; q < 9; q++)
+        snap_u[q] = atof(argv[++a]);
+    }
   }
 
   FILE *fin = fopen(inpath, "rb");
@@ -393,6 +403,25 @@ int main(int argc, char **argv)
   }
   if (fe)
     fclose(fe);
+  if (snap)
+  {
+    /* output_snapshots.c:141-226 with the given unit conversions, into
+     * ../output/<simulation>/Data/Data_0.bin relative to the working directory */
+    _conversion_dist_ = snap_u[0];
+    _conversion_velocity_ = snap_u[1];
+    _conversion_time_ = snap_u[2];
+    GL_cm_x = snap_u[3];
+    GL_cm_y = snap_u[4];
+    GL_cm_z = snap_u[5];
+    GL_cm_vx = snap_u[6];
+    GL_cm_vy = snap_u[7];
+    GL_cm_vz = snap_u[8];
+    compute_observables_FLAG = energy ? true : false;
+    if (energy)
+      observables();
+    if (output_snapshots((vtype)(nsteps * dt), 0) == _FAILURE_)
+      return 1;
+  }
   fprintf(stderr, "REF_DONE N=%d steps=%d levels_used=%d\n", n, nsteps, GL_tentacles_level_max);
   return 0;
 }

@@ -86,8 +86,13 @@ EXPORTED = [
     "qz_particle_updating_B", "qz_force", "qz_step", "qz_get_head_fields", "qz_set_head_potential",
     "qz_num_nodes", "qz_node_header", "qz_node_grid_lists", "qz_node_fields", "qz_get_stats", "qz_observables",
     "qz_timestep", "qz_force_vjp", "qz_adjoint_forward", "qz_adjoint_backward", "qz_keys", "qz_timing_enable", "qz_timing_report", "qz_sync",
-    "qz_bench_sweep", "qz_copy_probe",
+    "qz_bench_sweep", "qz_copy_probe", "qz_write_snapshot",
 ]
+
+
+class QzUnits(ctypes.Structure):
+    _fields_ = [("dist", ctypes.c_double), ("velocity", ctypes.c_double), ("time", ctypes.c_double),
+                ("cm", ctypes.c_double * 3), ("cm_v", ctypes.c_double * 3)]
 
 
 def load_library(path: str = LIB_PATH):
@@ -252,6 +257,17 @@ class Simulation:
 
     def sync(self):
         return _check(load_library().qz_sync(self._h), "sync")
+
+    def write_snapshot(self, path: str, actualtime: float, dist: float = 1.0, velocity: float = 1.0,
+                       time_unit: float = 1.0, cm=(0.0, 0.0, 0.0), cm_v=(0.0, 0.0, 0.0),
+                       observables: bool = True):
+        """The state as a snapshot file in the reference's layout (output_snapshots.c)."""
+        lib = load_library()
+        u = QzUnits(dist, velocity, time_unit, (ctypes.c_double * 3)(*cm), (ctypes.c_double * 3)(*cm_v))
+        lib.qz_write_snapshot.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(QzUnits), ctypes.c_double,
+                                          ctypes.c_int]
+        _check(lib.qz_write_snapshot(self._h, os.fsencode(path), ctypes.byref(u), actualtime, int(observables)),
+               "qz_write_snapshot")
 
     # ------------------------------------------------------------ measurement
     def bench_sweep(self, reps: int = 50):

@@ -154,7 +154,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
   bool keys_valid = false;
-  bool acc_live = false; // particle accelerations hold values (false: all zero, as reset leaves them)
+  bool acc_live = false;
+  std::vector<double> host_mass; // the masses as given (snapshots write them unchanged) // particle accelerations hold values (false: all zero, as reset leaves them)
   double *phi_prev = nullptr; // the previous converged head potential
   int phi_hist = 0;           // converged head potentials since the last reset of the history
   // captured outer iteration of the head solve, per (phi, rhs) pair: [0] the
@@ -380,6 +381,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
     std::vector<int> ids(N);
     double mt = 0.0;
+    host_mass.assign(src[6], src[6] + N);
     for (int i = 0; i < N; i++)
     {
       ids[i] = i;
@@ -780,6 +782,64 @@ template <typename P, typename G> struct Engine final : EngineBase
 
   bool timestep(double cfl, double dt_max, double *dt) override;
 
+  // Snapshot file in the reference layout (output_snapshots.c:141-226).  The
+  // kinetic energy and momentum are FP64 sums on the host over the downloaded
+  // velocities; the exact pairwise potential energy comes from the device.
+  bool write_snapshot(const char *path, const qz_units &u, double actualtime, bool with_obs) override
+  {
+    std::vector<double> x(N), y(N), z(N), vx(N), vy(N), vz(N);
+    double *dst[9] = {x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(), nullptr, nullptr, nullptr};
+    if (!get_particles(dst))
+      return false;
+    double en[3] = {0, 0, 0}, mom[3] = {0, 0, 0};
+    if (with_obs)
+    {
+      double o6[6];
+      if (!observables(1, o6))
+        return false;
+      double ek = 0.0; // observables.c:38-75 with the unit conversion
+      for (int i = 0; i < N; i++)
+      {
+        const double cx = vx[i] / u.velocity + u.cm_v[0], cy = vy[i] / u.velocity + u.cm_v[1],
+                     cz = vz[i] / u.velocity + u.cm_v[2];
+        ek += host_mass[i] * (cx * cx + cy * cy + cz * cz);
+        mom[0] += host_mass[i] * cx;
+        mom[1] += host_mass[i] * cy;
+        mom[2] += host_mass[i] * cz;
+      }
+      en[0] = 0.5 * ek;
+      en[1] = o6[1] * (u.time * u.time) / (u.dist * u.dist); // observables.c:110
+      en[2] = en[0] + en[1];
+    }
+    const double t = actualtime / u.time;
+    for (int i = 0; i < N; i++)
+    {
+      x[i] = (x[i] - 0.5) / u.dist + u.cm[0] + u.cm_v[0] * t;
+      y[i] = (y[i] - 0.5) / u.dist + u.cm[1] + u.cm_v[1] * t;
+      z[i] = (z[i] - 0.5) / u.dist + u.cm[2] + u.cm_v[2] * t;
+      vx[i] = vx[i] / u.velocity + u.cm_v[0];
+      vy[i] = vy[i] / u.velocity + u.cm_v[1];
+      vz[i] = vz[i] / u.velocity + u.cm_v[2];
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f)
+    {
+      set_error("Error opening snapshot file %s", path);
+      return false;
+    }
+    const double *arr[7] = {host_mass.data(), x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data()};
+    bool ok = true;
+    for (int a = 0; a < 7; a++)
+      ok = ok && fwrite(arr[a], sizeof(double), (size_t)N, f) == (size_t)N;
+    if (with_obs)
+      ok = ok && fwrite(en, sizeof(double), 3, f) == 3 && fwrite(mom, sizeof(double), 3, f) == 3;
+    ok = ok && fwrite(&t, sizeof(double), 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok)
+      set_error("writing snapshot %s failed", path);
+    return ok;
+  }
+
   // ---------------------------------------------------------------- adjoint (qz_adjoint.inl)
   void release_adjoint();
   bool ensure_adjoint();
@@ -1167,6 +1227,17 @@ int qz_timing_report(qz_ctx *ctx, char *names, int name_len, double *ms, int *la
     k++;
   }
   return k;
+}
+
+int qz_write_snapshot(qz_ctx *ctx, const char *path, const qz_units *units, double actualtime, int with_observables)
+{
+  QZ_CHECK_CTX(ctx);
+  if (!path || !units || units->dist == 0.0 || units->velocity == 0.0 || units->time == 0.0)
+  {
+    qz::set_error("qz_write_snapshot: bad arguments");
+    return 1;
+  }
+  return ctx->eng->write_snapshot(path, *units, actualtime, with_observables != 0) ? 0 : 1;
 }
 
 int qz_bench_sweep(qz_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch)

@@ -106,6 +106,7 @@ struct EngineBase
   virtual bool node_lists(int i, int *ixyz, int *bxyz, int *sxyz) = 0;
   virtual bool node_fields(int i, double *out5) = 0;
   virtual bool bench_sweep(int reps, double *ms, double *bytes) = 0;
+  virtual bool write_snapshot(const char *path, const qz_units &u, double actualtime, bool with_obs) = 0;
   virtual bool copy_probe(size_t bytes, double *ms) = 0;
 };
 
