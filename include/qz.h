@@ -130,11 +130,13 @@ int qz_observables(qz_ctx *ctx, int exact_pe, double *out6);
  * cell size of the finest node holding it, capped at dt_max */
 int qz_timestep(qz_ctx *ctx, double cfl, double dt_max, double *dt);
 
-/* Adjoint of the head-grid step (lmin == lmax).  Cotangent and gradient
- * arrays hold 3N doubles, x then y then z components, in the original
- * particle order.
+/* Adjoint of the step, head grid and refinement levels.  Cotangent and
+ * gradient arrays hold 3N doubles, x then y then z components, in the
+ * original particle order.  With levels the tree is held fixed (cell
+ * membership is piecewise constant in the positions).
  *  qz_force_vjp: xbar = J_A(x)^T abar for the force pass a = A(x) at the
- *    current positions (CIC deposit, multigrid solve, gradient, CIC gather).
+ *    current positions (CIC deposits, head and node solves with the parent
+ *    boundary averages, gradients, gather from the finest node).
  *  qz_adjoint_forward: from the current state (force pass done) run nsteps
  *    fixed-dt steps keeping checkpoints every checkpoint_every steps.
  *  qz_adjoint_backward: given the cotangents of the final positions and

@@ -1637,11 +1637,290 @@ static int adj_solve(int lmin, double *psi, const double *rhs, double tol)
   return -1;
 }
 
-/* xbar += J_A(x)^T abar at the current positions (head only).  abar, xbar: 3N. */
+/* ------------------------------------------------------------------------ */
+/* FP64 reverse mode of the force pass with refinement levels.  The tree is   */
+/* held fixed (cell membership is piecewise constant in the positions).       */
+/* Forward, per node in level order: rho from the particles of its cells      */
+/* (grid_density.c:113-267), boundary potential and accelerations from the    */
+/* parent average (potential_branches.c:204-331, grid_acceleration.c:316-383),*/
+/* interior solve (conjugate_gradient.c:322-463), interior central            */
+/* differences (grid_acceleration.c:36-76), gather from the finest node       */
+/* (particle_acceleration.c:134-355).  Reverse, finest level first:           */
+/*   Abar  = scatter of abar from the particles whose finest node it is,      */
+/*           plus the children's boundary Abar through the transposed average */
+/*   phibar = transposed central difference of Abar, plus the children's      */
+/*           boundary phibar through the transposed average                  */
+/*   L psi = phibar on the interior (zero on the rest): rhobar = psi          */
+/*   boundary phibar -= (1/H^2) sum of psi over its interior neighbours       */
+/* and x_p collects c_l m grad W_l . psi_l over every node holding p plus      */
+/* grad W_f . (abar . a_f) at its finest node f.                              */
+/* ------------------------------------------------------------------------ */
+typedef struct
+{
+  double *ab[3], *pb, *psi;
+  unsigned char *kind; /* 0 none, 1 simulation boundary, 2 boundary, 3 interior */
+} oadj;
+
+/* add v / 8 to the 8 parent points of child grid point c (transpose of parent_avg) */
+static void parent_avg_t(const onode *pt, double *f, const int *c, double v)
+{
+  int a[2][3];
+  for (int q = 0; q < 3; q++)
+  {
+    a[0][q] = (c[q] >> 1) - pt->ts[q];
+    a[1][q] = ((c[q] + 1) >> 1) - pt->ts[q];
+  }
+  int X = gdx(pt), XY = gdxy(pt);
+  for (int kk = 0; kk < 2; kk++)
+    for (int jj = 0; jj < 2; jj++)
+      for (int ii = 0; ii < 2; ii++)
+        f[a[ii][0] + a[jj][1] * X + a[kk][2] * XY] += 0.125 * v;
+}
+
+/* CIC weights and their x, y, z derivatives at the cell of p at level lv */
+static int cic_grad(const onode *nd, double x, double y, double z, double w[8], double dw[3][8])
+{
+  int lv = nd->lv;
+  double sc = (double)(1 << lv);
+  double px = x * sc, py = y * sc, pz = z * sc;
+  int fx = (int)px, fy = (int)py, fz = (int)pz;
+  double t[3] = {px - fx, py - fy, pz - fz};
+  for (int kk = 0; kk < 2; kk++)
+    for (int jj = 0; jj < 2; jj++)
+      for (int ii = 0; ii < 2; ii++)
+      {
+        int q = ii + 2 * jj + 4 * kk;
+        double wx = ii ? t[0] : 1.0 - t[0], wy = jj ? t[1] : 1.0 - t[1], wz = kk ? t[2] : 1.0 - t[2];
+        double dx = ii ? sc : -sc, dy = jj ? sc : -sc, dz = kk ? sc : -sc;
+        w[q] = wx * wy * wz;
+        dw[0][q] = dx * wy * wz;
+        dw[1][q] = wx * dy * wz;
+        dw[2][q] = wx * wy * dz;
+      }
+  return (fx - nd->ts[0]) + (fy - nd->ts[1]) * gdx(nd) + (fz - nd->ts[2]) * gdxy(nd);
+}
+
+/* L psi = b on the interior points of a node (the operator of
+ * conjugate_gradient.c:322-463), psi zero elsewhere; CG to rr <= 1e-28 rr0 */
+static int node_adj_solve(const onode *nd, const unsigned char *kind, double *psi, const double *b)
+{
+  int ni = nd->nintr, X = gdx(nd), XY = gdxy(nd);
+  double oneH2 = (double)(1 << nd->lv) * (double)(1 << nd->lv);
+  double *r = (double *)calloc((size_t)nd->gcap, sizeof(double));
+  double *p = (double *)calloc((size_t)nd->gcap, sizeof(double));
+  double *Ap = (double *)calloc((size_t)nd->gcap, sizeof(double));
+  double rs = 0.0;
+  memset(psi, 0, sizeof(double) * (size_t)nd->gcap);
+  for (int i = 0; i < ni; i++)
+  {
+    int g = nd->iidx[i];
+    r[g] = p[g] = b[g];
+    rs += b[g] * b[g];
+  }
+  double rs0 = rs;
+  int rc = 1;
+  for (int it = 0; it < 20 * ni + 100 && rs > 0.0; it++)
+  {
+    if (rs <= 1e-28 * rs0)
+    {
+      rc = 0;
+      break;
+    }
+    double pAp = 0.0;
+    for (int i = 0; i < ni; i++)
+    {
+      int g = nd->iidx[i];
+      Ap[g] = oneH2 * (p[g + 1] + p[g - 1] + p[g + X] + p[g - X] + p[g + XY] + p[g - XY] - 6.0 * p[g]);
+      pAp += p[g] * Ap[g];
+    }
+    double al = rs / pAp, rn = 0.0;
+    for (int i = 0; i < ni; i++)
+    {
+      int g = nd->iidx[i];
+      psi[g] += al * p[g];
+      r[g] -= al * Ap[g];
+      rn += r[g] * r[g];
+    }
+    for (int i = 0; i < ni; i++)
+    {
+      int g = nd->iidx[i];
+      p[g] = r[g] + rn / rs * p[g];
+    }
+    rs = rn;
+  }
+  if (rs == 0.0 || rs <= 1e-28 * rs0)
+    rc = 0;
+  (void)kind;
+  free(r);
+  free(p);
+  free(Ap);
+  return rc;
+}
+
+static int amr_force_vjp(oq_sim *s, const double *abar, double *xbar)
+{
+  int nn = s->nnodes, rc = 0;
+  size_t N = (size_t)s->n;
+  oadj *A = (oadj *)calloc((size_t)nn, sizeof(oadj));
+  for (int i = 0; i < nn; i++)
+  {
+    const onode *nd = &s->nodes[i];
+    for (int a = 0; a < 3; a++)
+      A[i].ab[a] = (double *)calloc((size_t)nd->gcap, sizeof(double));
+    A[i].pb = (double *)calloc((size_t)nd->gcap, sizeof(double));
+    A[i].psi = (double *)calloc((size_t)nd->gcap, sizeof(double));
+    A[i].kind = (unsigned char *)calloc((size_t)nd->gcap, 1);
+    for (int q = 0; q < nd->nsim; q++)
+      A[i].kind[nd->sidx[q]] = 1;
+    for (int q = 0; q < nd->nbdry; q++)
+      A[i].kind[nd->bidx[q]] = 2;
+    for (int q = 0; q < nd->nintr; q++)
+      A[i].kind[nd->iidx[q]] = 3;
+  }
+  /* transpose of the gather: abar to the finest node's points */
+  for (int i = 0; i < nn; i++)
+  {
+    const onode *nd = &s->nodes[i];
+    int X = gdx(nd), XY = gdxy(nd);
+    for (int c = 0; c < nd->ncell; c++)
+    {
+      int b = nd->cbox[c];
+      if (nd->nchild > 0 && nd->box[b] >= 0)
+        continue;
+      for (int q = nd->cell_start[b]; q < nd->cell_start[b + 1]; q++)
+      {
+        int p = nd->cell_list[q];
+        double w[8], dw[3][8];
+        int g = cic_grad(nd, s->x[p], s->y[p], s->z[p], w, dw);
+        for (int kk = 0; kk < 2; kk++)
+          for (int jj = 0; jj < 2; jj++)
+            for (int ii = 0; ii < 2; ii++)
+              for (int a = 0; a < 3; a++)
+                A[i].ab[a][g + ii + jj * X + kk * XY] += w[ii + 2 * jj + 4 * kk] * abar[a * N + p];
+      }
+    }
+  }
+  /* nodes are stored parents first: the reverse order has every child done
+   * before its parent */
+  for (int i = nn - 1; i >= 0 && !rc; i--)
+  {
+    onode *nd = &s->nodes[i];
+    oadj *J = &A[i];
+    double sc = (double)(1 << nd->lv), o2H = 0.5 * sc, oneH2 = sc * sc;
+    int X = gdx(nd), XY = gdxy(nd);
+    for (int q = 0; q < nd->nintr; q++)
+    {
+      int g = nd->iidx[q];
+      J->pb[g - 1] += o2H * J->ab[0][g];
+      J->pb[g + 1] -= o2H * J->ab[0][g];
+      J->pb[g - X] += o2H * J->ab[1][g];
+      J->pb[g + X] -= o2H * J->ab[1][g];
+      J->pb[g - XY] += o2H * J->ab[2][g];
+      J->pb[g + XY] -= o2H * J->ab[2][g];
+    }
+    if (i == 0)
+      break;
+    if (node_adj_solve(nd, J->kind, J->psi, J->pb))
+      rc = 1;
+    const int off[6] = {1, -1, X, -X, XY, -XY};
+    for (int q = 0; q < nd->nintr; q++)
+    {
+      int g = nd->iidx[q];
+      for (int e = 0; e < 6; e++)
+        if (J->kind[g + off[e]] == 2)
+          J->pb[g + off[e]] -= oneH2 * J->psi[g];
+    }
+    const onode *pt = &s->nodes[nd->parent];
+    oadj *Pj = &A[nd->parent];
+    for (int q = 0; q < nd->nbdry; q++)
+    {
+      int g = nd->bidx[q];
+      const int *c = &nd->bxyz[3 * q];
+      for (int a = 0; a < 3; a++)
+        parent_avg_t(pt, Pj->ab[a], c, J->ab[a][g]);
+      parent_avg_t(pt, Pj->pb, c, J->pb[g]);
+    }
+  }
+  /* the head solve on the packed cube with zero faces */
+  if (!rc)
+  {
+    onode *h = &s->nodes[0];
+    int n = 1 << s->lmin, n1 = n + 1;
+    size_t p3 = (size_t)n1 * n1 * n1;
+    double *rhs = (double *)calloc(p3, sizeof(double));
+    double *psi = (double *)calloc(p3, sizeof(double));
+    for (int q = 0; q < h->nintr; q++)
+    {
+      const int *c = &h->ixyz[3 * q];
+      rhs[c[0] + (size_t)n1 * (c[1] + (size_t)n1 * c[2])] = A[0].pb[h->iidx[q]];
+    }
+    if (adj_solve(s->lmin, psi, rhs, 1e-12) < 0)
+      rc = 1;
+    for (int q = 0; q < h->nintr; q++)
+    {
+      const int *c = &h->ixyz[3 * q];
+      A[0].psi[h->iidx[q]] = psi[c[0] + (size_t)n1 * (c[1] + (size_t)n1 * c[2])];
+    }
+    free(rhs);
+    free(psi);
+  }
+  /* particles: deposit term at every node holding p, gather term at the finest */
+  for (int i = 0; i < nn && !rc; i++)
+  {
+    const onode *nd = &s->nodes[i];
+    const oadj *J = &A[i];
+    double H = 1.0 / (1 << nd->lv);
+    double coeff = 4.0 * s->G * OQ_PI / (H * H * H);
+    int X = gdx(nd), XY = gdxy(nd);
+    for (int c = 0; c < nd->ncell; c++)
+    {
+      int b = nd->cbox[c];
+      int finest = !(nd->nchild > 0 && nd->box[b] >= 0);
+      for (int q = nd->cell_start[b]; q < nd->cell_start[b + 1]; q++)
+      {
+        int p = nd->cell_list[q];
+        double w[8], dw[3][8];
+        int g = cic_grad(nd, s->x[p], s->y[p], s->z[p], w, dw);
+        double ab0 = abar[p], ab1 = abar[N + p], ab2 = abar[2 * N + p];
+        for (int kk = 0; kk < 2; kk++)
+          for (int jj = 0; jj < 2; jj++)
+            for (int ii = 0; ii < 2; ii++)
+            {
+              int k = ii + 2 * jj + 4 * kk, gn = g + ii + jj * X + kk * XY;
+              double val = coeff * s->m[p] * J->psi[gn];
+              if (finest)
+                val += ab0 * nd->ax[gn] + ab1 * nd->ay[gn] + ab2 * nd->az[gn];
+              for (int a = 0; a < 3; a++)
+                xbar[a * N + p] += dw[a][k] * val;
+            }
+      }
+    }
+  }
+  for (int i = 0; i < nn; i++)
+  {
+    for (int a = 0; a < 3; a++)
+      free(A[i].ab[a]);
+    free(A[i].pb);
+    free(A[i].psi);
+    free(A[i].kind);
+  }
+  free(A);
+  return rc;
+}
+
+/* xbar += J_A(x)^T abar at the current positions.  abar, xbar: 3N. */
 int oq_force_vjp(oq_sim *s, const double *abar, double *xbar)
 {
-  if (s->lmin != s->lmax)
-    return 1;
+  if (s->lmin < s->lmax)
+  {
+    oq_build_tree(s);
+    oq_reset(s);
+    oq_grid_density(s);
+    if (oq_potential(s))
+      return 1;
+    oq_grid_acceleration(s);
+    return amr_force_vjp(s, abar, xbar);
+  }
   oq_reset(s);
   oq_grid_density(s);
   if (oq_potential(s))

@@ -1,4 +1,5 @@
-// Adjoint of the head-grid PM step (Engine<P,G> methods).  Included by qz_core.hip.
+// Adjoint of the PM step, head grid and refinement levels (Engine<P,G> methods).
+// Included by qz_core.hip.
 //
 // Step j of the fixed-dt KDK loop (particle_updating.c:92-147, 2677-2691) with
 // h = dt / 2 and a_j = A(x_j) the force pass at x_j:
@@ -7,9 +8,12 @@
 //   abar_j = h vbar_j + abarA;  xbar_j += J_A(x_j)^T abar_j
 //   ybar = s xbar_j;  ubar = s vbar_j + dt ybar;  xbar <- ybar, vbar <- ubar, abarA <- h ubar
 // and finally xbar_0 += J_A(x_0)^T abarA.  Forward states are recomputed from
-// checkpoints every `every` steps; within a segment the sorted positions, keys,
-// head cell ranges, grid accelerations and reflection flags of each step are
-// kept so the reverse sweep needs no forward Poisson solve.
+// checkpoints every `every` steps.  Head grid only: within a segment the sorted
+// positions, keys, head cell ranges, grid accelerations and reflection flags of
+// each step are kept, so the reverse sweep needs no forward Poisson solve.  With
+// refinement levels the positions, IDs and reflection flags are kept and the
+// reverse sweep redoes the force pass (tree, density, potentials, grid
+// accelerations) of each step before its VJP.
 #include "qz_adjoint_kernels.h"
 
 namespace qz
@@ -17,12 +21,23 @@ namespace qz
 
 template <typename P, typename G> void Engine<P, G>::release_adjoint()
 {
-  void *p1[] = {adj_xb, adj_vb, adj_ab, adj_abA, adj_psi, adj_slot, adj_gbar[0], adj_gbar[1], adj_gbar[2], adj_rhs};
+  void *p1[] = {adj_xb,     adj_vb,     adj_ab,     adj_abA,    adj_psi,    adj_slot,   adj_mid,
+                adj_gbar[0], adj_gbar[1], adj_gbar[2], adj_rhs,    adj_nab[0], adj_nab[1], adj_nab[2],
+                adj_npb,    adj_npsi,   adj_hpb,    adj_nrhs,   adj_kind,   adj_rkey[0], adj_rkey[1],
+                adj_rval[0], adj_rval[1], adj_bpre,  adj_rsrc,   adj_rtmp};
   for (void *q : p1)
     (void)hipFree(q);
-  adj_xb = adj_vb = adj_ab = adj_abA = adj_psi = nullptr;
-  adj_slot = nullptr;
-  adj_gbar[0] = adj_gbar[1] = adj_gbar[2] = adj_rhs = nullptr;
+  adj_xb = adj_vb = adj_ab = adj_abA = adj_psi = adj_npb = adj_npsi = adj_hpb = nullptr;
+  adj_slot = adj_mid = nullptr;
+  adj_gbar[0] = adj_gbar[1] = adj_gbar[2] = adj_rhs = adj_nrhs = nullptr;
+  adj_nab[0] = adj_nab[1] = adj_nab[2] = nullptr;
+  adj_kind = nullptr;
+  adj_rkey[0] = adj_rkey[1] = nullptr;
+  adj_rval[0] = adj_rval[1] = adj_bpre = nullptr;
+  adj_rsrc = nullptr;
+  adj_rtmp = nullptr;
+  adj_ncap = adj_rcap = adj_bpre_cap = 0;
+  adj_rtmp_bytes = 0;
   for (auto &c : ckpts)
   {
     for (int a = 0; a < 3; a++)
@@ -51,13 +66,13 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
 {
   if (adj_xb)
     return true;
-  if (cfg.lmin != cfg.lmax)
-  {
-    set_error("the adjoint covers the head-grid step (lmin == lmax)");
-    return false;
-  }
   size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
   size_t b = 0;
+  adj_mid = dmalloc<P>(N, b);
+  if (cfg.lmin < cfg.lmax)
+    adj_hpb = dmalloc<double>(npts, b);
+  if (!adj_mid || (cfg.lmin < cfg.lmax && !adj_hpb))
+    return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
   adj_xb = dmalloc<double>(3 * (size_t)N, b);
   adj_vb = dmalloc<double>(3 * (size_t)N, b);
   adj_ab = dmalloc<double>(3 * (size_t)N, b);
@@ -78,6 +93,8 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
 // grid accelerations of A(x) in gax/gay/gaz and the head cell ranges current.
 template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double *ab)
 {
+  if (has_tree())
+    return vjp_tree(ab);
   size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
   double H = 1.0 / n;
   // Gbar = scatter of abar (the deposit with abar as the mass, coefficient 1)
@@ -97,8 +114,193 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
     return false;
   const double coeff = 4.0 * cfg.G * kPi / (H * H * H);
   QZ_LAUNCHB("adj_gather", (24.0 + sizeof(P) + 4.0 + 48.0) * N + (8.0 + 3.0 * sizeof(G)) * npts,
-             (k_adjoint_gather<P, G>), nblk(N), pos[0], pos[1], pos[2], pa[9], id_a, adj_psi, gax, gay, gaz, ab,
+             (k_adjoint_gather<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi, gax, gay, gaz, ab,
              ab + N, ab + 2 * (size_t)N, adj_xb, adj_xb + N, adj_xb + 2 * (size_t)N, N, n, coeff);
+  return true;
+}
+
+// The force pass up to the grid accelerations: tree, density, head and node
+// potentials, grid accelerations (no-ops for the tree stages on the head grid).
+template <typename P, typename G> bool Engine<P, G>::force_fields()
+{
+  return reset() && tree_adaptation() && grid_density() && potential_head() && potential_branches() &&
+         grid_acceleration();
+}
+
+// Grow-only adjoint pools of the refinement levels (grid-pool layout) and the
+// transfer records (8 per boundary point of the largest level).
+template <typename P, typename G> bool Engine<P, G>::ensure_adjoint_tree()
+{
+  TreeDev &d = *tree->dev;
+  const long long ng = std::max<long long>(d.used_grid, 1);
+  if (ng > adj_ncap)
+  {
+    void *old[] = {adj_nab[0], adj_nab[1], adj_nab[2], adj_npb, adj_npsi, adj_nrhs, adj_kind};
+    for (void *q : old)
+      (void)hipFree(q);
+    adj_ncap = next_cap(adj_ncap, ng);
+    size_t b = 0;
+    for (int a = 0; a < 3; a++)
+      adj_nab[a] = dmalloc<double>(adj_ncap, b);
+    adj_npb = dmalloc<double>(adj_ncap, b);
+    adj_npsi = dmalloc<double>(adj_ncap, b);
+    adj_nrhs = dmalloc<G>(adj_ncap, b);
+    adj_kind = dmalloc<unsigned char>(adj_ncap, b);
+    if (!adj_nab[0] || !adj_nab[1] || !adj_nab[2] || !adj_npb || !adj_npsi || !adj_nrhs || !adj_kind)
+      return hip_ok(hipErrorOutOfMemory, "adjoint node pools");
+  }
+  long long nbmax = 1;
+  int cmax = 1;
+  for (size_t l = 1; l < d.lfirst.size(); l++)
+  {
+    long long nb = 0;
+    for (int q = d.lfirst[l]; q < d.lfirst[l] + d.lcount[l]; q++)
+      nb += d.hnodes[q].nbdry;
+    nbmax = std::max(nbmax, nb);
+    cmax = std::max(cmax, d.lcount[l]);
+  }
+  if (8 * nbmax > adj_rcap)
+  {
+    void *old[] = {adj_rkey[0], adj_rkey[1], adj_rval[0], adj_rval[1], adj_rsrc, adj_rtmp};
+    for (void *q : old)
+      (void)hipFree(q);
+    adj_rcap = next_cap(adj_rcap, 8 * nbmax);
+    if (adj_rcap > 0x7fffffffLL)
+    {
+      set_error("adjoint transfer records exceed 2^31 entries");
+      return false;
+    }
+    adj_rtmp_bytes = qz_sort_temp_bytes((int)adj_rcap, 32);
+    size_t b = 0;
+    for (int k = 0; k < 2; k++)
+    {
+      adj_rkey[k] = dmalloc<unsigned>(adj_rcap, b);
+      adj_rval[k] = dmalloc<int>(adj_rcap, b);
+    }
+    adj_rsrc = dmalloc<long long>(adj_rcap / 8 + 1, b);
+    if (!hip_ok(hipMalloc(&adj_rtmp, std::max<size_t>(adj_rtmp_bytes, 1)), "adjoint records") || !adj_rkey[0] ||
+        !adj_rkey[1] || !adj_rval[0] || !adj_rval[1] || !adj_rsrc)
+      return hip_ok(hipErrorOutOfMemory, "adjoint records");
+  }
+  if (cmax > adj_bpre_cap)
+  {
+    (void)hipFree(adj_bpre);
+    adj_bpre_cap = next_cap(adj_bpre_cap, cmax);
+    if (!hip_ok(hipMalloc((void **)&adj_bpre, sizeof(int) * adj_bpre_cap), "adjoint records"))
+      return false;
+  }
+  return true;
+}
+
+// adj_xb (ID order) += J_A(x)^T ab with refinement levels, after a force pass
+// at the current sorted positions (tree, node and head fields current).
+template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
+{
+  if (!ensure_adjoint_tree())
+    return false;
+  TreeDev &d = *tree->dev;
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+  const double H = 1.0 / n;
+  const size_t ng = (size_t)d.used_grid;
+  const int nl = (int)d.lfirst.size();
+  const Cube cb = make_cube(n + 1);
+  QZ_LAUNCH("adj_fill", k_fill<double>, nblk(ng), adj_npb, 0.0, ng);
+  QZ_LAUNCH("adj_fill", k_fill<unsigned char>, nblk(ng), adj_kind, (unsigned char)0, ng);
+  QZ_LAUNCH("adj_fill", k_fill<double>, nblk(npts), adj_hpb, 0.0, npts);
+  for (int l = 1; l < nl; l++)
+  {
+    dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
+    QZ_LAUNCH("adj_kinds", k_adj_kinds, grid, d.nodes, d.lfirst[l], d.sim_g, d.bdry_g, d.intr_g, adj_kind);
+  }
+  // Abar: scatter of abar over the unrefined cells of the head and of every level
+  for (int a = 0; a < 3; a++)
+  {
+    QZ_LAUNCHB("adj_slot", 12.0 * N, k_by_id_to_slot<P>, nblk(N), ab + (size_t)a * N, id_a, adj_slot, N);
+    QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
+               pos[1], pos[2], adj_slot, N, cfg.lmin, 3 * D, n - 1, tails);
+    QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_adj_head_scatter<G, G>),
+               nblk(npts), cell_start, d.head_child, tails, adj_gbar[a], cb);
+    for (int l = 1; l < nl; l++)
+    {
+      const int lv = cfg.lmin + l;
+      QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
+                 pos[1], pos[2], adj_slot, N, lv, 3 * (D - l), (1 << lv) - 1, tails);
+      dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
+      QZ_LAUNCH("adj_scatter_branch", (k_adj_branch_scatter<G>), grid, d.nodes, d.lfirst[l], d.bdry_g, d.bdry_xyz,
+                d.intr_g, d.intr_xyz, d.box, d.bstart, d.bend, tails, adj_nab[a]);
+    }
+  }
+  // levels, finest first: every child is done before its parent
+  std::vector<int> bpre;
+  for (int l = nl - 1; l >= 1; l--)
+  {
+    const int first = d.lfirst[l], count = d.lcount[l];
+    dim3 grid(blocks_per_node(d.lpmax[l]), count);
+    QZ_LAUNCH("adj_grad_t", k_adj_node_grad_t, grid, d.nodes, first, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz,
+              adj_kind, adj_nab[0], adj_nab[1], adj_nab[2], adj_npb);
+    QZ_LAUNCH("adj_fill", k_fill<double>, nblk(d.lng[l]), adj_npsi + d.lg0[l], 0.0, (size_t)d.lng[l]);
+    QZ_LAUNCH("adj_rhs", (k_adj_node_rhs<G>), grid, d.nodes, first, d.intr_g, adj_npb, adj_nrhs);
+    bool solved = false;
+    if (!branch_mg_level((size_t)l, solved, adj_npsi, adj_nrhs, true))
+      return false;
+    if (!solved)
+    {
+      set_error("The precision was not reached in the adjoint solve of level %d", cfg.lmin + l);
+      return false;
+    }
+    QZ_LAUNCH("adj_bdry", k_adj_node_bdry, grid, d.nodes, first, d.bdry_g, d.bdry_xyz, adj_kind, adj_npsi, adj_npb);
+    // boundary Abar and phibar to the parents (transposed 8-point average)
+    bpre.assign(count, 0);
+    long long nb = 0;
+    for (int q = 0; q < count; q++)
+    {
+      bpre[q] = (int)nb;
+      nb += d.hnodes[first + q].nbdry;
+    }
+    if (nb == 0)
+      continue;
+    const int R = (int)(8 * nb);
+    const bool to_head = l == 1;
+    const unsigned long long top = to_head ? (unsigned long long)npts : (unsigned long long)ng;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < top)
+      bits++;
+    if (!hip_ok(hipMemcpyAsync(adj_bpre, bpre.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream),
+                "adjoint records"))
+      return false;
+    QZ_LAUNCH("adj_xfer", k_adj_xfer_records, grid, d.nodes, first, adj_bpre, d.bdry_g, d.bdry_xyz, n, adj_rkey[0],
+              adj_rval[0], adj_rsrc);
+    timer.begin(stream);
+    if (!hip_ok(qz_sort_pairs(adj_rtmp, adj_rtmp_bytes, adj_rkey[0], adj_rkey[1], adj_rval[0], adj_rval[1], R, bits,
+                              stream),
+                "adjoint records"))
+      return false;
+    timer.end(stream, "adj_xfer_sort");
+    if (to_head)
+      QZ_LAUNCH("adj_xfer", (k_adj_xfer_apply<G>), nblk(R), adj_rkey[1], adj_rval[1], R, adj_rsrc, adj_nab[0],
+                adj_nab[1], adj_nab[2], adj_npb, adj_gbar[0], adj_gbar[1], adj_gbar[2], adj_hpb);
+    else
+      QZ_LAUNCH("adj_xfer", (k_adj_xfer_apply<double>), nblk(R), adj_rkey[1], adj_rval[1], R, adj_rsrc, adj_nab[0],
+                adj_nab[1], adj_nab[2], adj_npb, adj_nab[0], adj_nab[1], adj_nab[2], adj_npb);
+  }
+  // head: transposed central difference, the level-1 phibar, the adjoint solve
+  QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
+             adj_gbar[2], adj_rhs, cb, 1.0 / (2.0 * H));
+  QZ_LAUNCH("adj_head_add", k_adj_head_add<G>, nblk(npts), adj_rhs, adj_hpb, cb);
+  QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
+  if (!solve_head(adj_psi, adj_rhs, true))
+    return false;
+  QZ_LAUNCH("adj_gather_amr", (k_adjoint_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi,
+            gax, gay, gaz, d.head_child, d.nodes, d.box, adj_npsi, (const G *)d.ax, (const G *)d.ay, (const G *)d.az,
+            ab, ab + N, ab + 2 * (size_t)N, adj_xb, adj_xb + N, adj_xb + 2 * (size_t)N, N, cfg.lmin,
+            4.0 * cfg.G * kPi);
+  return hip_ok(hipGetLastError(), "adjoint");
+}
+
+// adj_mid[id] = the masses (ID order), from the current slot order.
+template <typename P, typename G> bool Engine<P, G>::masses_by_id()
+{
+  QZ_LAUNCH("adj_mass", k_slot_to_id<P>, nblk(N), pa[9], id_a, adj_mid, N);
   return true;
 }
 
@@ -106,8 +308,8 @@ template <typename P, typename G> bool Engine<P, G>::force_vjp(const double *aba
 {
   if (!ensure_adjoint())
     return false;
-  // force pass at the current positions (grid accelerations and cell ranges)
-  if (!reset() || !grid_density() || !potential_head() || !grid_acceleration())
+  // force pass at the current positions (tree, grid accelerations, cell ranges)
+  if (!masses_by_id() || !force_fields())
     return false;
   size_t n3 = 3 * (size_t)N;
   if (!hip_ok(hipMemcpy(adj_ab, abar, sizeof(double) * n3, hipMemcpyHostToDevice), "upload abar"))
@@ -168,8 +370,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
     }
     if (s == nsteps)
       break;
-    if (!update_A(dt) || !reset() || !grid_density() || !potential_head() || !grid_acceleration() ||
-        !particle_acceleration() || !update_B(dt))
+    if (!update_A(dt) || !force_fields() || !particle_acceleration() || !update_B(dt))
       return false;
   }
   return hip_ok(hipStreamSynchronize(stream), "adjoint_forward");
@@ -183,9 +384,12 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     set_error("adjoint_backward: run qz_adjoint_forward first");
     return false;
   }
+  const bool amr = cfg.lmin < cfg.lmax;
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, n3 = 3 * (size_t)N;
   const size_t ncell = (size_t)n * n * n;
   const double dt = adj_dt, h = 0.5 * dt;
+  if (!masses_by_id())
+    return false;
   // keep the final state to restore it afterwards
   Ckpt fin{};
   size_t b = 0;
@@ -201,26 +405,24 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   }
   fin.id = dmalloc<int>(N, b);
   (void)hipMemcpyAsync(fin.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
-  // per-step records for one segment
-  if ((int)recs.size() < adj_every + 1)
+  // per-step records for one segment (with levels: positions, IDs, reflections)
+  while ((int)recs.size() < adj_every + 1)
   {
-    while ((int)recs.size() < adj_every + 1)
+    StepRec r{};
+    size_t bb = 0;
+    for (int a = 0; a < 3; a++)
     {
-      StepRec r{};
-      size_t bb = 0;
-      for (int a = 0; a < 3; a++)
-      {
-        r.x[a] = dmalloc<double>(N, bb);
-        r.g[a] = dmalloc<G>(npts, bb);
-      }
-      r.id = dmalloc<int>(N, bb);
-      r.key = dmalloc<unsigned long long>(N, bb);
-      r.cstart = dmalloc<int>(ncell + 1, bb);
-      r.refl = dmalloc<unsigned char>(N, bb);
-      if (!r.x[0] || !r.x[1] || !r.x[2] || !r.g[0] || !r.g[1] || !r.g[2] || !r.id || !r.key || !r.cstart || !r.refl)
-        return hip_ok(hipErrorOutOfMemory, "adjoint step records");
-      recs.push_back(r);
+      r.x[a] = dmalloc<double>(N, bb);
+      r.g[a] = amr ? nullptr : dmalloc<G>(npts, bb);
     }
+    r.id = dmalloc<int>(N, bb);
+    r.key = amr ? nullptr : dmalloc<unsigned long long>(N, bb);
+    r.cstart = amr ? nullptr : dmalloc<int>(ncell + 1, bb);
+    r.refl = dmalloc<unsigned char>(N, bb);
+    if (!r.x[0] || !r.x[1] || !r.x[2] || !r.id || !r.refl ||
+        (!amr && (!r.g[0] || !r.g[1] || !r.g[2] || !r.key || !r.cstart)))
+      return hip_ok(hipErrorOutOfMemory, "adjoint step records");
+    recs.push_back(r);
   }
   auto restore_ckpt = [&](const Ckpt &c) {
     for (int a = 0; a < 3; a++)
@@ -229,28 +431,41 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       (void)hipMemcpyAsync(pa[3 + a], c.a[a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream);
     (void)hipMemcpyAsync(id_a, c.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
     keys_valid = false;
+    acc_live = true;
   };
-  // record the sorted state after a force pass (positions, ids, keys, ranges, grid a)
+  // record the sorted state after a force pass (positions, ids; head grid only:
+  // keys, ranges, grid a)
   auto save_rec = [&](StepRec &r) {
     for (int a = 0; a < 3; a++)
       (void)hipMemcpyAsync(r.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
     (void)hipMemcpyAsync(r.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+    if (amr)
+      return;
     (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
     (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
     G *src[3] = {gax, gay, gaz};
     for (int a = 0; a < 3; a++)
       (void)hipMemcpyAsync(r.g[a], src[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
   };
+  // make a recorded state current for its VJP (with levels: masses to the
+  // recorded order, then the force pass)
   auto load_rec = [&](const StepRec &r) {
     for (int a = 0; a < 3; a++)
       (void)hipMemcpyAsync(pos[a], r.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
     (void)hipMemcpyAsync(id_a, r.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+    if (amr)
+    {
+      QZ_LAUNCH("adj_mass", k_id_to_slot<P>, nblk(N), adj_mid, id_a, pa[9], N);
+      keys_valid = false;
+      return force_fields();
+    }
     (void)hipMemcpyAsync(key_out, r.key, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
     (void)hipMemcpyAsync(cell_start, r.cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
     G *dst[3] = {gax, gay, gaz};
     for (int a = 0; a < 3; a++)
       (void)hipMemcpyAsync(dst[a], r.g[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
     keys_valid = true;
+    return true;
   };
   if (!hip_ok(hipMemcpy(adj_xb, xbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload xbar") ||
       !hip_ok(hipMemcpy(adj_vb, vbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload vbar"))
@@ -260,25 +475,30 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   {
     int c0 = sc * adj_every;
     int c1 = std::min(c0 + adj_every, adj_nsteps);
-    // recompute the segment: rec[0] = state c0, rec[k] = state c0 + k
+    // recompute the segment: rec[0] = state c0 (kept for the first segment
+    // only, whose a_0 term closes the sweep), rec[k] = state c0 + k.  The
+    // checkpoint carries a_c0, so no force pass is needed to step from it.
     restore_ckpt(ckpts[sc]);
-    if (!reset() || !grid_density() || !potential_head() || !grid_acceleration())
-      return false;
-    save_rec(recs[0]);
+    if (sc == 0)
+    {
+      if (!amr && (!force_fields() || !particle_acceleration()))
+        return false;
+      save_rec(recs[0]);
+    }
     for (int j = c0 + 1; j <= c1; j++)
     {
       refl_rec = recs[j - c0].refl;
       bool ok = update_A(dt);
       refl_rec = nullptr;
-      if (!ok || !reset() || !grid_density() || !potential_head() || !grid_acceleration() ||
-          !particle_acceleration() || !update_B(dt))
+      if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt))
         return false;
       save_rec(recs[j - c0]);
     }
     // reverse sweep over the segment
     for (int j = c1; j > c0; j--)
     {
-      load_rec(recs[j - c0]);
+      if (!load_rec(recs[j - c0]))
+        return false;
       QZ_LAUNCH("adj_abar", k_adj_abar, nblk(n3), adj_vb, adj_abA, adj_ab, (int)n3, h);
       if (!vjp_at_current(adj_ab))
         return false;
@@ -286,14 +506,20 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     }
   }
   // a_0 = A(x_0) fed the first half kick
-  load_rec(recs[0]);
-  if (!vjp_at_current(adj_abA))
+  if (!load_rec(recs[0]) || !vjp_at_current(adj_abA))
     return false;
   if (!hip_ok(hipMemcpy(xb0, adj_xb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download xbar") ||
       !hip_ok(hipMemcpy(vb0, adj_vb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download vbar"))
     return false;
-  // back to the final forward state
+  // back to the final forward state: its fields (and tree) are rebuilt, then the
+  // state is restored once more for the exact accelerations (the state is in
+  // key order, so the sort of the force pass leaves the order unchanged); the
+  // head solve's starting-guess history restarts from the final potential
   restore_ckpt(fin);
+  if (!force_fields())
+    return false;
+  restore_ckpt(fin);
+  phi_hist = 0;
   if (!hip_ok(hipStreamSynchronize(stream), "adjoint_backward"))
     return false;
   for (int a = 0; a < 3; a++)

@@ -644,7 +644,7 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
                        d.bdry_xyz, d.intr_g, d.intr_xyz, d.phi, d.phi_old, phi, n, cfg.G * total_mass, zero_bc);
     timer.end(stream, "branch_init_pot");
     bool solved = false;
-    if (cfg.branch_solver == QZ_BRANCH_MG && !branch_mg_level(l, solved))
+    if (cfg.branch_solver == QZ_BRANCH_MG && !branch_mg_level(l, solved, d.phi, (const G *)d.rho, false))
       return false;
     if (!solved)
     {
@@ -840,8 +840,11 @@ bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long 
 // (convergence test of the reference's CG on every iteration), preconditioned
 // by V-cycle(s) in G precision.  `solved` stays false when a box is too large
 // for the hierarchy or the iteration cap is reached; the caller then runs CG
-// from the current phi.
-template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, bool &solved)
+// from the current phi.  `phi_pool` / `rho_pool` are the unknowns and the
+// right-hand side in the grid-pool layout; `adjoint` solves start from zero
+// with zero Dirichlet values and stop on the relative residual.
+template <typename P, typename G>
+bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint)
 {
   solved = false;
   TreeDev &d = *tree->dev;
@@ -1055,12 +1058,12 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
   QZ_LAUNCH("bmg_setup", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
   if (n_itab > 0)
     QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + gs + 8.0 + gs), (k_bmg_residual<G>), n_itab, d.nodes, first, bm,
-               itab, d.intr_g, d.phi, (const G *)d.rho, d.cg_r, r, d.part2);
+               itab, d.intr_g, phi_pool, rho_pool, d.cg_r, r, d.part2);
   auto batch = [&]() {
     for (int il = 0; il < 2; il++)
     {
       QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, cfg.tol, cfg.G,
-                d.done_count);
+                adjoint ? 1 : 0, d.done_count);
       if (n_itab == 0)
         continue;
       vcycle();
@@ -1071,7 +1074,7 @@ template <typename P, typename G> bool Engine<P, G>::branch_mg_level(size_t l, b
       QZ_LAUNCHB("bmg_apply", ni * 20.0, k_bmg_apply, n_itab, d.nodes, first, bm, itab, d.intr_g, d.cg_p, d.cg_Ap,
                  d.part1);
       QZ_LAUNCHB("bmg_update", ni * (4.0 + 48.0 + gs), (k_bmg_update<G>), n_itab, d.nodes, first, bm, itab, it_base,
-                 il, d.intr_g, d.phi, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
+                 il, d.intr_g, phi_pool, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
     }
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 2);
   };

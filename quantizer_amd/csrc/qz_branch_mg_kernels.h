@@ -40,6 +40,7 @@ struct BMg
   int ic0, ic1;    // the node's interior chunks (entries of the interior table)
   double rz[2];    // (-r).z of the last two iterations
   double alpha[2]; // step lengths of the last two iterations
+  double rs0;      // adjoint solves: r.r of the starting residual
 };
 
 struct Chunk
@@ -316,10 +317,11 @@ __global__ void k_bmg_update(const DNode *__restrict__ nodes, int first, BMg *__
 // [1-tol_change, 1+tol_change] of its previous value (poisson_error.c:36-81)
 // and the residual meets the CG criterion of conj_grad_error
 // (poisson_error.c:236-288): sum r^2 < nintr (tol 4 pi G rho_mean)^2.
-// One workgroup per node.
+// Adjoint solves (zero start, zero Dirichlet values) stop on the residual
+// alone: sum r^2 <= tol^2 times its starting value.  One workgroup per node.
 __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int first, const int *__restrict__ it_base,
                             int it_local, const double *__restrict__ part_rs, const double *__restrict__ part_viol,
-                            double tol, double Gc, int *__restrict__ done_count)
+                            double tol, double Gc, int adjoint, int *__restrict__ done_count)
 {
   BMg &b = bm[blockIdx.x];
   if (b.done)
@@ -328,13 +330,24 @@ __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int
   const int have_change = outer > 0;
   DNode &d = nodes[first + blockIdx.x];
   const double rs = fold_range(part_rs, b.ic0, b.ic1, false);
-  const double viol = have_change ? fold_range(part_viol, b.ic0, b.ic1, true) : 0.0;
+  const double viol = have_change && !adjoint ? fold_range(part_viol, b.ic0, b.ic1, true) : 0.0;
   if (threadIdx.x == 0)
   {
-    const double H = 1.0 / (1 << d.lv);
-    const double rhomean = 4.0 * Gc * kPiD * d.mass / ((double)d.ncell * H * H * H);
-    const double aux_tol = (double)d.nintr * (tol * rhomean) * (tol * rhomean);
-    if ((have_change && viol == 0.0 && rs < aux_tol) || d.nintr == 0)
+    bool ok;
+    if (adjoint)
+    {
+      if (outer == 0)
+        b.rs0 = rs;
+      ok = rs <= tol * tol * b.rs0;
+    }
+    else
+    {
+      const double H = 1.0 / (1 << d.lv);
+      const double rhomean = 4.0 * Gc * kPiD * d.mass / ((double)d.ncell * H * H * H);
+      const double aux_tol = (double)d.nintr * (tol * rhomean) * (tol * rhomean);
+      ok = have_change && viol == 0.0 && rs < aux_tol;
+    }
+    if (ok || d.nintr == 0)
     {
       b.done = outer + 1;
       d.done = outer + 1;

@@ -154,8 +154,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
   bool keys_valid = false;
-  bool acc_live = false;
-  std::vector<double> host_mass; // the masses as given (snapshots write them unchanged) // particle accelerations hold values (false: all zero, as reset leaves them)
+  bool acc_live = false;         // particle accelerations hold values (false: all zero, as reset leaves them)
+  std::vector<double> host_mass; // the masses as given (snapshots write them unchanged)
   double *phi_prev = nullptr; // the previous converged head potential
   int phi_hist = 0;           // converged head potentials since the last reset of the history
   // captured outer iteration of the head solve, per (phi, rhs) pair: [0] the
@@ -165,8 +165,21 @@ template <typename P, typename G> struct Engine final : EngineBase
   unsigned char *refl_rec = nullptr; // when set, update_A records reflected axes (ID order)
   // adjoint state (qz_adjoint.inl)
   double *adj_xb = nullptr, *adj_vb = nullptr, *adj_ab = nullptr, *adj_abA = nullptr, *adj_psi = nullptr;
-  P *adj_slot = nullptr;
+  P *adj_slot = nullptr, *adj_mid = nullptr; // a cotangent in slot order; the masses in ID order
   G *adj_gbar[3] = {nullptr, nullptr, nullptr}, *adj_rhs = nullptr;
+  // refinement levels, in the grid-pool layout: Abar (3), phibar and psi in
+  // FP64, the node solves' right-hand side, point kinds; the head phibar sent up
+  // by level 1; transfer records (keys, values, sources) and their sort buffer
+  double *adj_nab[3] = {nullptr, nullptr, nullptr}, *adj_npb = nullptr, *adj_npsi = nullptr, *adj_hpb = nullptr;
+  G *adj_nrhs = nullptr;
+  unsigned char *adj_kind = nullptr;
+  long long adj_ncap = 0;
+  unsigned *adj_rkey[2] = {nullptr, nullptr};
+  int *adj_rval[2] = {nullptr, nullptr}, *adj_bpre = nullptr;
+  long long *adj_rsrc = nullptr;
+  void *adj_rtmp = nullptr;
+  size_t adj_rtmp_bytes = 0;
+  long long adj_rcap = 0, adj_bpre_cap = 0;
   struct Ckpt
   {
     double *x[3];
@@ -843,6 +856,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   // ---------------------------------------------------------------- adjoint (qz_adjoint.inl)
   void release_adjoint();
   bool ensure_adjoint();
+  bool ensure_adjoint_tree();
+  bool force_fields();
+  bool vjp_tree(const double *ab);
+  bool masses_by_id();
   bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
   bool force_vjp(const double *abar, double *xbar) override;
   bool adjoint_forward(int nsteps, double dt, int every) override;
@@ -854,7 +871,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool refine_head_device(int &zone_cells);
   bool potential_branches() override;
   bool branch_cg_level(size_t l);
-  bool branch_mg_level(size_t l, bool &solved);
+  bool branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint);
   bool ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool);
   template <typename F> bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done);
   bool branch_density();

@@ -1,10 +1,11 @@
-"""Adjoint of the head-grid step on the GPU: force VJP and checkpointed trajectory
-gradients against the FP64 CPU reverse mode (oracle) and central differences."""
+"""Adjoint of the step on the GPU (head grid, and with refinement levels): force
+VJP and checkpointed trajectory gradients against the FP64 CPU reverse mode
+(oracle) and central differences."""
 import numpy as np
 import pytest
 
 from quantizer_amd import core
-from quantizer_amd.workloads import plummer, plummer_dt
+from quantizer_amd.workloads import clumps, plummer, plummer_dt, two_plummer
 
 pytestmark = pytest.mark.gpu
 
@@ -95,3 +96,91 @@ def test_trajectory_gradient_central_differences():
             bad += 1
     # a perturbed particle can cross a cell face, where CIC weights have a kink
     assert bad <= 2
+
+
+@pytest.mark.parametrize("prec,tol", [(core.PREC_FP64, 1e-8), (core.PREC_MIXED, 2e-4)])
+def test_force_vjp_amr_matches_oracle(oracle, prec, tol):
+    print(f"case: force vjp with levels prec={prec} clumps N=12000 lmin=4 lmax=6", flush=True)
+    p = clumps(12000, 4, n_clumps=5, seed=103, sigma_cells=(0.4, 1.0))
+    rng = np.random.default_rng(9)
+    ab = rng.standard_normal((3, p.n))
+    o = oracle.OracleSim(p, 4, 6)
+    o.set_tolerance(1e-12)
+    want = o.force_vjp(ab)
+    with core.Simulation(p, 4, 6, precision=prec, tol=1e-11) as s:
+        got = s.force_vjp(ab)
+        assert s.stats()["n_nodes"] == o.stats()["nodes"]
+    assert rel(got, want) < tol
+
+
+def _ke_weight(p, n_each):
+    return np.where(np.arange(p.n) >= n_each, p.m, 0.0)  # the lighter sphere
+
+
+@pytest.mark.parametrize("levels", [(5, 5), (4, 6)])
+@pytest.mark.parametrize("prec,tol", [(core.PREC_FP64, 1e-6), (core.PREC_MIXED, 5e-3)])
+def test_trajectory_adjoint_two_spheres_matches_oracle(oracle, levels, prec, tol):
+    """Final kinetic energy of the lighter of two Plummer spheres (particle
+    masses 2:1), 6 steps with checkpoints every 4, against the oracle's reverse
+    mode (head grid and two refinement levels)."""
+    lmin, lmax = levels
+    print(f"case: two-sphere trajectory adjoint prec={prec} levels={levels} N=3000 6 steps ckpt 4", flush=True)
+    p = two_plummer(1500, seed=31)
+    nsteps, dt = 6, plummer_dt(0.01, 80.0)
+    w = _ke_weight(p, 1500)
+    with core.Simulation(p, lmin, lmax, precision=prec, tol=1e-11) as s:
+        s.force()
+        s.adjoint_forward(nsteps, dt, 4)
+        fin = s.particles()
+        vbN = w * fin[3:6]
+        gx, gv = s.adjoint_backward(np.zeros_like(vbN), vbN)
+        after = s.particles()
+        if lmax > lmin:
+            assert s.stats()["n_nodes"] > 0
+    np.testing.assert_array_equal(after, fin)  # context left at the final state
+    o = oracle.OracleSim(p, lmin, lmax)
+    o.set_tolerance(1e-12)
+    o.force()
+    wx, wv = o.adjoint(nsteps, dt, np.zeros_like(vbN), vbN)
+    assert rel(gx, wx) < tol
+    assert rel(gv, wv) < tol
+
+
+def test_trajectory_gradient_amr_central_differences():
+    """Central differences of the final kinetic energy of the lighter sphere on
+    seeded coordinates, two refinement levels (FP64 path)."""
+    print("case: FD check with levels two spheres N=3000 lmin=4 lmax=6 4 steps", flush=True)
+    p = two_plummer(1500, seed=32)
+    lmin, lmax, nsteps, dt = 4, 6, 4, plummer_dt(0.01, 80.0)
+    w = _ke_weight(p, 1500)
+
+    def loss(parts):
+        with core.Simulation(parts, lmin, lmax, precision=core.PREC_FP64, tol=1e-12) as s:
+            s.force()
+            for _ in range(nsteps):
+                s.step(dt)
+            v = s.particles()[3:6]
+        return 0.5 * float(np.sum(w * np.sum(v * v, axis=0))), v
+
+    L0, vN = loss(p)
+    with core.Simulation(p, lmin, lmax, precision=core.PREC_FP64, tol=1e-12) as s:
+        s.force()
+        s.adjoint_forward(nsteps, dt, 3)
+        gx, gv = s.adjoint_backward(np.zeros_like(vN), w * vN)
+    rng = np.random.default_rng(11)
+    eps = 1e-5  # the FP64 sum of the loss limits central differences to ~1e-5 relative at 1e-6
+    floor = 1e-13 * abs(L0) / eps  # resolution of a difference of two FP64 losses
+    names = ["x", "y", "z", "vx", "vy", "vz"]
+    bad = 0
+    for _ in range(24):
+        k = int(rng.integers(0, 6))
+        i = int(rng.integers(0, p.n))
+        plus, minus = p.copy(), p.copy()
+        getattr(plus, names[k])[i] += eps
+        getattr(minus, names[k])[i] -= eps
+        fd = (loss(plus)[0] - loss(minus)[0]) / (2 * eps)
+        g = (gx if k < 3 else gv)[k % 3, i]
+        if abs(fd - g) > 2e-4 * abs(g) + floor:
+            bad += 1
+    # a perturbed particle can cross a cell face, where CIC weights have a kink
+    assert bad <= 1

@@ -236,3 +236,78 @@ def test_oracle_reverse_mode_matches_central_differences(oracle):
               _traj_loss(oracle, minus, lmin, nsteps, dt, target)) / (2 * eps)
         g = (gx if arr in "xyz" else gv)[comp, i]
         assert abs(fd - g) <= 1e-5 * max(abs(fd), 1e-12) + 1e-12, (arr, i, fd, g)
+
+
+def _ke_loss(oracle, parts, lmin, lmax, nsteps, dt, weight, tol):
+    o = oracle.OracleSim(parts, lmin, lmax)
+    o.set_tolerance(tol)
+    o.force()
+    for _ in range(nsteps):
+        o.step(dt)
+    v = o.particles()[3:6]
+    o.close()
+    return 0.5 * float(np.sum(weight * np.sum(v * v, axis=0))), v
+
+
+def test_oracle_amr_force_vjp_matches_central_differences(oracle):
+    """The FP64 VJP of the force pass with two refinement levels (the tree held
+    fixed) against central differences of sum abar . a(x).  Within a cell the
+    functional is quadratic in a coordinate, so central differences are exact up
+    to the solver tolerance; a missing boundary transfer or a sign slip shows as
+    errors of the size of the gradient itself."""
+    from quantizer_amd.workloads import clumps
+    p = clumps(12000, 4, n_clumps=5, seed=103, sigma_cells=(0.4, 1.0))
+    lmin, lmax, tol = 4, 6, 1e-12
+    rng = np.random.default_rng(5)
+    ab = rng.standard_normal((3, p.n))
+    o = oracle.OracleSim(p, lmin, lmax)
+    o.set_tolerance(tol)
+    g = o.force_vjp(ab)
+    assert o.stats()["nodes"] >= 3  # two levels, several nodes
+
+    def L(parts):
+        oo = oracle.OracleSim(parts, lmin, lmax)
+        oo.set_tolerance(tol)
+        oo.force()
+        a = oo.particles()[6:9]
+        oo.close()
+        return float(np.sum(ab * a))
+
+    scale = float(np.max(np.abs(g)))
+    eps = 1e-6
+    names = ["x", "y", "z"]
+    for _ in range(10):
+        i, c = int(rng.integers(0, p.n)), int(rng.integers(0, 3))
+        plus, minus = p.copy(), p.copy()
+        getattr(plus, names[c])[i] += eps
+        getattr(minus, names[c])[i] -= eps
+        fd = (L(plus) - L(minus)) / (2 * eps)
+        assert abs(fd - g[c, i]) <= 1e-6 * scale, (i, c, fd, g[c, i])
+
+
+def test_oracle_amr_reverse_mode_matches_central_differences(oracle):
+    """The FP64 reverse mode of the step with refinement levels: final kinetic
+    energy of the lighter of two Plummer spheres (masses 2:1) after 4 steps,
+    against central differences in positions and velocities."""
+    from quantizer_amd.workloads import plummer_dt, two_plummer
+    p = two_plummer(1500, seed=31)
+    lmin, lmax, nsteps, dt, tol = 4, 6, 4, plummer_dt(0.01, 80.0), 1e-12
+    weight = np.where(np.arange(p.n) >= 1500, p.m, 0.0)
+    _, vN = _ke_loss(oracle, p, lmin, lmax, nsteps, dt, weight, tol)
+    o = oracle.OracleSim(p, lmin, lmax)
+    o.set_tolerance(tol)
+    o.force()
+    gx, gv = o.adjoint(nsteps, dt, np.zeros_like(vN), weight * vN)
+    assert o.stats()["nodes"] >= 2
+    # eps = 1e-5: the loss is summed in FP64, so its rounding limits a central
+    # difference to about 1e-5 relative at eps = 1e-6
+    eps = 1e-5
+    for (arr, comp, i) in [("x", 0, 1600), ("y", 1, 2900), ("z", 2, 100), ("vx", 0, 1700), ("vz", 2, 2000),
+                           ("x", 0, 20)]:
+        plus, minus = p.copy(), p.copy()
+        getattr(plus, arr)[i] += eps
+        getattr(minus, arr)[i] -= eps
+        fd = (_ke_loss(oracle, plus, lmin, lmax, nsteps, dt, weight, tol)[0] -
+              _ke_loss(oracle, minus, lmin, lmax, nsteps, dt, weight, tol)[0]) / (2 * eps)
+        g = (gx if arr in "xyz" else gv)[comp, i]
+        assert abs(fd - g) <= 2e-4 * abs(g) + 1e-14, (arr, i, fd, g)
