@@ -18,6 +18,10 @@ the same workload for the speed-up.
 
 adjoint_fwd_ratio: Plummer N = 2^18, 64^3 head grid, 50 steps, checkpoints
 every 10: (checkpointed forward + reverse sweep) time over plain forward time.
+adjoint_amr: the same ratio with refinement levels: two Plummer spheres of
+2^23 particles each, 256^3 head grid, 3 refinement levels, the adjoint of the
+last 20 of 40 steps with checkpoints every 5, loss = final kinetic energy of
+the lighter sphere.
 
 roofline: the dominant kernel (the head's red-black Gauss-Seidel half sweep;
 its finest-level launches carry most of its time) replayed back to back on the
@@ -167,6 +171,67 @@ def adjoint_ratio(steps=50, every=10):
             "config": f"Plummer N=2^18, head 64^3 (lmin=6), {steps} steps, checkpoint every {every}, mixed"}
 
 
+def adjoint_full_combination(n_each=1 << 23, lmin=8, lmax=11, crit=8, warm=40, steps=20, every=5):
+    """Two Plummer spheres (2 x 2^23, radii 1 and 0.5, masses 2:1, 20a apart,
+    closing at 0.5) in an 80a box, 256^3 head grid, 3 refinement levels, mixed
+    path.  After `warm` forward steps: `steps` plain steps timed, then the same
+    `steps` as a checkpointed forward (every `every`) plus the reverse sweep of
+    the lighter sphere's final kinetic energy."""
+    import numpy as np
+    from quantizer_amd import core
+    from quantizer_amd.workloads import two_plummer
+    p = two_plummer(n_each, seed=20261203)
+    w = np.where(np.arange(p.n) >= n_each, p.m, 0.0)
+    with core.Simulation(p, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        s.force()
+        dt = s.timestep(0.5)
+        for k in range(warm):
+            s.step(dt)
+            if k % 10 == 9:
+                s.sync()
+                log(f"    adjoint workload: {k + 1} of {warm} steps")
+        s.sync()
+        state = s.particles()
+        st = s.stats()
+    from quantizer_amd.workloads import Particles
+    mid = Particles(*[np.ascontiguousarray(a) for a in state[:6]], p.m)
+    with core.Simulation(mid, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        s.force()
+        s.step(dt)
+        s.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            s.step(dt)
+        s.sync()
+        t_fwd = time.perf_counter() - t
+    log(f"    adjoint workload: forward {1e3 * t_fwd / steps:.2f} ms/step")
+    with core.Simulation(mid, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        s.force()
+        s.adjoint_forward(2, dt, 1)  # warm-up of the adjoint buffers
+        s.adjoint_backward(np.zeros((3, p.n)), np.zeros((3, p.n)))
+        s.sync()
+    with core.Simulation(mid, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        s.force()
+        s.sync()
+        t = time.perf_counter()
+        s.adjoint_forward(steps, dt, every)
+        s.sync()
+        t_af = time.perf_counter() - t
+        log(f"    adjoint workload: checkpointed forward {1e3 * t_af / steps:.2f} ms/step")
+        vN = s.particles()[3:6]
+        t = time.perf_counter()
+        gx, gv = s.adjoint_backward(np.zeros_like(vN), w * vN)
+        s.sync()
+        t_ab = time.perf_counter() - t
+        ok = bool(np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)))
+    return {"value": (t_af + t_ab) / t_fwd, "forward_ms_per_step": 1e3 * t_fwd / steps,
+            "checkpointed_forward_ms_per_step": 1e3 * t_af / steps, "reverse_ms_per_step": 1e3 * t_ab / steps,
+            "n_nodes": st["n_nodes"], "gradients_finite": ok, "dt": dt,
+            "config": f"two Plummer spheres 2x2^{n_each.bit_length() - 1}, 80a box, head {1 << lmin}^3, "
+                      f"lmax={lmax}, refine at >= {crit}; {warm} steps, then the adjoint of the last {steps} "
+                      f"(checkpoint every {every}), loss = final kinetic energy of the lighter sphere, mixed"}
+
+
 def precision_study(parts, lmin, lmax, crit, dt, steps=20):
     """Mixed against the all-FP64 GPU path on the same workload: head potential
     and grid acceleration, particle accelerations after the first force pass,
@@ -257,6 +322,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-precision", action="store_true")
+    ap.add_argument("--skip-adjoint-amr", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=150.0)
     ap.add_argument("--cpu-n", type=int, default=1 << 24, help="particles in the CPU sample")
     ap.add_argument("--inner", type=int, default=0)
@@ -301,6 +367,10 @@ def main():
     if not args.skip_adjoint:
         adj = adjoint_ratio()
         log(f"  adjoint/forward: {adj['value']:.3f} ({adj})")
+    adj_amr = None
+    if not args.skip_adjoint_amr:
+        adj_amr = adjoint_full_combination()
+        log(f"  adjoint/forward with refinement: {adj_amr['value']:.3f} ({adj_amr})")
     cpu = None
     if not args.skip_cpu:
         ns = min(N, args.cpu_n)
@@ -329,6 +399,7 @@ def main():
         "precision_vs_fp64": prec,
         "adjoint_fwd_ratio": adj["value"] if adj else None,
         "adjoint": adj,
+        "adjoint_amr": adj_amr,
         "solver": {k: st_mx[k] for k in ("head_outer", "head_vcycles", "head_residual", "n_nodes", "branch_points",
                                          "branch_intr", "branch_iterations", "branch_cg_levels")},
         "roofline": roof,

@@ -340,7 +340,7 @@ class Simulation:
         _check(load_library().qz_timestep(self._h, cfl, dt_max, _d(out)), "qz_timestep")
         return float(out[0])
 
-    # ------------------------------------------------------------ adjoint (head grid)
+    # ------------------------------------------------------------ adjoint
     def force_vjp(self, abar: np.ndarray) -> np.ndarray:
         """J_A(x)^T abar for the force pass at the current positions; abar is (3, N)."""
         a = np.ascontiguousarray(abar, dtype=np.float64).reshape(-1)

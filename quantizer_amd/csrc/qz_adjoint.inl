@@ -39,13 +39,7 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
   adj_ncap = adj_rcap = adj_bpre_cap = 0;
   adj_rtmp_bytes = 0;
   for (auto &c : ckpts)
-  {
-    for (int a = 0; a < 3; a++)
-      (void)hipFree(c.x[a]);
-    for (int a = 0; a < 7; a++)
-      (void)hipFree(c.a[a]);
-    (void)hipFree(c.id);
-  }
+    free_ckpt(c);
   ckpts.clear();
   for (auto &r : recs)
   {
@@ -108,10 +102,13 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
   }
   QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
              adj_gbar[2], adj_rhs, make_cube(n + 1), 1.0 / (2.0 * H));
-  // Lap psi = phibar with zero faces: the same multigrid, adjoint convergence test
-  QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
-  if (!solve_head(adj_psi, adj_rhs, true))
+  // Lap psi = phibar with zero faces: the same multigrid, adjoint convergence
+  // test; within a reverse sweep it starts from the previous step's psi
+  if (!adj_warm)
+    QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
+  if (!solve_head(adj_psi, adj_rhs, true, adj_warm))
     return false;
+  adj_warm = adj_sweep;
   const double coeff = 4.0 * cfg.G * kPi / (H * H * H);
   QZ_LAUNCHB("adj_gather", (24.0 + sizeof(P) + 4.0 + 48.0) * N + (8.0 + 3.0 * sizeof(G)) * npts,
              (k_adjoint_gather<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi, gax, gay, gaz, ab,
@@ -287,9 +284,11 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
   QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
              adj_gbar[2], adj_rhs, cb, 1.0 / (2.0 * H));
   QZ_LAUNCH("adj_head_add", k_adj_head_add<G>, nblk(npts), adj_rhs, adj_hpb, cb);
-  QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
-  if (!solve_head(adj_psi, adj_rhs, true))
+  if (!adj_warm)
+    QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
+  if (!solve_head(adj_psi, adj_rhs, true, adj_warm))
     return false;
+  adj_warm = adj_sweep;
   QZ_LAUNCH("adj_gather_amr", (k_adjoint_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi,
             gax, gay, gaz, d.head_child, d.nodes, d.box, adj_npsi, (const G *)d.ax, (const G *)d.ay, (const G *)d.az,
             ab, ab + N, ab + 2 * (size_t)N, adj_xb, adj_xb + N, adj_xb + 2 * (size_t)N, N, cfg.lmin,
@@ -315,9 +314,75 @@ template <typename P, typename G> bool Engine<P, G>::force_vjp(const double *aba
   if (!hip_ok(hipMemcpy(adj_ab, abar, sizeof(double) * n3, hipMemcpyHostToDevice), "upload abar"))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_xb, 0.0, n3);
+  adj_sweep = adj_warm = false;
   if (!vjp_at_current(adj_ab))
     return false;
   return hip_ok(hipMemcpy(xbar, adj_xb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download xbar");
+}
+
+// A checkpoint: the particle state in its sorted order plus the head potential
+// and the potential history the next head solve starts from, so that a
+// recomputation from it repeats the forward steps exactly.
+template <typename P, typename G> bool Engine<P, G>::save_ckpt(Ckpt &c)
+{
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+  size_t b = 0;
+  bool ok = true;
+  for (int a = 0; a < 3; a++)
+  {
+    c.x[a] = dmalloc<double>(N, b);
+    ok = ok && c.x[a] &&
+         hip_ok(hipMemcpyAsync(c.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
+  }
+  for (int a = 0; a < 7; a++)
+  {
+    c.a[a] = dmalloc<P>(N, b);
+    ok = ok && c.a[a] &&
+         hip_ok(hipMemcpyAsync(c.a[a], pa[3 + a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
+  }
+  c.id = dmalloc<int>(N, b);
+  ok = ok && c.id && hip_ok(hipMemcpyAsync(c.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
+  c.phi = dmalloc<double>(npts, b);
+  ok = ok && c.phi &&
+       hip_ok(hipMemcpyAsync(c.phi, phi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream), "checkpoint");
+  c.phi_prev = nullptr;
+  if (phi_prev)
+  {
+    c.phi_prev = dmalloc<double>(npts, b);
+    ok = ok && c.phi_prev &&
+         hip_ok(hipMemcpyAsync(c.phi_prev, phi_prev, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream),
+                "checkpoint");
+  }
+  c.hist = phi_hist;
+  return ok || hip_ok(hipErrorOutOfMemory, "checkpoint");
+}
+
+template <typename P, typename G> void Engine<P, G>::restore_ckpt(const Ckpt &c)
+{
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+  for (int a = 0; a < 3; a++)
+    (void)hipMemcpyAsync(pos[a], c.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
+  for (int a = 0; a < 7; a++)
+    (void)hipMemcpyAsync(pa[3 + a], c.a[a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream);
+  (void)hipMemcpyAsync(id_a, c.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+  (void)hipMemcpyAsync(phi, c.phi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream);
+  if (c.phi_prev && phi_prev)
+    (void)hipMemcpyAsync(phi_prev, c.phi_prev, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream);
+  phi_hist = c.phi_prev && phi_prev ? c.hist : 0;
+  keys_valid = false;
+  acc_live = true;
+}
+
+template <typename P, typename G> void Engine<P, G>::free_ckpt(Ckpt &c)
+{
+  for (int a = 0; a < 3; a++)
+    (void)hipFree(c.x[a]);
+  for (int a = 0; a < 7; a++)
+    (void)hipFree(c.a[a]);
+  (void)hipFree(c.id);
+  (void)hipFree(c.phi);
+  (void)hipFree(c.phi_prev);
+  c = Ckpt{};
 }
 
 // Checkpointed forward: from the current state (a force pass done), run nsteps
@@ -332,13 +397,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
     return false;
   }
   for (auto &c : ckpts)
-  {
-    for (int a = 0; a < 3; a++)
-      (void)hipFree(c.x[a]);
-    for (int a = 0; a < 7; a++)
-      (void)hipFree(c.a[a]);
-    (void)hipFree(c.id);
-  }
+    free_ckpt(c);
   ckpts.clear();
   adj_nsteps = nsteps;
   adj_every = every;
@@ -348,25 +407,10 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
     if (s % every == 0 && s < nsteps)
     {
       Ckpt c{};
-      size_t b = 0;
-      for (int a = 0; a < 3; a++)
-      {
-        c.x[a] = dmalloc<double>(N, b);
-        if (!c.x[a] || !hip_ok(hipMemcpyAsync(c.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream),
-                               "checkpoint"))
-          return false;
-      }
-      for (int a = 0; a < 7; a++)
-      {
-        c.a[a] = dmalloc<P>(N, b);
-        if (!c.a[a] || !hip_ok(hipMemcpyAsync(c.a[a], pa[3 + a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream),
-                               "checkpoint"))
-          return false;
-      }
-      c.id = dmalloc<int>(N, b);
-      if (!c.id || !hip_ok(hipMemcpyAsync(c.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream), "checkpoint"))
-        return false;
+      const bool ok = save_ckpt(c);
       ckpts.push_back(c);
+      if (!ok)
+        return false;
     }
     if (s == nsteps)
       break;
@@ -390,21 +434,22 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   const double dt = adj_dt, h = 0.5 * dt;
   if (!masses_by_id())
     return false;
-  // keep the final state to restore it afterwards
+  // keep the final state to restore it afterwards (freed on every return)
   Ckpt fin{};
-  size_t b = 0;
-  for (int a = 0; a < 3; a++)
+  struct FinGuard
   {
-    fin.x[a] = dmalloc<double>(N, b);
-    (void)hipMemcpyAsync(fin.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
-  }
-  for (int a = 0; a < 7; a++)
-  {
-    fin.a[a] = dmalloc<P>(N, b);
-    (void)hipMemcpyAsync(fin.a[a], pa[3 + a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream);
-  }
-  fin.id = dmalloc<int>(N, b);
-  (void)hipMemcpyAsync(fin.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+    Engine *e;
+    Ckpt *c;
+    ~FinGuard()
+    {
+      e->free_ckpt(*c);
+      e->adj_sweep = e->adj_warm = false;
+    }
+  } fin_guard{this, &fin};
+  if (!save_ckpt(fin))
+    return false;
+  adj_sweep = true;
+  adj_warm = false;
   // per-step records for one segment (with levels: positions, IDs, reflections)
   while ((int)recs.size() < adj_every + 1)
   {
@@ -424,15 +469,6 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       return hip_ok(hipErrorOutOfMemory, "adjoint step records");
     recs.push_back(r);
   }
-  auto restore_ckpt = [&](const Ckpt &c) {
-    for (int a = 0; a < 3; a++)
-      (void)hipMemcpyAsync(pos[a], c.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
-    for (int a = 0; a < 7; a++)
-      (void)hipMemcpyAsync(pa[3 + a], c.a[a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream);
-    (void)hipMemcpyAsync(id_a, c.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
-    keys_valid = false;
-    acc_live = true;
-  };
   // record the sorted state after a force pass (positions, ids; head grid only:
   // keys, ranges, grid a)
   auto save_rec = [&](StepRec &r) {
@@ -512,21 +548,13 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       !hip_ok(hipMemcpy(vb0, adj_vb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download vbar"))
     return false;
   // back to the final forward state: its fields (and tree) are rebuilt, then the
-  // state is restored once more for the exact accelerations (the state is in
-  // key order, so the sort of the force pass leaves the order unchanged); the
-  // head solve's starting-guess history restarts from the final potential
+  // state is restored once more for the exact accelerations, potential and
+  // potential history (the state is in key order, so the sort of the force
+  // pass leaves the order unchanged)
   restore_ckpt(fin);
   if (!force_fields())
     return false;
   restore_ckpt(fin);
-  phi_hist = 0;
-  if (!hip_ok(hipStreamSynchronize(stream), "adjoint_backward"))
-    return false;
-  for (int a = 0; a < 3; a++)
-    (void)hipFree(fin.x[a]);
-  for (int a = 0; a < 7; a++)
-    (void)hipFree(fin.a[a]);
-  (void)hipFree(fin.id);
   return hip_ok(hipStreamSynchronize(stream), "adjoint_backward");
 }
 

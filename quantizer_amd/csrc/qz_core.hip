@@ -185,6 +185,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     double *x[3];
     P *a[7];
     int *id;
+    double *phi, *phi_prev; // head potential and the previous one (the solve's starting guess)
+    int hist;
   };
   struct StepRec
   {
@@ -198,6 +200,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   std::vector<Ckpt> ckpts;
   std::vector<StepRec> recs;
   int adj_nsteps = 0, adj_every = 0;
+  bool adj_sweep = false; // a reverse sweep is running (its head solves warm-start)
+  bool adj_warm = false;  // adj_psi holds the previous adjoint head solution of this reverse sweep
   double adj_dt = 0.0;
 
   ~Engine() override { release(); }
@@ -513,7 +517,11 @@ template <typename P, typename G> struct Engine final : EngineBase
       if (move_acc || a < 6 || a == 9)
         std::swap(pa[a], pb[a]);
     std::swap(id_a, id_b);
-    QZ_LAUNCHB("cell_start", 8.0 * N + 4.0 * n * n * n, k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n, cell_start);
+    const size_t ncs = (size_t)n * n * n + 1;
+    QZ_LAUNCH("cell_start", k_fill<int>, nblk(ncs), cell_start, -1, ncs);
+    QZ_LAUNCHB("cell_start", 8.0 * N + 4.0 * n * n * n, k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n,
+               cell_start);
+    QZ_LAUNCH("cell_start", k_cell_start_gaps, nblk(ncs), key_out, N, D, n * n * n, cell_start);
     keys_valid = true;
     return true;
   }
@@ -630,9 +638,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   // (method 0): the RMS residual over interior points divided by 4 pi G
   // rho_mean, and the pointwise change of phi over the last outer iteration (the
   // reference compares with phi from iter_between_check_potential_solution = 5
-  // V-cycles earlier).  Adjoint solve (zero faces, ph starts at 0): the RMS
-  // residual relative to the RMS right-hand side.
-  bool solve_head(double *ph, const G *rh, bool adjoint)
+  // V-cycles earlier).  Adjoint solve (zero faces): the RMS residual relative to
+  // the RMS right-hand side; ph starts at 0, or (warm) at the previous adjoint
+  // solution of the reverse sweep.
+  bool solve_head(double *ph, const G *rh, bool adjoint, bool warm = false)
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
@@ -644,6 +653,17 @@ template <typename P, typename G> struct Engine final : EngineBase
     unsigned nb = nred(npts);
     int outers = 0, vcycles = 0;
     double change = 1e300, res = 0.0, scale = rhomean;
+    if (adjoint && warm)
+    {
+      QZ_LAUNCHB("outer_residual", (double)sizeof(G) * npts, k_interior_sumsq<G>, nb, rh, make_cube(n + 1), part);
+      hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+      if (!hip_ok(hipMemcpyAsync(hred, red, sizeof(double), hipMemcpyDeviceToHost, stream), "residual readback") ||
+          !hip_ok(hipStreamSynchronize(stream), "residual"))
+        return false;
+      scale = std::sqrt(hred[0] / ninterior);
+      if (scale == 0.0)
+        scale = 1.0;
+    }
     QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0],
                make_cube(n + 1), inv_h2, part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
@@ -654,7 +674,7 @@ template <typename P, typename G> struct Engine final : EngineBase
           !hip_ok(hipStreamSynchronize(stream), "residual"))
         return false;
       double rms = std::sqrt(hred[0] / ninterior);
-      if (adjoint && outer == 0)
+      if (adjoint && !warm && outer == 0)
         scale = rms > 0.0 ? rms : 1.0;
       res = rms / scale;
       if (outer > 0)
@@ -860,6 +880,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool force_fields();
   bool vjp_tree(const double *ab);
   bool masses_by_id();
+  bool save_ckpt(Ckpt &c);
+  void restore_ckpt(const Ckpt &c);
+  void free_ckpt(Ckpt &c);
   bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
   bool force_vjp(const double *abar, double *xbar) override;
   bool adjoint_forward(int nsteps, double dt, int every) override;

@@ -226,6 +226,12 @@ __global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__res
 }
 
 // start[c] = first sorted slot whose head cell is >= c, for c in [0, ncell].
+// Slot i owns the cells after the previous slot's cell up to its own; it writes
+// at most kCellFill of them (the last ones), so clustered particles (long runs
+// of empty cells) cost no long serial loops.  The rest keep the -1 of the fill
+// before and are found by k_cell_start_gaps.
+constexpr int kCellFill = 64;
+
 __global__ void k_cell_start(const unsigned long long *__restrict__ key, int N, int D, int ncell,
                              int *__restrict__ start)
 {
@@ -234,8 +240,28 @@ __global__ void k_cell_start(const unsigned long long *__restrict__ key, int N, 
     return;
   long long prev = i == 0 ? -1 : (long long)(key[i - 1] >> (3 * D));
   long long cur = i == N ? (long long)ncell : (long long)(key[i] >> (3 * D));
-  for (long long c = prev + 1; c <= cur; c++)
+  for (long long c = max(prev + 1, cur - (kCellFill - 1)); c <= cur; c++)
     start[c] = i;
+}
+
+// The cells k_cell_start left at -1: binary search for the first slot whose
+// head cell is >= c.
+__global__ void k_cell_start_gaps(const unsigned long long *__restrict__ key, int N, int D, int ncell,
+                                  int *__restrict__ start)
+{
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncell || start[c] >= 0)
+    return;
+  int lo = 0, hi = N;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi) >> 1;
+    if ((long long)(key[mid] >> (3 * D)) < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  start[c] = lo;
 }
 
 // A2 deposit, stage 1 (grid_density.c:35-267): every particle computes its 8
@@ -616,6 +642,29 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
       sq += res * res;
     }
     r[(t & 1) * hf + (t >> 1)] = (G)res;
+  }
+  sq = block_reduce<0>(sq);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = sq;
+}
+
+// Partial sums of rho^2 over the interior points (the residual of phi = 0).
+template <typename G> __global__ void k_interior_sumsq(const G *__restrict__ rho, Cube cb, double *__restrict__ part)
+{
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned npts = (unsigned)n1 * n1 * n1;
+  double sq = 0.0;
+  const unsigned chunk = ((npts + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  const unsigned t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(npts, t0 + chunk);
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+  {
+    int i, j, k;
+    cube_ijk(cb, t, i, j, k);
+    if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+    {
+      const double v = (double)rho[t];
+      sq += v * v;
+    }
   }
   sq = block_reduce<0>(sq);
   if (threadIdx.x == 0)

@@ -55,3 +55,28 @@ def test_head_steps_match_oracle(oracle, prec, tol_x, tol_v):
         assert np.max(np.abs(pg[k] - po[k])) < tol_x
     for k in range(3, 6):
         assert rel(pg[k], po[k]) < tol_v
+
+
+def test_clustered_particles_cell_ranges(oracle):
+    """Particles packed into two far-apart corner regions of a 128^3 head grid
+    (runs of millions of empty cells between occupied ones): density and
+    accelerations against the oracle."""
+    print("case: clustered cell ranges N=3000 lmin=7", flush=True)
+    rng = np.random.default_rng(21)
+    n = 3000
+    x = np.concatenate([rng.uniform(0.01, 0.03, n // 2), rng.uniform(0.97, 0.99, n - n // 2)])
+    y = np.concatenate([rng.uniform(0.01, 0.03, n // 2), rng.uniform(0.97, 0.99, n - n // 2)])
+    z = np.concatenate([rng.uniform(0.01, 0.03, n // 2), rng.uniform(0.97, 0.99, n - n // 2)])
+    zero = np.zeros(n)
+    from quantizer_amd.workloads import Particles
+    p = Particles(x, y, z, zero.copy(), zero.copy(), zero.copy(), np.full(n, 1.0 / n))
+    o = oracle.OracleSim(p, 7, 7)
+    o.force()
+    ho = o.head()
+    with core.Simulation(p, 7, 7, precision=core.PREC_FP64) as s:
+        s.force()
+        hg = s.head_fields()
+        pg = s.particles()
+    assert rel(hg[0], ho[0]) < 1e-12
+    for k in (6, 7, 8):
+        assert rel(pg[k], o.particles()[k]) < 1e-6
