@@ -385,6 +385,74 @@ template <typename P, typename G> void Engine<P, G>::free_ckpt(Ckpt &c)
   c = Ckpt{};
 }
 
+// Per-step records for one segment: positions and IDs in sorted order and the
+// reflected axes of the step; head grid only also keys, head cell ranges and
+// grid accelerations (with levels the force pass is redone instead).
+template <typename P, typename G> bool Engine<P, G>::ensure_recs()
+{
+  const bool amr = cfg.lmin < cfg.lmax;
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
+  while ((int)recs.size() < adj_every + 1)
+  {
+    StepRec r{};
+    size_t bb = 0;
+    for (int a = 0; a < 3; a++)
+    {
+      r.x[a] = dmalloc<double>(N, bb);
+      r.g[a] = amr ? nullptr : dmalloc<G>(npts, bb);
+    }
+    r.id = dmalloc<int>(N, bb);
+    r.key = amr ? nullptr : dmalloc<unsigned long long>(N, bb);
+    r.cstart = amr ? nullptr : dmalloc<int>(ncell + 1, bb);
+    r.refl = dmalloc<unsigned char>(N, bb);
+    const bool ok = r.x[0] && r.x[1] && r.x[2] && r.id && r.refl &&
+                    (amr || (r.g[0] && r.g[1] && r.g[2] && r.key && r.cstart));
+    recs.push_back(r);
+    if (!ok)
+      return hip_ok(hipErrorOutOfMemory, "adjoint step records");
+  }
+  return true;
+}
+
+// Record the sorted state after a force pass.
+template <typename P, typename G> void Engine<P, G>::save_rec(StepRec &r)
+{
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
+  for (int a = 0; a < 3; a++)
+    (void)hipMemcpyAsync(r.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
+  (void)hipMemcpyAsync(r.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+  if (cfg.lmin < cfg.lmax)
+    return;
+  (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
+  (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
+  G *src[3] = {gax, gay, gaz};
+  for (int a = 0; a < 3; a++)
+    (void)hipMemcpyAsync(r.g[a], src[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
+}
+
+// Make a recorded state current for its VJP (with levels: masses to the
+// recorded order, then the force pass).
+template <typename P, typename G> bool Engine<P, G>::load_rec(const StepRec &r)
+{
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
+  for (int a = 0; a < 3; a++)
+    (void)hipMemcpyAsync(pos[a], r.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
+  (void)hipMemcpyAsync(id_a, r.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+  if (cfg.lmin < cfg.lmax)
+  {
+    QZ_LAUNCH("adj_mass", k_id_to_slot<P>, nblk(N), adj_mid, id_a, pa[9], N);
+    keys_valid = false;
+    return force_fields();
+  }
+  (void)hipMemcpyAsync(key_out, r.key, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
+  (void)hipMemcpyAsync(cell_start, r.cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
+  G *dst[3] = {gax, gay, gaz};
+  for (int a = 0; a < 3; a++)
+    (void)hipMemcpyAsync(dst[a], r.g[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
+  keys_valid = true;
+  return true;
+}
+
 // Checkpointed forward: from the current state (a force pass done), run nsteps
 // fixed-dt steps, keeping the full particle state every `every` steps.
 template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps, double dt, int every)
@@ -402,6 +470,12 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
   adj_nsteps = nsteps;
   adj_every = every;
   adj_dt = dt;
+  adj_last_recorded = false;
+  if (!ensure_recs())
+    return false;
+  // the last segment [last, nsteps] is recorded on the way (the reverse sweep
+  // starts there, so it needs no recomputation)
+  const int last = ((nsteps - 1) / every) * every;
   for (int s = 0; s <= nsteps; s++)
   {
     if (s % every == 0 && s < nsteps)
@@ -412,11 +486,20 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
       if (!ok)
         return false;
     }
+    if (s == 0 && last == 0)
+      save_rec(recs[0]); // the force pass of state 0 is current
     if (s == nsteps)
       break;
-    if (!update_A(dt) || !force_fields() || !particle_acceleration() || !update_B(dt))
+    const bool rec = s + 1 > last;
+    refl_rec = rec ? recs[s + 1 - last].refl : nullptr;
+    const bool ok = update_A(dt);
+    refl_rec = nullptr;
+    if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt))
       return false;
+    if (rec)
+      save_rec(recs[s + 1 - last]);
   }
+  adj_last_recorded = true;
   return hip_ok(hipStreamSynchronize(stream), "adjoint_forward");
 }
 
@@ -428,9 +511,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     set_error("adjoint_backward: run qz_adjoint_forward first");
     return false;
   }
-  const bool amr = cfg.lmin < cfg.lmax;
-  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, n3 = 3 * (size_t)N;
-  const size_t ncell = (size_t)n * n * n;
+  const size_t n3 = 3 * (size_t)N;
   const double dt = adj_dt, h = 0.5 * dt;
   if (!masses_by_id())
     return false;
@@ -443,66 +524,16 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     ~FinGuard()
     {
       e->free_ckpt(*c);
-      e->adj_sweep = e->adj_warm = false;
+      e->adj_sweep = e->adj_warm = e->adj_last_recorded = false;
     }
   } fin_guard{this, &fin};
   if (!save_ckpt(fin))
     return false;
   adj_sweep = true;
   adj_warm = false;
-  // per-step records for one segment (with levels: positions, IDs, reflections)
-  while ((int)recs.size() < adj_every + 1)
-  {
-    StepRec r{};
-    size_t bb = 0;
-    for (int a = 0; a < 3; a++)
-    {
-      r.x[a] = dmalloc<double>(N, bb);
-      r.g[a] = amr ? nullptr : dmalloc<G>(npts, bb);
-    }
-    r.id = dmalloc<int>(N, bb);
-    r.key = amr ? nullptr : dmalloc<unsigned long long>(N, bb);
-    r.cstart = amr ? nullptr : dmalloc<int>(ncell + 1, bb);
-    r.refl = dmalloc<unsigned char>(N, bb);
-    if (!r.x[0] || !r.x[1] || !r.x[2] || !r.id || !r.refl ||
-        (!amr && (!r.g[0] || !r.g[1] || !r.g[2] || !r.key || !r.cstart)))
-      return hip_ok(hipErrorOutOfMemory, "adjoint step records");
-    recs.push_back(r);
-  }
-  // record the sorted state after a force pass (positions, ids; head grid only:
-  // keys, ranges, grid a)
-  auto save_rec = [&](StepRec &r) {
-    for (int a = 0; a < 3; a++)
-      (void)hipMemcpyAsync(r.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
-    (void)hipMemcpyAsync(r.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
-    if (amr)
-      return;
-    (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
-    (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
-    G *src[3] = {gax, gay, gaz};
-    for (int a = 0; a < 3; a++)
-      (void)hipMemcpyAsync(r.g[a], src[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
-  };
-  // make a recorded state current for its VJP (with levels: masses to the
-  // recorded order, then the force pass)
-  auto load_rec = [&](const StepRec &r) {
-    for (int a = 0; a < 3; a++)
-      (void)hipMemcpyAsync(pos[a], r.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
-    (void)hipMemcpyAsync(id_a, r.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
-    if (amr)
-    {
-      QZ_LAUNCH("adj_mass", k_id_to_slot<P>, nblk(N), adj_mid, id_a, pa[9], N);
-      keys_valid = false;
-      return force_fields();
-    }
-    (void)hipMemcpyAsync(key_out, r.key, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
-    (void)hipMemcpyAsync(cell_start, r.cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
-    G *dst[3] = {gax, gay, gaz};
-    for (int a = 0; a < 3; a++)
-      (void)hipMemcpyAsync(dst[a], r.g[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
-    keys_valid = true;
-    return true;
-  };
+  const bool amr = cfg.lmin < cfg.lmax;
+  if (!ensure_recs())
+    return false;
   if (!hip_ok(hipMemcpy(adj_xb, xbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload xbar") ||
       !hip_ok(hipMemcpy(adj_vb, vbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload vbar"))
     return false;
@@ -514,21 +545,32 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     // recompute the segment: rec[0] = state c0 (kept for the first segment
     // only, whose a_0 term closes the sweep), rec[k] = state c0 + k.  The
     // checkpoint carries a_c0, so no force pass is needed to step from it.
-    restore_ckpt(ckpts[sc]);
-    if (sc == 0)
+    // The last segment was recorded by adjoint_forward itself.
+    const bool recorded = adj_last_recorded && sc == (int)ckpts.size() - 1;
+    if (!recorded)
     {
-      if (!amr && (!force_fields() || !particle_acceleration()))
-        return false;
-      save_rec(recs[0]);
-    }
-    for (int j = c0 + 1; j <= c1; j++)
-    {
-      refl_rec = recs[j - c0].refl;
-      bool ok = update_A(dt);
-      refl_rec = nullptr;
-      if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt))
-        return false;
-      save_rec(recs[j - c0]);
+      restore_ckpt(ckpts[sc]);
+      if (sc == 0)
+      {
+        // head grid: the record of state 0 holds its fields; the force pass
+        // clears the accelerations, which the checkpoint then restores exactly
+        if (!amr)
+        {
+          if (!force_fields())
+            return false;
+          restore_ckpt(ckpts[sc]);
+        }
+        save_rec(recs[0]);
+      }
+      for (int j = c0 + 1; j <= c1; j++)
+      {
+        refl_rec = recs[j - c0].refl;
+        bool ok = update_A(dt);
+        refl_rec = nullptr;
+        if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt))
+          return false;
+        save_rec(recs[j - c0]);
+      }
     }
     // reverse sweep over the segment
     for (int j = c1; j > c0; j--)

@@ -376,11 +376,12 @@ __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg
   for (int i = c.begin + threadIdx.x; i < c.end; i += blockDim.x)
   {
     const long long p = alist[i];
+    const G rv = r[p]; // issued ahead of the neighbour loads
     const long long idx = 2 * (p - base) + color;
     G sum = zero_start ? (G)0
                        : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + X) >> 1] + o[(idx - X) >> 1] +
                              o[(idx + XY) >> 1] + o[(idx - XY) >> 1];
-    e[p] = (sum - h2 * r[p]) * (G)(1.0 / 6.0);
+    e[p] = (sum - h2 * rv) * (G)(1.0 / 6.0);
   }
 }
 
@@ -409,15 +410,20 @@ __global__ void k_bmg_restrict(const DNode *__restrict__ nodes, int first, const
   const G *eR = ef + b.off[m], *eB = ef + b.off[m] + hf;
   const G *rR = rf + b.off[m];
   const unsigned char *mR = mf + b.off[m];
-  // residual r - Lap e at the red fine point with linear index f (f even)
+  // residual r - Lap e at the red fine point with linear index f (f even), zero
+  // where the mask is clear.  An active coarse point has I in [1, X_c - 2] (the
+  // masks come from interior points by injection), so the 13 fine points and
+  // their neighbours stay inside the node's level-m block: the loads are safe
+  // unconditionally and the mask selects the value, which lets all loads issue
+  // together.
   auto red_res = [&](long long f) -> G {
     const long long h = f >> 1;
-    if (!mR[h])
-      return (G)0;
+    const bool on = mR[h] != 0;
     G lap = (eB[(f + 1) >> 1] + eB[(f - 1) >> 1] + eB[(f + Xf) >> 1] + eB[(f - Xf) >> 1] + eB[(f + XYf) >> 1] +
              eB[(f - XYf) >> 1] - (G)6 * eR[h]) *
             inv_h2;
-    return rR[h] - lap;
+    const G v = rR[h] - lap;
+    return on ? v : (G)0;
   };
   for (int i = c.begin + threadIdx.x; i < c.end; i += blockDim.x)
   {

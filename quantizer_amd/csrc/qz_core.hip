@@ -202,6 +202,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   int adj_nsteps = 0, adj_every = 0;
   bool adj_sweep = false; // a reverse sweep is running (its head solves warm-start)
   bool adj_warm = false;  // adj_psi holds the previous adjoint head solution of this reverse sweep
+  bool adj_last_recorded = false; // adjoint_forward recorded the last segment's steps
   double adj_dt = 0.0;
 
   ~Engine() override { release(); }
@@ -883,6 +884,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool save_ckpt(Ckpt &c);
   void restore_ckpt(const Ckpt &c);
   void free_ckpt(Ckpt &c);
+  bool ensure_recs();
+  void save_rec(StepRec &r);
+  bool load_rec(const StepRec &r);
   bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
   bool force_vjp(const double *abar, double *xbar) override;
   bool adjoint_forward(int nsteps, double dt, int every) override;

@@ -215,14 +215,27 @@ __global__ void k_permute(PtclArrays<P> src, PtclArrays<P> dst, const int *__res
   if (i >= N)
     return;
   int s = perm[i];
+  // all loads first, then all stores: the arrays are separate buffers, and
+  // interleaved (the compiler cannot rule out aliasing) every copy waited for
+  // its own load
+  double xv[3];
+  P av[7];
 #pragma unroll
   for (int a = 0; a < 3; a++)
-    dst.x[a][i] = src.x[a][s];
+    xv[a] = src.x[a][s];
 #pragma unroll
   for (int a = 0; a < 7; a++)
     if (move_acc || a < 3 || a == 6) // accelerations that are all zero stay where they are
-      dst.a[a][i] = src.a[a][s];
-  dst.id[i] = src.id[s];
+      av[a] = src.a[a][s];
+  const int idv = src.id[s];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    dst.x[a][i] = xv[a];
+#pragma unroll
+  for (int a = 0; a < 7; a++)
+    if (move_acc || a < 3 || a == 6)
+      dst.a[a][i] = av[a];
+  dst.id[i] = idv;
 }
 
 // start[c] = first sorted slot whose head cell is >= c, for c in [0, ncell].
@@ -730,6 +743,10 @@ __global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, Cube cb, G h2
   const unsigned h = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (h >= count)
     return;
+  // r is loaded first: issued ahead of the neighbour loads, its latency overlaps
+  // theirs (loaded after them, it cost a second memory round trip per wave)
+  const unsigned me = color * half + h;
+  const G rv = r[me];
   const unsigned idx = 2 * h + color;
   int i, j, k;
   cube_ijk(cb, idx, i, j, k);
@@ -742,8 +759,7 @@ __global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, Cube cb, G h2
   G sum = zero_start ? (G)0
                      : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + s1) >> 1] + o[(idx - s1) >> 1] +
                            o[(idx + s2) >> 1] + o[(idx - s2) >> 1];
-  const unsigned me = color * half + h;
-  e[me] = (sum - h2 * r[me]) * (G)(1.0 / 6.0);
+  e[me] = (sum - h2 * rv) * (G)(1.0 / 6.0);
 }
 
 // Residual of the fine error equation, full-weighting restricted to the coarse
@@ -770,10 +786,10 @@ __global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict
   const unsigned s1 = n1, s2 = (unsigned)n1 * n1;
   const unsigned hf = ((unsigned)n1 * n1 * n1 + 1) >> 1;
   const G *eB = e + hf; // black half
-  // red fine point f (even linear index): residual, zero on the faces
-  auto red_res = [&](unsigned f, int i, int j, int k) -> G {
-    if (i <= 0 || i >= n || j <= 0 || j >= n || k <= 0 || k >= n)
-      return (G)0;
+  // red fine point f (even linear index): residual.  For a coarse interior
+  // point (I, J, K in [1, nc - 1]) all 13 fine points lie in [1, n - 1], so no
+  // face test is needed; without one the loads of all 13 issue together.
+  auto red_res = [&](unsigned f, int, int, int) -> G {
     const unsigned hpos = f >> 1;
     G lap = (eB[(f + 1) >> 1] + eB[(f - 1) >> 1] + eB[(f + s1) >> 1] + eB[(f - s1) >> 1] + eB[(f + s2) >> 1] +
              eB[(f - s2) >> 1] - (G)6 * e[hpos]) *

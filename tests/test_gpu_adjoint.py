@@ -184,3 +184,26 @@ def test_trajectory_gradient_amr_central_differences():
             bad += 1
     # a perturbed particle can cross a cell face, where CIC weights have a kink
     assert bad <= 1
+
+
+@pytest.mark.parametrize("levels", [(5, 5), (4, 6)])
+def test_trajectory_adjoint_single_segment(oracle, levels):
+    """All steps in one checkpoint segment (recorded by the forward itself)."""
+    lmin, lmax = levels
+    print(f"case: single-segment trajectory adjoint levels={levels} N=3000 3 steps ckpt 5", flush=True)
+    p = two_plummer(1500, seed=33)
+    nsteps, dt = 3, plummer_dt(0.01, 80.0)
+    w = _ke_weight(p, 1500)
+    with core.Simulation(p, lmin, lmax, precision=core.PREC_FP64, tol=1e-11) as s:
+        s.force()
+        s.adjoint_forward(nsteps, dt, 5)
+        fin = s.particles()
+        vbN = w * fin[3:6]
+        gx, gv = s.adjoint_backward(np.zeros_like(vbN), vbN)
+        np.testing.assert_array_equal(s.particles(), fin)
+    o = oracle.OracleSim(p, lmin, lmax)
+    o.set_tolerance(1e-12)
+    o.force()
+    wx, wv = o.adjoint(nsteps, dt, np.zeros_like(vbN), vbN)
+    assert rel(gx, wx) < 1e-6
+    assert rel(gv, wv) < 1e-6
