@@ -225,17 +225,18 @@ __global__ void k_adj_branch_scatter(const DNode *__restrict__ nodes, int first,
   {
     int gl, gx, gy, gz;
     adj_point(d, t, bdry_g, bdry_xyz, intr_g, intr_xyz, gl, gx, gy, gz);
-    double acc = 0.0;
-    for (int ck = gz - 1; ck <= gz; ck++)
-      for (int cj = gy - 1; cj <= gy; cj++)
-        for (int ci = gx - 1; ci <= gx; ci++)
-        {
-          long long b = d.boff + (ci - d.ts[0]) + (long long)(cj - d.ts[1]) * rX + (long long)(ck - d.ts[2]) * rXY;
-          int a = bstart[b], e = bend[b];
-          if (a < e && box[b] < 0)
-            acc += cell_corner_sum(tails, a, e, (gx - ci) + 2 * (gy - cj) + 4 * (gz - ck));
-        }
-    out[d.goff + gl] = acc;
+    const long long b0 = d.boff + (gx - 1 - d.ts[0]) + (long long)(gy - 1 - d.ts[1]) * rX +
+                         (long long)(gz - 1 - d.ts[2]) * rXY;
+    int a[8], e[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+    {
+      const long long b = b0 + (u & 1) + ((u >> 1) & 1) * (long long)rX + (u >> 2) * (long long)rXY;
+      const int s0 = bstart[b], s1 = bend[b], st = box[b];
+      a[u] = st < 0 ? s0 : 0; // refined cells: their particles gather from the child
+      e[u] = st < 0 ? s1 : 0;
+    }
+    out[d.goff + gl] = fold8(tails, a, e);
   }
 }
 
@@ -250,17 +251,19 @@ __global__ void k_adj_head_scatter(const int *__restrict__ start, const int *__r
     return;
   int i, j, k;
   cube_ijk(cb, t, i, j, k);
-  double acc = 0.0;
-  for (int ck = max(k - 1, 0); ck <= min(k, n - 1); ck++)
-    for (int cj = max(j - 1, 0); cj <= min(j, n - 1); cj++)
-      for (int ci = max(i - 1, 0); ci <= min(i, n - 1); ci++)
-      {
-        int c = ci + n * (cj + n * ck);
-        int a = start[c], e = start[c + 1];
-        if (a < e && head_child[c] < 0)
-          acc += cell_corner_sum(tails, a, e, (i - ci) + 2 * (j - cj) + 4 * (k - ck));
-      }
-  out[t] = (G)acc;
+  int a[8], e[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+  {
+    const int ci = i - 1 + (u & 1), cj = j - 1 + ((u >> 1) & 1), ck = k - 1 + (u >> 2);
+    const bool in = ci >= 0 && ci < n && cj >= 0 && cj < n && ck >= 0 && ck < n;
+    const int c = in ? ci + n * (cj + n * ck) : 0;
+    const int s0 = start[c], s1 = start[c + 1], hc = head_child[c];
+    const bool ok = in && hc < 0;
+    a[u] = ok ? s0 : 0;
+    e[u] = ok ? s1 : 0;
+  }
+  out[t] = (G)fold8(tails, a, e);
 }
 
 // phibar += transposed interior central difference of Abar at the boundary and

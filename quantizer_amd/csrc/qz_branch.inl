@@ -235,7 +235,13 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
         !hip_ok(hipMalloc(&d.hsel_tmp, std::max<size_t>(d.hsel_bytes, 1)), "head marks"))
       return false;
   }
-  QZ_LAUNCHB("tree_head_mark", 5.0 * ncell, k_head_mark, nblk(ncell), cell_start, n, cfg.ref_criterion_ptcl, d.hmark);
+  // flags, then the x, y, z dilations (the head list is free scratch until the
+  // selection below fills it)
+  unsigned char *tmp = (unsigned char *)d.hlist;
+  QZ_LAUNCHB("tree_head_mark", 5.0 * ncell, k_head_flag, nblk(ncell), cell_start, ncell, cfg.ref_criterion_ptcl, tmp);
+  QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), tmp, d.hmark, cfg.lmin, 0);
+  QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), d.hmark, tmp, cfg.lmin, 1);
+  QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), tmp, d.hmark, cfg.lmin, 2);
   timer.begin(stream);
   if (!hip_ok(qz_select_flagged_index(d.hsel_tmp, d.hsel_bytes, d.hmark, d.hlist, d.flag_count, (int)ncell, stream),
               "head marks"))

@@ -389,27 +389,32 @@ __global__ void k_flag_cells(const DNode *__restrict__ nodes, const int *__restr
 // ------------------------------------------------------------------ head level marks and zones
 // fill_cell_ref for the head (tree_construction.c:527-575, n_exp = 1): a head
 // cell is marked when some cell of its 3x3x3 block inside the box holds at
-// least crit particles.
-__global__ void k_head_mark(const int *__restrict__ start, int n, int crit, unsigned char *__restrict__ mark)
+// least crit particles.  The block is a product of three intervals, so the mark
+// is a flag per cell followed by three one-axis dilations (k_head_dilate).
+__global__ void k_head_flag(const int *__restrict__ start, long long ncell, int crit, unsigned char *__restrict__ f)
 {
-  const long long ncell = (long long)n * n * n;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncell)
+    f[c] = start[c + 1] - start[c] >= crit ? 1 : 0;
+}
+
+// out = in OR its two neighbours along `axis` (inside the n^3 box of cells,
+// n = 2^lg).
+__global__ void k_head_dilate(const unsigned char *__restrict__ in, unsigned char *__restrict__ out, int lg,
+                              int axis)
+{
+  const long long n = 1LL << lg, ncell = n * n * n;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell)
     return;
-  const int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((long long)n * n));
-  unsigned char m = 0;
-  for (int kk = max(k - 1, 0); kk <= min(k + 1, n - 1) && !m; kk++)
-    for (int jj = max(j - 1, 0); jj <= min(j + 1, n - 1) && !m; jj++)
-    {
-      const long long row = (long long)n * (jj + (long long)n * kk);
-      for (int ii = max(i - 1, 0); ii <= min(i + 1, n - 1); ii++)
-        if (start[row + ii + 1] - start[row + ii] >= crit)
-        {
-          m = 1;
-          break;
-        }
-    }
-  mark[c] = m;
+  const int x = (int)((c >> (lg * axis)) & (n - 1));
+  const long long st = 1LL << (lg * axis);
+  unsigned char m = in[c];
+  if (x > 0)
+    m |= in[c - st];
+  if (x < n - 1)
+    m |= in[c + st];
+  out[c] = m;
 }
 
 // map[cell] = position of the marked cell in the ordered list; every list entry
@@ -1024,17 +1029,19 @@ __global__ void k_branch_deposit_fold(const DNode *__restrict__ nodes, int first
       xyz = intr_xyz;
     }
     int gx = xyz[3 * sl], gy = xyz[3 * sl + 1], gz = xyz[3 * sl + 2];
-    double acc = 0.0;
-    for (int ck = gz - 1; ck <= gz; ck++)
-      for (int cj = gy - 1; cj <= gy; cj++)
-        for (int ci = gx - 1; ci <= gx; ci++)
-        {
-          long long b = d.boff + (ci - d.ts[0]) + (long long)(cj - d.ts[1]) * rX + (long long)(ck - d.ts[2]) * rXY;
-          int a = bstart[b], e = bend[b];
-          if (a < e)
-            acc += cell_corner_sum(tails, a, e, (gx - ci) + 2 * (gy - cj) + 4 * (gz - ck));
-        }
-    rho[d.goff + gl] = (G)(coeff * acc);
+    // the 8 box cells around the point (all inside the box: node cells keep a
+    // padding of empty box cells)
+    const long long b0 = d.boff + (gx - 1 - d.ts[0]) + (long long)(gy - 1 - d.ts[1]) * rX +
+                         (long long)(gz - 1 - d.ts[2]) * rXY;
+    int a[8], e[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+    {
+      const long long b = b0 + (u & 1) + ((u >> 1) & 1) * (long long)rX + (u >> 2) * (long long)rXY;
+      a[u] = bstart[b];
+      e[u] = bend[b];
+    }
+    rho[d.goff + gl] = (G)(coeff * fold8(tails, a, e));
   }
 }
 

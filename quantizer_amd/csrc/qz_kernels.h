@@ -370,6 +370,40 @@ template <typename S> __device__ inline double cell_corner_sum(const S *__restri
   return acc;
 }
 
+// A grid point's fold over its 8 adjacent cells u = (ci - i + 1) + 2 (cj - j + 1)
+// + 4 (ck - k + 1), slot ranges [a[u], e[u]) (empty for cells that do not
+// count), each contributing its corner 7 - u.  The first 64-slot window of
+// every cell is loaded together (one memory round trip for the common case of
+// short cells); longer cells continue window by window.  Same sums, same order
+// as cell_corner_sum over the cells in u order.  Needs at least 8 tails.
+template <typename S> __device__ inline double fold8(const S *__restrict__ tails, const int (&a)[8], const int (&e)[8])
+{
+  S f[8];
+  int nxt[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+  {
+    const int tl = max(min(e[u], ((a[u] >> 6) + 1) << 6) - 1, 0);
+    nxt[u] = tl + 1;
+    f[u] = tails[8 * (size_t)tl + (7 - u)];
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+    if (a[u] < e[u])
+    {
+      double cs = (double)f[u];
+      for (int t = nxt[u]; t < e[u];)
+      {
+        const int tl = min(e[u], ((t >> 6) + 1) << 6) - 1;
+        cs += (double)tails[8 * (size_t)tl + (7 - u)];
+        t = tl + 1;
+      }
+      acc += cs;
+    }
+  return acc;
+}
+
 // A2 head deposit, stage 2: each grid point adds corner (i-ci, j-cj, k-ck) of
 // its 8 adjacent cells; coeff = 4 pi G / H^3 (grid_density.c:62).
 template <typename S, typename G>
@@ -382,27 +416,18 @@ __global__ void k_deposit_head_fold(const int *__restrict__ start, const S *__re
     return;
   int i, j, k;
   cube_ijk(cb, t, i, j, k);
-  double acc = 0.0;
-  for (int ck = k - 1; ck <= k; ck++)
+  int a[8], e[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++)
   {
-    if (ck < 0 || ck >= n)
-      continue;
-    for (int cj = j - 1; cj <= j; cj++)
-    {
-      if (cj < 0 || cj >= n)
-        continue;
-      for (int ci = i - 1; ci <= i; ci++)
-      {
-        if (ci < 0 || ci >= n)
-          continue;
-        int c = ci + n * (cj + n * ck);
-        int a = start[c], e = start[c + 1];
-        if (a < e)
-          acc += cell_corner_sum(tails, a, e, (i - ci) + 2 * (j - cj) + 4 * (k - ck));
-      }
-    }
+    const int ci = i - 1 + (u & 1), cj = j - 1 + ((u >> 1) & 1), ck = k - 1 + (u >> 2);
+    const bool ok = ci >= 0 && ci < n && cj >= 0 && cj < n && ck >= 0 && ck < n;
+    const int c = ok ? ci + n * (cj + n * ck) : 0;
+    const int s0 = start[c], s1 = start[c + 1];
+    a[u] = ok ? s0 : 0;
+    e[u] = ok ? s1 : 0;
   }
-  rho[t] = (G)(coeff * acc);
+  rho[t] = (G)(coeff * fold8(tails, a, e));
 }
 
 // A12 head gather (particle_acceleration.c:33-132).
@@ -642,19 +667,37 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
   // block walks one contiguous chunk (XCD-contiguous, see xcd_block)
   const unsigned chunk = ((npts + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
   const unsigned t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(npts, t0 + chunk);
-  for (unsigned t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+  // four points per pass, loads of all four first (face points load from a
+  // fixed interior point and are masked), so a thread's passes are not one
+  // memory round trip per point; the same points in the same order per thread
+  constexpr int U = 4;
+  const unsigned safe = s2 + s1 + 1;
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += U * blockDim.x)
   {
-    int i, j, k;
-    cube_ijk(cb, t, i, j, k);
-    double res = 0.0;
-    if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+    double res[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
     {
-      double c = phi[t];
-      res = (double)rho[t] + inv_h2 * (6.0 * c - phi[t + 1] - phi[t - 1] - phi[t + s1] - phi[t - s1] -
-                                       phi[t + s2] - phi[t - s2]);
-      sq += res * res;
+      const unsigned tt = t + u * blockDim.x;
+      int i, j, k;
+      cube_ijk(cb, tt, i, j, k);
+      in[u] = tt < t1 && i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
+      const unsigned q = in[u] ? tt : safe;
+      const double c = phi[q];
+      const double v = (double)rho[q] + inv_h2 * (6.0 * c - phi[q + 1] - phi[q - 1] - phi[q + s1] - phi[q - s1] -
+                                                  phi[q + s2] - phi[q - s2]);
+      res[u] = in[u] ? v : 0.0;
     }
-    r[(t & 1) * hf + (t >> 1)] = (G)res;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+      const unsigned tt = t + u * blockDim.x;
+      if (in[u])
+        sq += res[u] * res[u];
+      if (tt < t1)
+        r[(tt & 1) * hf + (tt >> 1)] = (G)res[u];
+    }
   }
   sq = block_reduce<0>(sq);
   if (threadIdx.x == 0)
