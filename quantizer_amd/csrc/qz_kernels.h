@@ -667,37 +667,19 @@ __global__ void k_outer_residual(const double *__restrict__ phi, const G *__rest
   // block walks one contiguous chunk (XCD-contiguous, see xcd_block)
   const unsigned chunk = ((npts + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
   const unsigned t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(npts, t0 + chunk);
-  // four points per pass, loads of all four first (face points load from a
-  // fixed interior point and are masked), so a thread's passes are not one
-  // memory round trip per point; the same points in the same order per thread
-  constexpr int U = 4;
-  const unsigned safe = s2 + s1 + 1;
-  for (unsigned t = t0 + threadIdx.x; t < t1; t += U * blockDim.x)
+  for (unsigned t = t0 + threadIdx.x; t < t1; t += blockDim.x)
   {
-    double res[U];
-    bool in[U];
-#pragma unroll
-    for (int u = 0; u < U; u++)
+    int i, j, k;
+    cube_ijk(cb, t, i, j, k);
+    double res = 0.0;
+    if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
     {
-      const unsigned tt = t + u * blockDim.x;
-      int i, j, k;
-      cube_ijk(cb, tt, i, j, k);
-      in[u] = tt < t1 && i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
-      const unsigned q = in[u] ? tt : safe;
-      const double c = phi[q];
-      const double v = (double)rho[q] + inv_h2 * (6.0 * c - phi[q + 1] - phi[q - 1] - phi[q + s1] - phi[q - s1] -
-                                                  phi[q + s2] - phi[q - s2]);
-      res[u] = in[u] ? v : 0.0;
+      double c = phi[t];
+      res = (double)rho[t] + inv_h2 * (6.0 * c - phi[t + 1] - phi[t - 1] - phi[t + s1] - phi[t - s1] -
+                                       phi[t + s2] - phi[t - s2]);
+      sq += res * res;
     }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-    {
-      const unsigned tt = t + u * blockDim.x;
-      if (in[u])
-        sq += res[u] * res[u];
-      if (tt < t1)
-        r[(tt & 1) * hf + (tt >> 1)] = (G)res[u];
-    }
+    r[(t & 1) * hf + (t >> 1)] = (G)res;
   }
   sq = block_reduce<0>(sq);
   if (threadIdx.x == 0)
