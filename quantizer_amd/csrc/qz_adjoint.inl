@@ -97,6 +97,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
     QZ_LAUNCHB("adj_slot", 12.0 * N, k_by_id_to_slot<P>, nblk(N), ab + (size_t)a * N, id_a, adj_slot, N);
     QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
                pos[1], pos[2], adj_slot, N, cfg.lmin, 3 * D, n - 1, tails);
+    cell_totals(cell_start, nullptr, (long long)n * n * n);
     QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>),
                nblk(npts), cell_start, tails, adj_gbar[a], make_cube(n + 1), 1.0);
   }
@@ -215,6 +216,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
     QZ_LAUNCHB("adj_slot", 12.0 * N, k_by_id_to_slot<P>, nblk(N), ab + (size_t)a * N, id_a, adj_slot, N);
     QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
                pos[1], pos[2], adj_slot, N, cfg.lmin, 3 * D, n - 1, tails);
+    cell_totals(cell_start, nullptr, (long long)n * n * n);
     QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_adj_head_scatter<G, G>),
                nblk(npts), cell_start, d.head_child, tails, adj_gbar[a], cb);
     for (int l = 1; l < nl; l++)
@@ -222,6 +224,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
       const int lv = cfg.lmin + l;
       QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
                  pos[1], pos[2], adj_slot, N, lv, 3 * (D - l), (1 << lv) - 1, tails);
+      cell_totals(d.cstart + d.lc0[l], d.cend + d.lc0[l], d.lncell[l]);
       dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
       QZ_LAUNCH("adj_scatter_branch", (k_adj_branch_scatter<G>), grid, d.nodes, d.lfirst[l], d.bdry_g, d.bdry_xyz,
                 d.intr_g, d.intr_xyz, d.box, d.bstart, d.bend, tails, adj_nab[a]);

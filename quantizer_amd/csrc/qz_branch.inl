@@ -611,6 +611,7 @@ template <typename P, typename G> bool Engine<P, G>::branch_density()
     const int lv = cfg.lmin + (int)l;
     QZ_LAUNCHB("deposit_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0], pos[1],
                pos[2], pa[9], N, lv, 3 * (D - (int)l), (1 << lv) - 1, tails);
+    cell_totals(d.cstart + d.lc0[l], d.cend + d.lc0[l], d.lncell[l]);
     dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
     timer.begin(stream);
     hipLaunchKernelGGL((k_branch_deposit_fold<G, G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,

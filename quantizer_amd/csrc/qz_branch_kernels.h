@@ -160,7 +160,7 @@ __global__ void k_expand_cells(const DNode *__restrict__ nodes, const int *__res
 // Per-node particle count and mass (tree_construction.c:2020-2022): x blocks
 // of each node stride over its cells; a cell's mass is the sum of its 8 CIC
 // corner sums (the weights of a particle add up to 1), read from the deposit
-// runs of this level.  Per-block partials folded in order.
+// runs of this level after k_cell_totals.  Per-block partials folded in order.
 template <typename S>
 __global__ void k_node_mass(const DNode *__restrict__ nodes, int first, const int *__restrict__ cstart,
                             const int *__restrict__ cend, const S *__restrict__ tails, double *__restrict__ part)
@@ -172,8 +172,12 @@ __global__ void k_node_mass(const DNode *__restrict__ nodes, int first, const in
     int a = cstart[d.coff + c], e = cend[d.coff + c];
     cnt += (double)(e - a);
     if (a < e)
-      for (int q = 0; q < 8; q++)
-        mass += cell_corner_sum(tails, a, e, q);
+    {
+      // the cell's 8 corner sums: the tail at its last slot (k_cell_totals)
+      const S *w = tails + 8 * (size_t)(e - 1);
+      mass += (((double)w[0] + (double)w[1]) + ((double)w[2] + (double)w[3])) +
+              (((double)w[4] + (double)w[5]) + ((double)w[6] + (double)w[7]));
+    }
   }
   mass = block_reduce<0>(mass);
   cnt = block_reduce<0>(cnt);

@@ -141,6 +141,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   int *slot_in = nullptr, *slot_out = nullptr;
   int *cell_start = nullptr;
   G *tails = nullptr; // 8 corner sums per run tail (deposit stage 1)
+  int *heavy = nullptr, *nheavy = nullptr; // cells of many windows (k_cell_totals) and their count
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   // head grid
@@ -283,6 +284,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     (void)hipFree(slot_out);
     (void)hipFree(cell_start);
     (void)hipFree(tails);
+    (void)hipFree(heavy);
+    (void)hipFree(nheavy);
     (void)hipFree(sort_tmp);
     (void)hipFree(rho);
     (void)hipFree(phi);
@@ -339,6 +342,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     slot_out = dmalloc<int>(N, bytes);
     cell_start = dmalloc<int>((size_t)n * n * n + 1, bytes);
     tails = dmalloc<G>(8 * (size_t)N, bytes);
+    heavy = dmalloc<int>((size_t)N / (64 * (kTotalsSerial - 1)) + 16, bytes);
+    nheavy = dmalloc<int>(1, bytes);
     end_bit = 3 * (cfg.lmin + D);
     sort_tmp_bytes = end_bit <= 32 ? qz_sort_temp_bytes(N, end_bit) : qz_sort64_temp_bytes(N, end_bit);
     sort_tmp = dmalloc<char>(sort_tmp_bytes, bytes);
@@ -347,8 +352,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     gax = dmalloc<G>(npts, bytes);
     gay = dmalloc<G>(npts, bytes);
     gaz = dmalloc<G>(npts, bytes);
-    if (!id_a || !id_b || !key_in || !key_out || !slot_in || !slot_out || !cell_start || !tails || !sort_tmp || !rho || !phi ||
-        !gax || !gay || !gaz)
+    if (!id_a || !id_b || !key_in || !key_out || !slot_in || !slot_out || !cell_start || !tails || !heavy || !nheavy ||
+        !sort_tmp || !rho || !phi || !gax || !gay || !gaz)
       return hip_ok(hipErrorOutOfMemory, "head arrays");
     // multigrid hierarchy: level 0 = head (n cells), halve down to 4 cells
     for (int m = n; m >= 4; m /= 2)
@@ -475,6 +480,17 @@ template <typename P, typename G> struct Engine final : EngineBase
     return hip_ok(hipGetLastError(), "reset"); // stream-ordered; readers synchronise
   }
 
+  // Window tails of the cells [cb, ce) (ce null: cb[c + 1]) summed into each
+  // cell's last slot, after k_deposit_runs and before any fold reads them.
+  void cell_totals(const int *cb, const int *ce, long long ncell)
+  {
+    if (ncell <= 0)
+      return;
+    QZ_LAUNCH("cell_totals", k_fill<int>, 1, nheavy, 0, (size_t)1);
+    QZ_LAUNCH("cell_totals", k_cell_totals<G>, nblk((size_t)ncell), cb, ce, ncell, tails, heavy, nheavy);
+    QZ_LAUNCH("cell_totals", k_cell_totals_heavy<G>, 1024, cb, ce, heavy, nheavy, tails);
+  }
+
   // Sort particles by octree key and rebuild the head cell ranges.
   bool sort_particles()
   {
@@ -536,6 +552,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     double coeff = 4.0 * cfg.G * kPi / (H * H * H);
     QZ_LAUNCHB("deposit_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0], pos[1],
                pos[2], pa[9], N, cfg.lmin, 3 * D, n - 1, tails);
+    cell_totals(cell_start, nullptr, (long long)n * n * n);
     QZ_LAUNCHB("deposit_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>), nblk(npts),
                cell_start, tails, rho, make_cube(n + 1), coeff);
     if (has_tree() && !branch_density())

@@ -357,50 +357,88 @@ __global__ void k_deposit_runs(const unsigned long long *__restrict__ key, const
 
 // Sum of corner q of the cell whose sorted slot range is [a, e): its runs end at
 // min(e, next 64-slot boundary) - 1.
-template <typename S> __device__ inline double cell_corner_sum(const S *__restrict__ tails, int a, int e, int q)
+// Cells whose slots span several 64-slot windows hold one run tail per window.
+// k_cell_totals / k_cell_totals_heavy replace the tail at such a cell's last
+// slot with the sum of all its window tails (windows in order, FP64 sums), so
+// every cell's 8 corner sums are then the tail at its last slot: a fold reads
+// one tail per cell.  No other run reads that slot.  Cells of up to
+// kTotalsSerial windows are summed by one thread; longer ones are listed and
+// summed by a whole workgroup (fixed reduction order).  Deterministic.
+constexpr int kTotalsSerial = 8;
+
+template <typename S>
+__global__ void k_cell_totals(const int *__restrict__ cb, const int *__restrict__ ce, long long ncell,
+                              S *__restrict__ tails, int *__restrict__ heavy, int *__restrict__ nheavy)
 {
-  double acc = 0.0;
-  int t = a;
-  while (t < e)
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell)
+    return;
+  const int a = cb[c], e = ce ? ce[c] : cb[c + 1];
+  if (e - a <= 0 || (a >> 6) == ((e - 1) >> 6))
+    return;
+  if (((e - 1) >> 6) - (a >> 6) + 1 > kTotalsSerial)
   {
-    int tl = min(e, ((t >> 6) + 1) << 6) - 1;
-    acc += (double)tails[8 * (size_t)tl + q];
+    heavy[atomicAdd(nheavy, 1)] = (int)c; // the list holds at most N / (64 (kTotalsSerial - 1)) + 1 cells
+    return;
+  }
+  double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = a; t < e;)
+  {
+    const int tl = min(e, ((t >> 6) + 1) << 6) - 1;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      sum[q] += (double)tails[8 * (size_t)tl + q];
     t = tl + 1;
   }
-  return acc;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    tails[8 * (size_t)(e - 1) + q] = (S)sum[q];
+}
+
+template <typename S>
+__global__ void k_cell_totals_heavy(const int *__restrict__ cb, const int *__restrict__ ce,
+                                    const int *__restrict__ heavy, const int *__restrict__ nheavy,
+                                    S *__restrict__ tails)
+{
+  const int nh = *nheavy;
+  for (int h = blockIdx.x; h < nh; h += gridDim.x)
+  {
+    const int c = heavy[h];
+    const int a = cb[c], e = ce ? ce[c] : cb[c + 1];
+    double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int w = (a >> 6) + (int)threadIdx.x; w <= ((e - 1) >> 6); w += blockDim.x)
+    {
+      const int tl = min(e, (w + 1) << 6) - 1;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        sum[q] += (double)tails[8 * (size_t)tl + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      sum[q] = block_reduce<0>(sum[q]); // ends with a barrier: every window is read before the write
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        tails[8 * (size_t)(e - 1) + q] = (S)sum[q];
+    __syncthreads();
+  }
 }
 
 // A grid point's fold over its 8 adjacent cells u = (ci - i + 1) + 2 (cj - j + 1)
 // + 4 (ck - k + 1), slot ranges [a[u], e[u]) (empty for cells that do not
-// count), each contributing its corner 7 - u.  The first 64-slot window of
-// every cell is loaded together (one memory round trip for the common case of
-// short cells); longer cells continue window by window.  Same sums, same order
-// as cell_corner_sum over the cells in u order.  Needs at least 8 tails.
+// count), each contributing its corner 7 - u from the tail at its last slot
+// (k_cell_totals has run).  The 8 loads issue together.  Needs N >= 1.
 template <typename S> __device__ inline double fold8(const S *__restrict__ tails, const int (&a)[8], const int (&e)[8])
 {
   S f[8];
-  int nxt[8];
 #pragma unroll
   for (int u = 0; u < 8; u++)
-  {
-    const int tl = max(min(e[u], ((a[u] >> 6) + 1) << 6) - 1, 0);
-    nxt[u] = tl + 1;
-    f[u] = tails[8 * (size_t)tl + (7 - u)];
-  }
+    f[u] = tails[8 * (size_t)max(e[u] - 1, 0) + (7 - u)];
   double acc = 0.0;
 #pragma unroll
   for (int u = 0; u < 8; u++)
     if (a[u] < e[u])
-    {
-      double cs = (double)f[u];
-      for (int t = nxt[u]; t < e[u];)
-      {
-        const int tl = min(e[u], ((t >> 6) + 1) << 6) - 1;
-        cs += (double)tails[8 * (size_t)tl + (7 - u)];
-        t = tl + 1;
-      }
-      acc += cs;
-    }
+      acc += (double)f[u];
   return acc;
 }
 
