@@ -51,7 +51,8 @@ struct TreeDev
   double *mpart = nullptr;
   long long mpart_cap = 0;
   int *done_count = nullptr;
-  int *hflag_count = nullptr; // pinned
+  int *hflag_count = nullptr; // pinned, 4 ints
+  hipEvent_t done_ev[2] = {nullptr, nullptr}; // pipelined batch loop
   std::vector<DNode> hnodes;
   // per relative level (index 1..): node range and pool ranges
   std::vector<int> lfirst, lcount;
@@ -151,6 +152,9 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
       (void)hipFree(q);
     if (d->hflag_count)
       (void)hipHostFree(d->hflag_count);
+    for (hipEvent_t e : d->done_ev)
+      if (e)
+        (void)hipEventDestroy(e);
     delete d;
   }
   delete tree->host;
@@ -541,25 +545,32 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.flag_count, 0, (size_t)1);
       QZ_LAUNCH("tree_flags", k_flag_cells, nblk(ncell), d.nodes, d.zone_node, first, (int)c0, (int)ncell, d.cstart,
                 d.cend, cfg.ref_criterion_ptcl, d.flag_count, d.flags, d.flag_cap);
-      if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "flags") ||
-          !hip_ok(hipStreamSynchronize(stream), "flags"))
-        return false;
-      int nf = std::min(*d.hflag_count, d.flag_cap);
-      std::vector<int> fl(2 * (size_t)nf);
-      if (nf > 0 && !hip_ok(hipMemcpy(fl.data(), d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost), "flags"))
-        return false;
-      std::vector<std::pair<int, int>> pr(nf);
-      for (int i = 0; i < nf; i++)
-        pr[i] = {fl[2 * i], fl[2 * i + 1]};
-      std::sort(pr.begin(), pr.end());
-      std::vector<int> fn(nf), fc(nf);
-      for (int i = 0; i < nf; i++)
+      std::vector<int> fn, fc;
       {
-        fn[i] = pr[i].first;
-        fc[i] = pr[i].second;
+        HostSpan span_f(timer, "tree_flags");
+        if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "flags") ||
+            !hip_ok(hipStreamSynchronize(stream), "flags"))
+          return false;
+        int nf = std::min(*d.hflag_count, d.flag_cap);
+        std::vector<int> fl(2 * (size_t)nf);
+        if (nf > 0 && !hip_ok(hipMemcpy(fl.data(), d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost), "flags"))
+          return false;
+        fn.resize(nf);
+        fc.resize(nf);
+        for (int i = 0; i < nf; i++)
+        {
+          fn[i] = fl[2 * i];
+          fc[i] = fl[2 * i + 1];
+        }
       }
       HostSpan span(timer, "tree_refine_level");
       th.refine_level(lr, fn, fc);
+      if (timer.on || timer.host_on)
+      {
+        timer.host("tree_refine_mark", th.ms_mark);
+        timer.host("tree_refine_zones", th.ms_zones);
+        timer.host("tree_refine_children", th.ms_children);
+      }
     }
     lr++;
   }
@@ -694,18 +705,33 @@ bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int cou
     (void)hipGraphDestroy(g);
   }
   all_done = false;
-  bool ok = true;
-  for (it = 0; it < max_it; it += per_batch)
-  {
+  // Pipelined: batch k + 1 is queued before the done count after batch k is
+  // read, so the device does not idle while the host looks.  A batch queued
+  // after convergence finds every node done and returns at once.
+  if (!d.done_ev[0] && (!hip_ok(hipEventCreateWithFlags(&d.done_ev[0], hipEventDisableTiming), "event") ||
+                        !hip_ok(hipEventCreateWithFlags(&d.done_ev[1], hipEventDisableTiming), "event")))
+    return false;
+  int *hc = d.hflag_count + 2; // two pinned slots, one per batch in flight
+  auto launch = [&](int slot) -> bool {
+    bool r = true;
     if (gexec)
-      ok = hip_ok(hipGraphLaunch(gexec, stream), "graph launch");
+      r = hip_ok(hipGraphLaunch(gexec, stream), "graph launch");
     else
       batch();
-    ok = ok && hip_ok(hipMemcpyAsync(d.hflag_count, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "done") &&
-         hip_ok(hipStreamSynchronize(stream), "done");
-    if (!ok)
-      break;
-    if (*d.hflag_count >= count)
+    return r && hip_ok(hipMemcpyAsync(hc + slot, d.done_count, sizeof(int), hipMemcpyDeviceToHost, stream), "done") &&
+           hip_ok(hipEventRecord(d.done_ev[slot], stream), "done");
+  };
+  bool ok = max_it > 0 && launch(0);
+  int launched = max_it > 0 ? per_batch : 0, slot = 0;
+  for (it = 0; ok && it < max_it; it += per_batch, slot ^= 1)
+  {
+    if (launched < max_it)
+    {
+      ok = launch(slot ^ 1);
+      launched += per_batch;
+    }
+    ok = ok && hip_ok(hipEventSynchronize(d.done_ev[slot]), "done");
+    if (ok && hc[slot] >= count)
     {
       all_done = true;
       it += per_batch;
@@ -714,7 +740,7 @@ bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int cou
   }
   if (gexec)
     (void)hipGraphExecDestroy(gexec);
-  return ok;
+  return ok && hip_ok(hipStreamSynchronize(stream), "iterate");
 }
 
 // Unpreconditioned CG per node, all nodes of the level batched

@@ -10,6 +10,7 @@
 #include "qz_tree.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 namespace qz
@@ -226,37 +227,56 @@ int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std:
 {
   int start = lev_start[lr], cnt = lev_count[lr];
   int next_first = (int)nodes.size();
-  size_t f = 0;
+  using clk = std::chrono::steady_clock;
+  auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  ms_mark = ms_zones = ms_children = 0.0;
+  // flags grouped by node (any order within a node: marking is a set operation)
+  const clk::time_point tg = clk::now();
+  std::vector<int> fbeg(cnt + 1, 0), fcell(flag_cell.size());
+  for (size_t f = 0; f < flag_node.size(); f++)
+    fbeg[flag_node[f] + 1]++;
+  for (int q = 0; q < cnt; q++)
+    fbeg[q + 1] += fbeg[q];
+  {
+    std::vector<int> at(fbeg.begin(), fbeg.end() - 1);
+    for (size_t f = 0; f < flag_node.size(); f++)
+      fcell[at[flag_node[f]]++] = flag_cell[f];
+  }
+  ms_mark += ms(tg, clk::now());
+  std::vector<unsigned char> mark;
   for (int q = 0; q < cnt; q++)
   {
+    if (fbeg[q] == fbeg[q + 1])
+      continue;
+    const clk::time_point t0 = clk::now();
     int g = start + q;
     HNode &nd = nodes[g];
     int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
-    std::vector<int> marked_pos;
-    while (f < flag_node.size() && flag_node[f] == q)
+    // fill_cell_ref with n_exp = 1: each flagged cell marks the existing cells
+    // of its 3x3x3 block (box status -3 -> -1); marks per cell position, read
+    // back in position order (the reference's cell_ref order)
+    mark.assign(nd.ncell, 0);
+    for (int f = fbeg[q]; f < fbeg[q + 1]; f++)
     {
-      int c = flag_cell[f++];
-      int b = nd.cbox[c];
-      if (nd.box[b] == -3)
-      {
-        nd.box[b] = -1;
-        marked_pos.push_back(c);
-      }
+      const int b = nd.cbox[fcell[f]];
       for (int kk = -1; kk <= 1; kk++)
         for (int jj = -1; jj <= 1; jj++)
           for (int ii = -1; ii <= 1; ii++)
           {
-            int nb = b + ii + jj * rX + kk * rXY;
+            const int nb = b + ii + jj * rX + kk * rXY;
             if (nd.box[nb] == -3)
             {
               nd.box[nb] = -1;
-              marked_pos.push_back(nd.pos_of_box[nb]);
+              mark[nd.pos_of_box[nb]] = 1;
             }
           }
     }
+    std::vector<int> marked_pos;
+    for (int c = 0; c < nd.ncell; c++)
+      if (mark[c])
+        marked_pos.push_back(c);
     if (marked_pos.empty())
       continue;
-    std::sort(marked_pos.begin(), marked_pos.end());
     std::vector<int> ref_box(marked_pos.size());
     std::vector<int> pc(3 * marked_pos.size());
     for (size_t t = 0; t < marked_pos.size(); t++)
@@ -267,13 +287,18 @@ int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std:
     }
     std::vector<int> zone_sizes;
     std::vector<std::vector<int>> zones;
+    const clk::time_point t1 = clk::now();
     zone_fill(nd.box, rX, rXY, ref_box, zone_sizes, zones);
+    const clk::time_point t2 = clk::now();
     int lv = nd.lv;
     int first = make_children(g, lv, pc.data(), zone_sizes, zones);
     HNode &nd2 = nodes[g]; // nodes may have been reallocated
     nd2.child.clear();
     for (int z = 0; z < (int)zone_sizes.size(); z++)
       nd2.child.push_back(first + z);
+    ms_mark += ms(t0, t1);
+    ms_zones += ms(t1, t2);
+    ms_children += ms(t2, clk::now());
   }
   int made = (int)nodes.size() - next_first;
   if (made > 0)

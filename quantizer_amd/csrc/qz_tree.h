@@ -38,6 +38,7 @@ struct HNode
 struct TreeHost
 {
   int lmin = 0, lmax = 0, n = 0, ref_crit = 8, extra = 10;
+  double ms_mark = 0.0, ms_zones = 0.0, ms_children = 0.0; // wall time of refine_level's parts (last call)
   std::vector<HNode> nodes;     // refinement nodes, level order (tentacles order, tree_construction.c:2465-2495)
   std::vector<int> lev_start, lev_count; // relative level 1.. -> ranges in nodes
 
@@ -47,7 +48,7 @@ struct TreeHost
   // cell coordinates of every zone in zone order (needed only when the
   // children can refine further, else null).  Returns the level-1 node count.
   int add_head_zones(int nz, const int *bbox6, const int *zsize, const int *cells);
-  // Level lr >= 1: flagged cells as (node position in level, cell position), sorted.
+  // Level lr >= 1: flagged cells as (node position in level, cell position), any order.
   int refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell);
   void clear();
 
