@@ -8,12 +8,10 @@
 //   abar_j = h vbar_j + abarA;  xbar_j += J_A(x_j)^T abar_j
 //   ybar = s xbar_j;  ubar = s vbar_j + dt ybar;  xbar <- ybar, vbar <- ubar, abarA <- h ubar
 // and finally xbar_0 += J_A(x_0)^T abarA.  Forward states are recomputed from
-// checkpoints every `every` steps.  Head grid only: within a segment the sorted
-// positions, keys, head cell ranges, grid accelerations and reflection flags of
-// each step are kept, so the reverse sweep needs no forward Poisson solve.  With
-// refinement levels the positions, IDs and reflection flags are kept and the
-// reverse sweep redoes the force pass (tree, density, potentials, grid
-// accelerations) of each step before its VJP.
+// checkpoints every `every` steps.  Within a segment the sorted positions, keys,
+// head cell ranges, grid accelerations and reflection flags of each step are
+// kept, with refinement levels also the tree and node accelerations, so the
+// reverse sweep needs no forward solve.
 #include "qz_adjoint_kernels.h"
 
 namespace qz
@@ -52,6 +50,7 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
     (void)hipFree(r.key);
     (void)hipFree(r.cstart);
     (void)hipFree(r.refl);
+    free_tree_snap(r.tsnap);
   }
   recs.clear();
 }
@@ -388,12 +387,12 @@ template <typename P, typename G> void Engine<P, G>::free_ckpt(Ckpt &c)
   c = Ckpt{};
 }
 
-// Per-step records for one segment: positions and IDs in sorted order and the
-// reflected axes of the step; head grid only also keys, head cell ranges and
-// grid accelerations (with levels the force pass is redone instead).
+// Per-step records for one segment: positions, IDs and keys in sorted order,
+// head cell ranges, head grid accelerations and the reflected axes of the step;
+// with refinement levels also the tree and node fields (TreeSnap), so the
+// reverse sweep needs no forward solve.
 template <typename P, typename G> bool Engine<P, G>::ensure_recs()
 {
-  const bool amr = cfg.lmin < cfg.lmax;
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
   while ((int)recs.size() < adj_every + 1)
   {
@@ -402,14 +401,14 @@ template <typename P, typename G> bool Engine<P, G>::ensure_recs()
     for (int a = 0; a < 3; a++)
     {
       r.x[a] = dmalloc<double>(N, bb);
-      r.g[a] = amr ? nullptr : dmalloc<G>(npts, bb);
+      r.g[a] = dmalloc<G>(npts, bb);
     }
     r.id = dmalloc<int>(N, bb);
-    r.key = amr ? nullptr : dmalloc<unsigned long long>(N, bb);
-    r.cstart = amr ? nullptr : dmalloc<int>(ncell + 1, bb);
+    r.key = dmalloc<unsigned long long>(N, bb);
+    r.cstart = dmalloc<int>(ncell + 1, bb);
     r.refl = dmalloc<unsigned char>(N, bb);
-    const bool ok = r.x[0] && r.x[1] && r.x[2] && r.id && r.refl &&
-                    (amr || (r.g[0] && r.g[1] && r.g[2] && r.key && r.cstart));
+    const bool ok =
+        r.x[0] && r.x[1] && r.x[2] && r.id && r.refl && r.g[0] && r.g[1] && r.g[2] && r.key && r.cstart;
     recs.push_back(r);
     if (!ok)
       return hip_ok(hipErrorOutOfMemory, "adjoint step records");
@@ -418,23 +417,23 @@ template <typename P, typename G> bool Engine<P, G>::ensure_recs()
 }
 
 // Record the sorted state after a force pass.
-template <typename P, typename G> void Engine<P, G>::save_rec(StepRec &r)
+template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r)
 {
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
   for (int a = 0; a < 3; a++)
     (void)hipMemcpyAsync(r.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(r.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
-  if (cfg.lmin < cfg.lmax)
-    return;
   (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
   G *src[3] = {gax, gay, gaz};
   for (int a = 0; a < 3; a++)
     (void)hipMemcpyAsync(r.g[a], src[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
+  if (has_tree())
+    return snap_tree(r.tsnap);
+  return true;
 }
 
-// Make a recorded state current for its VJP (with levels: masses to the
-// recorded order, then the force pass).
+// Make a recorded state current for its VJP.
 template <typename P, typename G> bool Engine<P, G>::load_rec(const StepRec &r)
 {
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
@@ -443,9 +442,13 @@ template <typename P, typename G> bool Engine<P, G>::load_rec(const StepRec &r)
   (void)hipMemcpyAsync(id_a, r.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
   if (cfg.lmin < cfg.lmax)
   {
-    QZ_LAUNCH("adj_mass", k_id_to_slot<P>, nblk(N), adj_mid, id_a, pa[9], N);
-    keys_valid = false;
-    return force_fields();
+    if (!r.tsnap)
+    {
+      set_error("adjoint: a step record holds no tree");
+      return false;
+    }
+    if (!restore_tree(r.tsnap))
+      return false;
   }
   (void)hipMemcpyAsync(key_out, r.key, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(cell_start, r.cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
@@ -489,8 +492,8 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
       if (!ok)
         return false;
     }
-    if (s == 0 && last == 0)
-      save_rec(recs[0]); // the force pass of state 0 is current
+    if (s == 0 && last == 0 && !save_rec(recs[0])) // the force pass of state 0 is current
+      return false;
     if (s == nsteps)
       break;
     const bool rec = s + 1 > last;
@@ -499,8 +502,8 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
     refl_rec = nullptr;
     if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt))
       return false;
-    if (rec)
-      save_rec(recs[s + 1 - last]);
+    if (rec && !save_rec(recs[s + 1 - last]))
+      return false;
   }
   adj_last_recorded = true;
   return hip_ok(hipStreamSynchronize(stream), "adjoint_forward");
@@ -534,7 +537,6 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     return false;
   adj_sweep = true;
   adj_warm = false;
-  const bool amr = cfg.lmin < cfg.lmax;
   if (!ensure_recs())
     return false;
   if (!hip_ok(hipMemcpy(adj_xb, xbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload xbar") ||
@@ -555,24 +557,21 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       restore_ckpt(ckpts[sc]);
       if (sc == 0)
       {
-        // head grid: the record of state 0 holds its fields; the force pass
+        // the record of state 0 holds its fields (and tree); the force pass
         // clears the accelerations, which the checkpoint then restores exactly
-        if (!amr)
-        {
-          if (!force_fields())
-            return false;
-          restore_ckpt(ckpts[sc]);
-        }
-        save_rec(recs[0]);
+        if (!force_fields())
+          return false;
+        restore_ckpt(ckpts[sc]);
+        if (!save_rec(recs[0]))
+          return false;
       }
       for (int j = c0 + 1; j <= c1; j++)
       {
         refl_rec = recs[j - c0].refl;
         bool ok = update_A(dt);
         refl_rec = nullptr;
-        if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt))
+        if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt) || !save_rec(recs[j - c0]))
           return false;
-        save_rec(recs[j - c0]);
       }
     }
     // reverse sweep over the segment

@@ -87,6 +87,36 @@ struct TreeDev
   long long used_zone = 0, used_cell = 0, used_box = 0, used_pt = 0, used_grid = 0;
 };
 
+// A copy of everything the adjoint of a force pass reads from the tree: the node
+// table, head children, box statuses and slot ranges, cell ranges, grid-point
+// lists, node grid accelerations, and the host-side level tables.  Device parts
+// are grow-only buffers, so a record is refilled without reallocation.
+struct TreeSnap
+{
+  struct Buf
+  {
+    void *p = nullptr;
+    size_t cap = 0;
+  };
+  Buf nodes, head_child, box, bstart, bend, cstart, cend, intr_g, bdry_g, sim_g, intr_xyz, bdry_xyz, sim_xyz, ax, ay, az;
+  int n_nodes = 0;
+  long long used_box = 0, used_cell = 0, used_pt = 0, used_grid = 0;
+  std::vector<DNode> hnodes;
+  std::vector<int> lfirst, lcount, lnbm;
+  std::vector<long long> lc0, lncell, lb0, lnb, lz0, lnz, lg0, lng, lpmax, lnintr;
+  void release()
+  {
+    Buf *all[] = {&nodes, &head_child, &box, &bstart, &bend, &cstart, &cend, &intr_g,
+                  &bdry_g, &sim_g, &intr_xyz, &bdry_xyz, &sim_xyz, &ax, &ay, &az};
+    for (Buf *b : all)
+    {
+      (void)hipFree(b->p);
+      b->p = nullptr;
+      b->cap = 0;
+    }
+  }
+};
+
 template <typename T> static bool regrow(T *&p, long long newcap, long long used, const char *what)
 {
   T *np = nullptr;
@@ -890,7 +920,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   double ptsm[kMaxMgLevels] = {0.0};   // points over nodes at MG level m
   for (int q = 0; q < count; q++)
   {
-    const HNode &hn = tree->host->nodes[first + q];
+    const DNode &hn = d.hnodes[first + q]; // box dimensions (the device table's copy)
     int M = 0, c[3] = {0, 0, 0};
     for (int t = 1; t <= kMaxMgLevels && !M; t++)
     {
@@ -1239,6 +1269,112 @@ template <typename P, typename G> bool Engine<P, G>::timestep(double cfl, double
     return false;
   *dt = -hred[0];
   return true;
+}
+
+
+// Copy the tree state of the current force pass into `t` (allocates only when
+// it grows).
+template <typename P, typename G> bool Engine<P, G>::snap_tree(TreeSnap *&t)
+{
+  if (!t)
+    t = new TreeSnap();
+  TreeDev &d = *tree->dev;
+  const size_t ncell_head = (size_t)n * n * n, gs = sizeof(G);
+  auto put = [&](TreeSnap::Buf &b, const void *src, size_t bytes) -> bool {
+    if (bytes > b.cap)
+    {
+      (void)hipFree(b.p);
+      b.p = nullptr;
+      b.cap = 0;
+      if (!hip_ok(hipMalloc(&b.p, std::max<size_t>(bytes + bytes / 4, 256)), "tree record"))
+        return false;
+      b.cap = bytes + bytes / 4;
+    }
+    return bytes == 0 || hip_ok(hipMemcpyAsync(b.p, src, bytes, hipMemcpyDeviceToDevice, stream), "tree record");
+  };
+  const size_t nb = (size_t)d.used_box, nc = (size_t)d.used_cell, np = (size_t)d.used_pt, ng = (size_t)d.used_grid;
+  t->n_nodes = tree->n_nodes;
+  t->used_box = d.used_box;
+  t->used_cell = d.used_cell;
+  t->used_pt = d.used_pt;
+  t->used_grid = d.used_grid;
+  t->hnodes = d.hnodes;
+  t->lfirst = d.lfirst;
+  t->lcount = d.lcount;
+  t->lnbm = d.lnbm;
+  t->lc0 = d.lc0;
+  t->lncell = d.lncell;
+  t->lb0 = d.lb0;
+  t->lnb = d.lnb;
+  t->lz0 = d.lz0;
+  t->lnz = d.lnz;
+  t->lg0 = d.lg0;
+  t->lng = d.lng;
+  t->lpmax = d.lpmax;
+  t->lnintr = d.lnintr;
+  return put(t->nodes, d.nodes, sizeof(DNode) * (size_t)tree->n_nodes) &&
+         put(t->head_child, d.head_child, sizeof(int) * ncell_head) && put(t->box, d.box, sizeof(int) * nb) &&
+         put(t->bstart, d.bstart, sizeof(int) * nb) && put(t->bend, d.bend, sizeof(int) * nb) &&
+         put(t->cstart, d.cstart, sizeof(int) * nc) && put(t->cend, d.cend, sizeof(int) * nc) &&
+         put(t->intr_g, d.intr_g, sizeof(int) * np) && put(t->bdry_g, d.bdry_g, sizeof(int) * np) &&
+         put(t->sim_g, d.sim_g, sizeof(int) * np) && put(t->intr_xyz, d.intr_xyz, 3 * sizeof(int) * np) &&
+         put(t->bdry_xyz, d.bdry_xyz, 3 * sizeof(int) * np) && put(t->sim_xyz, d.sim_xyz, 3 * sizeof(int) * np) &&
+         put(t->ax, d.ax, gs * ng) && put(t->ay, d.ay, gs * ng) && put(t->az, d.az, gs * ng);
+}
+
+// Make a recorded tree current (into the live grow-only pools).
+template <typename P, typename G> bool Engine<P, G>::restore_tree(const TreeSnap *t)
+{
+  TreeDev &d = *tree->dev;
+  if (t->n_nodes > d.node_cap || t->used_box > d.box_cap || t->used_cell > d.cell_cap || t->used_pt > d.pt_cap ||
+      t->used_grid > d.grid_cap)
+  {
+    set_error("restore_tree: a recorded tree exceeds the live pools");
+    return false;
+  }
+  const size_t ncell_head = (size_t)n * n * n, gs = sizeof(G);
+  auto get = [&](void *dst, const TreeSnap::Buf &b, size_t bytes) -> bool {
+    return bytes == 0 || hip_ok(hipMemcpyAsync(dst, b.p, bytes, hipMemcpyDeviceToDevice, stream), "tree record");
+  };
+  const size_t nb = (size_t)t->used_box, nc = (size_t)t->used_cell, np = (size_t)t->used_pt, ng = (size_t)t->used_grid;
+  if (!get(d.nodes, t->nodes, sizeof(DNode) * (size_t)t->n_nodes) ||
+      !get(d.head_child, t->head_child, sizeof(int) * ncell_head) || !get(d.box, t->box, sizeof(int) * nb) ||
+      !get(d.bstart, t->bstart, sizeof(int) * nb) || !get(d.bend, t->bend, sizeof(int) * nb) ||
+      !get(d.cstart, t->cstart, sizeof(int) * nc) || !get(d.cend, t->cend, sizeof(int) * nc) ||
+      !get(d.intr_g, t->intr_g, sizeof(int) * np) || !get(d.bdry_g, t->bdry_g, sizeof(int) * np) ||
+      !get(d.sim_g, t->sim_g, sizeof(int) * np) || !get(d.intr_xyz, t->intr_xyz, 3 * sizeof(int) * np) ||
+      !get(d.bdry_xyz, t->bdry_xyz, 3 * sizeof(int) * np) || !get(d.sim_xyz, t->sim_xyz, 3 * sizeof(int) * np) ||
+      !get(d.ax, t->ax, gs * ng) || !get(d.ay, t->ay, gs * ng) || !get(d.az, t->az, gs * ng))
+    return false;
+  tree->n_nodes = t->n_nodes;
+  d.used_box = t->used_box;
+  d.used_cell = t->used_cell;
+  d.used_pt = t->used_pt;
+  d.used_grid = t->used_grid;
+  d.hnodes = t->hnodes;
+  d.lfirst = t->lfirst;
+  d.lcount = t->lcount;
+  d.lnbm = t->lnbm;
+  d.lc0 = t->lc0;
+  d.lncell = t->lncell;
+  d.lb0 = t->lb0;
+  d.lnb = t->lnb;
+  d.lz0 = t->lz0;
+  d.lnz = t->lnz;
+  d.lg0 = t->lg0;
+  d.lng = t->lng;
+  d.lpmax = t->lpmax;
+  d.lnintr = t->lnintr;
+  return true;
+}
+
+template <typename P, typename G> void Engine<P, G>::free_tree_snap(TreeSnap *&t)
+{
+  if (!t)
+    return;
+  t->release();
+  delete t;
+  t = nullptr;
 }
 
 } // namespace qz

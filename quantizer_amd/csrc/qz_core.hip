@@ -197,6 +197,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     int *cstart;
     G *g[3];
     unsigned char *refl;
+    TreeSnap *tsnap; // with refinement levels: the tree and node fields of the step
   };
   std::vector<Ckpt> ckpts;
   std::vector<StepRec> recs;
@@ -902,7 +903,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   void restore_ckpt(const Ckpt &c);
   void free_ckpt(Ckpt &c);
   bool ensure_recs();
-  void save_rec(StepRec &r);
+  bool snap_tree(TreeSnap *&t);
+  bool restore_tree(const TreeSnap *t);
+  void free_tree_snap(TreeSnap *&t);
+  bool save_rec(StepRec &r);
   bool load_rec(const StepRec &r);
   bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
   bool force_vjp(const double *abar, double *xbar) override;
