@@ -1075,7 +1075,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   auto RS = [&](int m) { return (const Chunk *)d.chunks + rs0[m]; };
 
   const double ni = (double)d.lnintr[l], gs = (double)sizeof(G);
-  const int inner = std::max(1, cfg.inner_cycles);
+  const int inner = 1; // one V-cycle per PCG iteration (a second one costs more than the iterations it saves)
   const int pre = std::max(1, cfg.pre_sweeps), post = std::max(0, cfg.post_sweeps);
   int *it_base = d.done_count + 1;
   auto sweep = [&](int m, int color, int zs) {

@@ -1009,7 +1009,7 @@ qz_ctx *qz_create(const qz_config *cfg)
   }
   qz_config c = *cfg;
   if (c.inner_cycles <= 0)
-    c.inner_cycles = 1;
+    c.inner_cycles = 2; // two V-cycles per FP64 residual/correction: half the outer passes, same V-cycle count
   if (c.pre_sweeps <= 0)
     c.pre_sweeps = 2;
   if (c.post_sweeps <= 0)
