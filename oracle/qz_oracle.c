@@ -1384,6 +1384,81 @@ int oq_step(oq_sim *s, double dt)
 }
 
 /* ------------------------------------------------------------------------ */
+/* 8(f).2 CFL time step, time_step_method = 1 (timestep.c:73-195, 370-587,      */
+/* 589-699): per particle and axis the dt that moves it CFL * H of its finest    */
+/* node under constant velocity and acceleration, min over all, capped at      */
+/* dt_max.                                                                     */
+/* ------------------------------------------------------------------------ */
+static double cfl_axis(double v, double a, double H, double cfl, double dt)
+{
+  double av = fabs(v), aa = fabs(a), t;
+  if (aa < 1.0e-16)
+  {
+    if (av > 1.0e-16)
+    {
+      t = cfl * H / av; /* timestep.c:86-93 */
+      dt = dt < t ? dt : t;
+    }
+    return dt;
+  }
+  double c = v * v + 2.0 * aa * H * cfl; /* :96-106 */
+  if (1.0 - v * v / c > 1.0e-8)
+    t = (-av + sqrt(c)) / aa;
+  else
+  {
+    double eps = fabs(a * H * cfl / (v * v));
+    t = fabs(v / a) * (eps - eps * eps * 0.5);
+  }
+  return dt < t ? dt : t;
+}
+
+double oq_cfl_dt(int n, const double *v3, const double *a3, const double *H, double cfl, double dt_max)
+{
+  double dt = dt_max;
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++)
+      dt = cfl_axis(v3[(size_t)k * n + i], a3[(size_t)k * n + i], H[i], cfl, dt);
+  return dt;
+}
+
+double oq_timestep(const oq_sim *s, double cfl, double dt_max)
+{
+  double *H = (double *)malloc(sizeof(double) * (size_t)s->n);
+  if (s->lmin == s->lmax)
+    for (int i = 0; i < s->n; i++)
+      H[i] = 1.0 / (1 << s->lmin); /* timestep.c:73-195 */
+  else
+    for (int q = 0; q < s->nnodes; q++) /* :370-587: unrefined cells of every node */
+    {
+      const onode *nd = &s->nodes[q];
+      for (int c = 0; c < nd->ncell; c++)
+      {
+        int b = nd->cbox[c];
+        if (nd->nchild > 0 && nd->box[b] >= 0)
+          continue;
+        for (int t = nd->cell_start[b]; t < nd->cell_start[b + 1]; t++)
+          H[nd->cell_list[t]] = 1.0 / (1 << nd->lv);
+      }
+    }
+  double *v3 = (double *)malloc(sizeof(double) * 3 * (size_t)s->n), *a3 = v3 ? (double *)malloc(sizeof(double) * 3 * (size_t)s->n) : NULL;
+  double dt = dt_max;
+  if (v3 && a3)
+  {
+    memcpy(v3, s->vx, sizeof(double) * (size_t)s->n);
+    memcpy(v3 + s->n, s->vy, sizeof(double) * (size_t)s->n);
+    memcpy(v3 + 2 * (size_t)s->n, s->vz, sizeof(double) * (size_t)s->n);
+    memcpy(a3, s->ax, sizeof(double) * (size_t)s->n);
+    memcpy(a3 + s->n, s->ay, sizeof(double) * (size_t)s->n);
+    memcpy(a3 + 2 * (size_t)s->n, s->az, sizeof(double) * (size_t)s->n);
+    dt = oq_cfl_dt(s->n, v3, a3, H, cfl, dt_max);
+  }
+  free(v3);
+  free(a3);
+  free(H);
+  return dt;
+}
+
+/* ------------------------------------------------------------------------ */
 /* A16 observables.c:38-112, 622-659                                          */
 /* ------------------------------------------------------------------------ */
 void oq_observables(const oq_sim *s, double *out6)

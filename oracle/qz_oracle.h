@@ -74,6 +74,14 @@ int oq_head_solve(int lmin, double *phi, const double *rho, double node_mass, do
  * oq_force_vjp: xbar += J_A(x)^T abar for the force pass at the current positions.
  * oq_adjoint: from the current state (force pass done) run nsteps fixed-dt
  * steps and return the cotangents of the initial x, v from those of the final. */
+/*
+ * oq_cfl_dt: min over particles i and axes k of the CFL time step of velocity
+ * v3[k*n+i] and acceleration a3[k*n+i] with cell size H[i], capped at dt_max.
+ * oq_timestep: the same over the sim's particles, H of each particle's finest
+ * node (needs a force pass).
+ */
+double oq_cfl_dt(int n, const double *v3, const double *a3, const double *H, double cfl, double dt_max);
+double oq_timestep(const oq_sim *s, double cfl, double dt_max);
 int oq_force_vjp(oq_sim *s, const double *abar, double *xbar);
 int oq_adjoint(oq_sim *s, int nsteps, double dt, const double *xbN, const double *vbN, double *xb0, double *vb0);
 

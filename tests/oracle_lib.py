@@ -58,6 +58,10 @@ def _lib():
     lib.oq_head_solve.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     lib.oq_head_solve.restype = ctypes.c_int
     lib.oq_set_tolerance.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.oq_cfl_dt.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double]
+    lib.oq_cfl_dt.restype = ctypes.c_double
+    lib.oq_timestep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    lib.oq_timestep.restype = ctypes.c_double
     lib.oq_force_vjp.argtypes = [ctypes.c_void_p, _dp, _dp]
     lib.oq_force_vjp.restype = ctypes.c_int
     lib.oq_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
@@ -150,6 +154,9 @@ class OracleSim:
         out = np.empty(6)
         lib().oq_observables(self.h, _d(out))
         return out
+
+    def timestep(self, cfl=0.5, dt_max=1e300):
+        return float(lib().oq_timestep(self.h, cfl, dt_max))
 
     def force_vjp(self, abar):
         a = np.ascontiguousarray(abar, dtype=np.float64).reshape(-1)
@@ -265,3 +272,11 @@ def run_reference(parts, outdir, lmin, lmax, ref_crit, nsteps, dt, extra=(), bin
     if p.returncode != 0:
         raise RuntimeError(f"reference harness failed: {p.stderr}")
     return p.stderr
+
+
+def cfl_dt(v3, a3, H, cfl=0.5, dt_max=1e300):
+    """The oracle's CFL rule on explicit arrays (3 x n velocities and accelerations, n cell sizes)."""
+    v = np.ascontiguousarray(v3, dtype=np.float64).reshape(-1)
+    a = np.ascontiguousarray(a3, dtype=np.float64).reshape(-1)
+    h = np.ascontiguousarray(H, dtype=np.float64)
+    return float(lib().oq_cfl_dt(h.shape[0], _d(v), _d(a), _d(h), cfl, dt_max))

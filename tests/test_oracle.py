@@ -311,3 +311,59 @@ def test_oracle_amr_reverse_mode_matches_central_differences(oracle):
               _ke_loss(oracle, minus, lmin, lmax, nsteps, dt, weight, tol)[0]) / (2 * eps)
         g = (gx if arr in "xyz" else gv)[comp, i]
         assert abs(fd - g) <= 2e-4 * abs(g) + 1e-14, (arr, i, fd, g)
+
+
+def test_cfl_rule_closed_forms(oracle):
+    """The CFL rule (timestep.c:73-195) against its closed forms: velocity only
+    (dt = CFL H / |v|), acceleration only (dt = sqrt(2 CFL H / |a|)), both (the
+    root of |v| t + |a| t^2 / 2 = CFL H), the small-|a| expansion branch (within
+    its second-order error of the root), the minimum over particles and axes,
+    and the cap."""
+    cfl, H = 0.5, 1.0 / 64
+    z = np.zeros((3, 1))
+
+    def one(v, a, h=H, dt_max=1e300):
+        vv, aa = z.copy(), z.copy()
+        vv[0, 0], aa[0, 0] = v, a
+        return oracle.cfl_dt(vv, aa, [h], cfl, dt_max)
+
+    assert one(2.0, 0.0) == pytest.approx(cfl * H / 2.0, rel=1e-15)
+    assert one(-2.0, 0.0) == pytest.approx(cfl * H / 2.0, rel=1e-15)
+    assert one(0.0, 3.0) == pytest.approx(np.sqrt(2 * cfl * H / 3.0), rel=1e-14)
+    for v, a in [(0.3, 5.0), (-1.0, 0.2), (0.01, -40.0)]:
+        t = one(v, a)
+        assert abs(v) * t + 0.5 * abs(a) * t * t == pytest.approx(cfl * H, rel=1e-12)
+    v, a = 1.0e3, 1.0e-3  # |a| H CFL / v^2 ~ 1e-11: expansion branch
+    t = one(v, a)
+    eps = a * H * cfl / v ** 2
+    root = (-v + np.sqrt(v * v + 2 * a * H * cfl)) / a  # cancellation-prone; the expansion is the accurate one
+    assert t == pytest.approx(abs(v / a) * eps * (1 - 0.5 * eps), rel=1e-15)
+    assert t == pytest.approx(root, rel=1e-4)
+    assert one(0.0, 0.0) == 1e300
+    assert one(1e-3, 0.0, dt_max=1.0) == 1.0
+    # minimum over particles and axes with their own cell sizes
+    rng = np.random.default_rng(2)
+    v3, a3 = rng.normal(size=(3, 50)), rng.normal(size=(3, 50))
+    hs = 1.0 / 2.0 ** rng.integers(4, 9, 50)
+    each = []
+    for i in range(50):
+        for k in range(3):
+            vv, aa = z.copy(), z.copy()
+            vv[k, 0], aa[k, 0] = v3[k, i], a3[k, i]
+            each.append(oracle.cfl_dt(vv, aa, [hs[i]], cfl))
+    assert oracle.cfl_dt(v3, a3, hs, cfl) == min(each)
+
+
+def test_oracle_timestep_uses_finest_node(oracle):
+    """With refinement levels each particle uses the cell size of its finest
+    node: the sim's step equals the explicit rule with those sizes, and is
+    smaller than the head-only rule on the same state."""
+    from quantizer_amd.workloads import plummer
+    p = plummer(6000, seed=21)
+    o = oracle.OracleSim(p, 4, 6)
+    o.force()
+    st = o.particles()
+    dt = o.timestep(0.5)
+    head = oracle.cfl_dt(st[3:6], st[6:9], np.full(p.n, 1.0 / 16), 0.5)
+    finest = oracle.cfl_dt(st[3:6], st[6:9], np.full(p.n, 1.0 / 64), 0.5)
+    assert finest <= dt < head
