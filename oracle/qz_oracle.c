@@ -1384,6 +1384,52 @@ int oq_step(oq_sim *s, double dt)
 }
 
 /* ------------------------------------------------------------------------ */
+/* A16 grid potential energy (potential_energy_type = 1, observables.c:114-619): */
+/* 1/2 G sum m_i phi(x_i), phi interpolated with CIC weights at each particle's  */
+/* finest node (the head alone without levels).                                 */
+/* ------------------------------------------------------------------------ */
+static double pot_one(const oq_sim *s, const onode *nd, int p)
+{
+  int lv = nd->lv;
+  double px = s->x[p] * (1 << lv), py = s->y[p] * (1 << lv), pz = s->z[p] * (1 << lv);
+  int fx = (int)px, fy = (int)py, fz = (int)pz;
+  double wx1 = px - fx, wy1 = py - fy, wz1 = pz - fz;
+  double w[2][3] = {{1.0 - wx1, 1.0 - wy1, 1.0 - wz1}, {wx1, wy1, wz1}};
+  int X = gdx(nd), XY = gdxy(nd);
+  int g = (fx - nd->ts[0]) + (fy - nd->ts[1]) * X + (fz - nd->ts[2]) * XY;
+  double v = 0.0;
+  for (int kk = 0; kk < 2; kk++)
+    for (int jj = 0; jj < 2; jj++)
+      for (int ii = 0; ii < 2; ii++)
+        v += nd->pot[g + ii + jj * X + kk * XY] * w[ii][0] * w[jj][1] * w[kk][2];
+  return v;
+}
+
+double oq_grid_pe(const oq_sim *s)
+{
+  double e = 0.0;
+  if (s->lmin == s->lmax)
+  {
+    for (int p = 0; p < s->n; p++)
+      e += s->m[p] * pot_one(s, &s->nodes[0], p);
+  }
+  else
+    for (int q = 0; q < s->nnodes; q++)
+    {
+      const onode *nd = &s->nodes[q];
+      for (int c = 0; c < nd->ncell; c++)
+      {
+        int b = nd->cbox[c];
+        if (nd->nchild > 0 && nd->box[b] >= 0)
+          continue;
+        for (int t = nd->cell_start[b]; t < nd->cell_start[b + 1]; t++)
+          e += s->m[nd->cell_list[t]] * pot_one(s, nd, nd->cell_list[t]);
+      }
+    }
+  return 0.5 * s->G * e; /* observables.c:617 */
+}
+
+/* ------------------------------------------------------------------------ */
 /* 8(f).2 CFL time step, time_step_method = 1 (timestep.c:73-195, 370-587,      */
 /* 589-699): per particle and axis the dt that moves it CFL * H of its finest    */
 /* node under constant velocity and acceleration, min over all, capped at      */

@@ -82,6 +82,8 @@ int oq_head_solve(int lmin, double *phi, const double *rho, double node_mass, do
  */
 double oq_cfl_dt(int n, const double *v3, const double *a3, const double *H, double cfl, double dt_max);
 double oq_timestep(const oq_sim *s, double cfl, double dt_max);
+/* 1/2 G sum m phi at each particle's finest node (needs a force pass) */
+double oq_grid_pe(const oq_sim *s);
 int oq_force_vjp(oq_sim *s, const double *abar, double *xbar);
 int oq_adjoint(oq_sim *s, int nsteps, double dt, const double *xbN, const double *vbN, double *xb0, double *vb0);
 

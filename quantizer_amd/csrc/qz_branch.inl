@@ -1173,6 +1173,13 @@ template <typename P, typename G> bool Engine<P, G>::branch_particle_acceleratio
   return true;
 }
 
+template <typename P, typename G> void Engine<P, G>::branch_grid_pe(unsigned nb)
+{
+  TreeDev &d = *tree->dev;
+  QZ_LAUNCH("grid_pe", k_grid_pe_amr<P>, nb, pos[0], pos[1], pos[2], pa[9], phi, d.head_child, d.nodes, d.box, d.phi,
+            N, cfg.lmin, part);
+}
+
 template <typename P, typename G> bool Engine<P, G>::node_header(int i, int *h)
 {
   if (!has_tree() || i < 0 || i >= tree->n_nodes)

@@ -1155,6 +1155,60 @@ __device__ inline int finest_node(double px0, double py0, double pz0, const int 
   return found;
 }
 
+// Grid potential energy with refinement (observables.c:114-591): partial sums
+// of m phi(x), phi interpolated at the finest node holding each particle.
+template <typename P>
+__global__ void k_grid_pe_amr(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, const P *__restrict__ m, const double *__restrict__ hphi,
+                              const int *__restrict__ head_child, const DNode *__restrict__ nodes,
+                              const int *__restrict__ box, const double *__restrict__ bphi, int N, int lmin,
+                              double *__restrict__ part)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (p < N)
+  {
+    const int n = 1 << lmin;
+    const double px0 = x[p], py0 = y[p], pz0 = z[p];
+    int lv = lmin;
+    const int q = finest_node(px0, py0, pz0, head_child, nodes, box, lmin, lv);
+    const double s = (double)(1 << lv);
+    const double pxs = px0 * s, pys = py0 * s, pzs = pz0 * s;
+    int fx = (int)pxs, fy = (int)pys, fz = (int)pzs;
+    const double *f;
+    long long X, XY, g;
+    if (q < 0)
+    {
+      fx = min(fx, n - 1);
+      fy = min(fy, n - 1);
+      fz = min(fz, n - 1);
+      f = hphi;
+      X = n + 1;
+      XY = X * X;
+      g = fx + fy * X + fz * XY;
+    }
+    else
+    {
+      const DNode &d = nodes[q];
+      f = bphi + d.goff;
+      X = d.rd[0] + 1;
+      XY = X * (d.rd[1] + 1);
+      g = (fx - d.ts[0]) + (fy - d.ts[1]) * X + (fz - d.ts[2]) * XY;
+    }
+    const double wx1 = pxs - fx, wy1 = pys - fy, wz1 = pzs - fz;
+    const double wx[2] = {1.0 - wx1, wx1}, wy[2] = {1.0 - wy1, wy1}, wz[2] = {1.0 - wz1, wz1};
+    for (int kk = 0; kk < 2; kk++)
+      for (int jj = 0; jj < 2; jj++)
+        for (int ii = 0; ii < 2; ii++)
+          acc += f[g + ii + jj * X + kk * XY] * wx[ii] * wy[jj] * wz[kk];
+    acc *= (double)m[p];
+  }
+  acc = block_reduce<0>(acc);
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = acc;
+}
+
+
 // CFL time step with velocity and acceleration (timestep.c:73-195, 370-587):
 // per axis the dt that moves a particle CFL * H of its finest node, min over
 // particles and axes; per-block minima for a deterministic second stage.

@@ -816,6 +816,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     double ek = 0.5 * hred[0], px = hred[1], py = hred[2], pz = hred[3];
     if (exact_pe)
       QZ_LAUNCH("pairwise_pe", k_pairwise_pe<P>, nb, pos[0], pos[1], pos[2], pa[9], N, part);
+    else if (has_tree())
+      branch_grid_pe(nb);
     else
       QZ_LAUNCH("grid_pe", k_grid_pe_head<P>, nb, pos[0], pos[1], pos[2], pa[9], phi, N, n, part);
     hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
@@ -925,6 +927,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool branch_density();
   bool branch_grid_acceleration();
   bool branch_particle_acceleration();
+  void branch_grid_pe(unsigned nb); // partial sums of m phi at the finest node into part
   void release_tree();
   int num_nodes() override { return tree ? tree->n_nodes : 0; }
   bool node_header(int i, int *hdr) override;

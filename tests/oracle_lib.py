@@ -62,6 +62,8 @@ def _lib():
     lib.oq_cfl_dt.restype = ctypes.c_double
     lib.oq_timestep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     lib.oq_timestep.restype = ctypes.c_double
+    lib.oq_grid_pe.argtypes = [ctypes.c_void_p]
+    lib.oq_grid_pe.restype = ctypes.c_double
     lib.oq_force_vjp.argtypes = [ctypes.c_void_p, _dp, _dp]
     lib.oq_force_vjp.restype = ctypes.c_int
     lib.oq_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
@@ -157,6 +159,9 @@ class OracleSim:
 
     def timestep(self, cfl=0.5, dt_max=1e300):
         return float(lib().oq_timestep(self.h, cfl, dt_max))
+
+    def grid_pe(self):
+        return float(lib().oq_grid_pe(self.h))
 
     def force_vjp(self, abar):
         a = np.ascontiguousarray(abar, dtype=np.float64).reshape(-1)

@@ -1,6 +1,7 @@
-"""Integer keys (ptcl_idx_to_box_idx) and the CFL time step on the GPU against
-the oracle: keys bit-exact, including points on cell faces, below 0 and at or
-above 1; time steps to the accuracy of the accelerations they use."""
+"""Integer keys (ptcl_idx_to_box_idx), the CFL time step and the energy and
+momentum diagnostics on the GPU against the oracle: keys bit-exact, including
+points on cell faces, below 0 and at or above 1; time steps and grid energies to
+the accuracy of the fields they use; FP64 sums to round-off."""
 import numpy as np
 import pytest
 
@@ -41,3 +42,22 @@ def test_timestep_matches_oracle(oracle, levels):
         capped = s.timestep(0.5, dt_max=0.1 * want)
     assert got == pytest.approx(want, rel=1e-8)
     assert capped == 0.1 * want
+
+
+@pytest.mark.parametrize("levels", [(5, 5), (4, 6)])
+def test_observables_match_oracle(oracle, levels):
+    lmin, lmax = levels
+    print(f"case: observables levels={levels} two spheres N=3000", flush=True)
+    p = two_plummer(1500, seed=35)
+    o = oracle.OracleSim(p, lmin, lmax)
+    o.set_tolerance(1e-12)
+    o.force()
+    want = o.observables()
+    want_grid = o.grid_pe()
+    with core.Simulation(p, lmin, lmax, precision=core.PREC_FP64, tol=1e-12) as s:
+        s.force()
+        got = s.observables(exact_pe=True)
+        got_grid = s.observables(exact_pe=False)
+    np.testing.assert_allclose(got[:3], want[:3], rtol=1e-12)
+    np.testing.assert_allclose(got[3:], want[3:], rtol=1e-12, atol=1e-14)
+    assert got_grid[1] == pytest.approx(want_grid, rel=1e-8)

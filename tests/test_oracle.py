@@ -367,3 +367,28 @@ def test_oracle_timestep_uses_finest_node(oracle):
     head = oracle.cfl_dt(st[3:6], st[6:9], np.full(p.n, 1.0 / 16), 0.5)
     finest = oracle.cfl_dt(st[3:6], st[6:9], np.full(p.n, 1.0 / 64), 0.5)
     assert finest <= dt < head
+
+
+def test_grid_potential_energy_exact_on_linear_potential(oracle):
+    """observables.c:114-619 (potential_energy_type = 1): with a head potential
+    linear in x, y, z the CIC interpolation is exact, so 1/2 G sum m phi(x_i) is
+    known in closed form; with refinement levels the finest-node value is closer
+    to the exact pairwise energy than the head-only one."""
+    from quantizer_amd.workloads import plummer
+    p = plummer(3000, seed=41)
+    lmin = 5
+    n1 = (1 << lmin) + 1
+    o = oracle.OracleSim(p, lmin, lmin)
+    o.force()
+    g = np.arange(n1) / (n1 - 1.0)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    pot = 0.3 - 1.5 * x + 0.7 * y + 2.0 * z
+    o.set_head_pot(pot)
+    want = 0.5 * np.sum(p.m * (0.3 - 1.5 * p.x + 0.7 * p.y + 2.0 * p.z))  # G = 1
+    assert o.grid_pe() == pytest.approx(want, rel=1e-13)
+    # refinement: the finest-node energy approaches the pairwise one
+    head, amr = oracle.OracleSim(p, 4, 4), oracle.OracleSim(p, 4, 7)
+    head.force()
+    amr.force()
+    exact = amr.observables()[1]
+    assert abs(amr.grid_pe() - exact) < abs(head.grid_pe() - exact)
