@@ -80,3 +80,30 @@ def test_clustered_particles_cell_ranges(oracle):
     assert rel(hg[0], ho[0]) < 1e-12
     for k in (6, 7, 8):
         assert rel(pg[k], o.particles()[k]) < 1e-6
+
+
+@pytest.mark.parametrize("prec,tol_x,tol_drift", [(core.PREC_FP64, 1e-8, 1e-6), (core.PREC_MIXED, 1e-4, 1e-4)])
+def test_plummer_32_zero_boundary_energy_drift(oracle, prec, tol_x, tol_drift):
+    """The small plain-method configuration: Plummer N = 10^4, 32^3 head grid,
+    zero-potential faces, dt = 0.01 (Plummer units), 100 steps.  Positions
+    against the oracle, and the relative energy drift E(t)/E(0) - 1 (exact
+    pairwise potential energy) matching the oracle's."""
+    print(f"case: Plummer 1e4 32^3 zero faces 100 steps prec={prec}", flush=True)
+    p = plummer(10000)
+    lmin, nsteps, dt = 5, 100, plummer_dt(0.01)
+    o = oracle.OracleSim(p, lmin, lmin, head_bc=core.BC_ZERO)
+    o.force()
+    e0_ref = o.observables()[2]
+    for _ in range(nsteps):
+        o.step(dt)
+    drift_ref = o.observables()[2] / e0_ref - 1.0
+    x_ref = o.particles()[:3]
+    with core.Simulation(p, lmin, lmin, precision=prec, head_bc=core.BC_ZERO) as s:
+        s.force()
+        e0 = s.observables()[2]
+        for _ in range(nsteps):
+            s.step(dt)
+        drift = s.observables()[2] / e0 - 1.0
+        x = s.particles()[:3]
+    assert np.max(np.abs(x - x_ref)) < tol_x
+    assert abs(drift - drift_ref) < tol_drift, (drift, drift_ref)
