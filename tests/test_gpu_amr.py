@@ -71,15 +71,15 @@ def test_amr_geometry_and_fields_fp64(oracle, kind, n, lmin, lmax, crit, solver)
             assert np.array_equal(a[kind_ + "_xyz"], b[kind_ + "_xyz"]), kind_
         idx = np.concatenate([listed_index(b, k) for k in ("intr", "bdry", "sim")])
         assert rel(a["rho"][idx], b["rho"][idx]) < 1e-12
-        assert rel(a["pot"][idx], b["pot"][idx]) < 1e-8
+        assert rel(a["pot"][idx], b["pot"][idx]) < 1e-10
         for f in ("ax", "ay", "az"):
-            assert rel(a[f][idx], b[f][idx]) < 1e-6
+            assert rel(a[f][idx], b[f][idx]) < 1e-9
     ho = o.head()
     assert rel(hg[0], ho[0]) < 1e-12
-    assert rel(hg[1], ho[1]) < 1e-8
+    assert rel(hg[1], ho[1]) < 1e-10
     po = o.particles()
     for k in (6, 7, 8):
-        assert rel(pg[k], po[k]) < 1e-6
+        assert rel(pg[k], po[k]) < 1e-9
     assert st["n_nodes"] == len(on)
     check_solver(st, solver)
 
@@ -101,13 +101,14 @@ def test_amr_mixed_precision_fields(oracle, kind, n, lmin, lmax, crit, solver):
     assert len(gn) == len(on)
     for a, b in zip(gn, on):
         idx = np.concatenate([listed_index(b, k) for k in ("intr", "bdry", "sim")])
-        assert rel(a["rho"][idx], b["rho"][idx]) < 2e-6
-        assert rel(a["pot"][idx], b["pot"][idx]) < 2e-6
+        # FP32 fields after the FP64 refinement: 1e-5 is the bar; measured about 1e-7
+        assert rel(a["rho"][idx], b["rho"][idx]) < 1e-6
+        assert rel(a["pot"][idx], b["pot"][idx]) < 1e-6
         for f in ("ax", "ay", "az"):
-            assert rel(a[f][idx], b[f][idx]) < 1e-4
+            assert rel(a[f][idx], b[f][idx]) < 1e-6
     po = o.particles()
     for k in (6, 7, 8):
-        assert rel(pg[k], po[k]) < 1e-4
+        assert rel(pg[k], po[k]) < 1e-6
 
 
 @pytest.mark.parametrize("solver", SOLVERS)

@@ -13,8 +13,8 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("prec,tol_rho,tol_pot,tol_acc", [
-    (core.PREC_FP64, 1e-12, 1e-8, 1e-6),
-    (core.PREC_MIXED, 2e-6, 2e-6, 2e-5),
+    (core.PREC_FP64, 1e-12, 1e-10, 1e-9),
+    (core.PREC_MIXED, 1e-6, 1e-6, 1e-6),
 ])
 def test_head_force_pass_matches_oracle(oracle, prec, tol_rho, tol_pot, tol_acc):
     print(f"case: head force pass prec={prec} N=4000 lmin=5", flush=True)
