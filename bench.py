@@ -232,10 +232,11 @@ def adjoint_full_combination(n_each=1 << 23, lmin=8, lmax=11, crit=8, warm=40, s
                       f"(checkpoint every {every}), loss = final kinetic energy of the lighter sphere, mixed"}
 
 
-def precision_study(parts, lmin, lmax, crit, dt, steps=20):
+def precision_study(parts, lmin, lmax, crit, dt, steps=20, head_bc=0):
     """Mixed against the all-FP64 GPU path on the same workload: head potential
     and grid acceleration, particle accelerations after the first force pass,
-    then the divergence of the trajectories over `steps` fixed-dt steps."""
+    then the divergence of the trajectories over `steps` fixed-dt steps (dt None:
+    the CFL rule at t = 0, frozen)."""
     import numpy as np
     from quantizer_amd import core
 
@@ -243,7 +244,7 @@ def precision_study(parts, lmin, lmax, crit, dt, steps=20):
         return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
 
     out = {}
-    sims = [core.Simulation(parts, lmin, lmax, precision=p, ref_criterion_ptcl=crit)
+    sims = [core.Simulation(parts, lmin, lmax, precision=p, ref_criterion_ptcl=crit, head_bc=head_bc)
             for p in (core.PREC_MIXED, core.PREC_FP64)]
     try:
         for s in sims:
@@ -254,6 +255,8 @@ def precision_study(parts, lmin, lmax, crit, dt, steps=20):
         out["head_acceleration_rel_err"] = max(rel(hm[k], hd[k]) for k in (2, 3, 4))
         out["particle_acceleration_rel_err"] = max(rel(pm[k], pd[k]) for k in (6, 7, 8))
         del hm, hd
+        if dt is None:
+            dt = sims[1].timestep(cfl=0.5)
         for _ in range(steps):
             for s in sims:
                 s.step(dt)
@@ -267,6 +270,32 @@ def precision_study(parts, lmin, lmax, crit, dt, steps=20):
         for s in sims:
             s.close()
     return out
+
+
+def precision_configs():
+    """The precision study on each configuration of the brief (mixed against the
+    FP64 GPU path; the FP64 path itself is checked against the FP64 oracle by the
+    GPU tests)."""
+    from quantizer_amd import core
+    from quantizer_amd.workloads import clumps, plummer, plummer_dt, two_plummer
+    cases = [
+        ("plain: Plummer 1e4, 32^3, zero faces, dt 0.01, 100 steps",
+         lambda: plummer(10000), 5, 5, 8, core.BC_ZERO, plummer_dt(0.01), 100),
+        ("AMR: Plummer 2^20, 128^3, lmax 9, dt 0.005, 200 steps",
+         lambda: plummer(1 << 20), 7, 9, 8, core.BC_MONOPOLE, plummer_dt(0.005), 200),
+        ("precision: 2^24 (half uniform, half 64 clumps), 256^3, lmax 9, CFL dt, 20 steps",
+         lambda: clumps(1 << 24, 8, n_clumps=64), 8, 9, 8, core.BC_MONOPOLE, None, 20),
+        ("differentiation: Plummer 2^18, 64^3, dt 0.01, 50 steps",
+         lambda: plummer(1 << 18), 6, 6, 8, core.BC_MONOPOLE, plummer_dt(0.01), 50),
+        ("full: two Plummer spheres 2 x 2^23, 80a box, 256^3, lmax 11, CFL dt, 40 steps",
+         lambda: two_plummer(1 << 23), 8, 11, 8, core.BC_MONOPOLE, None, 40),
+    ]
+    res = {}
+    for name, make, lmin, lmax, crit, bc, dt, steps in cases:
+        t = time.time()
+        res[name] = precision_study(make(), lmin, lmax, crit, dt, steps, head_bc=bc)
+        log(f"  {name}: {res[name]} ({time.time() - t:.1f} s)")
+    return res
 
 
 def cpu_baseline(parts, lmin, lmax, crit, dt, budget_s):
@@ -329,12 +358,17 @@ def main():
     ap.add_argument("--branch-solver", choices=["cg", "mg"], default=None,
                     help="refinement-node solver (default: the library default)")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
+    ap.add_argument("--precision-configs", action="store_true",
+                    help="mixed against FP64 on every configuration; prints one JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.precision_configs:
+        print(json.dumps(precision_configs()), flush=True)
         return
     from quantizer_amd import core
     N, lmin, lmax, crit, kind = WORKLOADS[args.workload]
