@@ -595,11 +595,14 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
     int mc = lev_n[l + 1];
     size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
-    QZ_LAUNCHB("restrict", (double)sizeof(G) * (1.5 * np + nc), k_restrict_residual<G>, nblk(nc), lev_e[l], lev_r[l],
-               lev_r[l + 1], cf, make_cube(mc + 1), (G)((double)m * m));
+    const RestrictShape rs = restrict_shape(mc);
+    timer.begin(stream);
+    hipLaunchKernelGGL(k_restrict_residual<G>, dim3(rs.xt * rs.yt * rs.zt), dim3(rs.tx, rs.ty), 0, stream, lev_e[l],
+                       lev_r[l], lev_r[l + 1], mc, rs.yt, rs.zt, rs.L, (G)((double)m * m));
+    timer.end(stream, "restrict", (double)sizeof(G) * (1.5 * np + nc));
     vcycle(l + 1, true);
-    QZ_LAUNCHB("prolong", (double)sizeof(G) * (2.0 * np + nc), k_prolong_correct<G>, nblk(np), lev_e[l], lev_e[l + 1],
-               cf);
+    QZ_LAUNCHB("prolong", (double)sizeof(G) * (2.0 * np + nc), k_prolong_correct<G>, nblk((size_t)mc * mc * mc),
+               lev_e[l], lev_e[l + 1], make_cube(mc));
     for (int sw = 0; sw < cfg.post_sweeps; sw++)
     {
       QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, 0);

@@ -826,73 +826,175 @@ __global__ void k_rbgs(G *__restrict__ e, const G *__restrict__ r, Cube cb, G h2
 }
 
 // Residual of the fine error equation, full-weighting restricted to the coarse
-// grid (27-point stencil, weights 1/64 * (1,2,1)^3); rc at coarse interior.
-// It follows a red-black sweep pair that ended on black, which leaves a zero
-// residual at black fine points: only the 13 red points of the stencil count,
-// the centre (8/64) and the 12 points two offsets away (2/64).
+// grid (27-point stencil, weights 1/64 * (1,2,1)^3), written at the coarse
+// interior (the faces of rc are never written and stay zero).  It follows a
+// red-black sweep pair that ended on black, which leaves a zero residual at
+// black fine points: only red points count, the centre (8/64) and the 12
+// points with two odd offsets (2/64).  Each of those 12 is shared by four
+// coarse points, so coarse cell (I, J, K) owns three of them and computes each
+// once: xy at fine (2I+1, 2J+1, 2K), xz at (2I+1, 2J, 2K+1) and yz at
+// (2I, 2J+1, 2K+1).  Coarse point (I, J, K) sums xy of cells (I-1..I, J-1..J,
+// K), xz of (I-1..I, J, K-1..K) and yz of (I, J-1..J, K-1..K).
+//
+// A block of TX x TY threads holds cells (bx (TX-1) + tx, by (TY-1) + ty) and
+// marches up a chunk [K0, K1) of coarse planes: per plane it computes the
+// owned residuals (four of the black neighbours come from the previous plane
+// in registers), shares xy and the K-pair sums of xz and yz through LDS
+// (double-buffered by plane parity: one barrier per plane) and writes the
+// (TX-1) x (TY-1) coarse points of its tile; one plane below K0 supplies xz
+// and yz at K0 - 1.  The blocks are numbered chunk fastest, then y tile, then
+// x tile, in XCD-aware order.
 template <typename G>
-__global__ void k_restrict_residual(const G *__restrict__ e, const G *__restrict__ r, G *__restrict__ rc, Cube cf,
-                                    Cube cc, G inv_h2)
+__global__ void __launch_bounds__(1024) k_restrict_residual(const G *__restrict__ e, const G *__restrict__ r,
+                                                           G *__restrict__ rc, int nc, int yt, int zt, int L,
+                                                           G inv_h2)
 {
-  const int n1 = cf.n1, n = n1 - 1, nc1 = cc.n1, nc = nc1 - 1;
-  const unsigned npc = (unsigned)nc1 * nc1 * nc1, hc = (npc + 1) >> 1;
-  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= npc)
-    return;
-  int I, J, K;
-  cube_ijk(cc, t, I, J, K);
-  if (I == 0 || I == nc || J == 0 || J == nc || K == 0 || K == nc)
-  {
-    rc[(t & 1) * hc + (t >> 1)] = (G)0;
-    return;
-  }
-  const unsigned s1 = n1, s2 = (unsigned)n1 * n1;
-  const unsigned hf = ((unsigned)n1 * n1 * n1 + 1) >> 1;
+  __shared__ G s_xy[2][1024], s_xz[2][1024], s_yz[2][1024];
+  const int TX = blockDim.x, TY = blockDim.y, tx = threadIdx.x, ty = threadIdx.y;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int bz = (int)(b % (unsigned)zt), byx = (int)(b / (unsigned)zt);
+  const int by = byx % yt, bx = byx / yt;
+  const int Ic = bx * (TX - 1) + tx, Jc = by * (TY - 1) + ty;
+  const int K0 = 1 + bz * L, K1 = min(nc, K0 + L);
+  const bool cell = Ic < nc && Jc < nc;
+  const bool out = cell && tx > 0 && ty > 0;
+  const int me = ty * TX + tx;
+  const unsigned n1 = 2 * (unsigned)nc + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
+  const unsigned c1 = (unsigned)nc + 1, c2 = c1 * c1, hc = (c2 * c1 + 1) >> 1;
   const G *eB = e + hf; // black half
-  // red fine point f (even linear index): residual.  For a coarse interior
-  // point (I, J, K in [1, nc - 1]) all 13 fine points lie in [1, n - 1], so no
-  // face test is needed; without one the loads of all 13 issue together.
-  auto red_res = [&](unsigned f, int, int, int) -> G {
-    const unsigned hpos = f >> 1;
-    G lap = (eB[(f + 1) >> 1] + eB[(f - 1) >> 1] + eB[(f + s1) >> 1] + eB[(f - s1) >> 1] + eB[(f + s2) >> 1] +
-             eB[(f - s2) >> 1] - (G)6 * e[hpos]) *
-            inv_h2;
-    return r[hpos] - lap;
+  auto B = [&](unsigned x) { return eB[x >> 1]; };
+  // residual at red fine point x from the sum of its six black neighbours
+  auto res = [&](unsigned x, G nb) {
+    const unsigned hp = x >> 1;
+    return r[hp] - (nb - (G)6 * e[hp]) * inv_h2;
   };
-  const int i = 2 * I, j = 2 * J, k = 2 * K;
-  const unsigned f = (unsigned)i + s1 * (unsigned)j + s2 * (unsigned)k;
-  G edges = red_res(f + 1 + s1, i + 1, j + 1, k) + red_res(f + 1 - s1, i + 1, j - 1, k) +
-            red_res(f - 1 + s1, i - 1, j + 1, k) + red_res(f - 1 - s1, i - 1, j - 1, k) +
-            red_res(f + 1 + s2, i + 1, j, k + 1) + red_res(f + 1 - s2, i + 1, j, k - 1) +
-            red_res(f - 1 + s2, i - 1, j, k + 1) + red_res(f - 1 - s2, i - 1, j, k - 1) +
-            red_res(f + s1 + s2, i, j + 1, k + 1) + red_res(f + s1 - s2, i, j + 1, k - 1) +
-            red_res(f - s1 + s2, i, j - 1, k + 1) + red_res(f - s1 - s2, i, j - 1, k - 1);
-  rc[(t & 1) * hc + (t >> 1)] = (G)0.125 * red_res(f, i, j, k) + (G)(2.0 / 64.0) * edges;
+  // fine index of (2 Ic, 2 Jc, 2 (K0 - 1)).  For a cell (Ic, Jc < nc) every
+  // point read below has coordinates in [-1, 2 nc] with at most one -1, at a
+  // plane >= 0 (or >= 1 where the -1 is in z), so its index lies in the grid;
+  // the values read at a -1 feed residuals that no coarse point uses (xz of
+  // Jc = 0, yz of Ic = 0, the centre of the halo cells).
+  unsigned f = 2u * (unsigned)Ic + s1 * (2u * (unsigned)Jc) + s2 * (2u * (unsigned)(K0 - 1));
+  G xz0 = 0, yz0 = 0;   // xz, yz of the plane below
+  G cm2 = 0, cxy = 0, cp1 = 0, cps1 = 0; // black values carried to the next plane
+  if (cell)
+  {
+    const G bs2 = B(f + s2), bxy = B(f + 1 + s1 + s2), bxzk = B(f + 1 + 2 * s2), byzk = B(f + s1 + 2 * s2);
+    xz0 = res(f + 1 + s2, bs2 + B(f + 2 + s2) + B(f + 1 + s2 - s1) + bxy + B(f + 1) + bxzk);
+    yz0 = res(f + s1 + s2, B(f + s1 + s2 - 1) + bxy + bs2 + B(f + 2 * s1 + s2) + B(f + s1) + byzk);
+    cm2 = bs2;
+    cxy = bxy;
+    cp1 = bxzk;
+    cps1 = byzk;
+  }
+  for (int K = K0; K < K1; K++)
+  {
+    f += 2 * s2;
+    G ctr = 0, xy = 0, xz = 0, yz = 0;
+    if (cell)
+    {
+      // carried: cm2 = B(f - s2), cp1 = B(f + 1), cps1 = B(f + s1), cxy = B(f + 1 + s1 - s2)
+      const G bs2 = B(f + s2), bxy = B(f + 1 + s1 + s2), bxzk = B(f + 1 + 2 * s2), byzk = B(f + s1 + 2 * s2);
+      ctr = res(f, B(f - 1) + cp1 + B(f - s1) + cps1 + cm2 + bs2);
+      xy = res(f + 1 + s1, cps1 + B(f + 2 + s1) + cp1 + B(f + 1 + 2 * s1) + cxy + bxy);
+      xz = res(f + 1 + s2, bs2 + B(f + 2 + s2) + B(f + 1 + s2 - s1) + bxy + cp1 + bxzk);
+      yz = res(f + s1 + s2, B(f + s1 + s2 - 1) + bxy + bs2 + B(f + 2 * s1 + s2) + cps1 + byzk);
+      cm2 = bs2;
+      cxy = bxy;
+      cp1 = bxzk;
+      cps1 = byzk;
+    }
+    const int buf = K & 1;
+    s_xy[buf][me] = xy;
+    s_xz[buf][me] = xz0 + xz;
+    s_yz[buf][me] = yz0 + yz;
+    xz0 = xz;
+    yz0 = yz;
+    __syncthreads();
+    if (out)
+    {
+      const G edges = (s_xy[buf][me] + s_xy[buf][me - 1]) + (s_xy[buf][me - TX] + s_xy[buf][me - TX - 1]) +
+                      (s_xz[buf][me] + s_xz[buf][me - 1]) + (s_yz[buf][me] + s_yz[buf][me - TX]);
+      const unsigned q = (unsigned)Ic + c1 * (unsigned)Jc + c2 * (unsigned)K;
+      rc[(q & 1) * hc + (q >> 1)] = (G)0.125 * ctr + (G)(2.0 / 64.0) * edges;
+    }
+  }
+}
+
+// Launch shape of k_restrict_residual onto nc coarse cells per side: a tile
+// spans the whole x range where it can (TX = nc up to 256), TY rows make up to
+// 1024 threads, and the K chunk length L gives about 512 blocks.
+struct RestrictShape
+{
+  int tx, ty, xt, yt, zt, L;
+};
+inline RestrictShape restrict_shape(int nc)
+{
+  RestrictShape s;
+  s.tx = nc < 256 ? nc : 256;
+  s.ty = 1024 / s.tx < nc ? 1024 / s.tx : nc;
+  if (s.ty < 2)
+    s.ty = 2;
+  s.xt = (nc - 1 + s.tx - 2) / (s.tx - 1);
+  s.yt = (nc - 1 + s.ty - 2) / (s.ty - 1);
+  const long long cols = (long long)(nc - 1) * s.xt * s.yt;
+  s.L = (int)((cols + 511) / 512);
+  if (s.L < 2)
+    s.L = 2;
+  s.zt = (nc - 1 + s.L - 1) / s.L;
+  return s;
 }
 
 // Trilinear prolongation of the coarse correction into the fine interior
-// (multigrid.c:51-104) fused with the correction e += P ec (:199-209).
+// (multigrid.c:51-104) fused with the correction e += P ec (:199-209).  One
+// thread per coarse cell (I, J, K) in [0, nc)^3 (the Cube `cells` has nc per
+// side) updates the cell's 8 fine points (2I + a, 2J + b, 2K + c), a, b, c in
+// {0, 1}, from the cell's 8 corners: one coarse load per fine point where a
+// thread per fine point needs eight.  Fine points on the low faces (a
+// coordinate 0) are left alone; the high faces (2 nc) lie outside every cell.
+// For a fixed (a, b, c) the fine points of consecutive I are consecutive
+// entries of one colour half, so each of the 8 updates is a coalesced access.
 template <typename G>
-__global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, Cube cb)
+__global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, Cube cells)
 {
-  const int n1 = cb.n1, n = n1 - 1, nc1 = n / 2 + 1;
-  const unsigned npf = (unsigned)n1 * n1 * n1;
+  const int nc = cells.n1;
+  const unsigned ncell = (unsigned)nc * nc * nc;
   const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= npf)
+  if (t >= ncell)
     return;
-  int i, j, k;
-  cube_ijk(cb, t, i, j, k);
-  if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
-    return;
-  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-  const unsigned di = i & 1, dj = (j & 1) * (unsigned)nc1, dk = (k & 1) * (unsigned)nc1 * nc1;
-  const unsigned hc = ((unsigned)nc1 * nc1 * nc1 + 1) >> 1, hf = (npf + 1) >> 1;
-  const unsigned c0 = (unsigned)i0 + (unsigned)nc1 * ((unsigned)j0 + (unsigned)nc1 * k0);
-  auto C = [&](unsigned q) { return ec[(q & 1) * hc + (q >> 1)]; };
-  G v = (C(c0) + C(c0 + di) + C(c0 + dj) + C(c0 + di + dj) + C(c0 + dk) + C(c0 + di + dk) + C(c0 + dj + dk) +
-         C(c0 + di + dj + dk)) *
-        (G)0.125;
-  e[(t & 1) * hf + (t >> 1)] += v;
+  int I, J, K;
+  cube_ijk(cells, t, I, J, K);
+  const unsigned c1 = (unsigned)nc + 1, c2 = c1 * c1, hc = (c2 * c1 + 1) >> 1;
+  const unsigned q = (unsigned)I + c1 * (unsigned)J + c2 * (unsigned)K;
+  auto C = [&](unsigned d) {
+    const unsigned v = q + d;
+    return ec[(v & 1) * hc + (v >> 1)];
+  };
+  const G c000 = C(0), c100 = C(1), c010 = C(c1), c110 = C(1 + c1);
+  const G c001 = C(c2), c101 = C(1 + c2), c011 = C(c1 + c2), c111 = C(1 + c1 + c2);
+  const unsigned n1 = 2 * (unsigned)nc + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
+  const unsigned f = 2 * (unsigned)I + s1 * (2 * (unsigned)J) + s2 * (2 * (unsigned)K); // even: red
+  auto add = [&](unsigned d, G v) {
+    const unsigned w = f + d;
+    e[(w & 1) * hf + (w >> 1)] += v;
+  };
+  // sums along x of the four x edges of the cell
+  const G x00 = c000 + c100, x10 = c010 + c110, x01 = c001 + c101, x11 = c011 + c111;
+  const bool i0 = I > 0, j0 = J > 0, k0 = K > 0;
+  if (i0 && j0 && k0)
+    add(0, c000);
+  if (j0 && k0)
+    add(1, (G)0.5 * x00);
+  if (i0 && k0)
+    add(s1, (G)0.5 * (c000 + c010));
+  if (k0)
+    add(1 + s1, (G)0.25 * (x00 + x10));
+  if (i0 && j0)
+    add(s2, (G)0.5 * (c000 + c001));
+  if (j0)
+    add(1 + s2, (G)0.25 * (x00 + x01));
+  if (i0)
+    add(s1 + s2, (G)0.25 * ((c000 + c010) + (c001 + c011)));
+  add(1 + s1 + s2, (G)0.125 * ((x00 + x10) + (x01 + x11)));
 }
 
 // ------------------------------------------------------------------ V-cycle tail in one workgroup
