@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -149,7 +150,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *phi = nullptr;
   G *gax = nullptr, *gay = nullptr, *gaz = nullptr;
   std::vector<G *> lev_e, lev_r;
+  std::vector<G *> lev_t; // levels below the top: the pre-smoothed correction between k_mg_down and k_mg_up
   std::vector<int> lev_n;
+  bool mg_fused = true; // fused tile kernels on the levels below the top (QZ_HEAD_MG_FUSED=0 turns them off)
   double *part = nullptr; // reduction partials
   double *red = nullptr;  // device scalars
   double *hred = nullptr; // pinned host mirror
@@ -297,6 +300,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     {
       (void)hipFree(lev_e[l]);
       (void)hipFree(lev_r[l]);
+      (void)hipFree(lev_t[l]);
     }
     if (!lev_e.empty())
     {
@@ -320,6 +324,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     N = cfg.n_particles;
     n = 1 << cfg.lmin;
     D = std::min(cfg.lmax - cfg.lmin, 7);
+    if (const char *f = std::getenv("QZ_HEAD_MG_FUSED"))
+      mg_fused = f[0] != '0';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
@@ -363,7 +369,8 @@ template <typename P, typename G> struct Engine final : EngineBase
       lev_n.push_back(m);
       lev_e.push_back(dmalloc<G>(p1 * p1 * p1, bytes));
       lev_r.push_back(dmalloc<G>(p1 * p1 * p1, bytes));
-      if (!lev_e.back() || !lev_r.back())
+      lev_t.push_back(m < n ? dmalloc<G>(p1 * p1 * p1, bytes) : nullptr);
+      if (!lev_e.back() || !lev_r.back() || (m < n && !lev_t.back()))
         return hip_ok(hipErrorOutOfMemory, "multigrid levels");
     }
     if (lev_n.back() > 8)
@@ -575,14 +582,38 @@ template <typename P, typename G> struct Engine final : EngineBase
       timer.end(stream, "coarse_cg");
       return;
     }
-    // the small levels run in one workgroup (the top level keeps the per-level
-    // kernels: it may start from a non-zero correction)
+    // Below the top level (which keeps the per-level kernels: it may start from
+    // a non-zero correction) the coarsest levels run in one workgroup, and the
+    // levels of up to kMgFusedCells cells above them as two fused tile kernels
+    // each (default sweep counts); larger levels move enough data to keep the
+    // per-level kernels.
     if (l > 0 && zero_start && m <= Tail<G>::cells)
     {
       timer.begin(stream);
-      hipLaunchKernelGGL(k_vcycle_tail<G>, dim3(1), dim3(kTailThreads), 0, stream, lev_e[l], lev_r[l], m,
-                         cfg.pre_sweeps, cfg.post_sweeps, 64);
+      if (m == 16)
+        hipLaunchKernelGGL((k_vcycle_tail<G, 16>), dim3(1), dim3(kTailThreads), 0, stream, lev_e[l], lev_r[l],
+                           cfg.pre_sweeps, cfg.post_sweeps);
+      else
+        hipLaunchKernelGGL((k_vcycle_tail<G, 8>), dim3(1), dim3(kTailThreads), 0, stream, lev_e[l], lev_r[l],
+                           cfg.pre_sweeps, cfg.post_sweeps);
       timer.end(stream, "vcycle_tail");
+      return;
+    }
+    if (l > 0 && zero_start && mg_fused && cfg.pre_sweeps == 2 && cfg.post_sweeps == 2 && m <= kMgFusedCells)
+    {
+      const int T = m / (2 * kMgCT); // tiles per axis (m >= 16)
+      const int mc = lev_n[l + 1];
+      const size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
+      const G h2 = (G)(1.0 / ((double)m * m));
+      timer.begin(stream);
+      hipLaunchKernelGGL(k_mg_down<G>, dim3(T * T * T), dim3(kMgThreads), 0, stream, lev_t[l], lev_r[l],
+                         lev_r[l + 1], m, T, h2, (G)((double)m * m));
+      timer.end(stream, "mg_down", (double)sizeof(G) * (2.0 * np + nc));
+      vcycle(l + 1, true);
+      timer.begin(stream);
+      hipLaunchKernelGGL(k_mg_up<G>, dim3(T * T * T), dim3(kMgThreads), 0, stream, lev_e[l], lev_r[l], lev_t[l],
+                         lev_e[l + 1], m, T, h2);
+      timer.end(stream, "mg_up", (double)sizeof(G) * (3.0 * np + nc));
       return;
     }
     const size_t nhalf = rbhalf(np);

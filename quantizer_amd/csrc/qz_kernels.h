@@ -998,191 +998,203 @@ __global__ void k_prolong_correct(G *__restrict__ e, const G *__restrict__ ec, C
 }
 
 // ------------------------------------------------------------------ V-cycle tail in one workgroup
-// The coarse end of a head V-cycle (levels of at most kTailCells<G> cells per
-// side down to the coarsest) in a single launch: e and r of every tail level
-// staged in LDS (colour-split, like the global levels), the same sweeps,
-// 13-point restriction, trilinear prolongation and coarsest-level CG as the
-// per-level kernels, separated by workgroup barriers instead of launches.
+// The coarse end of a head V-cycle (levels of NTOP, NTOP / 2, ..., 4 cells per
+// side) in a single launch: e and r of every tail level staged in LDS
+// (colour-split, like the global levels), the same red-black sweeps, the
+// 13-point restriction of the red residuals, trilinear prolongation, and on
+// the 4-cell level an exact solve, separated by workgroup barriers instead of
+// launches.  Level sizes are compile-time constants.
 template <typename G> struct Tail
 {
-  static constexpr int cells = sizeof(G) == 4 ? 16 : 8;                 // finest tail level
-  static constexpr int points = sizeof(G) == 4 ? 4913 + 729 + 125 : 729 + 125; // all tail levels
+  static constexpr int cells = sizeof(G) == 4 ? 16 : 8; // finest tail level
 };
 constexpr int kTailThreads = 1024;
 
-template <typename G>
-__global__ void __launch_bounds__(kTailThreads)
-    k_vcycle_tail(G *__restrict__ e_top, const G *__restrict__ r_top, int n_top, int pre, int post, int cg_iters)
+template <int M> constexpr int tail_points()
 {
-  __shared__ G se[Tail<G>::points], sr[Tail<G>::points];
-  __shared__ double cx[125], crr[125], cp[125], cap[125];
-  int nl = 0, off[4], nn[4];
-  for (int m = n_top, o = 0; m >= 4; m /= 2, nl++)
+  if constexpr (M < 4)
+    return 0;
+  else
+    return (M + 1) * (M + 1) * (M + 1) + tail_points<M / 2>();
+}
+
+// red-black half sweep of Lap e = r on the M-cell level (faces stay zero)
+template <typename G, int M> __device__ inline void tail_sweep(G *E, const G *R, int color, bool zs)
+{
+  constexpr int P = M + 1, PP = P * P, NP = PP * P, HF = (NP + 1) / 2;
+  const G h2 = (G)(1.0 / ((double)M * M));
+  const int cnt = color == 0 ? HF : NP - HF;
+  const G *o = E + (color == 0 ? HF : 0);
+  for (int h = threadIdx.x; h < cnt; h += blockDim.x)
   {
-    nn[nl] = m;
-    off[nl] = o;
-    o += (m + 1) * (m + 1) * (m + 1);
-  }
-  const int tid = threadIdx.x, NT = blockDim.x;
-  {
-    const int np = (n_top + 1) * (n_top + 1) * (n_top + 1);
-    for (int t = tid; t < np; t += NT)
-      sr[t] = r_top[t];
+    const int idx = 2 * h + color;
+    const int i = idx % P, j = (idx / P) % P, k = idx / PP;
+    if (i == 0 || i == M || j == 0 || j == M || k == 0 || k == M)
+      continue;
+    const G rv = R[color * HF + h];
+    G sum = zs ? (G)0
+               : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + P) >> 1] + o[(idx - P) >> 1] + o[(idx + PP) >> 1] +
+                     o[(idx - PP) >> 1];
+    E[color * HF + h] = (sum - h2 * rv) * (G)(1.0 / 6.0);
   }
   __syncthreads();
-  auto P = [](int idx, int hf) { return (idx & 1) * hf + (idx >> 1); };
-  auto sweep = [&](int l, int color, int zs) {
-    const int m = nn[l], p = m + 1, pp = p * p, np = pp * p, hf = (np + 1) >> 1;
-    G *E = se + off[l];
-    const G *R = sr + off[l];
-    const G h2 = (G)(1.0 / ((double)m * m));
-    const G *o = E + (color == 0 ? hf : 0);
-    const int cnt = color == 0 ? hf : np - hf;
-    for (int h = tid; h < cnt; h += NT)
-    {
-      const int idx = 2 * h + color;
-      const int i = idx % p, j = (idx / p) % p, k = idx / pp;
-      if (i == 0 || i == m || j == 0 || j == m || k == 0 || k == m)
-      {
-        E[color * hf + h] = (G)0;
-        continue;
-      }
-      G sum = zs ? (G)0
-                 : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + p) >> 1] + o[(idx - p) >> 1] +
-                       o[(idx + pp) >> 1] + o[(idx - pp) >> 1];
-      E[color * hf + h] = (sum - h2 * R[color * hf + h]) * (G)(1.0 / 6.0);
-    }
-    __syncthreads();
-  };
-  auto restrict_ = [&](int l) { // level l -> l+1, after a sweep pair ending on black
-    const int m = nn[l], p = m + 1, pp = p * p, hf = (pp * p + 1) >> 1;
-    const int mc = nn[l + 1], pc = mc + 1, ppc = pc * pc, npc = ppc * pc, hc = (npc + 1) >> 1;
-    const G *E = se + off[l], *R = sr + off[l], *EB = E + hf;
-    G *RC = sr + off[l + 1];
-    const G inv_h2 = (G)((double)m * m);
-    auto red_res = [&](int f, int i, int j, int k) -> G {
-      if (i <= 0 || i >= m || j <= 0 || j >= m || k <= 0 || k >= m)
-        return (G)0;
-      G lap = (EB[(f + 1) >> 1] + EB[(f - 1) >> 1] + EB[(f + p) >> 1] + EB[(f - p) >> 1] + EB[(f + pp) >> 1] +
-               EB[(f - pp) >> 1] - (G)6 * E[f >> 1]) *
-              inv_h2;
-      return R[f >> 1] - lap;
-    };
-    for (int t = tid; t < npc; t += NT)
-    {
-      const int I = t % pc, J = (t / pc) % pc, K = t / ppc;
-      G v = (G)0;
-      if (!(I == 0 || I == mc || J == 0 || J == mc || K == 0 || K == mc))
-      {
-        const int i = 2 * I, j = 2 * J, k = 2 * K, f = i + p * j + pp * k;
-        G edges = red_res(f + 1 + p, i + 1, j + 1, k) + red_res(f + 1 - p, i + 1, j - 1, k) +
-                  red_res(f - 1 + p, i - 1, j + 1, k) + red_res(f - 1 - p, i - 1, j - 1, k) +
-                  red_res(f + 1 + pp, i + 1, j, k + 1) + red_res(f + 1 - pp, i + 1, j, k - 1) +
-                  red_res(f - 1 + pp, i - 1, j, k + 1) + red_res(f - 1 - pp, i - 1, j, k - 1) +
-                  red_res(f + p + pp, i, j + 1, k + 1) + red_res(f + p - pp, i, j + 1, k - 1) +
-                  red_res(f - p + pp, i, j - 1, k + 1) + red_res(f - p - pp, i, j - 1, k - 1);
-        v = (G)0.125 * red_res(f, i, j, k) + (G)(2.0 / 64.0) * edges;
-      }
-      RC[P(t, hc)] = v;
-    }
-    __syncthreads();
-  };
-  auto prolong = [&](int l) { // level l+1 -> l, e_l += P e_{l+1}
-    const int m = nn[l], p = m + 1, pp = p * p, np = pp * p, hf = (np + 1) >> 1;
-    const int pc = nn[l + 1] + 1, npc = pc * pc * pc, hc = (npc + 1) >> 1;
-    G *E = se + off[l];
-    const G *EC = se + off[l + 1];
-    for (int t = tid; t < np; t += NT)
-    {
-      const int i = t % p, j = (t / p) % p, k = t / pp;
-      if (i == 0 || i == m || j == 0 || j == m || k == 0 || k == m)
-        continue;
-      const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-      const int di = i & 1, dj = (j & 1) * pc, dk = (k & 1) * pc * pc;
-      const int c0 = i0 + pc * (j0 + pc * k0);
-      G v = (EC[P(c0, hc)] + EC[P(c0 + di, hc)] + EC[P(c0 + dj, hc)] + EC[P(c0 + di + dj, hc)] + EC[P(c0 + dk, hc)] +
-             EC[P(c0 + di + dk, hc)] + EC[P(c0 + dj + dk, hc)] + EC[P(c0 + di + dj + dk, hc)]) *
-            (G)0.125;
-      E[P(t, hf)] += v;
-    }
-    __syncthreads();
-  };
-  auto coarse = [&](int l) { // CG on the 5^3 level (A = -Lap, zero faces)
-    const int m = nn[l], p = m + 1, pp = p * p, np = pp * p, hf = (np + 1) >> 1;
-    const double inv_h2 = (double)m * m;
-    G *E = se + off[l];
-    const G *R = sr + off[l];
-    auto inside = [&](int t) {
-      const int i = t % p, j = (t / p) % p, k = t / pp;
-      return i > 0 && i < m && j > 0 && j < m && k > 0 && k < m;
-    };
-    for (int t = tid; t < np; t += NT)
-    {
-      cx[t] = 0.0;
-      crr[t] = inside(t) ? -(double)R[P(t, hf)] : 0.0;
-      cp[t] = crr[t];
-    }
-    __syncthreads();
-    double rr = 0.0;
-    for (int t = tid; t < np; t += NT)
-      rr += crr[t] * crr[t];
-    rr = block_reduce_n<0>(rr);
-    const double rr0 = rr;
-    for (int it = 0; it < cg_iters && rr > 1e-28 * rr0 && rr > 0.0; it++)
-    {
-      double pap = 0.0;
-      for (int t = tid; t < np; t += NT)
-      {
-        double v = 0.0;
-        if (inside(t))
-          v = inv_h2 * (6.0 * cp[t] - cp[t + 1] - cp[t - 1] - cp[t + p] - cp[t - p] - cp[t + pp] - cp[t - pp]);
-        cap[t] = v;
-        pap += cp[t] * v;
-      }
-      pap = block_reduce_n<0>(pap);
-      const double alpha = rr / pap;
-      double rn = 0.0;
-      for (int t = tid; t < np; t += NT)
-      {
-        cx[t] += alpha * cp[t];
-        crr[t] -= alpha * cap[t];
-        rn += crr[t] * crr[t];
-      }
-      rn = block_reduce_n<0>(rn);
-      const double beta = rn / rr;
-      for (int t = tid; t < np; t += NT)
-        cp[t] = crr[t] + beta * cp[t];
-      rr = rn;
-      __syncthreads();
-    }
-    for (int t = tid; t < np; t += NT)
-      E[P(t, hf)] = (G)cx[t];
-    __syncthreads();
-  };
-  for (int l = 0; l + 1 < nl; l++)
+}
+
+// full-weighting restriction M -> M/2 after a sweep pair ending on black: the
+// red residuals once into `res`, then the 13-point sums (zero on the faces)
+template <typename G, int M> __device__ inline void tail_restrict(const G *E, const G *R, G *RC, G *res)
+{
+  constexpr int P = M + 1, PP = P * P, NP = PP * P, HF = (NP + 1) / 2;
+  constexpr int MC = M / 2, PC = MC + 1, PPC = PC * PC, NPC = PPC * PC, HC = (NPC + 1) / 2;
+  const G inv_h2 = (G)((double)M * M);
+  const G *EB = E + HF;
+  for (int h = threadIdx.x; h < HF; h += blockDim.x)
   {
+    const int f = 2 * h;
+    const int i = f % P, j = (f / P) % P, k = f / PP;
+    G v = (G)0;
+    if (!(i == 0 || i == M || j == 0 || j == M || k == 0 || k == M))
+    {
+      const G lap = (EB[(f + 1) >> 1] + EB[(f - 1) >> 1] + EB[(f + P) >> 1] + EB[(f - P) >> 1] + EB[(f + PP) >> 1] +
+                     EB[(f - PP) >> 1] - (G)6 * E[h]) *
+                    inv_h2;
+      v = R[h] - lap;
+    }
+    res[h] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NPC; t += blockDim.x)
+  {
+    const int I = t % PC, J = (t / PC) % PC, K = t / PPC;
+    G v = (G)0;
+    if (!(I == 0 || I == MC || J == 0 || J == MC || K == 0 || K == MC))
+    {
+      const int f = 2 * I + P * 2 * J + PP * 2 * K;
+      auto S = [&](int d) { return res[(f + d) >> 1]; };
+      const G edges = S(1 + P) + S(1 - P) + S(-1 + P) + S(-1 - P) + S(1 + PP) + S(1 - PP) + S(-1 + PP) + S(-1 - PP) +
+                      S(P + PP) + S(P - PP) + S(-P + PP) + S(-P - PP);
+      v = (G)0.125 * S(0) + (G)(2.0 / 64.0) * edges;
+    }
+    RC[(t & 1) * HC + (t >> 1)] = v;
+  }
+  __syncthreads();
+}
+
+// trilinear prolongation M/2 -> M fused with the correction, one thread per
+// coarse cell (the formulas of k_prolong_correct)
+template <typename G, int M> __device__ inline void tail_prolong(G *E, const G *EC)
+{
+  constexpr int P = M + 1, PP = P * P, NP = PP * P, HF = (NP + 1) / 2;
+  constexpr int MC = M / 2, C1 = MC + 1, C2 = C1 * C1, HC = (C2 * C1 + 1) / 2;
+  for (int t = threadIdx.x; t < MC * MC * MC; t += blockDim.x)
+  {
+    const int I = t % MC, J = (t / MC) % MC, K = t / (MC * MC);
+    const int q = I + C1 * J + C2 * K;
+    auto C = [&](int d) { return EC[((q + d) & 1) * HC + ((q + d) >> 1)]; };
+    const G c000 = C(0), c100 = C(1), c010 = C(C1), c110 = C(1 + C1);
+    const G c001 = C(C2), c101 = C(1 + C2), c011 = C(C1 + C2), c111 = C(1 + C1 + C2);
+    const int f = 2 * I + P * 2 * J + PP * 2 * K;
+    auto add = [&](int d, G v) { E[((f + d) & 1) * HF + ((f + d) >> 1)] += v; };
+    const G x00 = c000 + c100, x10 = c010 + c110, x01 = c001 + c101, x11 = c011 + c111;
+    const bool i0 = I > 0, j0 = J > 0, k0 = K > 0;
+    if (i0 && j0 && k0)
+      add(0, c000);
+    if (j0 && k0)
+      add(1, (G)0.5 * x00);
+    if (i0 && k0)
+      add(P, (G)0.5 * (c000 + c010));
+    if (k0)
+      add(1 + P, (G)0.25 * (x00 + x10));
+    if (i0 && j0)
+      add(PP, (G)0.5 * (c000 + c001));
+    if (j0)
+      add(1 + PP, (G)0.25 * (x00 + x01));
+    if (i0)
+      add(P + PP, (G)0.25 * ((c000 + c010) + (c001 + c011)));
+    add(1 + P + PP, (G)0.125 * ((x00 + x10) + (x01 + x11)));
+  }
+  __syncthreads();
+}
+
+// The 4-cell level solved exactly: Lap e = r with zero faces on its 27
+// interior points is e = -(1/16) T^-1 r, T = 6 I - adjacency, and T is
+// diagonalised by the 3-point sine transform S (S_ai = sin(pi a i / 4) / sqrt 2,
+// a, i = 1..3, symmetric and orthogonal) with eigenvalues mu_a + mu_b + mu_c,
+// mu_a = 2 - 2 cos(pi a / 4).  Two passes of 27 threads, in FP64.
+template <typename G> __device__ inline void tail_coarse4(G *E, const G *R, double *cb)
+{
+  constexpr int P = 5, PP = 25, HF = 63;
+  const double s = 0.70710678118654752440; // 1 / sqrt 2 = cos(pi / 4)
+  const double S[3][3] = {{0.5, s, 0.5}, {s, 0.0, -s}, {0.5, -s, 0.5}};
+  const double mu[3] = {2.0 - 2.0 * s, 2.0, 2.0 + 2.0 * s};
+  const int tid = threadIdx.x;
+  if (tid < 27)
+  {
+    const int a = tid % 3, b = (tid / 3) % 3, c = tid / 9;
+    double B = 0.0;
+    for (int v = 0; v < 27; v++)
+    {
+      const int i = v % 3, j = (v / 3) % 3, k = v / 9;
+      const int g = (i + 1) + P * (j + 1) + PP * (k + 1);
+      B += S[a][i] * S[b][j] * S[c][k] * (double)R[(g & 1) * HF + (g >> 1)];
+    }
+    cb[tid] = B / (mu[a] + mu[b] + mu[c]);
+  }
+  __syncthreads();
+  if (tid < 27)
+  {
+    const int i = tid % 3, j = (tid / 3) % 3, k = tid / 9;
+    double x = 0.0;
+    for (int w = 0; w < 27; w++)
+    {
+      const int a = w % 3, b = (w / 3) % 3, c = w / 9;
+      x += S[a][i] * S[b][j] * S[c][k] * cb[w];
+    }
+    const int g = (i + 1) + P * (j + 1) + PP * (k + 1);
+    E[(g & 1) * HF + (g >> 1)] = (G)(-x / 16.0);
+  }
+  __syncthreads();
+}
+
+template <typename G, int M> __device__ inline void tail_level(G *se, G *sr, G *res, double *cb, int pre, int post)
+{
+  if constexpr (M == 4)
+    tail_coarse4<G>(se, sr, cb);
+  else
+  {
+    constexpr int NP = (M + 1) * (M + 1) * (M + 1);
     for (int s = 0; s < pre; s++)
     {
-      sweep(l, 0, s == 0);
-      sweep(l, 1, 0);
+      tail_sweep<G, M>(se, sr, 0, s == 0);
+      tail_sweep<G, M>(se, sr, 1, false);
     }
-    restrict_(l);
-  }
-  coarse(nl - 1);
-  for (int l = nl - 2; l >= 0; l--)
-  {
-    prolong(l);
+    tail_restrict<G, M>(se, sr, sr + NP, res);
+    tail_level<G, M / 2>(se + NP, sr + NP, res, cb, pre, post);
+    tail_prolong<G, M>(se, se + NP);
     for (int s = 0; s < post; s++)
     {
-      sweep(l, 0, 0);
-      sweep(l, 1, 0);
+      tail_sweep<G, M>(se, sr, 0, false);
+      tail_sweep<G, M>(se, sr, 1, false);
     }
   }
+}
+
+template <typename G, int NTOP>
+__global__ void __launch_bounds__(kTailThreads)
+    k_vcycle_tail(G *__restrict__ e_top, const G *__restrict__ r_top, int pre, int post)
+{
+  constexpr int NPT = tail_points<NTOP>(), NP0 = (NTOP + 1) * (NTOP + 1) * (NTOP + 1), HF0 = (NP0 + 1) / 2;
+  __shared__ G se[NPT], sr[NPT], res[HF0];
+  __shared__ double cb[27];
+  for (int t = threadIdx.x; t < NPT; t += blockDim.x)
   {
-    const int np = (n_top + 1) * (n_top + 1) * (n_top + 1);
-    for (int t = tid; t < np; t += NT)
-      e_top[t] = se[t];
+    se[t] = (G)0;
+    sr[t] = t < NP0 ? r_top[t] : (G)0;
   }
+  __syncthreads();
+  tail_level<G, NTOP>(se, sr, res, cb, pre, post);
+  for (int t = threadIdx.x; t < NP0; t += blockDim.x)
+    e_top[t] = se[t];
 }
 
 // Coarsest level: conjugate gradients in one workgroup (FP64 internally) on a
@@ -1240,6 +1252,239 @@ template <typename G> __global__ void k_coarse_cg(G *__restrict__ e, const G *__
   }
   for (int t = threadIdx.x; t < np; t += blockDim.x)
     e[rbpos(t, rbhalf(np))] = (G)xs[t];
+}
+
+// ------------------------------------------------------------------ fused levels below the top
+// Below the finest head level a grid is small enough that a launch costs more
+// than the data it moves.  There the two pre-smoothing red-black pairs and the
+// residual restriction of a level run as one kernel (k_mg_down), and the
+// prolongation with the two post-smoothing pairs as another (k_mg_up).  Each
+// workgroup owns a tile of kMgCT^3 coarse points and the fine points
+// [2 I0 - 1, 2 I0 + 2 kMgCT - 2] per axis (I0 the tile's first coarse index),
+// and stages the fine region [2 I0 - 5, 2 I0 + 2 kMgCT + 3] (kMgLF points per
+// axis, odd) in LDS.  Every sweep phase runs on a region one point narrower
+// per side than the phase before it, so the owned points end exactly as the
+// per-level kernels leave them (same update formulas and summation orders);
+// the halo is recomputed by the neighbouring tiles.  LDS holds e and r of the
+// region colour-split by local parity (kMgLF odd: local parity is the
+// coordinate-sum parity, and the global colour is it XOR that of the origin).
+constexpr int kMgCT = 4;                          // coarse points per tile side
+constexpr int kMgLF = 2 * kMgCT + 9;              // fine points per LDS region side
+constexpr int kMgNP = kMgLF * kMgLF * kMgLF;      // LDS region points
+constexpr int kMgHF = (kMgNP + 1) / 2;            // first (even local parity) half
+constexpr int kMgThreads = 512;
+constexpr int kMgFusedCells = 64; // largest level (cells per side) run fused
+
+__device__ inline bool mg_interior(int X, int Y, int Z, int m)
+{
+  return X >= 1 && X <= m - 1 && Y >= 1 && Y <= m - 1 && Z >= 1 && Z <= m - 1;
+}
+
+// One red-black half sweep on the local points of parity lp in the region
+// [S, kMgLF - 1 - S]^3 that are interior points of the level (m cells); ZS is
+// the zero-start form of the first sweep (k_rbgs with zero_start).  Slots are
+// laid out row by row, (kMgLF - 2S + 1) / 2 per row.
+template <typename G, int S, bool ZS>
+__device__ inline void mg_tile_sweep(G *sE, const G *sR, int lp, int fx, int fy, int fz, int m, G h2)
+{
+  constexpr int LF = kMgLF, LF2 = LF * LF, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
+  const G *o = sE + (1 - lp) * kMgHF;
+  for (int q = threadIdx.x; q < SLOTS; q += blockDim.x)
+  {
+    const int row = q / HX, u = q - row * HX;
+    const int y = S + row % L, z = S + row / L;
+    const int x = S + ((S + y + z + lp) & 1) + 2 * u;
+    if (x > LF - 1 - S || !mg_interior(fx + x, fy + y, fz + z, m))
+      continue;
+    const int idx = x + LF * y + LF2 * z;
+    const int p = lp * kMgHF + (idx >> 1);
+    const G rv = sR[p];
+    G sum = ZS ? (G)0
+               : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + LF) >> 1] + o[(idx - LF) >> 1] +
+                     o[(idx + LF2) >> 1] + o[(idx - LF2) >> 1];
+    sE[p] = (sum - h2 * rv) * (G)(1.0 / 6.0);
+  }
+  __syncthreads();
+}
+
+// Pre-smoothing (two pairs from a zero start) and full-weighting restriction
+// of the residual of level l (m cells) into rc of level l + 1, for the tiles
+// of T^3 (T = m / (2 kMgCT)); writes the smoothed correction of the owned fine
+// points to e (k_mg_up's et).
+// Matches k_rbgs x4 followed by k_restrict_residual point for point.
+template <typename G>
+__global__ void __launch_bounds__(kMgThreads)
+    k_mg_down(G *__restrict__ e, const G *__restrict__ r, G *__restrict__ rc, int m, int T, G h2, G inv_h2)
+{
+  constexpr int LF = kMgLF, LF2 = LF * LF, HF = kMgHF;
+  __shared__ G sE[kMgNP], sR[kMgNP];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int I0 = 1 + kMgCT * (b % T), J0 = 1 + kMgCT * ((b / T) % T), K0 = 1 + kMgCT * (b / (T * T));
+  const int fx = 2 * I0 - 5, fy = 2 * J0 - 5, fz = 2 * K0 - 5; // level coordinates of local (0, 0, 0)
+  const int goff = (fx + fy + fz) & 1;                        // local parity of the red points
+  const unsigned n1 = (unsigned)m + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
+  for (int t = tid; t < kMgNP; t += NT)
+  {
+    const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
+    const int X = fx + x, Y = fy + y, Z = fz + z;
+    G rv = (G)0;
+    if (X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m)
+    {
+      const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
+      rv = r[(g & 1) * hf + (g >> 1)];
+    }
+    const int p = (t & 1) * HF + (t >> 1);
+    sR[p] = rv;
+    sE[p] = (G)0;
+  }
+  __syncthreads();
+  const int red = goff, black = goff ^ 1;
+  mg_tile_sweep<G, 0, true>(sE, sR, red, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, 1, false>(sE, sR, black, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, 2, false>(sE, sR, red, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, 3, false>(sE, sR, black, fx, fy, fz, m, h2);
+  // residual r - Lap e at the red points of [4, LF - 5]^3, in place of r
+  {
+    constexpr int S = 4, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
+    const G *o = sE + black * HF;
+    for (int q = tid; q < SLOTS; q += NT)
+    {
+      const int row = q / HX, u = q - row * HX;
+      const int y = S + row % L, z = S + row / L;
+      const int x = S + ((S + y + z + red) & 1) + 2 * u;
+      if (x > LF - 1 - S || !mg_interior(fx + x, fy + y, fz + z, m))
+        continue;
+      const int idx = x + LF * y + LF2 * z;
+      const int p = red * HF + (idx >> 1);
+      const G nb = o[(idx - 1) >> 1] + o[(idx + 1) >> 1] + o[(idx - LF) >> 1] + o[(idx + LF) >> 1] +
+                   o[(idx - LF2) >> 1] + o[(idx + LF2) >> 1];
+      sR[p] = sR[p] - (nb - (G)6 * sE[p]) * inv_h2;
+    }
+    // the smoothed correction of the owned points [4, 3 + 2 kMgCT]^3
+    constexpr int OW = 2 * kMgCT;
+    for (int q = tid; q < OW * OW * OW; q += NT)
+    {
+      const int x = 4 + q % OW, y = 4 + (q / OW) % OW, z = 4 + q / (OW * OW);
+      const int X = fx + x, Y = fy + y, Z = fz + z;
+      if (!mg_interior(X, Y, Z, m))
+        continue;
+      const int idx = x + LF * y + LF2 * z;
+      const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
+      e[(g & 1) * hf + (g >> 1)] = sE[(idx & 1) * HF + (idx >> 1)];
+    }
+  }
+  __syncthreads();
+  // restriction to the tile's coarse interior points (summation order of k_restrict_residual)
+  const int mc = m / 2;
+  const unsigned c1 = (unsigned)mc + 1, c2 = c1 * c1, hc = (c2 * c1 + 1) >> 1;
+  const G *res = sR + red * HF;
+  for (int q = tid; q < kMgCT * kMgCT * kMgCT; q += NT)
+  {
+    const int a = q % kMgCT, bb = (q / kMgCT) % kMgCT, c = q / (kMgCT * kMgCT);
+    const int I = I0 + a, J = J0 + bb, K = K0 + c;
+    if (I > mc - 1 || J > mc - 1 || K > mc - 1)
+      continue;
+    const int idx = (2 * a + 5) + LF * (2 * bb + 5) + LF2 * (2 * c + 5);
+    auto R = [&](int d) { return res[(idx + d) >> 1]; };
+    const G ctr = R(0);
+    const G xy00 = R(1 + LF), xy10 = R(-1 + LF), xy01 = R(1 - LF), xy11 = R(-1 - LF);
+    const G xz0 = R(1 - LF2) + R(1 + LF2), xz1 = R(-1 - LF2) + R(-1 + LF2);
+    const G yz0 = R(LF - LF2) + R(LF + LF2), yz1 = R(-LF - LF2) + R(-LF + LF2);
+    const G edges = (xy00 + xy10) + (xy01 + xy11) + (xz0 + xz1) + (yz0 + yz1);
+    const unsigned qc = (unsigned)I + c1 * (unsigned)J + c2 * (unsigned)K;
+    rc[(qc & 1) * hc + (qc >> 1)] = (G)0.125 * ctr + (G)(2.0 / 64.0) * edges;
+  }
+}
+
+// Prolongation of the coarse correction ec (level l + 1) into level l's
+// pre-smoothed correction et (written by k_mg_down) and the two post-smoothing
+// pairs, for the same tiles as k_mg_down; the result goes to e (a separate
+// array: the tiles read et in their halos, which other tiles own).  Matches
+// k_prolong_correct followed by k_rbgs x4.
+template <typename G>
+__global__ void __launch_bounds__(kMgThreads)
+    k_mg_up(G *__restrict__ e, const G *__restrict__ r, const G *__restrict__ et, const G *__restrict__ ec, int m,
+            int T, G h2)
+{
+  constexpr int LF = kMgLF, LF2 = LF * LF, HF = kMgHF;
+  constexpr int LC = kMgCT + 6; // coarse points per axis under the region
+  __shared__ G sE[kMgNP], sR[kMgNP], sC[LC * LC * LC];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int I0 = 1 + kMgCT * (b % T), J0 = 1 + kMgCT * ((b / T) % T), K0 = 1 + kMgCT * (b / (T * T));
+  const int fx = 2 * I0 - 5, fy = 2 * J0 - 5, fz = 2 * K0 - 5;
+  const int cx = I0 - 3, cy = J0 - 3, cz = K0 - 3; // coarse coordinates of sC (0, 0, 0)
+  const int goff = (fx + fy + fz) & 1;
+  const int mc = m / 2;
+  const unsigned c1 = (unsigned)mc + 1, c2 = c1 * c1, hc = (c2 * c1 + 1) >> 1;
+  for (int t = tid; t < LC * LC * LC; t += NT)
+  {
+    const int X = cx + t % LC, Y = cy + (t / LC) % LC, Z = cz + t / (LC * LC);
+    G v = (G)0;
+    if (X >= 0 && X <= mc && Y >= 0 && Y <= mc && Z >= 0 && Z <= mc)
+    {
+      const unsigned g = (unsigned)X + c1 * (unsigned)Y + c2 * (unsigned)Z;
+      v = ec[(g & 1) * hc + (g >> 1)];
+    }
+    sC[t] = v;
+  }
+  __syncthreads();
+  const unsigned n1 = (unsigned)m + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
+  for (int t = tid; t < kMgNP; t += NT)
+  {
+    const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
+    const int X = fx + x, Y = fy + y, Z = fz + z;
+    G rv = (G)0, ev = (G)0;
+    if (X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m)
+    {
+      const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
+      const unsigned gp = (g & 1) * hf + (g >> 1);
+      rv = r[gp];
+      if (mg_interior(X, Y, Z, m))
+      {
+        // the trilinear value of k_prolong_correct for corner offsets (X & 1, Y & 1, Z & 1)
+        const int q0 = ((X >> 1) - cx) + LC * (((Y >> 1) - cy) + LC * ((Z >> 1) - cz));
+        const G c000 = sC[q0], c100 = sC[q0 + 1], c010 = sC[q0 + LC], c110 = sC[q0 + 1 + LC];
+        const G c001 = sC[q0 + LC * LC], c101 = sC[q0 + 1 + LC * LC], c011 = sC[q0 + LC + LC * LC],
+                c111 = sC[q0 + 1 + LC + LC * LC];
+        const G x00 = c000 + c100, x10 = c010 + c110, x01 = c001 + c101, x11 = c011 + c111;
+        G v;
+        switch ((X & 1) | ((Y & 1) << 1) | ((Z & 1) << 2))
+        {
+        case 0: v = c000; break;
+        case 1: v = (G)0.5 * x00; break;
+        case 2: v = (G)0.5 * (c000 + c010); break;
+        case 3: v = (G)0.25 * (x00 + x10); break;
+        case 4: v = (G)0.5 * (c000 + c001); break;
+        case 5: v = (G)0.25 * (x00 + x01); break;
+        case 6: v = (G)0.25 * ((c000 + c010) + (c001 + c011)); break;
+        default: v = (G)0.125 * ((x00 + x10) + (x01 + x11)); break;
+        }
+        ev = et[gp] + v;
+      }
+    }
+    const int p = (t & 1) * HF + (t >> 1);
+    sR[p] = rv;
+    sE[p] = ev;
+  }
+  __syncthreads();
+  const int red = goff, black = goff ^ 1;
+  mg_tile_sweep<G, 1, false>(sE, sR, red, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, 2, false>(sE, sR, black, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, 3, false>(sE, sR, red, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, 4, false>(sE, sR, black, fx, fy, fz, m, h2);
+  constexpr int OW = 2 * kMgCT;
+  for (int q = tid; q < OW * OW * OW; q += NT)
+  {
+    const int x = 4 + q % OW, y = 4 + (q / OW) % OW, z = 4 + q / (OW * OW);
+    const int X = fx + x, Y = fy + y, Z = fz + z;
+    if (!mg_interior(X, Y, Z, m))
+      continue;
+    const int idx = x + LF * y + LF2 * z;
+    const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
+    e[(g & 1) * hf + (g >> 1)] = sE[(idx & 1) * HF + (idx >> 1)];
+  }
 }
 
 // A11 head gradient on interior points (grid_acceleration.c:36-76); the faces
