@@ -17,7 +17,10 @@ residual/correction and refinement-node PCG).  The plain FP64 path is timed on
 the same workload for the speed-up.
 
 adjoint_fwd_ratio: Plummer N = 2^18, 64^3 head grid, 50 steps, checkpoints
-every 10: (checkpointed forward + reverse sweep) time over plain forward time.
+every 10: (checkpointed forward + reverse sweep) time over plain forward time,
+in steady state (a first adjoint run of the same length in the same context
+allocates its checkpoints and step records); reverse_to_forward is the reverse
+sweep alone over the plain forward.
 adjoint_amr: the same ratio with refinement levels: two Plummer spheres of
 2^23 particles each, 256^3 head grid, 3 refinement levels, the adjoint of the
 last 20 of 40 steps with checkpoints every 5, loss = final kinetic energy of
@@ -151,10 +154,12 @@ def adjoint_ratio(steps=50, every=10):
         s.sync()
         t_fwd = time.perf_counter() - t
     with core.Simulation(p, 6, 6, precision=core.PREC_MIXED) as s:
+        # steady state: a first run of the same length allocates the context's
+        # checkpoints and step records, then the state is set back
         s.force()
-        s.adjoint_forward(2, dt, 1)  # warm-up of the adjoint buffers
-        s.sync()
-    with core.Simulation(p, 6, 6, precision=core.PREC_MIXED) as s:
+        s.adjoint_forward(steps, dt, every)
+        s.adjoint_backward(np.zeros((3, p.n)), np.zeros((3, p.n)))
+        s.set_particles(p)
         s.force()
         s.sync()
         t = time.perf_counter()
@@ -166,7 +171,8 @@ def adjoint_ratio(steps=50, every=10):
         s.adjoint_backward(xb, np.zeros_like(xb))
         s.sync()
         t_ab = time.perf_counter() - t
-    return {"value": (t_af + t_ab) / t_fwd, "forward_ms_per_step": 1e3 * t_fwd / steps,
+    return {"value": (t_af + t_ab) / t_fwd, "reverse_to_forward": t_ab / t_fwd,
+            "forward_ms_per_step": 1e3 * t_fwd / steps,
             "checkpointed_forward_ms_per_step": 1e3 * t_af / steps, "reverse_ms_per_step": 1e3 * t_ab / steps,
             "config": f"Plummer N=2^18, head 64^3 (lmin=6), {steps} steps, checkpoint every {every}, mixed"}
 
@@ -206,11 +212,12 @@ def adjoint_full_combination(n_each=1 << 23, lmin=8, lmax=11, crit=8, warm=40, s
         t_fwd = time.perf_counter() - t
     log(f"    adjoint workload: forward {1e3 * t_fwd / steps:.2f} ms/step")
     with core.Simulation(mid, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        # steady state: a first run of the same length allocates the context's
+        # checkpoints and step records, then the state is set back
         s.force()
-        s.adjoint_forward(2, dt, 1)  # warm-up of the adjoint buffers
+        s.adjoint_forward(steps, dt, every)
         s.adjoint_backward(np.zeros((3, p.n)), np.zeros((3, p.n)))
-        s.sync()
-    with core.Simulation(mid, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit) as s:
+        s.set_particles(mid)
         s.force()
         s.sync()
         t = time.perf_counter()
@@ -224,7 +231,8 @@ def adjoint_full_combination(n_each=1 << 23, lmin=8, lmax=11, crit=8, warm=40, s
         s.sync()
         t_ab = time.perf_counter() - t
         ok = bool(np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)))
-    return {"value": (t_af + t_ab) / t_fwd, "forward_ms_per_step": 1e3 * t_fwd / steps,
+    return {"value": (t_af + t_ab) / t_fwd, "reverse_to_forward": t_ab / t_fwd,
+            "forward_ms_per_step": 1e3 * t_fwd / steps,
             "checkpointed_forward_ms_per_step": 1e3 * t_af / steps, "reverse_ms_per_step": 1e3 * t_ab / steps,
             "n_nodes": st["n_nodes"], "gradients_finite": ok, "dt": dt,
             "config": f"two Plummer spheres 2x2^{n_each.bit_length() - 1}, 80a box, head {1 << lmin}^3, "

@@ -198,6 +198,13 @@ class Simulation:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in particles.arrays()]
         _check(lib.qz_set_particles(self._h, *[_d(a) for a in arrs]), "qz_set_particles")
 
+    def set_particles(self, particles):
+        """Replace the particle state (qz_set_particles: the head potential history restarts)."""
+        if particles.n != self.n:
+            raise QuantizerError(f"set_particles: {particles.n} particles, the context holds {self.n}")
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in particles.arrays()]
+        _check(load_library().qz_set_particles(self._h, *[_d(a) for a in arrs]), "qz_set_particles")
+
     # ------------------------------------------------------------ lifetime
     def close(self):
         if getattr(self, "_h", None):

@@ -12,6 +12,8 @@
 // head cell ranges, grid accelerations and reflection flags of each step are
 // kept, with refinement levels also the tree and node accelerations, so the
 // reverse sweep needs no forward solve.
+#include <type_traits>
+
 #include "qz_adjoint_kernels.h"
 
 namespace qz
@@ -22,10 +24,11 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
   void *p1[] = {adj_xb,     adj_vb,     adj_ab,     adj_abA,    adj_psi,    adj_slot,   adj_mid,
                 adj_gbar[0], adj_gbar[1], adj_gbar[2], adj_rhs,    adj_nab[0], adj_nab[1], adj_nab[2],
                 adj_npb,    adj_npsi,   adj_hpb,    adj_nrhs,   adj_kind,   adj_rkey[0], adj_rkey[1],
-                adj_rval[0], adj_rval[1], adj_bpre,  adj_rsrc,   adj_rtmp};
+                adj_rval[0], adj_rval[1], adj_bpre,  adj_rsrc,   adj_rtmp,   adj_psi_prev};
   for (void *q : p1)
     (void)hipFree(q);
-  adj_xb = adj_vb = adj_ab = adj_abA = adj_psi = adj_npb = adj_npsi = adj_hpb = nullptr;
+  adj_xb = adj_vb = adj_ab = adj_abA = adj_psi = adj_psi_prev = adj_npb = adj_npsi = adj_hpb = nullptr;
+  adj_hist = 0;
   adj_slot = adj_mid = nullptr;
   adj_gbar[0] = adj_gbar[1] = adj_gbar[2] = adj_rhs = adj_nrhs = nullptr;
   adj_nab[0] = adj_nab[1] = adj_nab[2] = nullptr;
@@ -39,6 +42,8 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
   for (auto &c : ckpts)
     free_ckpt(c);
   ckpts.clear();
+  adj_nckpt = 0;
+  free_ckpt(adj_fin);
   for (auto &r : recs)
   {
     for (int a = 0; a < 3; a++)
@@ -71,14 +76,44 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
   adj_ab = dmalloc<double>(3 * (size_t)N, b);
   adj_abA = dmalloc<double>(3 * (size_t)N, b);
   adj_psi = dmalloc<double>(npts, b);
+  adj_psi_prev = dmalloc<double>(npts, b);
   adj_slot = dmalloc<P>(N, b);
   for (int a = 0; a < 3; a++)
     adj_gbar[a] = dmalloc<G>(npts, b);
   adj_rhs = dmalloc<G>(npts, b);
-  if (!adj_xb || !adj_vb || !adj_ab || !adj_abA || !adj_psi || !adj_slot || !adj_gbar[0] || !adj_gbar[1] ||
-      !adj_gbar[2] || !adj_rhs)
+  if (!adj_xb || !adj_vb || !adj_ab || !adj_abA || !adj_psi || !adj_psi_prev || !adj_slot || !adj_gbar[0] ||
+      !adj_gbar[1] || !adj_gbar[2] || !adj_rhs)
     return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
   bytes += b;
+  return true;
+}
+
+// The adjoint head solve Lap psi = phibar with zero faces: the same multigrid
+// with the adjoint convergence test.  Within a reverse sweep it starts from the
+// linear extrapolation of the last two adjoint solutions (from the last one
+// after the first step), as the potential solve does with its history.
+template <typename P, typename G> bool Engine<P, G>::adjoint_head_solve()
+{
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+  if (!adj_warm)
+  {
+    adj_hist = 0;
+    QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
+  }
+  else if (adj_hist == 2)
+    QZ_LAUNCHB("extrapolate", 32.0 * npts, k_extrapolate, nblk(npts), adj_psi, adj_psi_prev, npts);
+  if (!solve_head(adj_psi, adj_rhs, true, adj_warm))
+    return false;
+  if (adj_sweep)
+  {
+    if (adj_hist == 0 &&
+        !hip_ok(hipMemcpyAsync(adj_psi_prev, adj_psi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream),
+                "adjoint potential history"))
+      return false;
+    if (adj_hist < 2)
+      adj_hist++;
+  }
+  adj_warm = adj_sweep;
   return true;
 }
 
@@ -102,13 +137,8 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
   }
   QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
              adj_gbar[2], adj_rhs, make_cube(n + 1), 1.0 / (2.0 * H));
-  // Lap psi = phibar with zero faces: the same multigrid, adjoint convergence
-  // test; within a reverse sweep it starts from the previous step's psi
-  if (!adj_warm)
-    QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
-  if (!solve_head(adj_psi, adj_rhs, true, adj_warm))
+  if (!adjoint_head_solve())
     return false;
-  adj_warm = adj_sweep;
   const double coeff = 4.0 * cfg.G * kPi / (H * H * H);
   QZ_LAUNCHB("adj_gather", (24.0 + sizeof(P) + 4.0 + 48.0) * N + (8.0 + 3.0 * sizeof(G)) * npts,
              (k_adjoint_gather<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi, gax, gay, gaz, ab,
@@ -286,11 +316,8 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
   QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
              adj_gbar[2], adj_rhs, cb, 1.0 / (2.0 * H));
   QZ_LAUNCH("adj_head_add", k_adj_head_add<G>, nblk(npts), adj_rhs, adj_hpb, cb);
-  if (!adj_warm)
-    QZ_LAUNCHB("adj_fill", 8.0 * npts, k_fill<double>, nblk(npts), adj_psi, 0.0, npts);
-  if (!solve_head(adj_psi, adj_rhs, true, adj_warm))
+  if (!adjoint_head_solve())
     return false;
-  adj_warm = adj_sweep;
   QZ_LAUNCH("adj_gather_amr", (k_adjoint_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi,
             gax, gay, gaz, d.head_child, d.nodes, d.box, adj_npsi, (const G *)d.ax, (const G *)d.ay, (const G *)d.az,
             ab, ab + N, ab + 2 * (size_t)N, adj_xb, adj_xb + N, adj_xb + 2 * (size_t)N, N, cfg.lmin,
@@ -325,37 +352,36 @@ template <typename P, typename G> bool Engine<P, G>::force_vjp(const double *aba
 // A checkpoint: the particle state in its sorted order plus the head potential
 // and the potential history the next head solve starts from, so that a
 // recomputation from it repeats the forward steps exactly.
+// Copy the state into checkpoint c, allocating its buffers on first use (a
+// checkpoint is reused by later runs of the same context).
 template <typename P, typename G> bool Engine<P, G>::save_ckpt(Ckpt &c)
 {
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
   size_t b = 0;
+  auto get = [&](auto *&buf, size_t count) {
+    using T = std::remove_reference_t<decltype(*buf)>;
+    if (!buf)
+      buf = dmalloc<T>(count, b);
+    return buf != nullptr;
+  };
   bool ok = true;
   for (int a = 0; a < 3; a++)
-  {
-    c.x[a] = dmalloc<double>(N, b);
-    ok = ok && c.x[a] &&
+    ok = ok && get(c.x[a], N) &&
          hip_ok(hipMemcpyAsync(c.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
-  }
   for (int a = 0; a < 7; a++)
-  {
-    c.a[a] = dmalloc<P>(N, b);
-    ok = ok && c.a[a] &&
+    ok = ok && get(c.a[a], N) &&
          hip_ok(hipMemcpyAsync(c.a[a], pa[3 + a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
-  }
-  c.id = dmalloc<int>(N, b);
-  ok = ok && c.id && hip_ok(hipMemcpyAsync(c.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
-  c.phi = dmalloc<double>(npts, b);
-  ok = ok && c.phi &&
+  ok = ok && get(c.id, N) &&
+       hip_ok(hipMemcpyAsync(c.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream), "checkpoint");
+  ok = ok && get(c.phi, npts) &&
        hip_ok(hipMemcpyAsync(c.phi, phi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream), "checkpoint");
-  c.phi_prev = nullptr;
-  if (phi_prev)
-  {
-    c.phi_prev = dmalloc<double>(npts, b);
-    ok = ok && c.phi_prev &&
+  c.prev = phi_prev != nullptr;
+  if (c.prev)
+    ok = ok && get(c.phi_prev, npts) &&
          hip_ok(hipMemcpyAsync(c.phi_prev, phi_prev, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream),
                 "checkpoint");
-  }
   c.hist = phi_hist;
+  bytes += b;
   return ok || hip_ok(hipErrorOutOfMemory, "checkpoint");
 }
 
@@ -368,9 +394,9 @@ template <typename P, typename G> void Engine<P, G>::restore_ckpt(const Ckpt &c)
     (void)hipMemcpyAsync(pa[3 + a], c.a[a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(id_a, c.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(phi, c.phi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream);
-  if (c.phi_prev && phi_prev)
+  if (c.prev && phi_prev)
     (void)hipMemcpyAsync(phi_prev, c.phi_prev, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream);
-  phi_hist = c.phi_prev && phi_prev ? c.hist : 0;
+  phi_hist = c.prev && phi_prev ? c.hist : 0;
   keys_valid = false;
   acc_live = true;
 }
@@ -417,29 +443,50 @@ template <typename P, typename G> bool Engine<P, G>::ensure_recs()
 }
 
 // Record the sorted state after a force pass.
-template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r)
+template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r, bool handover)
 {
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
+  // positions and IDs carry on into the next step (copied).  With handover
+  // (the reverse sweep's recomputation, where the next force pass rebuilds
+  // keys, cell ranges and grid accelerations before anything reads them) the
+  // record takes those buffers and gives the engine its own free ones;
+  // otherwise (the forward run, whose state must stay whole) they are copied.
   for (int a = 0; a < 3; a++)
     (void)hipMemcpyAsync(r.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(r.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
-  (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
-  (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
-  G *src[3] = {gax, gay, gaz};
-  for (int a = 0; a < 3; a++)
-    (void)hipMemcpyAsync(r.g[a], src[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
+  G **gs[3] = {&gax, &gay, &gaz};
+  if (handover)
+  {
+    std::swap(r.key, key_out);
+    std::swap(r.cstart, cell_start);
+    for (int a = 0; a < 3; a++)
+      std::swap(r.g[a], *gs[a]);
+  }
+  else
+  {
+    (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
+    (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
+    for (int a = 0; a < 3; a++)
+      (void)hipMemcpyAsync(r.g[a], *gs[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
+  }
   if (has_tree())
     return snap_tree(r.tsnap);
   return true;
 }
 
 // Make a recorded state current for its VJP.
-template <typename P, typename G> bool Engine<P, G>::load_rec(const StepRec &r)
+template <typename P, typename G> bool Engine<P, G>::load_rec(StepRec &r)
 {
-  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
+  // the reverse step reads a record once: the engine takes its buffers by
+  // exchange (the record is refilled before it is read again)
   for (int a = 0; a < 3; a++)
-    (void)hipMemcpyAsync(pos[a], r.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
-  (void)hipMemcpyAsync(id_a, r.id, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
+    std::swap(pos[a], r.x[a]);
+  std::swap(id_a, r.id);
+  std::swap(key_out, r.key);
+  std::swap(cell_start, r.cstart);
+  std::swap(gax, r.g[0]);
+  std::swap(gay, r.g[1]);
+  std::swap(gaz, r.g[2]);
   if (cfg.lmin < cfg.lmax)
   {
     if (!r.tsnap)
@@ -450,11 +497,6 @@ template <typename P, typename G> bool Engine<P, G>::load_rec(const StepRec &r)
     if (!restore_tree(r.tsnap))
       return false;
   }
-  (void)hipMemcpyAsync(key_out, r.key, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
-  (void)hipMemcpyAsync(cell_start, r.cstart, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
-  G *dst[3] = {gax, gay, gaz};
-  for (int a = 0; a < 3; a++)
-    (void)hipMemcpyAsync(dst[a], r.g[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
   keys_valid = true;
   return true;
 }
@@ -470,9 +512,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
     set_error("adjoint_forward: nsteps and every must be positive");
     return false;
   }
-  for (auto &c : ckpts)
-    free_ckpt(c);
-  ckpts.clear();
+  adj_nckpt = 0; // the checkpoint buffers of earlier runs are reused
   adj_nsteps = nsteps;
   adj_every = every;
   adj_dt = dt;
@@ -486,10 +526,9 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
   {
     if (s % every == 0 && s < nsteps)
     {
-      Ckpt c{};
-      const bool ok = save_ckpt(c);
-      ckpts.push_back(c);
-      if (!ok)
+      if (adj_nckpt == (int)ckpts.size())
+        ckpts.push_back(Ckpt{});
+      if (!save_ckpt(ckpts[adj_nckpt++]))
         return false;
     }
     if (s == 0 && last == 0 && !save_rec(recs[0])) // the force pass of state 0 is current
@@ -512,7 +551,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
 template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const double *xbN, const double *vbN,
                                                                     double *xb0, double *vb0)
 {
-  if (ckpts.empty())
+  if (adj_nckpt == 0)
   {
     set_error("adjoint_backward: run qz_adjoint_forward first");
     return false;
@@ -521,18 +560,14 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   const double dt = adj_dt, h = 0.5 * dt;
   if (!masses_by_id())
     return false;
-  // keep the final state to restore it afterwards (freed on every return)
-  Ckpt fin{};
+  // keep the final state to restore it afterwards (the flags of the sweep are
+  // cleared on every return)
+  Ckpt &fin = adj_fin;
   struct FinGuard
   {
     Engine *e;
-    Ckpt *c;
-    ~FinGuard()
-    {
-      e->free_ckpt(*c);
-      e->adj_sweep = e->adj_warm = e->adj_last_recorded = false;
-    }
-  } fin_guard{this, &fin};
+    ~FinGuard() { e->adj_sweep = e->adj_warm = e->adj_last_recorded = false; }
+  } fin_guard{this};
   if (!save_ckpt(fin))
     return false;
   adj_sweep = true;
@@ -543,7 +578,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       !hip_ok(hipMemcpy(adj_vb, vbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload vbar"))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_abA, 0.0, n3);
-  for (int sc = (int)ckpts.size() - 1; sc >= 0; sc--)
+  for (int sc = adj_nckpt - 1; sc >= 0; sc--)
   {
     int c0 = sc * adj_every;
     int c1 = std::min(c0 + adj_every, adj_nsteps);
@@ -551,7 +586,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     // only, whose a_0 term closes the sweep), rec[k] = state c0 + k.  The
     // checkpoint carries a_c0, so no force pass is needed to step from it.
     // The last segment was recorded by adjoint_forward itself.
-    const bool recorded = adj_last_recorded && sc == (int)ckpts.size() - 1;
+    const bool recorded = adj_last_recorded && sc == adj_nckpt - 1;
     if (!recorded)
     {
       restore_ckpt(ckpts[sc]);
@@ -562,7 +597,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
         if (!force_fields())
           return false;
         restore_ckpt(ckpts[sc]);
-        if (!save_rec(recs[0]))
+        if (!save_rec(recs[0], true))
           return false;
       }
       for (int j = c0 + 1; j <= c1; j++)
@@ -570,7 +605,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
         refl_rec = recs[j - c0].refl;
         bool ok = update_A(dt);
         refl_rec = nullptr;
-        if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt) || !save_rec(recs[j - c0]))
+        if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt) || !save_rec(recs[j - c0], true))
           return false;
       }
     }

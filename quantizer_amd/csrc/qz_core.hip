@@ -169,6 +169,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   unsigned char *refl_rec = nullptr; // when set, update_A records reflected axes (ID order)
   // adjoint state (qz_adjoint.inl)
   double *adj_xb = nullptr, *adj_vb = nullptr, *adj_ab = nullptr, *adj_abA = nullptr, *adj_psi = nullptr;
+  double *adj_psi_prev = nullptr; // the previous adjoint head solution of a reverse sweep
+  int adj_hist = 0;               // adjoint head solutions of this reverse sweep (up to 2)
   P *adj_slot = nullptr, *adj_mid = nullptr; // a cotangent in slot order; the masses in ID order
   G *adj_gbar[3] = {nullptr, nullptr, nullptr}, *adj_rhs = nullptr;
   // refinement levels, in the grid-pool layout: Abar (3), phibar and psi in
@@ -191,6 +193,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     int *id;
     double *phi, *phi_prev; // head potential and the previous one (the solve's starting guess)
     int hist;
+    bool prev; // phi_prev holds the previous potential
   };
   struct StepRec
   {
@@ -202,7 +205,9 @@ template <typename P, typename G> struct Engine final : EngineBase
     unsigned char *refl;
     TreeSnap *tsnap; // with refinement levels: the tree and node fields of the step
   };
-  std::vector<Ckpt> ckpts;
+  std::vector<Ckpt> ckpts; // checkpoint buffers (kept for later runs); the first adj_nckpt are current
+  int adj_nckpt = 0;
+  Ckpt adj_fin{}; // the final state during a reverse sweep
   std::vector<StepRec> recs;
   int adj_nsteps = 0, adj_every = 0;
   bool adj_sweep = false; // a reverse sweep is running (its head solves warm-start)
@@ -369,8 +374,9 @@ template <typename P, typename G> struct Engine final : EngineBase
       lev_n.push_back(m);
       lev_e.push_back(dmalloc<G>(p1 * p1 * p1, bytes));
       lev_r.push_back(dmalloc<G>(p1 * p1 * p1, bytes));
-      lev_t.push_back(m < n ? dmalloc<G>(p1 * p1 * p1, bytes) : nullptr);
-      if (!lev_e.back() || !lev_r.back() || (m < n && !lev_t.back()))
+      const bool fusable = m <= kMgFusedCells && m >= 16;
+      lev_t.push_back(fusable ? dmalloc<G>(p1 * p1 * p1, bytes) : nullptr);
+      if (!lev_e.back() || !lev_r.back() || (fusable && !lev_t.back()))
         return hip_ok(hipErrorOutOfMemory, "multigrid levels");
     }
     if (lev_n.back() > 8)
@@ -582,11 +588,10 @@ template <typename P, typename G> struct Engine final : EngineBase
       timer.end(stream, "coarse_cg");
       return;
     }
-    // Below the top level (which keeps the per-level kernels: it may start from
-    // a non-zero correction) the coarsest levels run in one workgroup, and the
-    // levels of up to kMgFusedCells cells above them as two fused tile kernels
-    // each (default sweep counts); larger levels move enough data to keep the
-    // per-level kernels.
+    // Below the top level the coarsest levels run in one workgroup (zero start
+    // only), and the levels of 16 to kMgFusedCells cells (the top one too) as
+    // two fused tile kernels each with the default sweep counts; larger levels
+    // move enough data to keep the per-level kernels.
     if (l > 0 && zero_start && m <= Tail<G>::cells)
     {
       timer.begin(stream);
@@ -599,16 +604,20 @@ template <typename P, typename G> struct Engine final : EngineBase
       timer.end(stream, "vcycle_tail");
       return;
     }
-    if (l > 0 && zero_start && mg_fused && cfg.pre_sweeps == 2 && cfg.post_sweeps == 2 && m <= kMgFusedCells)
+    if (lev_t[l] && mg_fused && cfg.pre_sweeps == 2 && cfg.post_sweeps == 2)
     {
       const int T = m / (2 * kMgCT); // tiles per axis (m >= 16)
       const int mc = lev_n[l + 1];
       const size_t c1 = (size_t)mc + 1, nc = c1 * c1 * c1;
       const G h2 = (G)(1.0 / ((double)m * m));
       timer.begin(stream);
-      hipLaunchKernelGGL(k_mg_down<G>, dim3(T * T * T), dim3(kMgThreads), 0, stream, lev_t[l], lev_r[l],
-                         lev_r[l + 1], m, T, h2, (G)((double)m * m));
-      timer.end(stream, "mg_down", (double)sizeof(G) * (2.0 * np + nc));
+      if (zero_start)
+        hipLaunchKernelGGL((k_mg_down<G, true>), dim3(T * T * T), dim3(kMgThreads), 0, stream, lev_e[l], lev_t[l],
+                           lev_r[l], lev_r[l + 1], m, T, h2, (G)((double)m * m));
+      else
+        hipLaunchKernelGGL((k_mg_down<G, false>), dim3(T * T * T), dim3(kMgThreads), 0, stream, lev_e[l], lev_t[l],
+                           lev_r[l], lev_r[l + 1], m, T, h2, (G)((double)m * m));
+      timer.end(stream, "mg_down", (double)sizeof(G) * ((zero_start ? 2.0 : 3.0) * np + nc));
       vcycle(l + 1, true);
       timer.begin(stream);
       hipLaunchKernelGGL(k_mg_up<G>, dim3(T * T * T), dim3(kMgThreads), 0, stream, lev_e[l], lev_r[l], lev_t[l],
@@ -942,8 +951,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool snap_tree(TreeSnap *&t);
   bool restore_tree(const TreeSnap *t);
   void free_tree_snap(TreeSnap *&t);
-  bool save_rec(StepRec &r);
-  bool load_rec(const StepRec &r);
+  bool save_rec(StepRec &r, bool handover = false);
+  bool load_rec(StepRec &r);
+  bool adjoint_head_solve();
   bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
   bool force_vjp(const double *abar, double *xbar) override;
   bool adjoint_forward(int nsteps, double dt, int every) override;

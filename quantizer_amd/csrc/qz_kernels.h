@@ -1281,14 +1281,14 @@ __device__ inline bool mg_interior(int X, int Y, int Z, int m)
 }
 
 // One red-black half sweep on the local points of parity lp in the region
-// [S, kMgLF - 1 - S]^3 that are interior points of the level (m cells); ZS is
-// the zero-start form of the first sweep (k_rbgs with zero_start).  Slots are
-// laid out row by row, (kMgLF - 2S + 1) / 2 per row.
-template <typename G, int S, bool ZS>
+// [S, LF - 1 - S]^3 of an LF^3 LDS region (LF odd) that are interior points of
+// the level (m cells); ZS is the zero-start form of the first sweep (k_rbgs
+// with zero_start).  Slots are laid out row by row, (LF - 2S + 1) / 2 per row.
+template <typename G, int LF, int S, bool ZS>
 __device__ inline void mg_tile_sweep(G *sE, const G *sR, int lp, int fx, int fy, int fz, int m, G h2)
 {
-  constexpr int LF = kMgLF, LF2 = LF * LF, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
-  const G *o = sE + (1 - lp) * kMgHF;
+  constexpr int LF2 = LF * LF, HFL = (LF * LF2 + 1) / 2, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
+  const G *o = sE + (1 - lp) * HFL;
   for (int q = threadIdx.x; q < SLOTS; q += blockDim.x)
   {
     const int row = q / HX, u = q - row * HX;
@@ -1297,7 +1297,7 @@ __device__ inline void mg_tile_sweep(G *sE, const G *sR, int lp, int fx, int fy,
     if (x > LF - 1 - S || !mg_interior(fx + x, fy + y, fz + z, m))
       continue;
     const int idx = x + LF * y + LF2 * z;
-    const int p = lp * kMgHF + (idx >> 1);
+    const int p = lp * HFL + (idx >> 1);
     const G rv = sR[p];
     G sum = ZS ? (G)0
                : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + LF) >> 1] + o[(idx - LF) >> 1] +
@@ -1307,46 +1307,62 @@ __device__ inline void mg_tile_sweep(G *sE, const G *sR, int lp, int fx, int fy,
   __syncthreads();
 }
 
-// Pre-smoothing (two pairs from a zero start) and full-weighting restriction
-// of the residual of level l (m cells) into rc of level l + 1, for the tiles
-// of T^3 (T = m / (2 kMgCT)); writes the smoothed correction of the owned fine
-// points to e (k_mg_up's et).
-// Matches k_rbgs x4 followed by k_restrict_residual point for point.
-template <typename G>
+// Pre-smoothing (two pairs) and full-weighting restriction of the residual of
+// level l (m cells) into rc of level l + 1, for the tiles of T^3 (T = m / (2
+// kMgCT)); writes the smoothed correction of the owned fine points to et (for
+// k_mg_up).  ZS: the correction starts from zero (e is not read); otherwise
+// from e, whose first red sweep then needs one more halo point (the region is
+// kMgLF + 2 per side).  Matches k_rbgs x4 followed by k_restrict_residual point
+// for point.
+template <typename G, bool ZS>
 __global__ void __launch_bounds__(kMgThreads)
-    k_mg_down(G *__restrict__ e, const G *__restrict__ r, G *__restrict__ rc, int m, int T, G h2, G inv_h2)
+    k_mg_down(const G *__restrict__ e, G *__restrict__ et, const G *__restrict__ r, G *__restrict__ rc, int m, int T,
+              G h2, G inv_h2)
 {
-  constexpr int LF = kMgLF, LF2 = LF * LF, HF = kMgHF;
-  __shared__ G sE[kMgNP], sR[kMgNP];
+  constexpr int H = ZS ? 4 : 5;                // halo: phases before the residual
+  constexpr int LF = 2 * kMgCT + 1 + 2 * H, LF2 = LF * LF, NP = LF * LF2, HF = (NP + 1) / 2;
+  __shared__ G sE[NP], sR[NP];
   const int tid = threadIdx.x, NT = blockDim.x;
   const int b = (int)xcd_block(blockIdx.x, gridDim.x);
   const int I0 = 1 + kMgCT * (b % T), J0 = 1 + kMgCT * ((b / T) % T), K0 = 1 + kMgCT * (b / (T * T));
-  const int fx = 2 * I0 - 5, fy = 2 * J0 - 5, fz = 2 * K0 - 5; // level coordinates of local (0, 0, 0)
-  const int goff = (fx + fy + fz) & 1;                        // local parity of the red points
+  const int fx = 2 * I0 - 1 - H, fy = 2 * J0 - 1 - H, fz = 2 * K0 - 1 - H; // level coordinates of local (0, 0, 0)
+  const int goff = (fx + fy + fz) & 1;                                  // local parity of the red points
   const unsigned n1 = (unsigned)m + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
-  for (int t = tid; t < kMgNP; t += NT)
+  for (int t = tid; t < NP; t += NT)
   {
     const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
     const int X = fx + x, Y = fy + y, Z = fz + z;
-    G rv = (G)0;
+    G rv = (G)0, ev = (G)0;
     if (X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m)
     {
       const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
       rv = r[(g & 1) * hf + (g >> 1)];
+      if (!ZS && mg_interior(X, Y, Z, m))
+        ev = e[(g & 1) * hf + (g >> 1)];
     }
     const int p = (t & 1) * HF + (t >> 1);
     sR[p] = rv;
-    sE[p] = (G)0;
+    sE[p] = ev;
   }
   __syncthreads();
   const int red = goff, black = goff ^ 1;
-  mg_tile_sweep<G, 0, true>(sE, sR, red, fx, fy, fz, m, h2);
-  mg_tile_sweep<G, 1, false>(sE, sR, black, fx, fy, fz, m, h2);
-  mg_tile_sweep<G, 2, false>(sE, sR, red, fx, fy, fz, m, h2);
-  mg_tile_sweep<G, 3, false>(sE, sR, black, fx, fy, fz, m, h2);
-  // residual r - Lap e at the red points of [4, LF - 5]^3, in place of r
+  if constexpr (ZS)
   {
-    constexpr int S = 4, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
+    mg_tile_sweep<G, LF, 0, true>(sE, sR, red, fx, fy, fz, m, h2);
+    mg_tile_sweep<G, LF, 1, false>(sE, sR, black, fx, fy, fz, m, h2);
+    mg_tile_sweep<G, LF, 2, false>(sE, sR, red, fx, fy, fz, m, h2);
+    mg_tile_sweep<G, LF, 3, false>(sE, sR, black, fx, fy, fz, m, h2);
+  }
+  else
+  {
+    mg_tile_sweep<G, LF, 1, false>(sE, sR, red, fx, fy, fz, m, h2);
+    mg_tile_sweep<G, LF, 2, false>(sE, sR, black, fx, fy, fz, m, h2);
+    mg_tile_sweep<G, LF, 3, false>(sE, sR, red, fx, fy, fz, m, h2);
+    mg_tile_sweep<G, LF, 4, false>(sE, sR, black, fx, fy, fz, m, h2);
+  }
+  // residual r - Lap e at the red points of [H, LF - 1 - H]^3, in place of r
+  {
+    constexpr int S = H, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
     const G *o = sE + black * HF;
     for (int q = tid; q < SLOTS; q += NT)
     {
@@ -1361,17 +1377,17 @@ __global__ void __launch_bounds__(kMgThreads)
                    o[(idx - LF2) >> 1] + o[(idx + LF2) >> 1];
       sR[p] = sR[p] - (nb - (G)6 * sE[p]) * inv_h2;
     }
-    // the smoothed correction of the owned points [4, 3 + 2 kMgCT]^3
+    // the smoothed correction of the owned points [H, H + 2 kMgCT - 1]^3
     constexpr int OW = 2 * kMgCT;
     for (int q = tid; q < OW * OW * OW; q += NT)
     {
-      const int x = 4 + q % OW, y = 4 + (q / OW) % OW, z = 4 + q / (OW * OW);
+      const int x = H + q % OW, y = H + (q / OW) % OW, z = H + q / (OW * OW);
       const int X = fx + x, Y = fy + y, Z = fz + z;
       if (!mg_interior(X, Y, Z, m))
         continue;
       const int idx = x + LF * y + LF2 * z;
       const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
-      e[(g & 1) * hf + (g >> 1)] = sE[(idx & 1) * HF + (idx >> 1)];
+      et[(g & 1) * hf + (g >> 1)] = sE[(idx & 1) * HF + (idx >> 1)];
     }
   }
   __syncthreads();
@@ -1385,7 +1401,7 @@ __global__ void __launch_bounds__(kMgThreads)
     const int I = I0 + a, J = J0 + bb, K = K0 + c;
     if (I > mc - 1 || J > mc - 1 || K > mc - 1)
       continue;
-    const int idx = (2 * a + 5) + LF * (2 * bb + 5) + LF2 * (2 * c + 5);
+    const int idx = (2 * a + 1 + H) + LF * (2 * bb + 1 + H) + LF2 * (2 * c + 1 + H);
     auto R = [&](int d) { return res[(idx + d) >> 1]; };
     const G ctr = R(0);
     const G xy00 = R(1 + LF), xy10 = R(-1 + LF), xy01 = R(1 - LF), xy11 = R(-1 - LF);
@@ -1470,10 +1486,10 @@ __global__ void __launch_bounds__(kMgThreads)
   }
   __syncthreads();
   const int red = goff, black = goff ^ 1;
-  mg_tile_sweep<G, 1, false>(sE, sR, red, fx, fy, fz, m, h2);
-  mg_tile_sweep<G, 2, false>(sE, sR, black, fx, fy, fz, m, h2);
-  mg_tile_sweep<G, 3, false>(sE, sR, red, fx, fy, fz, m, h2);
-  mg_tile_sweep<G, 4, false>(sE, sR, black, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, kMgLF, 1, false>(sE, sR, red, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, kMgLF, 2, false>(sE, sR, black, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, kMgLF, 3, false>(sE, sR, red, fx, fy, fz, m, h2);
+  mg_tile_sweep<G, kMgLF, 4, false>(sE, sR, black, fx, fy, fz, m, h2);
   constexpr int OW = 2 * kMgCT;
   for (int q = tid; q < OW * OW * OW; q += NT)
   {
