@@ -915,6 +915,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   std::vector<BMg> &hb = d.hbmg;
   hb.assign(count, BMg{});
   int mmax = 0;
+  size_t tail_lds = 0; // LDS of the node tails (the largest over the nodes)
   long long pool = 0;
   long long npm[kMaxMgLevels] = {0};   // largest node grid at MG level m
   double ptsm[kMaxMgLevels] = {0.0};   // points over nodes at MG level m
@@ -956,6 +957,15 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       ptsm[m] += (double)np;
     }
     mmax = std::max(mmax, M);
+    // the node's LDS tail starts at the finest level whose tail fits
+    b.ts = M - 1;
+    for (int t = 0; t < M; t++)
+      if (bmg_tail_bytes<G>(b, t) <= kBmgTailLds)
+      {
+        b.ts = t;
+        break;
+      }
+    tail_lds = std::max(tail_lds, bmg_tail_bytes<G>(b, b.ts));
   }
   // ---- chunk tables: interior points per node; per MG level the linear points
   // of every node that has the level (mask set-up)
@@ -1031,8 +1041,9 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       !hip_ok(hipStreamSynchronize(stream), "regions"))
     return false;
   // active-list tables: sweeps per (level, colour) and prolongation per level over
-  // the nodes that smooth there (nlev > m + 1); restriction into level m over all
-  // nodes that have it
+  // the nodes that smooth there with the per-level kernels (m < ts: the levels
+  // from ts on run in the node's tail); restriction into level m over the nodes
+  // whose tail starts at or below it
   int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
   double act_s[kMaxMgLevels] = {0.0}, act_r[kMaxMgLevels] = {0.0}; // active points per table
   std::vector<int> reg0(count); // first region of each node
@@ -1047,7 +1058,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
     {
       sw0[m][c] = (int)tab.size();
       for (int q = 0; q < count; q++)
-        if (hb[q].nlev > m + 1)
+        if (hb[q].nlev > m + 1 && m < hb[q].ts)
         {
           const int rg = reg0[q] + 2 * m + c;
           add_chunks(q, rstart[rg], rstart[rg + 1] - rstart[rg], kBmgChunkAct, c);
@@ -1057,7 +1068,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
     }
     rs0[m] = (int)tab.size();
     for (int q = 0; q < count; q++)
-      if (hb[q].nlev > m)
+      if (hb[q].nlev > m && m <= hb[q].ts)
       {
         const int rg = reg0[q] + 2 * m;
         add_chunks(q, rstart[rg], rstart[rg + 2] - rstart[rg], kBmgChunkAct, 0);
@@ -1071,6 +1082,15 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
               "chunk table"))
     return false;
   itab = d.chunks;
+  static bool tail_attr = false; // the tail's LDS exceeds the default dynamic limit
+  if (!tail_attr)
+  {
+    if (!hip_ok(hipFuncSetAttribute((const void *)k_bmg_tail<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kBmgTailLds),
+                "node tail LDS"))
+      return false;
+    tail_attr = true;
+  }
   auto SW = [&](int m, int c) { return (const Chunk *)d.chunks + sw0[m][c]; };
   auto RS = [&](int m) { return (const Chunk *)d.chunks + rs0[m]; };
 
@@ -1099,7 +1119,10 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
           QZ_LAUNCHB("bmg_restrict", act_s[m] * (1.0 + 2.0 * gs) + act_r[m + 1] * (4.0 + gs), (k_bmg_restrict<G>),
                      rsn[m + 1], d.nodes, first, bm, RS(m + 1), d.alist, m, e, r, mk, r);
       }
-      QZ_LAUNCH("bmg_coarse", (k_bmg_coarse<G>), count, d.nodes, first, bm, e, r, mk, 100);
+      timer.begin(stream);
+      hipLaunchKernelGGL(k_bmg_tail<G>, dim3(count), dim3(kBmgTailThreads), tail_lds, stream, d.nodes, first, bm, e,
+                         r, mk, pre, post, 100);
+      timer.end(stream, "bmg_tail");
       for (int m = mmax - 2; m >= 0; m--)
       {
         const int np_ = swn[m][0] + swn[m][1];

@@ -36,6 +36,7 @@ struct BMg
   FDiv dx[kMaxMgLevels], dxy[kMaxMgLevels]; // divide by X[m], by X[m] Y[m]
   FDiv gx, gxy;                              // divide by the box grid's rd_x + 1, (rd_x + 1)(rd_y + 1)
   int nlev;
+  int ts;          // first level of the node's LDS tail (k_bmg_tail): levels ts .. nlev - 1
   int done;        // converged: outer iterations used
   int ic0, ic1;    // the node's interior chunks (entries of the interior table)
   double rz[2];    // (-r).z of the last two iterations
@@ -473,73 +474,252 @@ __global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restric
   }
 }
 
-// coarsest level: masked CG per node in one workgroup (FP64 in LDS)
+// coarsest level of a node: at most this many points (kBmgCoarseCells + 1 per axis)
 constexpr int kBmgCoarseMax = (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1);
-template <typename G>
-__global__ void __launch_bounds__(256)
-    k_bmg_coarse(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, G *__restrict__ e,
-                 const G *__restrict__ r, const unsigned char *__restrict__ mask, int iters)
+
+// ---------------------------------------------------------------- node tail in one workgroup
+// The coarse end of a node's V-cycle, levels ts .. nlev - 1, in one workgroup
+// per node with every level staged in LDS: the pre-smoothing sweeps and the
+// restriction on the way down, the masked CG of the coarsest level, the
+// prolongation and post-smoothing sweeps on the way up, separated by workgroup
+// barriers instead of launches.  Same update formulas and summation orders as
+// k_bmg_rbgs, k_bmg_restrict and k_bmg_prolong, and the CG of k_bmg_coarse.
+// The restriction into level ts (and the prolongation from it) stay with the
+// per-level kernels.  LDS: the coarsest level's FP64 CG vectors (x, r, p),
+// then per level e and r (G, colour-split as in the pools) and the active mask
+// as bits; ts is the smallest level whose tail fits kBmgTailLds.
+constexpr int kBmgTailThreads = 1024;
+constexpr size_t kBmgTailLds = 163840 - 2048; // the workgroup's LDS less the static reduction arrays
+
+template <typename G> __host__ __device__ inline size_t bmg_tail_bytes(const BMg &b, int ts)
 {
-  __shared__ double xs[kBmgCoarseMax], rs[kBmgCoarseMax], ps[kBmgCoarseMax];
+  const int mc = b.nlev - 1;
+  size_t s = 3 * sizeof(double) * (size_t)b.X[mc] * b.Y[mc] * b.Z[mc];
+  for (int m = ts; m < b.nlev; m++)
+  {
+    const size_t np = (size_t)b.X[m] * b.Y[m] * b.Z[m];
+    s += 2 * sizeof(G) * np + 4 * ((np + 31) / 32);
+    s = (s + 7) & ~(size_t)7;
+  }
+  return s;
+}
+
+template <typename G>
+__global__ void __launch_bounds__(kBmgTailThreads)
+    k_bmg_tail(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, G *__restrict__ e,
+               const G *__restrict__ r, const unsigned char *__restrict__ mask, int pre, int post, int iters)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ unsigned s_off[kMaxMgLevels][3]; // byte offsets of e, r, mask bits per level
   const BMg &b = bm[blockIdx.x];
   if (b.done)
     return;
-  const int m = b.nlev - 1;
-  const int X = b.X[m], Y = b.Y[m];
-  const int XY = X * Y, np = XY * b.Z[m], hf = (np + 1) >> 1;
-  const double H = (double)(1 << m) / (double)(1 << nodes[first + blockIdx.x].lv);
-  const double inv_h2 = 1.0 / (H * H);
-  G *eb = e + b.off[m];
-  const G *rb_ = r + b.off[m];
-  const unsigned char *mb = mask + b.off[m];
-  auto P = [&](int idx) { return (idx & 1) * hf + (idx >> 1); };
-  for (int t = threadIdx.x; t < np; t += blockDim.x)
+  const int ts = b.ts, M = b.nlev, mc = M - 1;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const double lvs = (double)(1 << nodes[first + blockIdx.x].lv);
+  const int npc = b.X[mc] * b.Y[mc] * b.Z[mc];
+  double *xs = (double *)smem, *rs = xs + npc, *ps = rs + npc;
+  if (tid == 0)
   {
-    bool in = mb[P(t)] != 0;
-    xs[t] = 0.0;
-    rs[t] = in ? -(double)rb_[P(t)] : 0.0; // A = -Lap (SPD): solve A x = -r
-    ps[t] = rs[t];
+    size_t o = 3 * sizeof(double) * (size_t)npc;
+    for (int m = ts; m < M; m++)
+    {
+      const size_t np = (size_t)b.X[m] * b.Y[m] * b.Z[m];
+      s_off[m][0] = (unsigned)o;
+      s_off[m][1] = (unsigned)(o + sizeof(G) * np);
+      s_off[m][2] = (unsigned)(o + 2 * sizeof(G) * np);
+      o += 2 * sizeof(G) * np + 4 * ((np + 31) / 32);
+      o = (o + 7) & ~(size_t)7;
+    }
   }
   __syncthreads();
-  double rr = 0.0;
-  for (int t = threadIdx.x; t < np; t += blockDim.x)
-    rr += rs[t] * rs[t];
-  rr = block_reduce<0>(rr);
-  const double rr0 = rr;
-  // a preconditioner's coarsest level: residual reduced 1e4-fold is ample (the
-  // smoothed levels above leave larger errors)
-  for (int it = 0; it < iters && rr > 1e-8 * rr0 && rr > 0.0; it++)
+  auto E = [&](int m) { return (G *)(smem + s_off[m][0]); };
+  auto R = [&](int m) { return (G *)(smem + s_off[m][1]); };
+  auto Bt = [&](int m) { return (unsigned *)(smem + s_off[m][2]); };
+  auto on = [&](int m, int pos) { return (Bt(m)[pos >> 5] >> (pos & 31)) & 1u; };
+  // stage: e = 0 and the mask bits on every tail level, r of level ts
+  for (int m = ts; m < M; m++)
   {
-    double pap = 0.0;
-    double ap_local[(kBmgCoarseMax + 255) / 256];
-    int q = 0;
-    for (int t = threadIdx.x; t < np; t += blockDim.x, q++)
+    const int np = b.X[m] * b.Y[m] * b.Z[m], nw = (np + 31) >> 5;
+    G *Em = E(m);
+    for (int t = tid; t < np; t += NT)
+      Em[t] = (G)0;
+    const unsigned char *mk = mask + b.off[m];
+    for (int w = tid; w < nw; w += NT)
     {
-      double v = 0.0;
-      if (mb[P(t)])
-        v = inv_h2 * (6.0 * ps[t] - ps[t + 1] - ps[t - 1] - ps[t + X] - ps[t - X] - ps[t + XY] - ps[t - XY]);
-      ap_local[q] = v;
-      pap += ps[t] * v;
+      unsigned bits = 0;
+      const int p0 = w << 5, p1 = min(np, p0 + 32);
+      for (int p = p0; p < p1; p++)
+        bits |= (mk[p] != 0 ? 1u : 0u) << (p - p0);
+      Bt(m)[w] = bits;
     }
-    pap = block_reduce<0>(pap);
-    double alpha = rr / pap;
-    double rn = 0.0;
-    q = 0;
-    for (int t = threadIdx.x; t < np; t += blockDim.x, q++)
-    {
-      xs[t] += alpha * ps[t];
-      rs[t] -= alpha * ap_local[q];
-      rn += rs[t] * rs[t];
-    }
-    rn = block_reduce<0>(rn);
-    double beta = rn / rr;
-    for (int t = threadIdx.x; t < np; t += blockDim.x)
-      ps[t] = rs[t] + beta * ps[t];
-    rr = rn;
-    __syncthreads();
   }
-  for (int t = threadIdx.x; t < np; t += blockDim.x)
-    eb[P(t)] = (G)xs[t];
+  {
+    const int np = b.X[ts] * b.Y[ts] * b.Z[ts];
+    const G *rg = r + b.off[ts];
+    G *Rt = R(ts);
+    for (int t = tid; t < np; t += NT)
+      Rt[t] = rg[t];
+  }
+  __syncthreads();
+  auto sweep = [&](int m, int color, int zs) {
+    const double H = (double)(1 << m) / lvs;
+    const G h2 = (G)(H * H);
+    const int X = b.X[m], XY = X * b.Y[m], np = XY * b.Z[m], hf = (np + 1) >> 1;
+    const int cnt = color == 0 ? hf : np - hf;
+    G *Em = E(m);
+    const G *Rm = R(m);
+    const G *o = Em + (color == 0 ? hf : 0);
+    for (int h = tid; h < cnt; h += NT)
+    {
+      const int pos = color * hf + h;
+      if (!on(m, pos))
+        continue;
+      const G rv = Rm[pos];
+      const int idx = 2 * h + color;
+      G sum = zs ? (G)0
+                 : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + X) >> 1] + o[(idx - X) >> 1] +
+                       o[(idx + XY) >> 1] + o[(idx - XY) >> 1];
+      Em[pos] = (sum - h2 * rv) * (G)(1.0 / 6.0);
+    }
+    __syncthreads();
+  };
+  auto restrict_ = [&](int m) { // level m -> m + 1 after a sweep pair ending on black
+    const double Hf = (double)(1 << m) / lvs;
+    const G inv_h2 = (G)(1.0 / (Hf * Hf));
+    const int Xf = b.X[m], Yf = b.Y[m], XYf = Xf * Yf, hf = (XYf * b.Z[m] + 1) >> 1;
+    const int Xc = b.X[m + 1], npc_ = Xc * b.Y[m + 1] * b.Z[m + 1], hc = (npc_ + 1) >> 1;
+    const G *eR = E(m), *eB = E(m) + hf, *rR = R(m);
+    G *Rc = R(m + 1);
+    auto red_res = [&](int f) -> G {
+      const int h = f >> 1;
+      const bool act = on(m, h) != 0;
+      G lap = (eB[(f + 1) >> 1] + eB[(f - 1) >> 1] + eB[(f + Xf) >> 1] + eB[(f - Xf) >> 1] + eB[(f + XYf) >> 1] +
+               eB[(f - XYf) >> 1] - (G)6 * eR[h]) *
+              inv_h2;
+      const G v = rR[h] - lap;
+      return act ? v : (G)0;
+    };
+    for (int pos = tid; pos < npc_; pos += NT)
+    {
+      if (!on(m + 1, pos))
+        continue;
+      const int t = pos < hc ? 2 * pos : 2 * (pos - hc) + 1;
+      const int JK = (int)fdiv((unsigned)t, b.dx[m + 1]), K = (int)fdiv((unsigned)t, b.dxy[m + 1]);
+      const int I = t - JK * Xc, J = JK - K * b.Y[m + 1];
+      const int f = 2 * I + Xf * (2 * J + Yf * 2 * K);
+      G edges = red_res(f + 1 + Xf) + red_res(f + 1 - Xf) + red_res(f - 1 + Xf) + red_res(f - 1 - Xf) +
+                red_res(f + 1 + XYf) + red_res(f + 1 - XYf) + red_res(f - 1 + XYf) + red_res(f - 1 - XYf) +
+                red_res(f + Xf + XYf) + red_res(f + Xf - XYf) + red_res(f - Xf + XYf) + red_res(f - Xf - XYf);
+      Rc[pos] = (G)0.125 * red_res(f) + (G)(2.0 / 64.0) * edges;
+    }
+    __syncthreads();
+  };
+  auto prolong = [&](int m) { // level m + 1 -> m, e_m += P e_{m+1} on the active points
+    const int Xf = b.X[m], Yf = b.Y[m], npf = Xf * Yf * b.Z[m], hf = (npf + 1) >> 1;
+    const int Xc = b.X[m + 1], Yc = b.Y[m + 1], npc_ = Xc * Yc * b.Z[m + 1], hc = (npc_ + 1) >> 1;
+    const G *c = E(m + 1);
+    G *Em = E(m);
+    auto PC = [&](int i, int j, int k) {
+      const int idx = i + Xc * (j + Yc * k);
+      return (idx & 1) * hc + (idx >> 1);
+    };
+    for (int pos = tid; pos < npf; pos += NT)
+    {
+      if (!on(m, pos))
+        continue;
+      const int t = pos < hf ? 2 * pos : 2 * (pos - hf) + 1;
+      const int JK = (int)fdiv((unsigned)t, b.dx[m]), k = (int)fdiv((unsigned)t, b.dxy[m]);
+      const int i = t - JK * Xf, j = JK - k * Yf;
+      const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+      const int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (k & 1);
+      const G v = (c[PC(i0, j0, k0)] + c[PC(i1, j0, k0)] + c[PC(i0, j1, k0)] + c[PC(i1, j1, k0)] +
+                   c[PC(i0, j0, k1)] + c[PC(i1, j0, k1)] + c[PC(i0, j1, k1)] + c[PC(i1, j1, k1)]) *
+                  (G)0.125;
+      Em[pos] += v;
+    }
+    __syncthreads();
+  };
+  auto coarse = [&]() { // masked CG on level mc in FP64 (k_bmg_coarse)
+    const int X = b.X[mc], XY = X * b.Y[mc], np = npc, hf = (np + 1) >> 1;
+    const double H = (double)(1 << mc) / lvs;
+    const double inv_h2 = 1.0 / (H * H);
+    G *Em = E(mc);
+    const G *Rm = R(mc);
+    auto P = [&](int idx) { return (idx & 1) * hf + (idx >> 1); };
+    for (int t = tid; t < np; t += NT)
+    {
+      const bool in = on(mc, P(t)) != 0;
+      xs[t] = 0.0;
+      rs[t] = in ? -(double)Rm[P(t)] : 0.0; // A = -Lap (SPD): solve A x = -r
+      ps[t] = rs[t];
+    }
+    __syncthreads();
+    double rr = 0.0;
+    for (int t = tid; t < np; t += NT)
+      rr += rs[t] * rs[t];
+    rr = block_reduce_n<0>(rr);
+    const double rr0 = rr;
+    constexpr int QL = (kBmgCoarseMax + kBmgTailThreads - 1) / kBmgTailThreads;
+    for (int it = 0; it < iters && rr > 1e-8 * rr0 && rr > 0.0; it++)
+    {
+      double pap = 0.0, ap_local[QL];
+      int q = 0;
+      for (int t = tid; t < np; t += NT, q++)
+      {
+        double v = 0.0;
+        if (on(mc, P(t)))
+          v = inv_h2 * (6.0 * ps[t] - ps[t + 1] - ps[t - 1] - ps[t + X] - ps[t - X] - ps[t + XY] - ps[t - XY]);
+        ap_local[q] = v;
+        pap += ps[t] * v;
+      }
+      pap = block_reduce_n<0>(pap);
+      const double alpha = rr / pap;
+      double rn = 0.0;
+      q = 0;
+      for (int t = tid; t < np; t += NT, q++)
+      {
+        xs[t] += alpha * ps[t];
+        rs[t] -= alpha * ap_local[q];
+        rn += rs[t] * rs[t];
+      }
+      rn = block_reduce_n<0>(rn);
+      const double beta = rn / rr;
+      for (int t = tid; t < np; t += NT)
+        ps[t] = rs[t] + beta * ps[t];
+      rr = rn;
+      __syncthreads();
+    }
+    for (int t = tid; t < np; t += NT)
+      Em[P(t)] = (G)xs[t];
+    __syncthreads();
+  };
+  for (int m = ts; m < mc; m++)
+  {
+    for (int sw = 0; sw < pre; sw++)
+    {
+      sweep(m, 0, sw == 0);
+      sweep(m, 1, 0);
+    }
+    restrict_(m);
+  }
+  coarse();
+  for (int m = mc - 1; m >= ts; m--)
+  {
+    prolong(m);
+    for (int sw = 0; sw < post; sw++)
+    {
+      sweep(m, 0, 0);
+      sweep(m, 1, 0);
+    }
+  }
+  {
+    const int np = b.X[ts] * b.Y[ts] * b.Z[ts];
+    G *eg = e + b.off[ts];
+    const G *Et = E(ts);
+    for (int t = tid; t < np; t += NT)
+      eg[t] = Et[t];
+  }
 }
 
 } // namespace qz
