@@ -483,7 +483,8 @@ constexpr int kBmgCoarseMax = (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1) * (k
 // restriction on the way down, the masked CG of the coarsest level, the
 // prolongation and post-smoothing sweeps on the way up, separated by workgroup
 // barriers instead of launches.  Same update formulas and summation orders as
-// k_bmg_rbgs, k_bmg_restrict and k_bmg_prolong, and the CG of k_bmg_coarse.
+// k_bmg_rbgs, k_bmg_restrict and k_bmg_prolong; the coarsest level by masked
+// CG in FP64 to a 1e4 residual reduction.
 // The restriction into level ts (and the prolongation from it) stay with the
 // per-level kernels.  LDS: the coarsest level's FP64 CG vectors (x, r, p),
 // then per level e and r (G, colour-split as in the pools) and the active mask
@@ -640,7 +641,7 @@ __global__ void __launch_bounds__(kBmgTailThreads)
     }
     __syncthreads();
   };
-  auto coarse = [&]() { // masked CG on level mc in FP64 (k_bmg_coarse)
+  auto coarse = [&]() { // masked CG on level mc in FP64
     const int X = b.X[mc], XY = X * b.Y[mc], np = npc, hf = (np + 1) >> 1;
     const double H = (double)(1 << mc) / lvs;
     const double inv_h2 = 1.0 / (H * H);
