@@ -505,6 +505,25 @@ template <typename G> __host__ __device__ inline size_t bmg_tail_bytes(const BMg
   return s;
 }
 
+// Workgroup sum with one barrier: each wave folds its lanes, lane 0 stores the
+// wave's partial in slot set `buf` of sh, and after the barrier every thread
+// adds the partials in wave order (the same value everywhere).  Alternating
+// buf between consecutive calls keeps a call's stores clear of the previous
+// call's reads (a thread reaches them only past the intervening barrier).
+__device__ inline double tail_sum(double v, double (*sh)[kBmgTailThreads / 64], int buf)
+{
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0)
+    sh[buf][threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  const int nw = (int)((blockDim.x + 63) >> 6);
+  for (int w = 0; w < nw; w++)
+    r += sh[buf][w];
+  return r;
+}
+
 template <typename G>
 __global__ void __launch_bounds__(kBmgTailThreads)
     k_bmg_tail(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, G *__restrict__ e,
@@ -512,6 +531,7 @@ __global__ void __launch_bounds__(kBmgTailThreads)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ unsigned s_off[kMaxMgLevels][3]; // byte offsets of e, r, mask bits per level
+  __shared__ double s_red[2][kBmgTailThreads / 64]; // tail_sum partials
   const BMg &b = bm[blockIdx.x];
   if (b.done)
     return;
@@ -538,7 +558,10 @@ __global__ void __launch_bounds__(kBmgTailThreads)
   auto R = [&](int m) { return (G *)(smem + s_off[m][1]); };
   auto Bt = [&](int m) { return (unsigned *)(smem + s_off[m][2]); };
   auto on = [&](int m, int pos) { return (Bt(m)[pos >> 5] >> (pos & 31)) & 1u; };
-  // stage: e = 0 and the mask bits on every tail level, r of level ts
+  // stage: e = 0 and the mask bits on every tail level, r of level ts.  The
+  // global loads go kStageU per thread at a time, all issued before their
+  // LDS stores; a wave's 64 mask bytes become two mask words by ballot.
+  constexpr int kStageU = 8;
   for (int m = ts; m < M; m++)
   {
     const int np = b.X[m] * b.Y[m] * b.Z[m], nw = (np + 31) >> 5;
@@ -546,21 +569,53 @@ __global__ void __launch_bounds__(kBmgTailThreads)
     for (int t = tid; t < np; t += NT)
       Em[t] = (G)0;
     const unsigned char *mk = mask + b.off[m];
-    for (int w = tid; w < nw; w += NT)
+    unsigned *Bm = Bt(m);
+    for (int t0 = 0; t0 < np; t0 += kStageU * NT)
     {
-      unsigned bits = 0;
-      const int p0 = w << 5, p1 = min(np, p0 + 32);
-      for (int p = p0; p < p1; p++)
-        bits |= (mk[p] != 0 ? 1u : 0u) << (p - p0);
-      Bt(m)[w] = bits;
+      unsigned char v[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++)
+      {
+        const int p = t0 + u * NT + tid;
+        v[u] = mk[p < np ? p : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++)
+      {
+        const int p = t0 + u * NT + tid;
+        const unsigned long long bal = __ballot(p < np && v[u] != 0);
+        if ((tid & 63) == 0)
+        {
+          const int w = (t0 + u * NT + tid) >> 5;
+          if (w < nw)
+            Bm[w] = (unsigned)bal;
+          if (w + 1 < nw)
+            Bm[w + 1] = (unsigned)(bal >> 32);
+        }
+      }
     }
   }
   {
     const int np = b.X[ts] * b.Y[ts] * b.Z[ts];
     const G *rg = r + b.off[ts];
     G *Rt = R(ts);
-    for (int t = tid; t < np; t += NT)
-      Rt[t] = rg[t];
+    for (int t0 = 0; t0 < np; t0 += kStageU * NT)
+    {
+      G v[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++)
+      {
+        const int p = t0 + u * NT + tid;
+        v[u] = rg[p < np ? p : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; u++)
+      {
+        const int p = t0 + u * NT + tid;
+        if (p < np)
+          Rt[p] = v[u];
+      }
+    }
   }
   __syncthreads();
   auto sweep = [&](int m, int color, int zs) {
@@ -659,7 +714,9 @@ __global__ void __launch_bounds__(kBmgTailThreads)
     double rr = 0.0;
     for (int t = tid; t < np; t += NT)
       rr += rs[t] * rs[t];
-    rr = block_reduce_n<0>(rr);
+    int buf = 0;
+    rr = tail_sum(rr, s_red, buf);
+    buf ^= 1;
     const double rr0 = rr;
     constexpr int QL = (kBmgCoarseMax + kBmgTailThreads - 1) / kBmgTailThreads;
     for (int it = 0; it < iters && rr > 1e-8 * rr0 && rr > 0.0; it++)
@@ -674,7 +731,8 @@ __global__ void __launch_bounds__(kBmgTailThreads)
         ap_local[q] = v;
         pap += ps[t] * v;
       }
-      pap = block_reduce_n<0>(pap);
+      pap = tail_sum(pap, s_red, buf);
+      buf ^= 1;
       const double alpha = rr / pap;
       double rn = 0.0;
       q = 0;
@@ -684,7 +742,8 @@ __global__ void __launch_bounds__(kBmgTailThreads)
         rs[t] -= alpha * ap_local[q];
         rn += rs[t] * rs[t];
       }
-      rn = block_reduce_n<0>(rn);
+      rn = tail_sum(rn, s_red, buf);
+      buf ^= 1;
       const double beta = rn / rr;
       for (int t = tid; t < np; t += NT)
         ps[t] = rs[t] + beta * ps[t];

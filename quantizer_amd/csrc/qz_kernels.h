@@ -1186,10 +1186,26 @@ __global__ void __launch_bounds__(kTailThreads)
   constexpr int NPT = tail_points<NTOP>(), NP0 = (NTOP + 1) * (NTOP + 1) * (NTOP + 1), HF0 = (NP0 + 1) / 2;
   __shared__ G se[NPT], sr[NPT], res[HF0];
   __shared__ double cb[27];
+  // the top level's r: every load of the thread issued before its LDS stores
+  constexpr int SU = (NP0 + kTailThreads - 1) / kTailThreads;
+  G rv[SU];
+#pragma unroll
+  for (int q = 0; q < SU; q++)
+  {
+    const int t = threadIdx.x + q * kTailThreads;
+    rv[q] = r_top[t < NP0 ? t : 0];
+  }
   for (int t = threadIdx.x; t < NPT; t += blockDim.x)
   {
     se[t] = (G)0;
-    sr[t] = t < NP0 ? r_top[t] : (G)0;
+    sr[t] = (G)0;
+  }
+#pragma unroll
+  for (int q = 0; q < SU; q++)
+  {
+    const int t = threadIdx.x + q * kTailThreads;
+    if (t < NP0)
+      sr[t] = rv[q];
   }
   __syncthreads();
   tail_level<G, NTOP>(se, sr, res, cb, pre, post);
@@ -1328,21 +1344,42 @@ __global__ void __launch_bounds__(kMgThreads)
   const int fx = 2 * I0 - 1 - H, fy = 2 * J0 - 1 - H, fz = 2 * K0 - 1 - H; // level coordinates of local (0, 0, 0)
   const int goff = (fx + fy + fz) & 1;                                  // local parity of the red points
   const unsigned n1 = (unsigned)m + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
-  for (int t = tid; t < NP; t += NT)
+  // staging: every global load of the thread is issued before the first LDS
+  // store (a load feeding a store per iteration costs a round trip each);
+  // points outside the grid read index 0 and are replaced by zero
   {
-    const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
-    const int X = fx + x, Y = fy + y, Z = fz + z;
-    G rv = (G)0, ev = (G)0;
-    if (X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m)
+    constexpr int SU = (NP + kMgThreads - 1) / kMgThreads;
+    G rv[SU], ev[SU];
+#pragma unroll
+    for (int q = 0; q < SU; q++)
     {
-      const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
-      rv = r[(g & 1) * hf + (g >> 1)];
-      if (!ZS && mg_interior(X, Y, Z, m))
-        ev = e[(g & 1) * hf + (g >> 1)];
+      const int t = tid + q * kMgThreads;
+      const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
+      const int X = fx + x, Y = fy + y, Z = fz + z;
+      const bool in = t < NP && X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m;
+      const unsigned g = in ? (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z : 0u;
+      const unsigned gp = (g & 1) * hf + (g >> 1);
+      const G a = r[gp];
+      rv[q] = in ? a : (G)0;
+      if (!ZS)
+      {
+        const G c = e[gp];
+        ev[q] = in && mg_interior(X, Y, Z, m) ? c : (G)0;
+      }
+      else
+        ev[q] = (G)0;
     }
-    const int p = (t & 1) * HF + (t >> 1);
-    sR[p] = rv;
-    sE[p] = ev;
+#pragma unroll
+    for (int q = 0; q < SU; q++)
+    {
+      const int t = tid + q * kMgThreads;
+      if (t < NP)
+      {
+        const int p = (t & 1) * HF + (t >> 1);
+        sR[p] = rv[q];
+        sE[p] = ev[q];
+      }
+    }
   }
   __syncthreads();
   const int red = goff, black = goff ^ 1;
@@ -1434,54 +1471,76 @@ __global__ void __launch_bounds__(kMgThreads)
   const int goff = (fx + fy + fz) & 1;
   const int mc = m / 2;
   const unsigned c1 = (unsigned)mc + 1, c2 = c1 * c1, hc = (c2 * c1 + 1) >> 1;
-  for (int t = tid; t < LC * LC * LC; t += NT)
-  {
-    const int X = cx + t % LC, Y = cy + (t / LC) % LC, Z = cz + t / (LC * LC);
-    G v = (G)0;
-    if (X >= 0 && X <= mc && Y >= 0 && Y <= mc && Z >= 0 && Z <= mc)
-    {
-      const unsigned g = (unsigned)X + c1 * (unsigned)Y + c2 * (unsigned)Z;
-      v = ec[(g & 1) * hc + (g >> 1)];
-    }
-    sC[t] = v;
-  }
-  __syncthreads();
   const unsigned n1 = (unsigned)m + 1, s1 = n1, s2 = n1 * n1, hf = (s2 * n1 + 1) >> 1;
-  for (int t = tid; t < kMgNP; t += NT)
+  // staging: all global loads (coarse correction, r and the pre-smoothed
+  // correction of the region) are issued before the first LDS store; points
+  // outside the grid read index 0 and are replaced by zero
+  constexpr int CU = (LC * LC * LC + kMgThreads - 1) / kMgThreads, SU = (kMgNP + kMgThreads - 1) / kMgThreads;
+  G cv[CU], rv[SU], tv[SU];
+#pragma unroll
+  for (int q = 0; q < CU; q++)
   {
+    const int t = tid + q * kMgThreads;
+    const int X = cx + t % LC, Y = cy + (t / LC) % LC, Z = cz + t / (LC * LC);
+    const bool in = t < LC * LC * LC && X >= 0 && X <= mc && Y >= 0 && Y <= mc && Z >= 0 && Z <= mc;
+    const unsigned g = in ? (unsigned)X + c1 * (unsigned)Y + c2 * (unsigned)Z : 0u;
+    const G v = ec[(g & 1) * hc + (g >> 1)];
+    cv[q] = in ? v : (G)0;
+  }
+#pragma unroll
+  for (int q = 0; q < SU; q++)
+  {
+    const int t = tid + q * kMgThreads;
     const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
     const int X = fx + x, Y = fy + y, Z = fz + z;
-    G rv = (G)0, ev = (G)0;
-    if (X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m)
+    const bool in = t < kMgNP && X >= 0 && X <= m && Y >= 0 && Y <= m && Z >= 0 && Z <= m;
+    const unsigned g = in ? (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z : 0u;
+    const unsigned gp = (g & 1) * hf + (g >> 1);
+    const G a = r[gp], c = et[gp];
+    rv[q] = in ? a : (G)0;
+    tv[q] = in ? c : (G)0;
+  }
+#pragma unroll
+  for (int q = 0; q < CU; q++)
+  {
+    const int t = tid + q * kMgThreads;
+    if (t < LC * LC * LC)
+      sC[t] = cv[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SU; q++)
+  {
+    const int t = tid + q * kMgThreads;
+    if (t >= kMgNP)
+      continue;
+    const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
+    const int X = fx + x, Y = fy + y, Z = fz + z;
+    G ev = (G)0;
+    if (mg_interior(X, Y, Z, m))
     {
-      const unsigned g = (unsigned)X + s1 * (unsigned)Y + s2 * (unsigned)Z;
-      const unsigned gp = (g & 1) * hf + (g >> 1);
-      rv = r[gp];
-      if (mg_interior(X, Y, Z, m))
+      // the trilinear value of k_prolong_correct for corner offsets (X & 1, Y & 1, Z & 1)
+      const int q0 = ((X >> 1) - cx) + LC * (((Y >> 1) - cy) + LC * ((Z >> 1) - cz));
+      const G c000 = sC[q0], c100 = sC[q0 + 1], c010 = sC[q0 + LC], c110 = sC[q0 + 1 + LC];
+      const G c001 = sC[q0 + LC * LC], c101 = sC[q0 + 1 + LC * LC], c011 = sC[q0 + LC + LC * LC],
+              c111 = sC[q0 + 1 + LC + LC * LC];
+      const G x00 = c000 + c100, x10 = c010 + c110, x01 = c001 + c101, x11 = c011 + c111;
+      G v;
+      switch ((X & 1) | ((Y & 1) << 1) | ((Z & 1) << 2))
       {
-        // the trilinear value of k_prolong_correct for corner offsets (X & 1, Y & 1, Z & 1)
-        const int q0 = ((X >> 1) - cx) + LC * (((Y >> 1) - cy) + LC * ((Z >> 1) - cz));
-        const G c000 = sC[q0], c100 = sC[q0 + 1], c010 = sC[q0 + LC], c110 = sC[q0 + 1 + LC];
-        const G c001 = sC[q0 + LC * LC], c101 = sC[q0 + 1 + LC * LC], c011 = sC[q0 + LC + LC * LC],
-                c111 = sC[q0 + 1 + LC + LC * LC];
-        const G x00 = c000 + c100, x10 = c010 + c110, x01 = c001 + c101, x11 = c011 + c111;
-        G v;
-        switch ((X & 1) | ((Y & 1) << 1) | ((Z & 1) << 2))
-        {
-        case 0: v = c000; break;
-        case 1: v = (G)0.5 * x00; break;
-        case 2: v = (G)0.5 * (c000 + c010); break;
-        case 3: v = (G)0.25 * (x00 + x10); break;
-        case 4: v = (G)0.5 * (c000 + c001); break;
-        case 5: v = (G)0.25 * (x00 + x01); break;
-        case 6: v = (G)0.25 * ((c000 + c010) + (c001 + c011)); break;
-        default: v = (G)0.125 * ((x00 + x10) + (x01 + x11)); break;
-        }
-        ev = et[gp] + v;
+      case 0: v = c000; break;
+      case 1: v = (G)0.5 * x00; break;
+      case 2: v = (G)0.5 * (c000 + c010); break;
+      case 3: v = (G)0.25 * (x00 + x10); break;
+      case 4: v = (G)0.5 * (c000 + c001); break;
+      case 5: v = (G)0.25 * (x00 + x01); break;
+      case 6: v = (G)0.25 * ((c000 + c010) + (c001 + c011)); break;
+      default: v = (G)0.125 * ((x00 + x10) + (x01 + x11)); break;
       }
+      ev = tv[q] + v;
     }
     const int p = (t & 1) * HF + (t >> 1);
-    sR[p] = rv;
+    sR[p] = rv[q];
     sE[p] = ev;
   }
   __syncthreads();
