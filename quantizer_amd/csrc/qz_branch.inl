@@ -457,10 +457,10 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       h.poff = dn.poff;
       for (int j = 0; !dev_zones && j < h.ncell / 8; j++)
       {
-        // the zone cell is the parent of octant 0 of child cell 8j
+        // zone cell j (parent-level coordinates) holds child cells 8j .. 8j + 7
         zn[zo] = q;
         for (int a = 0; a < 3; a++)
-          zx[3 * zo + a] = h.cxyz[3 * (8 * j) + a] >> 1;
+          zx[3 * zo + a] = h.zxyz[3 * j + a];
         zo++;
       }
       d.used_zone += h.ncell / 8;
@@ -597,9 +597,9 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       th.refine_level(lr, fn, fc);
       if (timer.on || timer.host_on)
       {
-        timer.host("tree_refine_mark", th.ms_mark);
-        timer.host("tree_refine_zones", th.ms_zones);
-        timer.host("tree_refine_children", th.ms_children);
+        timer.host("tree_refine_group", th.ms_mark);
+        timer.host("tree_refine_nodes", th.ms_zones);
+        timer.host("tree_refine_append", th.ms_children);
       }
     }
     lr++;

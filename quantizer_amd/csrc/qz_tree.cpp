@@ -10,8 +10,10 @@
 #include "qz_tree.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <thread>
 
 namespace qz
 {
@@ -86,17 +88,35 @@ static void zone_fill(std::vector<int> &box, int rX, int rXY, const std::vector<
     zones_by_ref[box[ref_box[i]]].push_back(i);
 }
 
-void TreeHost::build_node_box(HNode &nd)
+// f(0) .. f(count - 1) on up to kTreeThreads host threads, indices handed out
+// one at a time (the calls must be independent of each other)
+template <typename F> static void parallel_for(int count, F &&f)
+{
+  const int nth = std::max(1, std::min<int>({kTreeThreads, count, (int)std::thread::hardware_concurrency()}));
+  if (nth == 1)
+  {
+    for (int q = 0; q < count; q++)
+      f(q);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; t++)
+    pool.emplace_back([&]() {
+      for (int q = next++; q < count; q = next++)
+        f(q);
+    });
+  for (auto &th : pool)
+    th.join();
+}
+
+void TreeHost::build_node_box(HNode &nd) const
 {
   int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
   size_t cap = (size_t)nd.rd[0] * nd.rd[1] * nd.rd[2];
   nd.box.assign(cap, -4);
-  nd.pos_of_box.assign(cap, -1);
   for (int c = 0; c < nd.ncell; c++)
-  {
-    nd.box[nd.cbox[c]] = -3;
-    nd.pos_of_box[nd.cbox[c]] = c;
-  }
+    nd.box[nd.cell_box(c)] = -3;
   if (nd.sim_contact)
   {
     int top = (1 << nd.lv) - 1;
@@ -150,37 +170,22 @@ HNode TreeHost::child_geometry(int parent_global, int parent_lv, int zone, const
 }
 
 // Child cells: 8 per zone cell in zone order, octant order x fastest
-// (tree_construction.c:1200-1260), with their box indices; the box status of
-// nodes that can refine further.
-void TreeHost::child_cells(HNode &ch, const int *zxyz, int nzcell)
+// (tree_construction.c:1200-1260), kept as the zone cells (HNode::cell_xyz and
+// cell_box expand them); the box status of nodes that can refine further.
+void TreeHost::child_cells(HNode &ch, const int *zxyz, int nzcell) const
 {
-  ch.cxyz.resize(3 * (size_t)ch.ncell);
-  ch.cbox.resize(ch.ncell);
-  int rX = ch.rd[0], rXY = ch.rd[0] * ch.rd[1];
-  for (int j = 0; j < nzcell; j++)
-  {
-    int ax = 2 * zxyz[3 * j], ay = 2 * zxyz[3 * j + 1], az = 2 * zxyz[3 * j + 2];
-    for (int kk = 0; kk < 2; kk++)
-      for (int jj = 0; jj < 2; jj++)
-        for (int ii = 0; ii < 2; ii++)
-        {
-          int c = ii + 2 * jj + 4 * kk + 8 * j;
-          int gx = ax + ii, gy = ay + jj, gz = az + kk;
-          ch.cxyz[3 * c] = gx;
-          ch.cxyz[3 * c + 1] = gy;
-          ch.cxyz[3 * c + 2] = gz;
-          ch.cbox[c] = (gx - ch.ts[0]) + (gy - ch.ts[1]) * rX + (gz - ch.ts[2]) * rXY;
-        }
-  }
+  ch.zxyz.assign(zxyz, zxyz + 3 * (size_t)nzcell);
   if (ch.lv < lmax)
     build_node_box(ch);
 }
 
-int TreeHost::make_children(int parent_global, int parent_lv, const int *pcxyz, const std::vector<int> &zone_sizes,
-                            const std::vector<std::vector<int>> &zone_cells)
+std::vector<HNode> TreeHost::make_children(int parent_global, int parent_lv, const int *pcxyz,
+                                          const std::vector<int> &zone_sizes,
+                                          const std::vector<std::vector<int>> &zone_cells) const
 {
   int nz = (int)zone_sizes.size();
-  int first = (int)nodes.size();
+  std::vector<HNode> kids;
+  kids.reserve(nz);
   std::vector<int> zx;
   for (int z = 0; z < nz; z++)
   {
@@ -197,24 +202,27 @@ int TreeHost::make_children(int parent_global, int parent_lv, const int *pcxyz, 
       }
     HNode ch = child_geometry(parent_global, parent_lv, z, bmin, bmax, m);
     child_cells(ch, zx.data(), m);
-    nodes.push_back(std::move(ch));
+    kids.push_back(std::move(ch));
   }
-  return first;
+  return kids;
 }
 
 int TreeHost::add_head_zones(int nz, const int *bbox6, const int *zsize, const int *cells)
 {
   clear();
   const int first = (int)nodes.size();
-  size_t at = 0;
+  std::vector<size_t> at(nz + 1, 0);
   for (int z = 0; z < nz; z++)
-  {
+    at[z + 1] = at[z] + zsize[z];
+  nodes.resize(first + nz);
+  // the zones' nodes are independent: built on up to kTreeThreads threads
+  auto build = [&](int z) {
     HNode ch = child_geometry(-1, lmin, z, bbox6 + 6 * z, bbox6 + 6 * z + 3, zsize[z]);
     if (cells)
-      child_cells(ch, cells + 3 * at, zsize[z]);
-    at += zsize[z];
-    nodes.push_back(std::move(ch));
-  }
+      child_cells(ch, cells + 3 * at[z], zsize[z]);
+    nodes[first + z] = std::move(ch);
+  };
+  parallel_for(nz, build);
   if (nz > 0)
   {
     lev_start.push_back(first);
@@ -223,6 +231,11 @@ int TreeHost::add_head_zones(int nz, const int *bbox6, const int *zsize, const i
   return nz;
 }
 
+// Refinement of one level: per parent with flagged cells, the marks, the zones
+// and the child nodes.  Parents are independent (each owns its box statuses
+// and makes its own children), so they are spread over up to kTreeThreads
+// host threads, handed out one at a time; the children are then appended in
+// parent order, so the tree is the same as a serial pass builds.
 int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell)
 {
   int start = lev_start[lr], cnt = lev_count[lr];
@@ -242,64 +255,60 @@ int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std:
     for (size_t f = 0; f < flag_node.size(); f++)
       fcell[at[flag_node[f]]++] = flag_cell[f];
   }
-  ms_mark += ms(tg, clk::now());
-  std::vector<unsigned char> mark;
-  for (int q = 0; q < cnt; q++)
-  {
+  const clk::time_point t0 = clk::now();
+  ms_mark = ms(tg, t0);
+  std::vector<std::vector<HNode>> kids(cnt);
+  auto refine_one = [&](int q) {
     if (fbeg[q] == fbeg[q + 1])
-      continue;
-    const clk::time_point t0 = clk::now();
-    int g = start + q;
-    HNode &nd = nodes[g];
+      return;
+    HNode &nd = nodes[start + q];
     int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
     // fill_cell_ref with n_exp = 1: each flagged cell marks the existing cells
-    // of its 3x3x3 block (box status -3 -> -1); marks per cell position, read
-    // back in position order (the reference's cell_ref order)
-    mark.assign(nd.ncell, 0);
+    // of its 3x3x3 block (box status -3 -> -1); the marked cells are then read
+    // in cell position order (the reference's cell_ref order)
     for (int f = fbeg[q]; f < fbeg[q + 1]; f++)
     {
-      const int b = nd.cbox[fcell[f]];
+      const int b = nd.cell_box(fcell[f]);
       for (int kk = -1; kk <= 1; kk++)
         for (int jj = -1; jj <= 1; jj++)
           for (int ii = -1; ii <= 1; ii++)
           {
             const int nb = b + ii + jj * rX + kk * rXY;
             if (nd.box[nb] == -3)
-            {
               nd.box[nb] = -1;
-              mark[nd.pos_of_box[nb]] = 1;
-            }
           }
     }
     std::vector<int> marked_pos;
     for (int c = 0; c < nd.ncell; c++)
-      if (mark[c])
+      if (nd.box[nd.cell_box(c)] == -1)
         marked_pos.push_back(c);
     if (marked_pos.empty())
-      continue;
+      return;
     std::vector<int> ref_box(marked_pos.size());
     std::vector<int> pc(3 * marked_pos.size());
     for (size_t t = 0; t < marked_pos.size(); t++)
     {
-      ref_box[t] = nd.cbox[marked_pos[t]];
-      for (int a = 0; a < 3; a++)
-        pc[3 * t + a] = nd.cxyz[3 * marked_pos[t] + a];
+      ref_box[t] = nd.cell_box(marked_pos[t]);
+      nd.cell_xyz(marked_pos[t], &pc[3 * t]);
     }
     std::vector<int> zone_sizes;
     std::vector<std::vector<int>> zones;
-    const clk::time_point t1 = clk::now();
     zone_fill(nd.box, rX, rXY, ref_box, zone_sizes, zones);
-    const clk::time_point t2 = clk::now();
-    int lv = nd.lv;
-    int first = make_children(g, lv, pc.data(), zone_sizes, zones);
-    HNode &nd2 = nodes[g]; // nodes may have been reallocated
-    nd2.child.clear();
-    for (int z = 0; z < (int)zone_sizes.size(); z++)
-      nd2.child.push_back(first + z);
-    ms_mark += ms(t0, t1);
-    ms_zones += ms(t1, t2);
-    ms_children += ms(t2, clk::now());
+    kids[q] = make_children(start + q, nd.lv, pc.data(), zone_sizes, zones);
+  };
+  parallel_for(cnt, refine_one);
+  const clk::time_point t1 = clk::now();
+  ms_zones = ms(t0, t1);
+  for (int q = 0; q < cnt; q++)
+  {
+    HNode &nd = nodes[start + q];
+    nd.child.clear();
+    for (size_t z = 0; z < kids[q].size(); z++)
+      nd.child.push_back((int)nodes.size() + (int)z);
+    for (auto &k : kids[q])
+      nodes.push_back(std::move(k));
   }
+  ms_children = ms(t1, clk::now());
   int made = (int)nodes.size() - next_first;
   if (made > 0)
   {

@@ -23,22 +23,39 @@ struct HNode
   int bmax[3] = {0, 0, 0};
   bool sc[3] = {false, false, false};
   bool sim_contact = false;
-  // cells: 8 per zone cell of the parent, in zone order then octant order
-  std::vector<int> cxyz;      // 3 per cell, global level-lv coordinates
-  std::vector<int> cbox;      // box index of each cell
-  std::vector<int> pos_of_box; // box index -> cell position, -1 if not a cell (built for refinable nodes)
+  // cells: 8 per zone cell of the parent, in zone order then octant order (x
+  // fastest); cell c is octant c & 7 of zone cell c >> 3
+  std::vector<int> zxyz;      // 3 per zone cell, parent-level (lv - 1) coordinates
   std::vector<int> box;       // box status (built for refinable nodes)
   std::vector<int> child;     // global node indices by zone id
-  std::vector<int> marked;    // box indices whose status this build changed
   // pools on the device (filled by the device mirror)
   long long goff = 0, boff = 0, coff = 0, poff = 0;
   int gcap = 0, bcap = 0, ncell = 0, pcap = 0;
+
+  // global level-lv coordinates and box index of cell c
+  void cell_xyz(int c, int *g) const
+  {
+    const int j = c >> 3, o = c & 7;
+    g[0] = 2 * zxyz[3 * j] + (o & 1);
+    g[1] = 2 * zxyz[3 * j + 1] + ((o >> 1) & 1);
+    g[2] = 2 * zxyz[3 * j + 2] + (o >> 2);
+  }
+  int cell_box(int c) const
+  {
+    int g[3];
+    cell_xyz(c, g);
+    return (g[0] - ts[0]) + (g[1] - ts[1]) * rd[0] + (g[2] - ts[2]) * rd[0] * rd[1];
+  }
 };
+
+constexpr int kTreeThreads = 16; // host threads of refine_level
 
 struct TreeHost
 {
   int lmin = 0, lmax = 0, n = 0, ref_crit = 8, extra = 10;
-  double ms_mark = 0.0, ms_zones = 0.0, ms_children = 0.0; // wall time of refine_level's parts (last call)
+  // wall time of refine_level's parts (last call): grouping the flags, the
+  // per-parent marks, zones and children (threaded), appending the children
+  double ms_mark = 0.0, ms_zones = 0.0, ms_children = 0.0;
   std::vector<HNode> nodes;     // refinement nodes, level order (tentacles order, tree_construction.c:2465-2495)
   std::vector<int> lev_start, lev_count; // relative level 1.. -> ranges in nodes
 
@@ -54,10 +71,10 @@ struct TreeHost
 
 private:
   HNode child_geometry(int parent_global, int parent_lv, int zone, const int *bmin, const int *bmax, int nzcell) const;
-  void child_cells(HNode &ch, const int *zone_cell_xyz, int nzcell);
-  int make_children(int parent_global, int parent_lv, const int *cell_xyz_parent, const std::vector<int> &zone_sizes,
-                    const std::vector<std::vector<int>> &zones);
-  void build_node_box(HNode &nd);
+  void child_cells(HNode &ch, const int *zone_cell_xyz, int nzcell) const;
+  std::vector<HNode> make_children(int parent_global, int parent_lv, const int *cell_xyz_parent,
+                                   const std::vector<int> &zone_sizes, const std::vector<std::vector<int>> &zones) const;
+  void build_node_box(HNode &nd) const;
 };
 
 } // namespace qz
