@@ -7,6 +7,7 @@
 //   device: child cells, box status, particle ranges, masses, grid-point lists
 // Only the compact list of flagged cells crosses to the host, and only node
 // descriptors and zone cells cross back.
+#include <memory>
 #include "qz_branch_kernels.h"
 #include "qz_branch_mg_kernels.h"
 #include "qz_tree.h"
@@ -64,6 +65,10 @@ struct TreeDev
   BMg *bmg = nullptr;
   int bmg_cap = 0;
   void *mg_e = nullptr, *mg_r = nullptr;
+  void *mg_t = nullptr; // fused node levels: the pre-smoothed correction (k_bmg_down -> k_bmg_up)
+  NTile *ntiles = nullptr; // fused node levels: tile table
+  long long ntile_cap = 0;
+  std::vector<NTile> hntiles;
   unsigned char *mg_mask = nullptr;
   long long mg_cap = 0;
   // grid-point classification: chunk table, per-chunk counts and offsets
@@ -173,7 +178,7 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
                     d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
-                    d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_mask, d->hmark, d->hlist,
+                    d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_t, d->ntiles, d->mg_mask, d->hmark, d->hlist,
                     d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
                     d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->chunks, d->alist, d->acount,
                     d->sel_tmp, d->rbeg, d->rstart, d->ctab, d->chunk0, d->ccnt,
@@ -830,16 +835,18 @@ bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long 
   {
     (void)hipFree(d.mg_e);
     (void)hipFree(d.mg_r);
+    (void)hipFree(d.mg_t);
     (void)hipFree(d.mg_mask);
     (void)hipFree(d.alist);
     (void)hipFree(d.sel_tmp);
-    d.mg_e = d.mg_r = d.sel_tmp = nullptr;
+    d.mg_e = d.mg_r = d.mg_t = d.sel_tmp = nullptr;
     d.mg_mask = nullptr;
     d.alist = nullptr;
     d.mg_cap = next_cap(d.mg_cap, pool);
     d.sel_bytes = qz_select_temp_bytes((int)d.mg_cap);
     if (!hip_ok(hipMalloc(&d.mg_e, sizeof(G) * d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc(&d.mg_r, sizeof(G) * d.mg_cap), "multigrid pool") ||
+        !hip_ok(hipMalloc(&d.mg_t, sizeof(G) * d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc((void **)&d.mg_mask, d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc((void **)&d.alist, sizeof(int) * d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc(&d.sel_tmp, std::max<size_t>(d.sel_bytes, 1)), "multigrid pool"))
@@ -910,6 +917,7 @@ template <typename P, typename G>
 bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint)
 {
   solved = false;
+  auto prep = std::make_unique<HostSpan>(timer, "bmg_prepare");
   TreeDev &d = *tree->dev;
   const int first = d.lfirst[l], count = d.lcount[l];
   std::vector<BMg> &hb = d.hbmg;
@@ -1052,13 +1060,41 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
     reg0[q] = rr;
     rr += 2 * hb[q].nlev;
   }
+  // node levels m >= 1 that the per-level kernels would smooth run as the fused
+  // pair k_bmg_down / k_bmg_up (default sweep counts): tiles per (level, node).
+  // Level 0 stays with the active-list kernels: its tiles stage about ten
+  // times the points they own, which costs more there than the launches save.
+  const int inner = 1; // one V-cycle per PCG iteration (a second one costs more than the iterations it saves)
+  const bool nfuse = node_mg_fused && cfg.pre_sweeps == 2 && cfg.post_sweeps == 2;
+  auto fused = [&](int m) { return nfuse && m >= 1; };
+  int nt0[kMaxMgLevels] = {0}, ntn[kMaxMgLevels] = {0};
+  std::vector<NTile> &ntab = d.hntiles;
+  ntab.clear();
+  for (int m = 1; m < mmax; m++)
+  {
+    nt0[m] = (int)ntab.size();
+    for (int q = 0; fused(m) && q < count; q++)
+      if (hb[q].nlev > m + 1 && m < hb[q].ts)
+      {
+        // tiles of kNmCT coarse points cover the fine points [1, xm] per axis
+        // that can be active: x 2^m is an interior box point, so x <= (rd - 1) >> m
+        const DNode &hn = d.hnodes[first + q];
+        auto tiles = [&](int a, int Xm) { return (std::min(Xm - 2, (hn.rd[a] - 1) >> m) + 7) / 8; };
+        const int tx = tiles(0, hb[q].X[m]), ty = tiles(1, hb[q].Y[m]), tz = tiles(2, hb[q].Z[m]);
+        for (int c = 0; c < tz; c++)
+          for (int b2 = 0; b2 < ty; b2++)
+            for (int a = 0; a < tx; a++)
+              ntab.push_back(NTile{q, 1 + kNmCT * a, 1 + kNmCT * b2, 1 + kNmCT * c});
+      }
+    ntn[m] = (int)ntab.size() - nt0[m];
+  }
   for (int m = 0; m < mmax; m++)
   {
     for (int c = 0; c < 2; c++)
     {
       sw0[m][c] = (int)tab.size();
       for (int q = 0; q < count; q++)
-        if (hb[q].nlev > m + 1 && m < hb[q].ts)
+        if (hb[q].nlev > m + 1 && m < hb[q].ts && !fused(m))
         {
           const int rg = reg0[q] + 2 * m + c;
           add_chunks(q, rstart[rg], rstart[rg + 1] - rstart[rg], kBmgChunkAct, c);
@@ -1068,7 +1104,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
     }
     rs0[m] = (int)tab.size();
     for (int q = 0; q < count; q++)
-      if (hb[q].nlev > m && m <= hb[q].ts)
+      if (hb[q].nlev > m && m <= hb[q].ts && !(m >= 1 && fused(m - 1)))
       {
         const int rg = reg0[q] + 2 * m;
         add_chunks(q, rstart[rg], rstart[rg + 2] - rstart[rg], kBmgChunkAct, 0);
@@ -1082,6 +1118,18 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
               "chunk table"))
     return false;
   itab = d.chunks;
+  if ((long long)ntab.size() > d.ntile_cap)
+  {
+    (void)hipFree(d.ntiles);
+    d.ntile_cap = next_cap(d.ntile_cap, (long long)ntab.size());
+    if (!hip_ok(hipMalloc((void **)&d.ntiles, sizeof(NTile) * d.ntile_cap), "node tiles"))
+      return false;
+  }
+  if (!ntab.empty() &&
+      !hip_ok(hipMemcpyAsync(d.ntiles, ntab.data(), sizeof(NTile) * ntab.size(), hipMemcpyHostToDevice, stream),
+              "node tiles"))
+    return false;
+  G *et = (G *)d.mg_t;
   static bool tail_attr = false; // the tail's LDS exceeds the default dynamic limit
   if (!tail_attr)
   {
@@ -1095,7 +1143,6 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   auto RS = [&](int m) { return (const Chunk *)d.chunks + rs0[m]; };
 
   const double ni = (double)d.lnintr[l], gs = (double)sizeof(G);
-  const int inner = 1; // one V-cycle per PCG iteration (a second one costs more than the iterations it saves)
   const int pre = std::max(1, cfg.pre_sweeps), post = std::max(0, cfg.post_sweeps);
   int *it_base = d.done_count + 1;
   auto sweep = [&](int m, int color, int zs) {
@@ -1110,6 +1157,13 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
     {
       for (int m = 0; m + 1 < mmax; m++)
       {
+        if (fused(m))
+        {
+          if (ntn[m] > 0)
+            QZ_LAUNCH_T("bmg_down", kNmThreads, (k_bmg_down<G>), ntn[m], d.nodes, first, bm,
+                        (const NTile *)d.ntiles + nt0[m], m, r, et, (const unsigned char *)mk);
+          continue;
+        }
         for (int sw = 0; sw < pre; sw++)
         {
           sweep(m, 0, sw == 0 && (m > 0 || c == 0));
@@ -1125,6 +1179,14 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       timer.end(stream, "bmg_tail");
       for (int m = mmax - 2; m >= 0; m--)
       {
+        if (fused(m))
+        {
+          if (ntn[m] > 0)
+            QZ_LAUNCH_T("bmg_up", kNmThreads, (k_bmg_up<G>), ntn[m], d.nodes, first, bm,
+                        (const NTile *)d.ntiles + nt0[m], m, e, (const G *)et, (const G *)r,
+                        (const unsigned char *)mk);
+          continue;
+        }
         const int np_ = swn[m][0] + swn[m][1];
         if (np_ > 0)
           QZ_LAUNCHB("bmg_prolong", act_s[m] * (4.0 + 2.0 * gs) + act_r[m + 1] * gs, (k_bmg_prolong<G>), np_, bm,
@@ -1166,6 +1228,8 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   };
   int it = 0;
   bool all_done = false;
+  prep.reset();
+  HostSpan span_it(timer, "bmg_iterate");
   if (!iterate_batches(batch, 2, kBmgMaxOuter, count, it, all_done))
     return false;
   solved = all_done;

@@ -477,6 +477,259 @@ __global__ void k_bmg_prolong(const BMg *__restrict__ bm, const Chunk *__restric
 // coarsest level of a node: at most this many points (kBmgCoarseCells + 1 per axis)
 constexpr int kBmgCoarseMax = (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1) * (kBmgCoarseCells + 1);
 
+// ---------------------------------------------------------------- fused node levels
+// A node level m >= 1 that the tail does not hold runs, for all nodes that
+// smooth there, as two launches instead of ten: k_bmg_down (two masked
+// pre-smoothing pairs from a zero start and the masked restriction into level
+// m + 1) and k_bmg_up (the prolongation from m + 1 and two post-smoothing
+// pairs).  The tiling is that of the head's k_mg_down / k_mg_up: a tile owns
+// kNmCT^3 coarse points (I0 .. I0 + 3 per axis, coarse interior only) and the
+// fine points [2 I0 - 1, 2 I0 + 6], stages the fine region [2 I0 - 5, 2 I0 + 11]
+// (17 per axis) of the node's level in LDS with r, e and the active mask, and
+// narrows the region by one point per sweep phase, so the owned points end as
+// k_bmg_rbgs, k_bmg_restrict and k_bmg_prolong leave them (same formulas and
+// summation orders).  Points outside the node's padded box count as inactive.
+constexpr int kNmCT = 4, kNmLF = 17, kNmNP = kNmLF * kNmLF * kNmLF, kNmHF = (kNmNP + 1) / 2;
+constexpr int kNmThreads = 512;
+
+struct NTile
+{
+  int node, I0, J0, K0; // node position in the tree level, first coarse point of the tile
+};
+
+// masked red-black half sweep of the staged region [S, kNmLF - 1 - S]^3
+template <typename G, int S, bool ZS>
+__device__ inline void nm_sweep(G *sE, const G *sR, const unsigned char *sM, int lp, G h2)
+{
+  constexpr int LF = kNmLF, LF2 = LF * LF, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
+  const G *o = sE + (1 - lp) * kNmHF;
+  for (int q = threadIdx.x; q < SLOTS; q += blockDim.x)
+  {
+    const int row = q / HX, u = q - row * HX;
+    const int y = S + row % L, z = S + row / L;
+    const int x = S + ((S + y + z + lp) & 1) + 2 * u;
+    if (x > LF - 1 - S)
+      continue;
+    const int idx = x + LF * y + LF2 * z;
+    const int p = lp * kNmHF + (idx >> 1);
+    if (!sM[p])
+      continue;
+    const G rv = sR[p];
+    G sum = ZS ? (G)0
+               : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + LF) >> 1] + o[(idx - LF) >> 1] +
+                     o[(idx + LF2) >> 1] + o[(idx - LF2) >> 1];
+    sE[p] = (sum - h2 * rv) * (G)(1.0 / 6.0);
+  }
+  __syncthreads();
+}
+
+// stage r and the mask (and, for k_bmg_up, the pre-smoothed correction) of the
+// tile's fine region of node level m; all global loads before the LDS stores
+template <typename G, bool WITH_E>
+__device__ inline void nm_stage(const BMg &b, int m, int fx, int fy, int fz, const G *r, const unsigned char *mask,
+                                const G *et, G *sR, unsigned char *sM, G *sT)
+{
+  constexpr int LF = kNmLF, LF2 = LF * LF, SU = (kNmNP + kNmThreads - 1) / kNmThreads;
+  const int X = b.X[m], Y = b.Y[m], Z = b.Z[m];
+  const long long np = (long long)X * Y * Z, hf = (np + 1) >> 1;
+  const G *rb = r + b.off[m];
+  const unsigned char *mb = mask + b.off[m];
+  const G *tb = et + b.off[m];
+  G rv[SU], tv[SU];
+  unsigned char mv[SU];
+#pragma unroll
+  for (int q = 0; q < SU; q++)
+  {
+    const int t = threadIdx.x + q * kNmThreads;
+    const int x = t % LF, y = (t / LF) % LF, z = t / LF2;
+    const int gx = fx + x, gy = fy + y, gz = fz + z;
+    const bool in = t < kNmNP && gx >= 0 && gx < X && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
+    const long long g = in ? gx + (long long)X * (gy + (long long)Y * gz) : 0;
+    const long long gp = (g & 1) * hf + (g >> 1);
+    const G a = rb[gp];
+    const unsigned char c = mb[gp];
+    rv[q] = in ? a : (G)0;
+    mv[q] = in ? c : (unsigned char)0;
+    if (WITH_E)
+    {
+      // inactive points are Dirichlet zeros, whatever et holds there (points
+      // on the box faces belong to no tile and are never written)
+      const G ev = tb[gp];
+      tv[q] = in && c ? ev : (G)0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SU; q++)
+  {
+    const int t = threadIdx.x + q * kNmThreads;
+    if (t < kNmNP)
+    {
+      const int p = (t & 1) * kNmHF + (t >> 1);
+      sR[p] = rv[q];
+      sM[p] = mv[q];
+      sT[p] = WITH_E ? tv[q] : (G)0;
+    }
+  }
+}
+
+template <typename G>
+__global__ void __launch_bounds__(kNmThreads)
+    k_bmg_down(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, const NTile *__restrict__ tiles,
+               int m, G *__restrict__ r, G *__restrict__ et, const unsigned char *__restrict__ mask)
+{
+  constexpr int LF = kNmLF, LF2 = LF * LF, HF = kNmHF;
+  __shared__ G sE[kNmNP], sR[kNmNP];
+  __shared__ unsigned char sM[kNmNP];
+  const NTile tl = tiles[xcd_block(blockIdx.x, gridDim.x)];
+  const BMg &b = bm[tl.node];
+  if (b.done)
+    return;
+  const double H = (double)(1 << m) / (double)(1 << nodes[first + tl.node].lv);
+  const G h2 = (G)(H * H), inv_h2 = (G)(1.0 / (H * H));
+  const int fx = 2 * tl.I0 - 5, fy = 2 * tl.J0 - 5, fz = 2 * tl.K0 - 5;
+  const int red = (fx + fy + fz) & 1, black = red ^ 1; // local parity of the red points
+  nm_stage<G, false>(b, m, fx, fy, fz, r, mask, (const G *)nullptr, sR, sM, sE); // sE = 0
+  __syncthreads();
+  nm_sweep<G, 0, true>(sE, sR, sM, red, h2);
+  nm_sweep<G, 1, false>(sE, sR, sM, black, h2);
+  nm_sweep<G, 2, false>(sE, sR, sM, red, h2);
+  nm_sweep<G, 3, false>(sE, sR, sM, black, h2);
+  const int X = b.X[m], Y = b.Y[m], Z = b.Z[m];
+  const long long np = (long long)X * Y * Z, hf = (np + 1) >> 1;
+  {
+    // masked residual at the red points of [4, 12]^3, in place of r (k_bmg_restrict's red_res)
+    constexpr int S = 4, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
+    const G *o = sE + black * HF;
+    for (int q = threadIdx.x; q < SLOTS; q += blockDim.x)
+    {
+      const int row = q / HX, u = q - row * HX;
+      const int y = S + row % L, z = S + row / L;
+      const int x = S + ((S + y + z + red) & 1) + 2 * u;
+      if (x > LF - 1 - S)
+        continue;
+      const int idx = x + LF * y + LF2 * z;
+      const int p = red * HF + (idx >> 1);
+      const G lap = (o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + LF) >> 1] + o[(idx - LF) >> 1] +
+                     o[(idx + LF2) >> 1] + o[(idx - LF2) >> 1] - (G)6 * sE[p]) *
+                    inv_h2;
+      const G v = sR[p] - lap;
+      sR[p] = sM[p] ? v : (G)0;
+    }
+    // the smoothed correction of the owned points [4, 11]^3 inside the box
+    G *tb = et + b.off[m];
+    for (int q = threadIdx.x; q < 512; q += blockDim.x)
+    {
+      const int x = 4 + (q & 7), y = 4 + ((q >> 3) & 7), z = 4 + (q >> 6);
+      const int gx = fx + x, gy = fy + y, gz = fz + z;
+      if (gx < 0 || gx >= X || gy < 0 || gy >= Y || gz < 0 || gz >= Z)
+        continue;
+      const long long g = gx + (long long)X * (gy + (long long)Y * gz);
+      const int idx = x + LF * y + LF2 * z;
+      tb[(g & 1) * hf + (g >> 1)] = sE[(idx & 1) * HF + (idx >> 1)];
+    }
+  }
+  __syncthreads();
+  // restriction to the tile's active coarse points (k_bmg_restrict's order)
+  const int Xc = b.X[m + 1], Yc = b.Y[m + 1], Zc = b.Z[m + 1];
+  const long long npc = (long long)Xc * Yc * Zc, hc = (npc + 1) >> 1;
+  const G *res = sR + red * HF;
+  for (int q = threadIdx.x; q < kNmCT * kNmCT * kNmCT; q += blockDim.x)
+  {
+    const int I = tl.I0 + (q & 3), J = tl.J0 + ((q >> 2) & 3), K = tl.K0 + (q >> 4);
+    if (I > Xc - 2 || J > Yc - 2 || K > Zc - 2)
+      continue;
+    const long long c = I + (long long)Xc * (J + (long long)Yc * K);
+    const long long cp = b.off[m + 1] + (c & 1) * hc + (c >> 1);
+    if (!mask[cp])
+      continue;
+    const int idx = (2 * (q & 3) + 5) + LF * (2 * ((q >> 2) & 3) + 5) + LF2 * (2 * (q >> 4) + 5);
+    auto R = [&](int d) { return res[(idx + d) >> 1]; };
+    const G edges = R(1 + LF) + R(1 - LF) + R(-1 + LF) + R(-1 - LF) + R(1 + LF2) + R(1 - LF2) + R(-1 + LF2) +
+                    R(-1 - LF2) + R(LF + LF2) + R(LF - LF2) + R(-LF + LF2) + R(-LF - LF2);
+    r[cp] = (G)0.125 * R(0) + (G)(2.0 / 64.0) * edges;
+  }
+}
+
+template <typename G>
+__global__ void __launch_bounds__(kNmThreads)
+    k_bmg_up(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm, const NTile *__restrict__ tiles,
+             int m, G *__restrict__ e, const G *__restrict__ et, const G *__restrict__ r,
+             const unsigned char *__restrict__ mask)
+{
+  constexpr int LF = kNmLF, LF2 = LF * LF, HF = kNmHF;
+  constexpr int LC = kNmCT + 6; // coarse points per axis under the region
+  __shared__ G sE[kNmNP], sR[kNmNP], sC[LC * LC * LC];
+  __shared__ unsigned char sM[kNmNP];
+  const NTile tl = tiles[xcd_block(blockIdx.x, gridDim.x)];
+  const BMg &b = bm[tl.node];
+  if (b.done)
+    return;
+  const double H = (double)(1 << m) / (double)(1 << nodes[first + tl.node].lv);
+  const G h2 = (G)(H * H);
+  const int fx = 2 * tl.I0 - 5, fy = 2 * tl.J0 - 5, fz = 2 * tl.K0 - 5;
+  const int cx = tl.I0 - 3, cy = tl.J0 - 3, cz = tl.K0 - 3;
+  const int red = (fx + fy + fz) & 1, black = red ^ 1;
+  {
+    const int Xc = b.X[m + 1], Yc = b.Y[m + 1], Zc = b.Z[m + 1];
+    const long long npc = (long long)Xc * Yc * Zc, hc = (npc + 1) >> 1;
+    const G *cb = e + b.off[m + 1];
+    constexpr int CU = (LC * LC * LC + kNmThreads - 1) / kNmThreads;
+    G cv[CU];
+#pragma unroll
+    for (int q = 0; q < CU; q++)
+    {
+      const int t = threadIdx.x + q * kNmThreads;
+      const int gx = cx + t % LC, gy = cy + (t / LC) % LC, gz = cz + t / (LC * LC);
+      const bool in = t < LC * LC * LC && gx >= 0 && gx < Xc && gy >= 0 && gy < Yc && gz >= 0 && gz < Zc;
+      const long long g = in ? gx + (long long)Xc * (gy + (long long)Yc * gz) : 0;
+      const G v = cb[(g & 1) * hc + (g >> 1)];
+      cv[q] = in ? v : (G)0;
+    }
+#pragma unroll
+    for (int q = 0; q < CU; q++)
+    {
+      const int t = threadIdx.x + q * kNmThreads;
+      if (t < LC * LC * LC)
+        sC[t] = cv[q];
+    }
+  }
+  nm_stage<G, true>(b, m, fx, fy, fz, r, mask, et, sR, sM, sE); // sE = the pre-smoothed correction
+  __syncthreads();
+  // prolongation into the active fine points (k_bmg_prolong's order)
+  for (int t = threadIdx.x; t < kNmNP; t += blockDim.x)
+  {
+    const int p = (t & 1) * HF + (t >> 1);
+    if (!sM[p])
+      continue;
+    const int gx = fx + t % LF, gy = fy + (t / LF) % LF, gz = fz + t / LF2;
+    const int i0 = gx >> 1, j0 = gy >> 1, k0 = gz >> 1;
+    const int i1 = i0 + (gx & 1), j1 = j0 + (gy & 1), k1 = k0 + (gz & 1);
+    auto C = [&](int i, int j, int k) { return sC[(i - cx) + LC * ((j - cy) + LC * (k - cz))]; };
+    const G v = (C(i0, j0, k0) + C(i1, j0, k0) + C(i0, j1, k0) + C(i1, j1, k0) + C(i0, j0, k1) + C(i1, j0, k1) +
+                 C(i0, j1, k1) + C(i1, j1, k1)) *
+                (G)0.125;
+    sE[p] += v;
+  }
+  __syncthreads();
+  nm_sweep<G, 1, false>(sE, sR, sM, red, h2);
+  nm_sweep<G, 2, false>(sE, sR, sM, black, h2);
+  nm_sweep<G, 3, false>(sE, sR, sM, red, h2);
+  nm_sweep<G, 4, false>(sE, sR, sM, black, h2);
+  const int X = b.X[m], Y = b.Y[m], Z = b.Z[m];
+  const long long np = (long long)X * Y * Z, hf = (np + 1) >> 1;
+  G *ebase = e + b.off[m];
+  for (int q = threadIdx.x; q < 512; q += blockDim.x)
+  {
+    const int x = 4 + (q & 7), y = 4 + ((q >> 3) & 7), z = 4 + (q >> 6);
+    const int gx = fx + x, gy = fy + y, gz = fz + z;
+    if (gx < 0 || gx >= X || gy < 0 || gy >= Y || gz < 0 || gz >= Z)
+      continue;
+    const long long g = gx + (long long)X * (gy + (long long)Y * gz);
+    const int idx = x + LF * y + LF2 * z;
+    ebase[(g & 1) * hf + (g >> 1)] = sE[(idx & 1) * HF + (idx >> 1)];
+  }
+}
+
 // ---------------------------------------------------------------- node tail in one workgroup
 // The coarse end of a node's V-cycle, levels ts .. nlev - 1, in one workgroup
 // per node with every level staged in LDS: the pre-smoothing sweeps and the

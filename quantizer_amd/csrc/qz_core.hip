@@ -111,6 +111,13 @@ void Timer::collect()
 }
 
 #define QZ_LAUNCH(name, kern, grid, ...) QZ_LAUNCHB(name, 0.0, kern, grid, __VA_ARGS__)
+#define QZ_LAUNCH_T(name, threads, kern, grid, ...)                                                                   \
+  do                                                                                                                  \
+  {                                                                                                                   \
+    timer.begin(stream);                                                                                              \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, stream, __VA_ARGS__);                                      \
+    timer.end(stream, name, 0.0);                                                                                     \
+  } while (0)
 #define QZ_LAUNCHB(name, bytes, kern, grid, ...)                                                                      \
   do                                                                                                                  \
   {                                                                                                                   \
@@ -153,6 +160,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   std::vector<G *> lev_t; // levels below the top: the pre-smoothed correction between k_mg_down and k_mg_up
   std::vector<int> lev_n;
   bool mg_fused = true; // fused tile kernels on the levels below the top (QZ_HEAD_MG_FUSED=0 turns them off)
+  bool node_mg_fused = true; // fused node levels k_bmg_down / k_bmg_up (QZ_NODE_MG_FUSED=0 turns them off)
   double *part = nullptr; // reduction partials
   double *red = nullptr;  // device scalars
   double *hred = nullptr; // pinned host mirror
@@ -331,6 +339,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     D = std::min(cfg.lmax - cfg.lmin, 7);
     if (const char *f = std::getenv("QZ_HEAD_MG_FUSED"))
       mg_fused = f[0] != '0';
+    if (const char *f = std::getenv("QZ_NODE_MG_FUSED"))
+      node_mg_fused = f[0] != '0';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
