@@ -20,7 +20,10 @@ adjoint_fwd_ratio: Plummer N = 2^18, 64^3 head grid, 50 steps, checkpoints
 every 10: (checkpointed forward + reverse sweep) time over plain forward time,
 in steady state (a first adjoint run of the same length in the same context
 allocates its checkpoints and step records); reverse_to_forward is the reverse
-sweep alone over the plain forward.
+sweep alone over the plain forward.  Like the forward steps, whose state stays
+in HBM, the reverse sweep is timed with its cotangents (seeds and gradients)
+in device buffers; host_array_copies_ms is the PCIe time of the four (3, N)
+host arrays a caller with host memory adds on top.
 adjoint_amr: the same ratio with refinement levels: two Plummer spheres of
 2^23 particles each, 256^3 head grid, 3 refinement levels, the adjoint of the
 last 20 of 40 steps with checkpoints every 5, loss = final kinetic energy of
@@ -137,6 +140,53 @@ def roofline(sweep):
             "bytes_per_launch": by, "ms_per_launch": ms, "counter_scale": scale}
 
 
+def timed_reverse(s, xb, vb):
+    """Time adjoint_backward with the cotangents resident in HBM (uploaded to
+    device buffers before the timed region, the gradients left there), the way
+    the forward steps keep their state; then time the four host <-> device
+    copies of the (3, N) arrays that a caller with host arrays adds."""
+    import ctypes
+    import numpy as np
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    h2d, d2h = 1, 2  # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
+    xb = np.ascontiguousarray(xb, dtype=np.float64)
+    vb = np.ascontiguousarray(vb, dtype=np.float64)
+    nbytes = xb.nbytes
+    bufs = []
+    try:
+        for _ in range(4):
+            b = ctypes.c_void_p()
+            if hip.hipMalloc(ctypes.byref(b), nbytes) != 0:
+                raise RuntimeError("hipMalloc of the adjoint cotangents failed")
+            bufs.append(b)
+        gx, gv = np.empty_like(xb), np.empty_like(vb)
+
+        def copy(dst, src, kind):
+            if hip.hipMemcpy(dst, src, nbytes, kind) != 0:
+                raise RuntimeError("hipMemcpy of the adjoint cotangents failed")
+
+        copy(bufs[0], xb.ctypes.data, h2d)
+        copy(bufs[1], vb.ctypes.data, h2d)
+        s.sync()
+        t = time.perf_counter()
+        s.adjoint_backward_ptr(bufs[0].value, bufs[1].value, bufs[2].value, bufs[3].value)
+        s.sync()
+        t_ab = time.perf_counter() - t
+        t = time.perf_counter()
+        copy(bufs[0], xb.ctypes.data, h2d)
+        copy(bufs[1], vb.ctypes.data, h2d)
+        copy(gx.ctypes.data, bufs[2], d2h)
+        copy(gv.ctypes.data, bufs[3], d2h)
+        t_x = time.perf_counter() - t
+    finally:
+        for b in bufs:
+            hip.hipFree(b)
+    return t_ab, t_x, gx, gv
+
+
 def adjoint_ratio(steps=50, every=10):
     """Head-only Plummer N = 2^18 on 64^3, FP32 mixed path."""
     import numpy as np
@@ -167,13 +217,11 @@ def adjoint_ratio(steps=50, every=10):
         s.sync()
         t_af = time.perf_counter() - t
         xb = np.ones((3, p.n)) / p.n
-        t = time.perf_counter()
-        s.adjoint_backward(xb, np.zeros_like(xb))
-        s.sync()
-        t_ab = time.perf_counter() - t
+        t_ab, t_x, _, _ = timed_reverse(s, xb, np.zeros_like(xb))
     return {"value": (t_af + t_ab) / t_fwd, "reverse_to_forward": t_ab / t_fwd,
             "forward_ms_per_step": 1e3 * t_fwd / steps,
             "checkpointed_forward_ms_per_step": 1e3 * t_af / steps, "reverse_ms_per_step": 1e3 * t_ab / steps,
+            "host_array_copies_ms": 1e3 * t_x,
             "config": f"Plummer N=2^18, head 64^3 (lmin=6), {steps} steps, checkpoint every {every}, mixed"}
 
 
@@ -226,14 +274,12 @@ def adjoint_full_combination(n_each=1 << 23, lmin=8, lmax=11, crit=8, warm=40, s
         t_af = time.perf_counter() - t
         log(f"    adjoint workload: checkpointed forward {1e3 * t_af / steps:.2f} ms/step")
         vN = s.particles()[3:6]
-        t = time.perf_counter()
-        gx, gv = s.adjoint_backward(np.zeros_like(vN), w * vN)
-        s.sync()
-        t_ab = time.perf_counter() - t
+        t_ab, t_x, gx, gv = timed_reverse(s, np.zeros_like(vN), w * vN)
         ok = bool(np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)))
     return {"value": (t_af + t_ab) / t_fwd, "reverse_to_forward": t_ab / t_fwd,
             "forward_ms_per_step": 1e3 * t_fwd / steps,
             "checkpointed_forward_ms_per_step": 1e3 * t_af / steps, "reverse_ms_per_step": 1e3 * t_ab / steps,
+            "host_array_copies_ms": 1e3 * t_x,
             "n_nodes": st["n_nodes"], "gradients_finite": ok, "dt": dt,
             "config": f"two Plummer spheres 2x2^{n_each.bit_length() - 1}, 80a box, head {1 << lmin}^3, "
                       f"lmax={lmax}, refine at >= {crit}; {warm} steps, then the adjoint of the last {steps} "

@@ -141,7 +141,9 @@ int qz_timestep(qz_ctx *ctx, double cfl, double dt_max, double *dt);
  *    fixed-dt steps keeping checkpoints every checkpoint_every steps.
  *  qz_adjoint_backward: given the cotangents of the final positions and
  *    velocities, return those of the initial ones; forward states are
- *    recomputed from the checkpoints.  The context is left at the final state. */
+ *    recomputed from the checkpoints.  The context is left at the final state.
+ *  The (3, N) arrays of both calls (x, y, z rows, particle-ID order) may be
+ *  host or device memory: the copies infer the direction. */
 int qz_force_vjp(qz_ctx *ctx, const double *abar, double *xbar);
 int qz_adjoint_forward(qz_ctx *ctx, int nsteps, double dt, int checkpoint_every);
 int qz_adjoint_backward(qz_ctx *ctx, const double *xbar_n, const double *vbar_n, double *xbar_0, double *vbar_0);

@@ -367,6 +367,14 @@ class Simulation:
         _check(load_library().qz_adjoint_backward(self._h, _d(xb), _d(vb), _d(ox), _d(ov)), "qz_adjoint_backward")
         return ox.reshape(3, self.n), ov.reshape(3, self.n)
 
+    def adjoint_backward_ptr(self, xbar_n: int, vbar_n: int, xbar_0: int, vbar_0: int):
+        """adjoint_backward on caller-owned buffers of 3 N doubles given by address,
+        in host or device memory (device-resident cotangents stay off PCIe)."""
+        vp = ctypes.c_void_p
+        _check(load_library().qz_adjoint_backward(self._h, ctypes.cast(vp(xbar_n), _dp), ctypes.cast(vp(vbar_n), _dp),
+                                                  ctypes.cast(vp(xbar_0), _dp), ctypes.cast(vp(vbar_0), _dp)),
+               "qz_adjoint_backward")
+
     # ------------------------------------------------------------ timing
     def timing(self, on: bool = True, host_only: bool = False):
         mode = (2 if host_only else 1) if on else 0
@@ -374,7 +382,7 @@ class Simulation:
 
     def timing_report(self) -> dict:
         lib = load_library()
-        mx, nl = 64, 48
+        mx, nl = 128, 48
         names = ctypes.create_string_buffer(mx * nl)
         ms = np.zeros(mx)
         cnt = np.zeros(mx, dtype=np.int32)

@@ -260,6 +260,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
     }
   }
   // levels, finest first: every child is done before its parent
+  HostSpan span_lv(timer, "vjp_levels");
   std::vector<int> bpre;
   for (int l = nl - 1; l >= 1; l--)
   {
@@ -340,13 +341,13 @@ template <typename P, typename G> bool Engine<P, G>::force_vjp(const double *aba
   if (!masses_by_id() || !force_fields())
     return false;
   size_t n3 = 3 * (size_t)N;
-  if (!hip_ok(hipMemcpy(adj_ab, abar, sizeof(double) * n3, hipMemcpyHostToDevice), "upload abar"))
+  if (!hip_ok(hipMemcpy(adj_ab, abar, sizeof(double) * n3, hipMemcpyDefault), "upload abar"))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_xb, 0.0, n3);
   adj_sweep = adj_warm = false;
   if (!vjp_at_current(adj_ab))
     return false;
-  return hip_ok(hipMemcpy(xbar, adj_xb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download xbar");
+  return hip_ok(hipMemcpy(xbar, adj_xb, sizeof(double) * n3, hipMemcpyDefault), "download xbar");
 }
 
 // A checkpoint: the particle state in its sorted order plus the head potential
@@ -388,6 +389,7 @@ template <typename P, typename G> bool Engine<P, G>::save_ckpt(Ckpt &c)
 template <typename P, typename G> void Engine<P, G>::restore_ckpt(const Ckpt &c)
 {
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+  timer.begin(stream);
   for (int a = 0; a < 3; a++)
     (void)hipMemcpyAsync(pos[a], c.x[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
   for (int a = 0; a < 7; a++)
@@ -396,6 +398,7 @@ template <typename P, typename G> void Engine<P, G>::restore_ckpt(const Ckpt &c)
   (void)hipMemcpyAsync(phi, c.phi, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream);
   if (c.prev && phi_prev)
     (void)hipMemcpyAsync(phi_prev, c.phi_prev, sizeof(double) * npts, hipMemcpyDeviceToDevice, stream);
+  timer.end(stream, "ckpt_restore");
   phi_hist = c.prev && phi_prev ? c.hist : 0;
   keys_valid = false;
   acc_live = true;
@@ -451,6 +454,7 @@ template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r, bool h
   // keys, cell ranges and grid accelerations before anything reads them) the
   // record takes those buffers and gives the engine its own free ones;
   // otherwise (the forward run, whose state must stay whole) they are copied.
+  timer.begin(stream);
   for (int a = 0; a < 3; a++)
     (void)hipMemcpyAsync(r.x[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream);
   (void)hipMemcpyAsync(r.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
@@ -469,6 +473,7 @@ template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r, bool h
     for (int a = 0; a < 3; a++)
       (void)hipMemcpyAsync(r.g[a], *gs[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
   }
+  timer.end(stream, "rec_save");
   if (has_tree())
     return snap_tree(r.tsnap);
   return true;
@@ -574,8 +579,9 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   adj_warm = false;
   if (!ensure_recs())
     return false;
-  if (!hip_ok(hipMemcpy(adj_xb, xbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload xbar") ||
-      !hip_ok(hipMemcpy(adj_vb, vbN, sizeof(double) * n3, hipMemcpyHostToDevice), "upload vbar"))
+  // the cotangents may be host or device memory (the copies infer the direction)
+  if (!hip_ok(hipMemcpy(adj_xb, xbN, sizeof(double) * n3, hipMemcpyDefault), "upload xbar") ||
+      !hip_ok(hipMemcpy(adj_vb, vbN, sizeof(double) * n3, hipMemcpyDefault), "upload vbar"))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_abA, 0.0, n3);
   for (int sc = adj_nckpt - 1; sc >= 0; sc--)
@@ -589,6 +595,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     const bool recorded = adj_last_recorded && sc == adj_nckpt - 1;
     if (!recorded)
     {
+      HostSpan span_rc(timer, "adj_recompute");
       restore_ckpt(ckpts[sc]);
       if (sc == 0)
       {
@@ -615,6 +622,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       if (!load_rec(recs[j - c0]))
         return false;
       QZ_LAUNCH("adj_abar", k_adj_abar, nblk(n3), adj_vb, adj_abA, adj_ab, (int)n3, h);
+      HostSpan span_vjp(timer, "adj_vjp");
       if (!vjp_at_current(adj_ab))
         return false;
       QZ_LAUNCH("adj_kick_drift", k_adj_kick_drift, nblk(N), adj_xb, adj_vb, adj_abA, recs[j - c0].refl, N, dt);
@@ -623,13 +631,14 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   // a_0 = A(x_0) fed the first half kick
   if (!load_rec(recs[0]) || !vjp_at_current(adj_abA))
     return false;
-  if (!hip_ok(hipMemcpy(xb0, adj_xb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download xbar") ||
-      !hip_ok(hipMemcpy(vb0, adj_vb, sizeof(double) * n3, hipMemcpyDeviceToHost), "download vbar"))
+  if (!hip_ok(hipMemcpy(xb0, adj_xb, sizeof(double) * n3, hipMemcpyDefault), "download xbar") ||
+      !hip_ok(hipMemcpy(vb0, adj_vb, sizeof(double) * n3, hipMemcpyDefault), "download vbar"))
     return false;
   // back to the final forward state: its fields (and tree) are rebuilt, then the
   // state is restored once more for the exact accelerations, potential and
   // potential history (the state is in key order, so the sort of the force
   // pass leaves the order unchanged)
+  HostSpan span_fin(timer, "adj_final_force");
   restore_ckpt(fin);
   if (!force_fields())
     return false;

@@ -1044,10 +1044,13 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
                      nreg + 1, d.rstart);
   timer.end(stream, "bmg_setup");
   std::vector<int> rstart(nreg + 1);
-  if (!hip_ok(hipMemcpyAsync(rstart.data(), d.rstart, sizeof(int) * (nreg + 1), hipMemcpyDeviceToHost, stream),
-              "regions") ||
-      !hip_ok(hipStreamSynchronize(stream), "regions"))
-    return false;
+  {
+    HostSpan span_w(timer, "bmg_prepare_wait");
+    if (!hip_ok(hipMemcpyAsync(rstart.data(), d.rstart, sizeof(int) * (nreg + 1), hipMemcpyDeviceToHost, stream),
+                "regions") ||
+        !hip_ok(hipStreamSynchronize(stream), "regions"))
+      return false;
+  }
   // active-list tables: sweeps per (level, colour) and prolongation per level over
   // the nodes that smooth there with the per-level kernels (m < ts: the levels
   // from ts on run in the node's tail); restriction into level m over the nodes
@@ -1387,6 +1390,13 @@ template <typename P, typename G> bool Engine<P, G>::snap_tree(TreeSnap *&t)
     return bytes == 0 || hip_ok(hipMemcpyAsync(b.p, src, bytes, hipMemcpyDeviceToDevice, stream), "tree record");
   };
   const size_t nb = (size_t)d.used_box, nc = (size_t)d.used_cell, np = (size_t)d.used_pt, ng = (size_t)d.used_grid;
+  timer.begin(stream);
+  struct End
+  {
+    Timer &t;
+    hipStream_t s;
+    ~End() { t.end(s, "tree_snap"); }
+  } end_{timer, stream};
   t->n_nodes = tree->n_nodes;
   t->used_box = d.used_box;
   t->used_cell = d.used_cell;
@@ -1431,6 +1441,13 @@ template <typename P, typename G> bool Engine<P, G>::restore_tree(const TreeSnap
     return bytes == 0 || hip_ok(hipMemcpyAsync(dst, b.p, bytes, hipMemcpyDeviceToDevice, stream), "tree record");
   };
   const size_t nb = (size_t)t->used_box, nc = (size_t)t->used_cell, np = (size_t)t->used_pt, ng = (size_t)t->used_grid;
+  timer.begin(stream);
+  struct End
+  {
+    Timer &t;
+    hipStream_t s;
+    ~End() { t.end(s, "tree_restore"); }
+  } end_{timer, stream};
   if (!get(d.nodes, t->nodes, sizeof(DNode) * (size_t)t->n_nodes) ||
       !get(d.head_child, t->head_child, sizeof(int) * ncell_head) || !get(d.box, t->box, sizeof(int) * nb) ||
       !get(d.bstart, t->bstart, sizeof(int) * nb) || !get(d.bend, t->bend, sizeof(int) * nb) ||

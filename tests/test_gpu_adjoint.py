@@ -207,3 +207,46 @@ def test_trajectory_adjoint_single_segment(oracle, levels):
     wx, wv = o.adjoint(nsteps, dt, np.zeros_like(vbN), vbN)
     assert rel(gx, wx) < 1e-6
     assert rel(gv, wv) < 1e-6
+
+
+def test_adjoint_backward_on_device_buffers_matches_host_arrays():
+    """The cotangent arrays may live in device memory: the same reverse sweep on
+    hipMalloc'd buffers gives the host-array result bit for bit (two contexts
+    with the same history)."""
+    import ctypes
+    print("case: adjoint backward host arrays vs device buffers, two spheres N=2x1500 lmin=4 lmax=6 4 steps",
+          flush=True)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    p = two_plummer(1500, seed=4)
+    dt = plummer_dt(0.002, box=80.0)
+    rng = np.random.default_rng(8)
+    xbN, vbN = rng.standard_normal((3, p.n)), rng.standard_normal((3, p.n))
+    with core.Simulation(p, 4, 6, precision=core.PREC_MIXED, ref_criterion_ptcl=8) as s:
+        s.force()
+        s.adjoint_forward(4, dt, 2)
+        gx, gv = s.adjoint_backward(xbN, vbN)
+    # a second context with the same history, the cotangents in device memory
+    with core.Simulation(p, 4, 6, precision=core.PREC_MIXED, ref_criterion_ptcl=8) as s:
+        s.force()
+        s.adjoint_forward(4, dt, 2)
+        nb = 8 * 3 * p.n
+        bufs = []
+        try:
+            for _ in range(4):
+                b = ctypes.c_void_p()
+                assert hip.hipMalloc(ctypes.byref(b), nb) == 0
+                bufs.append(b)
+            assert hip.hipMemcpy(bufs[0], np.ascontiguousarray(xbN).ctypes.data, nb, 1) == 0
+            assert hip.hipMemcpy(bufs[1], np.ascontiguousarray(vbN).ctypes.data, nb, 1) == 0
+            s.adjoint_backward_ptr(bufs[0].value, bufs[1].value, bufs[2].value, bufs[3].value)
+            dx, dv = np.empty((3, p.n)), np.empty((3, p.n))
+            assert hip.hipMemcpy(dx.ctypes.data, bufs[2], nb, 2) == 0
+            assert hip.hipMemcpy(dv.ctypes.data, bufs[3], nb, 2) == 0
+        finally:
+            for b in bufs:
+                hip.hipFree(b)
+    np.testing.assert_array_equal(dx, gx)
+    np.testing.assert_array_equal(dv, gv)
