@@ -7,7 +7,6 @@
 //   device: child cells, box status, particle ranges, masses, grid-point lists
 // Only the compact list of flagged cells crosses to the host, and only node
 // descriptors and zone cells cross back.
-#include <memory>
 #include "qz_branch_kernels.h"
 #include "qz_branch_mg_kernels.h"
 #include "qz_tree.h"
@@ -69,6 +68,27 @@ struct TreeDev
   NTile *ntiles = nullptr; // fused node levels: tile table
   long long ntile_cap = 0;
   std::vector<NTile> hntiles;
+  // The node multigrid set-up of one tree level (bmg_prepare), kept for the
+  // solve of that level, which uses it once.
+  struct Plan
+  {
+    int level = -1;       // tree level the set-up belongs to, -1 when none is held
+    bool cg_only = false; // a box too large for the hierarchy: CG solves the level
+    bool ahead = false;   // built on the set-up stream (the solve waits for ahead_done)
+    bool nfuse = false;
+    int n_itab = 0, mmax = 0;
+    size_t tail_lds = 0;
+    int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
+    int nt0[kMaxMgLevels], ntn[kMaxMgLevels];
+    double act_s[kMaxMgLevels], act_r[kMaxMgLevels];
+  } plan;
+  // The first tree level's set-up runs ahead on a host thread and its own
+  // stream (after ahead_tree, the end of the tree build) while the head density
+  // and potential are computed.
+  std::thread ahead;
+  bool ahead_ok = true;
+  hipStream_t ahead_stream = nullptr;
+  hipEvent_t ahead_tree = nullptr, ahead_done = nullptr;
   unsigned char *mg_mask = nullptr;
   long long mg_cap = 0;
   // grid-point classification: chunk table, per-chunk counts and offsets
@@ -174,6 +194,13 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
   TreeDev *d = tree->dev;
   if (d)
   {
+    if (d->ahead.joinable())
+      d->ahead.join();
+    if (d->ahead_stream)
+      (void)hipStreamDestroy(d->ahead_stream);
+    for (hipEvent_t e : {d->ahead_tree, d->ahead_done})
+      if (e)
+        (void)hipEventDestroy(e);
     void *ptrs[] = {d->nodes, d->zone_node, d->zone_xyz, d->cell_xyz, d->cell_box, d->cstart, d->cend, d->box,
                     d->bstart, d->bend, d->intr_g, d->intr_xyz, d->bdry_g, d->bdry_xyz, d->sim_g, d->sim_xyz,
                     d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
@@ -385,6 +412,9 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         !hip_ok(hipHostMalloc((void **)&d->hflag_count, sizeof(int) * 4), "pinned"))
       return false;
   }
+  if (!join_ahead())
+    return false;
+  tree->dev->plan.level = -1;
   HostSpan span_all(timer, "tree_adaptation");
   if (!sort_particles())
     return false;
@@ -645,6 +675,10 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         !hip_ok(hipMalloc((void **)&d.part3, sizeof(double) * d.part_cap), "partials"))
       return false;
   }
+  // the first level's node multigrid set-up is host work that needs only the
+  // tree: run it during the head density and solve
+  if (setup_ahead && cfg.branch_solver == QZ_BRANCH_MG && d.lfirst.size() > 1 && !timer.on && !timer.host_on)
+    return start_ahead();
   return true;
 }
 
@@ -827,7 +861,7 @@ template <typename P, typename G> bool Engine<P, G>::branch_cg_level(size_t l)
 // Grow-only buffers of the batched node multigrid: level pools (pool points),
 // node table (count), chunk table, per-chunk partials, active list, regions.
 template <typename P, typename G>
-bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool)
+bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool, hipStream_t stream)
 {
   TreeDev &d = *tree->dev;
   const int count = (int)d.hbmg.size();
@@ -913,12 +947,14 @@ bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long 
 // from the current phi.  `phi_pool` / `rho_pool` are the unknowns and the
 // right-hand side in the grid-pool layout; `adjoint` solves start from zero
 // with zero Dirichlet values and stop on the relative residual.
-template <typename P, typename G>
-bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint)
+// Set-up of the batched node multigrid of tree level l, on `stream` (the
+// engine's stream, or the set-up stream of a run ahead): hierarchy table,
+// masks, active lists, chunk and tile tables, kept in d.plan for the solve.
+template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipStream_t stream)
 {
-  solved = false;
-  auto prep = std::make_unique<HostSpan>(timer, "bmg_prepare");
+  HostSpan prep(timer, "bmg_prepare");
   TreeDev &d = *tree->dev;
+  d.plan.level = -1;
   const int first = d.lfirst[l], count = d.lcount[l];
   std::vector<BMg> &hb = d.hbmg;
   hb.assign(count, BMg{});
@@ -944,7 +980,13 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
         M = t;
     }
     if (!M)
-      return true; // a box too large for the coarsest workgroup: CG solves this level
+    {
+      // a box too large for the coarsest workgroup: CG solves this level
+      d.plan = TreeDev::Plan{};
+      d.plan.level = (int)l;
+      d.plan.cg_only = true;
+      return true;
+    }
     BMg &b = hb[q];
     b.nlev = M;
     b.done = 0;
@@ -1000,7 +1042,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
     cn[m] = (int)tab.size() - c0[m];
   }
   const size_t dense_end = tab.size();
-  if (!ensure_bmg_buffers(n_itab, (long long)dense_end + 16LL * count * mmax + 64, pool))
+  if (!ensure_bmg_buffers(n_itab, (long long)dense_end + 16LL * count * mmax + 64, pool, stream))
     return false;
   G *e = (G *)d.mg_e, *r = (G *)d.mg_r;
   unsigned char *mk = d.mg_mask;
@@ -1067,7 +1109,6 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   // pair k_bmg_down / k_bmg_up (default sweep counts): tiles per (level, node).
   // Level 0 stays with the active-list kernels: its tiles stage about ten
   // times the points they own, which costs more there than the launches save.
-  const int inner = 1; // one V-cycle per PCG iteration (a second one costs more than the iterations it saves)
   const bool nfuse = node_mg_fused && cfg.pre_sweeps == 2 && cfg.post_sweeps == 2;
   auto fused = [&](int m) { return nfuse && m >= 1; };
   int nt0[kMaxMgLevels] = {0}, ntn[kMaxMgLevels] = {0};
@@ -1115,7 +1156,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       }
     rsn[m] = (int)tab.size() - rs0[m];
   }
-  if (!ensure_bmg_buffers(n_itab, (long long)tab.size(), pool) ||
+  if (!ensure_bmg_buffers(n_itab, (long long)tab.size(), pool, stream) ||
       !hip_ok(hipMemcpyAsync(d.chunks + dense_end, tab.data() + dense_end, sizeof(Chunk) * (tab.size() - dense_end),
                              hipMemcpyHostToDevice, stream),
               "chunk table"))
@@ -1132,7 +1173,6 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       !hip_ok(hipMemcpyAsync(d.ntiles, ntab.data(), sizeof(NTile) * ntab.size(), hipMemcpyHostToDevice, stream),
               "node tiles"))
     return false;
-  G *et = (G *)d.mg_t;
   static bool tail_attr = false; // the tail's LDS exceeds the default dynamic limit
   if (!tail_attr)
   {
@@ -1142,6 +1182,88 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       return false;
     tail_attr = true;
   }
+  TreeDev::Plan &pl = d.plan;
+  pl.cg_only = false;
+  pl.ahead = stream != this->stream;
+  pl.nfuse = nfuse;
+  pl.n_itab = n_itab;
+  pl.mmax = mmax;
+  pl.tail_lds = tail_lds;
+  std::memcpy(pl.sw0, sw0, sizeof(sw0));
+  std::memcpy(pl.swn, swn, sizeof(swn));
+  std::memcpy(pl.rs0, rs0, sizeof(rs0));
+  std::memcpy(pl.rsn, rsn, sizeof(rsn));
+  std::memcpy(pl.nt0, nt0, sizeof(nt0));
+  std::memcpy(pl.ntn, ntn, sizeof(ntn));
+  std::memcpy(pl.act_s, act_s, sizeof(act_s));
+  std::memcpy(pl.act_r, act_r, sizeof(act_r));
+  pl.level = (int)l;
+  return true;
+}
+
+// Wait for a set-up run ahead (if one is running); false when it failed.
+template <typename P, typename G> bool Engine<P, G>::join_ahead()
+{
+  if (!tree || !tree->dev || !tree->dev->ahead.joinable())
+    return true;
+  TreeDev &d = *tree->dev;
+  d.ahead.join();
+  if (!d.ahead_ok)
+    d.plan.level = -1;
+  return d.ahead_ok;
+}
+
+// Start the set-up of the first tree level on the set-up stream and a host
+// thread, ordered after the tree build on the engine's stream.
+template <typename P, typename G> bool Engine<P, G>::start_ahead()
+{
+  TreeDev &d = *tree->dev;
+  if (!d.ahead_stream && (!hip_ok(hipStreamCreate(&d.ahead_stream), "set-up stream") ||
+                          !hip_ok(hipEventCreateWithFlags(&d.ahead_tree, hipEventDisableTiming), "set-up event") ||
+                          !hip_ok(hipEventCreateWithFlags(&d.ahead_done, hipEventDisableTiming), "set-up event")))
+    return false;
+  if (!hip_ok(hipEventRecord(d.ahead_tree, stream), "set-up event"))
+    return false;
+  d.ahead_ok = true;
+  d.ahead = std::thread([this, &d]() {
+    d.ahead_ok = hip_ok(hipStreamWaitEvent(d.ahead_stream, d.ahead_tree, 0), "set-up stream") &&
+                 bmg_prepare(1, d.ahead_stream) && hip_ok(hipEventRecord(d.ahead_done, d.ahead_stream), "set-up event");
+  });
+  return true;
+}
+
+template <typename P, typename G>
+bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint)
+{
+  solved = false;
+  if (!join_ahead())
+    return false;
+  TreeDev &d = *tree->dev;
+  if (d.plan.level != (int)l && !bmg_prepare(l, stream))
+    return false;
+  const TreeDev::Plan pl = d.plan; // a set-up serves one solve
+  d.plan.level = -1;
+  if (pl.cg_only)
+    return true;
+  if (pl.ahead && !hip_ok(hipStreamWaitEvent(stream, d.ahead_done, 0), "set-up stream"))
+    return false;
+  const int first = d.lfirst[l], count = d.lcount[l];
+  const int n_itab = pl.n_itab, mmax = pl.mmax;
+  const size_t tail_lds = pl.tail_lds;
+  const auto &sw0 = pl.sw0;
+  const auto &swn = pl.swn;
+  const auto &rs0 = pl.rs0;
+  const auto &rsn = pl.rsn;
+  const auto &nt0 = pl.nt0;
+  const auto &ntn = pl.ntn;
+  const auto &act_s = pl.act_s;
+  const auto &act_r = pl.act_r;
+  auto fused = [&](int m) { return pl.nfuse && m >= 1; };
+  const int inner = 1; // one V-cycle per PCG iteration (a second one costs more than the iterations it saves)
+  G *e = (G *)d.mg_e, *r = (G *)d.mg_r, *et = (G *)d.mg_t;
+  unsigned char *mk = d.mg_mask;
+  BMg *bm = d.bmg;
+  const Chunk *itab = d.chunks;
   auto SW = [&](int m, int c) { return (const Chunk *)d.chunks + sw0[m][c]; };
   auto RS = [&](int m) { return (const Chunk *)d.chunks + rs0[m]; };
 
@@ -1231,7 +1353,6 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   };
   int it = 0;
   bool all_done = false;
-  prep.reset();
   HostSpan span_it(timer, "bmg_iterate");
   if (!iterate_batches(batch, 2, kBmgMaxOuter, count, it, all_done))
     return false;
@@ -1277,6 +1398,8 @@ template <typename P, typename G> bool Engine<P, G>::node_header(int i, int *h)
     set_error("node index out of range");
     return false;
   }
+  if (!join_ahead())
+    return false;
   // refresh the host mirror: counts and masses are filled on the device after the build
   if (i == 0 && !hip_ok(hipMemcpy(tree->dev->hnodes.data(), tree->dev->nodes, sizeof(DNode) * tree->n_nodes,
                                   hipMemcpyDeviceToHost),
@@ -1429,7 +1552,10 @@ template <typename P, typename G> bool Engine<P, G>::snap_tree(TreeSnap *&t)
 // Make a recorded tree current (into the live grow-only pools).
 template <typename P, typename G> bool Engine<P, G>::restore_tree(const TreeSnap *t)
 {
+  if (!join_ahead())
+    return false;
   TreeDev &d = *tree->dev;
+  d.plan.level = -1;
   if (t->n_nodes > d.node_cap || t->used_box > d.box_cap || t->used_cell > d.cell_cap || t->used_pt > d.pt_cap ||
       t->used_grid > d.grid_cap)
   {

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/qz.h"
@@ -161,6 +162,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   std::vector<int> lev_n;
   bool mg_fused = true; // fused tile kernels on the levels below the top (QZ_HEAD_MG_FUSED=0 turns them off)
   bool node_mg_fused = true; // fused node levels k_bmg_down / k_bmg_up (QZ_NODE_MG_FUSED=0 turns them off)
+  bool setup_ahead = true;   // first tree level's node set-up during the head solve (QZ_NODE_SETUP_AHEAD=0: in place)
   double *part = nullptr; // reduction partials
   double *red = nullptr;  // device scalars
   double *hred = nullptr; // pinned host mirror
@@ -328,6 +330,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     for (int q = 0; q < 2; q++)
       if (head_graph[q])
         (void)hipGraphExecDestroy(head_graph[q]);
+    join_ahead();
     release_adjoint();
     release_tree();
   }
@@ -341,6 +344,8 @@ template <typename P, typename G> struct Engine final : EngineBase
       mg_fused = f[0] != '0';
     if (const char *f = std::getenv("QZ_NODE_MG_FUSED"))
       node_mg_fused = f[0] != '0';
+    if (const char *f = std::getenv("QZ_NODE_SETUP_AHEAD"))
+      setup_ahead = f[0] != '0';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
@@ -976,7 +981,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool potential_branches() override;
   bool branch_cg_level(size_t l);
   bool branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint);
-  bool ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool);
+  bool bmg_prepare(size_t l, hipStream_t stream);
+  bool join_ahead() override;
+  bool start_ahead();
+  bool ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool, hipStream_t stream);
   template <typename F> bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done);
   bool branch_density();
   bool branch_grid_acceleration();
@@ -1323,6 +1331,8 @@ int qz_keys(const double *x, const double *y, const double *z, int n, int lv, co
 int qz_timing_enable(qz_ctx *ctx, int on)
 {
   QZ_CHECK_CTX(ctx);
+  if (!ctx->eng->join_ahead())
+    return 1;
   ctx->eng->timer.collect();
   ctx->eng->timer.acc.clear();
   ctx->eng->timer.on = on == 1;
