@@ -12,8 +12,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <thread>
+#include <vector>
 
 namespace qz
 {
@@ -88,26 +92,100 @@ static void zone_fill(std::vector<int> &box, int rX, int rXY, const std::vector<
     zones_by_ref[box[ref_box[i]]].push_back(i);
 }
 
-// f(0) .. f(count - 1) on up to kTreeThreads host threads, indices handed out
-// one at a time (the calls must be independent of each other)
+// Persistent host workers for the tree passes (thread start-up per pass cost
+// more than the passes of the small levels): kTreeThreads - 1 workers and the
+// calling thread take the indices of one job at a time; callers on several
+// threads are served in turn.
+class TreePool
+{
+public:
+  static TreePool &get()
+  {
+    static TreePool pool;
+    return pool;
+  }
+  int workers() const { return (int)th_.size(); }
+  void run(int count, const std::function<void(int)> &f)
+  {
+    std::lock_guard<std::mutex> one(job_);
+    {
+      std::lock_guard<std::mutex> lk(mx_);
+      f_ = &f;
+      count_ = count;
+      next_.store(0);
+      active_ = (int)th_.size();
+      gen_++;
+    }
+    wake_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mx_);
+    idle_.wait(lk, [&] { return active_ == 0; }); // every worker has left this job
+    f_ = nullptr;
+  }
+  ~TreePool()
+  {
+    {
+      std::lock_guard<std::mutex> lk(mx_);
+      stop_ = true;
+      gen_++;
+    }
+    wake_.notify_all();
+    for (auto &t : th_)
+      t.join();
+  }
+
+private:
+  TreePool()
+  {
+    const int hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nw = std::max(0, std::min(kTreeThreads, hw) - 1);
+    for (int t = 0; t < nw; t++)
+      th_.emplace_back([this]() { loop(); });
+  }
+  void work()
+  {
+    for (int q = next_++; q < count_; q = next_++)
+      (*f_)(q);
+  }
+  void loop()
+  {
+    unsigned long seen = 0;
+    for (;;)
+    {
+      std::unique_lock<std::mutex> lk(mx_);
+      wake_.wait(lk, [&] { return gen_ != seen; });
+      seen = gen_;
+      if (stop_)
+        return;
+      lk.unlock();
+      work();
+      lk.lock();
+      if (--active_ == 0)
+        idle_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex job_, mx_;
+  std::condition_variable wake_, idle_;
+  const std::function<void(int)> *f_ = nullptr;
+  int count_ = 0, active_ = 0;
+  std::atomic<int> next_{0};
+  unsigned long gen_ = 0;
+  bool stop_ = false;
+};
+
+// f(0) .. f(count - 1) on the tree workers, indices handed out one at a time
+// (the calls must be independent of each other)
 template <typename F> static void parallel_for(int count, F &&f)
 {
-  const int nth = std::max(1, std::min<int>({kTreeThreads, count, (int)std::thread::hardware_concurrency()}));
-  if (nth == 1)
+  if (count <= 1 || TreePool::get().workers() == 0)
   {
     for (int q = 0; q < count; q++)
       f(q);
     return;
   }
-  std::atomic<int> next(0);
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nth; t++)
-    pool.emplace_back([&]() {
-      for (int q = next++; q < count; q = next++)
-        f(q);
-    });
-  for (auto &th : pool)
-    th.join();
+  const std::function<void(int)> fn = [&](int q) { f(q); };
+  TreePool::get().run(count, fn);
 }
 
 void TreeHost::build_node_box(HNode &nd) const
