@@ -28,6 +28,12 @@ constexpr int kBmgMaxOuter = 60;    // outer iterations before the level is hand
 constexpr int kBmgChunkIntr = 1024; // interior points per chunk (4 per thread)
 constexpr int kBmgChunkGrid = 512;  // linear grid points per chunk (mask set-up)
 constexpr int kBmgChunkAct = 512;   // active points per chunk (sweeps, restriction, prolongation)
+// Over-relaxation of the red half sweeps of the node V-cycle above the tail
+// (red-black SOR, red points only: the black half sweep stays plain Gauss-
+// Seidel, so the black residual is zero and the red-point restriction holds).
+// 1.3 takes about 8 % fewer PCG iterations than 1 on a 2^24-particle clumps
+// workload; the tail keeps plain Gauss-Seidel.
+constexpr double kBmgRedOmega = 1.3;
 
 struct BMg
 {
@@ -357,8 +363,16 @@ __global__ void k_bmg_check(BMg *__restrict__ bm, DNode *__restrict__ nodes, int
   }
 }
 
-// red-black GS half sweep on Lap e = r at MG level m over the active points of
-// one colour (chunks of the active list: pool indices of the level's points)
+// red half sweep: e <- e + w (gs - e), from a zero start e <- w gs
+template <typename G> __device__ inline G bmg_red_relax(const G *e, long long p, G gs, int zero_start)
+{
+  const G w = (G)kBmgRedOmega;
+  return zero_start ? w * gs : e[p] + w * (gs - e[p]);
+}
+
+// red-black half sweep on Lap e = r at MG level m over the active points of one
+// colour (chunks of the active list: pool indices of the level's points); the
+// red half sweep over-relaxed by kBmgRedOmega
 template <typename G>
 __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg *__restrict__ bm,
                            const Chunk *__restrict__ tab, const int *__restrict__ alist, int m, G *__restrict__ e,
@@ -382,7 +396,8 @@ __global__ void k_bmg_rbgs(const DNode *__restrict__ nodes, int first, const BMg
     G sum = zero_start ? (G)0
                        : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + X) >> 1] + o[(idx - X) >> 1] +
                              o[(idx + XY) >> 1] + o[(idx - XY) >> 1];
-    e[p] = (sum - h2 * rv) * (G)(1.0 / 6.0);
+    const G gs = (sum - h2 * rv) * (G)(1.0 / 6.0);
+    e[p] = color != 0 ? gs : bmg_red_relax(e, p, gs, zero_start);
   }
 }
 
@@ -497,8 +512,9 @@ struct NTile
   int node, I0, J0, K0; // node position in the tree level, first coarse point of the tile
 };
 
-// masked red-black half sweep of the staged region [S, kNmLF - 1 - S]^3
-template <typename G, int S, bool ZS>
+// masked red-black half sweep of the staged region [S, kNmLF - 1 - S]^3 (RED:
+// the red half sweep, over-relaxed as in k_bmg_rbgs)
+template <typename G, int S, bool ZS, bool RED>
 __device__ inline void nm_sweep(G *sE, const G *sR, const unsigned char *sM, int lp, G h2)
 {
   constexpr int LF = kNmLF, LF2 = LF * LF, L = LF - 2 * S, HX = (L + 1) / 2, SLOTS = HX * L * L;
@@ -518,7 +534,8 @@ __device__ inline void nm_sweep(G *sE, const G *sR, const unsigned char *sM, int
     G sum = ZS ? (G)0
                : o[(idx + 1) >> 1] + o[(idx - 1) >> 1] + o[(idx + LF) >> 1] + o[(idx - LF) >> 1] +
                      o[(idx + LF2) >> 1] + o[(idx - LF2) >> 1];
-    sE[p] = (sum - h2 * rv) * (G)(1.0 / 6.0);
+    const G gs = (sum - h2 * rv) * (G)(1.0 / 6.0);
+    sE[p] = RED ? bmg_red_relax(sE, p, gs, ZS ? 1 : 0) : gs;
   }
   __syncthreads();
 }
@@ -590,10 +607,10 @@ __global__ void __launch_bounds__(kNmThreads)
   const int red = (fx + fy + fz) & 1, black = red ^ 1; // local parity of the red points
   nm_stage<G, false>(b, m, fx, fy, fz, r, mask, (const G *)nullptr, sR, sM, sE); // sE = 0
   __syncthreads();
-  nm_sweep<G, 0, true>(sE, sR, sM, red, h2);
-  nm_sweep<G, 1, false>(sE, sR, sM, black, h2);
-  nm_sweep<G, 2, false>(sE, sR, sM, red, h2);
-  nm_sweep<G, 3, false>(sE, sR, sM, black, h2);
+  nm_sweep<G, 0, true, true>(sE, sR, sM, red, h2);
+  nm_sweep<G, 1, false, false>(sE, sR, sM, black, h2);
+  nm_sweep<G, 2, false, true>(sE, sR, sM, red, h2);
+  nm_sweep<G, 3, false, false>(sE, sR, sM, black, h2);
   const int X = b.X[m], Y = b.Y[m], Z = b.Z[m];
   const long long np = (long long)X * Y * Z, hf = (np + 1) >> 1;
   {
@@ -711,10 +728,10 @@ __global__ void __launch_bounds__(kNmThreads)
     sE[p] += v;
   }
   __syncthreads();
-  nm_sweep<G, 1, false>(sE, sR, sM, red, h2);
-  nm_sweep<G, 2, false>(sE, sR, sM, black, h2);
-  nm_sweep<G, 3, false>(sE, sR, sM, red, h2);
-  nm_sweep<G, 4, false>(sE, sR, sM, black, h2);
+  nm_sweep<G, 1, false, true>(sE, sR, sM, red, h2);
+  nm_sweep<G, 2, false, false>(sE, sR, sM, black, h2);
+  nm_sweep<G, 3, false, true>(sE, sR, sM, red, h2);
+  nm_sweep<G, 4, false, false>(sE, sR, sM, black, h2);
   const int X = b.X[m], Y = b.Y[m], Z = b.Z[m];
   const long long np = (long long)X * Y * Z, hf = (np + 1) >> 1;
   G *ebase = e + b.off[m];
