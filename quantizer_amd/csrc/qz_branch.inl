@@ -87,6 +87,7 @@ struct TreeDev
   // and potential are computed.
   std::thread ahead;
   bool ahead_ok = true;
+  std::string ahead_err; // the set-up thread's error (errors are kept per thread)
   hipStream_t ahead_stream = nullptr;
   hipEvent_t ahead_tree = nullptr, ahead_done = nullptr;
   unsigned char *mg_mask = nullptr;
@@ -1173,15 +1174,12 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
       !hip_ok(hipMemcpyAsync(d.ntiles, ntab.data(), sizeof(NTile) * ntab.size(), hipMemcpyHostToDevice, stream),
               "node tiles"))
     return false;
-  static bool tail_attr = false; // the tail's LDS exceeds the default dynamic limit
-  if (!tail_attr)
-  {
-    if (!hip_ok(hipFuncSetAttribute((const void *)k_bmg_tail<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)kBmgTailLds),
-                "node tail LDS"))
-      return false;
-    tail_attr = true;
-  }
+  // the tail's LDS exceeds the default dynamic limit: raised once per process
+  // (a function-local static is initialised once, also with set-ups on two threads)
+  static const hipError_t tail_attr =
+      hipFuncSetAttribute((const void *)k_bmg_tail<G>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBmgTailLds);
+  if (!hip_ok(tail_attr, "node tail LDS"))
+    return false;
   TreeDev::Plan &pl = d.plan;
   pl.cg_only = false;
   pl.ahead = stream != this->stream;
@@ -1209,7 +1207,10 @@ template <typename P, typename G> bool Engine<P, G>::join_ahead()
   TreeDev &d = *tree->dev;
   d.ahead.join();
   if (!d.ahead_ok)
+  {
     d.plan.level = -1;
+    set_error("%s", d.ahead_err.c_str());
+  }
   return d.ahead_ok;
 }
 
@@ -1228,6 +1229,8 @@ template <typename P, typename G> bool Engine<P, G>::start_ahead()
   d.ahead = std::thread([this, &d]() {
     d.ahead_ok = hip_ok(hipStreamWaitEvent(d.ahead_stream, d.ahead_tree, 0), "set-up stream") &&
                  bmg_prepare(1, d.ahead_stream) && hip_ok(hipEventRecord(d.ahead_done, d.ahead_stream), "set-up event");
+    if (!d.ahead_ok)
+      d.ahead_err = g_last_error;
   });
   return true;
 }
