@@ -29,8 +29,8 @@ adjoint_amr: the same ratio with refinement levels: two Plummer spheres of
 last 20 of 40 steps with checkpoints every 5, loss = final kinetic energy of
 the lighter sphere.
 
-roofline: the dominant kernel (the head's red-black Gauss-Seidel half sweep;
-its finest-level launches carry most of its time) replayed back to back on the
+roofline: the dominant kernel (the head's red-black half sweep, here its red
+colour, over-relaxed; the finest-level launches carry most of its time) replayed back to back on the
 headline grid between two HIP events on the library's stream; algorithmic
 bytes per launch over that time.  traffic: HBM bytes per launch of the same
 kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/, scaled
@@ -136,7 +136,7 @@ def roofline(sweep):
     traffic, scale = traffic_from_profile()
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_rbgs (head red-black GS half sweep, finest level: 257^3 grid, one colour)",
+            "kernel": "k_rbgs (head red-black half sweep, red colour over-relaxed, finest level: 257^3 grid)",
             "bytes_per_launch": by, "ms_per_launch": ms, "counter_scale": scale}
 
 

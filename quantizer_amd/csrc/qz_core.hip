@@ -250,7 +250,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = (double)t / std::max(1, reps);
-    *bytes = 1.5 * sizeof(G) * (double)np; // own colour r + e, the other colour's e once
+    *bytes = 2.0 * sizeof(G) * (double)np; // own colour r, e read and written; the other colour's e once
     return ok;
   }
 
@@ -645,7 +645,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     for (int sw = 0; sw < cfg.pre_sweeps; sw++)
     {
       int z = (zero_start && sw == 0) ? 1 : 0;
-      QZ_LAUNCHB("rbgs", (z ? 1.0 : 1.5) * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, z);
+      QZ_LAUNCHB("rbgs", (z ? 1.0 : 2.0) * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, z);
       QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], cf, h2, 1, 0);
     }
     int mc = lev_n[l + 1];
@@ -660,7 +660,7 @@ template <typename P, typename G> struct Engine final : EngineBase
                lev_e[l], lev_e[l + 1], make_cube(mc));
     for (int sw = 0; sw < cfg.post_sweeps; sw++)
     {
-      QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, 0);
+      QZ_LAUNCHB("rbgs", 2.0 * sizeof(G) * np, k_rbgs<G>, nblk(nhalf), lev_e[l], lev_r[l], cf, h2, 0, 0);
       QZ_LAUNCHB("rbgs", 1.5 * sizeof(G) * np, k_rbgs<G>, nblk(np - nhalf), lev_e[l], lev_r[l], cf, h2, 1, 0);
     }
   }
