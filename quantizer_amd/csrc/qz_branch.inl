@@ -14,6 +14,22 @@
 namespace qz
 {
 
+// The node multigrid set-up of one tree level (bmg_prepare), kept for the
+// solve of that level, which uses it once.
+struct BmgPlan
+{
+  int level = -1;       // tree level the set-up belongs to, -1 when none is held
+  bool cg_only = false; // a box too large for the hierarchy: CG solves the level
+  bool ahead = false;   // built on the set-up stream (the solve waits for ahead_done)
+  bool nfuse = false;
+  int n_itab = 0, mmax = 0;
+  size_t tail_lds = 0;
+  int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
+  int nt0[kMaxMgLevels], ntn[kMaxMgLevels];
+  double act_s[kMaxMgLevels], act_r[kMaxMgLevels];
+  hipGraphExec_t gexec = nullptr; // the PCG batch captured with the set-up (run ahead only)
+};
+
 struct TreeDev
 {
   DNode *nodes = nullptr;
@@ -68,20 +84,14 @@ struct TreeDev
   NTile *ntiles = nullptr; // fused node levels: tile table
   long long ntile_cap = 0;
   std::vector<NTile> hntiles;
-  // The node multigrid set-up of one tree level (bmg_prepare), kept for the
-  // solve of that level, which uses it once.
-  struct Plan
+  BmgPlan plan;
+  void drop_plan()
   {
-    int level = -1;       // tree level the set-up belongs to, -1 when none is held
-    bool cg_only = false; // a box too large for the hierarchy: CG solves the level
-    bool ahead = false;   // built on the set-up stream (the solve waits for ahead_done)
-    bool nfuse = false;
-    int n_itab = 0, mmax = 0;
-    size_t tail_lds = 0;
-    int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
-    int nt0[kMaxMgLevels], ntn[kMaxMgLevels];
-    double act_s[kMaxMgLevels], act_r[kMaxMgLevels];
-  } plan;
+    if (plan.gexec)
+      (void)hipGraphExecDestroy(plan.gexec);
+    plan.gexec = nullptr;
+    plan.level = -1;
+  }
   // The first tree level's set-up runs ahead on a host thread and its own
   // stream (after ahead_tree, the end of the tree build) while the head density
   // and potential are computed.
@@ -197,6 +207,7 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
   {
     if (d->ahead.joinable())
       d->ahead.join();
+    d->drop_plan();
     if (d->ahead_stream)
       (void)hipStreamDestroy(d->ahead_stream);
     for (hipEvent_t e : {d->ahead_tree, d->ahead_done})
@@ -415,7 +426,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
   }
   if (!join_ahead())
     return false;
-  tree->dev->plan.level = -1;
+  tree->dev->drop_plan();
   HostSpan span_all(timer, "tree_adaptation");
   if (!sort_particles())
     return false;
@@ -758,11 +769,16 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
 // timed) until done_count reaches `count` or `max_it` iterations have run.
 template <typename P, typename G>
 template <typename F>
-bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done)
+bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done,
+                                   hipGraphExec_t given)
 {
   TreeDev &d = *tree->dev;
   hipGraphExec_t gexec = nullptr;
-  if (!timer.on)
+  if (given && timer.on)
+    (void)hipGraphExecDestroy(given); // timed runs launch eagerly
+  else if (given)
+    gexec = given; // captured with the set-up (owned from here on)
+  else if (!timer.on)
   {
     HostSpan span(timer, "graph_instantiate");
     hipGraph_t g = nullptr;
@@ -955,7 +971,7 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
 {
   HostSpan prep(timer, "bmg_prepare");
   TreeDev &d = *tree->dev;
-  d.plan.level = -1;
+  d.drop_plan();
   const int first = d.lfirst[l], count = d.lcount[l];
   std::vector<BMg> &hb = d.hbmg;
   hb.assign(count, BMg{});
@@ -983,7 +999,7 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
     if (!M)
     {
       // a box too large for the coarsest workgroup: CG solves this level
-      d.plan = TreeDev::Plan{};
+      d.plan = BmgPlan{};
       d.plan.level = (int)l;
       d.plan.cg_only = true;
       return true;
@@ -1180,7 +1196,7 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
       hipFuncSetAttribute((const void *)k_bmg_tail<G>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBmgTailLds);
   if (!hip_ok(tail_attr, "node tail LDS"))
     return false;
-  TreeDev::Plan &pl = d.plan;
+  BmgPlan &pl = d.plan;
   pl.cg_only = false;
   pl.ahead = stream != this->stream;
   pl.nfuse = nfuse;
@@ -1208,7 +1224,7 @@ template <typename P, typename G> bool Engine<P, G>::join_ahead()
   d.ahead.join();
   if (!d.ahead_ok)
   {
-    d.plan.level = -1;
+    d.drop_plan();
     set_error("%s", d.ahead_err.c_str());
   }
   return d.ahead_ok;
@@ -1219,7 +1235,8 @@ template <typename P, typename G> bool Engine<P, G>::join_ahead()
 template <typename P, typename G> bool Engine<P, G>::start_ahead()
 {
   TreeDev &d = *tree->dev;
-  if (!d.ahead_stream && (!hip_ok(hipStreamCreate(&d.ahead_stream), "set-up stream") ||
+  // non-blocking: a capture on it is not disturbed by legacy-stream calls of other threads
+  if (!d.ahead_stream && (!hip_ok(hipStreamCreateWithFlags(&d.ahead_stream, hipStreamNonBlocking), "set-up stream") ||
                           !hip_ok(hipEventCreateWithFlags(&d.ahead_tree, hipEventDisableTiming), "set-up event") ||
                           !hip_ok(hipEventCreateWithFlags(&d.ahead_done, hipEventDisableTiming), "set-up event")))
     return false;
@@ -1229,27 +1246,35 @@ template <typename P, typename G> bool Engine<P, G>::start_ahead()
   d.ahead = std::thread([this, &d]() {
     d.ahead_ok = hip_ok(hipStreamWaitEvent(d.ahead_stream, d.ahead_tree, 0), "set-up stream") &&
                  bmg_prepare(1, d.ahead_stream) && hip_ok(hipEventRecord(d.ahead_done, d.ahead_stream), "set-up event");
+    // the PCG batch of the level's potential solve, captured on the set-up stream
+    // (after ahead_done: nothing recorded during the capture is work of its own)
+    if (d.ahead_ok && !d.plan.cg_only)
+    {
+      hipGraph_t g = nullptr;
+      d.ahead_ok = hip_ok(hipStreamBeginCapture(d.ahead_stream, hipStreamCaptureModeRelaxed), "graph capture");
+      if (d.ahead_ok)
+      {
+        bmg_batch(1, d.plan, d.ahead_stream, d.phi, false);
+        d.ahead_ok = hip_ok(hipStreamEndCapture(d.ahead_stream, &g), "graph capture") &&
+                     hip_ok(hipGraphInstantiate(&d.plan.gexec, g, nullptr, nullptr, 0), "graph instantiate");
+      }
+      if (g)
+        (void)hipGraphDestroy(g);
+    }
     if (!d.ahead_ok)
       d.ahead_err = g_last_error;
   });
   return true;
 }
 
+// One batch of the node PCG of tree level l with set-up pl, issued on `stream`:
+// two iterations of check, V-cycle, dots, direction, apply and update, then the
+// iteration counter.  Captured into a graph (on the engine's stream, or ahead on
+// the set-up stream) or launched eagerly while kernels are timed.
 template <typename P, typename G>
-bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint)
+void Engine<P, G>::bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, double *phi_pool, bool adjoint)
 {
-  solved = false;
-  if (!join_ahead())
-    return false;
   TreeDev &d = *tree->dev;
-  if (d.plan.level != (int)l && !bmg_prepare(l, stream))
-    return false;
-  const TreeDev::Plan pl = d.plan; // a set-up serves one solve
-  d.plan.level = -1;
-  if (pl.cg_only)
-    return true;
-  if (pl.ahead && !hip_ok(hipStreamWaitEvent(stream, d.ahead_done, 0), "set-up stream"))
-    return false;
   const int first = d.lfirst[l], count = d.lcount[l];
   const int n_itab = pl.n_itab, mmax = pl.mmax;
   const size_t tail_lds = pl.tail_lds;
@@ -1327,6 +1352,47 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
       }
     }
   };
+  for (int il = 0; il < 2; il++)
+  {
+    QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, cfg.tol, cfg.G,
+              adjoint ? 1 : 0, d.done_count);
+    if (n_itab == 0)
+      continue;
+    vcycle();
+    QZ_LAUNCHB("bmg_zdots", ni * (4.0 + gs + 16.0), (k_bmg_zdots<G>), n_itab, d.nodes, first, bm, itab, d.intr_g,
+               (const G *)e, d.cg_r, d.cg_Ap, d.part1, d.part3);
+    QZ_LAUNCHB("bmg_direction", ni * (4.0 + gs + 16.0), (k_bmg_direction<G>), n_itab, d.nodes, first, bm, itab,
+               it_base, il, d.intr_g, (const G *)e, d.cg_p, d.part1, d.part3);
+    QZ_LAUNCHB("bmg_apply", ni * 20.0, k_bmg_apply, n_itab, d.nodes, first, bm, itab, d.intr_g, d.cg_p, d.cg_Ap,
+               d.part1);
+    QZ_LAUNCHB("bmg_update", ni * (4.0 + 48.0 + gs), (k_bmg_update<G>), n_itab, d.nodes, first, bm, itab, it_base,
+               il, d.intr_g, phi_pool, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
+  }
+  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 2);
+}
+
+template <typename P, typename G>
+bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint)
+{
+  solved = false;
+  if (!join_ahead())
+    return false;
+  TreeDev &d = *tree->dev;
+  if (d.plan.level != (int)l && !bmg_prepare(l, stream))
+    return false;
+  const BmgPlan pl = d.plan; // a set-up serves one solve (its graph moves to this solve)
+  d.plan.gexec = nullptr;
+  d.plan.level = -1;
+  if (pl.cg_only)
+    return true;
+  if (pl.ahead && !hip_ok(hipStreamWaitEvent(stream, d.ahead_done, 0), "set-up stream"))
+    return false;
+  const int first = d.lfirst[l], count = d.lcount[l];
+  const int n_itab = pl.n_itab;
+  G *r = (G *)d.mg_r;
+  BMg *bm = d.bmg;
+  const Chunk *itab = d.chunks;
+  const double ni = (double)d.lnintr[l], gs = (double)sizeof(G);
   // Flexible PCG in FP64 with the V-cycle as preconditioner.  Algorithmic bytes
   // per interior point: residual reads idx, phi, rho and writes r (FP64 + G);
   // zdots reads idx, z, r, q; direction reads idx, z, p, writes p; apply reads
@@ -1335,29 +1401,18 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   if (n_itab > 0)
     QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + gs + 8.0 + gs), (k_bmg_residual<G>), n_itab, d.nodes, first, bm,
                itab, d.intr_g, phi_pool, rho_pool, d.cg_r, r, d.part2);
-  auto batch = [&]() {
-    for (int il = 0; il < 2; il++)
-    {
-      QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, cfg.tol, cfg.G,
-                adjoint ? 1 : 0, d.done_count);
-      if (n_itab == 0)
-        continue;
-      vcycle();
-      QZ_LAUNCHB("bmg_zdots", ni * (4.0 + gs + 16.0), (k_bmg_zdots<G>), n_itab, d.nodes, first, bm, itab, d.intr_g,
-                 (const G *)e, d.cg_r, d.cg_Ap, d.part1, d.part3);
-      QZ_LAUNCHB("bmg_direction", ni * (4.0 + gs + 16.0), (k_bmg_direction<G>), n_itab, d.nodes, first, bm, itab,
-                 it_base, il, d.intr_g, (const G *)e, d.cg_p, d.part1, d.part3);
-      QZ_LAUNCHB("bmg_apply", ni * 20.0, k_bmg_apply, n_itab, d.nodes, first, bm, itab, d.intr_g, d.cg_p, d.cg_Ap,
-                 d.part1);
-      QZ_LAUNCHB("bmg_update", ni * (4.0 + 48.0 + gs), (k_bmg_update<G>), n_itab, d.nodes, first, bm, itab, it_base,
-                 il, d.intr_g, phi_pool, d.cg_p, d.cg_Ap, d.cg_r, r, d.part1, d.part2, d.part3, cfg.tol_change);
-    }
-    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, stream, it_base, 2);
-  };
+  // the batch captured ahead was built for this level's potential solve
+  hipGraphExec_t given = pl.gexec;
+  if (given && (phi_pool != d.phi || adjoint))
+  {
+    (void)hipGraphExecDestroy(given);
+    given = nullptr;
+  }
+  auto batch = [&]() { bmg_batch(l, pl, stream, phi_pool, adjoint); };
   int it = 0;
   bool all_done = false;
   HostSpan span_it(timer, "bmg_iterate");
-  if (!iterate_batches(batch, 2, kBmgMaxOuter, count, it, all_done))
+  if (!iterate_batches(batch, 2, kBmgMaxOuter, count, it, all_done, given))
     return false;
   solved = all_done;
   return true;
@@ -1558,7 +1613,7 @@ template <typename P, typename G> bool Engine<P, G>::restore_tree(const TreeSnap
   if (!join_ahead())
     return false;
   TreeDev &d = *tree->dev;
-  d.plan.level = -1;
+  d.drop_plan();
   if (t->n_nodes > d.node_cap || t->used_box > d.box_cap || t->used_cell > d.cell_cap || t->used_pt > d.pt_cap ||
       t->used_grid > d.grid_cap)
   {

@@ -985,7 +985,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool join_ahead() override;
   bool start_ahead();
   bool ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool, hipStream_t stream);
-  template <typename F> bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done);
+  template <typename F>
+  bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done,
+                       hipGraphExec_t given = nullptr);
+  void bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, double *phi_pool, bool adjoint);
   bool branch_density();
   bool branch_grid_acceleration();
   bool branch_particle_acceleration();

@@ -59,6 +59,7 @@ struct HostSpan
 
 struct TreeHost; // host tree (qz_tree.h)
 struct TreeDev;  // device mirror of the tree
+struct BmgPlan;  // node multigrid set-up of one tree level
 struct TreeSnap; // a recorded tree state (adjoint records)
 
 struct TreeState
