@@ -1386,7 +1386,11 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   if (pl.cg_only)
     return true;
   if (pl.ahead && !hip_ok(hipStreamWaitEvent(stream, d.ahead_done, 0), "set-up stream"))
+  {
+    if (pl.gexec)
+      (void)hipGraphExecDestroy(pl.gexec);
     return false;
+  }
   const int first = d.lfirst[l], count = d.lcount[l];
   const int n_itab = pl.n_itab;
   G *r = (G *)d.mg_r;
