@@ -416,7 +416,7 @@ def main():
                     help="mixed against FP64 on every configuration; prints one JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        print("bench.py: the library runs on one GPU only (no multi-GPU path); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     if args.pmc_probe:
         pmc_probe(args)
