@@ -785,10 +785,12 @@ bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int cou
     if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
       return false;
     batch();
-    if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
-        !hip_ok(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0), "graph instantiate"))
+    const bool made = hip_ok(hipStreamEndCapture(stream, &g), "graph capture") &&
+                      hip_ok(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0), "graph instantiate");
+    if (g)
+      (void)hipGraphDestroy(g);
+    if (!made)
       return false;
-    (void)hipGraphDestroy(g);
   }
   all_done = false;
   // Pipelined: batch k + 1 is queued before the done count after batch k is
@@ -824,9 +826,13 @@ bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int cou
       break;
     }
   }
+  // The batch queued ahead (a replay of gexec and its done-count copy) may still
+  // be in flight here, on the error path too: drain the stream before the
+  // executable graph, whose kernel arguments those launches read, is destroyed.
+  const bool drained = hip_ok(hipStreamSynchronize(stream), "iterate");
   if (gexec)
     (void)hipGraphExecDestroy(gexec);
-  return ok && hip_ok(hipStreamSynchronize(stream), "iterate");
+  return ok && drained;
 }
 
 // Unpreconditioned CG per node, all nodes of the level batched

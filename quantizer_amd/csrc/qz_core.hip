@@ -701,10 +701,15 @@ template <typename P, typename G> struct Engine final : EngineBase
       if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
         return false;
       head_outer_body(ph, rh, inner);
-      if (!hip_ok(hipStreamEndCapture(stream, &g), "graph capture") ||
-          !hip_ok(hipGraphInstantiate(&head_graph[slot], g, nullptr, nullptr, 0), "graph instantiate"))
+      const bool made = hip_ok(hipStreamEndCapture(stream, &g), "graph capture") &&
+                        hip_ok(hipGraphInstantiate(&head_graph[slot], g, nullptr, nullptr, 0), "graph instantiate");
+      if (g)
+        (void)hipGraphDestroy(g);
+      if (!made)
+      {
+        head_graph[slot] = nullptr;
         return false;
-      (void)hipGraphDestroy(g);
+      }
       head_graph_inner[slot] = inner;
     }
     return hip_ok(hipGraphLaunch(head_graph[slot], stream), "graph launch");
