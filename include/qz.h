@@ -177,6 +177,22 @@ typedef struct
 } qz_units;
 int qz_write_snapshot(qz_ctx *ctx, const char *path, const qz_units *units, double actualtime, int with_observables);
 
+/* FP64 diagnostics without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  The quantities of
+ * qz_observables with exact_pe = 1 (E_kin, E_pot from the exact pairwise sum
+ * observables.c:77-112, E_tot, px, py, pz; observables.c:38-75, 661-696) for n
+ * particles given as FP64 device arrays in any order; with_pe = 0 skips the
+ * O(n^2) pair sum and reports E_pot = 0.  out6 may be host or device memory.
+ * Partial sums live in a fixed device array of the library (2 MiB per GPU,
+ * held until the process ends), so n is limited to QZ_NOALLOC_MAX_PARTICLES;
+ * larger n is refused with an error.  Runs on the current device's null
+ * stream and returns when done.  Calls on one device share that array:
+ * concurrent calls on the same device race. */
+#define QZ_NOALLOC_MAX_PARTICLES (1 << 24)
+int qz_observables_noalloc(const double *x, const double *y, const double *z, const double *vx, const double *vy,
+                           const double *vz, const double *m, int n, double G, int with_pe, double *out6);
+
 /* measurement helpers (no counterpart in the reference): the head's finest red
  * half sweep of the multigrid smoother replayed `reps` times back to back,
  * returning ms and algorithmic bytes per launch; and one plain device copy of

@@ -137,6 +137,8 @@ def load_library(path: str = LIB_PATH):
     lib.qz_timing_enable.argtypes = [vp, ctypes.c_int]
     lib.qz_timing_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _dp, _ip, _dp, ctypes.c_int]
     lib.qz_timing_report.restype = ctypes.c_int
+    lib.qz_observables_noalloc.argtypes = [vp] * 7 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
+    lib.qz_observables_noalloc.restype = ctypes.c_int
     _LIB = lib
     return lib
 
@@ -406,4 +408,13 @@ def box_keys(x, y, z, lv: int, ts, rd) -> np.ndarray:
     ts = np.ascontiguousarray(ts, dtype=np.int32)
     rd = np.ascontiguousarray(rd, dtype=np.int32)
     _check(lib.qz_keys(_d(x), _d(y), _d(z), x.shape[0], lv, _i(ts), _i(rd), _i(out)), "qz_keys")
+    return out
+
+
+def observables_noalloc(dev_ptrs, n: int, G: float = 1.0, with_pe: bool = True) -> np.ndarray:
+    """qz_observables_noalloc on FP64 device arrays x, y, z, vx, vy, vz, m (device
+    addresses as ints, n particles): E_kin, E_pot, E_tot, px, py, pz."""
+    out = np.empty(6)
+    _check(load_library().qz_observables_noalloc(*[ctypes.c_void_p(p) for p in dev_ptrs], n, G,
+                                                 1 if with_pe else 0, _d(out)), "qz_observables_noalloc")
     return out

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1034,6 +1035,65 @@ struct qz_ctx
   EngineBase *eng;
 };
 
+// ------------------------------------------------- diagnostics without allocation
+// qz_observables_noalloc keeps its partial sums in this fixed array instead of
+// memory it allocates: kNoAllocBlocks x 4 block partials (E_kin and momentum;
+// the pairwise energy reuses the first kNoAllocBlocks), then 16 doubles for the
+// reduced sums and the six results.  Each device holds its own copy from module
+// load until the process ends.  Host code reaches it only through the device
+// address hipGetSymbolAddress returns (the symbol's host-side name is not a
+// device address).
+namespace
+{
+constexpr int kNoAllocBlocks = QZ_NOALLOC_MAX_PARTICLES / qz::kBlock;
+constexpr size_t kNoAllocDoubles = 4 * (size_t)kNoAllocBlocks + 16;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+double *scratch_for_current_device()
+{
+  int dev = -1;
+  if (!qz::hip_ok(hipGetDevice(&dev), "qz_observables_noalloc: device"))
+    return nullptr;
+  if (dev < 0 || dev >= kMaxDevices)
+  {
+    qz::set_error("qz_observables_noalloc: device %d outside the supported range", dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev])
+  {
+    void *p = nullptr;
+    if (!qz::hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "qz_observables_noalloc: scratch address"))
+      return nullptr;
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// out6 = E_kin, E_pot, E_tot, px, py, pz from the reduced sums (observables.c:38-112):
+// sums[0] = sum m v^2, sums[1..3] = sum m v, sums[4] = pairwise sum m_i m_j / r_ij.
+// Rounded operations keep the arithmetic identical to qz_observables' host code.
+__global__ void k_observables_finish(const double *__restrict__ sums, double G, int with_pe,
+                                     double *__restrict__ out6)
+{
+  if (threadIdx.x != 0)
+    return;
+  const double ek = 0.5 * sums[0];
+  const double ep = with_pe ? __dmul_rn(-G, sums[4]) : 0.0;
+  out6[0] = ek;
+  out6[1] = ep;
+  out6[2] = __dadd_rn(ek, ep);
+  out6[3] = sums[1];
+  out6[4] = sums[2];
+  out6[5] = sums[3];
+}
+} // namespace
+
 extern "C" {
 
 void qz_default_config(qz_config *c)
@@ -1406,6 +1466,44 @@ int qz_sync(qz_ctx *ctx)
 {
   QZ_CHECK_CTX(ctx);
   return qz::hip_ok(hipStreamSynchronize(ctx->eng->stream), "sync") ? 0 : 1;
+}
+
+int qz_observables_noalloc(const double *x, const double *y, const double *z, const double *vx, const double *vy,
+                           const double *vz, const double *m, int n, double G, int with_pe, double *out6)
+{
+  if (n < 0 || n > QZ_NOALLOC_MAX_PARTICLES)
+  {
+    qz::set_error("qz_observables_noalloc: n = %d outside [0, %d], the size of its fixed scratch array", n,
+                  QZ_NOALLOC_MAX_PARTICLES);
+    return 1;
+  }
+  if (!out6 || (n > 0 && (!vx || !vy || !vz || !m || (with_pe && (!x || !y || !z)))))
+  {
+    qz::set_error("qz_observables_noalloc: null array");
+    return 1;
+  }
+  double *scr = scratch_for_current_device();
+  if (!scr)
+    return 1;
+  double *sums = scr + 4 * (size_t)kNoAllocBlocks, *res = sums + 8;
+  // Every value read below is written first in the same call, so nothing left
+  // by an earlier call needs clearing; concurrent calls on one device race.
+  const int nb = n > 0 ? (n + qz::kBlock - 1) / qz::kBlock : 0;
+  if (nb > 0)
+    hipLaunchKernelGGL(qz::k_kin_mom<double>, dim3(nb), dim3(qz::kBlock), 0, 0, vx, vy, vz, m, n, scr);
+  for (int q = 0; q < 4; q++)
+    hipLaunchKernelGGL(qz::k_reduce_final<0>, dim3(1), dim3(qz::kBlock), 0, 0, scr + (size_t)q * nb, nb, sums + q);
+  if (with_pe)
+  {
+    if (nb > 0)
+      hipLaunchKernelGGL(qz::k_pairwise_pe<double>, dim3(nb), dim3(qz::kBlock), 0, 0, x, y, z, m, n, scr);
+    hipLaunchKernelGGL(qz::k_reduce_final<0>, dim3(1), dim3(qz::kBlock), 0, 0, scr, nb, sums + 4);
+  }
+  hipLaunchKernelGGL(k_observables_finish, dim3(1), dim3(64), 0, 0, sums, G, with_pe ? 1 : 0, res);
+  if (!qz::hip_ok(hipGetLastError(), "qz_observables_noalloc: launch"))
+    return 1;
+  return qz::hip_ok(hipMemcpy(out6, res, 6 * sizeof(double), hipMemcpyDefault), "qz_observables_noalloc: result") ? 0
+                                                                                                                   : 1;
 }
 
 } // extern "C"
