@@ -169,6 +169,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
   bool keys_valid = false;
+  double face_gm = 0.0; // G M of the head's monopole faces, fixed when the head is initialised
   bool acc_live = false;         // particle accelerations hold values (false: all zero, as reset leaves them)
   std::vector<double> host_mass; // the masses as given (snapshots write them unchanged)
   double *phi_prev = nullptr; // the previous converged head potential
@@ -450,7 +451,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     {
       phi_hist = 0;
       size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
-      QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1), cfg.G * total_mass,
+      face_gm = cfg.G * total_mass;
+      QZ_LAUNCH("init_head", k_init_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1), face_gm,
                 cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
     }
     keys_valid = false;
@@ -814,8 +816,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
-    QZ_LAUNCHB("grid_acc_head", (8.0 + 3.0 * sizeof(G)) * npts, k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz, make_cube(n + 1),
-               1.0 / (2.0 * H));
+    QZ_LAUNCHB("grid_acc_head", (8.0 + 3.0 * sizeof(G)) * npts, k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz,
+               make_cube(n + 1), 1.0 / (2.0 * H), face_gm, cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
     if (has_tree() && !branch_grid_acceleration())
       return false;
     return hip_ok(hipGetLastError(), "grid_acceleration"); // stream-ordered; readers synchronise

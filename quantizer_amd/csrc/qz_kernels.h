@@ -661,8 +661,33 @@ __global__ void k_grid_pe_head(const double *__restrict__ x, const double *__res
 }
 
 // ---------------------------------------------------------------- head grid
-// A10 (initial_potential_and_acceleration_head.c:297-335): monopole -GM/r and
-// its gradient about the box centre on every point of the solve domain.
+// The monopole about the box centre at grid point (i, j, k) of an n-cell grid
+// (initial_potential_and_acceleration_head.c:297-335; dist = 0 -> 1e-3 H), or
+// zeros for zero faces.
+template <typename G>
+__device__ inline void monopole_point(int i, int j, int k, int n, double GM, int zero_bc, double &phi, G &ax, G &ay,
+                                      G &az)
+{
+  if (zero_bc)
+  {
+    phi = 0.0;
+    ax = ay = az = (G)0;
+    return;
+  }
+  double H = 1.0 / n;
+  double ai = i * H - 0.5, aj = j * H - 0.5, ak = k * H - 0.5;
+  double dist = sqrt(ai * ai + aj * aj + ak * ak);
+  if (dist == 0.0)
+    dist = H * 1.0e-3;
+  double c = -GM / (dist * dist * dist);
+  phi = -GM / dist;
+  ax = (G)(c * ai);
+  ay = (G)(c * aj);
+  az = (G)(c * ak);
+}
+
+// A10: monopole -GM/r and its gradient about the box centre on every point of
+// the solve domain.
 template <typename G>
 __global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay, G *__restrict__ az,
                             Cube cb, double GM, int zero_bc)
@@ -673,22 +698,7 @@ __global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__r
     return;
   int i, j, k;
   cube_ijk(cb, t, i, j, k);
-  double H = 1.0 / n;
-  double ai = i * H - 0.5, aj = j * H - 0.5, ak = k * H - 0.5;
-  double dist = sqrt(ai * ai + aj * aj + ak * ak);
-  if (dist == 0.0)
-    dist = H * 1.0e-3;
-  if (zero_bc)
-  {
-    phi[t] = 0.0;
-    ax[t] = ay[t] = az[t] = (G)0;
-    return;
-  }
-  double c = -GM / (dist * dist * dist);
-  phi[t] = -GM / dist;
-  ax[t] = (G)(c * ai);
-  ay[t] = (G)(c * aj);
-  az[t] = (G)(c * ak);
+  monopole_point(i, j, k, n, GM, zero_bc, phi[t], ax[t], ay[t], az[t]);
 }
 
 // Outer FP64 residual r = rho - Lap(phi) on interior points (multigrid.c:140-150,
@@ -1581,7 +1591,7 @@ __global__ void __launch_bounds__(kMgThreads)
 // keep their A10 values.
 template <typename G>
 __global__ void k_grid_acc_head(const double *__restrict__ phi, G *__restrict__ ax, G *__restrict__ ay,
-                                G *__restrict__ az, Cube cb, double inv_2h)
+                                G *__restrict__ az, Cube cb, double inv_2h, double GM, int zero_bc)
 {
   const int n1 = cb.n1, n = n1 - 1;
   const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -1590,7 +1600,14 @@ __global__ void k_grid_acc_head(const double *__restrict__ phi, G *__restrict__ 
   int i, j, k;
   cube_ijk(cb, t, i, j, k);
   if (i == 0 || i == n || j == 0 || j == n || k == 0 || k == n)
+  {
+    // the faces keep the initial monopole values (the reference never updates
+    // them); they are rewritten here so the output depends on no earlier
+    // contents of the arrays, which the adjoint's step records exchange
+    double unused;
+    monopole_point(i, j, k, n, GM, zero_bc, unused, ax[t], ay[t], az[t]);
     return;
+  }
   size_t s1 = n1, s2 = (size_t)n1 * n1;
   ax[t] = (G)(inv_2h * (phi[t - 1] - phi[t + 1]));
   ay[t] = (G)(inv_2h * (phi[t - s1] - phi[t + s1]));
