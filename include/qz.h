@@ -177,6 +177,43 @@ typedef struct
 } qz_units;
 int qz_write_snapshot(qz_ctx *ctx, const char *path, const qz_units *units, double actualtime, int with_observables);
 
+/* The same file without stalling the caller (output_snapshots.c:141-226): the
+ * state is copied on the device, ordered after the work queued so far, and the
+ * call returns; a host thread of the context downloads the copy and writes
+ * `path` while later steps run.  obs6 (E_kin, E_pot, E_tot, px, py, pz, already
+ * in the caller's units: the reference's GL_energies and GL_momentum) is
+ * written after the arrays, or nothing when obs6 is NULL.  One snapshot is in
+ * flight per context: a second call first waits for the one before.
+ * qz_snapshot_wait returns once the last snapshot is written: 0, or 1 with the
+ * writer's error. */
+int qz_write_snapshot_async(qz_ctx *ctx, const char *path, const qz_units *units, double actualtime,
+                            const double *obs6);
+int qz_snapshot_wait(qz_ctx *ctx);
+
+/* Parameters.dat (output_main_parameters.c:167-230): the run's parameters as
+ * 38 "name value" lines in the reference's order (example_plot.py reads them),
+ * then the wall times in the reference's categories (times: 20 values in the
+ * order of the reference's GL_times[0..19], in seconds, or NULL for zeros).
+ * The configuration lines come from the context, the rest from info. */
+typedef struct
+{
+  double cfl;                /* _CFL_ */
+  double max_dt;             /* _MAX_dt_ (code units) */
+  double box_size;           /* _User_BoxSize_ */
+  double final_time;         /* actualtime at the end (code units; written divided by units->time) */
+  double total_mass_initial; /* GL_total_mass_initial */
+  double total_mass_final;   /* GL_total_mass_final */
+  int potential_energy_type; /* 0 exact pairwise, 1 grid interpolation */
+  int gc_iter;               /* Garbage_Collector_iter */
+  int n_particles_initial;   /* GL_no_ptcl_initial (0: the context's particle count) */
+  int number_timesteps;
+  int number_outputs;
+  int fr_output;
+  int max_iterations;        /* MaxIterations */
+  const double *times;
+} qz_run_info;
+int qz_write_parameters(qz_ctx *ctx, const char *path, const qz_units *units, const qz_run_info *info);
+
 /* FP64 diagnostics without allocation.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, hosts that forbid hipMalloc).  The quantities of

@@ -32,10 +32,20 @@ int particle_acceleration(void);                        /* particle_acceleration
 int reset(void);                                        /* reset.h:33, reset.c:29 */
 int ptcl_idx_to_box_idx(struct node *ptr_node, int ptcl_idx); /* ptcl_idx_to_box_idx.h:34, ptcl_idx_to_box_idx.c:30 */
 
-/* resident build only (also replaces particle_updating.o and tree_adaptation.o) */
+/* resident build only (also replaces particle_updating.o, tree_adaptation.o,
+ * timestep.o, observables.o, output_snapshots.o and garbage_collector.o):
+ * particles stay on the device, and GL_ptcl_* are refreshed only by
+ * qz_dropin_sync_host() */
 int particle_updating_A(vtype dt); /* particle_updating.h:33, particle_updating.c:2584 */
 void particle_updating_B(vtype dt); /* particle_updating.h:34, particle_updating.c:2787 */
 int tree_adaptation(void);          /* tree_adaptation.h:40, tree_adaptation.c:8250: the device rebuilds its tree */
+int timestep(vtype *ptr_dt);        /* timestep.h:32, timestep.c:589 (time_step_method = 1) */
+int observables(void);              /* observables.h:66, observables.c:661 */
+int output_snapshots(vtype actualtime, int snapshot); /* output_snapshots.h:100, output_snapshots.c:141 (written asynchronously) */
+int garbage_collector(void);        /* garbage_collector.h:34, garbage_collector.c:377: nothing to trim */
+/* build-only addition (no reference counterpart): copy the device state into
+ * GL_ptcl_x..az for a host that reads them; waits for pending snapshots */
+int qz_dropin_sync_host(void);
 
 #ifdef __cplusplus
 }

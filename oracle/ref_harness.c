@@ -8,7 +8,15 @@
  *
  * Usage:
  *   ref_harness <particles.bin> <outdir> <lmin> <lmax> <ref_crit> <nsteps> <dt>
- *               [--time] [--threads T] [--noenergy]
+ *               [--time] [--threads T] [--noenergy] [--adaptive FR]
+ *
+ * --adaptive FR runs main.c:341-573's loop instead of the fixed-dt one: the
+ * time step from timestep() at every step (dt is then _MAX_dt_), observables()
+ * and output_snapshots() before the loop, every FR steps and after the last,
+ * garbage_collector() every Garbage_Collector_iter steps, and
+ * output_main_parameters() at the end; the snapshots and Parameters.dat go to
+ * ../output relative to the working directory and dts.txt (one time step per
+ * line) to outdir.
  *
  * particles.bin: int32 N, then x,y,z,vx,vy,vz,m as N float64 each, already in
  * code units (box [0,1)^3, G = 1).
@@ -28,6 +36,17 @@
 #include "main.h"
 
 #include <stdint.h>
+
+/* The resident drop-in keeps the particles on the device and exports this to
+ * copy them into GL_ptcl_*; the reference's own objects have no such function. */
+#pragma weak qz_dropin_sync_host
+int qz_dropin_sync_host(void);
+
+static void sync_host(void)
+{
+  if (qz_dropin_sync_host)
+    qz_dropin_sync_host();
+}
 
 static double now_s(void)
 {
@@ -154,6 +173,7 @@ static void dump_head(const char *dir, const char *name)
 
 static void dump_particles(const char *dir, const char *name, int with_xv)
 {
+  sync_host();
   FILE *f = open_out(dir, name);
   int n = GL_no_ptcl_final;
   vtype *arrs[9] = {GL_ptcl_x, GL_ptcl_y, GL_ptcl_z, GL_ptcl_vx, GL_ptcl_vy, GL_ptcl_vz,
@@ -276,6 +296,43 @@ static int one_step(vtype dt)
   return _SUCCESS_;
 }
 
+/* main.c:341-573 from the initial force pass on, for nsteps steps */
+static int adaptive_run(const char *outdir, int nsteps, int fr, int energy)
+{
+  vtype actualtime = 0, dt = 0;
+  int Number_timesteps = 0, Number_outputs = 0;
+  compute_observables_FLAG = energy ? true : false;
+  FILE *fd = open_out(outdir, "dts.txt");
+  if (compute_observables_FLAG && observables() == _FAILURE_)
+    return _FAILURE_;
+  if (output_snapshots(actualtime, Number_outputs) == _FAILURE_)
+    return _FAILURE_;
+  while (Number_timesteps < nsteps)
+  {
+    if (timestep(&dt) == _FAILURE_)
+      return _FAILURE_;
+    fprintf(fd, "%.17e\n", (double)dt);
+    actualtime += dt;
+    if (one_step(dt) == _FAILURE_)
+      return _FAILURE_;
+    if (Number_timesteps % fr == 0 || Number_timesteps == nsteps - 1)
+    {
+      ++Number_outputs;
+      if (compute_observables_FLAG && observables() == _FAILURE_)
+        return _FAILURE_;
+      if (output_snapshots(actualtime, Number_outputs) == _FAILURE_)
+        return _FAILURE_;
+    }
+    if ((Number_timesteps + 1) % Garbage_Collector_iter == 0 && lmin < lmax && garbage_collector() == _FAILURE_)
+      return _FAILURE_;
+    Number_timesteps++;
+  }
+  fclose(fd);
+  GL_total_mass_final = GL_total_mass_initial;
+  output_main_parameters(actualtime, Number_timesteps, Number_outputs);
+  return _SUCCESS_;
+}
+
 int main(int argc, char **argv)
 {
   if (argc < 8)
@@ -290,7 +347,7 @@ int main(int argc, char **argv)
   int ref_crit = atoi(argv[5]);
   int nsteps = atoi(argv[6]);
   double dt = atof(argv[7]);
-  int do_time = 0, threads = 1, energy = 1, snap = 0;
+  int do_time = 0, threads = 1, energy = 1, snap = 0, adaptive_fr = 0;
   double snap_u[9] = {1, 1, 1, 0, 0, 0, 0, 0, 0}; /* dist, velocity, time, cm x y z, cm v x y z */
   for (int a = 8; a < argc; a++)
   {
@@ -300,6 +357,8 @@ int main(int argc, char **argv)
       threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--noenergy"))
       energy = 0;
+    else if (!strcmp(argv[a], "--adaptive") && a + 1 < argc)
+      adaptive_fr = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--snapshot") && a + 9 < argc)
     {
       snap = 1;
@@ -367,6 +426,20 @@ int main(int argc, char **argv)
       fprintf(fe, "%.17e %.17e %.17e %.17e %.17e %.17e\n", (double)GL_energies[0], (double)GL_energies[1],
               (double)GL_energies[2], (double)GL_momentum[0], (double)GL_momentum[1], (double)GL_momentum[2]);
     }
+  }
+
+  if (adaptive_fr > 0)
+  {
+    _MAX_dt_ = (vtype)dt;
+    time_step_method = 1;
+    if (adaptive_run(outdir, nsteps, adaptive_fr, energy) == _FAILURE_)
+    {
+      fprintf(stderr, "adaptive run failed\n");
+      return 1;
+    }
+    dump_particles(outdir, "sK_ptcl.bin", 1);
+    fprintf(stderr, "REF_DONE N=%d steps=%d levels_used=%d\n", n, nsteps, GL_tentacles_level_max);
+    return 0;
   }
 
   double t_steps = 0.0;

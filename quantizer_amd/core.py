@@ -95,6 +95,15 @@ class QzUnits(ctypes.Structure):
                 ("cm", ctypes.c_double * 3), ("cm_v", ctypes.c_double * 3)]
 
 
+class QzRunInfo(ctypes.Structure):
+    _fields_ = [("cfl", ctypes.c_double), ("max_dt", ctypes.c_double), ("box_size", ctypes.c_double),
+                ("final_time", ctypes.c_double), ("total_mass_initial", ctypes.c_double),
+                ("total_mass_final", ctypes.c_double), ("potential_energy_type", ctypes.c_int),
+                ("gc_iter", ctypes.c_int), ("n_particles_initial", ctypes.c_int),
+                ("number_timesteps", ctypes.c_int), ("number_outputs", ctypes.c_int), ("fr_output", ctypes.c_int),
+                ("max_iterations", ctypes.c_int), ("times", ctypes.POINTER(ctypes.c_double))]
+
+
 def load_library(path: str = LIB_PATH):
     """Load libqzhip.so and declare the C ABI of include/qz.h."""
     global _LIB
@@ -277,6 +286,39 @@ class Simulation:
                                           ctypes.c_int]
         _check(lib.qz_write_snapshot(self._h, os.fsencode(path), ctypes.byref(u), actualtime, int(observables)),
                "qz_write_snapshot")
+
+    def write_snapshot_async(self, path: str, actualtime: float, dist: float = 1.0, velocity: float = 1.0,
+                             time_unit: float = 1.0, cm=(0.0, 0.0, 0.0), cm_v=(0.0, 0.0, 0.0), obs6=None):
+        """qz_write_snapshot_async: returns once the state is copied on the device;
+        the file is written by a host thread (snapshot_wait joins it).  obs6:
+        E_kin, E_pot, E_tot, px, py, pz to write after the arrays, or None."""
+        lib = load_library()
+        self._units = QzUnits(dist, velocity, time_unit, (ctypes.c_double * 3)(*cm), (ctypes.c_double * 3)(*cm_v))
+        lib.qz_write_snapshot_async.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(QzUnits),
+                                                ctypes.c_double, _dp]
+        o = None if obs6 is None else np.ascontiguousarray(obs6, dtype=np.float64)
+        _check(lib.qz_write_snapshot_async(self._h, os.fsencode(path), ctypes.byref(self._units), actualtime,
+                                           None if o is None else _d(o)), "qz_write_snapshot_async")
+
+    def snapshot_wait(self):
+        lib = load_library()
+        lib.qz_snapshot_wait.argtypes = [ctypes.c_void_p]
+        return _check(lib.qz_snapshot_wait(self._h), "qz_snapshot_wait")
+
+    def write_parameters(self, path: str, time_unit: float = 1.0, times=None, **info):
+        """Parameters.dat in the reference layout (output_main_parameters.c); info
+        holds the QzRunInfo fields other than times."""
+        lib = load_library()
+        u = QzUnits(1.0, 1.0, time_unit, (ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(0, 0, 0))
+        t = None if times is None else np.ascontiguousarray(times, dtype=np.float64)
+        if t is not None and t.shape[0] != 20:
+            raise QuantizerError("write_parameters: times holds 20 values")
+        r = QzRunInfo(**info)
+        r.times = _d(t) if t is not None else None
+        lib.qz_write_parameters.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(QzUnits),
+                                            ctypes.POINTER(QzRunInfo)]
+        _check(lib.qz_write_parameters(self._h, os.fsencode(path), ctypes.byref(u), ctypes.byref(r)),
+               "qz_write_parameters")
 
     # ------------------------------------------------------------ measurement
     def bench_sweep(self, reps: int = 50):

@@ -168,6 +168,20 @@ template <typename P, typename G> struct Engine final : EngineBase
   double *red = nullptr;  // device scalars
   double *hred = nullptr; // pinned host mirror
   size_t part_cap = 0;
+  // asynchronous snapshots: a device copy of the sorted state, the pinned host
+  // buffer it is downloaded into, and the host thread that writes the file
+  struct SnapStage
+  {
+    double *pos[3] = {nullptr, nullptr, nullptr};
+    P *vel[3] = {nullptr, nullptr, nullptr};
+    int *id = nullptr;
+    unsigned char *host = nullptr; // pinned: x, y, z, vx, vy, vz, id
+    hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr;
+    std::thread writer;
+    bool ok = true;
+    std::string err;
+  } snap;
   bool keys_valid = false;
   double face_gm = 0.0; // G M of the head's monopole faces, fixed when the head is initialised
   bool acc_live = false;         // particle accelerations hold values (false: all zero, as reset leaves them)
@@ -287,6 +301,20 @@ template <typename P, typename G> struct Engine final : EngineBase
 
   void release()
   {
+    if (snap.writer.joinable())
+      snap.writer.join();
+    for (int a = 0; a < 3; a++)
+    {
+      (void)hipFree(snap.pos[a]);
+      (void)hipFree(snap.vel[a]);
+    }
+    (void)hipFree(snap.id);
+    if (snap.host)
+      (void)hipHostFree(snap.host);
+    if (snap.ready)
+      (void)hipEventDestroy(snap.ready);
+    if (snap.stream)
+      (void)hipStreamDestroy(snap.stream);
     for (int a = 0; a < 3; a++)
     {
       (void)hipFree(pos[a]);
@@ -960,6 +988,120 @@ template <typename P, typename G> struct Engine final : EngineBase
     return ok;
   }
 
+  // The snapshot file of write_snapshot from a host copy of the sorted state
+  // (positions, velocities, IDs), with the observables block given by the caller.
+  bool write_snapshot_file(const char *path, const qz_units &u, double actualtime, const double *obs6,
+                           const unsigned char *host)
+  {
+    const double *hx[3], *hdbl = (const double *)host;
+    for (int a = 0; a < 3; a++)
+      hx[a] = hdbl + (size_t)a * N;
+    const P *hv = (const P *)(host + 24 * (size_t)N);
+    const int *hid = (const int *)(host + (24 + 3 * sizeof(P)) * (size_t)N);
+    std::vector<double> out(6 * (size_t)N);
+    const double t = actualtime / u.time;
+    for (int i = 0; i < N; i++)
+    {
+      const size_t q = (size_t)hid[i];
+      for (int a = 0; a < 3; a++)
+      {
+        out[(size_t)a * N + q] = (hx[a][i] - 0.5) / u.dist + u.cm[a] + u.cm_v[a] * t;
+        out[(size_t)(3 + a) * N + q] = (double)hv[(size_t)a * N + i] / u.velocity + u.cm_v[a];
+      }
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f)
+    {
+      set_error("Error opening snapshot file %s", path);
+      return false;
+    }
+    bool ok = fwrite(host_mass.data(), sizeof(double), (size_t)N, f) == (size_t)N &&
+              fwrite(out.data(), sizeof(double), 6 * (size_t)N, f) == 6 * (size_t)N;
+    if (obs6)
+      ok = ok && fwrite(obs6, sizeof(double), 6, f) == 6;
+    ok = ok && fwrite(&t, sizeof(double), 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok)
+      set_error("writing snapshot %s failed", path);
+    return ok;
+  }
+
+  // Snapshot without stalling the step loop: the state is copied device to
+  // device on the engine's stream (ordered after the work queued so far), then
+  // a host thread downloads the copy on a stream of its own and writes the
+  // file while later steps run.  One snapshot is in flight at a time.
+  bool write_snapshot_async(const char *path, const qz_units &u, double actualtime, const double *obs6) override
+  {
+    if (!snapshot_wait())
+      return false;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "snapshot device"))
+      return false;
+    const size_t host_bytes = (24 + 3 * sizeof(P) + 4) * (size_t)N;
+    if (!snap.id)
+    {
+      size_t b = 0;
+      for (int a = 0; a < 3; a++)
+      {
+        snap.pos[a] = dmalloc<double>(N, b);
+        snap.vel[a] = dmalloc<P>(N, b);
+      }
+      snap.id = dmalloc<int>(N, b);
+      bytes += b;
+      if (!snap.id || !snap.pos[2] || !snap.vel[2] ||
+          !hip_ok(hipHostMalloc((void **)&snap.host, host_bytes), "snapshot staging") ||
+          !hip_ok(hipStreamCreateWithFlags(&snap.stream, hipStreamNonBlocking), "snapshot stream") ||
+          !hip_ok(hipEventCreateWithFlags(&snap.ready, hipEventDisableTiming), "snapshot event"))
+        return hip_ok(hipErrorOutOfMemory, "snapshot staging");
+    }
+    for (int a = 0; a < 3; a++)
+      if (!hip_ok(hipMemcpyAsync(snap.pos[a], pos[a], sizeof(double) * N, hipMemcpyDeviceToDevice, stream), "snapshot") ||
+          !hip_ok(hipMemcpyAsync(snap.vel[a], pa[3 + a], sizeof(P) * N, hipMemcpyDeviceToDevice, stream), "snapshot"))
+        return false;
+    if (!hip_ok(hipMemcpyAsync(snap.id, id_a, sizeof(int) * N, hipMemcpyDeviceToDevice, stream), "snapshot") ||
+        !hip_ok(hipEventRecord(snap.ready, stream), "snapshot"))
+      return false;
+    std::string file(path);
+    std::vector<double> obs(obs6 ? obs6 : nullptr, obs6 ? obs6 + 6 : nullptr);
+    snap.ok = true;
+    snap.err.clear();
+    snap.writer = std::thread([this, dev, file, u, actualtime, obs]() {
+      unsigned char *h = snap.host;
+      bool ok = hip_ok(hipSetDevice(dev), "snapshot device") &&
+                hip_ok(hipStreamWaitEvent(snap.stream, snap.ready, 0), "snapshot");
+      for (int a = 0; a < 3 && ok; a++)
+        ok = hip_ok(hipMemcpyAsync(h + 8 * (size_t)a * N, snap.pos[a], sizeof(double) * N, hipMemcpyDeviceToHost,
+                                   snap.stream),
+                    "snapshot download") &&
+             hip_ok(hipMemcpyAsync(h + (24 + sizeof(P) * a) * (size_t)N, snap.vel[a], sizeof(P) * N,
+                                   hipMemcpyDeviceToHost, snap.stream),
+                    "snapshot download");
+      ok = ok &&
+           hip_ok(hipMemcpyAsync(h + (24 + 3 * sizeof(P)) * (size_t)N, snap.id, sizeof(int) * N,
+                                 hipMemcpyDeviceToHost, snap.stream),
+                  "snapshot download") &&
+           hip_ok(hipStreamSynchronize(snap.stream), "snapshot download") &&
+           write_snapshot_file(file.c_str(), u, actualtime, obs.empty() ? nullptr : obs.data(), h);
+      snap.ok = ok;
+      if (!ok)
+        snap.err = g_last_error; // errors are kept per thread
+    });
+    return true;
+  }
+
+  bool snapshot_wait() override
+  {
+    if (snap.writer.joinable())
+      snap.writer.join();
+    if (!snap.ok)
+    {
+      set_error("%s", snap.err.c_str());
+      snap.ok = true;
+      return false;
+    }
+    return true;
+  }
+
   // ---------------------------------------------------------------- adjoint (qz_adjoint.inl)
   void release_adjoint();
   bool ensure_adjoint();
@@ -1440,6 +1582,113 @@ int qz_write_snapshot(qz_ctx *ctx, const char *path, const qz_units *units, doub
     return 1;
   }
   return ctx->eng->write_snapshot(path, *units, actualtime, with_observables != 0) ? 0 : 1;
+}
+
+int qz_write_snapshot_async(qz_ctx *ctx, const char *path, const qz_units *units, double actualtime,
+                            const double *obs6)
+{
+  QZ_CHECK_CTX(ctx);
+  if (!path || !units)
+  {
+    qz::set_error("qz_write_snapshot_async: null argument");
+    return 1;
+  }
+  return ctx->eng->write_snapshot_async(path, *units, actualtime, obs6) ? 0 : 1;
+}
+
+int qz_snapshot_wait(qz_ctx *ctx)
+{
+  QZ_CHECK_CTX(ctx);
+  return ctx->eng->snapshot_wait() ? 0 : 1;
+}
+
+// Parameters.dat in the layout of output_main_parameters.c:167-230: the 38
+// "name value" lines (example_plot.py reads these), then the wall times in
+// the reference's categories.  The solver lines describe this library's solve
+// under the reference's names; the reference's own tree-adaptation, potential
+// sub-timers and memory tables describe its host data structures and are not
+// written.
+int qz_write_parameters(qz_ctx *ctx, const char *path, const qz_units *units, const qz_run_info *info)
+{
+  QZ_CHECK_CTX(ctx);
+  if (!path || !units || !info)
+  {
+    qz::set_error("qz_write_parameters: null argument");
+    return 1;
+  }
+  const qz_config &c = ctx->eng->cfg;
+  FILE *f = fopen(path, "w");
+  if (!f)
+  {
+    qz::set_error("Error opening file %s", path);
+    return 1;
+  }
+  const int pre = c.pre_sweeps, post = c.post_sweeps;
+  fprintf(f, "lmin %d\n", c.lmin);
+  fprintf(f, "lmax %d\n", c.lmax);
+  fprintf(f, "_MAX_NUMBER_OF_ITERATIONS_IN_POISSON_EQUATION_ %d\n", c.max_iterations);
+  fprintf(f, "_ERROR_THRESHOLD_IN_THE_POISSON_EQUATION_ %1.5e\n", c.tol);
+  fprintf(f, "_ERROR_THRESHOLD_IN_THE_POISSON_EQUATION_2 %1.5e\n", c.tol_change);
+  fprintf(f, "check_poisson_error_method %d\n", 0); // RMS residual + pointwise change (poisson_error.c:36-133)
+  fprintf(f, "multigrid_cycle %d\n", 0);            // V-cycles
+  fprintf(f, "solverPreS %d\n", 0);                 // Gauss-Seidel family (red-black here)
+  fprintf(f, "solverfinddphic %d\n", 0);
+  fprintf(f, "solverPostS %d\n", 0);
+  fprintf(f, "_NiterPreS_ %d\n", pre);
+  fprintf(f, "_NiterPostS_ %d\n", post);
+  fprintf(f, "_Niterfinddphic_ %d\n", 2);
+  fprintf(f, "_Iter_branches_solver_ %d\n", c.max_iterations);
+  fprintf(f, "_w_SOR_ %1.5e\n", 1.3);      // node red half sweeps
+  fprintf(f, "_w_SOR_HEAD_ %1.5e\n", 1.4); // head red half sweeps
+  fprintf(f, "head_pot_method %d\n", 0);
+  fprintf(f, "_CFL_ %1.5e\n", info->cfl);
+  fprintf(f, "_MAX_dt_ %1.5e\n", info->max_dt);
+  fprintf(f, "ref_criterion_ptcl %d\n", c.ref_criterion_ptcl);
+  fprintf(f, "n_exp %d\n", c.n_exp);
+  fprintf(f, "force_stencil %d\n", 0); // 3-point central differences
+  fprintf(f, "potential_energy_type %d\n", info->potential_energy_type);
+  fprintf(f, "GC_iter %d\n", info->gc_iter);
+  fprintf(f, "Ngrid %d\n", (1 << c.lmin) + 1);
+  fprintf(f, "Nparticles_initial %d\n", info->n_particles_initial > 0 ? info->n_particles_initial : c.n_particles);
+  fprintf(f, "Nparticles_final %d\n", c.n_particles);
+  fprintf(f, "Number_timesteps\t%d\n", info->number_timesteps);
+  fprintf(f, "Number_outputs %d\n", info->number_outputs);
+  fprintf(f, "fr_output %d\n", info->fr_output);
+  fprintf(f, "MaxIterations %d\n", info->max_iterations);
+  fprintf(f, "BoxSize %1.5e\n", info->box_size);
+  fprintf(f, "vtype %d\n", 2); // double
+  fprintf(f, "Maxdt %1.5e years\n", info->final_time / units->time);
+  fprintf(f, "G %1.5e\n", c.G);
+  fprintf(f, "boundary_type %d\n", 1); // reflexive
+  fprintf(f, "GL_total_mass_initial %f\n", info->total_mass_initial);
+  fprintf(f, "GL_total_mass_final %f\n", info->total_mass_final);
+  static const char *names[16] = {"Global variables",    "Input",
+                                  "Initialization",      "Tree Construction",
+                                  "Grid Density",        "Total potential",
+                                  "Grid Acceleration",   "Particle acceleration",
+                                  "Time-step computing", "Particle Updating A",
+                                  "Tree Adaptation",     "Reset",
+                                  "Particle Updating B", "Observables",
+                                  "Garbage Collector",   "Output Main Parameters"};
+  double times[20] = {0.0};
+  if (info->times)
+    for (int i = 0; i < 20; i++)
+      times[i] = info->times[i];
+  double total = 0.0;
+  for (int i = 0; i < 20; i++)
+    total += times[i];
+  const double pct = total > 0.0 ? 100.0 / total : 0.0;
+  fprintf(f, "\n\nTIMES in seconds\n\n");
+  fprintf(f, "TOTAL %.3e\n", total);
+  for (int i = 0; i < 15; i++)
+    fprintf(f, "%s = %1.2e ~ %.1f %%\n", names[i], times[i], times[i] * pct);
+  fprintf(f, "Output Snapshots = %.2e ~ %.1f %%\n", times[19], times[19] * pct);
+  if (fclose(f) != 0)
+  {
+    qz::set_error("writing %s failed", path);
+    return 1;
+  }
+  return 0;
 }
 
 int qz_bench_sweep(qz_ctx *ctx, int reps, double *ms_per_launch, double *bytes_per_launch)

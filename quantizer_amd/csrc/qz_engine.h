@@ -109,6 +109,8 @@ struct EngineBase
   virtual bool node_fields(int i, double *out5) = 0;
   virtual bool bench_sweep(int reps, double *ms, double *bytes) = 0;
   virtual bool write_snapshot(const char *path, const qz_units &u, double actualtime, bool with_obs) = 0;
+  virtual bool write_snapshot_async(const char *path, const qz_units &u, double actualtime, const double *obs6) = 0;
+  virtual bool snapshot_wait() = 0;
   virtual bool copy_probe(size_t bytes, double *ms) = 0;
   virtual bool join_ahead() = 0; // wait for host work run ahead on a thread
 };

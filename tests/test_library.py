@@ -15,7 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBS = {"qz.h": [core.LIB_PATH],
         "qz_dropin.h": [os.path.join(ROOT, "quantizer_amd", "libqzdropin_resident.so")]}
 # functions of qz_dropin.h that only the resident build exports
-RESIDENT_ONLY = {"particle_updating_A", "particle_updating_B", "tree_adaptation"}
+RESIDENT_ONLY = {"particle_updating_A", "particle_updating_B", "tree_adaptation", "timestep", "observables",
+                 "output_snapshots", "garbage_collector", "qz_dropin_sync_host"}
 
 
 def declared_functions(header=None):
