@@ -148,11 +148,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
 
 // The force pass up to the grid accelerations: tree, density, head and node
 // potentials, grid accelerations (no-ops for the tree stages on the head grid).
-template <typename P, typename G> bool Engine<P, G>::force_fields()
-{
-  return reset() && tree_adaptation() && grid_density() && potential_head() && potential_branches() &&
-         grid_acceleration();
-}
+template <typename P, typename G> bool Engine<P, G>::force_fields() { return force_overlapped(false); }
 
 // Grow-only adjoint pools of the refinement levels (grid-pool layout) and the
 // transfer records (8 per boundary point of the largest level).

@@ -27,6 +27,8 @@ struct BmgPlan
   int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
   int nt0[kMaxMgLevels], ntn[kMaxMgLevels];
   double act_s[kMaxMgLevels], act_r[kMaxMgLevels];
+  double act_f[kMaxMgLevels]; // active points of the levels run by the fused tile kernels
+  double tail_act = 0.0;      // active points of the nodes' tail top levels
   hipGraphExec_t gexec = nullptr; // the PCG batch captured with the set-up (run ahead only)
 };
 
@@ -153,13 +155,20 @@ struct TreeSnap
   }
 };
 
-template <typename T> static bool regrow(T *&p, long long newcap, long long used, const char *what)
+// Grow a device buffer keeping its first `used` entries.  The copy runs on the
+// stream that writes the buffer (the tree may be built on a non-blocking
+// stream, which a null-stream copy would not wait for), and that stream is
+// drained before the old buffer is freed.
+template <typename T> static bool regrow(T *&p, long long newcap, long long used, const char *what, hipStream_t s)
 {
   T *np = nullptr;
   if (!hip_ok(hipMalloc((void **)&np, sizeof(T) * (size_t)std::max<long long>(newcap, 1)), what))
     return false;
-  if (p && used > 0)
-    (void)hipMemcpy(np, p, sizeof(T) * (size_t)used, hipMemcpyDeviceToDevice);
+  if (p && used > 0 &&
+      !hip_ok(hipMemcpyAsync(np, p, sizeof(T) * (size_t)used, hipMemcpyDeviceToDevice, s), what))
+    return false;
+  if (!hip_ok(hipStreamSynchronize(s), what))
+    return false;
   (void)hipFree(p);
   p = np;
   return true;
@@ -168,8 +177,10 @@ template <typename T> static bool regrow(T *&p, long long newcap, long long used
 static long long next_cap(long long cap, long long need) { return need <= cap ? cap : std::max(need, cap + cap / 2 + 1024); }
 
 // Grow-only buffers of the grid-point classification.
-static bool reserve_classify(TreeDev &d, int nch, int count)
+static bool reserve_classify(TreeDev &d, int nch, int count, hipStream_t s)
 {
+  if ((nch > d.ctab_cap || count + 1 > d.chunk0_cap) && !hip_ok(hipStreamSynchronize(s), "classify"))
+    return false; // the buffers about to be freed may still be read by the stream
   if (nch > d.ctab_cap)
   {
     (void)hipFree(d.ctab);
@@ -237,13 +248,13 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
 }
 
 // Reserve pools for the level about to be uploaded (grow-only, contents kept).
-template <typename G> static bool reserve_level(TreeDev *d, long long nz, long long ncell, long long nb, long long npt,
+template <typename G> static bool reserve_level(TreeDev *d, hipStream_t s, long long nz, long long ncell, long long nb, long long npt,
                                                 long long ng)
 {
   long long c = next_cap(d->zone_cap, d->used_zone + nz);
   if (c != d->zone_cap)
   {
-    if (!regrow(d->zone_node, c, d->used_zone, "zone pool") || !regrow(d->zone_xyz, 3 * c, 3 * d->used_zone, "zone pool"))
+    if (!regrow(d->zone_node, c, d->used_zone, "zone pool", s) || !regrow(d->zone_xyz, 3 * c, 3 * d->used_zone, "zone pool", s))
       return false;
     d->zone_cap = c;
   }
@@ -251,8 +262,8 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
   if (c != d->cell_cap)
   {
     long long u = d->used_cell;
-    if (!regrow(d->cell_box, c, u, "cell pool") || !regrow(d->cstart, c, u, "cell pool") ||
-        !regrow(d->cend, c, u, "cell pool") || !regrow(d->cell_xyz, 3 * c, 3 * u, "cell pool"))
+    if (!regrow(d->cell_box, c, u, "cell pool", s) || !regrow(d->cstart, c, u, "cell pool", s) ||
+        !regrow(d->cend, c, u, "cell pool", s) || !regrow(d->cell_xyz, 3 * c, 3 * u, "cell pool", s))
       return false;
     d->cell_cap = c;
   }
@@ -260,7 +271,7 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
   if (c != d->box_cap)
   {
     long long u = d->used_box;
-    if (!regrow(d->box, c, u, "box pool") || !regrow(d->bstart, c, u, "box pool") || !regrow(d->bend, c, u, "box pool"))
+    if (!regrow(d->box, c, u, "box pool", s) || !regrow(d->bstart, c, u, "box pool", s) || !regrow(d->bend, c, u, "box pool", s))
       return false;
     d->box_cap = c;
   }
@@ -268,10 +279,10 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
   if (c != d->pt_cap)
   {
     long long u = d->used_pt;
-    if (!regrow(d->intr_g, c, u, "point pool") || !regrow(d->bdry_g, c, u, "point pool") ||
-        !regrow(d->sim_g, c, u, "point pool") ||
-        !regrow(d->intr_xyz, 3 * c, 3 * u, "point pool") || !regrow(d->bdry_xyz, 3 * c, 3 * u, "point pool") ||
-        !regrow(d->sim_xyz, 3 * c, 3 * u, "point pool"))
+    if (!regrow(d->intr_g, c, u, "point pool", s) || !regrow(d->bdry_g, c, u, "point pool", s) ||
+        !regrow(d->sim_g, c, u, "point pool", s) ||
+        !regrow(d->intr_xyz, 3 * c, 3 * u, "point pool", s) || !regrow(d->bdry_xyz, 3 * c, 3 * u, "point pool", s) ||
+        !regrow(d->sim_xyz, 3 * c, 3 * u, "point pool", s))
       return false;
     d->pt_cap = c;
   }
@@ -280,10 +291,10 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
   {
     long long u = d->used_grid;
     G *rho = (G *)d->rho, *ax = (G *)d->ax, *ay = (G *)d->ay, *az = (G *)d->az;
-    if (!regrow(d->phi, c, u, "grid pool") || !regrow(d->phi_old, c, u, "grid pool") ||
-        !regrow(d->cg_r, c, u, "grid pool") || !regrow(d->cg_p, c, u, "grid pool") ||
-        !regrow(d->cg_Ap, c, u, "grid pool") || !regrow(rho, c, u, "grid pool") || !regrow(ax, c, u, "grid pool") ||
-        !regrow(ay, c, u, "grid pool") || !regrow(az, c, u, "grid pool"))
+    if (!regrow(d->phi, c, u, "grid pool", s) || !regrow(d->phi_old, c, u, "grid pool", s) ||
+        !regrow(d->cg_r, c, u, "grid pool", s) || !regrow(d->cg_p, c, u, "grid pool", s) ||
+        !regrow(d->cg_Ap, c, u, "grid pool", s) || !regrow(rho, c, u, "grid pool", s) || !regrow(ax, c, u, "grid pool", s) ||
+        !regrow(ay, c, u, "grid pool", s) || !regrow(az, c, u, "grid pool", s))
       return false;
     d->rho = rho;
     d->ax = ax;
@@ -300,8 +311,9 @@ template <typename G> static bool reserve_level(TreeDev *d, long long nz, long l
 // host receives zone bounding boxes and sizes, plus the zone cells only when the
 // children can refine further.  d.head_child serves as the cell -> list map
 // until the node upload overwrites the zone cells with node indices.
-template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zone_cells)
+template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zone_cells, hipStream_t s)
 {
+  hipStream_t stream = s; // every launch and copy below goes to the tree build's stream
   TreeDev &d = *tree->dev;
   TreeHost &th = *tree->host;
   const long long ncell = (long long)n * n * n;
@@ -337,6 +349,8 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
   }
   if (M + 1 > d.hcap)
   {
+    if (!hip_ok(hipStreamSynchronize(stream), "head zones"))
+      return false; // the buffers about to be replaced may still be read by the stream
     long long cap = next_cap(d.hcap, M + 1);
     int **ints[] = {&d.hpar, &d.hflag, &d.hrank, &d.hidx, &d.hidx_s, &d.hz_node, &d.hzstart};
     for (int **q : ints)
@@ -363,13 +377,15 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
       return false;
     d.hcap = cap;
   }
-  QZ_LAUNCH("tree_head_zones", k_head_index, nblk(M), d.hlist, M, d.head_child, d.hpar);
-  QZ_LAUNCH("tree_head_zones", k_head_hook, nblk(M), d.hlist, M, n, d.head_child, d.hpar);
-  QZ_LAUNCH("tree_head_zones", k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
+  // per marked cell: the list entry, the cell map, the six neighbours' map
+  // entries and the union-find parents
+  QZ_LAUNCHB("tree_head_zones", 12.0 * M, k_head_index, nblk(M), d.hlist, M, d.head_child, d.hpar);
+  QZ_LAUNCHB("tree_head_zones", 36.0 * M, k_head_hook, nblk(M), d.hlist, M, n, d.head_child, d.hpar);
+  QZ_LAUNCHB("tree_head_zones", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "head zones"))
     return false;
-  timer.end(stream, "tree_head_zones");
+  timer.end(stream, "tree_head_zones", 8.0 * (M + 1));
   if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.hrank + M, sizeof(int), hipMemcpyDeviceToHost, stream), "zones") ||
       !hip_ok(hipStreamSynchronize(stream), "zones"))
     return false;
@@ -377,14 +393,15 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
   int end_bit = 1;
   while ((1LL << end_bit) < nz)
     end_bit++;
-  QZ_LAUNCH("tree_head_zones", k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
+  QZ_LAUNCHB("tree_head_zones", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
   timer.begin(stream);
   if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream),
               "head zones"))
     return false;
-  timer.end(stream, "tree_head_zones");
-  QZ_LAUNCH("tree_head_zones", k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
-  QZ_LAUNCH("tree_head_zones", k_head_zone_cells, nblk(M), M, d.hzone_s, d.hidx_s, d.hlist, n, 0, d.hzstart, d.hbbox,
+  timer.end(stream, "tree_head_zones", 16.0 * M);
+  QZ_LAUNCHB("tree_head_zones", 24.0 * nz, k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
+  QZ_LAUNCHB("tree_head_zones", 40.0 * M, k_head_zone_cells, nblk(M), M, d.hzone_s, d.hidx_s, d.hlist, n, 0, d.hzstart,
+             d.hbbox,
             d.hz_node, d.hz_xyz);
   std::vector<int> zstart(nz + 1), bbox(6 * (size_t)nz), zsize(nz);
   const bool need_cells = cfg.lmin + 1 < cfg.lmax;
@@ -403,10 +420,9 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
   return true;
 }
 
-template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
+// The tree's host and device state, created on first use.
+template <typename P, typename G> bool Engine<P, G>::tree_state()
 {
-  if (cfg.lmin == cfg.lmax)
-    return true;
   if (!tree)
   {
     tree = new TreeState();
@@ -424,20 +440,43 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         !hip_ok(hipHostMalloc((void **)&d->hflag_count, sizeof(int) * 4), "pinned"))
       return false;
   }
-  if (!join_ahead())
+  return true;
+}
+
+template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
+{
+  if (cfg.lmin == cfg.lmax)
+    return true;
+  if (!tree_state() || !join_ahead())
     return false;
   tree->dev->drop_plan();
   HostSpan span_all(timer, "tree_adaptation");
-  if (!sort_particles())
+  if (!sort_particles() || !build_tree(stream))
     return false;
+  // the first level's node multigrid set-up is host work that needs only the
+  // tree: run it during the head density and solve
+  TreeDev &d = *tree->dev;
+  if (setup_ahead && cfg.branch_solver == QZ_BRANCH_MG && d.lfirst.size() > 1 && !timer.on && !timer.host_on)
+    return start_ahead(false);
+  return true;
+}
+
+// The tree below the head from the sorted particles (tree_construction.c rules):
+// head marks and zones on the device, the child boxes of every level on the
+// host, the node tables, cell and point lists on the device.  Everything is
+// issued on stream s (the engine's stream, or the set-up stream when the tree
+// is built while the head density and potential are computed).
+template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
+{
+  hipStream_t stream = s; // every launch and copy below goes to s
   TreeHost &th = *tree->host;
   TreeDev &d = *tree->dev;
   const size_t ncell_head = (size_t)n * n * n;
-  QZ_LAUNCH("tree_fill", k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
+  QZ_LAUNCHB("tree_fill", 4.0 * ncell_head, k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
   int head_zone_cells = 0;
   {
     HostSpan span(timer, "tree_head_zones");
-    if (!refine_head_device(head_zone_cells))
+    if (!refine_head_device(head_zone_cells, stream))
       return false;
   }
   d.hnodes.clear();
@@ -471,7 +510,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       ng += h.gcap;
       pmax = std::max<long long>(pmax, h.pcap);
     }
-    if (!reserve_level<G>(&d, nz, ncell, nb, npt, ng))
+    if (!reserve_level<G>(&d, stream, nz, ncell, nb, npt, ng))
       return false;
     const bool dev_zones = lr == 1; // head zones were built on the device
     std::vector<int> zn(dev_zones ? 0 : nz), zx(dev_zones ? 0 : 3 * nz);
@@ -532,7 +571,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
     if ((long long)d.hnodes.size() > d.node_cap)
     {
       long long cap = next_cap(d.node_cap, (long long)d.hnodes.size() + 64);
-      if (!regrow(d.nodes, cap, (long long)first, "node table"))
+      if (!regrow(d.nodes, cap, (long long)first, "node table", stream))
         return false;
       d.node_cap = (int)cap;
     }
@@ -552,8 +591,11 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
                 "zones"))
       return false;
     // ---- device: boxes, cells, ranges, masses, grid-point lists
-    QZ_LAUNCH("tree_boxes", k_init_boxes, nblk(nb), d.nodes, first, count, b0, nb, d.box, d.bstart, d.bend);
-    QZ_LAUNCH("tree_cells", k_expand_cells, nblk(ncell), d.nodes, d.zone_node, d.zone_xyz, (int)c0, (int)ncell,
+    QZ_LAUNCHB("tree_boxes", 12.0 * nb, k_init_boxes, nblk(nb), d.nodes, first, count, b0, nb, d.box, d.bstart,
+               d.bend); // box status and its particle range
+    // per cell: the zone cell in, its coordinates, box index and particle range out
+    QZ_LAUNCHB("tree_cells", 36.0 * ncell, k_expand_cells, nblk(ncell), d.nodes, d.zone_node, d.zone_xyz, (int)c0,
+               (int)ncell,
               cfg.lmin, D, key_out, N, d.cell_xyz, d.cell_box, d.cstart, d.cend, d.box, d.bstart, d.bend);
     {
       long long maxcell = 0;
@@ -565,7 +607,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       if (need > d.mpart_cap)
       {
         long long cap = next_cap(d.mpart_cap, need);
-        if (!regrow(d.mpart, cap, 0, "mass partials"))
+        if (!regrow(d.mpart, cap, 0, "mass partials", stream))
           return false;
         d.mpart_cap = cap;
       }
@@ -587,7 +629,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
       const int nch = (int)ct.size();
       if (nch > 0)
       {
-        if (!reserve_classify(d, nch, count))
+        if (!reserve_classify(d, nch, count, stream))
           return false;
         if (!hip_ok(hipMemcpyAsync(d.ctab, ct.data(), sizeof(int2) * nch, hipMemcpyHostToDevice, stream), "classify") ||
             !hip_ok(hipMemcpyAsync(d.chunk0, c0v.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, stream),
@@ -605,16 +647,17 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
                            nch, d.coff, d.chunk0, d.intr_g, d.intr_xyz, d.bdry_g, d.bdry_xyz, d.sim_g, d.sim_xyz);
         hipLaunchKernelGGL(k_classify_totals, dim3(nblk(count)), dim3(kBlock), 0, stream, d.nodes, first, count, nch,
                            d.ccnt, d.coff, d.chunk0);
-        timer.end(stream, "tree_classify");
+        timer.end(stream, "tree_classify", 8.0 * npt); // box statuses in, listed points (index + xyz) out
       }
     }
-    QZ_LAUNCH("tree_children", k_mark_children, nblk(nz), d.nodes, d.zone_node, d.zone_xyz, (int)z0, (int)nz, n,
+    QZ_LAUNCHB("tree_children", 20.0 * nz, k_mark_children, nblk(nz), d.nodes, d.zone_node, d.zone_xyz, (int)z0,
+               (int)nz, n,
               d.head_child, d.box);
     // zero the level's grid fields (points that are never listed stay 0, as calloc leaves them)
-    QZ_LAUNCH("tree_fill", k_fill<double>, nblk(ng), d.phi + g0, 0.0, (size_t)ng);
-    QZ_LAUNCH("tree_fill", k_fill<G>, nblk(ng), (G *)d.ax + g0, (G)0, (size_t)ng);
-    QZ_LAUNCH("tree_fill", k_fill<G>, nblk(ng), (G *)d.ay + g0, (G)0, (size_t)ng);
-    QZ_LAUNCH("tree_fill", k_fill<G>, nblk(ng), (G *)d.az + g0, (G)0, (size_t)ng);
+    QZ_LAUNCHB("tree_fill", 8.0 * ng, k_fill<double>, nblk(ng), d.phi + g0, 0.0, (size_t)ng);
+    QZ_LAUNCHB("tree_fill", (double)sizeof(G) * ng, k_fill<G>, nblk(ng), (G *)d.ax + g0, (G)0, (size_t)ng);
+    QZ_LAUNCHB("tree_fill", (double)sizeof(G) * ng, k_fill<G>, nblk(ng), (G *)d.ay + g0, (G)0, (size_t)ng);
+    QZ_LAUNCHB("tree_fill", (double)sizeof(G) * ng, k_fill<G>, nblk(ng), (G *)d.az + g0, (G)0, (size_t)ng);
     // ---- next level's refinement counts
     int lv = cfg.lmin + lr;
     if (lv < cfg.lmax)
@@ -630,7 +673,9 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
           return false;
         int nf = std::min(*d.hflag_count, d.flag_cap);
         std::vector<int> fl(2 * (size_t)nf);
-        if (nf > 0 && !hip_ok(hipMemcpy(fl.data(), d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost), "flags"))
+        if (nf > 0 &&
+            (!hip_ok(hipMemcpyAsync(fl.data(), d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost, stream), "flags") ||
+             !hip_ok(hipStreamSynchronize(stream), "flags")))
           return false;
         fn.resize(nf);
         fc.resize(nf);
@@ -678,6 +723,8 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
     need = std::max<long long>(need, (long long)d.lcount[l] * blocks_per_node(d.lpmax[l]));
   if (need > d.part_cap)
   {
+    if (!hip_ok(hipStreamSynchronize(stream), "partials"))
+      return false;
     (void)hipFree(d.part1);
     (void)hipFree(d.part2);
     (void)hipFree(d.part3);
@@ -687,11 +734,42 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
         !hip_ok(hipMalloc((void **)&d.part3, sizeof(double) * d.part_cap), "partials"))
       return false;
   }
-  // the first level's node multigrid set-up is host work that needs only the
-  // tree: run it during the head density and solve
-  if (setup_ahead && cfg.branch_solver == QZ_BRANCH_MG && d.lfirst.size() > 1 && !timer.on && !timer.host_on)
-    return start_ahead();
   return true;
+}
+
+// A force pass (reset, tree, density, potential, grid accelerations and, with
+// with_particles, the particle accelerations) in which the tree below the head
+// is built on the set-up thread and stream while the engine's stream deposits
+// the head density and solves the head potential, which do not depend on the
+// tree.  The node stages then run as in the stage sequence, with the same
+// kernels on the same data, so the results are those of the stages called one
+// by one.  With timing on, or QZ_NODE_SETUP_AHEAD=0, the stages run in order.
+template <typename P, typename G> bool Engine<P, G>::force_overlapped(bool with_particles)
+{
+  if (cfg.lmin == cfg.lmax || !setup_ahead || timer.on)
+    return reset() && tree_adaptation() && grid_density() && potential_head() && potential_branches() &&
+           grid_acceleration() && (!with_particles || particle_acceleration());
+  if (!reset() || !tree_state() || !join_ahead())
+    return false;
+  TreeDev &d = *tree->dev;
+  d.drop_plan();
+  if (!sort_particles() || !start_ahead(true))
+    return false;
+  bool head_ok;
+  {
+    HostSpan span(timer, "overlap_head");
+    head_ok = deposit_head() && potential_head();
+  }
+  {
+    HostSpan span(timer, "overlap_join_wait");
+    if (!join_ahead() || !head_ok)
+      return false;
+  }
+  if (!hip_ok(hipStreamWaitEvent(stream, d.ahead_done, 0), "set-up stream"))
+    return false;
+  return (!has_tree() || branch_density()) && potential_branches() && grid_acceleration() &&
+         (!with_particles || particle_acceleration()) &&
+         hip_ok(hipGetLastError(), "force pass"); // stream-ordered; readers synchronise
 }
 
 template <typename P, typename G> bool Engine<P, G>::branch_density()
@@ -699,7 +777,8 @@ template <typename P, typename G> bool Engine<P, G>::branch_density()
   TreeDev &d = *tree->dev;
   for (size_t l = 1; l < d.lfirst.size(); l++)
   {
-    QZ_LAUNCH("branch_fill", k_fill<G>, nblk(d.lng[l]), (G *)d.rho + d.lg0[l], (G)0, (size_t)d.lng[l]);
+    QZ_LAUNCHB("branch_fill", (double)sizeof(G) * d.lng[l], k_fill<G>, nblk(d.lng[l]), (G *)d.rho + d.lg0[l], (G)0,
+               (size_t)d.lng[l]);
     const int lv = cfg.lmin + (int)l;
     QZ_LAUNCHB("deposit_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0], pos[1],
                pos[2], pa[9], N, lv, 3 * (D - (int)l), (1 << lv) - 1, tails);
@@ -709,14 +788,14 @@ template <typename P, typename G> bool Engine<P, G>::branch_density()
     hipLaunchKernelGGL((k_branch_deposit_fold<G, G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
                        d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.intr_xyz, d.bstart, d.bend, tails, (G *)d.rho,
                        4.0 * cfg.G * kPi);
-    timer.end(stream, "deposit_branch_fold");
+    timer.end(stream, "deposit_branch_fold", 4.0 * d.lncell[l] + (double)sizeof(G) * d.lng[l]);
     // node masses for the CG threshold (poisson_error.c:240) from this level's runs
     timer.begin(stream);
     hipLaunchKernelGGL(k_node_mass<G>, dim3(d.lnbm[l], d.lcount[l]), dim3(kBlock), 0, stream, d.nodes, d.lfirst[l],
                        d.cstart, d.cend, tails, d.mpart);
     hipLaunchKernelGGL(k_node_mass_fold, dim3(d.lcount[l]), dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.mpart,
                        d.lnbm[l]);
-    timer.end(stream, "node_mass");
+    timer.end(stream, "node_mass", 8.0 * d.lncell[l]);
   }
   return true;
 }
@@ -741,7 +820,7 @@ template <typename P, typename G> bool Engine<P, G>::potential_branches()
     timer.begin(stream);
     hipLaunchKernelGGL(k_branch_init_pot, grid, dim3(kBlock), 0, stream, d.nodes, first, d.sim_g, d.sim_xyz, d.bdry_g,
                        d.bdry_xyz, d.intr_g, d.intr_xyz, d.phi, d.phi_old, phi, n, cfg.G * total_mass, zero_bc);
-    timer.end(stream, "branch_init_pot");
+    timer.end(stream, "branch_init_pot", 24.0 * d.lng[l]); // phi and phi_old out, parent averages
     bool solved = false;
     if (cfg.branch_solver == QZ_BRANCH_MG && !branch_mg_level(l, solved, d.phi, (const G *)d.rho, false))
       return false;
@@ -844,7 +923,8 @@ template <typename P, typename G> bool Engine<P, G>::branch_cg_level(size_t l)
   const int nbx = blocks_per_node(d.lpmax[l]);
   dim3 grid(nbx, count);
   const double ni = (double)d.lnintr[l];
-  QZ_LAUNCH("branch_fill", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
+  QZ_LAUNCHB("branch_fill", 8.0 * d.lng[l], k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0,
+             (size_t)d.lng[l]);
   timer.begin(stream);
   hipLaunchKernelGGL((k_cg_init<G>), grid, dim3(kBlock), 0, stream, d.nodes, first, d.intr_g, d.phi, (G *)d.rho,
                      d.cg_r, d.cg_p, d.part2);
@@ -1075,16 +1155,16 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
   if (!hip_ok(hipMemcpyAsync(d.chunks, tab.data(), sizeof(Chunk) * dense_end, hipMemcpyHostToDevice, stream),
               "chunk table"))
     return false;
-  QZ_LAUNCH("bmg_setup", k_fill<unsigned char>, nblk(pool), mk, (unsigned char)0, (size_t)pool);
-  QZ_LAUNCH("bmg_setup", k_fill<G>, nblk(pool), e, (G)0, (size_t)pool);
+  QZ_LAUNCHB("bmg_setup", (double)pool, k_fill<unsigned char>, nblk(pool), mk, (unsigned char)0, (size_t)pool);
+  QZ_LAUNCHB("bmg_setup", (double)sizeof(G) * pool, k_fill<G>, nblk(pool), e, (G)0, (size_t)pool);
   QZ_LAUNCH("bmg_setup", k_fill<int>, 1, d.done_count, 0, (size_t)2);
   if (!hip_ok(hipMemcpyAsync(bm, hb.data(), sizeof(BMg) * count, hipMemcpyHostToDevice, stream), "multigrid table"))
     return false;
   if (n_itab > 0)
-    QZ_LAUNCH("bmg_setup", k_bmg_mask0, n_itab, d.nodes, first, bm, itab, d.intr_g, mk);
+    QZ_LAUNCHB("bmg_setup", 5.0 * d.lnintr[l], k_bmg_mask0, n_itab, d.nodes, first, bm, itab, d.intr_g, mk);
   for (int m = 0; m + 1 < mmax; m++)
     if (cn[m + 1] > 0)
-      QZ_LAUNCH("bmg_setup", k_bmg_mask_coarsen, cn[m + 1], bm, C(m + 1), m, mk, mk);
+      QZ_LAUNCHB("bmg_setup", 2.0 * ptsm[m + 1], k_bmg_mask_coarsen, cn[m + 1], bm, C(m + 1), m, mk, mk);
 
   // ---- active points: the pool indices of all mask bytes, in pool order, and
   // where each (node, level, colour) region starts in that list
@@ -1107,7 +1187,7 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
     return false;
   hipLaunchKernelGGL(k_bmg_region_starts, dim3(nblk(nreg + 1)), dim3(kBlock), 0, stream, d.alist, d.acount, d.rbeg,
                      nreg + 1, d.rstart);
-  timer.end(stream, "bmg_setup");
+  timer.end(stream, "bmg_setup", 5.0 * pool); // mask bytes in, active point indices out
   std::vector<int> rstart(nreg + 1);
   {
     HostSpan span_w(timer, "bmg_prepare_wait");
@@ -1122,6 +1202,7 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
   // whose tail starts at or below it
   int sw0[kMaxMgLevels][2], swn[kMaxMgLevels][2], rs0[kMaxMgLevels], rsn[kMaxMgLevels];
   double act_s[kMaxMgLevels] = {0.0}, act_r[kMaxMgLevels] = {0.0}; // active points per table
+  double act_f[kMaxMgLevels] = {0.0}, tail_act = 0.0; // fused levels and tail top levels (byte accounting)
   std::vector<int> reg0(count); // first region of each node
   for (int q = 0, rr = 0; q < count; q++)
   {
@@ -1137,6 +1218,13 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
   int nt0[kMaxMgLevels] = {0}, ntn[kMaxMgLevels] = {0};
   std::vector<NTile> &ntab = d.hntiles;
   ntab.clear();
+  for (int q = 0, rr = 0; q < count; rr += 2 * hb[q].nlev, q++)
+  {
+    tail_act += rstart[rr + 2 * hb[q].ts + 2] - rstart[rr + 2 * hb[q].ts];
+    for (int m = 1; m < hb[q].ts && m + 1 < hb[q].nlev; m++)
+      if (fused(m))
+        act_f[m] += rstart[rr + 2 * m + 2] - rstart[rr + 2 * m];
+  }
   for (int m = 1; m < mmax; m++)
   {
     nt0[m] = (int)ntab.size();
@@ -1217,6 +1305,8 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
   std::memcpy(pl.ntn, ntn, sizeof(ntn));
   std::memcpy(pl.act_s, act_s, sizeof(act_s));
   std::memcpy(pl.act_r, act_r, sizeof(act_r));
+  std::memcpy(pl.act_f, act_f, sizeof(act_f));
+  pl.tail_act = tail_act;
   pl.level = (int)l;
   return true;
 }
@@ -1236,9 +1326,12 @@ template <typename P, typename G> bool Engine<P, G>::join_ahead()
   return d.ahead_ok;
 }
 
-// Start the set-up of the first tree level on the set-up stream and a host
-// thread, ordered after the tree build on the engine's stream.
-template <typename P, typename G> bool Engine<P, G>::start_ahead()
+// Start work on the set-up stream and a host thread, ordered after the work
+// queued so far on the engine's stream: with build, the tree below the head
+// (build_tree), then, when the tree has levels, the first level's node
+// multigrid set-up and its captured PCG batch.  ahead_done marks the end of the
+// stream work the engine's stream must wait for.
+template <typename P, typename G> bool Engine<P, G>::start_ahead(bool build)
 {
   TreeDev &d = *tree->dev;
   // non-blocking: a capture on it is not disturbed by legacy-stream calls of other threads
@@ -1246,15 +1339,22 @@ template <typename P, typename G> bool Engine<P, G>::start_ahead()
                           !hip_ok(hipEventCreateWithFlags(&d.ahead_tree, hipEventDisableTiming), "set-up event") ||
                           !hip_ok(hipEventCreateWithFlags(&d.ahead_done, hipEventDisableTiming), "set-up event")))
     return false;
-  if (!hip_ok(hipEventRecord(d.ahead_tree, stream), "set-up event"))
+  // the set-up stream's wait is queued here, before this thread can begin a
+  // graph capture on the engine's stream (the head solve): a wait issued during
+  // that capture, on an event of the capturing stream, is refused
+  if (!hip_ok(hipEventRecord(d.ahead_tree, stream), "set-up event") ||
+      !hip_ok(hipStreamWaitEvent(d.ahead_stream, d.ahead_tree, 0), "set-up stream"))
     return false;
   d.ahead_ok = true;
-  d.ahead = std::thread([this, &d]() {
-    d.ahead_ok = hip_ok(hipStreamWaitEvent(d.ahead_stream, d.ahead_tree, 0), "set-up stream") &&
-                 bmg_prepare(1, d.ahead_stream) && hip_ok(hipEventRecord(d.ahead_done, d.ahead_stream), "set-up event");
+  d.ahead = std::thread([this, &d, build]() {
+    HostSpan span(timer, build ? "ahead_tree_and_setup" : "ahead_setup");
+    d.ahead_ok = !build || build_tree(d.ahead_stream);
+    const bool levels = d.ahead_ok && d.lfirst.size() > 1 && cfg.branch_solver == QZ_BRANCH_MG;
+    d.ahead_ok = d.ahead_ok && (!levels || bmg_prepare(1, d.ahead_stream)) &&
+                 hip_ok(hipEventRecord(d.ahead_done, d.ahead_stream), "set-up event");
     // the PCG batch of the level's potential solve, captured on the set-up stream
     // (after ahead_done: nothing recorded during the capture is work of its own)
-    if (d.ahead_ok && !d.plan.cg_only)
+    if (d.ahead_ok && levels && !d.plan.cg_only)
     {
       hipGraph_t g = nullptr;
       d.ahead_ok = hip_ok(hipStreamBeginCapture(d.ahead_stream, hipStreamCaptureModeRelaxed), "graph capture");
@@ -1292,6 +1392,7 @@ void Engine<P, G>::bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, do
   const auto &ntn = pl.ntn;
   const auto &act_s = pl.act_s;
   const auto &act_r = pl.act_r;
+  const auto &act_f = pl.act_f;
   auto fused = [&](int m) { return pl.nfuse && m >= 1; };
   const int inner = 1; // one V-cycle per PCG iteration (a second one costs more than the iterations it saves)
   G *e = (G *)d.mg_e, *r = (G *)d.mg_r, *et = (G *)d.mg_t;
@@ -1319,8 +1420,11 @@ void Engine<P, G>::bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, do
         if (fused(m))
         {
           if (ntn[m] > 0)
-            QZ_LAUNCH_T("bmg_down", kNmThreads, (k_bmg_down<G>), ntn[m], d.nodes, first, bm,
-                        (const NTile *)d.ntiles + nt0[m], m, r, et, (const unsigned char *)mk);
+            // reads r and the mask bits of the level, writes the pre-smoothed
+            // correction and the next level's residual
+            QZ_LAUNCH_TB("bmg_down", kNmThreads, act_f[m] * (2.0 * gs + 0.125) + act_r[m + 1] * gs, (k_bmg_down<G>),
+                         ntn[m], d.nodes, first, bm,
+                         (const NTile *)d.ntiles + nt0[m], m, r, et, (const unsigned char *)mk);
           continue;
         }
         for (int sw = 0; sw < pre; sw++)
@@ -1335,15 +1439,18 @@ void Engine<P, G>::bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, do
       timer.begin(stream);
       hipLaunchKernelGGL(k_bmg_tail<G>, dim3(count), dim3(kBmgTailThreads), tail_lds, stream, d.nodes, first, bm, e,
                          r, mk, pre, post, 100);
-      timer.end(stream, "bmg_tail");
+      timer.end(stream, "bmg_tail", pl.tail_act * 2.0 * gs); // r in, e out at each tail's top level
       for (int m = mmax - 2; m >= 0; m--)
       {
         if (fused(m))
         {
           if (ntn[m] > 0)
-            QZ_LAUNCH_T("bmg_up", kNmThreads, (k_bmg_up<G>), ntn[m], d.nodes, first, bm,
-                        (const NTile *)d.ntiles + nt0[m], m, e, (const G *)et, (const G *)r,
-                        (const unsigned char *)mk);
+            // reads the coarse correction, the pre-smoothed correction, r and the
+            // mask bits, writes the correction
+            QZ_LAUNCH_TB("bmg_up", kNmThreads, act_r[m + 1] * gs + act_f[m] * (3.0 * gs + 0.125), (k_bmg_up<G>),
+                         ntn[m], d.nodes, first, bm,
+                         (const NTile *)d.ntiles + nt0[m], m, e, (const G *)et, (const G *)r,
+                         (const unsigned char *)mk);
           continue;
         }
         const int np_ = swn[m][0] + swn[m][1];
@@ -1360,7 +1467,8 @@ void Engine<P, G>::bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, do
   };
   for (int il = 0; il < 2; il++)
   {
-    QZ_LAUNCH("bmg_check", k_bmg_check, count, bm, d.nodes, first, it_base, il, d.part2, d.part3, cfg.tol, cfg.G,
+    QZ_LAUNCHB("bmg_check", (double)count * (sizeof(BMg) + sizeof(DNode) + 32.0), k_bmg_check, count, bm, d.nodes,
+               first, it_base, il, d.part2, d.part3, cfg.tol, cfg.G,
               adjoint ? 1 : 0, d.done_count);
     if (n_itab == 0)
       continue;
@@ -1407,7 +1515,7 @@ bool Engine<P, G>::branch_mg_level(size_t l, bool &solved, double *phi_pool, con
   // per interior point: residual reads idx, phi, rho and writes r (FP64 + G);
   // zdots reads idx, z, r, q; direction reads idx, z, p, writes p; apply reads
   // idx, p, writes q; update reads idx, phi, p, q, r, writes phi, r (FP64 + G).
-  QZ_LAUNCH("bmg_setup", k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
+  QZ_LAUNCHB("bmg_setup", 8.0 * d.lng[l], k_fill<double>, nblk(d.lng[l]), d.cg_p + d.lg0[l], 0.0, (size_t)d.lng[l]);
   if (n_itab > 0)
     QZ_LAUNCHB("bmg_residual", ni * (4.0 + 8.0 + gs + 8.0 + gs), (k_bmg_residual<G>), n_itab, d.nodes, first, bm,
                itab, d.intr_g, phi_pool, rho_pool, d.cg_r, r, d.part2);
@@ -1439,7 +1547,7 @@ template <typename P, typename G> bool Engine<P, G>::branch_grid_acceleration()
     hipLaunchKernelGGL((k_branch_grid_acc<G>), grid, dim3(kBlock), 0, stream, d.nodes, d.lfirst[l], d.sim_g,
                        d.sim_xyz, d.bdry_g, d.bdry_xyz, d.intr_g, d.phi, (G *)d.ax, (G *)d.ay, (G *)d.az, gax, gay, gaz,
                        n, cfg.G * total_mass, zero_bc);
-    timer.end(stream, "grid_acc_branch");
+    timer.end(stream, "grid_acc_branch", (8.0 + 3.0 * sizeof(G)) * d.lng[l]);
   }
   return true;
 }
@@ -1447,7 +1555,11 @@ template <typename P, typename G> bool Engine<P, G>::branch_grid_acceleration()
 template <typename P, typename G> bool Engine<P, G>::branch_particle_acceleration()
 {
   TreeDev &d = *tree->dev;
-  QZ_LAUNCH("gather_amr", (k_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, d.head_child, d.nodes,
+  // per particle: position in, its head cell's map entry, acceleration out;
+  // per grid point of the head and the nodes: the three accelerations
+  const double npts = (double)(n + 1) * (n + 1) * (n + 1);
+  QZ_LAUNCHB("gather_amr", (28.0 + 3.0 * sizeof(P)) * N + 3.0 * sizeof(G) * (npts + (double)d.used_grid),
+             (k_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, d.head_child, d.nodes,
             d.box, (G *)d.ax, (G *)d.ay, (G *)d.az, pa[6], pa[7], pa[8], N, cfg.lmin);
   return true;
 }

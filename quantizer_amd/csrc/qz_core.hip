@@ -76,6 +76,7 @@ void Timer::end(hipStream_t s, const char *name, double bytes)
 
 void Timer::host(const char *name, double ms)
 {
+  std::lock_guard<std::mutex> lock(host_mu);
   std::string nm = std::string("host:") + name;
   auto it = std::find_if(acc.begin(), acc.end(), [&](const Acc &x) { return x.name == nm; });
   if (it == acc.end())
@@ -113,12 +114,13 @@ void Timer::collect()
 }
 
 #define QZ_LAUNCH(name, kern, grid, ...) QZ_LAUNCHB(name, 0.0, kern, grid, __VA_ARGS__)
-#define QZ_LAUNCH_T(name, threads, kern, grid, ...)                                                                   \
+#define QZ_LAUNCH_T(name, threads, kern, grid, ...) QZ_LAUNCH_TB(name, threads, 0.0, kern, grid, __VA_ARGS__)
+#define QZ_LAUNCH_TB(name, threads, bytes, kern, grid, ...)                                                           \
   do                                                                                                                  \
   {                                                                                                                   \
     timer.begin(stream);                                                                                              \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, stream, __VA_ARGS__);                                      \
-    timer.end(stream, name, 0.0);                                                                                     \
+    timer.end(stream, name, (double)(bytes));                                                                         \
   } while (0)
 #define QZ_LAUNCHB(name, bytes, kern, grid, ...)                                                                      \
   do                                                                                                                  \
@@ -534,9 +536,9 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 6; a < 9; a++)
-      QZ_LAUNCH("reset", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
+      QZ_LAUNCHB("reset", (double)sizeof(P) * N, k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
     acc_live = false;
-    QZ_LAUNCH("reset", k_fill<G>, nblk(npts), rho, (G)0, npts);
+    QZ_LAUNCHB("reset", (double)sizeof(G) * npts, k_fill<G>, nblk(npts), rho, (G)0, npts);
     return hip_ok(hipGetLastError(), "reset"); // stream-ordered; readers synchronise
   }
 
@@ -547,7 +549,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     if (ncell <= 0)
       return;
     QZ_LAUNCH("cell_totals", k_fill<int>, 1, nheavy, 0, (size_t)1);
-    QZ_LAUNCH("cell_totals", k_cell_totals<G>, nblk((size_t)ncell), cb, ce, ncell, tails, heavy, nheavy);
+    QZ_LAUNCHB("cell_totals", 8.0 * ncell, k_cell_totals<G>, nblk((size_t)ncell), cb, ce, ncell, tails, heavy,
+               nheavy); // the cell ranges; multi-window cells add their tails
     QZ_LAUNCH("cell_totals", k_cell_totals_heavy<G>, 1024, cb, ce, heavy, nheavy, tails);
   }
 
@@ -605,6 +608,17 @@ template <typename P, typename G> struct Engine final : EngineBase
 
   bool grid_density() override
   {
+    if (!deposit_head())
+      return false;
+    if (has_tree() && !branch_density())
+      return false;
+    return hip_ok(hipGetLastError(), "grid_density"); // stream-ordered; readers synchronise
+  }
+
+  // The head's density (grid_density.c:35-111): CIC weights of every particle
+  // summed per cell in slot order, then per grid point over its 8 cells.
+  bool deposit_head()
+  {
     if (!sort_particles())
       return false;
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
@@ -615,9 +629,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     cell_totals(cell_start, nullptr, (long long)n * n * n);
     QZ_LAUNCHB("deposit_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>), nblk(npts),
                cell_start, tails, rho, make_cube(n + 1), coeff);
-    if (has_tree() && !branch_density())
-      return false;
-    return hip_ok(hipGetLastError(), "grid_density"); // stream-ordered; readers synchronise
+    return true;
   }
 
   // V-cycle on the error equation of level l; the correction starts from zero
@@ -647,7 +659,7 @@ template <typename P, typename G> struct Engine final : EngineBase
       else
         hipLaunchKernelGGL((k_vcycle_tail<G, 8>), dim3(1), dim3(kTailThreads), 0, stream, lev_e[l], lev_r[l],
                            cfg.pre_sweeps, cfg.post_sweeps);
-      timer.end(stream, "vcycle_tail");
+      timer.end(stream, "vcycle_tail", 2.0 * sizeof(G) * (m + 1.0) * (m + 1.0) * (m + 1.0)); // r in, e out
       return;
     }
     if (lev_t[l] && mg_fused && cfg.pre_sweeps == 2 && cfg.post_sweeps == 2)
@@ -1127,13 +1139,16 @@ template <typename P, typename G> struct Engine final : EngineBase
   // ---------------------------------------------------------------- branches (qz_branch.inl)
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
-  bool refine_head_device(int &zone_cells);
+  bool refine_head_device(int &zone_cells, hipStream_t s);
+  bool tree_state();
+  bool build_tree(hipStream_t s);
+  bool force_overlapped(bool with_particles) override;
   bool potential_branches() override;
   bool branch_cg_level(size_t l);
   bool branch_mg_level(size_t l, bool &solved, double *phi_pool, const G *rho_pool, bool adjoint);
   bool bmg_prepare(size_t l, hipStream_t stream);
   bool join_ahead() override;
-  bool start_ahead();
+  bool start_ahead(bool build);
   bool ensure_bmg_buffers(int n_itab, long long n_chunks, long long pool, hipStream_t stream);
   template <typename F>
   bool iterate_batches(F &&batch, int per_batch, int max_it, int count, int &it, bool &all_done,
@@ -1425,10 +1440,9 @@ int qz_particle_acceleration(qz_ctx *ctx)
 int qz_force(qz_ctx *ctx)
 {
   QZ_CHECK_CTX(ctx);
-  if (qz_reset(ctx) || qz_tree_adaptation(ctx) || qz_grid_density(ctx) || qz_potential(ctx) ||
-      qz_grid_acceleration(ctx) || qz_particle_acceleration(ctx))
-    return 1;
-  return 0;
+  // the stages of main.c:296-338 in order; the tree below the head is built
+  // while the head density and potential are computed (same results)
+  return ctx->eng->force_overlapped(true) ? 0 : 1;
 }
 
 int qz_step(qz_ctx *ctx, double dt)

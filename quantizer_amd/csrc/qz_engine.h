@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,7 @@ struct Timer
   std::vector<hipEvent_t> pending; // open begin() events; timed regions may nest
   std::vector<Rec> recs;
   std::vector<Acc> acc;
+  std::mutex host_mu; // host spans may be recorded by the set-up thread too
   void begin(hipStream_t s);
   void end(hipStream_t s, const char *name, double bytes = 0.0);
   void collect();
@@ -113,6 +115,7 @@ struct EngineBase
   virtual bool snapshot_wait() = 0;
   virtual bool copy_probe(size_t bytes, double *ms) = 0;
   virtual bool join_ahead() = 0; // wait for host work run ahead on a thread
+  virtual bool force_overlapped(bool with_particles) = 0; // qz_force with the tree built during the head solve
 };
 
 EngineBase *make_engine(const qz_config &cfg);
