@@ -4,12 +4,17 @@
 Workload (default, "c3"): N = 2^24 particles on a 256^3 head grid (lmin = 8)
 with one refinement level (lmax = 9, a cell refines at >= 8 particles), half the
 particles uniform and half in 64 Gaussian clumps (sigma uniform in [1, 4] head
-cells), equal masses, zero velocities.  A step is one fixed-dt KDK step of the
-reference loop (particle_updating_A, reset, tree rebuild, grid_density,
-potential to 1e-10 relative residual, grid_acceleration, particle_acceleration,
-particle_updating_B).  dt is the reference CFL rule (CFL 0.5) evaluated after
-the initial force pass and then frozen.  Inputs are resident in HBM before the
-timed region starts.
+cells), equal masses, zero velocities.  A step is one iteration of the
+reference main loop (main.c:369-478): timestep (the CFL 0.5 rule, capped at the
+t = 0 value as _MAX_dt_ caps it), then the KDK step (particle_updating_A, reset,
+tree rebuild, grid_density, potential to 1e-10 relative residual,
+grid_acceleration, particle_acceleration, particle_updating_B).  Inputs are
+resident in HBM before the timed region starts.
+
+mixed_fixed_dt: the mixed path with dt held at its t = 0 value (the earlier
+definition of this workload, kept for comparison; the reference's own main
+loop, run that way, files particles of the collapsing clumps outside their
+node's box by the fourth step and then reads past its cell lists).
 
 value: particle-steps/s of the mixed path (FP32 particle velocities and
 accelerations, FP32 grids and V-cycles; FP64 positions, potentials, outer
@@ -40,8 +45,13 @@ precision_vs_fp64: mixed against FP64 on the same workload: head potential,
 grid and particle accelerations after the first force pass, and positions and
 velocities after 20 steps.
 
-cpu_baseline: the compiled reference (oracle/_ref) on a bounded sample of the
-same workload, 1 thread.
+dropin_resident: the reference's own host program (its main loop over the
+stages, the same per-step time step) linked against the resident drop-in
+library, on the same workload: the host-inclusive rate of the drop-in boundary.
+
+cpu_baseline: the compiled reference (oracle/_ref) on the same workload, at 1
+thread and at the host's thread count (1 warm-up and 3 timed steps each, the
+same per-step time step), with the CPU model.
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
@@ -87,14 +97,22 @@ def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inne
     sim = core.Simulation(parts, lmin, lmax, precision=precision, ref_criterion_ptcl=crit, inner_cycles=inner,
                           branch_solver=solver)
     sim.force()
-    dt = sim.timestep(cfl=0.5)
-    log(f"  prec={precision} setup+force {time.time() - t0:.2f}s dt={dt:.3e} stats={sim.stats()}")
-    for _ in range(warmup):
+    dt0 = sim.timestep(cfl=0.5)
+    log(f"  prec={precision} setup+force {time.time() - t0:.2f}s dt0={dt0:.3e} stats={sim.stats()}")
+
+    def step():
+        # main.c:369-478: the CFL time step from the current state, then the
+        # step; dt0 (the t = 0 step) is the cap, as _MAX_dt_ is in the reference
+        dt = sim.timestep(0.5, dt0)
         sim.step(dt)
+        return dt
+
+    for _ in range(warmup):
+        step()
     sim.sync()
     t1 = time.perf_counter()
     for _ in range(steps):
-        sim.step(dt)
+        dt = step()
     sim.sync()
     elapsed = time.perf_counter() - t1
     st = sim.stats()
@@ -102,13 +120,35 @@ def run_path(parts, lmin, lmax, crit, precision, steps, warmup, prof_steps, inne
     if prof_steps > 0:
         sim.timing(True)
         for _ in range(prof_steps):
-            sim.step(dt)
+            step()
         sim.sync()
         report = sim.timing_report()
         sim.timing(False)
     sweep = sim.bench_sweep(sweep_reps) if sweep_reps > 0 else None
     sim.close()
-    return elapsed, dt, st, report, sweep
+    return elapsed, (dt0, dt), st, report, sweep
+
+
+def fixed_dt_leg(parts, lmin, lmax, crit, steps, warmup):
+    """The mixed path with dt held at its t = 0 value for every step: the
+    workload's earlier definition, kept so that figures stay comparable.  Beyond
+    a few steps that dt moves particles of the collapsing clumps across more
+    than a refinement node's box margin per step (the reference's own main loop
+    shrinks dt instead)."""
+    from quantizer_amd import core
+    sim = core.Simulation(parts, lmin, lmax, precision=core.PREC_MIXED, ref_criterion_ptcl=crit)
+    sim.force()
+    dt0 = sim.timestep(cfl=0.5)
+    for _ in range(warmup):
+        sim.step(dt0)
+    sim.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        sim.step(dt0)
+    sim.sync()
+    el = time.perf_counter() - t
+    sim.close()
+    return {"ms_per_step": 1e3 * el / steps, "value": parts.n * steps / el, "dt": dt0}
 
 
 def kernel_table(report, prof_steps):
@@ -118,6 +158,25 @@ def kernel_table(report, prof_steps):
     return {k: {"ms_per_step": round(v["ms"] / prof_steps, 4), "launches_per_step": v["launches"] / prof_steps,
                 "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 and v["bytes"] > 0 else None}
             for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms"])}
+
+
+def step_roofline(report, prof_steps, ms_step):
+    """The whole step against HBM: algorithmic bytes of every kernel launched in
+    a step (each input read once, each output written once; from the eager
+    profiling steps that follow the timed ones) over the timed step's wall time.
+    kernel_only_GBps divides the same bytes by the kernels' own time instead, so
+    the gap between the two is launch gaps and host work."""
+    kernels = {k: v for k, v in report.items() if not k.startswith("_") and not k.startswith("host:")}
+    if not kernels:
+        return None
+    by = sum(v["bytes"] for v in kernels.values()) / prof_steps
+    kms = sum(v["ms"] for v in kernels.values()) / prof_steps
+    unacc = sum(v["ms"] for v in kernels.values() if v["bytes"] <= 0) / prof_steps
+    achieved = by / (ms_step * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_step": by, "ms_per_step": ms_step,
+            "kernel_ms_per_step_eager": round(kms, 3), "kernel_only_GBps": round(by / (kms * 1e-3) / 1e9, 1),
+            "kernel_ms_without_bytes": round(unacc, 4)}
 
 
 def traffic_from_profile():
@@ -352,31 +411,106 @@ def precision_configs():
     return res
 
 
-def cpu_baseline(parts, lmin, lmax, crit, dt, budget_s):
-    """Time the compiled reference (oracle/_ref/ref_harness) on this workload: the
-    initial force pass, then one timed step, 1 thread."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    if not os.path.exists(ref):
-        return None
+def cpu_model():
+    """The host CPU's model name, family and model numbers (lscpu, else
+    /proc/cpuinfo)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        if "Model name" in kv:
+            extra = ", ".join(f"{k.lower()} {kv[k]}" for k in ("CPU family", "Model", "Stepping") if k in kv)
+            return kv["Model name"] + (f" ({extra})" if extra else "")
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_threads():
+    """Threads of this process's CPU share: the affinity mask, capped by
+    OMP_NUM_THREADS when the environment sets it."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def time_harness(binary, parts, lmin, lmax, crit, dt, threads, steps, budget_s):
+    """Run an oracle/_ref harness build with --time --cfl-steps: the initial
+    force pass, then `steps` steps (timestep() with dt as its cap, then the
+    step) of which the first is a warm-up; returns its REF_TIMING fields, or a
+    dict with "error"."""
     with tempfile.TemporaryDirectory() as td:
         inp = os.path.join(td, "p.bin")
         parts.write_bin(inp)
-        cmd = [ref, inp, td, str(lmin), str(lmax), str(crit), "1", repr(float(dt)), "--time", "--noenergy",
-               "--threads", "1"]
+        cmd = [binary, inp, td, str(lmin), str(lmax), str(crit), str(steps), repr(float(dt)), "--time",
+               "--cfl-steps", "--noenergy", "--threads", str(threads)]
+        t = time.time()
         try:
-            p = subprocess.run(cmd, capture_output=True, text=True, timeout=budget_s)
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=budget_s, cwd=td)
         except subprocess.TimeoutExpired:
-            return {"value": None, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
-                    "sample": f"timed out after {budget_s}s"}
+            return {"error": f"timed out after {budget_s}s"}
         line = [l for l in p.stderr.splitlines() if l.startswith("REF_TIMING")]
         if p.returncode != 0 or not line:
-            return None
+            return {"error": f"exit {p.returncode}: {p.stderr[-300:]}"}
         kv = dict(t.split("=") for t in line[0].split()[1:])
+        kv["wall_s"] = time.time() - t
+        return kv
+
+
+def cpu_baseline(parts, lmin, lmax, crit, dt, budget_s, steps=4):
+    """The compiled reference (oracle/_ref/ref_harness, the reference's own
+    sources) on this workload at 1 thread (the reference default,
+    global_variables.c:250) and at the host's thread count: the initial force
+    pass, one warm-up step, then steps - 1 timed steps."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(ref):
+        return None
+    res = {}
+    for threads in sorted({1, host_threads()}):
+        kv = time_harness(ref, parts, lmin, lmax, crit, dt, threads, steps, budget_s)
+        if "error" in kv:
+            res[threads] = {"error": kv["error"]}
+            continue
         sps = float(kv["s_per_step"])
-        return {"value": parts.n / sps, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
-                "sample": f"compiled reference, 1 thread, N={parts.n}, lmin={lmin}, lmax={lmax}: "
-                          f"initial force pass {float(kv['first_force_s']):.2f}s, then 1 timed step "
-                          f"({sps:.2f} s/step)"}
+        res[threads] = {"value": parts.n / sps, "s_per_step": sps, "timed_steps": int(kv["timed"]),
+                        "first_force_s": float(kv["first_force_s"]), "last_dt": float(kv["last_dt"])}
+        log(f"  cpu reference, {threads} thread(s): {sps:.2f} s/step")
+    best = max((t for t in res if "value" in res[t]), default=None)
+    if best is None:
+        return {"value": None, "unit": "particle-steps/s", "cores": None, "kind": "reference",
+                "sample": f"failed: {res}"}
+    return {"value": res[best]["value"], "unit": "particle-steps/s", "cores": best, "kind": "reference",
+            "cpu_model": cpu_model(), "by_threads": {str(t): v for t, v in res.items()},
+            "sample": f"compiled reference on the full workload (N={parts.n}, lmin={lmin}, lmax={lmax}, CFL 0.5 "
+                      f"time step at every step): initial force pass, 1 warm-up step, {steps - 1} timed steps, "
+                      f"at 1 and {best} threads"}
+
+
+def resident_dropin(parts, lmin, lmax, crit, dt, budget_s, steps=4):
+    """The reference's host program linked against libqzdropin_resident.so
+    (oracle/_ref/ref_harness_resident): its step loop with every stage on this
+    GPU, host calls included; 1 warm-up step, steps - 1 timed steps.  The
+    host-side set-up (reading the input, the reference's tree construction) is
+    outside the timed steps."""
+    binary = os.path.join(ROOT, "oracle", "_ref", "ref_harness_resident")
+    if not os.path.exists(binary):
+        return None
+    kv = time_harness(binary, parts, lmin, lmax, crit, dt, 1, steps, budget_s)
+    if "error" in kv:
+        return {"value": None, "error": kv["error"]}
+    sps = float(kv["s_per_step"])
+    return {"value": parts.n / sps, "unit": "particle-steps/s", "ms_per_step": 1e3 * sps,
+            "timed_steps": int(kv["timed"]), "precision": "fp64", "last_dt": float(kv["last_dt"]),
+            "sample": f"reference host program + resident drop-in, N={parts.n}, lmin={lmin}, lmax={lmax}, "
+                      f"FP64 (the drop-in default), 1 warm-up and {steps - 1} timed steps"}
 
 
 def pmc_probe(args):
@@ -406,7 +540,9 @@ def main():
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-precision", action="store_true")
     ap.add_argument("--skip-adjoint-amr", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=150.0)
+    ap.add_argument("--skip-resident", action="store_true")
+    ap.add_argument("--skip-fixed-dt", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=420.0)
     ap.add_argument("--cpu-n", type=int, default=1 << 24, help="particles in the CPU sample")
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--branch-solver", choices=["cg", "mg"], default=None,
@@ -432,11 +568,11 @@ def main():
     parts = make_particles(kind, N, lmin)
     log(f"  generated in {time.time() - t:.1f}s")
 
-    el_mx, dt, st_mx, rep_mx, sweep = run_path(parts, lmin, lmax, crit, core.PREC_MIXED, args.steps, args.warmup,
+    el_mx, (dt, dt_last), st_mx, rep_mx, sweep = run_path(parts, lmin, lmax, crit, core.PREC_MIXED, args.steps, args.warmup,
                                                args.prof_steps, args.inner, solver, args.sweep_reps)
     value = N * args.steps / el_mx
     ms_step = 1e3 * el_mx / args.steps
-    log(f"  mixed: {ms_step:.3f} ms/step, {value:.4e} particle-steps/s")
+    log(f"  mixed: {ms_step:.3f} ms/step, {value:.4e} particle-steps/s, last dt {dt_last:.4e}")
     fp64 = None
     if not args.skip_fp64:
         el64, _, st64, _, _ = run_path(parts, lmin, lmax, crit, core.PREC_FP64, args.steps, args.warmup, 0,
@@ -444,6 +580,10 @@ def main():
         fp64 = {"value": N * args.steps / el64, "ms_per_step": 1e3 * el64 / args.steps,
                 "head_vcycles": st64["head_vcycles"], "branch_iterations": st64["branch_iterations"]}
         log(f"  fp64: {fp64['ms_per_step']:.3f} ms/step")
+    fixed = None
+    if not args.skip_fixed_dt:
+        fixed = fixed_dt_leg(parts, lmin, lmax, crit, args.steps, args.warmup)
+        log(f"  mixed, dt fixed at its t=0 value: {fixed['ms_per_step']:.3f} ms/step")
     roof = roofline(sweep) if sweep else None
     if roof:
         log(f"  roofline: {roof['achieved']} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
@@ -459,6 +599,11 @@ def main():
     if not args.skip_adjoint_amr:
         adj_amr = adjoint_full_combination()
         log(f"  adjoint/forward with refinement: {adj_amr['value']:.3f} ({adj_amr})")
+    resident = None
+    if not args.skip_resident:
+        resident = resident_dropin(parts, lmin, lmax, crit, dt, args.cpu_budget)
+        if resident and resident.get("value"):
+            log(f"  resident drop-in (host program + GPU): {resident['ms_per_step']:.2f} ms/step")
     cpu = None
     if not args.skip_cpu:
         ns = min(N, args.cpu_n)
@@ -480,9 +625,12 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {kind}, N={N}, head {1 << lmin}^3 (lmin={lmin}), lmax={lmax}, "
-                               f"refine at >= {crit} particles/cell, tol 1e-10, fixed dt {dt:.4e} (CFL 0.5 at t=0)",
+                               f"refine at >= {crit} particles/cell, tol 1e-10, CFL 0.5 time step at every step "
+                               f"(cap {dt:.4e}, the t=0 value)",
+                   "last_timed_dt": dt_last,
                    "n_particles": N, "lmin": lmin, "lmax": lmax, "parallelism": "single GPU"},
         "fp64_plain": fp64,
+        "mixed_fixed_dt": fixed,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "precision_vs_fp64": prec,
         "adjoint_fwd_ratio": adj["value"] if adj else None,
@@ -491,7 +639,9 @@ def main():
         "solver": {k: st_mx[k] for k in ("head_outer", "head_vcycles", "head_residual", "n_nodes", "branch_points",
                                          "branch_intr", "branch_iterations", "branch_cg_levels")},
         "roofline": roof,
+        "roofline_step": step_roofline(rep_mx, max(1, args.prof_steps), ms_step) if rep_mx else None,
         "kernels": kernel_table(rep_mx, max(1, args.prof_steps)) if rep_mx else None,
+        "dropin_resident": resident,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

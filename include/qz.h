@@ -9,7 +9,7 @@
  * work it queued has finished; qz_last_error() describes the last failure.
  *
  * Entry points mirror the reference's per-step functions (main.c:369-535);
- * the reference-named drop-in wrappers are declared in quantizer_abi.h.
+ * the reference-named drop-in wrappers are declared in qz_dropin.h.
  */
 #ifndef QZ_H
 #define QZ_H

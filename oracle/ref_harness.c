@@ -8,7 +8,14 @@
  *
  * Usage:
  *   ref_harness <particles.bin> <outdir> <lmin> <lmax> <ref_crit> <nsteps> <dt>
- *               [--time] [--threads T] [--noenergy] [--adaptive FR]
+ *               [--time] [--threads T] [--noenergy] [--adaptive FR] [--cfl-steps]
+ *
+ * --cfl-steps (with --time) takes each timed step's dt from the reference's
+ * timestep() (timestep.c:589-699, CFL 0.5, capped at _MAX_dt_ = dt), as
+ * main.c:369-380 does, instead of holding dt fixed.  A fixed dt chosen at t = 0
+ * can let particles of a collapsing clump cross more than a node's box margin
+ * in one step; particle_updating.c:740-805 then files them in cells outside the
+ * node's box and grid_density.c:160-164 reads past ptr_box_idx.
  *
  * --adaptive FR runs main.c:341-573's loop instead of the fixed-dt one: the
  * time step from timestep() at every step (dt is then _MAX_dt_), observables()
@@ -347,7 +354,7 @@ int main(int argc, char **argv)
   int ref_crit = atoi(argv[5]);
   int nsteps = atoi(argv[6]);
   double dt = atof(argv[7]);
-  int do_time = 0, threads = 1, energy = 1, snap = 0, adaptive_fr = 0;
+  int do_time = 0, threads = 1, energy = 1, snap = 0, adaptive_fr = 0, cfl_steps = 0;
   double snap_u[9] = {1, 1, 1, 0, 0, 0, 0, 0, 0}; /* dist, velocity, time, cm x y z, cm v x y z */
   for (int a = 8; a < argc; a++)
   {
@@ -357,6 +364,8 @@ int main(int argc, char **argv)
       threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--noenergy"))
       energy = 0;
+    else if (!strcmp(argv[a], "--cfl-steps"))
+      cfl_steps = 1;
     else if (!strcmp(argv[a], "--adaptive") && a + 1 < argc)
       adaptive_fr = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--snapshot") && a + 9 < argc)
@@ -442,12 +451,27 @@ int main(int argc, char **argv)
     return 0;
   }
 
-  double t_steps = 0.0;
+  double t_steps = 0.0, step_dt = dt;
   int timed = 0;
+  if (cfl_steps)
+  {
+    _MAX_dt_ = (vtype)dt;
+    time_step_method = 1;
+  }
   for (int s = 0; s < nsteps; s++)
   {
     double ts = now_s();
-    if (one_step((vtype)dt) == _FAILURE_)
+    if (cfl_steps)
+    {
+      vtype v;
+      if (timestep(&v) == _FAILURE_)
+      {
+        fprintf(stderr, "timestep at step %d failed\n", s);
+        return 1;
+      }
+      step_dt = (double)v;
+    }
+    if (one_step((vtype)step_dt) == _FAILURE_)
     {
       fprintf(stderr, "step %d failed\n", s);
       return 1;
@@ -461,8 +485,9 @@ int main(int argc, char **argv)
   }
   if (do_time)
   {
-    fprintf(stderr, "REF_TIMING first_force_s=%.6f steps=%d timed=%d s_per_step=%.6f threads=%d N=%d lmin=%d lmax=%d levels_used=%d\n",
-            t_first, nsteps, timed, timed ? t_steps / timed : 0.0, threads, n, lmin, lmax, GL_tentacles_level_max);
+    fprintf(stderr, "REF_TIMING first_force_s=%.6f steps=%d timed=%d s_per_step=%.6f threads=%d N=%d lmin=%d lmax=%d levels_used=%d last_dt=%.17e\n",
+            t_first, nsteps, timed, timed ? t_steps / timed : 0.0, threads, n, lmin, lmax, GL_tentacles_level_max,
+            step_dt);
     return 0;
   }
   dump_particles(outdir, "sK_ptcl.bin", 1);
