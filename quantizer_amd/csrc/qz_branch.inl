@@ -853,11 +853,11 @@ bool Engine<P, G>::iterate_batches(F &&batch, int per_batch, int max_it, int cou
 {
   TreeDev &d = *tree->dev;
   hipGraphExec_t gexec = nullptr;
-  if (given && timer.on)
+  if (given && (timer.on || !batch_graphs))
     (void)hipGraphExecDestroy(given); // timed runs launch eagerly
   else if (given)
     gexec = given; // captured with the set-up (owned from here on)
-  else if (!timer.on)
+  else if (!timer.on && batch_graphs)
   {
     HostSpan span(timer, "graph_instantiate");
     hipGraph_t g = nullptr;
@@ -1354,7 +1354,7 @@ template <typename P, typename G> bool Engine<P, G>::start_ahead(bool build)
                  hip_ok(hipEventRecord(d.ahead_done, d.ahead_stream), "set-up event");
     // the PCG batch of the level's potential solve, captured on the set-up stream
     // (after ahead_done: nothing recorded during the capture is work of its own)
-    if (d.ahead_ok && levels && !d.plan.cg_only)
+    if (d.ahead_ok && levels && !d.plan.cg_only && batch_graphs)
     {
       hipGraph_t g = nullptr;
       d.ahead_ok = hip_ok(hipStreamBeginCapture(d.ahead_stream, hipStreamCaptureModeRelaxed), "graph capture");

@@ -166,6 +166,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool mg_fused = true; // fused tile kernels on the levels below the top (QZ_HEAD_MG_FUSED=0 turns them off)
   bool node_mg_fused = true; // fused node levels k_bmg_down / k_bmg_up (QZ_NODE_MG_FUSED=0 turns them off)
   bool setup_ahead = true;   // first tree level's node set-up during the head solve (QZ_NODE_SETUP_AHEAD=0: in place)
+  bool head_graphs = true;   // head outer iterations replayed as graphs (QZ_HEAD_GRAPH=0: launched one by one)
+  bool batch_graphs = true;  // node PCG batches replayed as graphs (QZ_BATCH_GRAPH=0: launched one by one)
   double *part = nullptr; // reduction partials
   double *red = nullptr;  // device scalars
   double *hred = nullptr; // pinned host mirror
@@ -378,6 +380,10 @@ template <typename P, typename G> struct Engine final : EngineBase
       node_mg_fused = f[0] != '0';
     if (const char *f = std::getenv("QZ_NODE_SETUP_AHEAD"))
       setup_ahead = f[0] != '0';
+    if (const char *f = std::getenv("QZ_HEAD_GRAPH"))
+      head_graphs = f[0] != '0';
+    if (const char *f = std::getenv("QZ_BATCH_GRAPH"))
+      batch_graphs = f[0] != '0';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
@@ -721,16 +727,17 @@ template <typename P, typename G> struct Engine final : EngineBase
     QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, ph, lev_e[0], make_cube(n + 1), cfg.tol_change,
                part);
     hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
-    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0],
-               make_cube(n + 1), 1.0 / (H * H), part);
-    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+    const unsigned nbr = outer_residual_blocks(n + 1);
+    QZ_LAUNCH_TB("outer_residual", dim3(64, kResTY), (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nbr, ph, rh,
+                 lev_r[0], n + 1, 1.0 / (H * H), part);
+    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nbr, red);
   }
 
   bool launch_head_body(double *ph, const G *rh, int inner, int slot)
   {
-    if (timer.on)
+    if (timer.on || !head_graphs)
     {
-      head_outer_body(ph, rh, inner); // eager launches so each kernel is timed
+      head_outer_body(ph, rh, inner); // eager launches (so that each kernel is timed)
       return true;
     }
     if (!head_graph[slot] || head_graph_inner[slot] != inner)
@@ -789,9 +796,10 @@ template <typename P, typename G> struct Engine final : EngineBase
       if (scale == 0.0)
         scale = 1.0;
     }
-    QZ_LAUNCHB("outer_residual", (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nb, ph, rh, lev_r[0],
-               make_cube(n + 1), inv_h2, part);
-    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+    const unsigned nbr = outer_residual_blocks(n + 1);
+    QZ_LAUNCH_TB("outer_residual", dim3(64, kResTY), (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nbr, ph, rh,
+                 lev_r[0], n + 1, inv_h2, part);
+    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nbr, red);
     bool ok = false;
     for (int outer = 0;; outer++)
     {

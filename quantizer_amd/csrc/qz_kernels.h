@@ -703,35 +703,92 @@ __global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__r
 
 // Outer FP64 residual r = rho - Lap(phi) on interior points (multigrid.c:140-150,
 // poisson_error.c:116-117), stored in the V-cycle type, with per-block sum of r^2.
-template <typename G>
-__global__ void k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho, G *__restrict__ r, Cube cb,
-                                 double inv_h2, double *__restrict__ part)
+//
+// A block is 64 lanes by kResTY rows of grid columns and marches up kResKC
+// planes: each lane keeps phi of its column at k - 1, k, k + 1 in registers
+// (loading k + 2 one plane ahead), and the in-plane neighbours come from one
+// LDS copy of the plane (double-buffered by plane parity, one barrier per
+// plane).  Lanes 0 and 63 carry the x halo (load only) and the first and last
+// rows also load the rows below and above the tile, so every phi value is read
+// from memory about once (the chunked one-point-per-thread form read each
+// plane three times).  The Laplacian is summed in the same order as before.
+constexpr int kResTY = 4, kResKC = 16, kResOX = 62;
+inline unsigned outer_residual_blocks(int n1)
 {
-  const int n1 = cb.n1, n = n1 - 1;
-  const unsigned npts = (unsigned)n1 * n1 * n1, hf = (npts + 1) >> 1;
-  const unsigned s1 = n1, s2 = (unsigned)n1 * n1;
+  return (unsigned)((n1 + kResOX - 1) / kResOX) * (unsigned)((n1 + kResTY - 1) / kResTY) *
+         (unsigned)((n1 + kResKC - 1) / kResKC);
+}
+
+template <typename G>
+__global__ void __launch_bounds__(64 * kResTY) k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho,
+                                                                G *__restrict__ r, int n1, double inv_h2,
+                                                                double *__restrict__ part)
+{
+  constexpr int TY = kResTY;
+  __shared__ double S[2][TY + 2][64];
+  const int n = n1 - 1;
+  const unsigned npts = (unsigned)n1 * n1 * n1, hf = (npts + 1) >> 1, s1 = n1, s2 = (unsigned)n1 * n1;
+  const int gx = (n1 + kResOX - 1) / kResOX, gy = (n1 + TY - 1) / TY;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int bx = (int)(lb % (unsigned)gx), by = (int)((lb / (unsigned)gx) % (unsigned)gy), bz = (int)(lb / (unsigned)(gx * gy));
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = bx * kResOX - 1 + tx, j = by * TY + ty; // lanes 0 and 63 hold the x halo
+  const int kb = bz * kResKC, ke = min(kb + kResKC, n1);
+  const bool col = i >= 0 && i <= n && j <= n;
+  const bool own = col && tx > 0 && tx < 63;
+  const bool inplane = own && i > 0 && i < n && j > 0 && j < n;
+  // halo rows of the tile: the first row also loads j - 1, the last row j + 1
+  const bool edge = ty == 0 || ty == TY - 1;
+  const int jh = ty == 0 ? j - 1 : j + 1;
+  const bool hcol = edge && i >= 0 && i <= n && jh >= 0 && jh <= n;
+  const int hrow = ty == 0 ? 0 : TY + 1;
+  const unsigned ic = (unsigned)min(max(i, 0), n);
+  unsigned t = ic + s1 * (unsigned)min(j, n) + s2 * (unsigned)kb;
+  unsigned th = ic + s1 * (unsigned)min(max(jh, 0), n) + s2 * (unsigned)kb;
+  double cm = (inplane && kb > 0) ? phi[t - s2] : 0.0;
+  double c0 = col ? phi[t] : 0.0;
+  double cp = (col && kb < n) ? phi[t + s2] : 0.0;
+  double h0 = hcol ? phi[th] : 0.0;
   double sq = 0.0;
-  // a bounded number of blocks keeps the second reduction stage short; each
-  // block walks one contiguous chunk (XCD-contiguous, see xcd_block)
-  const unsigned chunk = ((npts + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
-  const unsigned t0 = xcd_block(blockIdx.x, gridDim.x) * chunk, t1 = min(npts, t0 + chunk);
-  for (unsigned t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+  for (int k = kb; k < ke; k++, t += s2, th += s2)
   {
-    int i, j, k;
-    cube_ijk(cb, t, i, j, k);
+    const int buf = k & 1;
+    const bool at = inplane && k > 0 && k < n;
+    const double cpp = (col && k + 1 < n && k + 1 < ke) ? phi[t + 2 * s2] : 0.0;
+    const double hn = (hcol && k + 1 < ke) ? phi[th + s2] : 0.0;
+    const G rv = at ? rho[t] : (G)0;
+    S[buf][ty + 1][tx] = c0;
+    if (edge)
+      S[buf][hrow][tx] = h0;
+    __syncthreads();
     double res = 0.0;
-    if (i > 0 && i < n && j > 0 && j < n && k > 0 && k < n)
+    if (at)
     {
-      double c = phi[t];
-      res = (double)rho[t] + inv_h2 * (6.0 * c - phi[t + 1] - phi[t - 1] - phi[t + s1] - phi[t - s1] -
-                                       phi[t + s2] - phi[t - s2]);
+      res = (double)rv + inv_h2 * (6.0 * c0 - S[buf][ty + 1][tx + 1] - S[buf][ty + 1][tx - 1] - S[buf][ty + 2][tx] -
+                                   S[buf][ty][tx] - cp - cm);
       sq += res * res;
     }
-    r[(t & 1) * hf + (t >> 1)] = (G)res;
+    if (own)
+      r[(t & 1) * hf + (t >> 1)] = (G)res;
+    cm = c0;
+    c0 = cp;
+    cp = cpp;
+    h0 = hn;
   }
-  sq = block_reduce<0>(sq);
-  if (threadIdx.x == 0)
-    part[blockIdx.x] = sq;
+  // block sum over the 64 x TY lanes
+  __shared__ double ws[TY];
+  for (int off = 32; off > 0; off >>= 1)
+    sq += __shfl_down(sq, off, 64);
+  if (tx == 0)
+    ws[ty] = sq;
+  __syncthreads();
+  if (tx == 0 && ty == 0)
+  {
+    double v = 0.0;
+    for (int w = 0; w < TY; w++)
+      v += ws[w];
+    part[blockIdx.x] = v;
+  }
 }
 
 // Partial sums of rho^2 over the interior points (the residual of phi = 0).

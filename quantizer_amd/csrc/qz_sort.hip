@@ -1,23 +1,35 @@
-// Stable radix sorts, scans and index selection on the device (hipCUB).
+// Stable radix sorts, scans and index selection on the device (rocPRIM, hipCUB).
 // Kept in its own translation unit: rocPRIM's radix sort is slow to compile
 // and never changes with the rest of the library.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <rocprim/rocprim.hpp>
+
 #include "qz_internal.h"
+
+// 32-bit keys with int values: rocPRIM's onesweep radix sort with 8 bits per
+// pass and 1024-thread blocks of 8 items each, ranked by warp matching.  The
+// library's defaults for gfx950 fall back to an untuned configuration; this one
+// sorts 2^24 pairs of 27-bit keys in 0.50 ms on MI355X against 0.56 ms for
+// hipCUB's default, with identical output (the sort is stable either way).
+using SortConfig32 = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>, 8,
+                                        rocprim::block_radix_rank_algorithm::match>>;
 
 size_t qz_sort_temp_bytes(int n, int end_bit)
 {
   size_t bytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
-                                     (const int *)nullptr, (int *)nullptr, n, 0, end_bit);
+  (void)rocprim::radix_sort_pairs<SortConfig32>(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                                (const int *)nullptr, (int *)nullptr, (size_t)n, 0, end_bit);
   return bytes;
 }
 
 hipError_t qz_sort_pairs(void *tmp, size_t bytes, const unsigned *kin, unsigned *kout, const int *vin, int *vout,
                          int n, int end_bit, hipStream_t s)
 {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout, n, 0, end_bit, s);
+  return rocprim::radix_sort_pairs<SortConfig32>(tmp, bytes, kin, kout, vin, vout, (size_t)n, 0, end_bit, s);
 }
 
 size_t qz_sort64_temp_bytes(int n, int end_bit)
