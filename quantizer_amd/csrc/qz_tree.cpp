@@ -43,7 +43,7 @@ void TreeHost::clear()
 
 // 6-connected flood fill of the -1 cells (fill_zones_ref, tree_construction.c:748-891).
 // box: statuses, rX/rXY strides, cell_ref: box indices of marked cells in cell order.
-static void zone_fill(std::vector<int> &box, int rX, int rXY, const std::vector<int> &ref_box,
+static void zone_fill(int *box, int rX, int rXY, const std::vector<int> &ref_box,
                       std::vector<int> &zone_size, std::vector<std::vector<int>> &zones_by_ref)
 {
   int nref = (int)ref_box.size();
@@ -188,32 +188,165 @@ template <typename F> static void parallel_for(int count, F &&f)
   TreePool::get().run(count, fn);
 }
 
-void TreeHost::build_node_box(HNode &nd) const
+// f(begin, end) over consecutive chunks of [0, n) on the tree workers, at
+// least `grain` indices per chunk (one chunk, on the caller, when n is small).
+// Not to be called from inside a parallel_for job.
+template <typename F> static void parallel_range(size_t n, size_t grain, F &&f)
 {
-  int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
-  size_t cap = (size_t)nd.rd[0] * nd.rd[1] * nd.rd[2];
-  nd.box.assign(cap, -4);
-  for (int c = 0; c < nd.ncell; c++)
-    nd.box[nd.cell_box(c)] = -3;
-  if (nd.sim_contact)
+  const size_t workers = (size_t)TreePool::get().workers() + 1;
+  if (workers <= 1 || n < 2 * grain)
   {
-    int top = (1 << nd.lv) - 1;
-    for (int a = 0; a < 3; a++)
+    f((size_t)0, n);
+    return;
+  }
+  const int chunks = (int)std::min<size_t>((n + grain - 1) / grain, 4 * workers);
+  parallel_for(chunks, [&](int q) { f(n * (size_t)q / (size_t)chunks, n * (size_t)(q + 1) / (size_t)chunks); });
+}
+
+// The zones of zone_fill found on all tree threads: union-find over the marked
+// cells with 6-neighbour links, each link hooking the larger root under the
+// smaller (lock-free compare-and-swap), so a zone's root is its first marked
+// cell in cell_ref order.  Numbering the zones by root then gives zone_fill's
+// numbering (its flood fills start at the first unvisited marked cell), and the
+// member lists are built in cell_ref order as there.  The marked cells' box
+// entries hold their cell_ref index afterwards (zone_fill leaves the zone id;
+// neither is read again).
+static int uf_find(int *par, int x)
+{
+  for (;;)
+  {
+    const int p = __atomic_load_n(par + x, __ATOMIC_RELAXED);
+    if (p == x)
+      return x;
+    const int g = __atomic_load_n(par + p, __ATOMIC_RELAXED);
+    if (g != p)
+      __atomic_store_n(par + x, g, __ATOMIC_RELAXED); // path halving: g is in x's tree and <= p
+    x = p;
+  }
+}
+
+static void uf_unite(int *par, int a, int b)
+{
+  for (;;)
+  {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b)
+      return;
+    if (a < b)
+      std::swap(a, b);
+    int expect = a; // a is a root: hook it under the smaller root b
+    if (__atomic_compare_exchange_n(par + a, &expect, b, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+      return;
+  }
+}
+
+static void zone_union(int *box, int rX, int rXY, const std::vector<int> &ref_box, std::vector<int> &zone_size,
+                       std::vector<std::vector<int>> &zones_by_ref)
+{
+  const int nref = (int)ref_box.size();
+  std::vector<int> par(nref);
+  parallel_range((size_t)nref, 1 << 14, [&](size_t a, size_t b) {
+    for (size_t t = a; t < b; t++)
     {
-      if (!nd.sc[a])
-        continue;
-      int lo_end = nd.bmin[a] == 0 ? nd.bmin[a] - nd.ts[a] : 0;
-      int hi_beg = nd.bmax[a] == top ? nd.bmax[a] + 1 - nd.ts[a] : nd.rd[a];
-      for (int k = 0; k < nd.rd[2]; k++)
+      par[t] = (int)t;
+      box[ref_box[t]] = (int)t; // distinct cells: distinct entries
+    }
+  });
+  parallel_range((size_t)nref, 1 << 12, [&](size_t a, size_t b) {
+    for (size_t t = a; t < b; t++)
+    {
+      const int c = ref_box[t];
+      const int nbs[3] = {c + 1, c + rX, c + rXY};
+      for (int q = 0; q < 3; q++)
+      {
+        const int v = box[nbs[q]]; // >= 0: a marked cell's cell_ref index
+        if (v >= 0)
+          uf_unite(par.data(), (int)t, v);
+      }
+    }
+  });
+  std::vector<int> root(nref);
+  parallel_range((size_t)nref, 1 << 14, [&](size_t a, size_t b) {
+    for (size_t t = a; t < b; t++)
+      root[t] = uf_find(par.data(), (int)t);
+  });
+  // zones numbered by their first member; members in cell_ref order
+  std::vector<int> zone_of(nref, -1);
+  int nz = 0;
+  for (int t = 0; t < nref; t++)
+    if (root[t] == t)
+      zone_of[t] = nz++;
+  zone_size.assign(nz, 0);
+  for (int t = 0; t < nref; t++)
+    zone_size[zone_of[root[t]]]++;
+  zones_by_ref.assign(nz, {});
+  for (int z = 0; z < nz; z++)
+    zones_by_ref[z].reserve(zone_size[z]);
+  for (int t = 0; t < nref; t++)
+    zones_by_ref[zone_of[root[t]]].push_back(t);
+}
+
+// Box statuses of a node that can refine further (tree_construction.c:1262-1310):
+// -4 outside the node, -3 at its cells, -5 beyond the simulation box.  `wide`:
+// the fill, the cells and the outside planes are spread over the tree threads
+// (the three passes run one after the other, as in the serial form).
+void TreeHost::build_node_box(HNode &nd, bool wide) const
+{
+  const int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
+  const size_t cap = (size_t)nd.rd[0] * nd.rd[1] * nd.rd[2];
+  nd.box.reset(new int[cap]);
+  int *box = nd.box.get();
+  auto fill = [&](size_t a, size_t b) { std::fill(box + a, box + b, -4); };
+  auto cells = [&](size_t a, size_t b) {
+    for (size_t c = a; c < b; c++)
+      box[nd.cell_box((int)c)] = -3;
+  };
+  if (wide)
+  {
+    parallel_range(cap, (size_t)1 << 16, fill);
+    parallel_range((size_t)nd.ncell, (size_t)1 << 14, cells);
+  }
+  else
+  {
+    fill(0, cap);
+    cells(0, (size_t)nd.ncell);
+  }
+  if (!nd.sim_contact)
+    return;
+  const int top = (1 << nd.lv) - 1;
+  for (int a = 0; a < 3; a++)
+  {
+    if (!nd.sc[a])
+      continue;
+    const int lo_end = nd.bmin[a] == 0 ? nd.bmin[a] - nd.ts[a] : 0;
+    const int hi_beg = nd.bmax[a] == top ? nd.bmax[a] + 1 - nd.ts[a] : nd.rd[a];
+    auto planes = [&](size_t k0, size_t k1) {
+      for (int k = (int)k0; k < (int)k1; k++)
         for (int j = 0; j < nd.rd[1]; j++)
           for (int i = 0; i < nd.rd[0]; i++)
           {
-            int q = a == 0 ? i : (a == 1 ? j : k);
+            const int q = a == 0 ? i : (a == 1 ? j : k);
             if (q < lo_end || q >= hi_beg)
-              nd.box[i + j * rX + k * rXY] = -5;
+              box[i + j * rX + k * rXY] = -5;
           }
-    }
+    };
+    if (wide)
+      parallel_range((size_t)nd.rd[2], 1, planes);
+    else
+      planes(0, (size_t)nd.rd[2]);
   }
+}
+
+void TreeHost::build_boxes(int first, int count)
+{
+  std::vector<int> small, big;
+  for (int q = first; q < first + count; q++)
+    if (nodes[q].lv < lmax)
+      ((long long)nodes[q].bcap >= kTreeBigNode ? big : small).push_back(q);
+  parallel_for((int)small.size(), [&](int t) { build_node_box(nodes[small[t]], false); });
+  for (int q : big)
+    build_node_box(nodes[q], true);
 }
 
 // create_child_nodes box geometry (tree_construction.c:1092-1661, non-pbc paths):
@@ -247,39 +380,55 @@ HNode TreeHost::child_geometry(int parent_global, int parent_lv, int zone, const
   return ch;
 }
 
-// Child cells: 8 per zone cell in zone order, octant order x fastest
-// (tree_construction.c:1200-1260), kept as the zone cells (HNode::cell_xyz and
-// cell_box expand them); the box status of nodes that can refine further.
-void TreeHost::child_cells(HNode &ch, const int *zxyz, int nzcell) const
-{
-  ch.zxyz.assign(zxyz, zxyz + 3 * (size_t)nzcell);
-  if (ch.lv < lmax)
-    build_node_box(ch);
-}
-
 std::vector<HNode> TreeHost::make_children(int parent_global, int parent_lv, const int *pcxyz,
                                           const std::vector<int> &zone_sizes,
-                                          const std::vector<std::vector<int>> &zone_cells) const
+                                          const std::vector<std::vector<int>> &zone_cells, bool wide) const
 {
   int nz = (int)zone_sizes.size();
   std::vector<HNode> kids;
   kids.reserve(nz);
-  std::vector<int> zx;
   for (int z = 0; z < nz; z++)
   {
-    int bmin[3] = {1 << 30, 1 << 30, 1 << 30}, bmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-    const int m = (int)zone_cells[z].size();
-    zx.resize(3 * (size_t)m);
-    for (int t = 0; t < m; t++)
+    const std::vector<int> &mem = zone_cells[z];
+    const int m = (int)mem.size();
+    // child cells: 8 per zone cell in zone order, octant order x fastest
+    // (tree_construction.c:1200-1260), kept as the zone cells
+    std::vector<int> zx(3 * (size_t)m);
+    auto gather = [&](size_t t0, size_t t1, int *lo, int *hi) {
+      int l[3] = {lo[0], lo[1], lo[2]}, h[3] = {hi[0], hi[1], hi[2]}; // in registers, stored once
+      for (size_t t = t0; t < t1; t++)
+        for (int a = 0; a < 3; a++)
+        {
+          const int v = pcxyz[3 * (size_t)mem[t] + a];
+          zx[3 * t + a] = v;
+          l[a] = std::min(l[a], v);
+          h[a] = std::max(h[a], v);
+        }
       for (int a = 0; a < 3; a++)
       {
-        int v = pcxyz[3 * zone_cells[z][t] + a];
-        zx[3 * t + a] = v;
-        bmin[a] = std::min(bmin[a], v);
-        bmax[a] = std::max(bmax[a], v);
+        lo[a] = l[a];
+        hi[a] = h[a];
       }
+    };
+    int bmin[3] = {1 << 30, 1 << 30, 1 << 30}, bmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    if (wide && m >= (1 << 15))
+    {
+      const int nck = 64;
+      std::vector<int> lo(3 * nck, 1 << 30), hi(3 * nck, -(1 << 30));
+      parallel_for(nck, [&](int q) {
+        gather((size_t)m * q / nck, (size_t)m * (q + 1) / nck, &lo[3 * q], &hi[3 * q]);
+      });
+      for (int q = 0; q < nck; q++)
+        for (int a = 0; a < 3; a++)
+        {
+          bmin[a] = std::min(bmin[a], lo[3 * q + a]);
+          bmax[a] = std::max(bmax[a], hi[3 * q + a]);
+        }
+    }
+    else
+      gather(0, (size_t)m, bmin, bmax);
     HNode ch = child_geometry(parent_global, parent_lv, z, bmin, bmax, m);
-    child_cells(ch, zx.data(), m);
+    ch.zxyz = std::move(zx);
     kids.push_back(std::move(ch));
   }
   return kids;
@@ -297,10 +446,12 @@ int TreeHost::add_head_zones(int nz, const int *bbox6, const int *zsize, const i
   auto build = [&](int z) {
     HNode ch = child_geometry(-1, lmin, z, bbox6 + 6 * z, bbox6 + 6 * z + 3, zsize[z]);
     if (cells)
-      child_cells(ch, cells + 3 * at[z], zsize[z]);
+      ch.zxyz.assign(cells + 3 * at[z], cells + 3 * at[z + 1]);
     nodes[first + z] = std::move(ch);
   };
   parallel_for(nz, build);
+  if (cells)
+    build_boxes(first, nz);
   if (nz > 0)
   {
     lev_start.push_back(first);
@@ -309,11 +460,92 @@ int TreeHost::add_head_zones(int nz, const int *bbox6, const int *zsize, const i
   return nz;
 }
 
+// Marks, zones and children of one parent (tree_construction.c:508-926).
+// fill_cell_ref with n_exp = 1: each flagged cell marks the existing cells of
+// its 3x3x3 block (box status -3 -> -1); the marked cells are then read in cell
+// position order (the reference's cell_ref order) and flood-filled into zones.
+// With `wide` the marking and the scan run on all tree threads: marks of one
+// entry from several threads store the same value (relaxed atomic accesses),
+// and the scan's chunks are joined in cell order, so the result is the serial
+// one.  The flood fill stays serial.
+std::vector<HNode> TreeHost::refine_parent(int global, const int *fcell, int nflag, bool wide)
+{
+  HNode &nd = nodes[global];
+  const int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
+  int *box = nd.box.get();
+  auto mark = [&](size_t f0, size_t f1) {
+    for (size_t f = f0; f < f1; f++)
+    {
+      const int b = nd.cell_box(fcell[f]);
+      for (int kk = -1; kk <= 1; kk++)
+        for (int jj = -1; jj <= 1; jj++)
+          for (int ii = -1; ii <= 1; ii++)
+          {
+            int *e = box + (b + ii + jj * rX + kk * rXY);
+            if (__atomic_load_n(e, __ATOMIC_RELAXED) == -3)
+              __atomic_store_n(e, -1, __ATOMIC_RELAXED);
+          }
+    }
+  };
+  std::vector<int> marked_pos;
+  if (wide)
+  {
+    parallel_range((size_t)nflag, 4096, mark);
+    const int nck = 64;
+    std::vector<std::vector<int>> part(nck);
+    parallel_for(nck, [&](int q) {
+      const int c0 = (int)((long long)nd.ncell * q / nck), c1 = (int)((long long)nd.ncell * (q + 1) / nck);
+      std::vector<int> mine; // local: the vector headers in `part` share cache lines
+      for (int c = c0; c < c1; c++)
+        if (box[nd.cell_box(c)] == -1)
+          mine.push_back(c);
+      part[q] = std::move(mine);
+    });
+    size_t tot = 0;
+    for (auto &v : part)
+      tot += v.size();
+    marked_pos.reserve(tot);
+    for (auto &v : part)
+      marked_pos.insert(marked_pos.end(), v.begin(), v.end());
+  }
+  else
+  {
+    mark(0, (size_t)nflag);
+    for (int c = 0; c < nd.ncell; c++)
+      if (box[nd.cell_box(c)] == -1)
+        marked_pos.push_back(c);
+  }
+  if (marked_pos.empty())
+    return {};
+  const size_t nm = marked_pos.size();
+  std::vector<int> ref_box(nm), pc(3 * nm);
+  auto coords = [&](size_t t0, size_t t1) {
+    for (size_t t = t0; t < t1; t++)
+    {
+      ref_box[t] = nd.cell_box(marked_pos[t]);
+      nd.cell_xyz(marked_pos[t], &pc[3 * t]);
+    }
+  };
+  if (wide)
+    parallel_range(nm, 1 << 14, coords);
+  else
+    coords(0, nm);
+  std::vector<int> zone_sizes;
+  std::vector<std::vector<int>> zones;
+  if (wide)
+    zone_union(box, rX, rXY, ref_box, zone_sizes, zones);
+  else
+    zone_fill(box, rX, rXY, ref_box, zone_sizes, zones);
+  return make_children(global, nd.lv, pc.data(), zone_sizes, zones, wide);
+}
+
 // Refinement of one level: per parent with flagged cells, the marks, the zones
 // and the child nodes.  Parents are independent (each owns its box statuses
-// and makes its own children), so they are spread over up to kTreeThreads
-// host threads, handed out one at a time; the children are then appended in
-// parent order, so the tree is the same as a serial pass builds.
+// and makes its own children): the small ones are spread over up to
+// kTreeThreads host threads, one parent at a time each, and the big ones run
+// one after the other with their loops spread over the threads.  The children
+// are appended in parent order, so the tree is the same as a serial pass
+// builds, and then get their box statuses.
 int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell)
 {
   int start = lev_start[lr], cnt = lev_count[lr];
@@ -336,45 +568,16 @@ int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std:
   const clk::time_point t0 = clk::now();
   ms_mark = ms(tg, t0);
   std::vector<std::vector<HNode>> kids(cnt);
-  auto refine_one = [&](int q) {
-    if (fbeg[q] == fbeg[q + 1])
-      return;
-    HNode &nd = nodes[start + q];
-    int rX = nd.rd[0], rXY = nd.rd[0] * nd.rd[1];
-    // fill_cell_ref with n_exp = 1: each flagged cell marks the existing cells
-    // of its 3x3x3 block (box status -3 -> -1); the marked cells are then read
-    // in cell position order (the reference's cell_ref order)
-    for (int f = fbeg[q]; f < fbeg[q + 1]; f++)
-    {
-      const int b = nd.cell_box(fcell[f]);
-      for (int kk = -1; kk <= 1; kk++)
-        for (int jj = -1; jj <= 1; jj++)
-          for (int ii = -1; ii <= 1; ii++)
-          {
-            const int nb = b + ii + jj * rX + kk * rXY;
-            if (nd.box[nb] == -3)
-              nd.box[nb] = -1;
-          }
-    }
-    std::vector<int> marked_pos;
-    for (int c = 0; c < nd.ncell; c++)
-      if (nd.box[nd.cell_box(c)] == -1)
-        marked_pos.push_back(c);
-    if (marked_pos.empty())
-      return;
-    std::vector<int> ref_box(marked_pos.size());
-    std::vector<int> pc(3 * marked_pos.size());
-    for (size_t t = 0; t < marked_pos.size(); t++)
-    {
-      ref_box[t] = nd.cell_box(marked_pos[t]);
-      nd.cell_xyz(marked_pos[t], &pc[3 * t]);
-    }
-    std::vector<int> zone_sizes;
-    std::vector<std::vector<int>> zones;
-    zone_fill(nd.box, rX, rXY, ref_box, zone_sizes, zones);
-    kids[q] = make_children(start + q, nd.lv, pc.data(), zone_sizes, zones);
-  };
-  parallel_for(cnt, refine_one);
+  std::vector<int> small, big;
+  for (int q = 0; q < cnt; q++)
+    if (fbeg[q] < fbeg[q + 1])
+      ((long long)nodes[start + q].ncell >= kTreeBigNode ? big : small).push_back(q);
+  parallel_for((int)small.size(), [&](int t) {
+    const int q = small[t];
+    kids[q] = refine_parent(start + q, fcell.data() + fbeg[q], fbeg[q + 1] - fbeg[q], false);
+  });
+  for (int q : big)
+    kids[q] = refine_parent(start + q, fcell.data() + fbeg[q], fbeg[q + 1] - fbeg[q], true);
   const clk::time_point t1 = clk::now();
   ms_zones = ms(t0, t1);
   for (int q = 0; q < cnt; q++)
@@ -386,8 +589,9 @@ int TreeHost::refine_level(int lr, const std::vector<int> &flag_node, const std:
     for (auto &k : kids[q])
       nodes.push_back(std::move(k));
   }
-  ms_children = ms(t1, clk::now());
   int made = (int)nodes.size() - next_first;
+  build_boxes(next_first, made);
+  ms_children = ms(t1, clk::now());
   if (made > 0)
   {
     lev_start.push_back(next_first);

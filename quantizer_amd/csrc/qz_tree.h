@@ -4,7 +4,9 @@
 // status, grid-point classification, particle ranges, refinement counts) runs
 // on the device from what this produces.
 #pragma once
+#include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace qz
@@ -26,7 +28,7 @@ struct HNode
   // cells: 8 per zone cell of the parent, in zone order then octant order (x
   // fastest); cell c is octant c & 7 of zone cell c >> 3
   std::vector<int> zxyz;      // 3 per zone cell, parent-level (lv - 1) coordinates
-  std::vector<int> box;       // box status (built for refinable nodes)
+  std::unique_ptr<int[]> box; // box status, rd[0] rd[1] rd[2] entries (built for refinable nodes)
   std::vector<int> child;     // global node indices by zone id
   // pools on the device (filled by the device mirror)
   long long goff = 0, boff = 0, coff = 0, poff = 0;
@@ -49,12 +51,16 @@ struct HNode
 };
 
 constexpr int kTreeThreads = 16; // host threads of refine_level
+// Nodes with at least this many cells (or box entries) are worked on by all
+// tree threads at once; smaller ones are spread over the threads one node each.
+constexpr long long kTreeBigNode = 1 << 16;
 
 struct TreeHost
 {
   int lmin = 0, lmax = 0, n = 0, ref_crit = 8, extra = 10;
   // wall time of refine_level's parts (last call): grouping the flags, the
-  // per-parent marks, zones and children (threaded), appending the children
+  // per-parent marks, zones and children (threaded), appending the children and
+  // building their box statuses (threaded)
   double ms_mark = 0.0, ms_zones = 0.0, ms_children = 0.0;
   std::vector<HNode> nodes;     // refinement nodes, level order (tentacles order, tree_construction.c:2465-2495)
   std::vector<int> lev_start, lev_count; // relative level 1.. -> ranges in nodes
@@ -71,10 +77,14 @@ struct TreeHost
 
 private:
   HNode child_geometry(int parent_global, int parent_lv, int zone, const int *bmin, const int *bmax, int nzcell) const;
-  void child_cells(HNode &ch, const int *zone_cell_xyz, int nzcell) const;
   std::vector<HNode> make_children(int parent_global, int parent_lv, const int *cell_xyz_parent,
-                                   const std::vector<int> &zone_sizes, const std::vector<std::vector<int>> &zones) const;
-  void build_node_box(HNode &nd) const;
+                                   const std::vector<int> &zone_sizes, const std::vector<std::vector<int>> &zones,
+                                   bool wide) const;
+  // marks, zones and children of one parent; `wide`: spread its loops over the tree threads
+  std::vector<HNode> refine_parent(int global, const int *fcell, int nflag, bool wide);
+  // box statuses of the nodes [first, first + count) that can refine further
+  void build_boxes(int first, int count);
+  void build_node_box(HNode &nd, bool wide) const;
 };
 
 } // namespace qz
