@@ -1331,6 +1331,14 @@ template <typename P, typename G> bool Engine<P, G>::join_ahead()
 // (build_tree), then, when the tree has levels, the first level's node
 // multigrid set-up and its captured PCG batch.  ahead_done marks the end of the
 // stream work the engine's stream must wait for.
+//
+// Until join_ahead() the thread owns the host tree and every TreeDev buffer:
+// the head child map, flag and zone lists, node table, zone, cell, box and
+// point pools, the node field pools it zeroes (phi, ax, ay, az), the partial
+// sums part1-3 and mass partials it may reallocate, the classification tables,
+// and the node multigrid pools and plan.  The engine's stream meanwhile uses
+// only the particle arrays, the head grids and the engine's own reduction
+// buffers (part, red); every stage that reads the tree joins first.
 template <typename P, typename G> bool Engine<P, G>::start_ahead(bool build)
 {
   TreeDev &d = *tree->dev;
