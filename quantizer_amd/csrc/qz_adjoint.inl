@@ -398,6 +398,7 @@ template <typename P, typename G> void Engine<P, G>::restore_ckpt(const Ckpt &c)
   phi_hist = c.prev && phi_prev ? c.hist : 0;
   keys_valid = false;
   acc_live = true;
+  particles_changed();
 }
 
 template <typename P, typename G> void Engine<P, G>::free_ckpt(Ckpt &c)
@@ -483,6 +484,7 @@ template <typename P, typename G> bool Engine<P, G>::load_rec(StepRec &r)
   for (int a = 0; a < 3; a++)
     std::swap(pos[a], r.x[a]);
   std::swap(id_a, r.id);
+  particles_changed();
   std::swap(key_out, r.key);
   std::swap(cell_start, r.cstart);
   std::swap(gax, r.g[0]);

@@ -1568,7 +1568,8 @@ template <typename P, typename G> bool Engine<P, G>::branch_particle_acceleratio
   const double npts = (double)(n + 1) * (n + 1) * (n + 1);
   QZ_LAUNCHB("gather_amr", (28.0 + 3.0 * sizeof(P)) * N + 3.0 * sizeof(G) * (npts + (double)d.used_grid),
              (k_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, d.head_child, d.nodes,
-            d.box, (G *)d.ax, (G *)d.ay, (G *)d.az, pa[6], pa[7], pa[8], N, cfg.lmin);
+            d.box, (G *)d.ax, (G *)d.ay, (G *)d.az, pa[6], pa[7], pa[8], N, cfg.lmin, plev);
+  lev_head = false;
   return true;
 }
 
@@ -1664,18 +1665,29 @@ template <typename P, typename G> bool Engine<P, G>::node_fields(int i, double *
 template <typename P, typename G> bool Engine<P, G>::timestep(double cfl, double dt_max, double *dt)
 {
   unsigned nb = nblk(N);
-  const int *hc = has_tree() ? tree->dev->head_child : nullptr;
-  const DNode *nd = has_tree() ? tree->dev->nodes : nullptr;
-  const int *bx = has_tree() ? tree->dev->box : nullptr;
-  QZ_LAUNCHB("cfl_dt", 9.0 * sizeof(P) * N, k_cfl_dt<P>, nb, pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7],
-             pa[8], N, hc, nd, bx, cfg.lmin, cfl, dt_max, part);
-  // fold the per-block minima through max(-dt)
-  hipLaunchKernelGGL(k_negate, dim3(nb / kBlock + 1), dim3(kBlock), 0, stream, part, (int)nb);
-  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red);
+  // the minima the last kick folded in, when they are for this CFL number and
+  // nothing has changed since; otherwise one pass over the particles
+  const bool folded = dt_ready && cfl == dt_cfl;
+  double *src = folded ? dtpart : part;
+  if (!folded)
+  {
+    const int *hc = has_tree() ? tree->dev->head_child : nullptr;
+    const DNode *nd = has_tree() ? tree->dev->nodes : nullptr;
+    const int *bx = has_tree() ? tree->dev->box : nullptr;
+    QZ_LAUNCHB("cfl_dt", 9.0 * sizeof(P) * N, k_cfl_dt<P>, nb, pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6],
+               pa[7], pa[8], N, hc, nd, bx, cfg.lmin, cfl, dt_max, part);
+  }
+  // fold the per-block minima through max(-dt) (in place: the folded minima
+  // are used once)
+  hipLaunchKernelGGL(k_negate, dim3(nb / kBlock + 1), dim3(kBlock), 0, stream, src, (int)nb);
+  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, src, (int)nb, red);
+  dt_ready = false;
+  dt_cfl = cfl; // the CFL number the next kick folds in
   if (!hip_ok(hipMemcpyAsync(hred, red, sizeof(double), hipMemcpyDeviceToHost, stream), "readback") ||
       !hip_ok(hipStreamSynchronize(stream), "timestep"))
     return false;
-  *dt = -hred[0];
+  // min(dt_max, every particle's dt), as k_cfl_dt starts each particle from dt_max
+  *dt = folded ? std::min(dt_max, -hred[0]) : -hred[0];
   return true;
 }
 

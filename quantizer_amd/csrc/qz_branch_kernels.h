@@ -1057,7 +1057,7 @@ __global__ void k_gather_amr(const double *__restrict__ x, const double *__restr
                              const int *__restrict__ head_child, const DNode *__restrict__ nodes,
                              const int *__restrict__ box, const G *__restrict__ bax, const G *__restrict__ bay,
                              const G *__restrict__ baz, P *__restrict__ ax, P *__restrict__ ay, P *__restrict__ az,
-                             int N, int lmin)
+                             int N, int lmin, unsigned char *__restrict__ plev)
 {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N)
@@ -1127,6 +1127,8 @@ __global__ void k_gather_amr(const double *__restrict__ x, const double *__restr
   ax[p] = (P)sx;
   ay[p] = (P)sy;
   az[p] = (P)sz;
+  if (plev)
+    plev[p] = (unsigned char)lv; // the finest level holding the particle, for the next time step
 }
 
 // Finest node holding a position (the walk of k_gather_amr): returns the node
@@ -1211,28 +1213,8 @@ __global__ void k_grid_pe_amr(const double *__restrict__ x, const double *__rest
 
 // CFL time step with velocity and acceleration (timestep.c:73-195, 370-587):
 // per axis the dt that moves a particle CFL * H of its finest node, min over
-// particles and axes; per-block minima for a deterministic second stage.
-__device__ inline double cfl_axis(double v, double a, double H, double cfl, double dt)
-{
-  double av = fabs(v), aa = fabs(a);
-  if (aa < 1.0e-16)
-  {
-    if (av > 1.0e-16)
-      dt = fmin(dt, cfl * H / av);
-    return dt;
-  }
-  double c = v * v + 2.0 * aa * H * cfl;
-  double t;
-  if (1.0 - v * v / c > 1.0e-8)
-    t = (-av + sqrt(c)) / aa;
-  else
-  {
-    double eps = fabs(a * H * cfl / (v * v));
-    t = fabs(v / a) * (eps - eps * eps * 0.5);
-  }
-  return fmin(dt, t);
-}
-
+// particles and axes (cfl_axis, qz_kernels.h); per-block minima for a
+// deterministic second stage.
 template <typename P>
 __global__ void k_cfl_dt(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                          const P *__restrict__ vx, const P *__restrict__ vy, const P *__restrict__ vz,

@@ -189,6 +189,16 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool keys_valid = false;
   double face_gm = 0.0; // G M of the head's monopole faces, fixed when the head is initialised
   bool acc_live = false;         // particle accelerations hold values (false: all zero, as reset leaves them)
+  // The next CFL time step folded into the kick: plev (one byte per sorted
+  // particle) holds the finest level of each particle from the last gather,
+  // valid (lev_ready) until the particles move, are reordered or reloaded;
+  // dtpart holds the kick's per-block minima for CFL number dt_cfl, valid
+  // (dt_ready) until the particles change or a time step consumes them.
+  unsigned char *plev = nullptr;
+  double *dtpart = nullptr;
+  bool lev_ready = false, lev_head = false, dt_ready = false;
+  double dt_cfl = 0.5;
+  void particles_changed() { lev_ready = dt_ready = false; }
   std::vector<double> host_mass; // the masses as given (snapshots write them unchanged)
   double *phi_prev = nullptr; // the previous converged head potential
   int phi_hist = 0;           // converged head potentials since the last reset of the history
@@ -445,6 +455,10 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
     part_cap = std::max<size_t>(nblk(npts), 4 * (size_t)nblk(N)) + 16;
     part = dmalloc<double>(part_cap, bytes);
+    plev = dmalloc<unsigned char>(N, bytes);
+    dtpart = dmalloc<double>(nblk(N), bytes);
+    if (!part || !plev || !dtpart)
+      return hip_ok(hipErrorOutOfMemory, "reduction partials");
     red = dmalloc<double>(16, bytes);
     if (!hip_ok(hipHostMalloc((void **)&hred, 16 * sizeof(double)), "pinned scalars"))
       return false;
@@ -483,6 +497,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     for (int a = 6; a < 9; a++)
       QZ_LAUNCH("fill", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
     acc_live = false;
+    particles_changed();
     if (init_head)
     {
       phi_hist = 0;
@@ -529,12 +544,19 @@ template <typename P, typename G> struct Engine final : EngineBase
     QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P)) * N, k_update_A<P>, nblk(N), pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
               dt, refl_rec, refl_rec ? id_a : nullptr);
     keys_valid = false;
+    particles_changed();
     return hip_ok(hipGetLastError(), "particle_updating_A"); // stream-ordered; readers synchronise
   }
 
   bool update_B(double dt) override
   {
-    QZ_LAUNCHB("update_B", 9.0 * sizeof(P) * N, k_update_B<P>, nblk(N), pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, (P)dt);
+    // with the levels of the last gather at hand, the kick also folds the next
+    // CFL time step into per-block minima (one more byte read per particle)
+    const bool fold = lev_ready;
+    QZ_LAUNCHB("update_B", (9.0 * sizeof(P) + (fold && !lev_head ? 1.0 : 0.0)) * N, k_update_B<P>, nblk(N), pa[3],
+               pa[4], pa[5], pa[6], pa[7], pa[8], N, (P)dt, lev_head ? nullptr : plev, cfg.lmin, dt_cfl,
+               fold ? dtpart : nullptr);
+    dt_ready = fold;
     return hip_ok(hipGetLastError(), "particle_updating_B"); // stream-ordered; readers synchronise
   }
 
@@ -544,6 +566,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     for (int a = 6; a < 9; a++)
       QZ_LAUNCHB("reset", (double)sizeof(P) * N, k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
     acc_live = false;
+    particles_changed();
     QZ_LAUNCHB("reset", (double)sizeof(G) * npts, k_fill<G>, nblk(npts), rho, (G)0, npts);
     return hip_ok(hipGetLastError(), "reset"); // stream-ordered; readers synchronise
   }
@@ -603,6 +626,7 @@ template <typename P, typename G> struct Engine final : EngineBase
       if (move_acc || a < 6 || a == 9)
         std::swap(pa[a], pb[a]);
     std::swap(id_a, id_b);
+    particles_changed(); // reordered
     const size_t ncs = (size_t)n * n * n + 1;
     QZ_LAUNCH("cell_start", k_fill<int>, nblk(ncs), cell_start, -1, ncs);
     QZ_LAUNCHB("cell_start", 8.0 * N + 4.0 * n * n * n, k_cell_start, nblk((size_t)N + 1), key_out, N, D, n * n * n,
@@ -885,8 +909,11 @@ template <typename P, typename G> struct Engine final : EngineBase
     {
       QZ_LAUNCHB("gather_head", (24.0 + 3.0 * sizeof(P)) * N + 3.0 * sizeof(G) * npts, (k_gather_head<P, G>), nblk(N), pos[0], pos[1], pos[2], gax, gay, gaz, pa[6], pa[7],
                 pa[8], N, n);
+      lev_head = true; // every particle is on the head level
     }
     acc_live = true;
+    lev_ready = true;
+    dt_ready = false; // new accelerations
     return hip_ok(hipGetLastError(), "particle_acceleration"); // stream-ordered; readers synchronise
   }
 

@@ -542,23 +542,75 @@ __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, doubl
   vx[i] = vel[0];
   vy[i] = vel[1];
   vz[i] = vel[2];
-  ax[i] = acc[0];
-  ay[i] = acc[1];
-  az[i] = acc[2];
+  // the accelerations change only on a reflected axis
+  if (flags & 1)
+    ax[i] = acc[0];
+  if (flags & 2)
+    ay[i] = acc[1];
+  if (flags & 4)
+    az[i] = acc[2];
 }
 
-// A15 kick (particle_updating.c:2677-2691).
+// The CFL time step of one axis with velocity and acceleration
+// (timestep.c:73-195, 370-587): the dt that moves a particle CFL * H of its
+// finest node (H its cell size), folded into dt by fmin.
+__device__ inline double cfl_axis(double v, double a, double H, double cfl, double dt)
+{
+  double av = fabs(v), aa = fabs(a);
+  if (aa < 1.0e-16)
+  {
+    if (av > 1.0e-16)
+      dt = fmin(dt, cfl * H / av);
+    return dt;
+  }
+  double c = v * v + 2.0 * aa * H * cfl;
+  double t;
+  if (1.0 - v * v / c > 1.0e-8)
+    t = (-av + sqrt(c)) / aa;
+  else
+  {
+    double eps = fabs(a * H * cfl / (v * v));
+    t = fabs(v / a) * (eps - eps * eps * 0.5);
+  }
+  return fmin(dt, t);
+}
+
+// A15 kick (particle_updating.c:2677-2691).  With `part`, the kick also folds
+// the next CFL time step of each particle (main.c calls timestep() right after
+// this update, on these velocities and accelerations) into per-block minima:
+// plev holds each particle's finest level from the last gather (null: the
+// head level lmin for every particle).  The time step then needs only the
+// second stage of the reduction.
 template <typename P>
 __global__ void k_update_B(P *__restrict__ vx, P *__restrict__ vy, P *__restrict__ vz, const P *__restrict__ ax,
-                           const P *__restrict__ ay, const P *__restrict__ az, int N, P dt)
+                           const P *__restrict__ ay, const P *__restrict__ az, int N, P dt,
+                           const unsigned char *__restrict__ plev, int lmin, double cfl, double *__restrict__ part)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N)
-    return;
-  const P h = (P)0.5;
-  vx[i] += ax[i] * dt * h;
-  vy[i] += ay[i] * dt * h;
-  vz[i] += az[i] * dt * h;
+  double dtm = 1e300;
+  if (i < N)
+  {
+    const P h = (P)0.5;
+    const P a0 = ax[i], a1 = ay[i], a2 = az[i];
+    const P v0 = vx[i] + a0 * dt * h, v1 = vy[i] + a1 * dt * h, v2 = vz[i] + a2 * dt * h;
+    vx[i] = v0;
+    vy[i] = v1;
+    vz[i] = v2;
+    if (part)
+    {
+      const int lv = plev ? (int)plev[i] : lmin;
+      const double H = 1.0 / (1 << lv);
+      dtm = cfl_axis((double)v0, (double)a0, H, cfl, dtm);
+      dtm = cfl_axis((double)v1, (double)a1, H, cfl, dtm);
+      dtm = cfl_axis((double)v2, (double)a2, H, cfl, dtm);
+    }
+  }
+  if (part)
+  {
+    const double r = -block_reduce<1>(-dtm); // the block's minimum
+    if (threadIdx.x == 0)
+      part[blockIdx.x] = r;
+  }
 }
 
 // A16 sums: m v^2, m v (observables.c:38-75, 622-659); 4 partials per block.

@@ -45,6 +45,37 @@ def test_timestep_matches_oracle(oracle, levels):
 
 
 @pytest.mark.parametrize("levels", [(5, 5), (4, 6)])
+@pytest.mark.parametrize("prec", [core.PREC_FP64, core.PREC_MIXED])
+def test_timestep_folded_into_the_kick_is_exact(levels, prec):
+    """After a step the kick has folded the next CFL time step into per-block
+    minima (with each particle's finest level from the gather); a time step
+    taken from them must equal the one computed afresh from the particles bit
+    for bit: uncapped, capped, and after a change of CFL number."""
+    lmin, lmax = levels
+    print(f"case: folded CFL timestep levels={levels} prec={prec} two spheres N=3000", flush=True)
+    p = two_plummer(1500, seed=35)
+    with core.Simulation(p, lmin, lmax, precision=prec, tol=1e-12) as s:
+        s.force()
+        dt0 = s.timestep(0.5)
+        for _ in range(3):
+            s.step(s.timestep(0.5, dt0))
+        folded = s.timestep(0.5)  # the kick's minima
+        fresh = s.timestep(0.5)  # those are used once: computed from the particles
+        assert folded == fresh
+        s.step(fresh)
+        cap = 0.5 * fresh
+        c_folded, c_fresh = s.timestep(0.5, dt_max=cap), s.timestep(0.5, dt_max=cap)
+        assert c_folded == c_fresh == min(cap, s.timestep(0.5))
+        s.step(c_folded)
+        o_first = s.timestep(0.3)  # the kick folded CFL 0.5: computed afresh
+        o_fresh = s.timestep(0.3)
+        assert o_first == o_fresh
+        s.step(o_first)
+        o_folded, o_fresh = s.timestep(0.3), s.timestep(0.3)  # now folded at 0.3
+        assert o_folded == o_fresh
+
+
+@pytest.mark.parametrize("levels", [(5, 5), (4, 6)])
 def test_observables_match_oracle(oracle, levels):
     lmin, lmax = levels
     print(f"case: observables levels={levels} two spheres N=3000", flush=True)
