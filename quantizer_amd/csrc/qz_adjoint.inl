@@ -397,6 +397,7 @@ template <typename P, typename G> void Engine<P, G>::restore_ckpt(const Ckpt &c)
   timer.end(stream, "ckpt_restore");
   phi_hist = c.prev && phi_prev ? c.hist : 0;
   keys_valid = false;
+  keys_fresh = false;
   acc_live = true;
   particles_changed();
 }
@@ -485,6 +486,7 @@ template <typename P, typename G> bool Engine<P, G>::load_rec(StepRec &r)
     std::swap(pos[a], r.x[a]);
   std::swap(id_a, r.id);
   particles_changed();
+  keys_fresh = false;
   std::swap(key_out, r.key);
   std::swap(cell_start, r.cstart);
   std::swap(gax, r.g[0]);

@@ -186,7 +186,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     bool ok = true;
     std::string err;
   } snap;
-  bool keys_valid = false;
+  bool keys_valid = false;  // the particles are sorted and cell_start is current
+  bool keys_fresh = false;  // the unsorted keys and slots of the current positions are in the sort's input
   double face_gm = 0.0; // G M of the head's monopole faces, fixed when the head is initialised
   bool acc_live = false;         // particle accelerations hold values (false: all zero, as reset leaves them)
   // The next CFL time step folded into the kick: plev (one byte per sorted
@@ -498,6 +499,7 @@ template <typename P, typename G> struct Engine final : EngineBase
       QZ_LAUNCH("fill", k_fill<P>, nblk(N), pa[a], (P)0, (size_t)N);
     acc_live = false;
     particles_changed();
+    keys_fresh = false;
     if (init_head)
     {
       phi_hist = 0;
@@ -541,9 +543,14 @@ template <typename P, typename G> struct Engine final : EngineBase
   // ---------------------------------------------------------------- stages
   bool update_A(double dt) override
   {
-    QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P)) * N, k_update_A<P>, nblk(N), pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N,
-              dt, refl_rec, refl_rec ? id_a : nullptr);
+    // the drift also writes the new positions' keys in the sort's input form
+    const bool narrow = end_bit <= 32;
+    void *key = narrow ? (void *)key_out : (void *)key_in;
+    QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P) + (narrow ? 8.0 : 12.0)) * N, k_update_A<P>, nblk(N), pos[0],
+               pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, dt, refl_rec, refl_rec ? id_a : nullptr,
+               key, slot_in, cfg.lmin, D, narrow ? 1 : 0);
     keys_valid = false;
+    keys_fresh = true;
     particles_changed();
     return hip_ok(hipGetLastError(), "particle_updating_A"); // stream-ordered; readers synchronise
   }
@@ -588,12 +595,20 @@ template <typename P, typename G> struct Engine final : EngineBase
   {
     if (keys_valid)
       return true;
-    QZ_LAUNCHB("keys", (3.0 * 8 + 12.0) * N, k_keys<double>, nblk(N), pos[0], pos[1], pos[2], key_in, slot_in, N, cfg.lmin, D);
+    // keys fitting 32 bits are sorted as such (fewer bytes per pass), then widened
+    if (!keys_fresh)
+    {
+      if (end_bit <= 32)
+        QZ_LAUNCHB("keys", (3.0 * 8 + 8.0) * N, k_keys32<double>, nblk(N), pos[0], pos[1], pos[2],
+                   (unsigned *)key_out, slot_in, N, cfg.lmin, D);
+      else
+        QZ_LAUNCHB("keys", (3.0 * 8 + 12.0) * N, k_keys<double>, nblk(N), pos[0], pos[1], pos[2], key_in, slot_in, N,
+                   cfg.lmin, D);
+    }
+    keys_fresh = false;
     timer.begin(stream);
     if (end_bit <= 32)
     {
-      // keys fit 32 bits: sort them as such (fewer bytes per pass), then widen
-      QZ_LAUNCH("keys", k_narrow_keys, nblk(N), key_in, (unsigned *)key_out, N);
       if (!hip_ok(qz_sort_pairs(sort_tmp, sort_tmp_bytes, (const unsigned *)key_out, (unsigned *)key_in, slot_in,
                                 slot_out, N, end_bit, stream),
                   "radix sort"))

@@ -183,6 +183,19 @@ __global__ void k_keys(const P *__restrict__ x, const P *__restrict__ y, const P
   slot[i] = i;
 }
 
+// The same keys written as 32 bits (keys of at most 32 significant bits), the
+// sort's input form.
+template <typename P>
+__global__ void k_keys32(const P *__restrict__ x, const P *__restrict__ y, const P *__restrict__ z,
+                         unsigned *__restrict__ key, int *__restrict__ slot, int N, int lmin, int D)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N)
+    return;
+  key[i] = (unsigned)octree_key<P>(x[i], y[i], z[i], lmin, D);
+  slot[i] = i;
+}
+
 __global__ void k_narrow_keys(const unsigned long long *__restrict__ in, unsigned *__restrict__ out, int N)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,7 +523,8 @@ template <typename P>
 __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, P *__restrict__ vx,
                            P *__restrict__ vy, P *__restrict__ vz, P *__restrict__ ax, P *__restrict__ ay,
                            P *__restrict__ az, int N, double dt, unsigned char *__restrict__ refl,
-                           const int *__restrict__ id)
+                           const int *__restrict__ id, void *__restrict__ key, int *__restrict__ slot, int lmin, int D,
+                           int narrow)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N)
@@ -549,6 +563,17 @@ __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, doubl
     ay[i] = acc[1];
   if (flags & 4)
     az[i] = acc[2];
+  // the octree key of the new position for the next sort (k_keys / k_keys32
+  // on the same position), while it is in registers
+  if (key)
+  {
+    const unsigned long long k = octree_key<double>(pos[0], pos[1], pos[2], lmin, D);
+    if (narrow)
+      ((unsigned *)key)[i] = (unsigned)k;
+    else
+      ((unsigned long long *)key)[i] = k;
+    slot[i] = i;
+  }
 }
 
 // The CFL time step of one axis with velocity and acceleration
