@@ -70,6 +70,7 @@ struct TreeDev
   long long mpart_cap = 0;
   int *done_count = nullptr;
   int *hflag_count = nullptr; // pinned, 4 ints
+  int *hflags = nullptr;      // pinned mirror of flags (2 flag_cap ints): the refinement flags read by the host
   hipEvent_t done_ev[2] = {nullptr, nullptr}; // pipelined batch loop
   std::vector<DNode> hnodes;
   // per relative level (index 1..): node range and pool ranges
@@ -237,6 +238,8 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
       (void)hipFree(q);
     if (d->hflag_count)
       (void)hipHostFree(d->hflag_count);
+    if (d->hflags)
+      (void)hipHostFree(d->hflags);
     for (hipEvent_t e : d->done_ev)
       if (e)
         (void)hipEventDestroy(e);
@@ -437,7 +440,8 @@ template <typename P, typename G> bool Engine<P, G>::tree_state()
         !hip_ok(hipMalloc((void **)&d->flags, sizeof(int) * 2 * (size_t)d->flag_cap), "flags") ||
         !hip_ok(hipMalloc((void **)&d->flag_count, sizeof(int) * 4), "flag count") ||
         !hip_ok(hipMalloc((void **)&d->done_count, sizeof(int) * 4), "done count") ||
-        !hip_ok(hipHostMalloc((void **)&d->hflag_count, sizeof(int) * 4), "pinned"))
+        !hip_ok(hipHostMalloc((void **)&d->hflag_count, sizeof(int) * 4), "pinned") ||
+        !hip_ok(hipHostMalloc((void **)&d->hflags, sizeof(int) * 2 * (size_t)d->flag_cap), "pinned"))
       return false;
   }
   return true;
@@ -672,9 +676,11 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
             !hip_ok(hipStreamSynchronize(stream), "flags"))
           return false;
         int nf = std::min(*d.hflag_count, d.flag_cap);
-        std::vector<int> fl(2 * (size_t)nf);
+        // into pinned memory: a pageable copy of these megabytes stages through
+        // the runtime's bounce buffer at a fraction of the link's rate
+        const int *fl = d.hflags;
         if (nf > 0 &&
-            (!hip_ok(hipMemcpyAsync(fl.data(), d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost, stream), "flags") ||
+            (!hip_ok(hipMemcpyAsync(d.hflags, d.flags, sizeof(int) * 2 * nf, hipMemcpyDeviceToHost, stream), "flags") ||
              !hip_ok(hipStreamSynchronize(stream), "flags")))
           return false;
         fn.resize(nf);

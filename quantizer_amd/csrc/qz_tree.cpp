@@ -242,7 +242,7 @@ static void uf_unite(int *par, int a, int b)
 }
 
 static void zone_union(int *box, int rX, int rXY, const std::vector<int> &ref_box, std::vector<int> &zone_size,
-                       std::vector<std::vector<int>> &zones_by_ref)
+                       std::vector<std::vector<int>> &zones_by_ref, bool &one_zone)
 {
   const int nref = (int)ref_box.size();
   std::vector<int> par(nref);
@@ -267,10 +267,26 @@ static void zone_union(int *box, int rX, int rXY, const std::vector<int> &ref_bo
     }
   });
   std::vector<int> root(nref);
+  std::atomic<int> others{0}; // members whose root is not the first marked cell
   parallel_range((size_t)nref, 1 << 14, [&](size_t a, size_t b) {
+    int o = 0;
     for (size_t t = a; t < b; t++)
+    {
       root[t] = uf_find(par.data(), (int)t);
+      o += root[t] != 0;
+    }
+    if (o)
+      others.fetch_add(o, std::memory_order_relaxed);
   });
+  // one zone holding every marked cell (the usual case for a large parent):
+  // its members are the cells in cell_ref order, which the caller has
+  one_zone = others.load() == 0;
+  if (one_zone)
+  {
+    zone_size.assign(1, nref);
+    zones_by_ref.clear();
+    return;
+  }
   // zones numbered by their first member; members in cell_ref order
   std::vector<int> zone_of(nref, -1);
   int nz = 0;
@@ -532,11 +548,42 @@ std::vector<HNode> TreeHost::refine_parent(int global, const int *fcell, int nfl
     coords(0, nm);
   std::vector<int> zone_sizes;
   std::vector<std::vector<int>> zones;
+  bool one_zone = false;
   if (wide)
-    zone_union(box, rX, rXY, ref_box, zone_sizes, zones);
+    zone_union(box, rX, rXY, ref_box, zone_sizes, zones, one_zone);
   else
     zone_fill(box, rX, rXY, ref_box, zone_sizes, zones);
-  return make_children(global, nd.lv, pc.data(), zone_sizes, zones, wide);
+  if (!one_zone)
+    return make_children(global, nd.lv, pc.data(), zone_sizes, zones, wide);
+  // the single child of make_children for one zone of every marked cell: the
+  // zone cells are the marked cells' coordinates in order, the box their span
+  const int nck = 64;
+  std::vector<int> lo(3 * nck, 1 << 30), hi(3 * nck, -(1 << 30));
+  parallel_for(nck, [&](int q) {
+    int l[3] = {1 << 30, 1 << 30, 1 << 30}, h[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (size_t t = nm * q / nck; t < nm * (q + 1) / nck; t++)
+      for (int a = 0; a < 3; a++)
+      {
+        l[a] = std::min(l[a], pc[3 * t + a]);
+        h[a] = std::max(h[a], pc[3 * t + a]);
+      }
+    for (int a = 0; a < 3; a++)
+    {
+      lo[3 * q + a] = l[a];
+      hi[3 * q + a] = h[a];
+    }
+  });
+  int bmin[3] = {1 << 30, 1 << 30, 1 << 30}, bmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  for (int q = 0; q < nck; q++)
+    for (int a = 0; a < 3; a++)
+    {
+      bmin[a] = std::min(bmin[a], lo[3 * q + a]);
+      bmax[a] = std::max(bmax[a], hi[3 * q + a]);
+    }
+  std::vector<HNode> kids(1);
+  kids[0] = child_geometry(global, nd.lv, 0, bmin, bmax, (int)nm);
+  kids[0].zxyz = std::move(pc);
+  return kids;
 }
 
 // Refinement of one level: per parent with flagged cells, the marks, the zones
