@@ -348,8 +348,10 @@ def adjoint_full_combination(n_each=1 << 23, lmin=8, lmax=11, crit=8, warm=40, s
 def precision_study(parts, lmin, lmax, crit, dt, steps=20, head_bc=0):
     """Mixed against the all-FP64 GPU path on the same workload: head potential
     and grid acceleration, particle accelerations after the first force pass,
-    then the divergence of the trajectories over `steps` fixed-dt steps (dt None:
-    the CFL rule at t = 0, frozen)."""
+    then the divergence of the trajectories over `steps` steps (dt None: the
+    CFL 0.5 time step at every step, from the FP64 path and capped at its t = 0
+    value, the same dt for both paths), and each path's relative drift of the
+    total energy (grid-interpolated potential energy) over those steps."""
     import numpy as np
     from quantizer_amd import core
 
@@ -368,17 +370,21 @@ def precision_study(parts, lmin, lmax, crit, dt, steps=20, head_bc=0):
         out["head_acceleration_rel_err"] = max(rel(hm[k], hd[k]) for k in (2, 3, 4))
         out["particle_acceleration_rel_err"] = max(rel(pm[k], pd[k]) for k in (6, 7, 8))
         del hm, hd
-        if dt is None:
-            dt = sims[1].timestep(cfl=0.5)
+        e0 = [s.observables(exact_pe=False)[2] for s in sims]
+        dt0 = sims[1].timestep(cfl=0.5)
         for _ in range(steps):
+            h = dt if dt is not None else sims[1].timestep(0.5, dt0)
             for s in sims:
-                s.step(dt)
+                s.step(h)
+        e1 = [s.observables(exact_pe=False)[2] for s in sims]
         pm, pd = sims[0].particles(), sims[1].particles()
         dx = np.sqrt(np.sum((pm[:3] - pd[:3]) ** 2, axis=0))
         cell = 1.0 / (1 << lmin)
         out[f"positions_after_{steps}_steps_max_head_cells"] = float(dx.max() / cell)
         out[f"positions_after_{steps}_steps_rms_head_cells"] = float(np.sqrt(np.mean(dx ** 2)) / cell)
         out[f"velocities_after_{steps}_steps_rel_err"] = max(rel(pm[k], pd[k]) for k in (3, 4, 5))
+        out[f"energy_drift_{steps}_steps_mixed"] = float(abs(e1[0] - e0[0]) / abs(e0[0]))
+        out[f"energy_drift_{steps}_steps_fp64"] = float(abs(e1[1] - e0[1]) / abs(e0[1]))
     finally:
         for s in sims:
             s.close()
@@ -396,11 +402,11 @@ def precision_configs():
          lambda: plummer(10000), 5, 5, 8, core.BC_ZERO, plummer_dt(0.01), 100),
         ("AMR: Plummer 2^20, 128^3, lmax 9, dt 0.005, 200 steps",
          lambda: plummer(1 << 20), 7, 9, 8, core.BC_MONOPOLE, plummer_dt(0.005), 200),
-        ("precision: 2^24 (half uniform, half 64 clumps), 256^3, lmax 9, CFL dt, 20 steps",
+        ("precision: 2^24 (half uniform, half 64 clumps), 256^3, lmax 9, CFL dt every step, 20 steps",
          lambda: clumps(1 << 24, 8, n_clumps=64), 8, 9, 8, core.BC_MONOPOLE, None, 20),
         ("differentiation: Plummer 2^18, 64^3, dt 0.01, 50 steps",
          lambda: plummer(1 << 18), 6, 6, 8, core.BC_MONOPOLE, plummer_dt(0.01), 50),
-        ("full: two Plummer spheres 2 x 2^23, 80a box, 256^3, lmax 11, CFL dt, 40 steps",
+        ("full: two Plummer spheres 2 x 2^23, 80a box, 256^3, lmax 11, CFL dt every step, 40 steps",
          lambda: two_plummer(1 << 23), 8, 11, 8, core.BC_MONOPOLE, None, 40),
     ]
     res = {}
@@ -589,7 +595,7 @@ def main():
         log(f"  roofline: {roof['achieved']} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
     prec = None
     if not args.skip_precision and not args.skip_fp64:
-        prec = precision_study(parts, lmin, lmax, crit, dt)
+        prec = precision_study(parts, lmin, lmax, crit, None)
         log(f"  precision study (mixed vs FP64): {prec}")
     adj = None
     if not args.skip_adjoint:
