@@ -2,11 +2,14 @@
 //
 // Tree build per step (the reference adapts its tree in tree_adaptation.c; this
 // rebuilds it the way tree_construction.c:2521-2577 builds it):
-//   device: refinement counts per cell from the sorted octree keys
-//   host:   dilated marks, 6-connected zones, child boxes (qz_tree.cpp)
+//   device: refinement counts per cell from the sorted octree keys, dilated
+//           marks, 6-connected zones (union-find), zone bounding boxes
+//   host:   child box geometry from each zone's bounding box (qz_tree.cpp)
 //   device: child cells, box status, particle ranges, masses, grid-point lists
-// Only the compact list of flagged cells crosses to the host, and only node
-// descriptors and zone cells cross back.
+// Per level only the zones' parents, bounding boxes and sizes cross to the
+// host, and only node descriptors cross back.  With QZ_HOST_TREE=1 the marks
+// and zones below the head are found on the host instead (qz_tree.cpp), from
+// the compact list of flagged cells.
 #include "qz_branch_kernels.h"
 #include "qz_branch_mg_kernels.h"
 #include "qz_tree.h"
@@ -61,7 +64,15 @@ struct TreeDev
   int *hpar = nullptr, *hflag = nullptr, *hrank = nullptr, *hidx = nullptr, *hidx_s = nullptr, *hz_node = nullptr,
       *hz_xyz = nullptr, *hzstart = nullptr, *hbbox = nullptr;
   unsigned *hzone = nullptr, *hzone_s = nullptr;
+  int *hzpar = nullptr; // levels below the head: the parent of each zone
   long long hcap = 0;
+  // levels below the head: flagged-cell map over the level's box pool, marks
+  // and ordered list of the level's marked cells, box-entry -> list map
+  unsigned char *lbflag = nullptr, *lmark = nullptr;
+  int *lbmap = nullptr, *llist = nullptr;
+  long long lbox_cap = 0, lcell_cap = 0;
+  void *lsel_tmp = nullptr;
+  size_t lsel_bytes = 0;
   void *htmp = nullptr;
   size_t htmp_bytes = 0;
   double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
@@ -231,7 +242,8 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
                     d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_t, d->ntiles, d->mg_mask, d->hmark, d->hlist,
                     d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
-                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->chunks, d->alist, d->acount,
+                    d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->hzpar, d->lbflag, d->lmark,
+                    d->lbmap, d->llist, d->lsel_tmp, d->chunks, d->alist, d->acount,
                     d->sel_tmp, d->rbeg, d->rstart, d->ctab, d->chunk0, d->ccnt,
                     d->coff, d->cscan_tmp};
     for (void *q : ptrs)
@@ -308,6 +320,43 @@ template <typename G> static bool reserve_level(TreeDev *d, hipStream_t s, long 
   return true;
 }
 
+// The union-find and zone arrays of the marked cells of one level (head or
+// nodes), for `need` entries (grow-only, contents not kept).
+static bool reserve_zone_scratch(TreeDev &d, long long need, hipStream_t stream)
+{
+  if (need <= d.hcap)
+    return true;
+  if (!hip_ok(hipStreamSynchronize(stream), "zones"))
+    return false; // the buffers about to be replaced may still be read by the stream
+  long long cap = next_cap(d.hcap, need);
+  int **ints[] = {&d.hpar, &d.hflag, &d.hrank, &d.hidx, &d.hidx_s, &d.hz_node, &d.hzstart, &d.hzpar};
+  for (int **q : ints)
+  {
+    (void)hipFree(*q);
+    *q = nullptr;
+    if (!hip_ok(hipMalloc((void **)q, sizeof(int) * (size_t)cap), "zones"))
+      return false;
+  }
+  (void)hipFree(d.hz_xyz);
+  (void)hipFree(d.hbbox);
+  (void)hipFree(d.hzone);
+  (void)hipFree(d.hzone_s);
+  (void)hipFree(d.htmp);
+  d.hz_xyz = d.hbbox = nullptr;
+  d.hzone = d.hzone_s = nullptr;
+  d.htmp = nullptr;
+  d.hcap = 0;
+  d.htmp_bytes = std::max(qz_sort_temp_bytes((int)cap, 32), qz_scan_temp_bytes((int)cap));
+  if (!hip_ok(hipMalloc((void **)&d.hz_xyz, sizeof(int) * 3 * (size_t)cap), "zones") ||
+      !hip_ok(hipMalloc((void **)&d.hbbox, sizeof(int) * 6 * (size_t)cap), "zones") ||
+      !hip_ok(hipMalloc((void **)&d.hzone, sizeof(unsigned) * (size_t)cap), "zones") ||
+      !hip_ok(hipMalloc((void **)&d.hzone_s, sizeof(unsigned) * (size_t)cap), "zones") ||
+      !hip_ok(hipMalloc(&d.htmp, d.htmp_bytes), "zones"))
+    return false;
+  d.hcap = cap;
+  return true;
+}
+
 // Head level marks and zones on the device (fill_cell_ref + fill_zones_ref for
 // the head, tree_construction.c:527-926): marks from the particle counts of the
 // sorted keys, ordered compaction, union-find zones, a stable sort by zone.  The
@@ -350,41 +399,13 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
     th.add_head_zones(0, nullptr, nullptr, nullptr);
     return true;
   }
-  if (M + 1 > d.hcap)
-  {
-    if (!hip_ok(hipStreamSynchronize(stream), "head zones"))
-      return false; // the buffers about to be replaced may still be read by the stream
-    long long cap = next_cap(d.hcap, M + 1);
-    int **ints[] = {&d.hpar, &d.hflag, &d.hrank, &d.hidx, &d.hidx_s, &d.hz_node, &d.hzstart};
-    for (int **q : ints)
-    {
-      (void)hipFree(*q);
-      *q = nullptr;
-      if (!hip_ok(hipMalloc((void **)q, sizeof(int) * (size_t)cap), "head zones"))
-        return false;
-    }
-    (void)hipFree(d.hz_xyz);
-    (void)hipFree(d.hbbox);
-    (void)hipFree(d.hzone);
-    (void)hipFree(d.hzone_s);
-    (void)hipFree(d.htmp);
-    d.hz_xyz = d.hbbox = nullptr;
-    d.hzone = d.hzone_s = nullptr;
-    d.htmp = nullptr;
-    d.htmp_bytes = std::max(qz_sort_temp_bytes((int)cap, 32), qz_scan_temp_bytes((int)cap));
-    if (!hip_ok(hipMalloc((void **)&d.hz_xyz, sizeof(int) * 3 * (size_t)cap), "head zones") ||
-        !hip_ok(hipMalloc((void **)&d.hbbox, sizeof(int) * 6 * (size_t)cap), "head zones") ||
-        !hip_ok(hipMalloc((void **)&d.hzone, sizeof(unsigned) * (size_t)cap), "head zones") ||
-        !hip_ok(hipMalloc((void **)&d.hzone_s, sizeof(unsigned) * (size_t)cap), "head zones") ||
-        !hip_ok(hipMalloc(&d.htmp, d.htmp_bytes), "head zones"))
-      return false;
-    d.hcap = cap;
-  }
+  if (!reserve_zone_scratch(d, M + 1, stream))
+    return false;
   // per marked cell: the list entry, the cell map, the six neighbours' map
   // entries and the union-find parents
   QZ_LAUNCHB("tree_head_zones", 12.0 * M, k_head_index, nblk(M), d.hlist, M, d.head_child, d.hpar);
-  QZ_LAUNCHB("tree_head_zones", 36.0 * M, k_head_hook, nblk(M), d.hlist, M, n, d.head_child, d.hpar);
-  QZ_LAUNCHB("tree_head_zones", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
+  QZ_LAUNCHB("tree_head_hook", 36.0 * M, k_head_hook, nblk(M), d.hlist, M, n, d.head_child, d.hpar);
+  QZ_LAUNCHB("tree_head_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "head zones"))
     return false;
@@ -401,13 +422,13 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
   if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream),
               "head zones"))
     return false;
-  timer.end(stream, "tree_head_zones", 16.0 * M);
+  timer.end(stream, "tree_head_sort", 16.0 * M);
   QZ_LAUNCHB("tree_head_zones", 24.0 * nz, k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
   QZ_LAUNCHB("tree_head_zones", 40.0 * M, k_head_zone_cells, nblk(M), M, d.hzone_s, d.hidx_s, d.hlist, n, 0, d.hzstart,
              d.hbbox,
             d.hz_node, d.hz_xyz);
   std::vector<int> zstart(nz + 1), bbox(6 * (size_t)nz), zsize(nz);
-  const bool need_cells = cfg.lmin + 1 < cfg.lmax;
+  const bool need_cells = host_refine && cfg.lmin + 1 < cfg.lmax;
   std::vector<int> cells(need_cells ? 3 * (size_t)M : 0);
   if (!hip_ok(hipMemcpyAsync(zstart.data(), d.hzstart, sizeof(int) * nz, hipMemcpyDeviceToHost, stream), "zones") ||
       !hip_ok(hipMemcpyAsync(bbox.data(), d.hbbox, sizeof(int) * 6 * nz, hipMemcpyDeviceToHost, stream), "zones") ||
@@ -420,6 +441,121 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
   for (int z = 0; z < nz; z++)
     zsize[z] = zstart[z + 1] - zstart[z];
   th.add_head_zones(nz, bbox.data(), zsize.data(), need_cells ? cells.data() : nullptr);
+  return true;
+}
+
+// Marks and zones of the nodes of tree level lr on the device, and the next
+// level's nodes from them (fill_cell_ref + fill_zones_ref + create_child_nodes
+// per parent, tree_construction.c:508-926, 1092-1661): the level's cells are
+// [c0, c0 + ncell) and its box pool entries [b0, b0 + nb).  The marks, the
+// union-find and the zone sort are those of the head level run over all
+// parents at once; the host receives per zone its parent, bounding box and
+// size, and the zone cells of the next level stay in d.hz_node / d.hz_xyz for
+// its upload.  zone_cells: the number of zone cells (marked cells).
+template <typename P, typename G>
+bool Engine<P, G>::refine_level_device(long long c0, long long ncell, long long b0, long long nb, int &zone_cells,
+                                       hipStream_t s)
+{
+  hipStream_t stream = s; // every launch and copy below goes to the tree build's stream
+  TreeDev &d = *tree->dev;
+  TreeHost &th = *tree->host;
+  zone_cells = 0;
+  if (ncell > d.lcell_cap)
+  {
+    if (!hip_ok(hipStreamSynchronize(stream), "level marks"))
+      return false;
+    (void)hipFree(d.lmark);
+    (void)hipFree(d.llist);
+    (void)hipFree(d.lsel_tmp);
+    d.lmark = nullptr;
+    d.llist = nullptr;
+    d.lsel_tmp = nullptr;
+    d.lcell_cap = 0;
+    const long long cap = next_cap(0, ncell);
+    d.lsel_bytes = std::max<size_t>(qz_select_temp_bytes((int)cap), 1);
+    if (!hip_ok(hipMalloc((void **)&d.lmark, (size_t)cap), "level marks") ||
+        !hip_ok(hipMalloc((void **)&d.llist, sizeof(int) * (size_t)cap), "level marks") ||
+        !hip_ok(hipMalloc(&d.lsel_tmp, d.lsel_bytes), "level marks"))
+      return false;
+    d.lcell_cap = cap;
+  }
+  if (nb > d.lbox_cap)
+  {
+    if (!hip_ok(hipStreamSynchronize(stream), "level marks"))
+      return false;
+    (void)hipFree(d.lbflag);
+    (void)hipFree(d.lbmap);
+    d.lbflag = nullptr;
+    d.lbmap = nullptr;
+    d.lbox_cap = 0;
+    const long long cap = next_cap(0, nb);
+    if (!hip_ok(hipMalloc((void **)&d.lbflag, (size_t)cap), "level marks") ||
+        !hip_ok(hipMalloc((void **)&d.lbmap, sizeof(int) * (size_t)cap), "level marks"))
+      return false;
+    d.lbox_cap = cap;
+  }
+  if (th.extra < 2)
+  {
+    set_error("tree_adaptation: child boxes need a margin of at least one cell");
+    return false;
+  }
+  QZ_LAUNCHB("tree_level_mark", 1.0 * nb, k_fill<unsigned char>, nblk(nb), d.lbflag, (unsigned char)0, (size_t)nb);
+  QZ_LAUNCHB("tree_level_mark", 16.0 * ncell, k_level_flag, nblk(ncell), d.nodes, d.zone_node, (int)c0, (int)ncell, b0,
+             d.cell_box, d.cstart, d.cend, cfg.ref_criterion_ptcl, d.lbflag);
+  QZ_LAUNCHB("tree_level_mark", 12.0 * ncell, k_level_mark, nblk(ncell), d.nodes, d.zone_node, (int)c0, (int)ncell, b0,
+             d.cell_box, d.lbflag, d.lmark);
+  QZ_LAUNCHB("tree_level_mark", 4.0 * nb, k_fill<int>, nblk(nb), d.lbmap, -1, (size_t)nb);
+  timer.begin(stream);
+  if (!hip_ok(qz_select_flagged_index(d.lsel_tmp, d.lsel_bytes, d.lmark, d.llist, d.flag_count, (int)ncell, stream),
+              "level marks"))
+    return false;
+  timer.end(stream, "tree_level_select", 5.0 * ncell);
+  if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "marks") ||
+      !hip_ok(hipStreamSynchronize(stream), "marks"))
+    return false;
+  const int M = *d.hflag_count;
+  zone_cells = M;
+  if (M == 0)
+    return true;
+  if (!reserve_zone_scratch(d, M + 1, stream))
+    return false;
+  QZ_LAUNCHB("tree_level_index", 16.0 * M, k_level_index, nblk(M), d.llist, M, (int)c0, d.nodes, d.zone_node,
+             d.cell_box, b0, d.lbmap);
+  QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_level_hook_local, nblk(M), d.llist, M, (int)c0, d.nodes, d.zone_node,
+             d.cell_box, b0, d.lbmap, d.hpar);
+  QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_level_hook_global, nblk(M), d.llist, M, (int)c0, d.nodes, d.zone_node,
+             d.cell_box, b0, d.lbmap, d.hpar);
+  QZ_LAUNCHB("tree_level_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
+  timer.begin(stream);
+  if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "level zones"))
+    return false;
+  timer.end(stream, "tree_level_scan", 8.0 * (M + 1));
+  if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.hrank + M, sizeof(int), hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipStreamSynchronize(stream), "zones"))
+    return false;
+  const int nz = *d.hflag_count;
+  int end_bit = 1;
+  while ((1LL << end_bit) < nz)
+    end_bit++;
+  QZ_LAUNCHB("tree_level_zones", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
+  timer.begin(stream);
+  if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream),
+              "level zones"))
+    return false;
+  timer.end(stream, "tree_level_sort", 16.0 * M);
+  QZ_LAUNCHB("tree_level_zones", 24.0 * nz, k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
+  QZ_LAUNCHB("tree_level_zone_cells", 44.0 * M, k_level_zone_cells, nblk((M + kZoneRun - 1) / kZoneRun), M, d.hzone_s, d.hidx_s, d.llist, (int)c0,
+             d.cell_xyz, d.zone_node, (int)th.nodes.size(), d.hzstart, d.hzpar, d.hbbox, d.hz_node, d.hz_xyz);
+  std::vector<int> zstart(nz + 1), zpar(nz), bbox(6 * (size_t)nz), zsize(nz);
+  if (!hip_ok(hipMemcpyAsync(zstart.data(), d.hzstart, sizeof(int) * nz, hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipMemcpyAsync(zpar.data(), d.hzpar, sizeof(int) * nz, hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipMemcpyAsync(bbox.data(), d.hbbox, sizeof(int) * 6 * nz, hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipStreamSynchronize(stream), "zones"))
+    return false;
+  zstart[nz] = M;
+  for (int z = 0; z < nz; z++)
+    zsize[z] = zstart[z + 1] - zstart[z];
+  th.add_level_zones(nz, zpar.data(), bbox.data(), zsize.data());
   return true;
 }
 
@@ -466,7 +602,7 @@ template <typename P, typename G> bool Engine<P, G>::tree_adaptation()
 }
 
 // The tree below the head from the sorted particles (tree_construction.c rules):
-// head marks and zones on the device, the child boxes of every level on the
+// marks and zones of every level on the device, the child box geometry on the
 // host, the node tables, cell and point lists on the device.  Everything is
 // issued on stream s (the engine's stream, or the set-up stream when the tree
 // is built while the head density and potential are computed).
@@ -477,10 +613,10 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
   TreeDev &d = *tree->dev;
   const size_t ncell_head = (size_t)n * n * n;
   QZ_LAUNCHB("tree_fill", 4.0 * ncell_head, k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
-  int head_zone_cells = 0;
+  int zone_cells = 0; // zone cells of the next level, when its zones were found on the device
   {
     HostSpan span(timer, "tree_head_zones");
-    if (!refine_head_device(head_zone_cells, stream))
+    if (!refine_head_device(zone_cells, stream))
       return false;
   }
   d.hnodes.clear();
@@ -516,7 +652,7 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
     }
     if (!reserve_level<G>(&d, stream, nz, ncell, nb, npt, ng))
       return false;
-    const bool dev_zones = lr == 1; // head zones were built on the device
+    const bool dev_zones = lr == 1 || !host_refine; // zones found on the device
     std::vector<int> zn(dev_zones ? 0 : nz), zx(dev_zones ? 0 : 3 * nz);
     long long zo = 0;
     for (int q = first; q < first + count; q++)
@@ -579,9 +715,9 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
         return false;
       d.node_cap = (int)cap;
     }
-    if (dev_zones && nz != head_zone_cells)
+    if (dev_zones && nz != zone_cells)
     {
-      set_error("tree_adaptation: head zone cells %lld != %d", nz, head_zone_cells);
+      set_error("tree_adaptation: level %d zone cells %lld != %d", lr, nz, zone_cells);
       return false;
     }
     if (!hip_ok(hipMemcpyAsync(d.nodes + first, d.hnodes.data() + first, sizeof(DNode) * count,
@@ -664,7 +800,13 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
     QZ_LAUNCHB("tree_fill", (double)sizeof(G) * ng, k_fill<G>, nblk(ng), (G *)d.az + g0, (G)0, (size_t)ng);
     // ---- next level's refinement counts
     int lv = cfg.lmin + lr;
-    if (lv < cfg.lmax)
+    if (lv < cfg.lmax && !host_refine)
+    {
+      HostSpan span(timer, "tree_refine_level");
+      if (!refine_level_device(c0, ncell, b0, nb, zone_cells, stream))
+        return false;
+    }
+    else if (lv < cfg.lmax)
     {
       QZ_LAUNCH("tree_fill", k_fill<int>, 1, d.flag_count, 0, (size_t)1);
       QZ_LAUNCH("tree_flags", k_flag_cells, nblk(ncell), d.nodes, d.zone_node, first, (int)c0, (int)ncell, d.cstart,

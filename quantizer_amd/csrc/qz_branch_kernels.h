@@ -587,6 +587,205 @@ __global__ void k_head_zone_cells(int M, const unsigned *__restrict__ zk, const 
     }
 }
 
+// ------------------------------------------------------------------ node level marks and zones
+// fill_cell_ref for the nodes of one level (tree_construction.c:527-575, n_exp = 1):
+// a cell of a node is marked when a cell of its 3x3x3 block holds at least crit
+// particles.  The flagged cells are first written into a byte map over the
+// level's box pool (bflag, zero elsewhere), then every cell ORs its 27 box
+// neighbours.  Cells lie at least extra / 2 >= 1 box entries inside their box
+// (child box geometry), so the block never leaves it; entries outside the node
+// or beyond the simulation box are never flagged.
+__global__ void k_level_flag(const DNode *__restrict__ nodes, const int *__restrict__ zone_node, int c0, int ncell,
+                             long long b0, const int *__restrict__ cell_box, const int *__restrict__ cstart,
+                             const int *__restrict__ cend, int crit, unsigned char *__restrict__ bflag)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ncell)
+    return;
+  const int cs = c0 + s;
+  if (cend[cs] - cstart[cs] >= crit)
+    bflag[nodes[zone_node[cs >> 3]].boff - b0 + cell_box[cs]] = 1;
+}
+
+__global__ void k_level_mark(const DNode *__restrict__ nodes, const int *__restrict__ zone_node, int c0, int ncell,
+                             long long b0, const int *__restrict__ cell_box, const unsigned char *__restrict__ bflag,
+                             unsigned char *__restrict__ mark)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ncell)
+    return;
+  const int cs = c0 + s;
+  const DNode &d = nodes[zone_node[cs >> 3]];
+  const long long rX = d.rd[0], rXY = (long long)d.rd[0] * d.rd[1];
+  const unsigned char *f = bflag + (d.boff - b0 + cell_box[cs]);
+  unsigned char m = 0;
+  for (int kk = -1; kk <= 1; kk++)
+    for (int jj = -1; jj <= 1; jj++)
+      for (int ii = -1; ii <= 1; ii++)
+        m |= f[ii + jj * rX + kk * rXY];
+  mark[s] = m;
+}
+
+// map[box entry of marked cell i] = i.  The list holds level cell positions
+// (cell c0 + list[i]) in cell order.
+__global__ void k_level_index(const int *__restrict__ list, int M, int c0, const DNode *__restrict__ nodes,
+                              const int *__restrict__ zone_node, const int *__restrict__ cell_box, long long b0,
+                              int *__restrict__ map)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M)
+    return;
+  const int cs = c0 + list[i];
+  map[nodes[zone_node[cs >> 3]].boff - b0 + cell_box[cs]] = i;
+}
+
+// 6-connected zones of each node's marked cells (fill_zones_ref,
+// tree_construction.c:748-891): a marked cell joins its marked +x, +y and +z
+// box neighbours, which belong to the same node.  As for the head, the root of
+// a zone is its first cell in cell order and the zones are numbered by their
+// roots; nodes are in level order, so zone numbers run over the parents in
+// order and over each parent's zones in the order the reference creates them.
+//
+// Two passes.  k_level_hook_local joins, in LDS, the pairs whose cells both lie
+// in the block's kBlock consecutive list entries (consecutive entries are 2x2x2
+// octant groups of consecutive parent cells, so a block covers a compact piece
+// of a node and holds most of its pairs) and stores each entry's parent as the
+// smallest entry of its block-local set, which keeps parent <= entry.
+// k_level_hook_global then joins the pairs that cross blocks in the global
+// forest, whose trees start one level deep.  A global pass over every pair
+// builds long parent chains on large zones and walks them from every cell.
+__device__ inline void level_cell_neighbours(const int *__restrict__ list, int i, int c0,
+                                             const DNode *__restrict__ nodes, const int *__restrict__ zone_node,
+                                             const int *__restrict__ cell_box, long long b0,
+                                             const int *__restrict__ map, int nb[3])
+{
+  const int cs = c0 + list[i];
+  const DNode &d = nodes[zone_node[cs >> 3]];
+  const long long b = d.boff - b0 + cell_box[cs];
+  const long long step[3] = {1, d.rd[0], (long long)d.rd[0] * d.rd[1]};
+  for (int a = 0; a < 3; a++)
+    nb[a] = map[b + step[a]];
+}
+
+__global__ void __launch_bounds__(kBlock) k_level_hook_local(const int *__restrict__ list, int M, int c0,
+                                                             const DNode *__restrict__ nodes,
+                                                             const int *__restrict__ zone_node,
+                                                             const int *__restrict__ cell_box, long long b0,
+                                                             const int *__restrict__ map, int *__restrict__ par)
+{
+  __shared__ int lp[kBlock];
+  const int t = threadIdx.x, i0 = blockIdx.x * kBlock, i = i0 + t;
+  const int cnt = min(kBlock, M - i0);
+  lp[t] = t;
+  __syncthreads();
+  if (i < M)
+  {
+    int nb[3];
+    level_cell_neighbours(list, i, c0, nodes, zone_node, cell_box, b0, map, nb);
+    for (int a = 0; a < 3; a++)
+      if (nb[a] >= i0 && nb[a] < i0 + cnt)
+        uf_union(lp, t, nb[a] - i0);
+  }
+  __syncthreads();
+  const int r = uf_find(lp, t);
+  if (i < M)
+    par[i] = i0 + r;
+}
+
+__global__ void k_level_hook_global(const int *__restrict__ list, int M, int c0, const DNode *__restrict__ nodes,
+                                    const int *__restrict__ zone_node, const int *__restrict__ cell_box, long long b0,
+                                    const int *__restrict__ map, int *par)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M)
+    return;
+  const int i0 = i - i % kBlock, i1 = i0 + kBlock;
+  int nb[3];
+  level_cell_neighbours(list, i, c0, nodes, zone_node, cell_box, b0, map, nb);
+  for (int a = 0; a < 3; a++)
+    if (nb[a] >= 0 && (nb[a] < i0 || nb[a] >= i1))
+      uf_union(par, i, nb[a]);
+}
+
+// After the stable sort by zone: zone starts, parents (global node index of
+// the zone's node), bounding boxes (level cell coordinates), and the zone-cell
+// arrays of the next level's upload (zone order, cell order within a zone).
+// Each thread takes kZoneRun consecutive entries and folds their bounding box
+// while the zone stays the same (a zone ending inside the run is flushed with
+// atomics); the runs still open at the threads' last entries are combined over
+// the wavefront, so a zone of a million cells costs a few thousand atomics.
+constexpr int kZoneRun = 8;
+__global__ void k_level_zone_cells(int M, const unsigned *__restrict__ zk, const int *__restrict__ ival,
+                                   const int *__restrict__ list, int c0, const int *__restrict__ cell_xyz,
+                                   const int *__restrict__ zone_node, int first, int *__restrict__ zstart,
+                                   int *__restrict__ zpar, int *__restrict__ bbox, int *__restrict__ zone_node_out,
+                                   int *__restrict__ zone_xyz_out)
+{
+  const long long t0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * kZoneRun;
+  unsigned z = 0xffffffffu;
+  int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+  for (int k = 0; k < kZoneRun && t0 + k < M; k++)
+  {
+    const int t = (int)t0 + k;
+    const unsigned zt = zk[t];
+    const int cs = c0 + list[ival[t]];
+    if (t == 0 || zk[t - 1] != zt)
+    {
+      zstart[zt] = t;
+      zpar[zt] = zone_node[cs >> 3];
+    }
+    if (zt != z)
+    {
+      if (z != 0xffffffffu)
+        for (int a = 0; a < 3; a++)
+        {
+          atomicMin(&bbox[6 * z + a], lo[a]);
+          atomicMax(&bbox[6 * z + 3 + a], hi[a]);
+        }
+      z = zt;
+      for (int a = 0; a < 3; a++)
+      {
+        lo[a] = 0x7fffffff;
+        hi[a] = -1;
+      }
+    }
+    zone_node_out[t] = first + (int)zt;
+    for (int a = 0; a < 3; a++)
+    {
+      const int v = cell_xyz[3 * cs + a];
+      zone_xyz_out[3 * t + a] = v;
+      lo[a] = min(lo[a], v);
+      hi[a] = max(hi[a], v);
+    }
+  }
+  // the open runs, combined over lanes of equal zone (entries are sorted by
+  // zone, so equal zones at two lanes hold every lane between them)
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1)
+  {
+    const unsigned oz = __shfl_down(z, off, 64);
+    int olo[3], ohi[3];
+    for (int a = 0; a < 3; a++)
+    {
+      olo[a] = __shfl_down(lo[a], off, 64);
+      ohi[a] = __shfl_down(hi[a], off, 64);
+    }
+    if (lane + off < 64 && oz == z)
+      for (int a = 0; a < 3; a++)
+      {
+        lo[a] = min(lo[a], olo[a]);
+        hi[a] = max(hi[a], ohi[a]);
+      }
+  }
+  const unsigned pz = __shfl_up(z, 1, 64);
+  if (z != 0xffffffffu && (lane == 0 || pz != z))
+    for (int a = 0; a < 3; a++)
+    {
+      atomicMin(&bbox[6 * z + a], lo[a]);
+      atomicMax(&bbox[6 * z + 3 + a], hi[a]);
+    }
+}
+
 // Refined cells point at their child node: head_child[c] / box[parent] = child index.
 __global__ void k_mark_children(const DNode *__restrict__ nodes, const int *__restrict__ zone_node,
                                 const int *__restrict__ zone_xyz, int z0, int nz, int n, int *__restrict__ head_child,

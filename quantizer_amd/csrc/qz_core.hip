@@ -168,6 +168,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool setup_ahead = true;   // first tree level's node set-up during the head solve (QZ_NODE_SETUP_AHEAD=0: in place)
   bool head_graphs = true;   // head outer iterations replayed as graphs (QZ_HEAD_GRAPH=0: launched one by one)
   bool batch_graphs = true;  // node PCG batches replayed as graphs (QZ_BATCH_GRAPH=0: launched one by one)
+  bool host_refine = false;  // marks, zones and child boxes below the head on the host (QZ_HOST_TREE=1)
   double *part = nullptr; // reduction partials
   double *red = nullptr;  // device scalars
   double *hred = nullptr; // pinned host mirror
@@ -395,6 +396,8 @@ template <typename P, typename G> struct Engine final : EngineBase
       head_graphs = f[0] != '0';
     if (const char *f = std::getenv("QZ_BATCH_GRAPH"))
       batch_graphs = f[0] != '0';
+    if (const char *f = std::getenv("QZ_HOST_TREE"))
+      host_refine = f[0] == '1';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
@@ -1190,6 +1193,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
   bool refine_head_device(int &zone_cells, hipStream_t s);
+  bool refine_level_device(long long c0, long long ncell, long long b0, long long nb, int &zone_cells, hipStream_t s);
   bool tree_state();
   bool build_tree(hipStream_t s);
   bool force_overlapped(bool with_particles) override;

@@ -476,6 +476,28 @@ int TreeHost::add_head_zones(int nz, const int *bbox6, const int *zsize, const i
   return nz;
 }
 
+int TreeHost::add_level_zones(int nz, const int *parent, const int *bbox6, const int *zsize)
+{
+  const int first = (int)nodes.size();
+  nodes.resize(first + nz);
+  int zone = 0;
+  for (int z = 0; z < nz; z++)
+  {
+    zone = z > 0 && parent[z] == parent[z - 1] ? zone + 1 : 0;
+    HNode &p = nodes[parent[z]];
+    nodes[first + z] = child_geometry(parent[z], p.lv, zone, bbox6 + 6 * z, bbox6 + 6 * z + 3, zsize[z]);
+    if (zone == 0)
+      p.child.clear();
+    p.child.push_back(first + z);
+  }
+  if (nz > 0)
+  {
+    lev_start.push_back(first);
+    lev_count.push_back(nz);
+  }
+  return nz;
+}
+
 // Marks, zones and children of one parent (tree_construction.c:508-926).
 // fill_cell_ref with n_exp = 1: each flagged cell marks the existing cells of
 // its 3x3x3 block (box status -3 -> -1); the marked cells are then read in cell

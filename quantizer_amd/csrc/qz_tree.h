@@ -1,8 +1,10 @@
-// Host side of the refinement tree: the irregular, graph-shaped part of
-// tree_construction.c (dilated refinement marks, 6-connected zones, child box
-// geometry).  Bulk per-cell and per-grid-point work (child cell expansion, box
-// status, grid-point classification, particle ranges, refinement counts) runs
-// on the device from what this produces.
+// Host side of the refinement tree: the node list and the child box geometry
+// of tree_construction.c.  Marks and zones are found on the device and arrive
+// as zone bounding boxes (add_head_zones, add_level_zones); refine_level finds
+// them on the host instead (dilated refinement marks, 6-connected zones), from
+// the flagged cells.  Bulk per-cell and per-grid-point work (child cell
+// expansion, box status, grid-point classification, particle ranges,
+// refinement counts) runs on the device from what this produces.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -73,6 +75,13 @@ struct TreeHost
   int add_head_zones(int nz, const int *bbox6, const int *zsize, const int *cells);
   // Level lr >= 1: flagged cells as (node position in level, cell position), any order.
   int refine_level(int lr, const std::vector<int> &flag_node, const std::vector<int> &flag_cell);
+  // The next level from zones found on the device: per zone its parent (global
+  // node index), the bounding box of its cells in parent-level coordinates (min
+  // xyz, max xyz) and its cell count; zones in parent order and, within a
+  // parent, in the order of their first cells.  The zone cells stay on the
+  // device, so these nodes cannot be refined by refine_level.  Returns the
+  // number of nodes made.
+  int add_level_zones(int nz, const int *parent, const int *bbox6, const int *zsize);
   void clear();
 
 private:
