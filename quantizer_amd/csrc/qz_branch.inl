@@ -403,8 +403,12 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
     return false;
   // per marked cell: the list entry, the cell map, the six neighbours' map
   // entries and the union-find parents
-  QZ_LAUNCHB("tree_head_zones", 12.0 * M, k_head_index, nblk(M), d.hlist, M, d.head_child, d.hpar);
-  QZ_LAUNCHB("tree_head_hook", 36.0 * M, k_head_hook, nblk(M), d.hlist, M, n, d.head_child, d.hpar);
+  QZ_LAUNCHB("tree_head_zones", 12.0 * M, k_head_index, nblk(M), d.hlist, M, d.head_child);
+  {
+    const ZoneNbr g{d.hlist, d.head_child, n, 0, nullptr, nullptr, nullptr, 0};
+    QZ_LAUNCHB("tree_head_hook", 28.0 * M, k_zone_hook_local, nblk(M), g, M, d.hpar, d.hidx);
+    QZ_LAUNCHB("tree_head_hook", 28.0 * M, k_zone_hook_global, nblk(M), g, M, (const int *)d.hidx, d.hpar);
+  }
   QZ_LAUNCHB("tree_head_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "head zones"))
@@ -521,10 +525,11 @@ bool Engine<P, G>::refine_level_device(long long c0, long long ncell, long long 
     return false;
   QZ_LAUNCHB("tree_level_index", 16.0 * M, k_level_index, nblk(M), d.llist, M, (int)c0, d.nodes, d.zone_node,
              d.cell_box, b0, d.lbmap);
-  QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_level_hook_local, nblk(M), d.llist, M, (int)c0, d.nodes, d.zone_node,
-             d.cell_box, b0, d.lbmap, d.hpar);
-  QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_level_hook_global, nblk(M), d.llist, M, (int)c0, d.nodes, d.zone_node,
-             d.cell_box, b0, d.lbmap, d.hpar);
+  {
+    const ZoneNbr g{d.llist, d.lbmap, 0, (int)c0, d.nodes, d.zone_node, d.cell_box, b0};
+    QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_zone_hook_local, nblk(M), g, M, d.hpar, d.hidx);
+    QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_zone_hook_global, nblk(M), g, M, (const int *)d.hidx, d.hpar);
+  }
   QZ_LAUNCHB("tree_level_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "level zones"))

@@ -421,15 +421,12 @@ __global__ void k_head_dilate(const unsigned char *__restrict__ in, unsigned cha
   out[c] = m;
 }
 
-// map[cell] = position of the marked cell in the ordered list; every list entry
-// starts as its own union-find root
-__global__ void k_head_index(const int *__restrict__ list, int M, int *__restrict__ map, int *__restrict__ par)
+// map[cell] = position of the marked cell in the ordered list
+__global__ void k_head_index(const int *__restrict__ list, int M, int *__restrict__ map)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M)
-    return;
-  map[list[i]] = i;
-  par[i] = i;
+  if (i < M)
+    map[list[i]] = i;
 }
 
 // Union-find with parent[x] <= x at all times: roots link only under smaller
@@ -474,33 +471,8 @@ __device__ inline void uf_union(int *par, int a, int b)
 // 6-connected zones of the marked cells (fill_zones_ref, tree_construction.c:748-891):
 // each marked cell joins its marked +x, +y and +z neighbours.  The zone's root is
 // its first cell in cell order, which is the cell the reference's flood fill
-// starts the zone from; zones are numbered by their roots in cell order.
-__global__ void k_head_hook(const int *__restrict__ list, int M, int n, const int *__restrict__ map, int *par)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M)
-    return;
-  const int c = list[i];
-  const int x = c % n, y = (c / n) % n, z = c / (n * n);
-  if (x + 1 < n)
-  {
-    const int j = map[c + 1];
-    if (j >= 0)
-      uf_union(par, i, j);
-  }
-  if (y + 1 < n)
-  {
-    const int j = map[c + n];
-    if (j >= 0)
-      uf_union(par, i, j);
-  }
-  if (z + 1 < n)
-  {
-    const int j = map[c + n * n];
-    if (j >= 0)
-      uf_union(par, i, j);
-  }
-}
+// starts the zone from; zones are numbered by their roots in cell order.  The
+// joins run in two passes (k_zone_hook_local, k_zone_hook_global), below.
 
 // par[i] = root; rootflag[i] = 1 for roots, rootflag[M] = 0 (for the count)
 __global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag)
@@ -645,33 +617,51 @@ __global__ void k_level_index(const int *__restrict__ list, int M, int c0, const
 // a zone is its first cell in cell order and the zones are numbered by their
 // roots; nodes are in level order, so zone numbers run over the parents in
 // order and over each parent's zones in the order the reference creates them.
-//
-// Two passes.  k_level_hook_local joins, in LDS, the pairs whose cells both lie
-// in the block's kBlock consecutive list entries (consecutive entries are 2x2x2
-// octant groups of consecutive parent cells, so a block covers a compact piece
-// of a node and holds most of its pairs) and stores each entry's parent as the
-// smallest entry of its block-local set, which keeps parent <= entry.
-// k_level_hook_global then joins the pairs that cross blocks in the global
-// forest, whose trees start one level deep.  A global pass over every pair
-// builds long parent chains on large zones and walks them from every cell.
-__device__ inline void level_cell_neighbours(const int *__restrict__ list, int i, int c0,
-                                             const DNode *__restrict__ nodes, const int *__restrict__ zone_node,
-                                             const int *__restrict__ cell_box, long long b0,
-                                             const int *__restrict__ map, int nb[3])
-{
-  const int cs = c0 + list[i];
-  const DNode &d = nodes[zone_node[cs >> 3]];
-  const long long b = d.boff - b0 + cell_box[cs];
-  const long long step[3] = {1, d.rd[0], (long long)d.rd[0] * d.rd[1]};
-  for (int a = 0; a < 3; a++)
-    nb[a] = map[b + step[a]];
-}
 
-__global__ void __launch_bounds__(kBlock) k_level_hook_local(const int *__restrict__ list, int M, int c0,
-                                                             const DNode *__restrict__ nodes,
-                                                             const int *__restrict__ zone_node,
-                                                             const int *__restrict__ cell_box, long long b0,
-                                                             const int *__restrict__ map, int *__restrict__ par)
+// The +x, +y, +z neighbours of marked cell i as list entries (-1: not marked),
+// for the head (n > 0: list holds head cell indices, map is the n^3 cell map)
+// or for the nodes of a level (n = 0: list holds level cell positions, map
+// covers the level's box pool).
+struct ZoneNbr
+{
+  const int *list, *map;
+  int n;
+  int c0;
+  const DNode *nodes;
+  const int *zone_node, *cell_box;
+  long long b0;
+  __device__ void get(int i, int nb[3]) const
+  {
+    if (n > 0)
+    {
+      const int c = list[i];
+      const int x = c % n, y = (c / n) % n, z = c / (n * n);
+      nb[0] = x + 1 < n ? map[c + 1] : -1;
+      nb[1] = y + 1 < n ? map[c + n] : -1;
+      nb[2] = z + 1 < n ? map[c + n * n] : -1;
+      return;
+    }
+    // node cells lie at least one box entry inside their box (child box geometry)
+    const int cs = c0 + list[i];
+    const DNode &d = nodes[zone_node[cs >> 3]];
+    const long long b = d.boff - b0 + cell_box[cs];
+    nb[0] = map[b + 1];
+    nb[1] = map[b + d.rd[0]];
+    nb[2] = map[b + (long long)d.rd[0] * d.rd[1]];
+  }
+};
+
+// Two passes.  k_zone_hook_local joins, in LDS, the pairs whose cells both lie
+// in the block's kBlock consecutive list entries (for nodes these are 2x2x2
+// octant groups of consecutive parent cells, a compact piece of a node; for the
+// head a few rows) and stores, as parent and as lroot, the smallest entry of
+// each entry's block-local set (so parent <= entry holds).  k_zone_hook_global
+// then joins the pairs that cross blocks, by their local roots, once per
+// distinct pair of local roots in a block (an LDS hash set): a global pass over
+// every pair builds long parent chains on large zones and walks them from
+// every cell.
+__global__ void __launch_bounds__(kBlock) k_zone_hook_local(ZoneNbr g, int M, int *__restrict__ par,
+                                                            int *__restrict__ lroot)
 {
   __shared__ int lp[kBlock];
   const int t = threadIdx.x, i0 = blockIdx.x * kBlock, i = i0 + t;
@@ -681,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) k_level_hook_local(const int *__restri
   if (i < M)
   {
     int nb[3];
-    level_cell_neighbours(list, i, c0, nodes, zone_node, cell_box, b0, map, nb);
+    g.get(i, nb);
     for (int a = 0; a < 3; a++)
       if (nb[a] >= i0 && nb[a] < i0 + cnt)
         uf_union(lp, t, nb[a] - i0);
@@ -689,22 +679,50 @@ __global__ void __launch_bounds__(kBlock) k_level_hook_local(const int *__restri
   __syncthreads();
   const int r = uf_find(lp, t);
   if (i < M)
+  {
     par[i] = i0 + r;
+    lroot[i] = i0 + r;
+  }
 }
 
-__global__ void k_level_hook_global(const int *__restrict__ list, int M, int c0, const DNode *__restrict__ nodes,
-                                    const int *__restrict__ zone_node, const int *__restrict__ cell_box, long long b0,
-                                    const int *__restrict__ map, int *par)
+constexpr int kZoneHashBits = 10; // > 3 kBlock entries: the set never fills
+__global__ void __launch_bounds__(kBlock) k_zone_hook_global(ZoneNbr g, int M, const int *__restrict__ lroot,
+                                                             int *par)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr unsigned kSize = 1u << kZoneHashBits;
+  constexpr unsigned long long kEmpty = ~0ull;
+  __shared__ unsigned long long set[kSize];
+  for (unsigned t = threadIdx.x; t < kSize; t += blockDim.x)
+    set[t] = kEmpty;
+  __syncthreads();
+  const int i0 = blockIdx.x * kBlock, i = i0 + threadIdx.x;
   if (i >= M)
     return;
-  const int i0 = i - i % kBlock, i1 = i0 + kBlock;
   int nb[3];
-  level_cell_neighbours(list, i, c0, nodes, zone_node, cell_box, b0, map, nb);
+  g.get(i, nb);
+  const int ri = lroot[i];
   for (int a = 0; a < 3; a++)
-    if (nb[a] >= 0 && (nb[a] < i0 || nb[a] >= i1))
-      uf_union(par, i, nb[a]);
+  {
+    const int j = nb[a];
+    if (j < 0 || (j >= i0 && j < i0 + kBlock))
+      continue;
+    const int rj = lroot[j];
+    const unsigned lo = (unsigned)min(ri, rj), hi = (unsigned)max(ri, rj);
+    const unsigned long long key = ((unsigned long long)lo << 32) | hi;
+    unsigned h = ((lo * 0x9E3779B1u) ^ (hi * 0x85EBCA77u)) >> (32 - kZoneHashBits);
+    while (true)
+    {
+      const unsigned long long old = atomicCAS(&set[h], kEmpty, key);
+      if (old == kEmpty)
+      {
+        uf_union(par, (int)lo, (int)hi);
+        break;
+      }
+      if (old == key)
+        break;
+      h = (h + 1) & (kSize - 1);
+    }
+  }
 }
 
 // After the stable sort by zone: zone starts, parents (global node index of
