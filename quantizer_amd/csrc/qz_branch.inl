@@ -59,8 +59,12 @@ struct TreeDev
   // marked cells, union-find and zone arrays sized by the marked count
   unsigned char *hmark = nullptr;
   int *hlist = nullptr;
-  void *hsel_tmp = nullptr;
-  size_t hsel_bytes = 0;
+  // ordered compaction of byte flags (select_flags): per-block counts, their
+  // exclusive scan and the scan's scratch
+  int *sel_cnt = nullptr, *sel_off = nullptr;
+  long long sel_cap = 0;
+  void *sel_scan = nullptr;
+  size_t sel_scan_bytes = 0;
   int *hpar = nullptr, *hflag = nullptr, *hrank = nullptr, *hidx = nullptr, *hidx_s = nullptr, *hz_node = nullptr,
       *hz_xyz = nullptr, *hzstart = nullptr, *hbbox = nullptr;
   unsigned *hzone = nullptr, *hzone_s = nullptr;
@@ -71,8 +75,6 @@ struct TreeDev
   unsigned char *lbflag = nullptr, *lmark = nullptr;
   int *lbmap = nullptr, *llist = nullptr;
   long long lbox_cap = 0, lcell_cap = 0;
-  void *lsel_tmp = nullptr;
-  size_t lsel_bytes = 0;
   void *htmp = nullptr;
   size_t htmp_bytes = 0;
   double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
@@ -129,8 +131,6 @@ struct TreeDev
   long long chunk_cap = 0;
   int *alist = nullptr, *acount = nullptr; // active points (pool indices) and their count
   long long alist_cap = 0;
-  void *sel_tmp = nullptr;
-  size_t sel_bytes = 0;
   std::vector<int> hrbeg; // region begins in the pools
   int *rbeg = nullptr, *rstart = nullptr;
   long long reg_cap = 0;
@@ -241,10 +241,9 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
                     d->phi, d->phi_old, d->cg_r, d->cg_p, d->cg_Ap, d->rho, d->ax,
                     d->ay, d->az, d->head_child, d->flags, d->flag_count, d->part1, d->part2, d->part3,
                     d->done_count, d->mpart, d->bmg, d->mg_e, d->mg_r, d->mg_t, d->ntiles, d->mg_mask, d->hmark, d->hlist,
-                    d->hsel_tmp, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
+                    d->sel_cnt, d->sel_off, d->sel_scan, d->hpar, d->hflag, d->hrank, d->hidx, d->hidx_s, d->hz_node, d->hz_xyz,
                     d->hzstart, d->hbbox, d->hzone, d->hzone_s, d->htmp, d->hzpar, d->lbflag, d->lmark,
-                    d->lbmap, d->llist, d->lsel_tmp, d->chunks, d->alist, d->acount,
-                    d->sel_tmp, d->rbeg, d->rstart, d->ctab, d->chunk0, d->ccnt,
+                    d->lbmap, d->llist, d->chunks, d->alist, d->acount, d->rbeg, d->rstart, d->ctab, d->chunk0, d->ccnt,
                     d->coff, d->cscan_tmp};
     for (void *q : ptrs)
       (void)hipFree(q);
@@ -320,6 +319,62 @@ template <typename G> static bool reserve_level(TreeDev *d, hipStream_t s, long 
   return true;
 }
 
+// The indices i < n with f[i] != 0, ascending, into out, and their count into
+// *num (device), on `stream` (k_select_count, a scan of the block counts,
+// k_select_write).
+static bool select_flags(TreeDev &d, const unsigned char *f, long long n, int *out, int *num, hipStream_t stream)
+{
+  const long long nb = std::max<long long>(1, (n + kSelItems - 1) / kSelItems);
+  if (nb + 1 > d.sel_cap)
+  {
+    if (!hip_ok(hipStreamSynchronize(stream), "selection"))
+      return false; // the buffers about to be replaced may still be read by the stream
+    (void)hipFree(d.sel_cnt);
+    (void)hipFree(d.sel_off);
+    (void)hipFree(d.sel_scan);
+    d.sel_cnt = d.sel_off = nullptr;
+    d.sel_scan = nullptr;
+    d.sel_cap = 0;
+    const long long cap = next_cap(0, nb + 1);
+    d.sel_scan_bytes = std::max<size_t>(qz_scan_temp_bytes((int)cap), 1);
+    if (!hip_ok(hipMalloc((void **)&d.sel_cnt, sizeof(int) * (size_t)cap), "selection") ||
+        !hip_ok(hipMalloc((void **)&d.sel_off, sizeof(int) * (size_t)cap), "selection") ||
+        !hip_ok(hipMalloc(&d.sel_scan, d.sel_scan_bytes), "selection"))
+      return false;
+    d.sel_cap = cap;
+  }
+  hipLaunchKernelGGL(k_select_count, dim3((unsigned)nb), dim3(kBlock), 0, stream, f, n, d.sel_cnt);
+  if (!hip_ok(qz_exclusive_scan(d.sel_scan, d.sel_scan_bytes, d.sel_cnt, d.sel_off, (int)nb + 1, stream), "selection"))
+    return false;
+  hipLaunchKernelGGL(k_select_write, dim3((unsigned)nb), dim3(kBlock), 0, stream, f, n, (const int *)d.sel_off, out,
+                     num);
+  return hip_ok(hipGetLastError(), "selection");
+}
+
+// Zone of every marked cell (the rank of its root) into d.hzone_s and the list
+// entries in zone order into d.hidx_s: a stable sort by zone, or, when the
+// zones already are runs of consecutive entries (`unsorted` false: every root
+// at least its predecessor's), the entries as they are.
+template <typename P, typename G> bool Engine<P, G>::zones_in_order(int M, int nz, bool unsorted, hipStream_t s)
+{
+  hipStream_t stream = s;
+  TreeDev &d = *tree->dev;
+  if (!unsorted)
+  {
+    QZ_LAUNCHB("tree_zone_order", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone_s, d.hidx_s);
+    return true;
+  }
+  int end_bit = 1;
+  while ((1LL << end_bit) < nz)
+    end_bit++;
+  QZ_LAUNCHB("tree_zone_order", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
+  timer.begin(stream);
+  if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream), "zones"))
+    return false;
+  timer.end(stream, "tree_zone_sort", 16.0 * M);
+  return true;
+}
+
 // The union-find and zone arrays of the marked cells of one level (head or
 // nodes), for `need` entries (grow-only, contents not kept).
 static bool reserve_zone_scratch(TreeDev &d, long long need, hipStream_t stream)
@@ -371,24 +426,32 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
   const long long ncell = (long long)n * n * n;
   if (!d.hmark)
   {
-    d.hsel_bytes = qz_select_temp_bytes((int)ncell);
     if (!hip_ok(hipMalloc((void **)&d.hmark, (size_t)ncell), "head marks") ||
-        !hip_ok(hipMalloc((void **)&d.hlist, sizeof(int) * (size_t)ncell), "head marks") ||
-        !hip_ok(hipMalloc(&d.hsel_tmp, std::max<size_t>(d.hsel_bytes, 1)), "head marks"))
+        !hip_ok(hipMalloc((void **)&d.hlist, sizeof(int) * (size_t)ncell), "head marks"))
       return false;
   }
-  // flags, then the x, y, z dilations (the head list is free scratch until the
+  // flags and their x, y, z dilations (the head list is free scratch until the
   // selection below fills it)
   unsigned char *tmp = (unsigned char *)d.hlist;
-  QZ_LAUNCHB("tree_head_mark", 5.0 * ncell, k_head_flag, nblk(ncell), cell_start, ncell, cfg.ref_criterion_ptcl, tmp);
-  QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), tmp, d.hmark, cfg.lmin, 0);
-  QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), d.hmark, tmp, cfg.lmin, 1);
-  QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), tmp, d.hmark, cfg.lmin, 2);
+  if (n >= 16)
+  {
+    const long long nvec = ncell / 16;
+    QZ_LAUNCHB("tree_head_mark", 4.0 * ncell + ncell, k_head_flag_x, nblk(nvec), cell_start, ncell, cfg.lmin,
+               cfg.ref_criterion_ptcl, (uint4 *)tmp);
+    QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate_yz, nblk(nvec), (const uint4 *)tmp, nvec, cfg.lmin,
+               (uint4 *)d.hmark);
+  }
+  else
+  {
+    QZ_LAUNCHB("tree_head_mark", 5.0 * ncell, k_head_flag, nblk(ncell), cell_start, ncell, cfg.ref_criterion_ptcl, tmp);
+    QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), tmp, d.hmark, cfg.lmin, 0);
+    QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), d.hmark, tmp, cfg.lmin, 1);
+    QZ_LAUNCHB("tree_head_mark", 2.0 * ncell, k_head_dilate, nblk(ncell), tmp, d.hmark, cfg.lmin, 2);
+  }
   timer.begin(stream);
-  if (!hip_ok(qz_select_flagged_index(d.hsel_tmp, d.hsel_bytes, d.hmark, d.hlist, d.flag_count, (int)ncell, stream),
-              "head marks"))
+  if (!select_flags(d, d.hmark, ncell, d.hlist, d.flag_count, stream))
     return false;
-  timer.end(stream, "tree_head_select", 5.0 * ncell);
+  timer.end(stream, "tree_head_select", 2.0 * ncell);
   if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "marks") ||
       !hip_ok(hipStreamSynchronize(stream), "marks"))
     return false;
@@ -409,24 +472,20 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
     QZ_LAUNCHB("tree_head_hook", 28.0 * M, k_zone_hook_local, nblk(M), g, M, d.hpar, d.hidx);
     QZ_LAUNCHB("tree_head_hook", 28.0 * M, k_zone_hook_global, nblk(M), g, M, (const int *)d.hidx, d.hpar);
   }
-  QZ_LAUNCHB("tree_head_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
+  QZ_LAUNCH("tree_head_zones", k_fill<int>, 1, d.flag_count + 1, 0, (size_t)1);
+  QZ_LAUNCHB("tree_head_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag, d.flag_count + 1);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "head zones"))
     return false;
   timer.end(stream, "tree_head_zones", 8.0 * (M + 1));
   if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.hrank + M, sizeof(int), hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipMemcpyAsync(d.hflag_count + 1, d.flag_count + 1, sizeof(int), hipMemcpyDeviceToHost, stream),
+              "zones") ||
       !hip_ok(hipStreamSynchronize(stream), "zones"))
     return false;
-  const int nz = *d.hflag_count;
-  int end_bit = 1;
-  while ((1LL << end_bit) < nz)
-    end_bit++;
-  QZ_LAUNCHB("tree_head_zones", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
-  timer.begin(stream);
-  if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream),
-              "head zones"))
+  const int nz = d.hflag_count[0];
+  if (!zones_in_order(M, nz, d.hflag_count[1] != 0, stream))
     return false;
-  timer.end(stream, "tree_head_sort", 16.0 * M);
   QZ_LAUNCHB("tree_head_zones", 24.0 * nz, k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
   QZ_LAUNCHB("tree_head_zones", 40.0 * M, k_head_zone_cells, nblk(M), M, d.hzone_s, d.hidx_s, d.hlist, n, 0, d.hzstart,
              d.hbbox,
@@ -470,16 +529,12 @@ bool Engine<P, G>::refine_level_device(long long c0, long long ncell, long long 
       return false;
     (void)hipFree(d.lmark);
     (void)hipFree(d.llist);
-    (void)hipFree(d.lsel_tmp);
     d.lmark = nullptr;
     d.llist = nullptr;
-    d.lsel_tmp = nullptr;
     d.lcell_cap = 0;
     const long long cap = next_cap(0, ncell);
-    d.lsel_bytes = std::max<size_t>(qz_select_temp_bytes((int)cap), 1);
     if (!hip_ok(hipMalloc((void **)&d.lmark, (size_t)cap), "level marks") ||
-        !hip_ok(hipMalloc((void **)&d.llist, sizeof(int) * (size_t)cap), "level marks") ||
-        !hip_ok(hipMalloc(&d.lsel_tmp, d.lsel_bytes), "level marks"))
+        !hip_ok(hipMalloc((void **)&d.llist, sizeof(int) * (size_t)cap), "level marks"))
       return false;
     d.lcell_cap = cap;
   }
@@ -510,10 +565,9 @@ bool Engine<P, G>::refine_level_device(long long c0, long long ncell, long long 
              d.cell_box, d.lbflag, d.lmark);
   QZ_LAUNCHB("tree_level_mark", 4.0 * nb, k_fill<int>, nblk(nb), d.lbmap, -1, (size_t)nb);
   timer.begin(stream);
-  if (!hip_ok(qz_select_flagged_index(d.lsel_tmp, d.lsel_bytes, d.lmark, d.llist, d.flag_count, (int)ncell, stream),
-              "level marks"))
+  if (!select_flags(d, d.lmark, ncell, d.llist, d.flag_count, stream))
     return false;
-  timer.end(stream, "tree_level_select", 5.0 * ncell);
+  timer.end(stream, "tree_level_select", 2.0 * ncell);
   if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.flag_count, sizeof(int), hipMemcpyDeviceToHost, stream), "marks") ||
       !hip_ok(hipStreamSynchronize(stream), "marks"))
     return false;
@@ -530,24 +584,21 @@ bool Engine<P, G>::refine_level_device(long long c0, long long ncell, long long 
     QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_zone_hook_local, nblk(M), g, M, d.hpar, d.hidx);
     QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_zone_hook_global, nblk(M), g, M, (const int *)d.hidx, d.hpar);
   }
-  QZ_LAUNCHB("tree_level_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag);
+  QZ_LAUNCH("tree_level_zones", k_fill<int>, 1, d.flag_count + 1, 0, (size_t)1);
+  QZ_LAUNCHB("tree_level_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag,
+             d.flag_count + 1);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "level zones"))
     return false;
   timer.end(stream, "tree_level_scan", 8.0 * (M + 1));
   if (!hip_ok(hipMemcpyAsync(d.hflag_count, d.hrank + M, sizeof(int), hipMemcpyDeviceToHost, stream), "zones") ||
+      !hip_ok(hipMemcpyAsync(d.hflag_count + 1, d.flag_count + 1, sizeof(int), hipMemcpyDeviceToHost, stream),
+              "zones") ||
       !hip_ok(hipStreamSynchronize(stream), "zones"))
     return false;
-  const int nz = *d.hflag_count;
-  int end_bit = 1;
-  while ((1LL << end_bit) < nz)
-    end_bit++;
-  QZ_LAUNCHB("tree_level_zones", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
-  timer.begin(stream);
-  if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream),
-              "level zones"))
+  const int nz = d.hflag_count[0];
+  if (!zones_in_order(M, nz, d.hflag_count[1] != 0, stream))
     return false;
-  timer.end(stream, "tree_level_sort", 16.0 * M);
   QZ_LAUNCHB("tree_level_zones", 24.0 * nz, k_bbox_init, nblk(6LL * nz), nz, d.hbbox);
   QZ_LAUNCHB("tree_level_zone_cells", 44.0 * M, k_level_zone_cells, nblk((M + kZoneRun - 1) / kZoneRun), M, d.hzone_s, d.hidx_s, d.llist, (int)c0,
              d.cell_xyz, d.zone_node, (int)th.nodes.size(), d.hzstart, d.hzpar, d.hbbox, d.hz_node, d.hz_xyz);
@@ -1128,18 +1179,15 @@ bool Engine<P, G>::ensure_bmg_buffers(int n_itab, long long n_chunks, long long 
     (void)hipFree(d.mg_t);
     (void)hipFree(d.mg_mask);
     (void)hipFree(d.alist);
-    (void)hipFree(d.sel_tmp);
-    d.mg_e = d.mg_r = d.mg_t = d.sel_tmp = nullptr;
+    d.mg_e = d.mg_r = d.mg_t = nullptr;
     d.mg_mask = nullptr;
     d.alist = nullptr;
     d.mg_cap = next_cap(d.mg_cap, pool);
-    d.sel_bytes = qz_select_temp_bytes((int)d.mg_cap);
     if (!hip_ok(hipMalloc(&d.mg_e, sizeof(G) * d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc(&d.mg_r, sizeof(G) * d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc(&d.mg_t, sizeof(G) * d.mg_cap), "multigrid pool") ||
         !hip_ok(hipMalloc((void **)&d.mg_mask, d.mg_cap), "multigrid pool") ||
-        !hip_ok(hipMalloc((void **)&d.alist, sizeof(int) * d.mg_cap), "multigrid pool") ||
-        !hip_ok(hipMalloc(&d.sel_tmp, std::max<size_t>(d.sel_bytes, 1)), "multigrid pool"))
+        !hip_ok(hipMalloc((void **)&d.alist, sizeof(int) * d.mg_cap), "multigrid pool"))
       return false;
   }
   if (!d.acount && !hip_ok(hipMalloc((void **)&d.acount, sizeof(int) * 4), "multigrid pool"))
@@ -1333,8 +1381,7 @@ template <typename P, typename G> bool Engine<P, G>::bmg_prepare(size_t l, hipSt
   const int nreg = (int)rbeg.size();
   rbeg.push_back((int)pool);
   timer.begin(stream);
-  if (!hip_ok(qz_select_flagged_index(d.sel_tmp, d.sel_bytes, mk, d.alist, d.acount, (int)pool, stream),
-              "active points") ||
+  if (!select_flags(d, mk, pool, d.alist, d.acount, stream) ||
       !hip_ok(hipMemcpyAsync(d.rbeg, rbeg.data(), sizeof(int) * (nreg + 1), hipMemcpyHostToDevice, stream),
               "regions"))
     return false;

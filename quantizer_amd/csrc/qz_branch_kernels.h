@@ -390,6 +390,98 @@ __global__ void k_flag_cells(const DNode *__restrict__ nodes, const int *__restr
   }
 }
 
+// ------------------------------------------------------------------ ordered compaction of byte flags
+// The indices i < n with f[i] != 0, ascending.  A block takes kSelItems
+// consecutive flags, kSelPer per thread (one 16-byte load where the run is
+// whole): k_select_count stores each block's count (and a zero after the last
+// block, so that the exclusive scan of the counts ends with the total),
+// k_select_write recounts, scans the block's threads and writes the indices
+// from the block's first slot.
+constexpr int kSelPer = 16;
+constexpr int kSelItems = kSelPer * kBlock;
+
+__device__ inline int nonzero_bytes(unsigned w)
+{
+  return __popc((((w & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w) & 0x80808080u);
+}
+
+__device__ inline void load_flags16(const unsigned char *__restrict__ f, long long n, long long i0, unsigned w[4])
+{
+  if (i0 + kSelPer <= n)
+  {
+    const uint4 v = *(const uint4 *)(f + i0);
+    w[0] = v.x;
+    w[1] = v.y;
+    w[2] = v.z;
+    w[3] = v.w;
+    return;
+  }
+  for (int q = 0; q < 4; q++)
+  {
+    w[q] = 0;
+    for (int k = 0; k < 4; k++)
+      if (i0 + 4 * q + k < n)
+        w[q] |= (unsigned)f[i0 + 4 * q + k] << (8 * k);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_select_count(const unsigned char *__restrict__ f, long long n,
+                                                         int *__restrict__ bcount)
+{
+  __shared__ int ws[kBlock / 64];
+  const long long i0 = ((long long)blockIdx.x * kBlock + threadIdx.x) * kSelPer;
+  unsigned w[4] = {0, 0, 0, 0};
+  if (i0 < n)
+    load_flags16(f, n, i0, w);
+  int c = nonzero_bytes(w[0]) + nonzero_bytes(w[1]) + nonzero_bytes(w[2]) + nonzero_bytes(w[3]);
+  for (int off = 32; off > 0; off >>= 1)
+    c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0)
+    ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    int t = 0;
+    for (int q = 0; q < kBlock / 64; q++)
+      t += ws[q];
+    bcount[blockIdx.x] = t;
+    if (blockIdx.x == 0)
+      bcount[gridDim.x] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_select_write(const unsigned char *__restrict__ f, long long n,
+                                                         const int *__restrict__ boff, int *__restrict__ out,
+                                                         int *__restrict__ num)
+{
+  __shared__ int ws[kBlock / 64];
+  const long long i0 = ((long long)blockIdx.x * kBlock + threadIdx.x) * kSelPer;
+  unsigned w[4] = {0, 0, 0, 0};
+  if (i0 < n)
+    load_flags16(f, n, i0, w);
+  const int c = nonzero_bytes(w[0]) + nonzero_bytes(w[1]) + nonzero_bytes(w[2]) + nonzero_bytes(w[3]);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int incl = c;
+  for (int off = 1; off < 64; off <<= 1)
+  {
+    const int o = __shfl_up(incl, off, 64);
+    if (lane >= off)
+      incl += o;
+  }
+  if (lane == 63)
+    ws[wv] = incl;
+  __syncthreads();
+  int pos = boff[blockIdx.x] + incl - c;
+  for (int q = 0; q < wv; q++)
+    pos += ws[q];
+  for (int q = 0; q < 4; q++)
+    for (int k = 0; k < 4; k++)
+      if ((w[q] >> (8 * k)) & 0xffu)
+        out[pos++] = (int)(i0 + 4 * q + k);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    *num = boff[gridDim.x];
+}
+
 // ------------------------------------------------------------------ head level marks and zones
 // fill_cell_ref for the head (tree_construction.c:527-575, n_exp = 1): a head
 // cell is marked when some cell of its 3x3x3 block inside the box holds at
@@ -419,6 +511,72 @@ __global__ void k_head_dilate(const unsigned char *__restrict__ in, unsigned cha
   if (x < n - 1)
     m |= in[c + st];
   out[c] = m;
+}
+
+// The same marks 16 cells per thread (rows of n >= 16 cells are whole 16-byte
+// vectors): k_head_flag_x writes the flags ORed along x (the count of a cell
+// is start[c + 1] - start[c]), k_head_dilate_yz ORs the nine x-dilated rows at
+// y - 1 .. y + 1, z - 1 .. z + 1 inside the box.
+__global__ void k_head_flag_x(const int *__restrict__ start, long long ncell, int lg, int crit, uint4 *__restrict__ fx)
+{
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long c0 = v * 16;
+  if (c0 >= ncell)
+    return;
+  const int n = 1 << lg, x0 = (int)(c0 & (n - 1));
+  const bool left = x0 > 0, right = x0 + 16 < n;
+  int st[19]; // start[c0 - 1 .. c0 + 17]
+  for (int k = 0; k < 19; k++)
+  {
+    const long long c = c0 - 1 + k;
+    st[k] = ((k == 0 && !left) || (k == 18 && !right)) ? 0 : start[c];
+  }
+  unsigned fl[18]; // flags of cells c0 - 1 .. c0 + 16
+  for (int k = 0; k < 18; k++)
+    fl[k] = st[k + 1] - st[k] >= crit ? 1u : 0u;
+  if (!left)
+    fl[0] = 0;
+  if (!right)
+    fl[17] = 0;
+  unsigned w[4];
+  for (int q = 0; q < 4; q++)
+  {
+    w[q] = 0;
+    for (int k = 0; k < 4; k++)
+    {
+      const int j = 4 * q + k + 1; // cell c0 + 4 q + k
+      w[q] |= (fl[j - 1] | fl[j] | fl[j + 1]) << (8 * k);
+    }
+  }
+  fx[v] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ void k_head_dilate_yz(const uint4 *__restrict__ fx, long long nvec, int lg, uint4 *__restrict__ mark)
+{
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvec)
+    return;
+  const int n = 1 << lg;
+  const long long c0 = v * 16;
+  const int y = (int)((c0 >> lg) & (n - 1)), z = (int)(c0 >> (2 * lg));
+  const long long sy = (long long)n / 16, sz = (long long)n * n / 16;
+  uint4 m = make_uint4(0, 0, 0, 0);
+  for (int dz = -1; dz <= 1; dz++)
+  {
+    if (z + dz < 0 || z + dz >= n)
+      continue;
+    for (int dy = -1; dy <= 1; dy++)
+    {
+      if (y + dy < 0 || y + dy >= n)
+        continue;
+      const uint4 o = fx[v + dy * sy + dz * sz];
+      m.x |= o.x;
+      m.y |= o.y;
+      m.z |= o.z;
+      m.w |= o.w;
+    }
+  }
+  mark[v] = m;
 }
 
 // map[cell] = position of the marked cell in the ordered list
@@ -474,8 +632,11 @@ __device__ inline void uf_union(int *par, int a, int b)
 // starts the zone from; zones are numbered by their roots in cell order.  The
 // joins run in two passes (k_zone_hook_local, k_zone_hook_global), below.
 
-// par[i] = root; rootflag[i] = 1 for roots, rootflag[M] = 0 (for the count)
-__global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag)
+// par[i] = root; rootflag[i] = 1 for roots, rootflag[M] = 0 (for the count).
+// *unsorted (zeroed before) becomes nonzero when some entry's root is smaller
+// than its predecessor's: the zones are then not runs of consecutive entries
+// and need the sort by zone.
+__global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag, int *__restrict__ unsorted)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > M)
@@ -486,6 +647,8 @@ __global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag)
     return;
   }
   const int r = uf_find(par, i);
+  if (i > 0 && r < uf_find(par, i - 1))
+    atomicOr(unsorted, 1);
   par[i] = r;
   rootflag[i] = r == i;
 }

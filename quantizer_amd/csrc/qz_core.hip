@@ -1193,6 +1193,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool has_tree() const { return tree != nullptr && tree->n_nodes > 0; }
   bool tree_adaptation() override;
   bool refine_head_device(int &zone_cells, hipStream_t s);
+  bool zones_in_order(int M, int nz, bool unsorted, hipStream_t s);
   bool refine_level_device(long long c0, long long ncell, long long b0, long long nb, int &zone_cells, hipStream_t s);
   bool tree_state();
   bool build_tree(hipStream_t s);

@@ -12,7 +12,3 @@ hipError_t qz_sort64_pairs(void *tmp, size_t bytes, const unsigned long long *ki
                            const int *vin, int *vout, int n, int end_bit, hipStream_t s);
 size_t qz_scan_temp_bytes(int n);
 hipError_t qz_exclusive_scan(void *tmp, size_t bytes, const int *in, int *out, int n, hipStream_t s);
-// indices i < n with flags[i] != 0, ascending; *num = their count (device)
-size_t qz_select_temp_bytes(int n);
-hipError_t qz_select_flagged_index(void *tmp, size_t bytes, const unsigned char *flags, int *out, int *num, int n,
-                                   hipStream_t s);

@@ -1,4 +1,4 @@
-// Stable radix sorts, scans and index selection on the device (rocPRIM, hipCUB).
+// Stable radix sorts and scans on the device (rocPRIM, hipCUB).
 // Kept in its own translation unit: rocPRIM's radix sort is slow to compile
 // and never changes with the rest of the library.
 #include <hip/hip_runtime.h>
@@ -59,18 +59,4 @@ size_t qz_scan_temp_bytes(int n)
 hipError_t qz_exclusive_scan(void *tmp, size_t bytes, const int *in, int *out, int n, hipStream_t s)
 {
   return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, s);
-}
-
-size_t qz_select_temp_bytes(int n)
-{
-  size_t bytes = 0;
-  (void)hipcub::DeviceSelect::Flagged(nullptr, bytes, hipcub::CountingInputIterator<int>(0),
-                                      (const unsigned char *)nullptr, (int *)nullptr, (int *)nullptr, n);
-  return bytes;
-}
-
-hipError_t qz_select_flagged_index(void *tmp, size_t bytes, const unsigned char *flags, int *out, int *num, int n,
-                                   hipStream_t s)
-{
-  return hipcub::DeviceSelect::Flagged(tmp, bytes, hipcub::CountingInputIterator<int>(0), flags, out, num, n, s);
 }
