@@ -792,7 +792,7 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
     // per cell: the zone cell in, its coordinates, box index and particle range out
     QZ_LAUNCHB("tree_cells", 36.0 * ncell, k_expand_cells, nblk(ncell), d.nodes, d.zone_node, d.zone_xyz, (int)c0,
                (int)ncell,
-              cfg.lmin, D, key_out, N, d.cell_xyz, d.cell_box, d.cstart, d.cend, d.box, d.bstart, d.bend);
+              cfg.lmin, D, key_out, cell_start, d.cell_xyz, d.cell_box, d.cstart, d.cend, d.box, d.bstart, d.bend);
     {
       long long maxcell = 0;
       for (int q = first; q < first + count; q++)

@@ -126,12 +126,14 @@ __global__ void k_init_boxes(const DNode *__restrict__ nodes, int first, int cou
 
 // Child cells (tree_construction.c:1663-1716): cell s of the level is octant
 // (s & 7) of zone cell s >> 3; mark it existing (-3) and find its particle
-// slot range in the sorted octree keys.
+// slot range in the sorted octree keys.  The keys of a cell lie inside its
+// parent cell's range (head: head_start, the head cells' first slots; a node:
+// the slot range of its box entry), so the searches run there.
 __global__ void k_expand_cells(const DNode *__restrict__ nodes, const int *__restrict__ zone_node,
                                const int *__restrict__ zone_xyz, int c0, int ncell, int lmin, int D,
-                               const unsigned long long *__restrict__ keys, int N, int *__restrict__ cell_xyz,
-                               int *__restrict__ cell_box, int *__restrict__ cstart, int *__restrict__ cend,
-                               int *__restrict__ box, int *__restrict__ bstart, int *__restrict__ bend)
+                               const unsigned long long *__restrict__ keys, const int *__restrict__ head_start,
+                               int *__restrict__ cell_xyz, int *__restrict__ cell_box, int *__restrict__ cstart,
+                               int *__restrict__ cend, int *__restrict__ box, int *bstart, int *bend)
 {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ncell)
@@ -147,9 +149,25 @@ __global__ void k_expand_cells(const DNode *__restrict__ nodes, const int *__res
   cell_xyz[3 * cs + 1] = gy;
   cell_xyz[3 * cs + 2] = gz;
   cell_box[cs] = b;
+  int pa, pe; // the parent cell's slot range
+  const int zx = zone_xyz[3 * z], zy = zone_xyz[3 * z + 1], zz = zone_xyz[3 * z + 2];
+  if (d.parent < 0)
+  {
+    const int n = 1 << lmin;
+    const int hc = zx + n * (zy + n * zz);
+    pa = head_start[hc];
+    pe = head_start[hc + 1];
+  }
+  else
+  {
+    const DNode &p = nodes[d.parent];
+    const long long pb = p.boff + (zx - p.ts[0]) + (zy - p.ts[1]) * p.rd[0] + (long long)(zz - p.ts[2]) * p.rd[0] * p.rd[1];
+    pa = bstart[pb];
+    pe = bend[pb];
+  }
   unsigned long long lo, hi;
   cell_key_range(gx, gy, gz, d.lv - lmin, lmin, D, lo, hi);
-  int a = lower_bound_u64(keys, N, lo), e = lower_bound_u64(keys, N, hi);
+  int a = pa + lower_bound_u64(keys + pa, pe - pa, lo), e = pa + lower_bound_u64(keys + pa, pe - pa, hi);
   cstart[cs] = a;
   cend[cs] = e;
   box[d.boff + b] = -3;
