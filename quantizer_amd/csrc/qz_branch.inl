@@ -361,13 +361,15 @@ template <typename P, typename G> bool Engine<P, G>::zones_in_order(int M, int n
   TreeDev &d = *tree->dev;
   if (!unsorted)
   {
-    QZ_LAUNCHB("tree_zone_order", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone_s, d.hidx_s);
+    QZ_LAUNCHB("tree_zone_order", 16.0 * M, k_head_zone_of, nblk(M), M, (const int *)d.hidx, d.hrank, d.hzone_s,
+               d.hidx_s);
     return true;
   }
   int end_bit = 1;
   while ((1LL << end_bit) < nz)
     end_bit++;
-  QZ_LAUNCHB("tree_zone_order", 16.0 * M, k_head_zone_of, nblk(M), M, d.hpar, d.hrank, d.hzone, d.hidx);
+  QZ_LAUNCHB("tree_zone_order", 16.0 * M, k_head_zone_of, nblk(M), M, (const int *)d.hidx, d.hrank, d.hzone,
+             d.hidx);
   timer.begin(stream);
   if (!hip_ok(qz_sort_pairs(d.htmp, d.htmp_bytes, d.hzone, d.hzone_s, d.hidx, d.hidx_s, M, end_bit, stream), "zones"))
     return false;
@@ -473,7 +475,8 @@ template <typename P, typename G> bool Engine<P, G>::refine_head_device(int &zon
     QZ_LAUNCHB("tree_head_hook", 28.0 * M, k_zone_hook_global, nblk(M), g, M, (const int *)d.hidx, d.hpar);
   }
   QZ_LAUNCH("tree_head_zones", k_fill<int>, 1, d.flag_count + 1, 0, (size_t)1);
-  QZ_LAUNCHB("tree_head_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag, d.flag_count + 1);
+  QZ_LAUNCHB("tree_head_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hidx, d.hflag,
+             d.flag_count + 1);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "head zones"))
     return false;
@@ -585,7 +588,7 @@ bool Engine<P, G>::refine_level_device(long long c0, long long ncell, long long 
     QZ_LAUNCHB("tree_level_hook", 28.0 * M, k_zone_hook_global, nblk(M), g, M, (const int *)d.hidx, d.hpar);
   }
   QZ_LAUNCH("tree_level_zones", k_fill<int>, 1, d.flag_count + 1, 0, (size_t)1);
-  QZ_LAUNCHB("tree_level_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hflag,
+  QZ_LAUNCHB("tree_level_compress", 8.0 * (M + 1), k_head_compress, nblk(M + 1), M, d.hpar, d.hidx, d.hflag,
              d.flag_count + 1);
   timer.begin(stream);
   if (!hip_ok(qz_exclusive_scan(d.htmp, d.htmp_bytes, d.hflag, d.hrank, M + 1, stream), "level zones"))

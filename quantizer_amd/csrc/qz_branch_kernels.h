@@ -650,11 +650,16 @@ __device__ inline void uf_union(int *par, int a, int b)
 // starts the zone from; zones are numbered by their roots in cell order.  The
 // joins run in two passes (k_zone_hook_local, k_zone_hook_global), below.
 
-// par[i] = root; rootflag[i] = 1 for roots, rootflag[M] = 0 (for the count).
+// root[i] = the root of entry i; rootflag[i] = 1 for roots, rootflag[M] = 0
+// (for the count).  The roots go to their own array: a find's path halving in
+// one thread can store a non-root ancestor into par[i] after thread i stored
+// the root there, so par is not read for results after the joins; every find
+// still returns the true root (parents only ever move to ancestors).
 // *unsorted (zeroed before) becomes nonzero when some entry's root is smaller
 // than its predecessor's: the zones are then not runs of consecutive entries
 // and need the sort by zone.
-__global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag, int *__restrict__ unsorted)
+__global__ void k_head_compress(int M, int *par, int *__restrict__ root, int *__restrict__ rootflag,
+                                int *__restrict__ unsorted)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > M)
@@ -667,17 +672,20 @@ __global__ void k_head_compress(int M, int *par, int *__restrict__ rootflag, int
   const int r = uf_find(par, i);
   if (i > 0 && r < uf_find(par, i - 1))
     atomicOr(unsorted, 1);
-  par[i] = r;
+  root[i] = r;
   rootflag[i] = r == i;
 }
 
-__global__ void k_head_zone_of(int M, const int *__restrict__ par, const int *__restrict__ rank,
-                               unsigned *__restrict__ zone, int *__restrict__ idx)
+// zone[i] = rank of entry i's root; idx[i] = i.  idx may be the root array
+// (each thread reads its root before it writes its index).
+__global__ void k_head_zone_of(int M, const int *root, const int *__restrict__ rank, unsigned *__restrict__ zone,
+                               int *idx)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M)
     return;
-  zone[i] = (unsigned)rank[par[i]];
+  const int r = root[i];
+  zone[i] = (unsigned)rank[r];
   idx[i] = i;
 }
 
