@@ -53,7 +53,8 @@ typedef struct
   double tol;             /* RMS residual / (4 pi G rho_mean) threshold (global_variables.c:318) */
   double tol_change;      /* pointwise |phi/phi_old - 1| threshold (global_variables.c:319) */
   int max_iterations;     /* V-cycle / CG iteration cap (global_variables.c:313) */
-  int inner_cycles;       /* head: V-cycles per outer FP64 refinement iteration (0 = default, 2) */
+  int inner_cycles;       /* head: V-cycles per outer FP64 refinement iteration (0 = chosen per iteration
+                             from its residual, 1 to 4) */
   int pre_sweeps;         /* red-black GS sweeps before restriction */
   int post_sweeps;        /* red-black GS sweeps after prolongation */
   int branch_solver;      /* QZ_BRANCH_* solver of the refinement-node boxes */

@@ -21,7 +21,8 @@ namespace qz
 
 template <typename P, typename G> void Engine<P, G>::release_adjoint()
 {
-  void *p1[] = {adj_xb,     adj_vb,     adj_ab,     adj_abA,    adj_psi,    adj_slot,   adj_mid,
+  void *p1[] = {adj_xb,     adj_vb,     adj_ab,     adj_abA,    adj_psi,    adj_abP,    adj_m,      adj_m2,
+                adj_xb2,    adj_vb2,
                 adj_gbar[0], adj_gbar[1], adj_gbar[2], adj_rhs,    adj_nab[0], adj_nab[1], adj_nab[2],
                 adj_npb,    adj_npsi,   adj_hpb,    adj_nrhs,   adj_kind,   adj_rkey[0], adj_rkey[1],
                 adj_rval[0], adj_rval[1], adj_bpre,  adj_rsrc,   adj_rtmp,   adj_psi_prev};
@@ -29,7 +30,8 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
     (void)hipFree(q);
   adj_xb = adj_vb = adj_ab = adj_abA = adj_psi = adj_psi_prev = adj_npb = adj_npsi = adj_hpb = nullptr;
   adj_hist = 0;
-  adj_slot = adj_mid = nullptr;
+  adj_abP = adj_m = adj_m2 = nullptr;
+  adj_xb2 = adj_vb2 = nullptr;
   adj_gbar[0] = adj_gbar[1] = adj_gbar[2] = adj_rhs = adj_nrhs = nullptr;
   adj_nab[0] = adj_nab[1] = adj_nab[2] = nullptr;
   adj_kind = nullptr;
@@ -54,6 +56,7 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
     (void)hipFree(r.id);
     (void)hipFree(r.key);
     (void)hipFree(r.cstart);
+    (void)hipFree(r.perm);
     (void)hipFree(r.refl);
     free_tree_snap(r.tsnap);
   }
@@ -66,10 +69,17 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
     return true;
   size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
   size_t b = 0;
-  adj_mid = dmalloc<P>(N, b);
+  adj_m = dmalloc<P>(N, b);
+  adj_m2 = dmalloc<P>(N, b);
   if (cfg.lmin < cfg.lmax)
     adj_hpb = dmalloc<double>(npts, b);
-  if (!adj_mid || (cfg.lmin < cfg.lmax && !adj_hpb))
+  if (!adj_m || !adj_m2 || (cfg.lmin < cfg.lmax && !adj_hpb))
+    return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
+  adj_xb2 = dmalloc<double>(3 * (size_t)N, b);
+  adj_vb2 = dmalloc<double>(3 * (size_t)N, b);
+  if (!std::is_same<P, double>::value)
+    adj_abP = dmalloc<P>(3 * (size_t)N, b);
+  if (!adj_xb2 || !adj_vb2 || (!std::is_same<P, double>::value && !adj_abP))
     return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
   adj_xb = dmalloc<double>(3 * (size_t)N, b);
   adj_vb = dmalloc<double>(3 * (size_t)N, b);
@@ -77,11 +87,10 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
   adj_abA = dmalloc<double>(3 * (size_t)N, b);
   adj_psi = dmalloc<double>(npts, b);
   adj_psi_prev = dmalloc<double>(npts, b);
-  adj_slot = dmalloc<P>(N, b);
   for (int a = 0; a < 3; a++)
     adj_gbar[a] = dmalloc<G>(npts, b);
   adj_rhs = dmalloc<G>(npts, b);
-  if (!adj_xb || !adj_vb || !adj_ab || !adj_abA || !adj_psi || !adj_psi_prev || !adj_slot || !adj_gbar[0] ||
+  if (!adj_xb || !adj_vb || !adj_ab || !adj_abA || !adj_psi || !adj_psi_prev || !adj_gbar[0] ||
       !adj_gbar[1] || !adj_gbar[2] || !adj_rhs)
     return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
   bytes += b;
@@ -117,20 +126,35 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_head_solve()
   return true;
 }
 
-// adj_xb (ID order) += J_A(x)^T ab at the current sorted positions, with the
-// grid accelerations of A(x) in gax/gay/gaz and the head cell ranges current.
+// abar (3 N, slot order) as the deposit's mass type: the array itself when P
+// is double, else a converted copy.
+template <typename P, typename G> const P *Engine<P, G>::abar_as_mass(const double *ab)
+{
+  if constexpr (std::is_same<P, double>::value)
+    return ab;
+  else
+  {
+    const size_t n3 = 3 * (size_t)N;
+    QZ_LAUNCHB("adj_abar_mass", (8.0 + sizeof(P)) * n3, k_to_mass_type<P>, nblk(n3), ab, adj_abP, n3);
+    return adj_abP;
+  }
+}
+
+// adj_xb += J_A(x)^T ab at the current sorted positions, both in the current
+// slot order (the masses adj_m too), with the grid accelerations of A(x) in
+// gax/gay/gaz and the head cell ranges current.
 template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double *ab)
 {
   if (has_tree())
     return vjp_tree(ab);
   size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
   double H = 1.0 / n;
+  const P *abm = abar_as_mass(ab);
   // Gbar = scatter of abar (the deposit with abar as the mass, coefficient 1)
   for (int a = 0; a < 3; a++)
   {
-    QZ_LAUNCHB("adj_slot", 12.0 * N, k_by_id_to_slot<P>, nblk(N), ab + (size_t)a * N, id_a, adj_slot, N);
     QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
-               pos[1], pos[2], adj_slot, N, cfg.lmin, 3 * D, n - 1, tails);
+               pos[1], pos[2], abm + (size_t)a * N, N, cfg.lmin, 3 * D, n - 1, tails);
     cell_totals(cell_start, nullptr, (long long)n * n * n);
     QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>),
                nblk(npts), cell_start, tails, adj_gbar[a], make_cube(n + 1), 1.0);
@@ -141,7 +165,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
     return false;
   const double coeff = 4.0 * cfg.G * kPi / (H * H * H);
   QZ_LAUNCHB("adj_gather", (24.0 + sizeof(P) + 4.0 + 48.0) * N + (8.0 + 3.0 * sizeof(G)) * npts,
-             (k_adjoint_gather<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi, gax, gay, gaz, ab,
+             (k_adjoint_gather<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_m, adj_psi, gax, gay, gaz, ab,
              ab + N, ab + 2 * (size_t)N, adj_xb, adj_xb + N, adj_xb + 2 * (size_t)N, N, n, coeff);
   return true;
 }
@@ -215,8 +239,8 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint_tree()
   return true;
 }
 
-// adj_xb (ID order) += J_A(x)^T ab with refinement levels, after a force pass
-// at the current sorted positions (tree, node and head fields current).
+// adj_xb += J_A(x)^T ab with refinement levels, after a force pass at the
+// current sorted positions (tree, node and head fields current); slot order.
 template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
 {
   if (!ensure_adjoint_tree())
@@ -236,11 +260,12 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
     QZ_LAUNCH("adj_kinds", k_adj_kinds, grid, d.nodes, d.lfirst[l], d.sim_g, d.bdry_g, d.intr_g, adj_kind);
   }
   // Abar: scatter of abar over the unrefined cells of the head and of every level
+  const P *abm = abar_as_mass(ab);
   for (int a = 0; a < 3; a++)
   {
-    QZ_LAUNCHB("adj_slot", 12.0 * N, k_by_id_to_slot<P>, nblk(N), ab + (size_t)a * N, id_a, adj_slot, N);
+    const P *abma = abm + (size_t)a * N;
     QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
-               pos[1], pos[2], adj_slot, N, cfg.lmin, 3 * D, n - 1, tails);
+               pos[1], pos[2], abma, N, cfg.lmin, 3 * D, n - 1, tails);
     cell_totals(cell_start, nullptr, (long long)n * n * n);
     QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_adj_head_scatter<G, G>),
                nblk(npts), cell_start, d.head_child, tails, adj_gbar[a], cb);
@@ -248,7 +273,7 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
     {
       const int lv = cfg.lmin + l;
       QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
-                 pos[1], pos[2], adj_slot, N, lv, 3 * (D - l), (1 << lv) - 1, tails);
+                 pos[1], pos[2], abma, N, lv, 3 * (D - l), (1 << lv) - 1, tails);
       cell_totals(d.cstart + d.lc0[l], d.cend + d.lc0[l], d.lncell[l]);
       dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
       QZ_LAUNCH("adj_scatter_branch", (k_adj_branch_scatter<G>), grid, d.nodes, d.lfirst[l], d.bdry_g, d.bdry_xyz,
@@ -315,35 +340,46 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
   QZ_LAUNCH("adj_head_add", k_adj_head_add<G>, nblk(npts), adj_rhs, adj_hpb, cb);
   if (!adjoint_head_solve())
     return false;
-  QZ_LAUNCH("adj_gather_amr", (k_adjoint_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_mid, id_a, adj_psi,
+  QZ_LAUNCH("adj_gather_amr", (k_adjoint_gather_amr<P, G>), nblk(N), pos[0], pos[1], pos[2], adj_m, adj_psi,
             gax, gay, gaz, d.head_child, d.nodes, d.box, adj_npsi, (const G *)d.ax, (const G *)d.ay, (const G *)d.az,
             ab, ab + N, ab + 2 * (size_t)N, adj_xb, adj_xb + N, adj_xb + 2 * (size_t)N, N, cfg.lmin,
             4.0 * cfg.G * kPi);
   return hip_ok(hipGetLastError(), "adjoint");
 }
 
-// adj_mid[id] = the masses (ID order), from the current slot order.
-template <typename P, typename G> bool Engine<P, G>::masses_by_id()
+// Three cotangent components (stride N) between ID order and the current slot
+// order of the particles: out[s] = in[id[s]] (to_slots) or out[id[s]] = in[s].
+template <typename P, typename G> bool Engine<P, G>::slot_cotangents(const double *in, double *out, bool to_slots)
 {
-  QZ_LAUNCH("adj_mass", k_slot_to_id<P>, nblk(N), pa[9], id_a, adj_mid, N);
-  return true;
+  for (int a = 0; a < 3; a++)
+  {
+    const size_t o = (size_t)a * N;
+    if (to_slots)
+      QZ_LAUNCHB("adj_order", 20.0 * N, k_id_to_slot<double>, nblk(N), in + o, id_a, out + o, N);
+    else
+      QZ_LAUNCHB("adj_order", 20.0 * N, k_slot_to_id<double>, nblk(N), in + o, id_a, out + o, N);
+  }
+  return hip_ok(hipGetLastError(), "adjoint cotangent order");
 }
 
 template <typename P, typename G> bool Engine<P, G>::force_vjp(const double *abar, double *xbar)
 {
   if (!ensure_adjoint())
     return false;
-  // force pass at the current positions (tree, grid accelerations, cell ranges)
-  if (!masses_by_id() || !force_fields())
+  // force pass at the current positions (tree, grid accelerations, cell ranges);
+  // it leaves the particles sorted, the masses with them
+  if (!force_fields())
     return false;
   size_t n3 = 3 * (size_t)N;
-  if (!hip_ok(hipMemcpy(adj_ab, abar, sizeof(double) * n3, hipMemcpyDefault), "upload abar"))
+  if (!hip_ok(hipMemcpyAsync(adj_m, pa[9], sizeof(P) * N, hipMemcpyDeviceToDevice, stream), "adjoint masses") ||
+      !hip_ok(hipMemcpy(adj_abA, abar, sizeof(double) * n3, hipMemcpyDefault), "upload abar") ||
+      !slot_cotangents(adj_abA, adj_ab, true))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_xb, 0.0, n3);
   adj_sweep = adj_warm = false;
-  if (!vjp_at_current(adj_ab))
+  if (!vjp_at_current(adj_ab) || !slot_cotangents(adj_xb, adj_abA, false))
     return false;
-  return hip_ok(hipMemcpy(xbar, adj_xb, sizeof(double) * n3, hipMemcpyDefault), "download xbar");
+  return hip_ok(hipMemcpy(xbar, adj_abA, sizeof(double) * n3, hipMemcpyDefault), "download xbar");
 }
 
 // A checkpoint: the particle state in its sorted order plus the head potential
@@ -433,9 +469,10 @@ template <typename P, typename G> bool Engine<P, G>::ensure_recs()
     r.id = dmalloc<int>(N, bb);
     r.key = dmalloc<unsigned long long>(N, bb);
     r.cstart = dmalloc<int>(ncell + 1, bb);
+    r.perm = dmalloc<int>(N, bb);
     r.refl = dmalloc<unsigned char>(N, bb);
-    const bool ok =
-        r.x[0] && r.x[1] && r.x[2] && r.id && r.refl && r.g[0] && r.g[1] && r.g[2] && r.key && r.cstart;
+    const bool ok = r.x[0] && r.x[1] && r.x[2] && r.id && r.refl && r.g[0] && r.g[1] && r.g[2] && r.key &&
+                    r.cstart && r.perm;
     recs.push_back(r);
     if (!ok)
       return hip_ok(hipErrorOutOfMemory, "adjoint step records");
@@ -461,6 +498,7 @@ template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r, bool h
   {
     std::swap(r.key, key_out);
     std::swap(r.cstart, cell_start);
+    std::swap(r.perm, slot_out); // the sort's output, rewritten by the next sort
     for (int a = 0; a < 3; a++)
       std::swap(r.g[a], *gs[a]);
   }
@@ -468,6 +506,7 @@ template <typename P, typename G> bool Engine<P, G>::save_rec(StepRec &r, bool h
   {
     (void)hipMemcpyAsync(r.key, key_out, sizeof(unsigned long long) * N, hipMemcpyDeviceToDevice, stream);
     (void)hipMemcpyAsync(r.cstart, cell_start, sizeof(int) * (ncell + 1), hipMemcpyDeviceToDevice, stream);
+    (void)hipMemcpyAsync(r.perm, slot_out, sizeof(int) * N, hipMemcpyDeviceToDevice, stream);
     for (int a = 0; a < 3; a++)
       (void)hipMemcpyAsync(r.g[a], *gs[a], sizeof(G) * npts, hipMemcpyDeviceToDevice, stream);
   }
@@ -563,8 +602,6 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   }
   const size_t n3 = 3 * (size_t)N;
   const double dt = adj_dt, h = 0.5 * dt;
-  if (!masses_by_id())
-    return false;
   // keep the final state to restore it afterwards (the flags of the sweep are
   // cleared on every return)
   Ckpt &fin = adj_fin;
@@ -579,9 +616,14 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
   adj_warm = false;
   if (!ensure_recs())
     return false;
-  // the cotangents may be host or device memory (the copies infer the direction)
-  if (!hip_ok(hipMemcpy(adj_xb, xbN, sizeof(double) * n3, hipMemcpyDefault), "upload xbar") ||
-      !hip_ok(hipMemcpy(adj_vb, vbN, sizeof(double) * n3, hipMemcpyDefault), "upload vbar"))
+  // the cotangents may be host or device memory (the copies infer the
+  // direction); they arrive in ID order and are kept in the slot order of the
+  // state they belong to, which is the particles' current order at the final
+  // state (the masses likewise)
+  if (!hip_ok(hipMemcpy(adj_ab, xbN, sizeof(double) * n3, hipMemcpyDefault), "upload xbar") ||
+      !hip_ok(hipMemcpy(adj_abA, vbN, sizeof(double) * n3, hipMemcpyDefault), "upload vbar") ||
+      !slot_cotangents(adj_ab, adj_xb, true) || !slot_cotangents(adj_abA, adj_vb, true) ||
+      !hip_ok(hipMemcpyAsync(adj_m, pa[9], sizeof(P) * N, hipMemcpyDeviceToDevice, stream), "adjoint masses"))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_abA, 0.0, n3);
   for (int sc = adj_nckpt - 1; sc >= 0; sc--)
@@ -625,14 +667,21 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       HostSpan span_vjp(timer, "adj_vjp");
       if (!vjp_at_current(adj_ab))
         return false;
+      // to the slot order before this step's sort, where its drift ran
+      QZ_LAUNCHB("adj_unsort", (2.0 * (48.0 + sizeof(P)) + 4.0) * N, k_adj_unsort<P>, nblk(N), recs[j - c0].perm,
+                 adj_xb, adj_vb, adj_m, adj_xb2, adj_vb2, adj_m2, N);
+      std::swap(adj_xb, adj_xb2);
+      std::swap(adj_vb, adj_vb2);
+      std::swap(adj_m, adj_m2);
       QZ_LAUNCH("adj_kick_drift", k_adj_kick_drift, nblk(N), adj_xb, adj_vb, adj_abA, recs[j - c0].refl, N, dt);
     }
   }
-  // a_0 = A(x_0) fed the first half kick
-  if (!load_rec(recs[0]) || !vjp_at_current(adj_abA))
+  // a_0 = A(x_0) fed the first half kick; then back to ID order
+  if (!load_rec(recs[0]) || !vjp_at_current(adj_abA) || !slot_cotangents(adj_xb, adj_ab, false) ||
+      !slot_cotangents(adj_vb, adj_abA, false))
     return false;
-  if (!hip_ok(hipMemcpy(xb0, adj_xb, sizeof(double) * n3, hipMemcpyDefault), "download xbar") ||
-      !hip_ok(hipMemcpy(vb0, adj_vb, sizeof(double) * n3, hipMemcpyDefault), "download vbar"))
+  if (!hip_ok(hipMemcpy(xb0, adj_ab, sizeof(double) * n3, hipMemcpyDefault), "download xbar") ||
+      !hip_ok(hipMemcpy(vb0, adj_abA, sizeof(double) * n3, hipMemcpyDefault), "download vbar"))
     return false;
   // back to the final forward state: its fields (and tree) are rebuilt, then the
   // state is restored once more for the exact accelerations, potential and

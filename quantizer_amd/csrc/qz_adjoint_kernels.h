@@ -19,16 +19,6 @@
 namespace qz
 {
 
-// dst[p] = src[id[p]] (ID order -> sorted slot order)
-template <typename P>
-__global__ void k_by_id_to_slot(const double *__restrict__ src, const int *__restrict__ id, P *__restrict__ dst,
-                                int N)
-{
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < N)
-    dst[p] = (P)src[id[p]];
-}
-
 // dst[id[p]] = src[p] (sorted slot order -> ID order) and back
 template <typename T>
 __global__ void k_slot_to_id(const T *__restrict__ src, const int *__restrict__ id, T *__restrict__ dst, int N)
@@ -44,6 +34,44 @@ __global__ void k_id_to_slot(const T *__restrict__ src, const int *__restrict__ 
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < N)
     dst[p] = src[id[p]];
+}
+
+// dst[i] = (P) src[i] over n entries (a cotangent as the deposit's mass type)
+template <typename P>
+__global__ void k_to_mass_type(const double *__restrict__ src, P *__restrict__ dst, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    dst[i] = (P)src[i];
+}
+
+// The cotangents of positions and velocities (3 components each, stride N)
+// and the masses, from the slot order of a step's force pass to the slot order
+// before that step's sort: the sort put the particle of old slot perm[s] at
+// slot s, so dst[perm[s]] = src[s].  The permutation is mostly local (a
+// particle moves less than a cell per step), so the stores stay near-coalesced.
+template <typename P>
+__global__ void k_adj_unsort(const int *__restrict__ perm, const double *__restrict__ xb,
+                             const double *__restrict__ vb, const P *__restrict__ m, double *__restrict__ xb_out,
+                             double *__restrict__ vb_out, P *__restrict__ m_out, int N)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N)
+    return;
+  const int d = perm[s];
+  double x[3], v[3];
+  for (int a = 0; a < 3; a++)
+  {
+    x[a] = xb[(size_t)a * N + s];
+    v[a] = vb[(size_t)a * N + s];
+  }
+  const P mm = m[s];
+  for (int a = 0; a < 3; a++)
+  {
+    xb_out[(size_t)a * N + d] = x[a];
+    vb_out[(size_t)a * N + d] = v[a];
+  }
+  m_out[d] = mm;
 }
 
 // Transpose of the interior central difference: phibar[q] = sum_a (Gbar_a[q + e_a] - Gbar_a[q - e_a]) / 2H,
@@ -80,11 +108,11 @@ __global__ void k_grad_transpose(const G *__restrict__ gx, const G *__restrict__
   out[t] = (G)(v * inv_2h);
 }
 
-// xbar_i += sum_g grad W(x_i, g) (coeff m_i psi_g + abar_i . G_g); m, abar and xbar in ID order.
+// xbar_i += sum_g grad W(x_i, g) (coeff m_i psi_g + abar_i . G_g); m, abar and xbar in the
+// slot order of the positions.
 template <typename P, typename G>
 __global__ void k_adjoint_gather(const double *__restrict__ x, const double *__restrict__ y,
-                                 const double *__restrict__ z, const P *__restrict__ m, const int *__restrict__ id,
-                                 const double *__restrict__ psi, const G *__restrict__ gax, const G *__restrict__ gay,
+                                 const double *__restrict__ z, const P *__restrict__ m, const double *__restrict__ psi, const G *__restrict__ gax, const G *__restrict__ gay,
                                  const G *__restrict__ gaz, const double *__restrict__ abx,
                                  const double *__restrict__ aby, const double *__restrict__ abz,
                                  double *__restrict__ xbx, double *__restrict__ xby, double *__restrict__ xbz, int N,
@@ -100,7 +128,7 @@ __global__ void k_adjoint_gather(const double *__restrict__ x, const double *__r
   double tx = pxs - fx, ty = pys - fy, tz = pzs - fz;
   double wx[2] = {1.0 - tx, tx}, wy[2] = {1.0 - ty, ty}, wz[2] = {1.0 - tz, tz};
   double dw[2] = {-s, s};
-  int q = id[p];
+  const int q = p;
   double ab[3] = {abx[q], aby[q], abz[q]};
   double cm = coeff * (double)m[q];
   double gxs = 0, gys = 0, gzs = 0;
@@ -467,10 +495,10 @@ __device__ inline CicGrad cic_grad(double x, double y, double z, int lv, int cla
 }
 
 // xbar_i += sum over the nodes holding i (head included) of grad W (c_l m psi)
-// plus grad W (abar . a) at the finest one; m, abar and xbar in ID order.
+// plus grad W (abar . a) at the finest one; m, abar and xbar in the slot order of the positions.
 template <typename P, typename G>
 __global__ void k_adjoint_gather_amr(const double *__restrict__ x, const double *__restrict__ y,
-                                     const double *__restrict__ z, const P *__restrict__ m, const int *__restrict__ id,
+                                     const double *__restrict__ z, const P *__restrict__ m,
                                      const double *__restrict__ hpsi, const G *__restrict__ hax,
                                      const G *__restrict__ hay, const G *__restrict__ haz,
                                      const int *__restrict__ head_child, const DNode *__restrict__ nodes,
@@ -485,7 +513,7 @@ __global__ void k_adjoint_gather_amr(const double *__restrict__ x, const double 
     return;
   const int n = 1 << lmin;
   const double px = x[p], py = y[p], pz = z[p];
-  const int q = id[p];
+  const int q = p;
   const double ab[3] = {abx[q], aby[q], abz[q]};
   const double mm = (double)m[q];
   double gxs = 0.0, gys = 0.0, gzs = 0.0;

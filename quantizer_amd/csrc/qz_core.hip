@@ -206,14 +206,20 @@ template <typename P, typename G> struct Engine final : EngineBase
   int phi_hist = 0;           // converged head potentials since the last reset of the history
   // captured outer iteration of the head solve, per (phi, rhs) pair: [0] the
   // potential, [1] the adjoint solve
-  hipGraphExec_t head_graph[2] = {nullptr, nullptr};
-  int head_graph_inner[2] = {-1, -1};
-  unsigned char *refl_rec = nullptr; // when set, update_A records reflected axes (ID order)
+  // and each V-cycle count of an outer iteration (1 .. kMaxInner)
+  static constexpr int kMaxInner = 4;
+  hipGraphExec_t head_graph[2][kMaxInner] = {};
+  unsigned char *refl_rec = nullptr; // when set, update_A records reflected axes (its slot order)
   // adjoint state (qz_adjoint.inl)
   double *adj_xb = nullptr, *adj_vb = nullptr, *adj_ab = nullptr, *adj_abA = nullptr, *adj_psi = nullptr;
   double *adj_psi_prev = nullptr; // the previous adjoint head solution of a reverse sweep
   int adj_hist = 0;               // adjoint head solutions of this reverse sweep (up to 2)
-  P *adj_slot = nullptr, *adj_mid = nullptr; // a cotangent in slot order; the masses in ID order
+  // The cotangents (adj_xb, adj_vb, adj_ab, adj_abA) and the masses adj_m are
+  // kept in the slot order of the state they belong to; adj_xb2, adj_vb2 and
+  // adj_m2 receive them in the previous step's order.  adj_abP: abar as the
+  // deposit's mass type (unused when P is double).
+  double *adj_xb2 = nullptr, *adj_vb2 = nullptr;
+  P *adj_abP = nullptr, *adj_m = nullptr, *adj_m2 = nullptr;
   G *adj_gbar[3] = {nullptr, nullptr, nullptr}, *adj_rhs = nullptr;
   // refinement levels, in the grid-pool layout: Abar (3), phibar and psi in
   // FP64, the node solves' right-hand side, point kinds; the head phibar sent up
@@ -243,6 +249,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     int *id;
     unsigned long long *key;
     int *cstart;
+    int *perm; // the step's sort: slot s holds the particle of slot perm[s] before it
     G *g[3];
     unsigned char *refl;
     TreeSnap *tsnap; // with refinement levels: the tree and node fields of the step
@@ -374,8 +381,9 @@ template <typename P, typename G> struct Engine final : EngineBase
     if (hred)
       (void)hipHostFree(hred);
     for (int q = 0; q < 2; q++)
-      if (head_graph[q])
-        (void)hipGraphExecDestroy(head_graph[q]);
+      for (int c = 0; c < kMaxInner; c++)
+        if (head_graph[q][c])
+          (void)hipGraphExecDestroy(head_graph[q][c]);
     join_ahead();
     release_adjoint();
     release_tree();
@@ -550,8 +558,8 @@ template <typename P, typename G> struct Engine final : EngineBase
     const bool narrow = end_bit <= 32;
     void *key = narrow ? (void *)key_out : (void *)key_in;
     QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P) + (narrow ? 8.0 : 12.0)) * N, k_update_A<P>, nblk(N), pos[0],
-               pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, dt, refl_rec, refl_rec ? id_a : nullptr,
-               key, slot_in, cfg.lmin, D, narrow ? 1 : 0);
+               pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, dt, refl_rec, key,
+               slot_in, cfg.lmin, D, narrow ? 1 : 0);
     keys_valid = false;
     keys_fresh = true;
     particles_changed();
@@ -782,29 +790,46 @@ template <typename P, typename G> struct Engine final : EngineBase
       head_outer_body(ph, rh, inner); // eager launches (so that each kernel is timed)
       return true;
     }
-    if (!head_graph[slot] || head_graph_inner[slot] != inner)
+    hipGraphExec_t &ge = head_graph[slot][inner - 1];
+    if (!ge)
     {
-      if (head_graph[slot])
-      {
-        (void)hipGraphExecDestroy(head_graph[slot]);
-        head_graph[slot] = nullptr;
-      }
       hipGraph_t g = nullptr;
       if (!hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "graph capture"))
         return false;
       head_outer_body(ph, rh, inner);
       const bool made = hip_ok(hipStreamEndCapture(stream, &g), "graph capture") &&
-                        hip_ok(hipGraphInstantiate(&head_graph[slot], g, nullptr, nullptr, 0), "graph instantiate");
+                        hip_ok(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
       if (g)
         (void)hipGraphDestroy(g);
       if (!made)
       {
-        head_graph[slot] = nullptr;
+        ge = nullptr;
         return false;
       }
-      head_graph_inner[slot] = inner;
     }
-    return hip_ok(hipGraphLaunch(head_graph[slot], stream), "graph launch");
+    return hip_ok(hipGraphLaunch(ge, stream), "graph launch");
+  }
+
+  // V-cycles of the next outer iteration (cfg.inner_cycles when set).  From the
+  // relative residual res of the last one: once it is below tol, one V-cycle,
+  // whose correction the pointwise change test measures (the potential solve
+  // needs that pass; the adjoint solve has then converged); otherwise enough
+  // V-cycles, at an assumed reduction of kVcycleRate each, to bring it to
+  // kVerifyRes (the potential: where the change over one more V-cycle passes the
+  // test) or below tol (the adjoint), and at most kMaxInner, which is where the
+  // FP32 correction of the mixed path stops gaining.
+  int next_inner(double res, bool adjoint) const
+  {
+    if (cfg.inner_cycles > 0)
+      return std::min(cfg.inner_cycles, kMaxInner);
+    if (res < cfg.tol)
+      return 1;
+    constexpr double kVcycleRate = 0.05, kVerifyRes = 1e-9;
+    const double target = adjoint ? 0.5 * cfg.tol : kVerifyRes;
+    if (!(res > target))
+      return 1;
+    const int c = (int)std::ceil(std::log(target / res) / std::log(kVcycleRate));
+    return std::max(1, std::min(c, kMaxInner));
   }
 
   // Head solve: FP64 outer residual/correction around inner V-cycles in G.
@@ -822,8 +847,6 @@ template <typename P, typename G> struct Engine final : EngineBase
     double inv_h2 = 1.0 / (H * H);
     double rhomean = 4.0 * cfg.G * kPi * total_mass / ((double)n * n * n * H * H * H);
     double ninterior = (double)(n - 1) * (n - 1) * (n - 1);
-    int inner = cfg.inner_cycles;
-    int max_outer = std::max(1, cfg.max_iterations / std::max(1, inner));
     unsigned nb = nred(npts);
     int outers = 0, vcycles = 0;
     double change = 1e300, res = 0.0, scale = rhomean;
@@ -859,8 +882,9 @@ template <typename P, typename G> struct Engine final : EngineBase
         ok = true; // change flag values > 1 mark points outside [1-tol, 1+tol]
         break;
       }
-      if (outer == max_outer)
+      if (vcycles >= cfg.max_iterations)
         break;
+      const int inner = next_inner(res, adjoint);
       if (!launch_head_body(ph, rh, inner, adjoint ? 1 : 0))
         return false;
       vcycles += inner;
@@ -1173,7 +1197,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool ensure_adjoint_tree();
   bool force_fields();
   bool vjp_tree(const double *ab);
-  bool masses_by_id();
+  bool slot_cotangents(const double *in, double *out, bool to_slots);
   bool save_ckpt(Ckpt &c);
   void restore_ckpt(const Ckpt &c);
   void free_ckpt(Ckpt &c);
@@ -1184,7 +1208,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool save_rec(StepRec &r, bool handover = false);
   bool load_rec(StepRec &r);
   bool adjoint_head_solve();
-  bool vjp_at_current(const double *ab); // xbar (adj_xb, ID order) += J_A(x)^T ab
+  const P *abar_as_mass(const double *ab);
+  bool vjp_at_current(const double *ab); // xbar (adj_xb) += J_A(x)^T ab, both in the current slot order
   bool force_vjp(const double *abar, double *xbar) override;
   bool adjoint_forward(int nsteps, double dt, int every) override;
   bool adjoint_backward(const double *xbN, const double *vbN, double *xb0, double *vb0) override;
@@ -1355,8 +1380,8 @@ qz_ctx *qz_create(const qz_config *cfg)
     return nullptr;
   }
   qz_config c = *cfg;
-  if (c.inner_cycles <= 0)
-    c.inner_cycles = 2; // two V-cycles per FP64 residual/correction: half the outer passes, same V-cycle count
+  if (c.inner_cycles < 0)
+    c.inner_cycles = 0; // 0: V-cycles per outer iteration chosen from its residual
   if (c.pre_sweeps <= 0)
     c.pre_sweeps = 2;
   if (c.post_sweeps <= 0)

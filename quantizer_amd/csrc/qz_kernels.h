@@ -523,8 +523,7 @@ template <typename P>
 __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, P *__restrict__ vx,
                            P *__restrict__ vy, P *__restrict__ vz, P *__restrict__ ax, P *__restrict__ ay,
                            P *__restrict__ az, int N, double dt, unsigned char *__restrict__ refl,
-                           const int *__restrict__ id, void *__restrict__ key, int *__restrict__ slot, int lmin, int D,
-                           int narrow)
+                           void *__restrict__ key, int *__restrict__ slot, int lmin, int D, int narrow)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N)
@@ -549,7 +548,7 @@ __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, doubl
     }
   }
   if (refl)
-    refl[id[i]] = flags; // reflected axes, for the adjoint of this step
+    refl[i] = flags; // reflected axes in this slot order, for the adjoint of this step
   x[i] = pos[0];
   y[i] = pos[1];
   z[i] = pos[2];
