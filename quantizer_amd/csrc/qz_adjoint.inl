@@ -22,7 +22,7 @@ namespace qz
 template <typename P, typename G> void Engine<P, G>::release_adjoint()
 {
   void *p1[] = {adj_xb,     adj_vb,     adj_ab,     adj_abA,    adj_psi,    adj_abP,    adj_m,      adj_m2,
-                adj_xb2,    adj_vb2,
+                adj_xb2,    adj_vb2,    adj_tails[0], adj_tails[1],
                 adj_gbar[0], adj_gbar[1], adj_gbar[2], adj_rhs,    adj_nab[0], adj_nab[1], adj_nab[2],
                 adj_npb,    adj_npsi,   adj_hpb,    adj_nrhs,   adj_kind,   adj_rkey[0], adj_rkey[1],
                 adj_rval[0], adj_rval[1], adj_bpre,  adj_rsrc,   adj_rtmp,   adj_psi_prev};
@@ -32,6 +32,7 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
   adj_hist = 0;
   adj_abP = adj_m = adj_m2 = nullptr;
   adj_xb2 = adj_vb2 = nullptr;
+  adj_tails[0] = adj_tails[1] = nullptr;
   adj_gbar[0] = adj_gbar[1] = adj_gbar[2] = adj_rhs = adj_nrhs = nullptr;
   adj_nab[0] = adj_nab[1] = adj_nab[2] = nullptr;
   adj_kind = nullptr;
@@ -77,6 +78,9 @@ template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
     return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
   adj_xb2 = dmalloc<double>(3 * (size_t)N, b);
   adj_vb2 = dmalloc<double>(3 * (size_t)N, b);
+  for (int a = 0; a < 2; a++)
+    if (!(adj_tails[a] = dmalloc<G>(8 * (size_t)N, b)))
+      return hip_ok(hipErrorOutOfMemory, "adjoint buffers");
   if (!std::is_same<P, double>::value)
     adj_abP = dmalloc<P>(3 * (size_t)N, b);
   if (!adj_xb2 || !adj_vb2 || (!std::is_same<P, double>::value && !adj_abP))
@@ -140,6 +144,26 @@ template <typename P, typename G> const P *Engine<P, G>::abar_as_mass(const doub
   }
 }
 
+// Run tails of the three components of abar (stride N, mass type) at level lv:
+// tails (x), adj_tails[0] (y), adj_tails[1] (z).
+template <typename P, typename G> void Engine<P, G>::scatter_runs3(const P *abm, int lv)
+{
+  const int l = lv - cfg.lmin;
+  QZ_LAUNCHB("adj_scatter_runs", (32.0 + 3.0 * sizeof(P) + 24.0) * N, (k_deposit_runs3<P, G>), nblk(N), key_out,
+             pos[0], pos[1], pos[2], abm, N, lv, 3 * (D - l), (1 << lv) - 1, tails, adj_tails[0], adj_tails[1]);
+}
+
+// cell_totals over the three tail arrays of scatter_runs3.
+template <typename P, typename G> void Engine<P, G>::cell_totals3(const int *cb, const int *ce, long long ncell)
+{
+  if (ncell <= 0)
+    return;
+  QZ_LAUNCH("cell_totals", k_fill<int>, 1, nheavy, 0, (size_t)1);
+  QZ_LAUNCHB("cell_totals", 8.0 * ncell, k_cell_totals3<G>, nblk((size_t)ncell), cb, ce, ncell, tails, adj_tails[0],
+             adj_tails[1], heavy, nheavy);
+  QZ_LAUNCH("cell_totals", k_cell_totals_heavy3<G>, 1024, cb, ce, heavy, nheavy, tails, adj_tails[0], adj_tails[1]);
+}
+
 // adj_xb += J_A(x)^T ab at the current sorted positions, both in the current
 // slot order (the masses adj_m too), with the grid accelerations of A(x) in
 // gax/gay/gaz and the head cell ranges current.
@@ -149,16 +173,13 @@ template <typename P, typename G> bool Engine<P, G>::vjp_at_current(const double
     return vjp_tree(ab);
   size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
   double H = 1.0 / n;
-  const P *abm = abar_as_mass(ab);
-  // Gbar = scatter of abar (the deposit with abar as the mass, coefficient 1)
-  for (int a = 0; a < 3; a++)
-  {
-    QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
-               pos[1], pos[2], abm + (size_t)a * N, N, cfg.lmin, 3 * D, n - 1, tails);
-    cell_totals(cell_start, nullptr, (long long)n * n * n);
-    QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_deposit_head_fold<G, G>),
-               nblk(npts), cell_start, tails, adj_gbar[a], make_cube(n + 1), 1.0);
-  }
+  // Gbar = scatter of abar (the deposit with abar as the mass, coefficient 1),
+  // the three components in one pass
+  scatter_runs3(abar_as_mass(ab), cfg.lmin);
+  cell_totals3(cell_start, nullptr, (long long)n * n * n);
+  QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + 3.0 * sizeof(G) * npts, (k_adj_head_scatter3<G, G>), nblk(npts),
+             cell_start, (const int *)nullptr, tails, adj_tails[0], adj_tails[1], adj_gbar[0], adj_gbar[1],
+             adj_gbar[2], make_cube(n + 1));
   QZ_LAUNCHB("adj_grad_t", 4.0 * sizeof(G) * npts, k_grad_transpose<G>, nblk(npts), adj_gbar[0], adj_gbar[1],
              adj_gbar[2], adj_rhs, make_cube(n + 1), 1.0 / (2.0 * H));
   if (!adjoint_head_solve())
@@ -261,24 +282,18 @@ template <typename P, typename G> bool Engine<P, G>::vjp_tree(const double *ab)
   }
   // Abar: scatter of abar over the unrefined cells of the head and of every level
   const P *abm = abar_as_mass(ab);
-  for (int a = 0; a < 3; a++)
+  scatter_runs3(abm, cfg.lmin);
+  cell_totals3(cell_start, nullptr, (long long)n * n * n);
+  QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + 3.0 * sizeof(G) * npts, (k_adj_head_scatter3<G, G>), nblk(npts),
+             cell_start, d.head_child, tails, adj_tails[0], adj_tails[1], adj_gbar[0], adj_gbar[1], adj_gbar[2], cb);
+  for (int l = 1; l < nl; l++)
   {
-    const P *abma = abm + (size_t)a * N;
-    QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
-               pos[1], pos[2], abma, N, cfg.lmin, 3 * D, n - 1, tails);
-    cell_totals(cell_start, nullptr, (long long)n * n * n);
-    QZ_LAUNCHB("adj_scatter_fold", 4.0 * n * n * n + (double)sizeof(G) * npts, (k_adj_head_scatter<G, G>),
-               nblk(npts), cell_start, d.head_child, tails, adj_gbar[a], cb);
-    for (int l = 1; l < nl; l++)
-    {
-      const int lv = cfg.lmin + l;
-      QZ_LAUNCHB("adj_scatter_runs", (24.0 + sizeof(P) + 8.0) * N, (k_deposit_runs<P, G>), nblk(N), key_out, pos[0],
-                 pos[1], pos[2], abma, N, lv, 3 * (D - l), (1 << lv) - 1, tails);
-      cell_totals(d.cstart + d.lc0[l], d.cend + d.lc0[l], d.lncell[l]);
-      dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
-      QZ_LAUNCH("adj_scatter_branch", (k_adj_branch_scatter<G>), grid, d.nodes, d.lfirst[l], d.bdry_g, d.bdry_xyz,
-                d.intr_g, d.intr_xyz, d.box, d.bstart, d.bend, tails, adj_nab[a]);
-    }
+    scatter_runs3(abm, cfg.lmin + l);
+    cell_totals3(d.cstart + d.lc0[l], d.cend + d.lc0[l], d.lncell[l]);
+    dim3 grid(blocks_per_node(d.lpmax[l]), d.lcount[l]);
+    QZ_LAUNCH("adj_scatter_branch", (k_adj_branch_scatter3<G>), grid, d.nodes, d.lfirst[l], d.bdry_g, d.bdry_xyz,
+              d.intr_g, d.intr_xyz, d.box, d.bstart, d.bend, tails, adj_tails[0], adj_tails[1], adj_nab[0],
+              adj_nab[1], adj_nab[2]);
   }
   // levels, finest first: every child is done before its parent
   HostSpan span_lv(timer, "vjp_levels");

@@ -74,6 +74,141 @@ __global__ void k_adj_unsort(const int *__restrict__ perm, const double *__restr
   m_out[d] = mm;
 }
 
+// The scatter of the three components of abar (stride N, the deposit's mass
+// type) in one pass over the keys and positions: k_deposit_runs with three
+// masses, each component's run tails in its own array (t0, t1, t2).
+template <typename P, typename S>
+__global__ void k_deposit_runs3(const unsigned long long *__restrict__ key, const double *__restrict__ x,
+                                const double *__restrict__ y, const double *__restrict__ z, const P *__restrict__ m,
+                                int N, int lv, int seg_shift, int clamp_top, S *__restrict__ t0, S *__restrict__ t1,
+                                S *__restrict__ t2)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = p < N;
+  const unsigned long long c = live ? (key[p] >> seg_shift) : ~0ull;
+  double v[3][8];
+  if (live)
+  {
+    const double s = (double)(1 << lv);
+    const double px = x[p] * s, py = y[p] * s, pz = z[p] * s;
+    const int fx = min((int)px, clamp_top), fy = min((int)py, clamp_top), fz = min((int)pz, clamp_top);
+    const double wx1 = px - fx, wy1 = py - fy, wz1 = pz - fz;
+    const double wx2 = 1.0 - wx1, wy2 = 1.0 - wy1, wz2 = 1.0 - wz1;
+    const double w[8] = {wx2 * wy2 * wz2, wx1 * wy2 * wz2, wx2 * wy1 * wz2, wx1 * wy1 * wz2,
+                         wx2 * wy2 * wz1, wx1 * wy2 * wz1, wx2 * wy1 * wz1, wx1 * wy1 * wz1};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+    {
+      const double mm = (double)m[(size_t)a * N + p];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        v[a][q] = mm * w[q];
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        v[a][q] = 0.0;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1)
+  {
+    const unsigned long long co = shfl_up_u64(c, d);
+    const bool same = lane >= d && co == c;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+      {
+        const double o = __shfl_up(v[a][q], d, 64);
+        if (same)
+          v[a][q] += o;
+      }
+  }
+  const unsigned long long cn = shfl_down_u64(c, 1);
+  if (live && (lane == 63 || p == N - 1 || cn != c))
+  {
+    S *const t[3] = {t0, t1, t2};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        t[a][8 * (size_t)p + q] = (S)v[a][q];
+  }
+}
+
+// k_cell_totals and k_cell_totals_heavy over three tail arrays (one heavy list).
+template <typename S>
+__global__ void k_cell_totals3(const int *__restrict__ cb, const int *__restrict__ ce, long long ncell,
+                               S *__restrict__ t0, S *__restrict__ t1, S *__restrict__ t2, int *__restrict__ heavy,
+                               int *__restrict__ nheavy)
+{
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell)
+    return;
+  const int a = cb[c], e = ce ? ce[c] : cb[c + 1];
+  if (e - a <= 0 || (a >> 6) == ((e - 1) >> 6))
+    return;
+  if (((e - 1) >> 6) - (a >> 6) + 1 > kTotalsSerial)
+  {
+    heavy[atomicAdd(nheavy, 1)] = (int)c;
+    return;
+  }
+  S *const t[3] = {t0, t1, t2};
+  for (int k = 0; k < 3; k++)
+  {
+    double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int u = a; u < e;)
+    {
+      const int tl = min(e, ((u >> 6) + 1) << 6) - 1;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        sum[q] += (double)t[k][8 * (size_t)tl + q];
+      u = tl + 1;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      t[k][8 * (size_t)(e - 1) + q] = (S)sum[q];
+  }
+}
+
+template <typename S>
+__global__ void k_cell_totals_heavy3(const int *__restrict__ cb, const int *__restrict__ ce,
+                                     const int *__restrict__ heavy, const int *__restrict__ nheavy, S *__restrict__ t0,
+                                     S *__restrict__ t1, S *__restrict__ t2)
+{
+  const int nh = *nheavy;
+  S *const t[3] = {t0, t1, t2};
+  for (int h = blockIdx.x; h < nh; h += gridDim.x)
+  {
+    const int c = heavy[h];
+    const int a = cb[c], e = ce ? ce[c] : cb[c + 1];
+    for (int k = 0; k < 3; k++)
+    {
+      double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int w = (a >> 6) + (int)threadIdx.x; w <= ((e - 1) >> 6); w += blockDim.x)
+      {
+        const int tl = min(e, (w + 1) << 6) - 1;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          sum[q] += (double)t[k][8 * (size_t)tl + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        sum[q] = block_reduce<0>(sum[q]); // ends with a barrier: every window is read before the write
+      if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          t[k][8 * (size_t)(e - 1) + q] = (S)sum[q];
+      __syncthreads();
+    }
+  }
+}
+
 // Transpose of the interior central difference: phibar[q] = sum_a (Gbar_a[q + e_a] - Gbar_a[q - e_a]) / 2H,
 // each term present when that neighbour is an interior point; zero on the faces.
 template <typename G>
@@ -268,6 +403,39 @@ __global__ void k_adj_branch_scatter(const DNode *__restrict__ nodes, int first,
   }
 }
 
+// k_adj_branch_scatter for the three components at once.
+template <typename S>
+__global__ void k_adj_branch_scatter3(const DNode *__restrict__ nodes, int first, const int *__restrict__ bdry_g,
+                                     const int *__restrict__ bdry_xyz, const int *__restrict__ intr_g,
+                                     const int *__restrict__ intr_xyz, const int *__restrict__ box,
+                                     const int *__restrict__ bstart, const int *__restrict__ bend,
+                                     const S *__restrict__ t0, const S *__restrict__ t1, const S *__restrict__ t2,
+                                      double *__restrict__ o0, double *__restrict__ o1, double *__restrict__ o2)
+{
+  const DNode &d = nodes[first + blockIdx.y];
+  const int tot = d.nbdry + d.nintr;
+  const int rX = d.rd[0], rXY = d.rd[0] * d.rd[1];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x)
+  {
+    int gl, gx, gy, gz;
+    adj_point(d, t, bdry_g, bdry_xyz, intr_g, intr_xyz, gl, gx, gy, gz);
+    const long long b0 = d.boff + (gx - 1 - d.ts[0]) + (long long)(gy - 1 - d.ts[1]) * rX +
+                         (long long)(gz - 1 - d.ts[2]) * rXY;
+    int a[8], e[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+    {
+      const long long b = b0 + (u & 1) + ((u >> 1) & 1) * (long long)rX + (u >> 2) * (long long)rXY;
+      const int s0 = bstart[b], s1 = bend[b], st = box[b];
+      a[u] = st < 0 ? s0 : 0; // refined cells: their particles gather from the child
+      e[u] = st < 0 ? s1 : 0;
+    }
+    o0[d.goff + gl] = fold8(t0, a, e);
+    o1[d.goff + gl] = fold8(t1, a, e);
+    o2[d.goff + gl] = fold8(t2, a, e);
+  }
+}
+
 // Head Abar: the head deposit fold with coefficient 1 over the unrefined head cells.
 template <typename S, typename G>
 __global__ void k_adj_head_scatter(const int *__restrict__ start, const int *__restrict__ head_child,
@@ -292,6 +460,37 @@ __global__ void k_adj_head_scatter(const int *__restrict__ start, const int *__r
     e[u] = ok ? s1 : 0;
   }
   out[t] = (G)fold8(tails, a, e);
+}
+
+
+// k_adj_head_scatter for the three components at once (head_child null: every
+// head cell counts, the head grid without levels).
+template <typename S, typename G>
+__global__ void k_adj_head_scatter3(const int *__restrict__ start, const int *__restrict__ head_child,
+                                    const S *__restrict__ t0, const S *__restrict__ t1, const S *__restrict__ t2,
+                                    G *__restrict__ o0, G *__restrict__ o1, G *__restrict__ o2, Cube cb)
+{
+  const int n1 = cb.n1, n = n1 - 1;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= (unsigned)n1 * n1 * n1)
+    return;
+  int i, j, k;
+  cube_ijk(cb, t, i, j, k);
+  int a[8], e[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+  {
+    const int ci = i - 1 + (u & 1), cj = j - 1 + ((u >> 1) & 1), ck = k - 1 + (u >> 2);
+    const bool in = ci >= 0 && ci < n && cj >= 0 && cj < n && ck >= 0 && ck < n;
+    const int c = in ? ci + n * (cj + n * ck) : 0;
+    const int s0 = start[c], s1 = start[c + 1];
+    const bool ok = in && (!head_child || head_child[c] < 0);
+    a[u] = ok ? s0 : 0;
+    e[u] = ok ? s1 : 0;
+  }
+  o0[t] = (G)fold8(t0, a, e);
+  o1[t] = (G)fold8(t1, a, e);
+  o2[t] = (G)fold8(t2, a, e);
 }
 
 // phibar += transposed interior central difference of Abar at the boundary and

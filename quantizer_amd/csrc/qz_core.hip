@@ -219,6 +219,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   // adj_m2 receive them in the previous step's order.  adj_abP: abar as the
   // deposit's mass type (unused when P is double).
   double *adj_xb2 = nullptr, *adj_vb2 = nullptr;
+  G *adj_tails[2] = {nullptr, nullptr}; // run tails of abar's y and z components (x: tails)
   P *adj_abP = nullptr, *adj_m = nullptr, *adj_m2 = nullptr;
   G *adj_gbar[3] = {nullptr, nullptr, nullptr}, *adj_rhs = nullptr;
   // refinement levels, in the grid-pool layout: Abar (3), phibar and psi in
@@ -1209,6 +1210,8 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool load_rec(StepRec &r);
   bool adjoint_head_solve();
   const P *abar_as_mass(const double *ab);
+  void scatter_runs3(const P *abm, int lv);
+  void cell_totals3(const int *cb, const int *ce, long long ncell);
   bool vjp_at_current(const double *ab); // xbar (adj_xb) += J_A(x)^T ab, both in the current slot order
   bool force_vjp(const double *abar, double *xbar) override;
   bool adjoint_forward(int nsteps, double dt, int every) override;
