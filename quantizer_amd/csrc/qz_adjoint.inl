@@ -47,7 +47,21 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
   ckpts.clear();
   adj_nckpt = 0;
   free_ckpt(adj_fin);
-  for (auto &r : recs)
+  free_recs(recs);
+  free_recs(recs2);
+  if (helper) // its reverse-sweep thread is always joined before a call returns
+  {
+    const hipStream_t st = helper->stream;
+    (void)hipStreamSynchronize(st);
+    delete helper;
+    (void)hipStreamDestroy(st);
+    helper = nullptr;
+  }
+}
+
+template <typename P, typename G> void Engine<P, G>::free_recs(std::vector<StepRec> &v)
+{
+  for (auto &r : v)
   {
     for (int a = 0; a < 3; a++)
     {
@@ -61,7 +75,7 @@ template <typename P, typename G> void Engine<P, G>::release_adjoint()
     (void)hipFree(r.refl);
     free_tree_snap(r.tsnap);
   }
-  recs.clear();
+  v.clear();
 }
 
 template <typename P, typename G> bool Engine<P, G>::ensure_adjoint()
@@ -469,7 +483,7 @@ template <typename P, typename G> void Engine<P, G>::free_ckpt(Ckpt &c)
 // head cell ranges, head grid accelerations and the reflected axes of the step;
 // with refinement levels also the tree and node fields (TreeSnap), so the
 // reverse sweep needs no forward solve.
-template <typename P, typename G> bool Engine<P, G>::ensure_recs()
+template <typename P, typename G> bool Engine<P, G>::ensure_recs(std::vector<StepRec> &recs)
 {
   const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1, ncell = (size_t)n * n * n;
   while ((int)recs.size() < adj_every + 1)
@@ -576,7 +590,7 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
   adj_every = every;
   adj_dt = dt;
   adj_last_recorded = false;
-  if (!ensure_recs())
+  if (!ensure_recs(recs))
     return false;
   // the last segment [last, nsteps] is recorded on the way (the reverse sweep
   // starts there, so it needs no recomputation)
@@ -607,6 +621,67 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_forward(int nsteps,
   return hip_ok(hipStreamSynchronize(stream), "adjoint_forward");
 }
 
+// Recompute segment [c0, c1] from its checkpoint into the records rs: rs[0] =
+// state c0 (for the first segment only, whose a_0 term closes the sweep),
+// rs[k] = state c0 + k.  The checkpoint carries a_c0, so no force pass is
+// needed to step from it.  Ends with the stream drained.
+template <typename P, typename G>
+bool Engine<P, G>::recompute_segment(const Ckpt &ck, std::vector<StepRec> &rs, int c0, int c1, double dt)
+{
+  restore_ckpt(ck);
+  if (c0 == 0)
+  {
+    // the record of state 0 holds its fields (and tree); the force pass clears
+    // the accelerations, which the checkpoint then restores exactly
+    if (!force_fields())
+      return false;
+    restore_ckpt(ck);
+    if (!save_rec(rs[0], true))
+      return false;
+  }
+  for (int j = c0 + 1; j <= c1; j++)
+  {
+    refl_rec = rs[j - c0].refl;
+    bool ok = update_A(dt);
+    refl_rec = nullptr;
+    if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt) || !save_rec(rs[j - c0], true))
+      return false;
+  }
+  return hip_ok(hipStreamSynchronize(stream), "recompute");
+}
+
+// The helper engine of the reverse sweep: the same configuration, its own
+// buffers and a non-blocking stream, the same total mass and head faces, and a
+// potential history buffer so that a checkpoint restores the same starting
+// guess.  It launches eagerly (no graph capture while this engine captures).
+template <typename P, typename G> bool Engine<P, G>::make_helper()
+{
+  if (helper)
+    return true;
+  auto *hp = new Engine<P, G>();
+  hp->cfg = cfg;
+  const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+  if (!hip_ok(hipStreamCreateWithFlags(&hp->stream, hipStreamNonBlocking), "helper stream"))
+  {
+    delete hp;
+    return false;
+  }
+  if (!hp->init() || !(hp->phi_prev = dmalloc<double>(npts, hp->bytes)))
+  {
+    const hipStream_t st = hp->stream;
+    delete hp;
+    (void)hipStreamDestroy(st);
+    set_error("adjoint_backward: the helper engine could not be created (device memory)");
+    return false;
+  }
+  hp->head_graphs = hp->batch_graphs = false;
+  hp->total_mass = total_mass;
+  hp->face_gm = face_gm;
+  hp->host_mass = host_mass;
+  helper = hp;
+  return true;
+}
+
 template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const double *xbN, const double *vbN,
                                                                     double *xb0, double *vb0)
 {
@@ -629,8 +704,50 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
     return false;
   adj_sweep = true;
   adj_warm = false;
-  if (!ensure_recs())
+  // with more than one segment, the recomputation of the segment before the one
+  // being reversed runs on the helper engine (its own buffers and stream) on a
+  // host thread: A holds the records being reversed, B the ones being filled
+  const bool overlap = adj_overlap && adj_nckpt > 1;
+  if (!ensure_recs(recs) || (overlap && (!ensure_recs(recs2) || !make_helper())))
     return false;
+  std::vector<StepRec> *A = &recs, *B = &recs2;
+  // while the helper works, neither engine captures graphs (no capture can then
+  // see the other thread's allocations or synchronisations); launched one by
+  // one the solves take the same time
+  struct LaunchMode
+  {
+    Engine *e;
+    bool hg, bg;
+    ~LaunchMode()
+    {
+      e->head_graphs = hg;
+      e->batch_graphs = bg;
+    }
+  } launch_mode{this, head_graphs, batch_graphs};
+  if (overlap)
+    head_graphs = batch_graphs = false;
+  int dev = 0;
+  if (overlap && !hip_ok(hipGetDevice(&dev), "adjoint device"))
+    return false;
+  struct Ahead
+  {
+    std::thread th;
+    bool ok = true;
+    std::string err;
+    ~Ahead()
+    {
+      if (th.joinable())
+        th.join();
+    }
+    bool join()
+    {
+      if (th.joinable())
+        th.join();
+      if (!ok)
+        set_error("%s", err.c_str());
+      return ok;
+    }
+  } ahead;
   // the cotangents may be host or device memory (the copies infer the
   // direction); they arrive in ID order and are kept in the slot order of the
   // state they belong to, which is the particles' current order at the final
@@ -641,58 +758,64 @@ template <typename P, typename G> bool Engine<P, G>::adjoint_backward(const doub
       !hip_ok(hipMemcpyAsync(adj_m, pa[9], sizeof(P) * N, hipMemcpyDeviceToDevice, stream), "adjoint masses"))
     return false;
   QZ_LAUNCH("adj_fill", k_fill<double>, nblk(n3), adj_abA, 0.0, n3);
+  bool have = adj_last_recorded; // A holds the records of segment adj_nckpt - 1
   for (int sc = adj_nckpt - 1; sc >= 0; sc--)
   {
-    int c0 = sc * adj_every;
-    int c1 = std::min(c0 + adj_every, adj_nsteps);
-    // recompute the segment: rec[0] = state c0 (kept for the first segment
-    // only, whose a_0 term closes the sweep), rec[k] = state c0 + k.  The
-    // checkpoint carries a_c0, so no force pass is needed to step from it.
-    // The last segment was recorded by adjoint_forward itself.
-    const bool recorded = adj_last_recorded && sc == adj_nckpt - 1;
-    if (!recorded)
+    const int c0 = sc * adj_every;
+    const int c1 = std::min(c0 + adj_every, adj_nsteps);
+    if (!have)
     {
       HostSpan span_rc(timer, "adj_recompute");
-      restore_ckpt(ckpts[sc]);
-      if (sc == 0)
-      {
-        // the record of state 0 holds its fields (and tree); the force pass
-        // clears the accelerations, which the checkpoint then restores exactly
-        if (!force_fields())
-          return false;
-        restore_ckpt(ckpts[sc]);
-        if (!save_rec(recs[0], true))
-          return false;
-      }
-      for (int j = c0 + 1; j <= c1; j++)
-      {
-        refl_rec = recs[j - c0].refl;
-        bool ok = update_A(dt);
-        refl_rec = nullptr;
-        if (!ok || !force_fields() || !particle_acceleration() || !update_B(dt) || !save_rec(recs[j - c0], true))
-          return false;
-      }
+      if (!recompute_segment(ckpts[sc], *A, c0, c1, dt))
+        return false;
+    }
+    if (overlap && sc > 0)
+    {
+      // B is free: the reverse sweep that read it ended with a stream drain
+      const int p0 = (sc - 1) * adj_every;
+      Engine *hp = helper;
+      std::vector<StepRec> *rb = B;
+      const Ckpt *ck = &ckpts[sc - 1];
+      ahead.ok = true;
+      ahead.th = std::thread([hp, rb, ck, p0, dt, dev, this, &ahead]() {
+        ahead.ok = hip_ok(hipSetDevice(dev), "adjoint device") &&
+                   hp->recompute_segment(*ck, *rb, p0, std::min(p0 + adj_every, adj_nsteps), dt);
+        if (!ahead.ok)
+          ahead.err = g_last_error;
+      });
     }
     // reverse sweep over the segment
     for (int j = c1; j > c0; j--)
     {
-      if (!load_rec(recs[j - c0]))
+      StepRec &r = (*A)[j - c0];
+      if (!load_rec(r))
         return false;
       QZ_LAUNCH("adj_abar", k_adj_abar, nblk(n3), adj_vb, adj_abA, adj_ab, (int)n3, h);
       HostSpan span_vjp(timer, "adj_vjp");
       if (!vjp_at_current(adj_ab))
         return false;
       // to the slot order before this step's sort, where its drift ran
-      QZ_LAUNCHB("adj_unsort", (2.0 * (48.0 + sizeof(P)) + 4.0) * N, k_adj_unsort<P>, nblk(N), recs[j - c0].perm,
-                 adj_xb, adj_vb, adj_m, adj_xb2, adj_vb2, adj_m2, N);
+      QZ_LAUNCHB("adj_unsort", (2.0 * (48.0 + sizeof(P)) + 4.0) * N, k_adj_unsort<P>, nblk(N), r.perm, adj_xb,
+                 adj_vb, adj_m, adj_xb2, adj_vb2, adj_m2, N);
       std::swap(adj_xb, adj_xb2);
       std::swap(adj_vb, adj_vb2);
       std::swap(adj_m, adj_m2);
-      QZ_LAUNCH("adj_kick_drift", k_adj_kick_drift, nblk(N), adj_xb, adj_vb, adj_abA, recs[j - c0].refl, N, dt);
+      QZ_LAUNCH("adj_kick_drift", k_adj_kick_drift, nblk(N), adj_xb, adj_vb, adj_abA, r.refl, N, dt);
+    }
+    have = false;
+    if (overlap && sc > 0)
+    {
+      HostSpan span_w(timer, "adj_recompute_wait");
+      // A's buffers are free once the sweep has drained; B is complete once the
+      // helper's thread (which drains the helper's stream) has ended
+      if (!hip_ok(hipStreamSynchronize(stream), "reverse sweep") || !ahead.join())
+        return false;
+      std::swap(A, B);
+      have = true;
     }
   }
   // a_0 = A(x_0) fed the first half kick; then back to ID order
-  if (!load_rec(recs[0]) || !vjp_at_current(adj_abA) || !slot_cotangents(adj_xb, adj_ab, false) ||
+  if (!load_rec((*A)[0]) || !vjp_at_current(adj_abA) || !slot_cotangents(adj_xb, adj_ab, false) ||
       !slot_cotangents(adj_vb, adj_abA, false))
     return false;
   if (!hip_ok(hipMemcpy(xb0, adj_ab, sizeof(double) * n3, hipMemcpyDefault), "download xbar") ||

@@ -258,7 +258,12 @@ template <typename P, typename G> struct Engine final : EngineBase
   std::vector<Ckpt> ckpts; // checkpoint buffers (kept for later runs); the first adj_nckpt are current
   int adj_nckpt = 0;
   Ckpt adj_fin{}; // the final state during a reverse sweep
-  std::vector<StepRec> recs;
+  std::vector<StepRec> recs, recs2; // two record sets: one reversed while the other is recomputed
+  // The reverse sweep's helper: a second engine that recomputes the segment
+  // before the one being reversed, on a host thread and its own stream
+  // (QZ_ADJ_OVERLAP=0: every segment is recomputed in turn on this engine).
+  Engine *helper = nullptr;
+  bool adj_overlap = true;
   int adj_nsteps = 0, adj_every = 0;
   bool adj_sweep = false; // a reverse sweep is running (its head solves warm-start)
   bool adj_warm = false;  // adj_psi holds the previous adjoint head solution of this reverse sweep
@@ -407,6 +412,8 @@ template <typename P, typename G> struct Engine final : EngineBase
       batch_graphs = f[0] != '0';
     if (const char *f = std::getenv("QZ_HOST_TREE"))
       host_refine = f[0] == '1';
+    if (const char *f = std::getenv("QZ_ADJ_OVERLAP"))
+      adj_overlap = f[0] != '0';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
@@ -1202,7 +1209,10 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool save_ckpt(Ckpt &c);
   void restore_ckpt(const Ckpt &c);
   void free_ckpt(Ckpt &c);
-  bool ensure_recs();
+  bool ensure_recs(std::vector<StepRec> &recs);
+  void free_recs(std::vector<StepRec> &v);
+  bool recompute_segment(const Ckpt &ck, std::vector<StepRec> &rs, int c0, int c1, double dt);
+  bool make_helper();
   bool snap_tree(TreeSnap *&t);
   bool restore_tree(const TreeSnap *t);
   void free_tree_snap(TreeSnap *&t);
