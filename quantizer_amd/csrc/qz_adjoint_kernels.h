@@ -114,21 +114,9 @@ __global__ void k_deposit_runs3(const unsigned long long *__restrict__ key, cons
       for (int q = 0; q < 8; q++)
         v[a][q] = 0.0;
   }
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1)
-  {
-    const unsigned long long co = shfl_up_u64(c, d);
-    const bool same = lane >= d && co == c;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-      {
-        const double o = __shfl_up(v[a][q], d, 64);
-        if (same)
-          v[a][q] += o;
-      }
-  }
+  wave_seg_scan(v[0], c);
+  wave_seg_scan(v[1], c);
+  wave_seg_scan(v[2], c);
   const unsigned long long cn = shfl_down_u64(c, 1);
   if (live && (lane == 63 || p == N - 1 || cn != c))
   {

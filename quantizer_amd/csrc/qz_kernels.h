@@ -296,6 +296,49 @@ __global__ void k_cell_start_gaps(const unsigned long long *__restrict__ key, in
 // (key >> seg_shift).  The last lane of each run stores the run's 8 corner sums
 // at its slot.  A cell's total is the ordered sum of its runs' tails, one per
 // 64-slot window it overlaps.  Deterministic: fixed shuffle order, no atomics.
+//
+// The scan moves values between lanes with DPP (data-parallel primitives: a
+// lane reads another lane's register as an instruction operand, no LDS
+// traffic): row_shr 1, 2, 4, 8 scan each row of 16 lanes, then row_bcast 15
+// adds lane 15's sum to the next row (rows 1 and 3) and row_bcast 31 adds lane
+// 31's sum to rows 2 and 3.  Each step adds the source's sum only where the
+// source lane holds the same cell: the keys are sorted, so the lanes between
+// them hold it too and every lane ends with its run's sum up to itself.
+template <int CTRL, int ROWS> __device__ inline unsigned dpp_u32(unsigned old, unsigned v)
+{
+  return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWS, 0xf, false);
+}
+
+// one step of the segmented scan over K values per lane (lanes without a
+// source read the key ~0, which no live cell has, and the value 0)
+template <int CTRL, int ROWS, int K>
+__device__ inline void seg_scan_step(double (&v)[K], unsigned klo, unsigned khi)
+{
+  const unsigned slo = dpp_u32<CTRL, ROWS>(~0u, klo), shi = dpp_u32<CTRL, ROWS>(~0u, khi);
+  const bool same = slo == klo && shi == khi;
+#pragma unroll
+  for (int q = 0; q < K; q++)
+  {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v[q]);
+    const unsigned lo = dpp_u32<CTRL, ROWS>(0u, (unsigned)b), hi = dpp_u32<CTRL, ROWS>(0u, (unsigned)(b >> 32));
+    const double o = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    if (same)
+      v[q] += o;
+  }
+}
+
+// inclusive segmented scan of v over the wavefront, segments = equal keys c
+// (non-decreasing along the lanes); every lane of the wavefront takes part
+template <int K> __device__ inline void wave_seg_scan(double (&v)[K], unsigned long long c)
+{
+  const unsigned klo = (unsigned)c, khi = (unsigned)(c >> 32);
+  seg_scan_step<0x111, 0xf>(v, klo, khi); // row_shr:1
+  seg_scan_step<0x112, 0xf>(v, klo, khi); // row_shr:2
+  seg_scan_step<0x114, 0xf>(v, klo, khi); // row_shr:4
+  seg_scan_step<0x118, 0xf>(v, klo, khi); // row_shr:8
+  seg_scan_step<0x142, 0xa>(v, klo, khi); // row_bcast:15, rows 1 and 3
+  seg_scan_step<0x143, 0xc>(v, klo, khi); // row_bcast:31, rows 2 and 3
+}
 __device__ inline unsigned long long shfl_up_u64(unsigned long long v, int d)
 {
   unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
@@ -345,19 +388,7 @@ __global__ void k_deposit_runs(const unsigned long long *__restrict__ key, const
     for (int q = 0; q < 8; q++)
       v[q] = 0.0;
   }
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1)
-  {
-    unsigned long long co = shfl_up_u64(c, d);
-    bool same = lane >= d && co == c;
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-    {
-      double o = __shfl_up(v[q], d, 64);
-      if (same)
-        v[q] += o;
-    }
-  }
+  wave_seg_scan(v, c);
   unsigned long long cn = shfl_down_u64(c, 1);
   bool tail = live && (lane == 63 || p == N - 1 || cn != c);
   if (tail)
