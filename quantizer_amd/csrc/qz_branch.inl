@@ -351,6 +351,79 @@ static bool select_flags(TreeDev &d, const unsigned char *f, long long n, int *o
   return hip_ok(hipGetLastError(), "selection");
 }
 
+// Re-sort after a drift from a sorted state (the moved flags current): the
+// unsorted keys (narrow: key_out as 32 bits, else key_in) partitioned into
+// stayers and movers, the movers radix-sorted, then merged by (key, slot) into
+// the sorted 64-bit keys key_out and slot_out, the permutation a stable radix
+// sort of all keys gives.  One host read of the mover count, queued behind the
+// partition so that the device works while the host waits.
+template <typename P, typename G> bool Engine<P, G>::resort_keys()
+{
+  const bool narrow = end_bit <= 32;
+  const size_t ks = narrow ? sizeof(unsigned) : sizeof(unsigned long long);
+  const long long per = (long long)kBlock * kSelPer;
+  const int nb = (int)((N + per - 1) / per);
+  const int tiles = (N + kMergeTile - 1) / kMergeTile;
+  if (!rs_cnt)
+  {
+    size_t b = 0;
+    rs_cnt = dmalloc<int>(nb + 1, b);
+    rs_off = dmalloc<int>(nb + 1, b);
+    rs_split = dmalloc<int>(tiles + 1, b);
+    for (int q = 0; q < 3; q++)
+    {
+      rs_idx[q] = dmalloc<int>(N, b);
+      rs_key[q] = dmalloc<char>(ks * (size_t)N, b);
+    }
+    rs_scan_bytes = std::max<size_t>(qz_scan_temp_bytes(nb + 1), 1);
+    rs_scan = dmalloc<char>(rs_scan_bytes, b);
+    bytes += b;
+    if (!rs_cnt || !rs_off || !rs_split || !rs_idx[0] || !rs_idx[1] || !rs_idx[2] || !rs_key[0] || !rs_key[1] ||
+        !rs_key[2] || !rs_scan || !hip_ok(hipHostMalloc((void **)&rs_hcount, sizeof(int)), "re-sort count"))
+      return hip_ok(hipErrorOutOfMemory, "re-sort buffers");
+  }
+  using U = unsigned long long;
+  hipLaunchKernelGGL(k_select_count, dim3(nb), dim3(kBlock), 0, stream, moved, (long long)N, rs_cnt);
+  if (!hip_ok(qz_exclusive_scan(rs_scan, rs_scan_bytes, rs_cnt, rs_off, nb + 1, stream), "re-sort scan") ||
+      !hip_ok(hipMemcpyAsync(rs_hcount, rs_off + nb, sizeof(int), hipMemcpyDeviceToHost, stream), "re-sort count"))
+    return false;
+  if (narrow)
+    hipLaunchKernelGGL(k_part_write<unsigned>, dim3(nb), dim3(kBlock), 0, stream, moved, (const unsigned *)key_out,
+                       (long long)N, rs_off, (unsigned *)rs_key[0], rs_idx[0], (unsigned *)rs_key[1], rs_idx[1]);
+  else
+    hipLaunchKernelGGL(k_part_write<U>, dim3(nb), dim3(kBlock), 0, stream, moved, (const U *)key_in, (long long)N,
+                       rs_off, (U *)rs_key[0], rs_idx[0], (U *)rs_key[1], rs_idx[1]);
+  if (!hip_ok(hipStreamSynchronize(stream), "re-sort count"))
+    return false;
+  const int M = *rs_hcount, S = N - M; // movers, stayers
+  rs_movers = M;
+  if (narrow)
+  {
+    const unsigned *sk = (const unsigned *)rs_key[0], *mk = (const unsigned *)rs_key[1];
+    unsigned *mks = (unsigned *)rs_key[2];
+    if (M > 0 && !hip_ok(qz_sort_pairs(sort_tmp, sort_tmp_bytes, mk, mks, rs_idx[1], rs_idx[2], M, end_bit, stream),
+                         "re-sort movers"))
+      return false;
+    hipLaunchKernelGGL(k_merge_splits<unsigned>, dim3(nblk(tiles + 1)), dim3(kBlock), 0, stream, sk, rs_idx[0], S,
+                       (const unsigned *)mks, rs_idx[2], M, tiles, rs_split);
+    hipLaunchKernelGGL((k_merge<unsigned, U>), dim3(tiles), dim3(kMergeThreads), 0, stream, sk, rs_idx[0], S,
+                       (const unsigned *)mks, rs_idx[2], M, rs_split, key_out, slot_out);
+  }
+  else
+  {
+    const U *sk = (const U *)rs_key[0], *mk = (const U *)rs_key[1];
+    U *mks = (U *)rs_key[2];
+    if (M > 0 && !hip_ok(qz_sort64_pairs(sort_tmp, sort_tmp_bytes, mk, mks, rs_idx[1], rs_idx[2], M, end_bit, stream),
+                         "re-sort movers"))
+      return false;
+    hipLaunchKernelGGL(k_merge_splits<U>, dim3(nblk(tiles + 1)), dim3(kBlock), 0, stream, sk, rs_idx[0], S,
+                       (const U *)mks, rs_idx[2], M, tiles, rs_split);
+    hipLaunchKernelGGL((k_merge<U, U>), dim3(tiles), dim3(kMergeThreads), 0, stream, sk, rs_idx[0], S, (const U *)mks,
+                       rs_idx[2], M, rs_split, key_out, slot_out);
+  }
+  return hip_ok(hipGetLastError(), "re-sort");
+}
+
 // Zone of every marked cell (the rank of its root) into d.hzone_s and the list
 // entries in zone order into d.hidx_s: a stable sort by zone, or, when the
 // zones already are runs of consecutive entries (`unsorted` false: every root

@@ -500,6 +500,173 @@ __global__ void __launch_bounds__(kBlock) k_select_write(const unsigned char *__
     *num = boff[gridDim.x];
 }
 
+// ------------------------------------------------------------------ re-sort of near-sorted keys
+// After a drift from a sorted state most particles keep their key.  Those that
+// keep it ("stayers") are still in key order, and in slot order within a key;
+// the others ("movers", flagged by the drift) are radix-sorted by key alone
+// (stable, so in slot order within a key) and the two runs are merged by
+// (key, old slot).  The result is the permutation of a stable radix sort of all
+// keys, exactly.
+
+// Stable partition of the keys by the moved flags: movers to mk/mi at their
+// rank among the movers, stayers to sk/si at theirs (slot minus the movers
+// before it).  A workgroup takes the kBlock * kSelPer entries that
+// k_select_count counted for it (moff: the exclusive scan of those counts), in
+// kSelPer rounds of kBlock consecutive entries, so that loads and stores are
+// coalesced; ranks within a round come from wavefront ballots.
+template <typename K>
+__global__ void __launch_bounds__(kBlock) k_part_write(const unsigned char *__restrict__ moved,
+                                                       const K *__restrict__ key, long long n,
+                                                       const int *__restrict__ moff, K *__restrict__ sk,
+                                                       int *__restrict__ si, K *__restrict__ mk,
+                                                       int *__restrict__ mi)
+{
+  __shared__ int wcnt[kBlock / 64];
+  const long long base = (long long)blockIdx.x * kBlock * kSelPer;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int mbase = moff[blockIdx.x]; // movers before this round
+  for (int q = 0; q < kSelPer; q++)
+  {
+    const long long i = base + (long long)q * kBlock + threadIdx.x;
+    const bool live = i < n;
+    const bool mv = live && moved[i] != 0;
+    const unsigned long long bm = __ballot(mv);
+    const int wrank = __popcll(bm & ((1ull << lane) - 1ull));
+    if (lane == 0)
+      wcnt[wv] = __popcll(bm);
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < kBlock / 64; w++)
+    {
+      const int c = wcnt[w];
+      before += w < wv ? c : 0;
+      tot += c;
+    }
+    __syncthreads(); // wcnt is rewritten next round
+    if (live)
+    {
+      const K kv = key[i];
+      const int r = mbase + before + wrank; // movers before entry i
+      if (mv)
+      {
+        mk[r] = kv;
+        mi[r] = (int)i;
+      }
+      else
+      {
+        sk[i - r] = kv;
+        si[i - r] = (int)i;
+      }
+    }
+    mbase += tot;
+  }
+}
+
+template <typename K> __device__ inline bool key_slot_less(K ka, int ia, K kb, int ib)
+{
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+// Entries of A among the first d of the merge of A[0, na) and B[0, nb), both
+// ordered by (key, slot) (merge path: the first a with A[a] not before
+// B[d - 1 - a]).
+template <typename K>
+__device__ inline int merge_split(const K *ak, const int *ai, int na, const K *bk, const int *bi, int nb, int d)
+{
+  int lo = max(0, d - nb), hi = min(d, na);
+  while (lo < hi)
+  {
+    const int mid = (lo + hi) >> 1;
+    if (key_slot_less(ak[mid], ai[mid], bk[d - 1 - mid], bi[d - 1 - mid]))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+constexpr int kMergeThreads = 256, kMergeItems = 8, kMergeTile = kMergeThreads * kMergeItems;
+
+// split[t] = entries of A among the first t kMergeTile outputs, t = 0 .. tiles
+template <typename K>
+__global__ void k_merge_splits(const K *__restrict__ ak, const int *__restrict__ ai, int na,
+                               const K *__restrict__ bk, const int *__restrict__ bi, int nb, int tiles,
+                               int *__restrict__ split)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > tiles)
+    return;
+  split[t] = merge_split(ak, ai, na, bk, bi, nb, min(na + nb, t * kMergeTile));
+}
+
+// One tile of kMergeTile outputs per workgroup: its A and B ranges staged in
+// LDS, each thread merges kMergeItems consecutive outputs from its own merge
+// path split, the results go back through LDS for coalesced stores (keys
+// written as KO: 32-bit keys leave widened to 64 bits).
+template <typename K, typename KO>
+__global__ void __launch_bounds__(kMergeThreads) k_merge(const K *__restrict__ ak, const int *__restrict__ ai, int na,
+                                                         const K *__restrict__ bk, const int *__restrict__ bi,
+                                                         int nb, const int *__restrict__ split,
+                                                         KO *__restrict__ ok, int *__restrict__ oi)
+{
+  __shared__ K sk[kMergeTile];
+  __shared__ int sx[kMergeTile];
+  const int n = na + nb;
+  const int d0 = blockIdx.x * kMergeTile, d1 = min(n, d0 + kMergeTile);
+  const int a0 = split[blockIdx.x], a1 = split[blockIdx.x + 1];
+  const int b0 = d0 - a0;
+  const int la = a1 - a0, len = d1 - d0, lb = len - la;
+  for (int t = threadIdx.x; t < len; t += kMergeThreads)
+  {
+    if (t < la)
+    {
+      sk[t] = ak[a0 + t];
+      sx[t] = ai[a0 + t];
+    }
+    else
+    {
+      sk[t] = bk[b0 + t - la];
+      sx[t] = bi[b0 + t - la];
+    }
+  }
+  __syncthreads();
+  const int t0 = threadIdx.x * kMergeItems;
+  K rk[kMergeItems];
+  int rx[kMergeItems];
+  int cnt = 0;
+  if (t0 < len)
+  {
+    int i = merge_split(sk, sx, la, sk + la, sx + la, lb, t0);
+    int j = t0 - i;
+    cnt = min(len - t0, kMergeItems);
+#pragma unroll
+    for (int q = 0; q < kMergeItems; q++)
+      if (q < cnt)
+      {
+        const bool take_a = j >= lb || (i < la && key_slot_less(sk[i], sx[i], sk[la + j], sx[la + j]));
+        const int p = take_a ? i : la + j;
+        rk[q] = sk[p];
+        rx[q] = sx[p];
+        i += take_a ? 1 : 0;
+        j += take_a ? 0 : 1;
+      }
+  }
+  __syncthreads(); // every thread has read its inputs from LDS
+#pragma unroll
+  for (int q = 0; q < kMergeItems; q++)
+    if (q < cnt)
+    {
+      sk[t0 + q] = rk[q];
+      sx[t0 + q] = rx[q];
+    }
+  __syncthreads();
+  for (int t = threadIdx.x; t < len; t += kMergeThreads)
+  {
+    ok[d0 + t] = (KO)sk[t];
+    oi[d0 + t] = sx[t];
+  }
+}
+
 // ------------------------------------------------------------------ head level marks and zones
 // fill_cell_ref for the head (tree_construction.c:527-575, n_exp = 1): a head
 // cell is marked when some cell of its 3x3x3 block inside the box holds at

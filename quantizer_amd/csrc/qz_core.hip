@@ -201,6 +201,21 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool lev_ready = false, lev_head = false, dt_ready = false;
   double dt_cfl = 0.5;
   void particles_changed() { lev_ready = dt_ready = false; }
+  // Re-sort of near-sorted keys (QZ_RESORT=0: a full radix sort every time):
+  // the drift from a sorted state flags the particles whose key changed
+  // (moved; moved_valid records that the last drift wrote the flags, which
+  // describe the keys while keys_fresh holds), the sort then partitions, sorts
+  // the movers and merges.  Scratch: block counts and offsets, partitioned and
+  // sorted keys and slots, merge splits, a pinned count.
+  bool resort = true, moved_valid = false;
+  unsigned char *moved = nullptr;
+  int *rs_cnt = nullptr, *rs_off = nullptr, *rs_split = nullptr, *rs_hcount = nullptr;
+  int *rs_idx[3] = {nullptr, nullptr, nullptr};
+  void *rs_key[3] = {nullptr, nullptr, nullptr};
+  void *rs_scan = nullptr;
+  size_t rs_scan_bytes = 0;
+  int rs_movers = 0; // movers of the last re-sort
+  bool resort_keys();
   std::vector<double> host_mass; // the masses as given (snapshots write them unchanged)
   double *phi_prev = nullptr; // the previous converged head potential
   int phi_hist = 0;           // converged head potentials since the last reset of the history
@@ -360,6 +375,12 @@ template <typename P, typename G> struct Engine final : EngineBase
     (void)hipFree(key_out);
     (void)hipFree(slot_in);
     (void)hipFree(slot_out);
+    void *rs[] = {moved, rs_cnt, rs_off, rs_split, rs_idx[0], rs_idx[1], rs_idx[2], rs_key[0], rs_key[1], rs_key[2],
+                  rs_scan};
+    for (void *q : rs)
+      (void)hipFree(q);
+    if (rs_hcount)
+      (void)hipHostFree(rs_hcount);
     (void)hipFree(cell_start);
     (void)hipFree(tails);
     (void)hipFree(heavy);
@@ -414,6 +435,8 @@ template <typename P, typename G> struct Engine final : EngineBase
       host_refine = f[0] == '1';
     if (const char *f = std::getenv("QZ_ADJ_OVERLAP"))
       adj_overlap = f[0] != '0';
+    if (const char *f = std::getenv("QZ_RESORT"))
+      resort = f[0] != '0';
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     for (int a = 0; a < 3; a++)
     {
@@ -562,15 +585,20 @@ template <typename P, typename G> struct Engine final : EngineBase
   // ---------------------------------------------------------------- stages
   bool update_A(double dt) override
   {
-    // the drift also writes the new positions' keys in the sort's input form
+    // the drift also writes the new positions' keys in the sort's input form,
+    // and, from a sorted state, which particles changed key
     const bool narrow = end_bit <= 32;
     void *key = narrow ? (void *)key_out : (void *)key_in;
-    QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P) + (narrow ? 8.0 : 12.0)) * N, k_update_A<P>, nblk(N), pos[0],
-               pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, dt, refl_rec, key,
-               slot_in, cfg.lmin, D, narrow ? 1 : 0);
+    const bool track = resort && keys_valid;
+    if (track && !moved && !(moved = dmalloc<unsigned char>(N, bytes)))
+      return hip_ok(hipErrorOutOfMemory, "moved flags");
+    QZ_LAUNCHB("update_A", (6.0 * 8 + 9.0 * sizeof(P) + (narrow ? 8.0 : 12.0) + (track ? 1.0 : 0.0)) * N,
+               k_update_A<P>, nblk(N), pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6], pa[7], pa[8], N, dt,
+               refl_rec, key, slot_in, cfg.lmin, D, narrow ? 1 : 0, track ? moved : nullptr);
     keys_valid = false;
     keys_fresh = true;
     particles_changed();
+    moved_valid = track;
     return hip_ok(hipGetLastError(), "particle_updating_A"); // stream-ordered; readers synchronise
   }
 
@@ -624,9 +652,15 @@ template <typename P, typename G> struct Engine final : EngineBase
         QZ_LAUNCHB("keys", (3.0 * 8 + 12.0) * N, k_keys<double>, nblk(N), pos[0], pos[1], pos[2], key_in, slot_in, N,
                    cfg.lmin, D);
     }
+    const bool near = keys_fresh && moved_valid; // the drift's flags describe these keys
     keys_fresh = false;
     timer.begin(stream);
-    if (end_bit <= 32)
+    if (near)
+    {
+      if (!resort_keys()) // leaves 64-bit sorted keys in key_out
+        return false;
+    }
+    else if (end_bit <= 32)
     {
       if (!hip_ok(qz_sort_pairs(sort_tmp, sort_tmp_bytes, (const unsigned *)key_out, (unsigned *)key_in, slot_in,
                                 slot_out, N, end_bit, stream),
@@ -637,7 +671,14 @@ template <typename P, typename G> struct Engine final : EngineBase
     else if (!hip_ok(qz_sort64_pairs(sort_tmp, sort_tmp_bytes, key_in, key_out, slot_in, slot_out, N, end_bit, stream),
                      "radix sort"))
       return false;
-    timer.end(stream, "sort", (end_bit <= 32 ? 16.0 : 24.0) * N);
+    const double kb = end_bit <= 32 ? 4.0 : 8.0; // key bytes
+    // full sort: keys and slots read and written once per pass (three passes
+    // of 9 bits for 32-bit keys); re-sort: flags and keys read, partition
+    // written, the movers sorted, the merge read and written
+    if (near)
+      timer.end(stream, "resort", (kb + 1.0 + 2.0 * (kb + 4.0)) * N + 6.0 * (kb + 4.0) * rs_movers);
+    else
+      timer.end(stream, "sort", 2.0 * (kb + 4.0) * N * (end_bit <= 32 ? 3.0 : (end_bit + 7) / 8));
     PtclArrays<P> src, dst;
     for (int a = 0; a < 3; a++)
     {

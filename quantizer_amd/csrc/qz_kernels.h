@@ -554,7 +554,8 @@ template <typename P>
 __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, P *__restrict__ vx,
                            P *__restrict__ vy, P *__restrict__ vz, P *__restrict__ ax, P *__restrict__ ay,
                            P *__restrict__ az, int N, double dt, unsigned char *__restrict__ refl,
-                           void *__restrict__ key, int *__restrict__ slot, int lmin, int D, int narrow)
+                           void *__restrict__ key, int *__restrict__ slot, int lmin, int D, int narrow,
+                           unsigned char *__restrict__ moved)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N)
@@ -563,6 +564,8 @@ __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, doubl
   const P dtp = (P)dt;
   unsigned char flags = 0;
   double pos[3] = {x[i], y[i], z[i]};
+  // the key before the drift, for the moved flag of the re-sort
+  const unsigned long long k_old = moved ? octree_key<double>(pos[0], pos[1], pos[2], lmin, D) : 0ull;
   P vel[3] = {vx[i], vy[i], vz[i]};
   P acc[3] = {ax[i], ay[i], az[i]};
 #pragma unroll
@@ -603,6 +606,8 @@ __global__ void k_update_A(double *__restrict__ x, double *__restrict__ y, doubl
     else
       ((unsigned long long *)key)[i] = k;
     slot[i] = i;
+    if (moved)
+      moved[i] = k != k_old ? 1 : 0;
   }
 }
 
