@@ -332,6 +332,13 @@ __device__ inline void seg_scan_step(double (&v)[K], unsigned klo, unsigned khi)
 template <int K> __device__ inline void wave_seg_scan(double (&v)[K], unsigned long long c)
 {
   const unsigned klo = (unsigned)c, khi = (unsigned)(c >> 32);
+  // a wavefront whose lanes all hold different cells (sparse regions: one
+  // particle per cell) has nothing to scan
+  {
+    const unsigned plo = dpp_u32<0x138, 0xf>(~0u, klo), phi = dpp_u32<0x138, 0xf>(~0u, khi); // wave_shr:1
+    if (__ballot(plo == klo && phi == khi) == 0ull)
+      return;
+  }
   seg_scan_step<0x111, 0xf>(v, klo, khi); // row_shr:1
   seg_scan_step<0x112, 0xf>(v, klo, khi); // row_shr:2
   seg_scan_step<0x114, 0xf>(v, klo, khi); // row_shr:4
