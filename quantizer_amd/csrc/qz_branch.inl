@@ -1027,9 +1027,13 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
 template <typename P, typename G> bool Engine<P, G>::force_overlapped(bool with_particles)
 {
   if (cfg.lmin == cfg.lmax || !setup_ahead || timer.on)
-    return reset() && tree_adaptation() && grid_density() && potential_head() && potential_branches() &&
+    return (with_particles ? reset_marks() : reset()) && tree_adaptation() && grid_density() && potential_head() &&
+           potential_branches() &&
            grid_acceleration() && (!with_particles || particle_acceleration());
-  if (!reset() || !tree_state() || !join_ahead())
+  // with the particles' accelerations gathered at the end of this pass, reset
+  // need not zero them (nothing reads them before the gather writes every
+  // particle's), nor the head density (its fold writes every grid point)
+  if (!(with_particles ? reset_marks() : reset()) || !tree_state() || !join_ahead())
     return false;
   TreeDev &d = *tree->dev;
   d.drop_plan();

@@ -614,6 +614,14 @@ template <typename P, typename G> struct Engine final : EngineBase
     return hip_ok(hipGetLastError(), "particle_updating_B"); // stream-ordered; readers synchronise
   }
 
+  // reset's bookkeeping without its fills (see force_overlapped)
+  bool reset_marks()
+  {
+    acc_live = false;
+    particles_changed();
+    return true;
+  }
+
   bool reset() override
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
