@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -138,7 +139,12 @@ private:
   TreePool()
   {
     const int hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nw = std::max(0, std::min(kTreeThreads, hw) - 1);
+    // QZ_TREE_THREADS caps the threads (the caller's included), for hosts that
+    // keep their own cores busy meanwhile (an OpenMP program around the library)
+    int cap = kTreeThreads;
+    if (const char *f = std::getenv("QZ_TREE_THREADS"))
+      cap = std::max(1, std::min(kTreeThreads, std::atoi(f)));
+    const int nw = std::max(0, std::min(cap, hw) - 1);
     for (int t = 0; t < nw; t++)
       th_.emplace_back([this]() { loop(); });
   }
