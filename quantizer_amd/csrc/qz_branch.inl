@@ -53,6 +53,9 @@ struct TreeDev
   long long grid_cap = 0;
   size_t gsize = 0; // sizeof(G)
   int *head_child = nullptr;
+  // head cells whose head_child entry the last build set, listed in hlist[0,
+  // hc_set) (every other entry is -1); -1 when unknown (new map, restored tree)
+  int hc_set = -1;
   int *flags = nullptr, *flag_count = nullptr;
   int flag_cap = 0;
   // head level marks and zones (device): mark per head cell, ordered list of
@@ -744,13 +747,19 @@ template <typename P, typename G> bool Engine<P, G>::build_tree(hipStream_t s)
   TreeHost &th = *tree->host;
   TreeDev &d = *tree->dev;
   const size_t ncell_head = (size_t)n * n * n;
-  QZ_LAUNCHB("tree_fill", 4.0 * ncell_head, k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
+  // back to all -1: only the cells the last build set, when they are known
+  if (d.hc_set < 0)
+    QZ_LAUNCHB("tree_fill", 4.0 * ncell_head, k_fill<int>, nblk(ncell_head), d.head_child, -1, ncell_head);
+  else if (d.hc_set > 0)
+    QZ_LAUNCHB("tree_fill", 8.0 * d.hc_set, k_head_unset, nblk(d.hc_set), d.hlist, d.hc_set, d.head_child);
+  d.hc_set = -1;
   int zone_cells = 0; // zone cells of the next level, when its zones were found on the device
   {
     HostSpan span(timer, "tree_head_zones");
     if (!refine_head_device(zone_cells, stream))
       return false;
   }
+  d.hc_set = zone_cells; // the marked head cells, all of them zone cells
   d.hnodes.clear();
   d.lfirst.assign(1, 0);
   d.lcount.assign(1, 1);
@@ -2054,6 +2063,7 @@ template <typename P, typename G> bool Engine<P, G>::restore_tree(const TreeSnap
     hipStream_t s;
     ~End() { t.end(s, "tree_restore"); }
   } end_{timer, stream};
+  d.hc_set = -1; // the recorded map's cells are not those in hlist
   if (!get(d.nodes, t->nodes, sizeof(DNode) * (size_t)t->n_nodes) ||
       !get(d.head_child, t->head_child, sizeof(int) * ncell_head) || !get(d.box, t->box, sizeof(int) * nb) ||
       !get(d.bstart, t->bstart, sizeof(int) * nb) || !get(d.bend, t->bend, sizeof(int) * nb) ||

@@ -772,6 +772,14 @@ __global__ void k_head_index(const int *__restrict__ list, int M, int *__restric
     map[list[i]] = i;
 }
 
+// Listed cells of a map back to -1 (the map is -1 everywhere else).
+__global__ void k_head_unset(const int *__restrict__ list, int M, int *__restrict__ map)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M)
+    map[list[i]] = -1;
+}
+
 // Union-find with parent[x] <= x at all times: roots link only under smaller
 // roots (atomic compare-and-swap on the root), path halving only replaces a
 // parent by one of its ancestors, so racing finds and links are harmless and
