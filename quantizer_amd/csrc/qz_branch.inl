@@ -1966,10 +1966,14 @@ template <typename P, typename G> bool Engine<P, G>::timestep(double cfl, double
     QZ_LAUNCHB("cfl_dt", 9.0 * sizeof(P) * N, k_cfl_dt<P>, nb, pos[0], pos[1], pos[2], pa[3], pa[4], pa[5], pa[6],
                pa[7], pa[8], N, hc, nd, bx, cfg.lmin, cfl, dt_max, part);
   }
-  // fold the per-block minima through max(-dt) (in place: the folded minima
-  // are used once)
-  hipLaunchKernelGGL(k_negate, dim3(nb / kBlock + 1), dim3(kBlock), 0, stream, src, (int)nb);
-  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, src, (int)nb, red);
+  // fold the per-block minima through max(-dt) in two stages, the first into
+  // part past the minima (part holds at least 4 nb values) or into part when
+  // the minima are the kick's
+  const unsigned ns = std::min<unsigned>(kBlock, (nb + kBlock - 1) / kBlock);
+  const int chunk = (int)((nb + ns - 1) / ns);
+  double *stage = src == part ? part + nb : part;
+  hipLaunchKernelGGL(k_reduce_neg_max_stage, dim3(ns), dim3(kBlock), 0, stream, src, (int)nb, chunk, stage);
+  hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, stage, (int)ns, red);
   dt_ready = false;
   dt_cfl = cfl; // the CFL number the next kick folds in
   if (!hip_ok(hipMemcpyAsync(hred, red, sizeof(double), hipMemcpyDeviceToHost, stream), "readback") ||

@@ -136,6 +136,9 @@ template <int OP> __device__ inline double block_reduce_n(double v)
 template <int OP> __global__ void k_reduce_final(const double *__restrict__ part, int np, double *__restrict__ out)
 {
   double v = OP == 0 ? 0.0 : -1.0e308;
+  // unrolled so that eight loads are in flight; the values are still folded
+  // one after another in index order
+#pragma unroll 8
   for (int i = threadIdx.x; i < np; i += blockDim.x)
     v = OP == 0 ? v + part[i] : fmax(v, part[i]);
   v = block_reduce<OP>(v);
@@ -143,11 +146,19 @@ template <int OP> __global__ void k_reduce_final(const double *__restrict__ part
     *out = v;
 }
 
-__global__ void k_negate(double *__restrict__ a, int n)
+// First stage of the max fold of many per-block minima: block b folds -part
+// over [b chunk, (b + 1) chunk) into out[b] (one load per thread per chunk of
+// kBlock, instead of a single block walking every partial).
+__global__ void k_reduce_neg_max_stage(const double *__restrict__ part, int np, int chunk, double *__restrict__ out)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    a[i] = -a[i];
+  double v = -1.0e308;
+  const int i1 = min(np, (int)blockIdx.x * chunk + chunk);
+#pragma unroll 4
+  for (int i = blockIdx.x * chunk + threadIdx.x; i < i1; i += blockDim.x)
+    v = fmax(v, -part[i]);
+  v = block_reduce<1>(v);
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = v;
 }
 
 // ---------------------------------------------------------------- particles
