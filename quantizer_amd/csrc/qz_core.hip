@@ -821,6 +821,16 @@ template <typename P, typename G> struct Engine final : EngineBase
     }
   }
 
+  // FP64 residual of Lap_h ph = rh into lev_r[0] and its sum of squares into red[0]
+  void outer_residual(double *ph, const G *rh, double inv_h2)
+  {
+    const size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
+    const unsigned nbr = outer_residual_blocks(n + 1);
+    QZ_LAUNCH_TB("outer_residual", dim3(kResTX, kResTY), (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nbr, ph, rh,
+                 lev_r[0], n + 1, inv_h2, part);
+    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nbr, red);
+  }
+
   // One outer iteration of a head solve Lap_h ph = rh after the first residual:
   // `inner` V-cycles in G on the correction, FP64 correction with the pointwise
   // change test, then the new FP64 residual and its sum of squares.
@@ -834,10 +844,7 @@ template <typename P, typename G> struct Engine final : EngineBase
     QZ_LAUNCHB("outer_correct", (16.0 + sizeof(G)) * npts, k_outer_correct<G>, nb, ph, lev_e[0], make_cube(n + 1), cfg.tol_change,
                part);
     hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kBlock), 0, stream, part, (int)nb, red + 1);
-    const unsigned nbr = outer_residual_blocks(n + 1);
-    QZ_LAUNCH_TB("outer_residual", dim3(64, kResTY), (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nbr, ph, rh,
-                 lev_r[0], n + 1, 1.0 / (H * H), part);
-    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nbr, red);
+    outer_residual(ph, rh, 1.0 / (H * H));
   }
 
   bool launch_head_body(double *ph, const G *rh, int inner, int slot)
@@ -918,10 +925,7 @@ template <typename P, typename G> struct Engine final : EngineBase
       if (scale == 0.0)
         scale = 1.0;
     }
-    const unsigned nbr = outer_residual_blocks(n + 1);
-    QZ_LAUNCH_TB("outer_residual", dim3(64, kResTY), (8.0 + 2.0 * sizeof(G)) * npts, k_outer_residual<G>, nbr, ph, rh,
-                 lev_r[0], n + 1, inv_h2, part);
-    hipLaunchKernelGGL(k_reduce_final<0>, dim3(1), dim3(kBlock), 0, stream, part, (int)nbr, red);
+    outer_residual(ph, rh, inv_h2);
     bool ok = false;
     for (int outer = 0;; outer++)
     {

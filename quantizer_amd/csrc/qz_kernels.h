@@ -833,89 +833,108 @@ __global__ void k_init_head(double *__restrict__ phi, G *__restrict__ ax, G *__r
 
 // Outer FP64 residual r = rho - Lap(phi) on interior points (multigrid.c:140-150,
 // poisson_error.c:116-117), stored in the V-cycle type, with per-block sum of r^2.
+// Only interior points are written: r stays 0 on the faces from its allocation.
 //
-// A block is 64 lanes by kResTY rows of grid columns and marches up kResKC
-// planes: each lane keeps phi of its column at k - 1, k, k + 1 in registers
-// (loading k + 2 one plane ahead), and the in-plane neighbours come from one
-// LDS copy of the plane (double-buffered by plane parity, one barrier per
-// plane).  Lanes 0 and 63 carry the x halo (load only) and the first and last
-// rows also load the rows below and above the tile, so every phi value is read
-// from memory about once (the chunked one-point-per-thread form read each
-// plane three times).  The Laplacian is summed in the same order as before.
-constexpr int kResTY = 4, kResKC = 16, kResOX = 62;
+// A block is kResTX lanes by kResTY rows of interior columns and marches up
+// kResKC interior planes: each lane keeps phi of its column at k - 1, k, k + 1
+// in registers (loading k + 3 while plane k is computed), and the in-plane
+// neighbours come from one LDS copy of the plane (double-buffered by plane
+// parity, one barrier per plane).  The first and last lanes of a row also load
+// the x neighbours outside the tile, and the first and last rows the rows below
+// and above it, so every phi value is read from memory about once.  A row of
+// 256 lanes spans the 255 interior columns of a 257^3 grid; 4 rows by 32
+// planes make 512 workgroups of 1024 lanes there, two per CU, and took 62 us
+// against 73 us for 64 x 4 x 16 and 66-70 us for the other shapes measured.
+// The Laplacian is summed in a fixed order per point.
+constexpr int kResTX = 256, kResTY = 4, kResKC = 32;
 inline unsigned outer_residual_blocks(int n1)
 {
-  return (unsigned)((n1 + kResOX - 1) / kResOX) * (unsigned)((n1 + kResTY - 1) / kResTY) *
-         (unsigned)((n1 + kResKC - 1) / kResKC);
+  const unsigned m = (unsigned)(n1 - 2); // interior points per axis
+  return ((m + kResTX - 1) / kResTX) * ((m + kResTY - 1) / kResTY) * ((m + kResKC - 1) / kResKC);
 }
 
 template <typename G>
-__global__ void __launch_bounds__(64 * kResTY) k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho,
-                                                                G *__restrict__ r, int n1, double inv_h2,
-                                                                double *__restrict__ part)
+__global__ void __launch_bounds__(kResTX * kResTY) k_outer_residual(const double *__restrict__ phi, const G *__restrict__ rho,
+                                                                    G *__restrict__ r, int n1, double inv_h2,
+                                                                    double *__restrict__ part)
 {
-  constexpr int TY = kResTY;
-  __shared__ double S[2][TY + 2][64];
-  const int n = n1 - 1;
+  constexpr int TX = kResTX, TY = kResTY, KC = kResKC;
+  static_assert(TX % 64 == 0 && TY >= 2, "rows of whole wavefronts, at least two rows");
+  __shared__ double S[2][TY + 2][TX + 2];
+  const int n = n1 - 1, m = n1 - 2;
   const unsigned npts = (unsigned)n1 * n1 * n1, hf = (npts + 1) >> 1, s1 = n1, s2 = (unsigned)n1 * n1;
-  const int gx = (n1 + kResOX - 1) / kResOX, gy = (n1 + TY - 1) / TY;
+  const int gx = (m + TX - 1) / TX, gy = (m + TY - 1) / TY;
   const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int bx = (int)(lb % (unsigned)gx), by = (int)((lb / (unsigned)gx) % (unsigned)gy), bz = (int)(lb / (unsigned)(gx * gy));
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = bx * kResOX - 1 + tx, j = by * TY + ty; // lanes 0 and 63 hold the x halo
-  const int kb = bz * kResKC, ke = min(kb + kResKC, n1);
-  const bool col = i >= 0 && i <= n && j <= n;
-  const bool own = col && tx > 0 && tx < 63;
-  const bool inplane = own && i > 0 && i < n && j > 0 && j < n;
-  // halo rows of the tile: the first row also loads j - 1, the last row j + 1
-  const bool edge = ty == 0 || ty == TY - 1;
-  const int jh = ty == 0 ? j - 1 : j + 1;
-  const bool hcol = edge && i >= 0 && i <= n && jh >= 0 && jh <= n;
-  const int hrow = ty == 0 ? 0 : TY + 1;
-  const unsigned ic = (unsigned)min(max(i, 0), n);
-  unsigned t = ic + s1 * (unsigned)min(j, n) + s2 * (unsigned)kb;
-  unsigned th = ic + s1 * (unsigned)min(max(jh, 0), n) + s2 * (unsigned)kb;
-  double cm = (inplane && kb > 0) ? phi[t - s2] : 0.0;
+  const int i = 1 + bx * TX + tx, j = 1 + by * TY + ty;
+  const int kb = 1 + bz * KC, ke = min(kb + KC, n); // output planes [kb, ke)
+  const bool col = i <= n && j <= n; // a grid column: its values are neighbours of the outputs
+  const bool out = i < n && j < n;   // an interior column: one output per plane
+  // y neighbours outside the tile (row 0: j - 1, last row: j + 1) and x
+  // neighbours outside it (lane 0: i - 1, the last lane: i + 1), for output columns
+  const bool hcol = out && (ty == 0 || ty == TY - 1);
+  const int jh = ty == 0 ? j - 1 : j + 1, hrow = ty == 0 ? 0 : TY + 1;
+  const bool xcol = out && (tx == 0 || tx == TX - 1);
+  const int ih = tx == 0 ? i - 1 : i + 1, xslot = tx == 0 ? 0 : TX + 1;
+  const unsigned ic = (unsigned)min(i, n), jc = (unsigned)min(j, n);
+  unsigned t = ic + s1 * jc + s2 * (unsigned)kb;
+  unsigned th = ic + s1 * (unsigned)min(jh, n) + s2 * (unsigned)kb;
+  unsigned tq = (unsigned)min(ih, n) + s1 * jc + s2 * (unsigned)kb;
+  // loads run two planes ahead of the plane being computed (rho one plane)
+  const bool two = kb + 1 < ke; // a second plane (then kb + 2 <= n)
+  double cm = out ? phi[t - s2] : 0.0;
   double c0 = col ? phi[t] : 0.0;
-  double cp = (col && kb < n) ? phi[t + s2] : 0.0;
-  double h0 = hcol ? phi[th] : 0.0;
+  double cp = col ? phi[t + s2] : 0.0; // kb + 1 <= n
+  double cpp = (col && two) ? phi[t + 2 * s2] : 0.0;
+  double h0 = hcol ? phi[th] : 0.0, h1 = (hcol && two) ? phi[th + s2] : 0.0;
+  double x0 = xcol ? phi[tq] : 0.0, x1 = (xcol && two) ? phi[tq + s2] : 0.0;
+  G r0 = out ? rho[t] : (G)0;
   double sq = 0.0;
-  for (int k = kb; k < ke; k++, t += s2, th += s2)
+  for (int k = kb; k < ke; k++, t += s2, th += s2, tq += s2)
   {
     const int buf = k & 1;
-    const bool at = inplane && k > 0 && k < n;
-    const double cpp = (col && k + 1 < n && k + 1 < ke) ? phi[t + 2 * s2] : 0.0;
-    const double hn = (hcol && k + 1 < ke) ? phi[th + s2] : 0.0;
-    const G rv = at ? rho[t] : (G)0;
-    S[buf][ty + 1][tx] = c0;
-    if (edge)
-      S[buf][hrow][tx] = h0;
+    const bool next = k + 1 < ke, third = k + 2 < ke; // then k + 3 <= n
+    const double c3 = (col && third) ? phi[t + 3 * s2] : 0.0;
+    const double h2 = (hcol && third) ? phi[th + 2 * s2] : 0.0;
+    const double x2 = (xcol && third) ? phi[tq + 2 * s2] : 0.0;
+    const G r1 = (out && next) ? rho[t + s2] : (G)0;
+    S[buf][ty + 1][tx + 1] = c0;
+    if (hcol)
+      S[buf][hrow][tx + 1] = h0;
+    if (xcol)
+      S[buf][ty + 1][xslot] = x0;
     __syncthreads();
-    double res = 0.0;
-    if (at)
+    if (out)
     {
-      res = (double)rv + inv_h2 * (6.0 * c0 - S[buf][ty + 1][tx + 1] - S[buf][ty + 1][tx - 1] - S[buf][ty + 2][tx] -
-                                   S[buf][ty][tx] - cp - cm);
+      const double res = (double)r0 + inv_h2 * (6.0 * c0 - S[buf][ty + 1][tx + 2] - S[buf][ty + 1][tx] -
+                                                S[buf][ty + 2][tx + 1] - S[buf][ty][tx + 1] - cp - cm);
       sq += res * res;
-    }
-    if (own)
       r[(t & 1) * hf + (t >> 1)] = (G)res;
+    }
     cm = c0;
     c0 = cp;
     cp = cpp;
-    h0 = hn;
+    cpp = c3;
+    h0 = h1;
+    h1 = h2;
+    x0 = x1;
+    x1 = x2;
+    r0 = r1;
   }
-  // block sum over the 64 x TY lanes
-  __shared__ double ws[TY];
+  // block sum over the TX x TY lanes
+  constexpr int W = TX * TY / 64;
+  __shared__ double ws[W];
   for (int off = 32; off > 0; off >>= 1)
     sq += __shfl_down(sq, off, 64);
-  if (tx == 0)
-    ws[ty] = sq;
+  const int wv = (ty * TX + tx) >> 6;
+  if ((tx & 63) == 0)
+    ws[wv] = sq;
   __syncthreads();
   if (tx == 0 && ty == 0)
   {
     double v = 0.0;
-    for (int w = 0; w < TY; w++)
+    for (int w = 0; w < W; w++)
       v += ws[w];
     part[blockIdx.x] = v;
   }
