@@ -119,6 +119,7 @@ struct TreeDev
   std::string ahead_err; // the set-up thread's error (errors are kept per thread)
   hipStream_t ahead_stream = nullptr;
   hipEvent_t ahead_tree = nullptr, ahead_done = nullptr;
+  hipEvent_t acc_go = nullptr, acc_done = nullptr; // head accelerations on the set-up stream
   unsigned char *mg_mask = nullptr;
   long long mg_cap = 0;
   // grid-point classification: chunk table, per-chunk counts and offsets
@@ -236,7 +237,7 @@ template <typename P, typename G> void Engine<P, G>::release_tree()
     d->drop_plan();
     if (d->ahead_stream)
       (void)hipStreamDestroy(d->ahead_stream);
-    for (hipEvent_t e : {d->ahead_tree, d->ahead_done})
+    for (hipEvent_t e : {d->ahead_tree, d->ahead_done, d->acc_go, d->acc_done})
       if (e)
         (void)hipEventDestroy(e);
     void *ptrs[] = {d->nodes, d->zone_node, d->zone_xyz, d->cell_xyz, d->cell_box, d->cstart, d->cend, d->box,
@@ -1060,9 +1061,32 @@ template <typename P, typename G> bool Engine<P, G>::force_overlapped(bool with_
   }
   if (!hip_ok(hipStreamWaitEvent(stream, d.ahead_done, 0), "set-up stream"))
     return false;
-  return (!has_tree() || branch_density()) && potential_branches() && grid_acceleration() &&
-         (!with_particles || particle_acceleration()) &&
-         hip_ok(hipGetLastError(), "force pass"); // stream-ordered; readers synchronise
+  // the head accelerations need only the head potential: they run on the
+  // set-up stream beside the node solves, which leave most CUs idle, and the
+  // engine's stream waits for them before the node accelerations
+  if (!d.acc_go && (!hip_ok(hipEventCreateWithFlags(&d.acc_go, hipEventDisableTiming), "set-up event") ||
+                    !hip_ok(hipEventCreateWithFlags(&d.acc_done, hipEventDisableTiming), "set-up event")))
+    return false;
+  if (!hip_ok(hipEventRecord(d.acc_go, stream), "set-up event") ||
+      !hip_ok(hipStreamWaitEvent(d.ahead_stream, d.acc_go, 0), "set-up stream"))
+    return false;
+  grid_acc_head(d.ahead_stream);
+  if (!hip_ok(hipEventRecord(d.acc_done, d.ahead_stream), "set-up event"))
+    return false;
+  head_acc_queued = true;
+  const bool ok = (!has_tree() || branch_density()) && potential_branches() && grid_acceleration() &&
+                  (!with_particles || particle_acceleration()) &&
+                  hip_ok(hipGetLastError(), "force pass"); // stream-ordered; readers synchronise
+  if (head_acc_queued) // a stage failed before the wait: keep later passes in order
+    (void)wait_head_acc();
+  return ok;
+}
+
+// The engine's stream waits for the head accelerations queued on the set-up stream.
+template <typename P, typename G> bool Engine<P, G>::wait_head_acc()
+{
+  head_acc_queued = false;
+  return hip_ok(hipStreamWaitEvent(stream, tree->dev->acc_done, 0), "set-up stream");
 }
 
 template <typename P, typename G> bool Engine<P, G>::branch_density()

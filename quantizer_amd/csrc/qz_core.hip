@@ -191,6 +191,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   bool keys_fresh = false;  // the unsorted keys and slots of the current positions are in the sort's input
   double face_gm = 0.0; // G M of the head's monopole faces, fixed when the head is initialised
   bool acc_live = false;         // particle accelerations hold values (false: all zero, as reset leaves them)
+  bool head_acc_queued = false;  // this pass's head accelerations are queued on the set-up stream
   // The next CFL time step folded into the kick: plev (one byte per sorted
   // particle) holds the finest level of each particle from the last gather,
   // valid (lev_ready) until the particles move, are reordered or reloaded;
@@ -987,12 +988,20 @@ template <typename P, typename G> struct Engine final : EngineBase
     return ok;
   }
 
-  bool grid_acceleration() override
+  void grid_acc_head(hipStream_t stream)
   {
     size_t n1 = (size_t)n + 1, npts = n1 * n1 * n1;
     double H = 1.0 / n;
     QZ_LAUNCHB("grid_acc_head", (8.0 + 3.0 * sizeof(G)) * npts, k_grid_acc_head<G>, nblk(npts), phi, gax, gay, gaz,
                make_cube(n + 1), 1.0 / (2.0 * H), face_gm, cfg.head_bc == QZ_BC_ZERO ? 1 : 0);
+  }
+
+  bool grid_acceleration() override
+  {
+    if (!head_acc_queued)
+      grid_acc_head(stream);
+    else if (!wait_head_acc())
+      return false;
     if (has_tree() && !branch_grid_acceleration())
       return false;
     return hip_ok(hipGetLastError(), "grid_acceleration"); // stream-ordered; readers synchronise
@@ -1302,6 +1311,7 @@ template <typename P, typename G> struct Engine final : EngineBase
   void bmg_batch(size_t l, const BmgPlan &pl, hipStream_t stream, double *phi_pool, bool adjoint);
   bool branch_density();
   bool branch_grid_acceleration();
+  bool wait_head_acc();
   bool branch_particle_acceleration();
   void branch_grid_pe(unsigned nb); // partial sums of m phi at the finest node into part
   void release_tree();

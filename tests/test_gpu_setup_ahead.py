@@ -1,7 +1,8 @@
 """The first tree level's node multigrid set-up runs on a host thread and a
 second stream during the head density and solve (QZ_NODE_SETUP_AHEAD=0 builds
-it in place, in the node solve).  The same tables reach the same kernels, so
-fields and trajectories agree bit for bit."""
+it in place, in the node solve).  With the set-up ahead, the head
+accelerations also run on that stream, beside the node solves.  The same
+tables reach the same kernels, so fields and trajectories agree bit for bit."""
 import os
 
 import numpy as np
