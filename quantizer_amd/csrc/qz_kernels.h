@@ -735,7 +735,10 @@ __global__ void k_pairwise_pe(const double *__restrict__ x, const double *__rest
     mi = m[i];
   }
   double acc = 0.0;
-  for (int base = 0; base < N; base += kBlock)
+  // only j < i contributes: tiles past the block's last i are skipped (the
+  // bound is the same for every thread of the block)
+  const int jend = min(N, (int)(blockIdx.x + 1) * (int)blockDim.x);
+  for (int base = 0; base < jend; base += kBlock)
   {
     int j = base + threadIdx.x;
     tx[threadIdx.x] = j < N ? (double)x[j] : 0.0;
